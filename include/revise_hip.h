@@ -1,0 +1,249 @@
+/*
+ * revise_hip.h -- C ABI of the MI355X (gfx950) sparse->dense interpolation and
+ * volume ray-casting path of ReVisE.
+ *
+ * The reference (deadmorous/revise) exposes this path as C++ plug-in points;
+ * each entry point below names the reference interface it replaces
+ * (file:line, paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - Every function returns an int status: RV_OK (0) or an RV_ERR_* code.
+ *    No C++ exception crosses this boundary; rv_last_error() returns a
+ *    thread-local message for the last failure on the calling thread.
+ *  - Pointers named d_* are device pointers on the current HIP device, owned
+ *    by the caller.  The library keeps no pointer past a call, except inside
+ *    the opaque handles (rv_interp_batch, rv_frame_plan, rv_worker), which
+ *    document what they hold.
+ *  - Work is enqueued on the given stream (0 = the default stream) and is
+ *    asynchronous unless stated otherwise.
+ *  - Dense bricks are (2^depth+1)^3 nodes, x fastest, then y, then z
+ *    (IndexTransform.hpp:250-257).  Node coordinates (n2i) are uint16 x 3
+ *    (BlockTreeNodeCoord.hpp:24).  The "no field" sentinel in sparse values is
+ *    1e30f and becomes NaN in dense bricks (BlockTreeFieldProvider.hpp:137-139,
+ *    DenseFieldInterpolator.hpp:416-418).
+ */
+#ifndef REVISE_HIP_H
+#define REVISE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* rv_stream_t; /* == hipStream_t */
+
+enum {
+    RV_OK = 0,
+    RV_ERR_INVALID = 1,     /* bad argument (size, depth, null pointer, ...) */
+    RV_ERR_HIP = 2,         /* a HIP runtime call failed */
+    RV_ERR_OOM = 3,         /* device allocation failed */
+    RV_ERR_CANCELLED = 4,   /* the caller's cancel flag was set */
+    RV_ERR_UNSUPPORTED = 5, /* valid request this build does not serve */
+    RV_ERR_INTERNAL = 6
+};
+
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char* rv_last_error(void);
+/* Library ABI version (major*10000 + minor*100 + patch). */
+int rv_version(void);
+
+/* ------------------------------------------------------------------------
+ * Block tree and node index map (host)
+ * ------------------------------------------------------------------------ */
+
+/* Depth of a subtree, min(actual depth, max_depth).
+ * Replaces CompressedBlockTree::depthUpTo (CompressedBlockTree.hpp:290-300),
+ * as used for Metadata subtree depth (Metadata.hpp:414-416).
+ * children: nnodes x 8 child indices (0 = none), node 0 = subtree root,
+ * child local index bit0 = x, bit1 = y, bit2 = z. */
+int rv_subtree_depth(const uint32_t* children, uint32_t nnodes, uint32_t max_depth,
+                     uint32_t* depth_out);
+
+/* n2i list of a subtree (first-visit order of block corners in a pre-order
+ * walk limited to `depth` levels).  Replaces BlockTreeNodes<3,BT>::BlockTreeNodes
+ * (BlockTreeNodes.hpp:66-80, 166-181) / Metadata::blockTreeNodes
+ * (Metadata.hpp:255-268).  n2i: cap x 3 uint16.  *count receives the node
+ * count; RV_ERR_INVALID if cap is too small or the tree is malformed. */
+int rv_subtree_n2i(const uint32_t* children, uint32_t nnodes, uint32_t depth, uint16_t* n2i,
+                   uint64_t cap, uint64_t* count);
+
+/* Node -> dense ordinal map on the device: d_out[i] = vertexIndexToOrdinal(n2i[i], depth).
+ * Replaces I3::vertexIndexToOrdinal (DenseFieldInterpolator_cu_3.cu:24-30). */
+int rv_node_ordinals(const uint16_t* d_n2i, uint64_t n, uint32_t depth, uint32_t* d_out,
+                     rv_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Sparse -> dense interpolation (device)
+ * ------------------------------------------------------------------------ */
+
+/* Number of nodes of a dense brick, (2^depth+1)^3. */
+uint64_t rv_brick_nodes(uint32_t depth);
+
+/* Gather plan of a subtree: d_plan[(2^depth+1)^3] receives, per dense node,
+ * the index of its sparse value or 0xFFFFFFFF.  A plan depends only on the
+ * subtree's n2i, so callers cache it per subtree (as the reference caches
+ * BlockTreeNodes, BlockTreeFieldService.hpp:133-137).  Entries whose
+ * coordinates fall outside the brick are ignored and counted into
+ * *d_bad_count when it is not NULL (a device uint32 the caller zeroes). */
+int rv_interp_plan_build(const uint16_t* d_n2i, uint64_t n, uint32_t depth, uint32_t* d_plan,
+                         uint32_t* d_bad_count, rv_stream_t stream);
+
+/* Interpolate one brick from its sparse values through a plan.
+ * Replaces DenseFieldInterpolator_cu_3::interpolate
+ * (DenseFieldInterpolator_cu_3.cu:197-253; DenseFieldInterpolatorArr_cu_3.cu:197-257),
+ * with the CPU semantics of DenseFieldInterpolator<3>::interpolate
+ * (DenseFieldInterpolator.hpp:198-419), bit for bit. */
+int rv_interp_brick_planned(const float* d_sparse, const uint32_t* d_plan, uint32_t depth,
+                            float* d_dense, rv_stream_t stream);
+
+/* Plan + interpolate in one call (the plain drop-in for
+ * DenseFieldInterpolatorTpl_cu_3::interpolate(denseDev, sparse, nodes),
+ * DenseFieldInterpolatorTpl_cu_3.hpp:103-107).  d_workspace: caller-owned
+ * scratch of rv_brick_nodes(depth) uint32; if NULL the library allocates it
+ * stream-ordered for the call. */
+int rv_interp_brick(const float* d_sparse, const uint16_t* d_n2i, uint64_t n, uint32_t depth,
+                    float* d_dense, uint32_t* d_workspace, rv_stream_t stream);
+
+/* One brick of a batch.  dense16 != NULL stores fp16 instead of fp32
+ * (exactly one of dense / dense16 is set). */
+typedef struct rv_brick_job {
+    const float* sparse;
+    const uint32_t* plan;
+    float* dense;
+    uint16_t* dense16;
+    uint32_t depth;
+    uint32_t reserved;
+} rv_brick_job;
+
+/* Batched interpolation of many bricks of any depths in one launch.
+ * There is no reference counterpart: the reference interpolates one brick per
+ * call (DenseFieldRenderer_cu.cu:504-545).  The handle owns a device copy of
+ * the job table and its tile index; the bricks stay caller-owned. */
+typedef struct rv_interp_batch rv_interp_batch;
+int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count, rv_interp_batch** out);
+int rv_interp_batch_run(rv_interp_batch* batch, rv_stream_t stream);
+/* Total tiles (workgroups) of one run, for roofline accounting. */
+int rv_interp_batch_info(const rv_interp_batch* batch, uint64_t* tiles, uint64_t* dense_nodes);
+void rv_interp_batch_destroy(rv_interp_batch* batch);
+
+/* ------------------------------------------------------------------------
+ * Per-block ray-cast (device) and its host parameter setup
+ * ------------------------------------------------------------------------ */
+
+/* Parameters of one block render: RenderKernelParam (RenderKernelParam.hpp:26-116)
+ * restricted to the Argb mode, plus the per-block constants the reference
+ * recomputes per pixel (DenseFieldRenderer_cu.cu:164-181, renderArgb.hpp:34-38). */
+typedef struct rv_block_params {
+    uint32_t vp_w, vp_h;                 /* viewport */
+    uint32_t view_x, view_y, view_w, view_h; /* view rectangle (max excluded) */
+    float center[3];                     /* cube centre, world */
+    float half;                          /* cube half size */
+    uint32_t level;                      /* cube level */
+    uint32_t tex_depth;                  /* brick depth: 2^tex_depth+1 nodes per edge */
+    float fmin, fmax;                    /* field range */
+    float pos[3], e1[3], e2[3], n[3];    /* eye */
+    float w, h, pixel_size;              /* screen */
+    float threshold;                     /* absolute threshold */
+    float quality;                       /* sampling quality */
+    float alpha_cutoff;                  /* early ray termination (reference: 0.996) */
+    /* derived */
+    uint32_t nodes;                      /* 2^tex_depth+1 */
+    float texel;                         /* 2*half/nodes */
+    float tex_origin[3];                 /* centre - (half + texel/2) */
+    float tcf;                           /* nodes / (2*(half + texel/2)) */
+    float i_layer_count;                 /* 100 / (2^(tex_depth+level) * quality) */
+    float opacity_factor;                /* (threshold-fmin)/(fmax-fmin) */
+} rv_block_params;
+
+/* Host parameter setup of DenseFieldRenderer_cu::Impl::renderDenseField
+ * (DenseFieldRenderer_cu.cu:317-453): cube, camera basis, view rectangle,
+ * threshold.  view16: row-major world->eye rigid transform (the reference's
+ * VsRenderSharedState::cameraTransform, VsRenderSharedState.hpp:29-46).
+ * *live = 0 when the view rectangle is empty (the reference returns early). */
+int rv_block_params_setup(const float* view16, uint32_t vp_w, uint32_t vp_h, float fov_y_deg,
+                          uint32_t level, const uint32_t index[3], uint32_t tex_depth, float fmin,
+                          float fmax, float threshold_rel, float quality, float alpha_cutoff,
+                          rv_block_params* out, int* live);
+
+/* Clear a viewport to 0.  Replaces DenseFieldRenderer_cu::clearViewport
+ * (DenseFieldRenderer_cu.cu:268-277) for the Argb mode. */
+int rv_clear_viewport(uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, rv_stream_t stream);
+
+/* Ray-cast one block into the ARGB viewport (composited over what is there).
+ * Replaces renderKernel<FieldMode::Argb> (DenseFieldRenderer_cu.cu:62-215,
+ * renderArgb.hpp:30-55).  d_tf: tf_n x 4 floats (RGBA), sampled like the
+ * reference's normalized, linear, clamped 1-D texture. */
+int rv_render_block_argb(const rv_block_params* params, const float* d_brick, const float* d_tf,
+                         uint32_t tf_n, uint32_t* d_viewport, rv_stream_t stream);
+
+/* Piecewise-linear colour table from control points.  Replaces
+ * s3vs::makeColorTransferVector (ColorTransferFunction.hpp:27-69).
+ * xs: count sorted positions, colors: count x 4, out: color_count x 4. */
+int rv_color_transfer_vector(const float* xs, const float* colors, uint32_t count,
+                             uint32_t color_count, float* out);
+
+/* ------------------------------------------------------------------------
+ * Block ordering (host)
+ * ------------------------------------------------------------------------ */
+
+/* Back-to-front order of the blocks of an index box.  Replaces
+ * BackToFrontOrder<MacroBlock<3>>::range(eye) (BackToFrontOrder.hpp:61-359,
+ * MacroBlock.hpp:29-67).  out: cap x 3 indices. */
+int rv_back_to_front(const uint32_t imin[3], const uint32_t imax[3], const float cmin[3],
+                     const float cmax[3], const float eye[3], uint32_t* out, uint64_t cap,
+                     uint64_t* count);
+
+/* Eye position of a camera transform.  Replaces s3vs::eyeFromTransform
+ * (BlockOrderUtil.hpp:54-59). */
+int rv_eye_from_view(const float* view16, float eye[3]);
+
+/* World box of an index box at a level.  Replaces s3vs::boundingBoxOfIndexBox
+ * (BlockOrderUtil.hpp:29-51). */
+int rv_index_box_coords(uint32_t level, const uint32_t bmin[3], const uint32_t bmax[3],
+                        float cmin[3], float cmax[3]);
+
+/* Worker boxes of a level, sorted back to front.  Replaces
+ * IndexCubeSplitStorage::getBoxesSorted (IndexCubeSplitStorage.hpp:52-99) over
+ * IndexCubeSplitter<3> (IndexCubeSplitter.hpp:35-196).  boxes: cap x 6
+ * {min x,y,z, max x,y,z}. */
+int rv_index_cube_split_sorted(uint32_t level, uint32_t split_count, const float eye[3],
+                               uint32_t* boxes, uint64_t cap, uint64_t* count);
+
+/* Back-to-front order of a set of disjoint blocks of mixed levels.  No
+ * reference counterpart (the reference renders one level per frame,
+ * VsRenderer.cpp:1067-1083); reduces to rv_back_to_front when all blocks share
+ * one level.  blocks: n x 4 {level, x, y, z}; order_out: n indices into blocks. */
+int rv_mixed_back_to_front(const uint32_t* blocks, uint64_t n, const float eye[3],
+                           uint32_t* order_out);
+
+/* ------------------------------------------------------------------------
+ * Image parts: crop and blend (device)
+ * ------------------------------------------------------------------------ */
+
+/* Bounding rectangle of pixels with non-zero alpha; synchronous (waits for
+ * the stream).  Replaces getBoundRect (VsWorker_cu.cpp:51-88).  rect:
+ * {x, y, w, h}; *nonempty = 0 if all pixels are transparent. */
+int rv_bound_rect(const uint32_t* d_pixels, uint32_t w, uint32_t h, int32_t rect[4],
+                  int* nonempty, rv_stream_t stream);
+
+/* Blend an image part into a destination image, 8-bit integer "over".
+ * Replaces s3vs::blendImagePart (BlendImagePart.hpp:61-145).  The part
+ * (sw x sh at src_x, src_y) must lie inside dst (dw x dh at dst_off). */
+int rv_blend_image_part(uint32_t* d_dst, int32_t dw, int32_t dh, int32_t dst_off_x,
+                        int32_t dst_off_y, const uint32_t* d_src, int32_t sw, int32_t sh,
+                        int32_t src_x, int32_t src_y, int just_copy, rv_stream_t stream);
+
+/* Fill n uint32 with a value (background fill of FinalComposeFunc,
+ * VsRenderer.cpp:475-519). */
+int rv_fill_u32(uint32_t* d_dst, uint64_t n, uint32_t value, rv_stream_t stream);
+
+/* floatRgbToUint32Argb (VsRenderer.cpp:115-122). */
+uint32_t rv_background_argb(float r, float g, float b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* REVISE_HIP_H */

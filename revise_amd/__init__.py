@@ -1,0 +1,122 @@
+"""revise_amd: MI355X (gfx950) sparse->dense interpolation and volume
+ray-casting path of ReVisE, behind a C ABI (include/revise_hip.h).
+
+This module is a thin ctypes binding of librevise_hip.so for Python callers
+(tests, bench).  It fails loudly when the library is missing: there is no
+CPU fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librevise_hip.so")
+
+RV_OK = 0
+
+
+class RvError(RuntimeError):
+    pass
+
+
+class BlockParams(C.Structure):
+    _fields_ = [
+        ("vp_w", C.c_uint32), ("vp_h", C.c_uint32),
+        ("view_x", C.c_uint32), ("view_y", C.c_uint32), ("view_w", C.c_uint32), ("view_h", C.c_uint32),
+        ("center", C.c_float * 3), ("half", C.c_float), ("level", C.c_uint32),
+        ("tex_depth", C.c_uint32), ("fmin", C.c_float), ("fmax", C.c_float),
+        ("pos", C.c_float * 3), ("e1", C.c_float * 3), ("e2", C.c_float * 3), ("n", C.c_float * 3),
+        ("w", C.c_float), ("h", C.c_float), ("pixel_size", C.c_float),
+        ("threshold", C.c_float), ("quality", C.c_float), ("alpha_cutoff", C.c_float),
+        ("nodes", C.c_uint32), ("texel", C.c_float), ("tex_origin", C.c_float * 3), ("tcf", C.c_float),
+        ("i_layer_count", C.c_float), ("opacity_factor", C.c_float),
+    ]
+
+
+class BrickJob(C.Structure):
+    _fields_ = [("sparse", C.c_void_p), ("plan", C.c_void_p), ("dense", C.c_void_p),
+                ("dense16", C.c_void_p), ("depth", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+_u32p = C.POINTER(C.c_uint32)
+_u16p = C.POINTER(C.c_uint16)
+_f32p = C.POINTER(C.c_float)
+_vp = C.c_void_p
+
+# name: (restype, argtypes)
+_SIGS = {
+    "rv_last_error": (C.c_char_p, []),
+    "rv_version": (C.c_int, []),
+    "rv_subtree_depth": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, _u32p]),
+    "rv_subtree_n2i": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, _u16p, C.c_uint64,
+                                 C.POINTER(C.c_uint64)]),
+    "rv_node_ordinals": (C.c_int, [_vp, C.c_uint64, C.c_uint32, _vp, _vp]),
+    "rv_brick_nodes": (C.c_uint64, [C.c_uint32]),
+    "rv_interp_plan_build": (C.c_int, [_vp, C.c_uint64, C.c_uint32, _vp, _vp, _vp]),
+    "rv_interp_brick_planned": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
+    "rv_interp_brick": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint32, _vp, _vp, _vp]),
+    "rv_interp_batch_create": (C.c_int, [C.POINTER(BrickJob), C.c_uint32, C.POINTER(C.c_void_p)]),
+    "rv_interp_batch_run": (C.c_int, [_vp, _vp]),
+    "rv_interp_batch_info": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "rv_interp_batch_destroy": (None, [_vp]),
+    "rv_block_params_setup": (C.c_int, [_f32p, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32, _u32p,
+                                        C.c_uint32, C.c_float, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.POINTER(BlockParams), C.POINTER(C.c_int)]),
+    "rv_clear_viewport": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _vp]),
+    "rv_render_block_argb": (C.c_int, [C.POINTER(BlockParams), _vp, _vp, C.c_uint32, _vp, _vp]),
+    "rv_color_transfer_vector": (C.c_int, [_f32p, _f32p, C.c_uint32, C.c_uint32, _f32p]),
+    "rv_back_to_front": (C.c_int, [_u32p, _u32p, _f32p, _f32p, _f32p, _u32p, C.c_uint64,
+                                   C.POINTER(C.c_uint64)]),
+    "rv_eye_from_view": (C.c_int, [_f32p, _f32p]),
+    "rv_index_box_coords": (C.c_int, [C.c_uint32, _u32p, _u32p, _f32p, _f32p]),
+    "rv_index_cube_split_sorted": (C.c_int, [C.c_uint32, C.c_uint32, _f32p, _u32p, C.c_uint64,
+                                             C.POINTER(C.c_uint64)]),
+    "rv_mixed_back_to_front": (C.c_int, [_u32p, C.c_uint64, _f32p, _u32p]),
+    "rv_bound_rect": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_int32), C.POINTER(C.c_int),
+                                _vp]),
+    "rv_blend_image_part": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _vp, C.c_int32,
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_int, _vp]),
+    "rv_fill_u32": (C.c_int, [_vp, C.c_uint64, C.c_uint32, _vp]),
+    "rv_background_argb": (C.c_uint32, [C.c_float, C.c_float, C.c_float]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load librevise_hip.so (built by revise_amd.build); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RvError("librevise_hip.so is not built: run `python -m revise_amd.build` "
+                          "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            if not hasattr(L, name):
+                continue
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc):
+    if rc != RV_OK:
+        msg = lib().rv_last_error()
+        raise RvError("revise_hip error %d: %s" % (rc, msg.decode() if msg else ""))
+    return rc
+
+
+def stream_ptr(stream=None):
+    """hipStream_t of a torch stream (None -> torch's current stream)."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr())

@@ -1,0 +1,234 @@
+// Plain per-block Argb ray-cast (the drop-in for renderKernel<FieldMode::Argb>,
+// DenseFieldRenderer_cu.cu:62-215), viewport clear, crop rectangle and 8-bit
+// sub-image blending for gfx950.
+//
+// The block kernel runs one thread per pixel of the block's view rectangle in
+// 16 x 16 pixel workgroups (one wave = 16 x 4 pixels, so a wave's rays are a
+// compact bundle through the brick and share L1/L2 lines of the brick).  The
+// colour table is staged in LDS as float4.  Every arithmetic step follows the
+// reference's operand order (see render_common.hpp), so the 8-bit viewport is
+// bit-identical to the CPU restatement.
+
+#include "render_common.hpp"
+#include "rv_internal.hpp"
+
+namespace rv {
+
+int fill_u32(uint32_t* d, uint64_t n, uint32_t value, hipStream_t s);
+
+namespace {
+
+constexpr uint32_t kTfMax = 4096;
+
+__global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
+                                                           const float* __restrict__ brick,
+                                                           const float* __restrict__ tf,
+                                                           uint32_t tf_n,
+                                                           uint32_t* __restrict__ viewport)
+{
+    extern __shared__ float4 tfs[];
+    for (uint32_t i = threadIdx.x; i < tf_n; i += 256u)
+        tfs[i] = reinterpret_cast<const float4*>(tf)[i];
+    __syncthreads();
+    const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+    const uint32_t ox = blockIdx.x * 16u + lx, oy = blockIdx.y * 16u + ly;
+    if (ox >= p.view_w || oy >= p.view_h)
+        return;
+    const uint32_t vx = p.view_x + ox, vy = p.view_y + oy;
+    const RaySeg r = setup_ray(p, vx, vy);
+    if (!r.hit)
+        return;
+    const float4 rgba = march(p, brick, tfs, (int)tf_n, r);
+    const uint32_t pix = (p.vp_h - 1u - vy) * p.vp_w + vx;
+    viewport[pix] = blend_above_u32(viewport[pix], rgba);
+}
+
+// Bounding rectangle of non-transparent pixels: per-workgroup reduction, then
+// 4 atomics into out = {minX, minY, maxX, maxY}.
+__global__ __launch_bounds__(256) void bound_rect_kernel(const uint32_t* __restrict__ pix,
+                                                         uint32_t w, uint32_t h, int* out)
+{
+    __shared__ int red[4][8];
+    int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
+    const uint64_t total = (uint64_t)w * h;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256u) {
+        if (pix[i] >> 24) {
+            const int y = (int)(i / w), x = (int)(i % w);
+            mnx = min(mnx, x);
+            mxx = max(mxx, x);
+            mny = min(mny, y);
+            mxy = max(mxy, y);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        mnx = min(mnx, __shfl_down(mnx, off));
+        mny = min(mny, __shfl_down(mny, off));
+        mxx = max(mxx, __shfl_down(mxx, off));
+        mxy = max(mxy, __shfl_down(mxy, off));
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[0][wave] = mnx;
+        red[1][wave] = mny;
+        red[2][wave] = mxx;
+        red[3][wave] = mxy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int wv = 1; wv < 4; ++wv) {
+            red[0][0] = min(red[0][0], red[0][wv]);
+            red[1][0] = min(red[1][0], red[1][wv]);
+            red[2][0] = max(red[2][0], red[2][wv]);
+            red[3][0] = max(red[3][0], red[3][wv]);
+        }
+        if (red[2][0] >= 0) {
+            atomicMin(out + 0, red[0][0]);
+            atomicMin(out + 1, red[1][0]);
+            atomicMax(out + 2, red[2][0]);
+            atomicMax(out + 3, red[3][0]);
+        }
+    }
+}
+
+__global__ void init_rect_kernel(int* out)
+{
+    if (threadIdx.x == 0) {
+        out[0] = 0x7fffffff;
+        out[1] = 0x7fffffff;
+        out[2] = -1;
+        out[3] = -1;
+    }
+}
+
+// blendImagePart (BlendImagePart.hpp:61-145): one thread per source pixel.
+__global__ __launch_bounds__(256) void blend_part_kernel(uint32_t* __restrict__ dst, int dw,
+                                                         int dx, int dy,
+                                                         const uint32_t* __restrict__ src, int sw,
+                                                         int sh, int just_copy)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= sw || y >= sh)
+        return;
+    const uint32_t s = src[(size_t)y * sw + x];
+    uint32_t& d = dst[(size_t)dw * (dy + y) + dx + x];
+    if (just_copy) {
+        d = s;
+        return;
+    }
+    const uint32_t as = s >> 24;
+    if (as == 0u)
+        return;
+    if (as == 255u) {
+        d = s;
+        return;
+    }
+    const uint32_t dv = d;
+    const uint32_t ad = dv >> 24;
+    if (ad == 0u) {
+        d = s;
+        return;
+    }
+    const uint32_t a = as + ad * (255u - as) / 255u;
+    uint32_t out = a << 24;
+    for (uint32_t k = 0; k < 3; ++k) {
+        const uint32_t cs = (s >> (8 * k)) & 0xffu, cd = (dv >> (8 * k)) & 0xffu;
+        const uint32_t c = (cs * as + cd * (a - as)) / a;
+        out |= (c & 255u) << (8 * k);
+    }
+    d = out;
+}
+
+} // namespace
+} // namespace rv
+
+using namespace rv;
+
+extern "C" int rv_clear_viewport(uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                 rv_stream_t stream)
+{
+    RV_REQUIRE(d_viewport || (uint64_t)vp_w * vp_h == 0, "rv_clear_viewport: null viewport");
+    return fill_u32(d_viewport, (uint64_t)vp_w * vp_h, 0u, S(stream));
+}
+
+extern "C" int rv_fill_u32(uint32_t* d_dst, uint64_t n, uint32_t value, rv_stream_t stream)
+{
+    RV_REQUIRE(d_dst || n == 0, "rv_fill_u32: null pointer");
+    return fill_u32(d_dst, n, value, S(stream));
+}
+
+extern "C" int rv_render_block_argb(const rv_block_params* params, const float* d_brick,
+                                    const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
+                                    rv_stream_t stream)
+{
+    RV_REQUIRE(params && d_brick && d_tf && d_viewport, "rv_render_block_argb: null pointer");
+    RV_REQUIRE(tf_n >= 1 && tf_n <= kTfMax, "rv_render_block_argb: tf_n must be in [1, 4096]");
+    const rv_block_params& p = *params;
+    RV_REQUIRE(p.tex_depth <= 10 && p.nodes == (1u << p.tex_depth) + 1u,
+               "rv_render_block_argb: inconsistent tex_depth/nodes (use rv_block_params_setup)");
+    RV_REQUIRE(p.fmax > p.fmin, "rv_render_block_argb: field range must be non-empty");
+    RV_REQUIRE((uint64_t)p.view_x + p.view_w <= p.vp_w && (uint64_t)p.view_y + p.view_h <= p.vp_h,
+               "rv_render_block_argb: view rectangle outside the viewport");
+    if (p.view_w == 0 || p.view_h == 0)
+        return RV_OK;
+    dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
+    hipLaunchKernelGGL(render_block_kernel, grid, dim3(256), tf_n * sizeof(float4), S(stream), p,
+                       d_brick, d_tf, tf_n, d_viewport);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
+
+extern "C" int rv_bound_rect(const uint32_t* d_pixels, uint32_t w, uint32_t h, int32_t rect[4],
+                             int* nonempty, rv_stream_t stream)
+{
+    RV_REQUIRE(d_pixels && rect && nonempty, "rv_bound_rect: null pointer");
+    *nonempty = 0;
+    rect[0] = rect[1] = rect[2] = rect[3] = 0;
+    if ((uint64_t)w * h == 0)
+        return RV_OK;
+    int* d_out = nullptr;
+    RV_HIP(hipMallocAsync((void**)&d_out, 16, S(stream)));
+    int host[4];
+    hipLaunchKernelGGL(init_rect_kernel, dim3(1), dim3(64), 0, S(stream), d_out);
+    uint64_t blocks = ((uint64_t)w * h + 255) / 256;
+    if (blocks > 4096)
+        blocks = 4096;
+    hipLaunchKernelGGL(bound_rect_kernel, dim3((uint32_t)blocks), dim3(256), 0, S(stream), d_pixels,
+                       w, h, d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host, d_out, 16, hipMemcpyDeviceToHost, S(stream));
+    hipError_t e2 = hipFreeAsync(d_out, S(stream));
+    if (e == hipSuccess)
+        e = e2;
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(S(stream));
+    if (e != hipSuccess)
+        return hip_fail(e, "rv_bound_rect", __FILE__, __LINE__);
+    if (host[2] < 0)
+        return RV_OK;
+    *nonempty = 1;
+    rect[0] = host[0];
+    rect[1] = host[1];
+    rect[2] = host[2] - host[0] + 1;
+    rect[3] = host[3] - host[1] + 1;
+    return RV_OK;
+}
+
+extern "C" int rv_blend_image_part(uint32_t* d_dst, int32_t dw, int32_t dh, int32_t dst_off_x,
+                                   int32_t dst_off_y, const uint32_t* d_src, int32_t sw,
+                                   int32_t sh, int32_t src_x, int32_t src_y, int just_copy,
+                                   rv_stream_t stream)
+{
+    if (sw <= 0 || sh <= 0)
+        return RV_OK;
+    RV_REQUIRE(d_dst && d_src, "rv_blend_image_part: null pointer");
+    const int dx = src_x - dst_off_x, dy = src_y - dst_off_y;
+    RV_REQUIRE(dx >= 0 && dy >= 0 && dx + sw <= dw && dy + sh <= dh,
+               "rv_blend_image_part: part outside destination");
+    dim3 grid((sw + 63) / 64, (sh + 3) / 4);
+    hipLaunchKernelGGL(blend_part_kernel, grid, dim3(256), 0, S(stream), d_dst, dw, dx, dy, d_src,
+                       sw, sh, just_copy);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}

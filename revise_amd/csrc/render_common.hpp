@@ -1,0 +1,261 @@
+// Device helpers shared by the ray-cast kernels: the per-sample operations of
+// the reference Argb renderer (renderArgb.hpp:30-55, RenderFuncInput.hpp:42-49,
+// cuda_blend.hpp:24-44, cuda_argb_transform.hpp:26-43) in a fixed fp32
+// operation order, so every kernel of this library produces the same bits.
+#pragma once
+
+#include "../../include/revise_hip.h"
+
+#include <hip/hip_runtime.h>
+
+namespace rv {
+
+// pow replacement for 0 <= b <= 1, e > 0 (the reference calls CUDA powf,
+// renderArgb.hpp:45).  Method: b = m 2^k with m in [sqrt(1/2), sqrt(2));
+// log2 b = k + (2/ln2) atanh(s), s = (m-1)/(m+1), by the odd Taylor series of
+// atanh to s^9; 2^y = 2^floor(y) P(y - floor(y)) with P a degree-6 polynomial
+// fitted (Lawson-reweighted least squares) to 2^f on [0,1]; relative error
+// below 1e-6 over the renderer's range.  Results under 2^-126 are 0.
+__device__ __forceinline__ float pow01(float b, float e)
+{
+    if (!(b >= 0.f))
+        return __builtin_nanf("");
+    if (b == 0.f)
+        return 0.f;
+    if (b == 1.f)
+        return 1.f;
+    uint32_t bits = __float_as_uint(b);
+    int k = (int)((bits >> 23) & 0xffu) - 127;
+    uint32_t mbits = (bits & 0x007fffffu) | 0x3f800000u;
+    if (((bits >> 23) & 0xffu) == 0u) {
+        const float bs = b * 4294967296.0f;
+        bits = __float_as_uint(bs);
+        k = (int)((bits >> 23) & 0xffu) - 127 - 32;
+        mbits = (bits & 0x007fffffu) | 0x3f800000u;
+    }
+    float m = __uint_as_float(mbits);
+    if (m >= 1.41421356f) {
+        m = m * 0.5f;
+        k = k + 1;
+    }
+    const float s = (m - 1.0f) / (m + 1.0f);
+    const float s2 = s * s;
+    float poly = 0.111111111f;
+    poly = poly * s2 + 0.142857143f;
+    poly = poly * s2 + 0.2f;
+    poly = poly * s2 + 0.333333333f;
+    poly = poly * s2 + 1.0f;
+    const float l = (s * poly) * 2.88539008f;
+    const float y = e * ((float)k + l);
+    if (y < -126.0f)
+        return 0.f;
+    if (y >= 0.0f)
+        return 1.0f;
+    const float fl = floorf(y);
+    const float f = y - fl;
+    float p = 0x1.c7204cp-13f;
+    p = p * f + 0x1.4619a2p-10f;
+    p = p * f + 0x1.3d27f6p-7f;
+    p = p * f + 0x1.c68502p-5f;
+    p = p * f + 0x1.ebfd9ep-3f;
+    p = p * f + 0x1.62e42ap-1f;
+    p = p * f + 1.0f;
+    return ldexpf(p, (int)fl);
+}
+
+__device__ __forceinline__ int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return (1.f - t) * a + t * b; }
+
+// Texel index pair and weight along one axis, unnormalized coordinates with
+// texel centres at i + 0.5 and clamp addressing.
+__device__ __forceinline__ void axis_taps(float t, int n, int& i0, int& i1, float& f)
+{
+    float v = t - 0.5f;
+    if (!(v > -1.f))
+        v = -1.f;
+    if (v > (float)n)
+        v = (float)n;
+    const float fl = floorf(v);
+    f = v - fl;
+    const int i = (int)fl;
+    i0 = clampi(i, 0, n - 1);
+    i1 = clampi(i + 1, 0, n - 1);
+}
+
+// Exact fp32 trilinear fetch (the tex3D of RenderFuncInput.hpp:46-49).
+template <class T>
+__device__ __forceinline__ float fetch3(const T* __restrict__ brick, int n, float tx, float ty,
+                                        float tz)
+{
+    int x0, x1, y0, y1, z0, z1;
+    float fx, fy, fz;
+    axis_taps(tx, n, x0, x1, fx);
+    axis_taps(ty, n, y0, y1, fy);
+    axis_taps(tz, n, z0, z1, fz);
+    const uint32_t a = ((uint32_t)z0 * n + y0) * n + x0;
+    const uint32_t dx = (uint32_t)(x1 - x0);
+    const uint32_t dy = (uint32_t)(y1 - y0) * n;
+    const uint32_t dz = (uint32_t)(z1 - z0) * n * n;
+    const float v000 = (float)brick[a], v100 = (float)brick[a + dx];
+    const float v010 = (float)brick[a + dy], v110 = (float)brick[a + dy + dx];
+    const float v001 = (float)brick[a + dz], v101 = (float)brick[a + dz + dx];
+    const float v011 = (float)brick[a + dz + dy], v111 = (float)brick[a + dz + dy + dx];
+    const float c00 = lerpf(v000, v100, fx);
+    const float c10 = lerpf(v010, v110, fx);
+    const float c01 = lerpf(v001, v101, fx);
+    const float c11 = lerpf(v011, v111, fx);
+    const float c0 = lerpf(c00, c10, fy);
+    const float c1 = lerpf(c01, c11, fy);
+    return lerpf(c0, c1, fz);
+}
+
+// Normalized, linear, clamped 1-D fetch of the colour table (tex1D,
+// DenseFieldRenderer_cu.cu:573-589) from a float4 table in LDS.
+__device__ __forceinline__ float4 fetch_tf(const float4* tf, int N, float u)
+{
+    float x = u * (float)N - 0.5f;
+    if (!(x > -1.f))
+        x = -1.f;
+    if (x > (float)N)
+        x = (float)N;
+    const float fl = floorf(x);
+    const float f = x - fl;
+    const int i = (int)fl;
+    const float4 a = tf[clampi(i, 0, N - 1)];
+    const float4 b = tf[clampi(i + 1, 0, N - 1)];
+    return make_float4(lerpf(a.x, b.x, f), lerpf(a.y, b.y, f), lerpf(a.z, b.z, f),
+                       lerpf(a.w, b.w, f));
+}
+
+struct RaySeg {
+    bool hit;
+    int sample_count;
+    float px, py, pz;   // first sample position
+    float sx, sy, sz;   // step
+};
+
+// Ray setup and slab test, DenseFieldRenderer_cu.cu:66-181.
+__device__ __forceinline__ RaySeg setup_ray(const rv_block_params& p, uint32_t vx, uint32_t vy)
+{
+    RaySeg r;
+    r.hit = false;
+    const float x = (-0.5f + (vx + 0.5f) / p.vp_w) * p.w;
+    const float y = (-0.5f + (vy + 0.5f) / p.vp_h) * p.h;
+    const float rx = (p.n[0] + p.e1[0] * x) + p.e2[0] * y;
+    const float ry = (p.n[1] + p.e1[1] * x) + p.e2[1] * y;
+    const float rz = (p.n[2] + p.e1[2] * x) + p.e2[2] * y;
+    const float rv[3] = {rx, ry, rz};
+    const float ev[3] = {p.pos[0], p.pos[1], p.pos[2]};
+    float tauEnter = 1.f, tauExit = 0.f;
+    const float h = p.half;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (!(fabsf(rv[a]) > 1e-10f))
+            continue;
+        const int o1 = a == 0 ? 1 : 0, o2 = a == 2 ? 1 : 2;
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const float face = side == 0 ? p.center[a] - h : p.center[a] + h;
+            const float tau = (face - ev[a]) / rv[a];
+            const float q1 = ev[o1] + rv[o1] * tau;
+            const float q2 = ev[o2] + rv[o2] * tau;
+            if (fabsf(q1 - p.center[o1]) <= h && fabsf(q2 - p.center[o2]) <= h) {
+                const bool enter = side == 0 ? (rv[a] > 0) : !(rv[a] > 0);
+                if (enter)
+                    tauEnter = tau;
+                else
+                    tauExit = tau;
+            }
+        }
+    }
+    if (tauEnter < 1)
+        tauEnter = 1;
+    if (!(tauExit > tauEnter))
+        return r;
+    r.hit = true;
+    const float dt = tauExit - tauEnter;
+    const float dotrr = (rx * rx + ry * ry) + rz * rz;
+    const float l1 = (fabsf(rx) + fabsf(ry)) + fabsf(rz);
+    r.sample_count = (int)(p.quality * dt * dotrr / (p.texel * l1));
+    const float k = dt / (float)r.sample_count;
+    r.sx = rx * k;
+    r.sy = ry * k;
+    r.sz = rz * k;
+    r.px = ev[0] + rx * tauEnter;
+    r.py = ev[1] + ry * tauEnter;
+    r.pz = ev[2] + rz * tauEnter;
+    return r;
+}
+
+// One sample's colour after the transfer table and the opacity correction
+// (renderArgb.hpp:42-45); v must not be NaN.
+__device__ __forceinline__ float4 classify(const rv_block_params& p, const float4* tf, int tf_n,
+                                           float v)
+{
+    const float v01 = (v - p.fmin) / (p.fmax - p.fmin);
+    float4 c = fetch_tf(tf, tf_n, v01);
+    c.w = 1.f - pow01(1.f - c.w * p.opacity_factor, p.i_layer_count);
+    return c;
+}
+
+// blendBehind (cuda_blend.hpp:24-33): acc is in front of c.
+__device__ __forceinline__ void blend_behind(float4& acc, const float4& c)
+{
+    const float da = c.w * (1.f - acc.w);
+    const float a = acc.w + da;
+    if (a > 0.f) {
+        const float ka = acc.w / a, kb = da / a;
+        acc.x = acc.x * ka + c.x * kb;
+        acc.y = acc.y * ka + c.y * kb;
+        acc.z = acc.z * ka + c.z * kb;
+    }
+    acc.w = a;
+}
+
+// uint32ArgbToFloatRgba + blendAbove + floatRgbaToUint32Argb
+// (cuda_argb_transform.hpp:26-43, cuda_blend.hpp:35-44).
+__device__ __forceinline__ uint32_t blend_above_u32(uint32_t argb, const float4& in)
+{
+    float ex = ((argb >> 16) & 0xffu) / 255.f;
+    float ey = ((argb >> 8) & 0xffu) / 255.f;
+    float ez = (argb & 0xffu) / 255.f;
+    const float ew = ((argb >> 24) & 0xffu) / 255.f;
+    const float da = ew * (1.f - in.w);
+    const float a = in.w + da;
+    if (a > 0.f) {
+        const float ka = da / a, kb = in.w / a;
+        ex = ex * ka + in.x * kb;
+        ey = ey * ka + in.y * kb;
+        ez = ez * ka + in.z * kb;
+    }
+    return ((unsigned)(a * 255.999f) << 24) + ((unsigned)(ex * 255.999f) << 16) +
+           ((unsigned)(ey * 255.999f) << 8) + (unsigned)(ez * 255.999f);
+}
+
+// March one block segment from rgba = 0 (renderArgb.hpp:39-51).
+template <class T>
+__device__ __forceinline__ float4 march(const rv_block_params& p, const T* __restrict__ brick,
+                                        const float4* tf, int tf_n, RaySeg r)
+{
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int n = (int)p.nodes;
+    for (int i = 1; i < r.sample_count; ++i) {
+        const float tx = (r.px - p.tex_origin[0]) * p.tcf;
+        const float ty = (r.py - p.tex_origin[1]) * p.tcf;
+        const float tz = (r.pz - p.tex_origin[2]) * p.tcf;
+        const float v = fetch3(brick, n, tx, ty, tz);
+        if (v == v) {
+            const float4 c = classify(p, tf, tf_n, v);
+            blend_behind(acc, c);
+            if (acc.w > p.alpha_cutoff)
+                break;
+        }
+        r.px = r.px + r.sx;
+        r.py = r.py + r.sy;
+        r.pz = r.pz + r.sz;
+    }
+    return acc;
+}
+
+} // namespace rv

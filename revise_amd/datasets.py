@@ -1,0 +1,217 @@
+"""Seeded synthetic block-tree datasets for the benchmark configurations.
+
+A dataset is a set of subtree roots ("blocks": level + index in [-5,5]^3,
+the reference's world cube) each with a local block tree (children array,
+node 0 = root, child local index bit0 = x), its n2i node list and fp32
+sparse values sampled from an analytic field at the node positions.  Nodes
+outside a block's occupied region carry the reference's "no field" value
+1e30.
+
+Field: sum of isotropic Gaussians (seeded centres/sigmas) plus optional
+multi-octave value noise, evaluated in float64 on the unit cube and rounded
+to float32, like the reference's exact-field providers sample a function at
+tree nodes (BlockTreeFieldFromFunc.hpp:51-87).
+"""
+import math
+
+import numpy as np
+
+from . import api
+
+NO_FIELD = np.float32(1e30)
+
+
+# ---------------------------------------------------------------- fields
+
+class Field:
+    def __init__(self, seed, n_gauss=3, centre=(0.2, 0.8), sigma=(0.05, 0.15), noise_octaves=0,
+                 noise_amp=0.1):
+        rng = np.random.default_rng(seed)
+        self.c = rng.uniform(centre[0], centre[1], size=(n_gauss, 3))
+        self.s = rng.uniform(sigma[0], sigma[1], size=n_gauss)
+        self.octaves = noise_octaves
+        self.amp = noise_amp
+        # value-noise lattices of 4, 8, 16, ... cells per axis
+        self.lat = [rng.uniform(-1.0, 1.0, size=(4 * 2 ** o + 1,) * 3) for o in range(noise_octaves)]
+
+    def __call__(self, u):
+        """u: (..., 3) float64 unit-cube coordinates -> float64 values."""
+        u = np.asarray(u, dtype=np.float64)
+        f = np.zeros(u.shape[:-1])
+        for c, s in zip(self.c, self.s):
+            d2 = ((u - c) ** 2).sum(-1)
+            f += np.exp(-d2 / (2 * s * s))
+        for o, L in enumerate(self.lat):
+            cells = L.shape[0] - 1
+            x = np.clip(u, 0.0, 1.0) * cells
+            i = np.minimum(np.floor(x).astype(np.int64), cells - 1)
+            t = x - i
+            t = t * t * (3 - 2 * t)  # smoothstep
+            v = 0.0
+            for dz in (0, 1):
+                for dy in (0, 1):
+                    for dx in (0, 1):
+                        w = ((t[..., 0] if dx else 1 - t[..., 0]) *
+                             (t[..., 1] if dy else 1 - t[..., 1]) *
+                             (t[..., 2] if dz else 1 - t[..., 2]))
+                        v = v + w * L[i[..., 0] + dx, i[..., 1] + dy, i[..., 2] + dz]
+            f += self.amp * v / (2 ** o)
+        return f
+
+    def grad_norm(self, u, h=1e-4):
+        g = np.zeros(np.asarray(u).shape[:-1])
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            g += ((self(u + e) - self(u - e)) / (2 * h)) ** 2
+        return np.sqrt(g)
+
+
+# ---------------------------------------------------------------- trees
+
+def full_subtree(depth):
+    """Children array of a fully refined subtree of the given depth
+    (breadth-first numbering)."""
+    counts = [8 ** l for l in range(depth + 1)]
+    total = sum(counts)
+    ch = np.zeros((total, 8), dtype=np.uint32)
+    start = 0
+    for l in range(depth):
+        n = counts[l]
+        nxt = start + n
+        ch[start:start + n] = (nxt + np.arange(n * 8, dtype=np.uint32)).reshape(n, 8)
+        start = nxt
+    return ch
+
+
+def refined_subtree(depth, refine):
+    """Children array of a subtree refined where refine(level, locs) is True.
+    refine gets the relative level and an (n,3) int array of block locations
+    at that level and returns a bool array."""
+    nodes_loc = [np.zeros((1, 3), dtype=np.int64)]
+    level_start = [0]
+    ch_rows = []
+    total = 1
+    for l in range(depth):
+        locs = nodes_loc[-1]
+        mask = np.asarray(refine(l, locs), dtype=bool)
+        rows = np.zeros((locs.shape[0], 8), dtype=np.uint32)
+        k = np.arange(8)
+        bits = np.stack([k & 1, (k >> 1) & 1, (k >> 2) & 1], axis=1)
+        sel = np.nonzero(mask)[0]
+        nchild = sel.size * 8
+        rows[sel] = (total + np.arange(nchild, dtype=np.uint32)).reshape(sel.size, 8)
+        ch_rows.append(rows)
+        child_locs = (locs[sel][:, None, :] * 2 + bits[None, :, :]).reshape(-1, 3)
+        nodes_loc.append(child_locs)
+        level_start.append(total)
+        total += nchild
+        if nchild == 0:
+            break
+    ch_rows.append(np.zeros((nodes_loc[-1].shape[0], 8), dtype=np.uint32))
+    ch = np.concatenate(ch_rows[: len(nodes_loc)], axis=0)
+    assert ch.shape[0] == total
+    return ch
+
+
+# ---------------------------------------------------------------- blocks
+
+class Block:
+    __slots__ = ("level", "index", "depth", "n2i", "sparse", "children")
+
+    def __init__(self, level, index, depth, n2i, sparse, children=None):
+        self.level = level
+        self.index = tuple(int(v) for v in index)
+        self.depth = depth
+        self.n2i = n2i
+        self.sparse = sparse
+        self.children = children
+
+
+def node_unit_coords(level, index, depth, n2i):
+    """Unit-cube coordinates of a block's nodes."""
+    scale = 1.0 / (2 ** (level + depth))
+    base = np.asarray(index, dtype=np.float64) * (2 ** depth)
+    return (base[None, :] + n2i.astype(np.float64)) * scale
+
+
+def sample(field, level, index, depth, n2i, occupied=None):
+    u = node_unit_coords(level, index, depth, n2i)
+    v = field(u).astype(np.float32)
+    if occupied is not None:
+        v = np.where(occupied(u), v, NO_FIELD).astype(np.float32)
+    return v
+
+
+def field_range(blocks):
+    lo, hi = np.inf, -np.inf
+    for b in blocks:
+        v = b.sparse[b.sparse != NO_FIELD]
+        if v.size:
+            lo = min(lo, float(v.min()))
+            hi = max(hi, float(v.max()))
+    return np.float32(lo), np.float32(hi)
+
+
+def uniform_level(level, depth, field, n2i=None):
+    """All 8^level blocks of a level, each a fully refined depth-`depth`
+    subtree (full fill)."""
+    if n2i is None:
+        n2i = api.subtree_n2i(full_subtree(depth), depth)
+    blocks = []
+    m = 2 ** level
+    for z in range(m):
+        for y in range(m):
+            for x in range(m):
+                blocks.append(Block(level, (x, y, z), depth, n2i,
+                                    sample(field, level, (x, y, z), depth, n2i)))
+    return blocks
+
+
+# ---------------------------------------------------------------- transfer function
+
+def seeded_tf(seed, size=256, points=6):
+    """Seeded piecewise-linear RGBA ramps sampled into a size x 4 table with
+    makeColorTransferVector (ColorTransferFunction.hpp:27-69)."""
+    rng = np.random.default_rng(seed)
+    xs = np.sort(rng.uniform(0.0, 1.0, size=points - 2))
+    xs = np.concatenate([[0.0], xs, [1.0]]).astype(np.float32)
+    cols = rng.uniform(0.0, 1.0, size=(points, 4)).astype(np.float32)
+    cols[0, 3] = 0.0  # transparent at the bottom of the range
+    return api.color_transfer_vector(xs, cols, size)
+
+
+# ---------------------------------------------------------------- camera
+
+def look_at(eye, center=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0)):
+    """Row-major world->eye rigid transform of a right-handed camera looking
+    along -Z (the VisualizationLibrary getLookAt convention)."""
+    eye = np.asarray(eye, dtype=np.float64)
+    f = np.asarray(center, dtype=np.float64) - eye
+    f /= np.linalg.norm(f)
+    s = np.cross(f, np.asarray(up, dtype=np.float64))
+    s /= np.linalg.norm(s)
+    u = np.cross(s, f)
+    R = np.stack([s, u, -f])
+    t = -R @ eye
+    M = np.eye(4)
+    M[:3, :3] = R
+    M[:3, 3] = t
+    return M.astype(np.float32)
+
+
+def orbit_camera(angle_deg, fov_y_deg=30.0, elevation_deg=20.0):
+    """Camera on the orbit of the reference's default distance
+    L = 10 sqrt(3) 1.01 / (2 sin(fovY/2)) (CameraController.cpp:88-101)."""
+    L = 10.0 * math.sqrt(3.0) * 1.01 / (2.0 * math.sin(0.5 * math.radians(fov_y_deg)))
+    a, e = math.radians(angle_deg), math.radians(elevation_deg)
+    eye = L * np.array([math.cos(e) * math.sin(a), math.sin(e), math.cos(e) * math.cos(a)])
+    return look_at(eye)
+
+
+def default_camera(fov_y_deg=30.0):
+    """CameraController::resetCameraTransform: eye along (2,1,3)."""
+    d = np.array([2.0, 1.0, 3.0])
+    d /= np.linalg.norm(d)
+    L = 10.0 * math.sqrt(3.0) * 1.01 / (2.0 * math.sin(0.5 * math.radians(fov_y_deg)))
+    return look_at(d * L)

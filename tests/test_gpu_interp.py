@@ -1,0 +1,144 @@
+"""HIP interpolation against the CPU oracle (DenseFieldInterpolator.hpp:198-419
+restated): bit-exact index maps and bit-exact dense bricks (the bar is 1e-6
+relative; the kernels keep the CPU operand order, so results are identical)."""
+import numpy as np
+import pytest
+
+from revise_amd import api, datasets
+from tests.gpu_util import bits_equal_nan, say, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed, depth, p_refine, p_nofield=0.0):
+    rng = np.random.default_rng(seed)
+    ch = datasets.refined_subtree(depth, lambda l, locs: rng.random(locs.shape[0]) < p_refine)
+    d = api.subtree_depth(ch, depth)
+    n2i = api.subtree_n2i(ch, d)
+    vals = rng.normal(size=n2i.shape[0]).astype(np.float32)
+    if p_nofield:
+        vals[rng.random(n2i.shape[0]) < p_nofield] = np.float32(1e30)
+    return d, n2i, vals
+
+
+def test_ordinals_bit_exact(orc):
+    import torch
+    for seed in range(4):
+        d, n2i, _ = _case(seed, 6, 0.3)
+        say("ordinals seed", seed, "depth", d, "n", n2i.shape[0])
+        out = torch.empty(n2i.shape[0], dtype=torch.int32, device="cuda")
+        api.node_ordinals(to_dev(n2i.view(np.int16)), d, out)
+        got = out.cpu().numpy().view(np.uint32)
+        assert np.array_equal(got, orc.ordinals(n2i, d))
+
+
+@pytest.mark.parametrize("seed,depth,p,pnf", [
+    (0, 0, 0.0, 0.0), (1, 1, 1.0, 0.0), (2, 2, 0.6, 0.0), (3, 3, 0.5, 0.05), (4, 4, 0.4, 0.0),
+    (5, 5, 0.35, 0.02), (6, 6, 0.3, 0.0), (7, 6, 0.45, 0.01), (8, 6, 1.0, 0.0), (9, 7, 0.25, 0.0),
+    (10, 8, 0.12, 0.0)])
+def test_interp_planned_bit_exact(orc, seed, depth, p, pnf):
+    import torch
+    d, n2i, vals = _case(seed, depth, p, pnf)
+    say("interp seed", seed, "depth", d, "nodes", n2i.shape[0])
+    V = api.brick_nodes(d)
+    plan = torch.empty(V, dtype=torch.int32, device="cuda")
+    dense = torch.empty(V, dtype=torch.float32, device="cuda")
+    bad = torch.zeros(1, dtype=torch.int32, device="cuda")
+    dn2i = to_dev(n2i.view(np.int16))
+    api.interp_plan_build(dn2i, d, plan, bad)
+    api.interp_brick_planned(to_dev(vals), plan, d, dense)
+    got = dense.cpu().numpy()
+    exp = orc.interpolate(vals, n2i, d)
+    assert int(bad.item()) == 0
+    assert bits_equal_nan(got, exp)
+
+
+def test_interp_unplanned_and_workspace(orc):
+    import torch
+    d, n2i, vals = _case(21, 5, 0.4, 0.02)
+    V = api.brick_nodes(d)
+    exp = orc.interpolate(vals, n2i, d)
+    dn2i, dv = to_dev(n2i.view(np.int16)), to_dev(vals)
+    a = torch.empty(V, dtype=torch.float32, device="cuda")
+    api.interp_brick(dv, dn2i, d, a)  # library-allocated workspace
+    ws = torch.empty(V, dtype=torch.int32, device="cuda")
+    b = torch.empty(V, dtype=torch.float32, device="cuda")
+    api.interp_brick(dv, dn2i, d, b, ws)
+    assert bits_equal_nan(a.cpu().numpy(), exp)
+    assert bits_equal_nan(b.cpu().numpy(), exp)
+
+
+def test_interp_plan_counts_bad_coordinates():
+    import torch
+    n2i = np.array([[0, 0, 0], [3, 0, 0], [0, 9, 0]], dtype=np.uint16)  # depth 2: max coord 4
+    plan = torch.empty(api.brick_nodes(2), dtype=torch.int32, device="cuda")
+    bad = torch.zeros(1, dtype=torch.int32, device="cuda")
+    api.interp_plan_build(to_dev(n2i.view(np.int16)), 2, plan, bad)
+    assert int(bad.item()) == 1
+    p = plan.cpu().numpy().view(np.uint32)
+    assert p[0] == 0 and p[3] == 1 and (p == 0xFFFFFFFF).sum() == p.size - 2
+
+
+def test_interp_trilinear_field_reproduced():
+    """Closed form: a globally trilinear field sampled at the nodes of any
+    refinement pattern is reproduced at every interpolated node (edge, face and
+    cell means are exact for trilinear functions)."""
+    import torch
+    rng = np.random.default_rng(4)
+    ch = datasets.refined_subtree(6, lambda l, locs: rng.random(locs.shape[0]) < 0.3)
+    d = api.subtree_depth(ch, 6)
+    n2i = api.subtree_n2i(ch, d)
+    c = rng.uniform(-1, 1, size=8)
+    def f(x, y, z):
+        return c[0] + c[1] * x + c[2] * y + c[3] * z + c[4] * x * y + c[5] * y * z + c[6] * x * z + c[7] * x * y * z
+    u = n2i.astype(np.float64) / 2 ** d
+    vals = f(u[:, 0], u[:, 1], u[:, 2]).astype(np.float32)
+    V = api.brick_nodes(d)
+    dense = torch.empty(V, dtype=torch.float32, device="cuda")
+    api.interp_brick(to_dev(vals), to_dev(n2i.view(np.int16)), d, dense)
+    n = 2 ** d + 1
+    g = np.arange(n) / 2 ** d
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    exp = f(x, y, z).reshape(-1)
+    assert np.max(np.abs(dense.cpu().numpy() - exp)) < 1e-5
+
+
+def test_interp_batch_mixed_depths_fp32_fp16(orc):
+    import torch
+    cases = [_case(100 + i, dep, p, 0.01) for i, (dep, p) in
+             enumerate([(6, 0.3), (4, 0.5), (6, 1.0), (2, 0.9), (5, 0.4), (0, 0.0), (7, 0.2)])]
+    jobs, outs, exps = [], [], []
+    for i, (d, n2i, vals) in enumerate(cases):
+        V = api.brick_nodes(d)
+        plan = torch.empty(V, dtype=torch.int32, device="cuda")
+        api.interp_plan_build(to_dev(n2i.view(np.int16)), d, plan)
+        dt = torch.float16 if i % 2 else torch.float32
+        dense = torch.empty(V, dtype=dt, device="cuda")
+        jobs.append((to_dev(vals), plan, dense, d))
+        outs.append(dense)
+        exps.append(orc.interpolate(vals, n2i, d))
+    b = api.InterpBatch(jobs)
+    tiles, nodes = b.info()
+    assert nodes == sum(api.brick_nodes(c[0]) for c in cases)
+    say("batch tiles", tiles)
+    b.run()
+    torch.cuda.synchronize()
+    for i, (o, e) in enumerate(zip(outs, exps)):
+        got = o.cpu().numpy()
+        if got.dtype == np.float16:
+            assert bits_equal_nan(got.astype(np.float32), e.astype(np.float16).astype(np.float32))
+        else:
+            assert bits_equal_nan(got, e)
+    b.close()
+
+
+def test_interp_full_fill_is_scatter(orc):
+    """Full-fill subtree: no node is interpolated, the brick is the scatter."""
+    import torch
+    d = 4
+    n2i = api.subtree_n2i(datasets.full_subtree(d), d)
+    vals = np.arange(n2i.shape[0], dtype=np.float32)
+    dense = torch.empty(api.brick_nodes(d), dtype=torch.float32, device="cuda")
+    api.interp_brick(to_dev(vals), to_dev(n2i.view(np.int16)), d, dense)
+    out = dense.cpu().numpy()
+    assert np.array_equal(out[orc.ordinals(n2i, d)], vals)
