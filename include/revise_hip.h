@@ -242,6 +242,122 @@ int rv_fill_u32(uint32_t* d_dst, uint64_t n, uint32_t value, rv_stream_t stream)
 /* floatRgbToUint32Argb (VsRenderer.cpp:115-122). */
 uint32_t rv_background_argb(float r, float g, float b);
 
+
+/* ------------------------------------------------------------------------
+ * Batched persistent frame (device)
+ * ------------------------------------------------------------------------
+ * No single reference counterpart: it replaces the per-block loop of
+ * VsWorker_cu::renderScenePart (VsWorker_cu.cpp:254-277) -- one
+ * DenseFieldRenderer_cu::renderDenseField call, i.e. one interpolation and one
+ * renderKernel launch, per block -- by one launch per frame over all blocks of
+ * one GPU's share, of uneven levels, composited front to back in registers
+ * (blendBehind, cuda_blend.hpp:24-33) instead of through the 8-bit viewport.
+ * The handle owns device copies of the block table, per-camera parameters,
+ * block orders (rv_mixed_back_to_front) and view rectangles, sync counters,
+ * and an optional captured graph; bricks, sparse values, plans, colour table
+ * and outputs stay caller-owned. */
+
+typedef struct rv_frame_block {
+    uint32_t level;          /* block (subtree root) level */
+    uint32_t index[3];       /* block index at that level */
+    uint32_t depth;          /* brick depth */
+    const float* sparse;     /* sparse values (for per-frame interpolation), may be NULL */
+    const uint32_t* plan;    /* gather plan (rv_interp_plan_build), may be NULL */
+    float* brick;            /* fp32 brick, or */
+    uint16_t* brick16;       /* fp16 brick; one kind for all blocks of a plan */
+} rv_frame_block;
+
+typedef struct rv_frame_desc {
+    uint32_t vp_w, vp_h;
+    float fov_y_deg;
+    float fmin, fmax, threshold_rel, quality;
+    float alpha_cutoff;      /* per-block segment early termination (reference 0.996) */
+    float acc_cutoff;        /* stop compositing a pixel once its alpha exceeds this */
+    const float* d_tf;       /* tf_n x 4 colour table on the device */
+    uint32_t tf_n;           /* 1..1024 */
+    uint32_t tile_w, tile_h; /* screen tile, tile_w*tile_h == 512; 0,0 = 32x16 */
+} rv_frame_desc;
+
+typedef struct rv_frame_plan rv_frame_plan;
+
+int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nblocks, const rv_frame_desc* desc,
+                         rv_frame_plan** out);
+/* Cameras (row-major world->eye transforms, ncams x 16 floats): per camera the
+ * host computes every block's parameters (rv_block_params_setup) and the
+ * front-to-back order, and uploads them. */
+int rv_frame_plan_set_cameras(rv_frame_plan* plan, const float* views16, uint32_t ncams);
+/* One frame for camera `cam`: a counter-reset kernel and one persistent
+ * launch.  interpolate != 0 rebuilds every brick from its sparse values
+ * within the same launch.  d_argb: vp_w*vp_h ARGB8 (rows bottom-up, like the
+ * reference viewport); d_rgba: optional vp_w*vp_h*4 float output. */
+int rv_frame_render(rv_frame_plan* plan, uint32_t cam, uint32_t* d_argb, float* d_rgba,
+                    int interpolate, rv_stream_t stream);
+/* Capture the frames of all cameras into one HIP graph (camera c writes
+ * d_argb + c*frame_stride) and replay it with rv_frame_graph_launch. */
+int rv_frame_graph_build(rv_frame_plan* plan, uint32_t* d_argb, uint64_t frame_stride,
+                         int interpolate, rv_stream_t stream);
+int rv_frame_graph_launch(rv_frame_plan* plan, rv_stream_t stream);
+/* Sticky error flags of the last frames (1: a brick wait gave up); synchronous. */
+int rv_frame_status(rv_frame_plan* plan, uint32_t* error_flags);
+int rv_frame_get_tile(const rv_frame_plan* plan, uint32_t* tile_w, uint32_t* tile_h);
+/* Pick the screen-tile shape by timing candidates on camera 0; the choice is
+ * cached (one JSON object per line) in cache_path when it is not NULL. */
+int rv_frame_autotune(rv_frame_plan* plan, uint32_t* d_argb, int interpolate,
+                      const char* cache_path, rv_stream_t stream, uint32_t* tile_w,
+                      uint32_t* tile_h);
+void rv_frame_plan_destroy(rv_frame_plan* plan);
+
+/* ------------------------------------------------------------------------
+ * Worker (host + device): the VsWorkerInterface drop-in behind a C ABI
+ * ------------------------------------------------------------------------
+ * rv_worker bundles a field service (subtrees by level/index with their n2i
+ * and sparse values, copied to the device; gather plans cached per subtree),
+ * the render shared state, and a VsWorkerHip (include/revise/vs_worker_hip.hpp).
+ * It owns its device buffers and one HIP stream. */
+
+typedef struct rv_worker rv_worker;
+
+typedef struct rv_render_state {
+    const float* view16;     /* row-major world->eye transform (cameraTransform) */
+    uint32_t vp_w, vp_h;     /* viewportSize */
+    float fov_y_deg;         /* fovY */
+    float quality;           /* renderQuality */
+    float threshold_rel;     /* threshold, relative to the field range */
+    float alpha_cutoff;      /* early ray termination (reference 0.996) */
+    float fmin, fmax;        /* field range (BlockTreeFieldService::fieldRange) */
+    float background[3];     /* backgroundColor */
+    const float* tf;         /* host colour table, tf_n x 4 */
+    uint32_t tf_n;
+} rv_render_state;
+
+/* VsWorkerInterface::initialize: selects GPU thread_id % device count
+ * (VsWorker_cu.cpp:96-104, selectGpu.cu:6-9). */
+int rv_worker_create(uint32_t thread_id, rv_worker** out);
+/* Register a subtree root of the field (BlockTreeFieldProvider data of one
+ * subtree, BlockTreeFieldProvider.hpp:156-168): n2i (n x 3) and sparse values
+ * are host arrays, copied. */
+int rv_worker_add_subtree(rv_worker* w, uint32_t level, const uint32_t index[3], uint32_t depth,
+                          const uint16_t* n2i, uint64_t n, const float* sparse);
+/* setRenderSharedState + updateState (VsWorkerInterface.hpp:38-111). */
+int rv_worker_set_state(rv_worker* w, const rv_render_state* state);
+/* renderScenePart (VsWorker_cu.cpp:225-313): clear, BackToFrontOrder over the
+ * index box, interpolate + ray-cast each block, crop to the non-transparent
+ * rectangle.  rect = {x, y, w, h} (w = h = 0 for an empty part); bits
+ * receives w*h ARGB pixels (cap pixels available).  *cancel is polled between
+ * blocks; when set the call returns RV_ERR_CANCELLED with an empty part.
+ * An absent block gives RV_ERR_INVALID ("Block does not exist"). */
+int rv_worker_render_scene_part(rv_worker* w, uint32_t level, const uint32_t box_min[3],
+                                const uint32_t box_max[3], const volatile int* cancel,
+                                int32_t rect[4], uint32_t* bits, uint64_t cap);
+/* One level's frame with split_count worker boxes on this GPU: boxes from
+ * rv_index_cube_split_sorted, one part each, assembled pairwise on the device
+ * (AssembleFunc, VsRenderer.cpp:403-473, tree of RenderGraphGenerator.cpp:37-129)
+ * and composed over the background (FinalComposeFunc, VsRenderer.cpp:475-519).
+ * frame: vp_w*vp_h ARGB. */
+int rv_worker_compose_frame(rv_worker* w, uint32_t level, uint32_t split_count, uint32_t* frame,
+                            const volatile int* cancel);
+void rv_worker_destroy(rv_worker* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1208,8 +1208,8 @@ extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, u
                                            float fov_y_deg, const orc_block_ref* blocks,
                                            uint32_t nblocks, float fmin, float fmax,
                                            float threshold_rel, float quality, float alpha_cutoff,
-                                           const float* tf, uint32_t tf_n, float* out,
-                                           uint32_t nthreads)
+                                           float acc_cutoff, const float* tf, uint32_t tf_n,
+                                           float* out, uint32_t nthreads)
 {
     std::vector<orc_block_params> params(nblocks);
     std::vector<Derived> der(nblocks);
@@ -1227,6 +1227,8 @@ extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, u
                 float acc[4] = {0, 0, 0, 0};
                 /* front to back: the reverse of the back-to-front list */
                 for (uint32_t bi = nblocks; bi-- > 0;) {
+                    if (acc[3] > acc_cutoff)
+                        break; /* the pixel is opaque: nothing behind shows */
                     if (!live[bi])
                         continue;
                     const orc_block_params* p = &params[bi];

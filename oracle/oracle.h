@@ -151,7 +151,9 @@ void orc_render_share(const orc_frame_desc* d, const float* const* bricks, uint3
 
 /* Float front-to-back composite of per-block segments for a list of blocks
  * given in back-to-front order (the compositing operator of the reference,
- * evaluated without the per-block 8-bit viewport round trip).
+ * evaluated without the per-block 8-bit viewport round trip).  Blocks behind a
+ * pixel whose accumulated alpha exceeds acc_cutoff are skipped (acc_cutoff >=
+ * 1 composites every block).
  * out_rgba: vp_w*vp_h*4 floats (non-premultiplied, like the reference). */
 typedef struct orc_block_ref {
     uint32_t level;
@@ -163,7 +165,8 @@ typedef struct orc_block_ref {
 void orc_render_float_composite(const float* view16, uint32_t vp_w, uint32_t vp_h, float fov_y_deg,
                                 const orc_block_ref* blocks_b2f, uint32_t nblocks, float fmin,
                                 float fmax, float threshold_rel, float quality, float alpha_cutoff,
-                                const float* tf, uint32_t tf_n, float* out_rgba, uint32_t nthreads);
+                                float acc_cutoff, const float* tf, uint32_t tf_n, float* out_rgba,
+                                uint32_t nthreads);
 
 #ifdef __cplusplus
 }

@@ -94,8 +94,8 @@ def lib():
         L.orc_render_share.argtypes = [C.POINTER(FrameDesc), C.POINTER(f32p), u32p, C.c_uint32]
         L.orc_render_float_composite.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_float,
                                                  C.POINTER(BlockRef), C.c_uint32, C.c_float, C.c_float,
-                                                 C.c_float, C.c_float, C.c_float, f32p, C.c_uint32,
-                                                 f32p, C.c_uint32]
+                                                 C.c_float, C.c_float, C.c_float, C.c_float, f32p,
+                                                 C.c_uint32, f32p, C.c_uint32]
         _lib = L
     return _lib
 
@@ -266,7 +266,7 @@ def render_share(view16, vp_w, vp_h, fov, level, box_min, box_max, tex_depth, fm
 
 
 def render_float_composite(view16, vp_w, vp_h, fov, blocks_b2f, fmin, fmax, thr_rel, quality,
-                           alpha_cutoff, tf, nthreads=None):
+                           alpha_cutoff, tf, acc_cutoff=2.0, nthreads=None):
     """blocks_b2f: list of (level, (ix,iy,iz), tex_depth, brick) in back-to-front order."""
     v = np.ascontiguousarray(view16, dtype=np.float32).reshape(16)
     tf = np.ascontiguousarray(tf, dtype=np.float32)
@@ -282,6 +282,6 @@ def render_float_composite(view16, vp_w, vp_h, fov, blocks_b2f, fmin, fmax, thr_
         arr[i].brick = _p(b, C.c_float)
     out = np.zeros((vp_h, vp_w, 4), dtype=np.float32)
     lib().orc_render_float_composite(_p(v, C.c_float), vp_w, vp_h, fov, arr, len(blocks_b2f), fmin,
-                                     fmax, thr_rel, quality, alpha_cutoff, _p(tf, C.c_float),
+                                     fmax, thr_rel, quality, alpha_cutoff, acc_cutoff, _p(tf, C.c_float),
                                      tf.shape[0], _p(out, C.c_float), nthreads or os.cpu_count())
     return out

@@ -32,6 +32,27 @@ class BlockParams(C.Structure):
     ]
 
 
+class FrameBlock(C.Structure):
+    _fields_ = [("level", C.c_uint32), ("index", C.c_uint32 * 3), ("depth", C.c_uint32),
+                ("sparse", C.c_void_p), ("plan", C.c_void_p), ("brick", C.c_void_p),
+                ("brick16", C.c_void_p)]
+
+
+class FrameDesc(C.Structure):
+    _fields_ = [("vp_w", C.c_uint32), ("vp_h", C.c_uint32), ("fov_y_deg", C.c_float),
+                ("fmin", C.c_float), ("fmax", C.c_float), ("threshold_rel", C.c_float),
+                ("quality", C.c_float), ("alpha_cutoff", C.c_float), ("acc_cutoff", C.c_float),
+                ("d_tf", C.c_void_p), ("tf_n", C.c_uint32), ("tile_w", C.c_uint32),
+                ("tile_h", C.c_uint32)]
+
+
+class RenderState(C.Structure):
+    _fields_ = [("view16", C.c_void_p), ("vp_w", C.c_uint32), ("vp_h", C.c_uint32),
+                ("fov_y_deg", C.c_float), ("quality", C.c_float), ("threshold_rel", C.c_float),
+                ("alpha_cutoff", C.c_float), ("fmin", C.c_float), ("fmax", C.c_float),
+                ("background", C.c_float * 3), ("tf", C.c_void_p), ("tf_n", C.c_uint32)]
+
+
 class BrickJob(C.Structure):
     _fields_ = [("sparse", C.c_void_p), ("plan", C.c_void_p), ("dense", C.c_void_p),
                 ("dense16", C.c_void_p), ("depth", C.c_uint32), ("reserved", C.c_uint32)]
@@ -77,6 +98,23 @@ _SIGS = {
                                       C.c_int32, C.c_int32, C.c_int32, C.c_int, _vp]),
     "rv_fill_u32": (C.c_int, [_vp, C.c_uint64, C.c_uint32, _vp]),
     "rv_background_argb": (C.c_uint32, [C.c_float, C.c_float, C.c_float]),
+    "rv_frame_plan_create": (C.c_int, [C.POINTER(FrameBlock), C.c_uint32, C.POINTER(FrameDesc),
+                                       C.POINTER(C.c_void_p)]),
+    "rv_frame_plan_set_cameras": (C.c_int, [_vp, _f32p, C.c_uint32]),
+    "rv_frame_render": (C.c_int, [_vp, C.c_uint32, _vp, _vp, C.c_int, _vp]),
+    "rv_frame_graph_build": (C.c_int, [_vp, _vp, C.c_uint64, C.c_int, _vp]),
+    "rv_frame_graph_launch": (C.c_int, [_vp, _vp]),
+    "rv_frame_status": (C.c_int, [_vp, _u32p]),
+    "rv_frame_get_tile": (C.c_int, [_vp, _u32p, _u32p]),
+    "rv_frame_autotune": (C.c_int, [_vp, _vp, C.c_int, C.c_char_p, _vp, _u32p, _u32p]),
+    "rv_frame_plan_destroy": (None, [_vp]),
+    "rv_worker_create": (C.c_int, [C.c_uint32, C.POINTER(C.c_void_p)]),
+    "rv_worker_add_subtree": (C.c_int, [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p, C.c_uint64, _f32p]),
+    "rv_worker_set_state": (C.c_int, [_vp, C.POINTER(RenderState)]),
+    "rv_worker_render_scene_part": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_int),
+                                              C.POINTER(C.c_int32), _u32p, C.c_uint64]),
+    "rv_worker_compose_frame": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _u32p, C.POINTER(C.c_int)]),
+    "rv_worker_destroy": (None, [_vp]),
 }
 
 _lib = None

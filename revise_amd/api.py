@@ -4,7 +4,8 @@ import ctypes as C
 
 import numpy as np
 
-from . import BlockParams, BrickJob, RvError, check, lib, ptr, stream_ptr
+from . import (BlockParams, BrickJob, FrameBlock, FrameDesc, RenderState, RvError, check, lib, ptr,
+               stream_ptr)
 
 
 def _np(a, dt):
@@ -193,6 +194,138 @@ def blend_image_part(d_dst, dw, dh, dst_off, d_src, sw, sh, src_xy, just_copy=Fa
 
 def fill_u32(d, n, value, stream=None):
     check(lib().rv_fill_u32(ptr(d), n, value, stream_ptr(stream)))
+
+
+class FramePlan:
+    """Batched persistent frame (rv_frame_plan).  blocks: list of dicts with
+    level, index, depth, brick (torch fp32/fp16 tensor), optional sparse and
+    plan tensors."""
+
+    def __init__(self, blocks, vp_w, vp_h, fov, fmin, fmax, thr_rel, quality, alpha_cutoff,
+                 acc_cutoff, d_tf, tile=(0, 0)):
+        import torch
+        arr = (FrameBlock * len(blocks))()
+        self._keep = [d_tf]
+        for i, b in enumerate(blocks):
+            arr[i].level = b["level"]
+            for k in range(3):
+                arr[i].index[k] = b["index"][k]
+            arr[i].depth = b["depth"]
+            if b.get("sparse") is not None:
+                arr[i].sparse = b["sparse"].data_ptr()
+                arr[i].plan = b["plan"].data_ptr()
+            if b["brick"].dtype == torch.float16:
+                arr[i].brick16 = b["brick"].data_ptr()
+            else:
+                arr[i].brick = b["brick"].data_ptr()
+            self._keep.append(b)
+        d = FrameDesc(vp_w, vp_h, fov, fmin, fmax, thr_rel, quality, alpha_cutoff, acc_cutoff,
+                      d_tf.data_ptr(), d_tf.shape[0], tile[0], tile[1])
+        h = C.c_void_p()
+        check(lib().rv_frame_plan_create(arr, len(blocks), C.byref(d), C.byref(h)))
+        self.h = h
+        self.W, self.H = vp_w, vp_h
+
+    def set_cameras(self, views):
+        v = _np(views, np.float32).reshape(-1, 16)
+        check(lib().rv_frame_plan_set_cameras(self.h, _p(v, C.c_float), v.shape[0]))
+
+    def render(self, cam, d_argb, d_rgba=None, interpolate=True, stream=None):
+        check(lib().rv_frame_render(self.h, cam, ptr(d_argb), ptr(d_rgba) if d_rgba is not None else None,
+                                    int(interpolate), stream_ptr(stream)))
+
+    def graph_build(self, d_argb, frame_stride, interpolate=True, stream=None):
+        check(lib().rv_frame_graph_build(self.h, ptr(d_argb), frame_stride, int(interpolate),
+                                         stream_ptr(stream)))
+
+    def graph_launch(self, stream=None):
+        check(lib().rv_frame_graph_launch(self.h, stream_ptr(stream)))
+
+    def status(self):
+        v = C.c_uint32()
+        check(lib().rv_frame_status(self.h, C.byref(v)))
+        return v.value
+
+    def tile(self):
+        w, h = C.c_uint32(), C.c_uint32()
+        check(lib().rv_frame_get_tile(self.h, C.byref(w), C.byref(h)))
+        return w.value, h.value
+
+    def autotune(self, d_argb, interpolate=True, cache_path=None, stream=None):
+        w, h = C.c_uint32(), C.c_uint32()
+        check(lib().rv_frame_autotune(self.h, ptr(d_argb), int(interpolate),
+                                      cache_path.encode() if cache_path else None, stream_ptr(stream),
+                                      C.byref(w), C.byref(h)))
+        return w.value, h.value
+
+    def close(self):
+        if self.h:
+            lib().rv_frame_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Worker:
+    """rv_worker: VsWorkerInterface drop-in (field service + render state +
+    renderScenePart + on-device share compositing)."""
+
+    def __init__(self, thread_id=0):
+        h = C.c_void_p()
+        check(lib().rv_worker_create(thread_id, C.byref(h)))
+        self.h = h
+        self.W = self.H = 0
+
+    def add_subtree(self, level, index, depth, n2i, sparse):
+        idx = _np(index, np.uint32)
+        n2i = _np(n2i, np.uint16)
+        sp = _np(sparse, np.float32)
+        check(lib().rv_worker_add_subtree(self.h, level, _p(idx, C.c_uint32), depth, _p(n2i, C.c_uint16),
+                                          n2i.shape[0], _p(sp, C.c_float)))
+
+    def set_state(self, view16, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin, fmax, background,
+                  tf):
+        self._view = _np(view16, np.float32).reshape(16)
+        self._tf = _np(tf, np.float32)
+        st = RenderState(self._view.ctypes.data, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin,
+                         fmax, (C.c_float * 3)(*background), self._tf.ctypes.data, self._tf.shape[0])
+        check(lib().rv_worker_set_state(self.h, C.byref(st)))
+        self.W, self.H = vp_w, vp_h
+
+    def render_scene_part(self, level, box_min, box_max, cancel=False):
+        bmin, bmax = _np(box_min, np.uint32), _np(box_max, np.uint32)
+        cap = self.W * self.H
+        bits = np.zeros(max(cap, 1), dtype=np.uint32)
+        rect = (C.c_int32 * 4)()
+        flag = C.c_int(1 if cancel else 0)
+        rc = lib().rv_worker_render_scene_part(self.h, level, _p(bmin, C.c_uint32), _p(bmax, C.c_uint32),
+                                               C.byref(flag), rect, _p(bits, C.c_uint32), cap)
+        if rc == 4:
+            return None
+        check(rc)
+        x, y, w, h = tuple(rect)
+        return (x, y, w, h), bits[: w * h].reshape(h, w).copy()
+
+    def compose_frame(self, level, split_count):
+        out = np.zeros((self.H, self.W), dtype=np.uint32)
+        flag = C.c_int(0)
+        check(lib().rv_worker_compose_frame(self.h, level, split_count, _p(out, C.c_uint32), C.byref(flag)))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().rv_worker_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 __all__ = [n for n in dir() if not n.startswith("_")] + ["RvError"]

@@ -1,0 +1,586 @@
+// Batched persistent frame: interpolation of every brick and ray-casting of
+// the whole frame in one launch per frame, graph-capturable.
+//
+// Work items, handed out in order by one atomic queue counter:
+//   [0, n_interp)           brick tiles (the interpolation tile of interp.hip);
+//   [n_interp, n_interp+nt) screen tiles of tw x th pixels (tw*th = 512).
+// A screen tile composites, for each of its pixels, the per-block ray
+// segments of every block whose view rectangle covers it, front to back, in
+// registers (blendBehind of the reference, cuda_blend.hpp:24-33), and stops as
+// soon as every pixel of the tile is opaque.  Each block segment is sampled
+// and classified exactly like the per-block renderer (renderArgb.hpp:30-55),
+// so the result is the reference's compositing of the same segments without
+// the per-block 8-bit viewport round trip.
+//
+// A screen tile that reaches a block whose brick is still being interpolated
+// waits on that block's tile counter (release/acquire at agent scope).  All
+// brick tiles are dequeued before any screen tile, and a workgroup holding a
+// brick tile never waits, so every wait ends; a wait is additionally bounded
+// (it sets an error flag and gives up after ~2^22 sleeps).
+//
+// The blocks of a screen tile are found by testing the block view
+// rectangles (camera dependent, one uint4 per block) in front-to-back order,
+// 512 at a time, and compacting the hits into LDS with wave ballots.
+
+#include "interp_device.hpp"
+#include "order.hpp"
+#include "render_common.hpp"
+#include "rv_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace rv {
+
+int fill_u32(uint32_t* d, uint64_t n, uint32_t value, hipStream_t s);
+
+namespace {
+
+struct FrameArgs {
+    const Job* jobs;               // one interpolation job per block (block order)
+    uint32_t njobs;
+    uint32_t n_interp;             // brick tiles of this frame (0: bricks are resident)
+    const rv_block_params* params; // per block
+    const uint4* rects;            // per block view rectangle {x0, y0, x1, y1}, x1 = x0 if dead
+    const uint32_t* f2b;           // block indices, front to back
+    const void* const* bricks;     // per block, float or half
+    const uint32_t* need;          // per block: tiles of its brick
+    uint32_t nblocks;
+    const float4* tf;
+    uint32_t tf_n;
+    uint32_t W, H, tw, th, ntx, nty;
+    float acc_cutoff;
+    uint32_t* argb;                // W*H, rows bottom-up like the reference viewport
+    float4* rgba;                  // optional float output, same layout
+    uint32_t* counters;            // [0] queue, [1] error, [2 + b] tiles done of block b
+};
+
+constexpr uint32_t kT = 512;
+constexpr uint32_t kMaxTf = 1024;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+template <class BrickT>
+__device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint32_t* list,
+                            uint32_t* wave_cnt)
+{
+    const uint32_t tid = threadIdx.x;
+    const uint32_t tx = tile % a.ntx, ty = tile / a.ntx;
+    const uint32_t x0 = tx * a.tw, y0 = ty * a.th;
+    const uint32_t x1 = min(x0 + a.tw, a.W), y1 = min(y0 + a.th, a.H);
+    const uint32_t vx = x0 + tid % a.tw, vy = y0 + tid / a.tw;
+    const bool inside = vx < a.W && vy < a.H;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool opaque = !inside;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    bool finished = false;
+    for (uint32_t c = 0; c < a.nblocks && !finished; c += kT) {
+        // compact the blocks of this chunk whose rectangles meet the tile
+        const uint32_t i = c + tid;
+        uint32_t b = 0;
+        bool hit = false;
+        if (i < a.nblocks) {
+            b = a.f2b[i];
+            const uint4 r = a.rects[b];
+            hit = r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0;
+        }
+        const uint64_t ball = __ballot(hit);
+        if (lane == 0)
+            wave_cnt[wave] = (uint32_t)__popcll(ball);
+        __syncthreads();
+        uint32_t off = 0, total = 0;
+        for (uint32_t w = 0; w < kT / 64u; ++w) {
+            const uint32_t n = wave_cnt[w];
+            off += w < wave ? n : 0u;
+            total += n;
+        }
+        if (hit)
+            list[off + (uint32_t)__popcll(ball & ((1ull << lane) - 1ull))] = b;
+        __syncthreads();
+        for (uint32_t k = 0; k < total; ++k) {
+            const uint32_t blk = uniform(list[k]);
+            if (a.n_interp) {
+                if (tid == 0) {
+                    const uint32_t need = a.need[blk];
+                    uint32_t spins = 0;
+                    while (__hip_atomic_load(a.counters + 2 + blk, __ATOMIC_ACQUIRE,
+                                             __HIP_MEMORY_SCOPE_AGENT) < need) {
+                        __builtin_amdgcn_s_sleep(8);
+                        if (++spins > kSpinLimit) {
+                            atomicOr(a.counters + 1, 1u);
+                            break;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            const rv_block_params p = a.params[blk];
+            if (!opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
+                vy < p.view_y + p.view_h) {
+                const RaySeg r = setup_ray(p, vx, vy);
+                if (r.hit) {
+                    const float4 seg =
+                        march(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), tfs, (int)a.tf_n, r);
+                    blend_behind(acc, seg);
+                    opaque = acc.w > a.acc_cutoff;
+                }
+            }
+            if (__syncthreads_and(opaque)) {
+                finished = true;
+                break;
+            }
+        }
+        __syncthreads(); // list is reused by the next chunk
+    }
+    if (inside) {
+        const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
+        a.argb[pix] = ((unsigned)(acc.w * 255.999f) << 24) + ((unsigned)(acc.x * 255.999f) << 16) +
+                      ((unsigned)(acc.y * 255.999f) << 8) + (unsigned)(acc.z * 255.999f);
+        if (a.rgba)
+            a.rgba[pix] = acc;
+    }
+}
+
+template <class BrickT>
+__global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
+{
+    __shared__ __align__(16) float A[kTileFloats]; // interpolation tile; colour table + list for screen tiles
+    __shared__ float lat[32];
+    __shared__ uint32_t wave_cnt[kT / 64u];
+    __shared__ uint32_t item_s;
+    float4* tfs = reinterpret_cast<float4*>(A);
+    uint32_t* list = reinterpret_cast<uint32_t*>(A) + 4u * kMaxTf;
+    const uint32_t n_total = a.n_interp + a.ntx * a.nty;
+    bool tf_loaded = false;
+    for (;;) {
+        if (threadIdx.x == 0)
+            item_s = atomicAdd(a.counters, 1u);
+        __syncthreads();
+        const uint32_t item = item_s;
+        __syncthreads();
+        if (item >= n_total)
+            break;
+        if (item < a.n_interp) {
+            uint32_t lo = 0, hi = a.njobs;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.jobs[mid].tile_start <= item)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const Job job = a.jobs[lo];
+            interp_tile(job, item - job.tile_start, A, lat);
+            __threadfence();
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_fetch_add(a.counters + 2 + lo, 1u, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            continue;
+        }
+        if (!tf_loaded) {
+            for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT)
+                tfs[i] = a.tf[i];
+            __syncthreads();
+            tf_loaded = true;
+        }
+        render_tile<BrickT>(a, item - a.n_interp, tfs, list, wave_cnt);
+    }
+}
+
+__global__ void frame_reset_kernel(uint32_t* counters, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (i != 1u)
+            counters[i] = 0u;
+}
+
+} // namespace
+} // namespace rv
+
+using namespace rv;
+
+struct rv_frame_plan {
+    uint32_t nblocks = 0;
+    std::vector<rv_frame_block> blocks;
+    rv_frame_desc desc{};
+    bool fp16 = false;
+    Job* d_jobs = nullptr;
+    uint32_t n_interp = 0;
+    uint32_t* d_need = nullptr;
+    const void** d_bricks = nullptr;
+    uint32_t* d_counters = nullptr;
+    uint32_t ncams = 0;
+    std::vector<float> views;
+    rv_block_params* d_params = nullptr;
+    uint4* d_rects = nullptr;
+    uint32_t* d_f2b = nullptr;
+    uint32_t grid = 0;
+    uint32_t tw = 32, th = 16;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipStream_t capture_stream = nullptr;
+    bool can_interp = true;
+
+    ~rv_frame_plan()
+    {
+        if (graph_exec)
+            (void)hipGraphExecDestroy(graph_exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        if (capture_stream)
+            (void)hipStreamDestroy(capture_stream);
+        for (void* p : {(void*)d_jobs, (void*)d_need, (void*)d_bricks, (void*)d_counters,
+                        (void*)d_params, (void*)d_rects, (void*)d_f2b})
+            if (p)
+                (void)hipFree(p);
+    }
+};
+
+namespace {
+
+int upload(void** dst, const void* src, size_t bytes)
+{
+    if (*dst)
+        (void)hipFree(*dst);
+    *dst = nullptr;
+    if (!bytes)
+        return RV_OK;
+    RV_HIP(hipMalloc(dst, bytes));
+    RV_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return RV_OK;
+}
+
+int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba, int interpolate,
+                 hipStream_t s, uint32_t tw, uint32_t th)
+{
+    FrameArgs a{};
+    a.jobs = P->d_jobs;
+    a.njobs = P->nblocks;
+    a.n_interp = interpolate ? P->n_interp : 0u;
+    a.params = P->d_params + (size_t)cam * P->nblocks;
+    a.rects = P->d_rects + (size_t)cam * P->nblocks;
+    a.f2b = P->d_f2b + (size_t)cam * P->nblocks;
+    a.bricks = P->d_bricks;
+    a.need = P->d_need;
+    a.nblocks = P->nblocks;
+    a.tf = reinterpret_cast<const float4*>(P->desc.d_tf);
+    a.tf_n = P->desc.tf_n;
+    a.W = P->desc.vp_w;
+    a.H = P->desc.vp_h;
+    a.tw = tw;
+    a.th = th;
+    a.ntx = (a.W + tw - 1) / tw;
+    a.nty = (a.H + th - 1) / th;
+    a.acc_cutoff = P->desc.acc_cutoff;
+    a.argb = d_argb;
+    a.rgba = reinterpret_cast<float4*>(d_rgba);
+    a.counters = P->d_counters;
+    hipLaunchKernelGGL(frame_reset_kernel, dim3(std::max(1u, (P->nblocks + 2 + 255) / 256)),
+                       dim3(256), 0, s, P->d_counters, P->nblocks + 2);
+    RV_LAUNCH_CHECK();
+    if (P->fp16)
+        hipLaunchKernelGGL(frame_kernel<__half>, dim3(P->grid), dim3(kT), 0, s, a);
+    else
+        hipLaunchKernelGGL(frame_kernel<float>, dim3(P->grid), dim3(kT), 0, s, a);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
+
+} // namespace
+
+extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nblocks,
+                                    const rv_frame_desc* desc, rv_frame_plan** out)
+{
+    RV_TRY
+    RV_REQUIRE(out && desc, "rv_frame_plan_create: null pointer");
+    *out = nullptr;
+    RV_REQUIRE(nblocks > 0 && blocks, "rv_frame_plan_create: no blocks");
+    RV_REQUIRE(desc->vp_w > 0 && desc->vp_h > 0 && desc->vp_w <= 16384 && desc->vp_h <= 16384,
+               "rv_frame_plan_create: bad viewport");
+    RV_REQUIRE(desc->d_tf && desc->tf_n >= 1 && desc->tf_n <= kMaxTf,
+               "rv_frame_plan_create: colour table must have 1..1024 entries");
+    RV_REQUIRE(desc->fmax > desc->fmin && desc->quality > 0.f, "rv_frame_plan_create: bad range");
+    auto* P = new rv_frame_plan;
+    std::unique_ptr<rv_frame_plan> guard(P);
+    P->nblocks = nblocks;
+    P->blocks.assign(blocks, blocks + nblocks);
+    P->desc = *desc;
+    const bool fp16 = blocks[0].brick16 != nullptr;
+    std::vector<Job> jobs(nblocks);
+    std::vector<uint32_t> need(nblocks);
+    std::vector<const void*> bricks(nblocks);
+    uint64_t tiles = 0;
+    for (uint32_t i = 0; i < nblocks; ++i) {
+        const rv_frame_block& b = blocks[i];
+        RV_REQUIRE(b.depth <= kMaxDepth, "rv_frame_plan_create: depth must be <= 8");
+        RV_REQUIRE(b.level <= 16, "rv_frame_plan_create: level too large");
+        for (int k = 0; k < 3; ++k)
+            RV_REQUIRE(b.index[k] < (1u << b.level), "rv_frame_plan_create: index outside level");
+        RV_REQUIRE((b.brick16 != nullptr) == fp16 && (b.brick != nullptr) != fp16,
+                   "rv_frame_plan_create: all blocks must use the same brick type");
+        const Geom g = geom_of(b.depth);
+        jobs[i] = Job{b.sparse, b.plan, b.brick, reinterpret_cast<__half*>(b.brick16), b.depth,
+                      (uint32_t)tiles};
+        need[i] = g.ntiles;
+        bricks[i] = fp16 ? (const void*)b.brick16 : (const void*)b.brick;
+        if (!b.sparse || !b.plan)
+            P->can_interp = false;
+        tiles += g.ntiles;
+    }
+    RV_REQUIRE(tiles < 0x40000000ull, "rv_frame_plan_create: too many brick tiles");
+    P->fp16 = fp16;
+    P->n_interp = (uint32_t)tiles;
+    int rc;
+    if ((rc = upload((void**)&P->d_jobs, jobs.data(), sizeof(Job) * nblocks)))
+        return rc;
+    if ((rc = upload((void**)&P->d_need, need.data(), 4ull * nblocks)))
+        return rc;
+    if ((rc = upload((void**)&P->d_bricks, bricks.data(), sizeof(void*) * nblocks)))
+        return rc;
+    RV_HIP(hipMalloc((void**)&P->d_counters, 4ull * (nblocks + 2)));
+    RV_HIP(hipMemset(P->d_counters, 0, 4ull * (nblocks + 2)));
+    // persistent grid: every workgroup resident at once
+    int dev = 0, cus = 0, per_cu = 0;
+    RV_HIP(hipGetDevice(&dev));
+    RV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (fp16)
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<__half>, kT, 0));
+    else
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<float>, kT, 0));
+    RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
+    P->grid = (uint32_t)(cus * per_cu);
+    if (desc->tile_w && desc->tile_h) {
+        RV_REQUIRE(desc->tile_w * desc->tile_h == kT, "rv_frame_plan_create: tile_w*tile_h must be 512");
+        P->tw = desc->tile_w;
+        P->th = desc->tile_h;
+    }
+    *out = guard.release();
+    return RV_OK;
+    RV_CATCH
+}
+
+extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16, uint32_t ncams)
+{
+    RV_TRY
+    RV_REQUIRE(P && views16 && ncams > 0, "rv_frame_plan_set_cameras: bad arguments");
+    if (P->graph_exec) {
+        (void)hipGraphExecDestroy(P->graph_exec);
+        P->graph_exec = nullptr;
+    }
+    if (P->graph) {
+        (void)hipGraphDestroy(P->graph);
+        P->graph = nullptr;
+    }
+    const uint32_t nb = P->nblocks;
+    std::vector<rv_block_params> params((size_t)ncams * nb);
+    std::vector<uint4> rects((size_t)ncams * nb);
+    std::vector<uint32_t> f2b((size_t)ncams * nb);
+    std::vector<uint32_t> keys(4ull * nb);
+    for (uint32_t i = 0; i < nb; ++i) {
+        keys[4 * i] = P->blocks[i].level;
+        for (int k = 0; k < 3; ++k)
+            keys[4 * i + 1 + k] = P->blocks[i].index[k];
+    }
+    const rv_frame_desc& d = P->desc;
+    std::vector<uint32_t> order;
+    for (uint32_t c = 0; c < ncams; ++c) {
+        const float* M = views16 + 16ull * c;
+        for (uint32_t i = 0; i < nb; ++i) {
+            rv_block_params& p = params[(size_t)c * nb + i];
+            const int live = block_params(M, d.vp_w, d.vp_h, d.fov_y_deg, P->blocks[i].level,
+                                          P->blocks[i].index, P->blocks[i].depth, d.fmin, d.fmax,
+                                          d.threshold_rel, d.quality, d.alpha_cutoff, p);
+            rects[(size_t)c * nb + i] =
+                live ? make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h)
+                     : make_uint4(0, 0, 0, 0);
+        }
+        float eye[3];
+        eye_from_view(M, eye);
+        int rc = mixed_back_to_front(keys.data(), nb, eye, order);
+        if (rc)
+            return rc;
+        for (uint32_t i = 0; i < nb; ++i)
+            f2b[(size_t)c * nb + i] = order[nb - 1 - i];
+    }
+    int rc;
+    if ((rc = upload((void**)&P->d_params, params.data(), sizeof(rv_block_params) * params.size())))
+        return rc;
+    if ((rc = upload((void**)&P->d_rects, rects.data(), sizeof(uint4) * rects.size())))
+        return rc;
+    if ((rc = upload((void**)&P->d_f2b, f2b.data(), 4ull * f2b.size())))
+        return rc;
+    P->ncams = ncams;
+    P->views.assign(views16, views16 + 16ull * ncams);
+    return RV_OK;
+    RV_CATCH
+}
+
+extern "C" int rv_frame_render(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba,
+                               int interpolate, rv_stream_t stream)
+{
+    RV_REQUIRE(P && d_argb, "rv_frame_render: null pointer");
+    RV_REQUIRE(cam < P->ncams, "rv_frame_render: camera index out of range (set cameras first)");
+    RV_REQUIRE(!interpolate || P->can_interp, "rv_frame_render: blocks have no sparse values/plans");
+    return launch_frame(P, cam, d_argb, d_rgba, interpolate, S(stream), P->tw, P->th);
+}
+
+extern "C" int rv_frame_graph_build(rv_frame_plan* P, uint32_t* d_argb, uint64_t frame_stride,
+                                    int interpolate, rv_stream_t stream)
+{
+    RV_REQUIRE(P && d_argb && P->ncams > 0, "rv_frame_graph_build: bad arguments");
+    RV_REQUIRE(!interpolate || P->can_interp, "rv_frame_graph_build: blocks have no sparse values/plans");
+    if (P->graph_exec) {
+        (void)hipGraphExecDestroy(P->graph_exec);
+        P->graph_exec = nullptr;
+    }
+    if (P->graph) {
+        (void)hipGraphDestroy(P->graph);
+        P->graph = nullptr;
+    }
+    // Capture on a stream of the plan's own (the legacy default stream cannot
+    // be captured); the graph is replayed on the caller's stream.
+    (void)stream;
+    if (!P->capture_stream)
+        RV_HIP(hipStreamCreateWithFlags(&P->capture_stream, hipStreamNonBlocking));
+    hipStream_t s = P->capture_stream;
+    RV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = RV_OK;
+    for (uint32_t c = 0; c < P->ncams && !rc; ++c)
+        rc = launch_frame(P, c, d_argb + frame_stride * c, nullptr, interpolate, s, P->tw, P->th);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g)
+            (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+    P->graph = g;
+    RV_HIP(hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0));
+    return RV_OK;
+}
+
+extern "C" int rv_frame_graph_launch(rv_frame_plan* P, rv_stream_t stream)
+{
+    RV_REQUIRE(P && P->graph_exec, "rv_frame_graph_launch: no graph built");
+    RV_HIP(hipGraphLaunch(P->graph_exec, S(stream)));
+    return RV_OK;
+}
+
+extern "C" int rv_frame_status(rv_frame_plan* P, uint32_t* error_flags)
+{
+    RV_REQUIRE(P && error_flags, "rv_frame_status: null pointer");
+    uint32_t v = 0;
+    RV_HIP(hipMemcpy(&v, P->d_counters + 1, 4, hipMemcpyDeviceToHost));
+    *error_flags = v;
+    return RV_OK;
+}
+
+extern "C" int rv_frame_get_tile(const rv_frame_plan* P, uint32_t* tile_w, uint32_t* tile_h)
+{
+    RV_REQUIRE(P && tile_w && tile_h, "rv_frame_get_tile: null pointer");
+    *tile_w = P->tw;
+    *tile_h = P->th;
+    return RV_OK;
+}
+
+// Autotuner: time every candidate screen-tile shape on the plan's camera 0
+// (hip events, median of 3 after a warm-up) and keep the fastest.  Results
+// are cached in a small JSON file keyed by device name and workload
+// signature, so later runs reuse the choice without timing.
+extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpolate,
+                                 const char* cache_path, rv_stream_t stream, uint32_t* tile_w,
+                                 uint32_t* tile_h)
+{
+    RV_TRY
+    RV_REQUIRE(P && d_argb && P->ncams > 0, "rv_frame_autotune: bad arguments");
+    RV_REQUIRE(!interpolate || P->can_interp, "rv_frame_autotune: blocks have no sparse values/plans");
+    static const uint32_t cand[][2] = {{32, 16}, {16, 32}, {64, 8}, {128, 4}, {8, 64}};
+    int dev = 0;
+    RV_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    RV_HIP(hipGetDeviceProperties(&prop, dev));
+    std::ostringstream sig;
+    sig << prop.gcnArchName << "|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
+        << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth;
+    const std::string key = sig.str();
+    if (cache_path && *cache_path) {
+        std::ifstream f(cache_path);
+        std::string line;
+        while (std::getline(f, line)) {
+            // one entry per line: {"key": "...", "tile": [w, h], ...}
+            const std::string k = "\"key\": \"" + key + "\"";
+            if (line.find(k) == std::string::npos)
+                continue;
+            unsigned w = 0, h = 0;
+            const size_t t = line.find("\"tile\": [");
+            if (t != std::string::npos && std::sscanf(line.c_str() + t + 9, "%u, %u", &w, &h) == 2 &&
+                w * h == kT) {
+                P->tw = w;
+                P->th = h;
+                if (tile_w)
+                    *tile_w = w;
+                if (tile_h)
+                    *tile_h = h;
+                return RV_OK;
+            }
+        }
+    }
+    hipStream_t s = S(stream);
+    hipEvent_t e0, e1;
+    RV_HIP(hipEventCreate(&e0));
+    RV_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    uint32_t bw = P->tw, bh = P->th;
+    std::ostringstream times;
+    for (const auto& c : cand) {
+        float t[3];
+        int rc = launch_frame(P, 0, d_argb, nullptr, interpolate, s, c[0], c[1]);
+        for (int r = 0; r < 3 && !rc; ++r) {
+            (void)hipEventRecord(e0, s);
+            rc = launch_frame(P, 0, d_argb, nullptr, interpolate, s, c[0], c[1]);
+            (void)hipEventRecord(e1, s);
+            (void)hipEventSynchronize(e1);
+            (void)hipEventElapsedTime(&t[r], e0, e1);
+        }
+        if (rc) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return rc;
+        }
+        std::sort(t, t + 3);
+        times << (times.tellp() > 0 ? ", " : "") << "\"" << c[0] << "x" << c[1] << "\": " << t[1];
+        if (t[1] < best) {
+            best = t[1];
+            bw = c[0];
+            bh = c[1];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P->tw = bw;
+    P->th = bh;
+    if (tile_w)
+        *tile_w = bw;
+    if (tile_h)
+        *tile_h = bh;
+    if (cache_path && *cache_path) {
+        std::ofstream f(cache_path, std::ios::app);
+        f << "{\"key\": \"" << key << "\", \"tile\": [" << bw << ", " << bh << "], \"ms\": {"
+          << times.str() << "}}\n";
+    }
+    return RV_OK;
+    RV_CATCH
+}
+
+extern "C" void rv_frame_plan_destroy(rv_frame_plan* P) { delete P; }

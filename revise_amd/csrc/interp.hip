@@ -27,6 +27,7 @@
 // Memory layout: the dense brick is x fastest; a tile writes (T or T+1) full
 // rows of the brick per z-plane, i.e. contiguous runs of 16 x 260 B for d = 6.
 
+#include "interp_device.hpp"
 #include "rv_internal.hpp"
 
 #include <hip/hip_fp16.h>
@@ -37,204 +38,6 @@
 namespace rv {
 
 namespace {
-
-constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
-constexpr float kNoField = 1e30f;
-constexpr uint32_t kThreads = 512;
-constexpr uint32_t kMaxDepth = 8;
-// Largest tile: 65 x 17 x 17 floats.  The coarse lattice (<= 17^3) reuses it.
-constexpr uint32_t kTileFloats = 65u * 17u * 17u;
-
-struct Geom {
-    uint32_t d, n, t, T, X, tiles_x, tiles_yz, ntiles, m;
-};
-
-__host__ __device__ inline Geom geom_of(uint32_t d)
-{
-    Geom g;
-    g.d = d;
-    g.n = (1u << d) + 1u;
-    g.t = d < 4u ? d : 4u;
-    g.T = 1u << g.t;
-    g.X = d < 6u ? (1u << d) : 64u;
-    g.tiles_x = (1u << d) / g.X;
-    g.tiles_yz = (1u << d) / g.T;
-    g.ntiles = g.tiles_x * g.tiles_yz * g.tiles_yz;
-    g.m = (1u << d) >> g.t; // lattice cells per axis
-    return g;
-}
-
-__device__ __forceinline__ uint32_t ordinal3(uint32_t x, uint32_t y, uint32_t z, uint32_t d)
-{
-    uint32_t r = (z << d) + z + y;
-    r = (r << d) + r + x;
-    return r;
-}
-
-__device__ __forceinline__ bool is_nan(float v) { return v != v; }
-
-struct Job {
-    const float* sparse;
-    const uint32_t* plan;
-    float* dense;
-    __half* dense16;
-    uint32_t depth;
-    uint32_t tile_start;
-};
-
-// Complete the levels with cube sizes s_start, s_start/2, ..., 2 (in box units)
-// on a box of (X+1) x (T+1) x (T+1) values at A (x fastest).  Three phases per
-// level: nodes with one, two, three coordinates an odd multiple of h = s/2.
-__device__ void refine_box(float* A, uint32_t X, uint32_t T, uint32_t s_start)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t nwaves = kThreads / 64u;
-    const uint32_t rowlen = X + 1u;
-    const uint32_t plane = rowlen * (T + 1u);
-    for (uint32_t s = s_start; s >= 2u; s >>= 1) {
-        const uint32_t h = s >> 1;
-        const uint32_t ny = T / h + 1u;   // sub-lattice rows per axis (y and z)
-        const uint32_t nx = X / h + 1u;   // sub-lattice nodes per row
-        const uint32_t nrows = ny * ny;
-        const uint32_t sx = h, sy = h * rowlen, sz = h * plane;
-        for (uint32_t phase = 1; phase <= 3; ++phase) {
-            for (uint32_t q = wave; q < nrows; q += nwaves) {
-                const uint32_t j = q % ny, k = q / ny;
-                const uint32_t oy = j & 1u, oz = k & 1u;
-                // rows whose y/z parities cannot produce this phase
-                if (oy + oz > phase || oy + oz + 1u < phase)
-                    continue;
-                float* row = A + (k * h) * plane + (j * h) * rowlen;
-                for (uint32_t i = lane; i < nx; i += 64u) {
-                    const uint32_t ox = i & 1u;
-                    if (ox + oy + oz != phase)
-                        continue;
-                    float* p = row + i * h;
-                    if (!is_nan(*p))
-                        continue;
-                    float v;
-                    if (phase == 1u) {
-                        const uint32_t st = ox ? sx : (oy ? sy : sz);
-                        const float a = p[-(int)st], b = p[st];
-                        v = (a == kNoField || b == kNoField) ? kNoField : 0.5f * (a + b);
-                    } else if (phase == 2u) {
-                        // odd axes d1 < d2
-                        const uint32_t st1 = ox ? sx : sy;
-                        const uint32_t st2 = oz ? sz : sy;
-                        const float s1 = p[-(int)st1], e1 = p[st1];
-                        const float s2 = p[-(int)st2], e2 = p[st2];
-                        v = (s1 == kNoField || e1 == kNoField || s2 == kNoField || e2 == kNoField)
-                                ? kNoField
-                                : 0.25f * (((s1 + e1) + s2) + e2);
-                    } else {
-                        const float s1 = p[-(int)sx], e1 = p[sx];
-                        const float s2 = p[-(int)sy], e2 = p[sy];
-                        const float s3 = p[-(int)sz], e3 = p[sz];
-                        v = (s1 == kNoField || e1 == kNoField || s2 == kNoField ||
-                             e2 == kNoField || s3 == kNoField || e3 == kNoField)
-                                ? kNoField
-                                : (float)(1. / 6) * (((((s1 + e1) + s2) + e2) + s3) + e3);
-                    }
-                    *p = v;
-                }
-            }
-            __syncthreads();
-        }
-    }
-}
-
-__device__ __forceinline__ float gather(const Job& job, uint32_t ord)
-{
-    const uint32_t s = __builtin_nontemporal_load(job.plan + ord);
-    return s == kNoIndex ? __builtin_nanf("") : job.sparse[s];
-}
-
-__device__ void interp_tile(const Job& job, uint32_t local_tile, float* A, float* lat)
-{
-    const Geom g = geom_of(job.depth);
-    const uint32_t d = g.d, X = g.X, T = g.T;
-    const uint32_t tx = local_tile % g.tiles_x;
-    const uint32_t tyz = local_tile / g.tiles_x;
-    const uint32_t ty = tyz % g.tiles_yz, tz = tyz / g.tiles_yz;
-    const uint32_t x0 = tx * X, y0 = ty * T, z0 = tz * T;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t nwaves = kThreads / 64u;
-    const uint32_t nlat_x = X / T + 1u;
-
-    // 1. coarse levels on the stride-T lattice of the whole brick
-    if (g.m >= 2u) {
-        const uint32_t mp = g.m + 1u;
-        const uint32_t total = mp * mp * mp;
-        for (uint32_t q = threadIdx.x; q < total; q += kThreads) {
-            const uint32_t i = q % mp, j = (q / mp) % mp, k = q / (mp * mp);
-            A[q] = gather(job, ordinal3(i * T, j * T, k * T, d));
-        }
-        __syncthreads();
-        refine_box(A, g.m, g.m, g.m);
-        const uint32_t nl = nlat_x * 4u;
-        if (threadIdx.x < nl) {
-            const uint32_t li = threadIdx.x % nlat_x, c = threadIdx.x / nlat_x;
-            const uint32_t I = x0 / T + li, J = y0 / T + (c & 1u), K = z0 / T + (c >> 1);
-            lat[threadIdx.x] = A[(K * mp + J) * mp + I];
-        }
-        __syncthreads();
-    }
-
-    // 2. gather the tile
-    const uint32_t rowlen = X + 1u;
-    const uint32_t tp1 = T + 1u;
-    const uint32_t nrows = tp1 * tp1;
-    for (uint32_t r = wave; r < nrows; r += nwaves) {
-        const uint32_t y = r % tp1, z = r / tp1;
-        const uint32_t base = ordinal3(x0, y0 + y, z0 + z, d);
-        float* dst = A + r * rowlen;
-        for (uint32_t x = lane; x < rowlen; x += 64u)
-            dst[x] = gather(job, base + x);
-    }
-    __syncthreads();
-    if (g.m >= 2u) {
-        const uint32_t nl = nlat_x * 4u;
-        if (threadIdx.x < nl) {
-            const uint32_t li = threadIdx.x % nlat_x, c = threadIdx.x / nlat_x;
-            const uint32_t y = (c & 1u) * T, z = (c >> 1) * T;
-            A[(z * tp1 + y) * rowlen + li * T] = lat[threadIdx.x];
-        }
-        __syncthreads();
-    }
-
-    // 3. finer levels inside the tile
-    refine_box(A, X, T, T);
-
-    // 4. store the owned half-open box (+ far faces of the brick)
-    const uint32_t ox = (tx + 1u == g.tiles_x) ? rowlen : X;
-    const uint32_t oy = (ty + 1u == g.tiles_yz) ? tp1 : T;
-    const uint32_t oz = (tz + 1u == g.tiles_yz) ? tp1 : T;
-    const uint32_t orows = oy * oz;
-    for (uint32_t r = wave; r < orows; r += nwaves) {
-        const uint32_t y = r % oy, z = r / oy;
-        const uint32_t base = ordinal3(x0, y0 + y, z0 + z, d);
-        const float* src = A + (z * tp1 + y) * rowlen;
-        for (uint32_t x = lane; x < ox; x += 64u) {
-            float v = src[x];
-            if (v == kNoField)
-                v = __builtin_nanf("");
-            if (job.dense16)
-                job.dense16[base + x] = __float2half(v);
-            else
-                __builtin_nontemporal_store(v, job.dense + base + x);
-        }
-    }
-}
-
-// Workgroup -> tile mapping that keeps consecutive tiles (the tiles of one
-// brick) on one XCD: workgroups are dispatched round-robin over the 8 XCDs.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t nblocks)
-{
-    const uint32_t per = nblocks >> 3; // nblocks is a multiple of 8
-    return (bid & 7u) * per + (bid >> 3);
-}
 
 __global__ __launch_bounds__(kThreads) void interp_single_kernel(Job job, uint32_t ntiles)
 {

@@ -1,0 +1,574 @@
+// Drop-in C++ worker layer (see include/revise/vs_worker_hip.hpp) and its
+// C ABI wrappers.  Host code only: every device operation goes through the C
+// ABI of this library on the worker's own stream.
+#include "../../include/revise/vs_worker_hip.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+namespace rv {
+void set_error(const std::string& msg);
+}
+
+namespace revise {
+
+void throwIfFailed(int status)
+{
+    if (status != RV_OK)
+        throw HipError(status, rv_last_error());
+}
+
+static void hipCheck(hipError_t e, const char* what)
+{
+    if (e != hipSuccess)
+        throw HipError(e == hipErrorOutOfMemory ? RV_ERR_OOM : RV_ERR_HIP,
+                       std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static hipStream_t hs(rv_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---------------------------------------------------------------- field service
+
+struct FieldServiceHip::Subtree {
+    unsigned depth = 0;
+    uint64_t n = 0;
+    float* sparse = nullptr;
+    uint16_t* n2i = nullptr;
+    uint32_t* plan = nullptr; // built on first use (cached like the reference's n2i)
+    ~Subtree()
+    {
+        if (sparse)
+            (void)hipFree(sparse);
+        if (n2i)
+            (void)hipFree(n2i);
+        if (plan)
+            (void)hipFree(plan);
+    }
+};
+
+FieldServiceHip::FieldServiceHip() = default;
+FieldServiceHip::~FieldServiceHip() = default;
+
+void FieldServiceHip::addSubtree(unsigned level, const std::array<uint32_t, 3>& index,
+                                 unsigned depth, const uint16_t* n2i, uint64_t n, const float* sparse)
+{
+    if (depth > 8)
+        throw HipError(RV_ERR_INVALID, "addSubtree: depth must be <= 8");
+    auto st = std::make_unique<Subtree>();
+    st->depth = depth;
+    st->n = n;
+    if (n) {
+        hipCheck(hipMalloc((void**)&st->sparse, 4 * n), "hipMalloc");
+        hipCheck(hipMemcpy(st->sparse, sparse, 4 * n, hipMemcpyHostToDevice), "hipMemcpy");
+        hipCheck(hipMalloc((void**)&st->n2i, 6 * n), "hipMalloc");
+        hipCheck(hipMemcpy(st->n2i, n2i, 6 * n, hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    m_subtrees[{level, index[0], index[1], index[2]}] = std::move(st);
+}
+
+void FieldServiceHip::setFieldRange(float fmin, float fmax)
+{
+    m_fmin = fmin;
+    m_fmax = fmax;
+}
+
+bool FieldServiceHip::hasSubtree(unsigned level, const std::array<uint32_t, 3>& i) const
+{
+    return m_subtrees.count({level, i[0], i[1], i[2]}) != 0;
+}
+
+unsigned FieldServiceHip::subtreeDepth(unsigned level, const std::array<uint32_t, 3>& i) const
+{
+    auto it = m_subtrees.find({level, i[0], i[1], i[2]});
+    if (it == m_subtrees.end())
+        throw std::range_error("Block does not exist");
+    return it->second->depth;
+}
+
+void FieldServiceHip::interpolate(unsigned level, const std::array<uint32_t, 3>& i, float* d_dense,
+                                  rv_stream_t stream)
+{
+    auto it = m_subtrees.find({level, i[0], i[1], i[2]});
+    if (it == m_subtrees.end())
+        throw std::range_error("Block does not exist");
+    Subtree& st = *it->second;
+    if (!st.plan) {
+        hipCheck(hipMalloc((void**)&st.plan, 4 * rv_brick_nodes(st.depth)), "hipMalloc");
+        throwIfFailed(rv_interp_plan_build(st.n2i, st.n, st.depth, st.plan, nullptr, stream));
+    }
+    throwIfFailed(rv_interp_brick_planned(st.sparse, st.plan, st.depth, d_dense, stream));
+}
+
+// ---------------------------------------------------------------- renderer
+
+DenseFieldRendererHip::DenseFieldRendererHip()
+{
+    hipStream_t s;
+    hipCheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    m_stream = reinterpret_cast<rv_stream_t>(s);
+}
+
+DenseFieldRendererHip::~DenseFieldRendererHip()
+{
+    if (m_tf)
+        (void)hipFree(m_tf);
+    if (m_brick)
+        (void)hipFree(m_brick);
+    if (m_stream)
+        (void)hipStreamDestroy(hs(m_stream));
+}
+
+void DenseFieldRendererHip::setRenderSharedState(const RenderSharedState* s)
+{
+    if (s != m_state) {
+        m_state = s;
+        m_tfDirty = true;
+        m_lastBlock = {~0u, 0, 0, 0};
+    }
+}
+
+void DenseFieldRendererHip::clearViewport(uint32_t* d_viewport) const
+{
+    throwIfFailed(rv_clear_viewport(d_viewport, m_state->viewportSize[0], m_state->viewportSize[1],
+                                    m_stream));
+}
+
+void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
+                                             const std::array<uint32_t, 3>& index,
+                                             const std::atomic<bool>& isCancelled)
+{
+    renderDenseField(vp, level, index, [&] { return isCancelled.load(); });
+}
+
+void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
+                                             const std::array<uint32_t, 3>& index,
+                                             const std::function<bool()>& isCancelled)
+{
+    const RenderSharedState& st = *m_state;
+    FieldServiceHip& fs = *st.fieldSvc;
+    const unsigned depth = fs.subtreeDepth(level, index);
+    // makeField (DenseFieldRenderer_cu.cu:504-545): one brick, re-interpolated
+    // when the block changes.
+    const uint64_t nodes = rv_brick_nodes(depth);
+    if (nodes > m_brickNodes) {
+        if (m_brick)
+            (void)hipFree(m_brick);
+        m_brick = nullptr;
+        hipCheck(hipMalloc((void**)&m_brick, 4 * nodes), "hipMalloc");
+        m_brickNodes = nodes;
+        m_lastBlock = {~0u, 0, 0, 0};
+    }
+    const std::array<uint32_t, 4> key{level, index[0], index[1], index[2]};
+    if (key != m_lastBlock) {
+        fs.interpolate(level, index, m_brick, m_stream);
+        m_lastBlock = key;
+    }
+    // makeColorTransferTexture (:573-589)
+    if (m_tfDirty) {
+        const uint32_t n = (uint32_t)(st.colorTable.size() / 4);
+        if (n == 0)
+            throw HipError(RV_ERR_INVALID, "renderDenseField: empty colour table");
+        if (n != m_tfN) {
+            if (m_tf)
+                (void)hipFree(m_tf);
+            m_tf = nullptr;
+            hipCheck(hipMalloc((void**)&m_tf, 16 * n), "hipMalloc");
+            m_tfN = n;
+        }
+        hipCheck(hipMemcpyAsync(m_tf, st.colorTable.data(), 16 * n, hipMemcpyHostToDevice,
+                                hs(m_stream)),
+                 "hipMemcpyAsync");
+        m_tfDirty = false;
+    }
+    if (isCancelled())
+        return;
+    const auto range = fs.fieldRange();
+    rv_block_params p;
+    int live = 0;
+    throwIfFailed(rv_block_params_setup(st.cameraTransform.data(), st.viewportSize[0],
+                                        st.viewportSize[1], st.fovY, level, index.data(), depth,
+                                        range[0], range[1], st.threshold, st.renderQuality,
+                                        st.alphaCutoff, &p, &live));
+    if (!live)
+        return;
+    throwIfFailed(rv_render_block_argb(&p, m_brick, m_tf, m_tfN, vp, m_stream));
+}
+
+// ---------------------------------------------------------------- worker
+
+VsWorkerHip::VsWorkerHip() = default;
+
+VsWorkerHip::~VsWorkerHip()
+{
+    if (m_viewport)
+        (void)hipFree(m_viewport);
+    if (m_part)
+        (void)hipFree(m_part);
+}
+
+void VsWorkerHip::initialize(const std::string&, bool, const std::string&, size_t threadId)
+{
+    int n = 0;
+    hipCheck(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (n <= 0)
+        throw HipError(RV_ERR_HIP, "no GPU");
+    hipCheck(hipSetDevice((int)(threadId % (size_t)n)), "hipSetDevice"); // selectGpu.cu:6-9
+}
+
+void VsWorkerHip::setRenderSharedState(const RenderSharedState* s)
+{
+    m_state = s;
+    m_renderer.setRenderSharedState(s);
+}
+
+void VsWorkerHip::updateState(unsigned flags)
+{
+    // VsWorker_cu::updateState (VsWorker_cu.cpp:122-152) does nothing for the
+    // CUDA path; here a colour-table change must reach the device table.
+    constexpr unsigned kUpdateColorTransfer = 1u << 7; // VsWorkerInterface.hpp:41-57
+    if (flags & kUpdateColorTransfer)
+        m_renderer.markColorTableDirty();
+}
+
+RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
+                                           const std::atomic<bool>& isCancelled)
+{
+    return renderScenePart(sub, [&] { return isCancelled.load(); });
+}
+
+RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
+                                           const std::function<bool()>& isCancelled)
+{
+    RgbaImagePart part;
+    m_partRect = {0, 0, 0, 0};
+    m_cancelled = false;
+    if (sub.empty())
+        return part;
+    const RenderSharedState& st = *m_state;
+    const uint32_t W = st.viewportSize[0], H = st.viewportSize[1];
+    const uint64_t px = (uint64_t)W * H;
+    if (px > m_viewportPixels) {
+        if (m_viewport)
+            (void)hipFree(m_viewport);
+        m_viewport = nullptr;
+        hipCheck(hipMalloc((void**)&m_viewport, 4 * px), "hipMalloc");
+        m_viewportPixels = px;
+    }
+    m_renderer.clearViewport(m_viewport);
+    // MacroBlock over the index box + BackToFrontOrder from the eye
+    // (VsWorker_cu.cpp:254-262)
+    float cmin[3], cmax[3], eye[3];
+    throwIfFailed(rv_index_box_coords(sub.level, sub.boxMin.data(), sub.boxMax.data(), cmin, cmax));
+    throwIfFailed(rv_eye_from_view(st.cameraTransform.data(), eye));
+    uint64_t count = 1;
+    for (int a = 0; a < 3; ++a)
+        count *= sub.boxMax[a] - sub.boxMin[a];
+    std::vector<uint32_t> order(3 * count);
+    uint64_t got = 0;
+    throwIfFailed(rv_back_to_front(sub.boxMin.data(), sub.boxMax.data(), cmin, cmax, eye,
+                                   order.data(), count, &got));
+    for (uint64_t i = 0; i < got; ++i) {
+        if (isCancelled()) {
+            m_cancelled = true;
+            break;
+        }
+        const std::array<uint32_t, 3> idx{order[3 * i], order[3 * i + 1], order[3 * i + 2]};
+        m_renderer.renderDenseField(m_viewport, sub.level, idx, isCancelled);
+    }
+    if (m_cancelled || isCancelled()) {
+        m_cancelled = true;
+        hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
+        return part;
+    }
+    // Crop to the non-transparent rectangle (VsWorker_cu.cpp:279-306), on
+    // the device; only the part crosses to the host.
+    int32_t rect[4];
+    int nonempty = 0;
+    throwIfFailed(rv_bound_rect(m_viewport, W, H, rect, &nonempty, stream()));
+    if (!nonempty)
+        return part;
+    const uint64_t n = (uint64_t)rect[2] * rect[3];
+    if (n > m_partCap) {
+        if (m_part)
+            (void)hipFree(m_part);
+        m_part = nullptr;
+        hipCheck(hipMalloc((void**)&m_part, 4 * n), "hipMalloc");
+        m_partCap = n;
+    }
+    hipCheck(hipMemcpy2DAsync(m_part, 4 * (size_t)rect[2], m_viewport + (size_t)rect[1] * W + rect[0],
+                              4 * (size_t)W, 4 * (size_t)rect[2], rect[3], hipMemcpyDeviceToDevice,
+                              hs(stream())),
+             "hipMemcpy2DAsync");
+    part.origin = {rect[0], rect[1]};
+    part.size = {rect[2], rect[3]};
+    part.bits.resize(4 * n);
+    hipCheck(hipMemcpyAsync(part.bits.data(), m_part, 4 * n, hipMemcpyDeviceToHost, hs(stream())),
+             "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
+    m_partRect = {rect[0], rect[1], rect[2], rect[3]};
+    return part;
+}
+
+const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const
+{
+    rect = m_partRect;
+    return m_part;
+}
+
+// ---------------------------------------------------------------- compositing
+
+namespace {
+
+struct DevPart {
+    int x = 0, y = 0, w = 0, h = 0;
+    uint32_t* pix = nullptr;
+};
+
+struct DevBuffers {
+    std::vector<uint32_t*> all;
+    ~DevBuffers()
+    {
+        for (auto* p : all)
+            (void)hipFree(p);
+    }
+    uint32_t* alloc(uint64_t n)
+    {
+        uint32_t* p = nullptr;
+        hipCheck(hipMalloc((void**)&p, 4 * std::max<uint64_t>(n, 1)), "hipMalloc");
+        all.push_back(p);
+        return p;
+    }
+};
+
+} // namespace
+
+std::vector<uint32_t> composeFrame(VsWorkerHip& worker, unsigned level, unsigned splitCount,
+                                   const std::function<bool()>& isCancelled)
+{
+    const RenderSharedState& st = *worker.renderSharedState();
+    const uint32_t W = st.viewportSize[0], H = st.viewportSize[1];
+    float eye[3];
+    throwIfFailed(rv_eye_from_view(st.cameraTransform.data(), eye));
+    std::vector<uint32_t> boxes(6ull * (splitCount + 8));
+    uint64_t nb = 0;
+    throwIfFailed(rv_index_cube_split_sorted(level, splitCount, eye, boxes.data(), splitCount + 8, &nb));
+    DevBuffers bufs;
+    hipStream_t s = hs(worker.stream());
+    std::vector<DevPart> layer;
+    for (uint64_t i = 0; i < nb; ++i) {
+        SubtreeSetDescription sub;
+        sub.level = level;
+        for (int a = 0; a < 3; ++a) {
+            sub.boxMin[a] = boxes[6 * i + a];
+            sub.boxMax[a] = boxes[6 * i + 3 + a];
+        }
+        worker.renderScenePart(sub, isCancelled);
+        if (worker.lastCancelled())
+            return {};
+        std::array<int, 4> r;
+        const uint32_t* d = worker.devicePart(r);
+        DevPart p;
+        p.x = r[0];
+        p.y = r[1];
+        p.w = r[2];
+        p.h = r[3];
+        if (p.w * p.h > 0) {
+            p.pix = bufs.alloc((uint64_t)p.w * p.h);
+            hipCheck(hipMemcpyAsync(p.pix, d, 4ull * p.w * p.h, hipMemcpyDeviceToDevice, s),
+                     "hipMemcpyAsync");
+        }
+        layer.push_back(p);
+    }
+    // AssembleFunc (VsRenderer.cpp:403-473) over the RenderGraphGenerator tree
+    // (RenderGraphGenerator.cpp:64-84): parts paired in order, the earlier
+    // (farther) part copied, the later blended over it; the bounding box
+    // includes both parts' origin and far corner even for empty parts.
+    while (layer.size() > 1) {
+        std::vector<DevPart> next;
+        for (size_t i = 1; i < layer.size(); i += 2) {
+            const DevPart& a = layer[i - 1];
+            const DevPart& b = layer[i];
+            int x0 = std::min({a.x, a.x + a.w, b.x, b.x + b.w});
+            int y0 = std::min({a.y, a.y + a.h, b.y, b.y + b.h});
+            int x1 = std::max({a.x, a.x + a.w, b.x, b.x + b.w});
+            int y1 = std::max({a.y, a.y + a.h, b.y, b.y + b.h});
+            DevPart m;
+            m.x = x0;
+            m.y = y0;
+            m.w = x1 - x0;
+            m.h = y1 - y0;
+            m.pix = bufs.alloc((uint64_t)m.w * m.h);
+            throwIfFailed(rv_fill_u32(m.pix, (uint64_t)m.w * m.h, 0u, worker.stream()));
+            if (a.pix)
+                throwIfFailed(rv_blend_image_part(m.pix, m.w, m.h, m.x, m.y, a.pix, a.w, a.h, a.x,
+                                                  a.y, 1, worker.stream()));
+            if (b.pix)
+                throwIfFailed(rv_blend_image_part(m.pix, m.w, m.h, m.x, m.y, b.pix, b.w, b.h, b.x,
+                                                  b.y, 0, worker.stream()));
+            next.push_back(m);
+        }
+        if (layer.size() % 2)
+            next.push_back(layer.back());
+        layer.swap(next);
+    }
+    // FinalComposeFunc (VsRenderer.cpp:475-519)
+    uint32_t* frame = bufs.alloc((uint64_t)W * H);
+    throwIfFailed(rv_fill_u32(frame, (uint64_t)W * H,
+                              rv_background_argb(st.backgroundColor[0], st.backgroundColor[1],
+                                                 st.backgroundColor[2]),
+                              worker.stream()));
+    if (!layer.empty() && layer[0].pix)
+        throwIfFailed(rv_blend_image_part(frame, W, H, 0, 0, layer[0].pix, layer[0].w, layer[0].h,
+                                          layer[0].x, layer[0].y, 0, worker.stream()));
+    std::vector<uint32_t> out((size_t)W * H);
+    hipCheck(hipMemcpyAsync(out.data(), frame, 4ull * W * H, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return out;
+}
+
+} // namespace revise
+
+// ---------------------------------------------------------------- C ABI
+
+struct rv_worker {
+    revise::FieldServiceHip field;
+    revise::RenderSharedState state;
+    revise::VsWorkerHip worker;
+};
+
+#define W_TRY try {
+#define W_CATCH                                                                        \
+    }                                                                                  \
+    catch (const revise::HipError& e)                                                  \
+    {                                                                                  \
+        rv::set_error(e.what());                                                       \
+        return e.code;                                                                 \
+    }                                                                                  \
+    catch (const std::range_error& e)                                                  \
+    {                                                                                  \
+        rv::set_error(e.what());                                                       \
+        return RV_ERR_INVALID;                                                         \
+    }                                                                                  \
+    catch (const std::bad_alloc&)                                                      \
+    {                                                                                  \
+        rv::set_error("host allocation failed");                                       \
+        return RV_ERR_OOM;                                                             \
+    }                                                                                  \
+    catch (const std::exception& e)                                                    \
+    {                                                                                  \
+        rv::set_error(e.what());                                                       \
+        return RV_ERR_INTERNAL;                                                        \
+    }
+
+extern "C" int rv_worker_create(uint32_t thread_id, rv_worker** out)
+{
+    W_TRY
+    if (!out) {
+        rv::set_error("rv_worker_create: null out");
+        return RV_ERR_INVALID;
+    }
+    *out = nullptr;
+    auto w = std::make_unique<rv_worker>();
+    w->worker.initialize("", false, "", thread_id);
+    w->state.fieldSvc = &w->field;
+    w->worker.setRenderSharedState(&w->state);
+    *out = w.release();
+    return RV_OK;
+    W_CATCH
+}
+
+extern "C" int rv_worker_add_subtree(rv_worker* w, uint32_t level, const uint32_t index[3],
+                                     uint32_t depth, const uint16_t* n2i, uint64_t n,
+                                     const float* sparse)
+{
+    W_TRY
+    if (!w || !index || (n && (!n2i || !sparse))) {
+        rv::set_error("rv_worker_add_subtree: bad arguments");
+        return RV_ERR_INVALID;
+    }
+    w->field.addSubtree(level, {index[0], index[1], index[2]}, depth, n2i, n, sparse);
+    return RV_OK;
+    W_CATCH
+}
+
+extern "C" int rv_worker_set_state(rv_worker* w, const rv_render_state* s)
+{
+    W_TRY
+    if (!w || !s || !s->view16 || !s->tf || s->tf_n == 0 || !(s->fmax > s->fmin)) {
+        rv::set_error("rv_worker_set_state: bad arguments");
+        return RV_ERR_INVALID;
+    }
+    revise::RenderSharedState& st = w->state;
+    std::copy(s->view16, s->view16 + 16, st.cameraTransform.begin());
+    st.viewportSize = {s->vp_w, s->vp_h};
+    st.fovY = s->fov_y_deg;
+    st.renderQuality = s->quality;
+    st.threshold = s->threshold_rel;
+    st.alphaCutoff = s->alpha_cutoff;
+    st.backgroundColor = {s->background[0], s->background[1], s->background[2]};
+    st.colorTable.assign(s->tf, s->tf + 4ull * s->tf_n);
+    w->field.setFieldRange(s->fmin, s->fmax);
+    w->worker.setRenderSharedState(&st);
+    w->worker.updateState(1u << 7);
+    return RV_OK;
+    W_CATCH
+}
+
+extern "C" int rv_worker_render_scene_part(rv_worker* w, uint32_t level, const uint32_t bmin[3],
+                                           const uint32_t bmax[3], const volatile int* cancel,
+                                           int32_t rect[4], uint32_t* bits, uint64_t cap)
+{
+    W_TRY
+    if (!w || !bmin || !bmax || !rect) {
+        rv::set_error("rv_worker_render_scene_part: bad arguments");
+        return RV_ERR_INVALID;
+    }
+    revise::SubtreeSetDescription sub;
+    sub.level = level;
+    for (int a = 0; a < 3; ++a) {
+        sub.boxMin[a] = bmin[a];
+        sub.boxMax[a] = bmax[a];
+    }
+    auto part = w->worker.renderScenePart(sub, [cancel] { return cancel && *cancel != 0; });
+    rect[0] = part.origin[0];
+    rect[1] = part.origin[1];
+    rect[2] = part.size[0];
+    rect[3] = part.size[1];
+    if (w->worker.lastCancelled()) {
+        rv::set_error("rv_worker_render_scene_part: cancelled");
+        return RV_ERR_CANCELLED;
+    }
+    const uint64_t n = part.bits.size() / 4;
+    if (n > cap || (n && !bits)) {
+        rv::set_error("rv_worker_render_scene_part: output capacity too small");
+        return RV_ERR_INVALID;
+    }
+    if (n)
+        std::memcpy(bits, part.bits.data(), part.bits.size());
+    return RV_OK;
+    W_CATCH
+}
+
+extern "C" int rv_worker_compose_frame(rv_worker* w, uint32_t level, uint32_t split_count,
+                                       uint32_t* frame, const volatile int* cancel)
+{
+    W_TRY
+    if (!w || !frame || split_count == 0) {
+        rv::set_error("rv_worker_compose_frame: bad arguments");
+        return RV_ERR_INVALID;
+    }
+    auto out = revise::composeFrame(w->worker, level, split_count,
+                                    [cancel] { return cancel && *cancel != 0; });
+    if (out.empty()) {
+        rv::set_error("rv_worker_compose_frame: cancelled");
+        return RV_ERR_CANCELLED;
+    }
+    std::memcpy(frame, out.data(), 4 * out.size());
+    return RV_OK;
+    W_CATCH
+}
+
+extern "C" void rv_worker_destroy(rv_worker* w) { delete w; }

@@ -1,0 +1,125 @@
+"""The VsWorkerInterface drop-in (rv_worker) against the oracle: a worker
+share (renderScenePart: back-to-front block loop, per-block interpolation and
+ray-cast, crop) and a multi-box frame with on-device assembly and final
+composition (AssembleFunc / FinalComposeFunc restated in the oracle's
+blendImagePart), all bit-exact; cancellation and absent blocks."""
+import numpy as np
+import pytest
+
+from revise_amd import RvError, api, datasets
+from tests.gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+FOV, Q, THR, CUT = 30.0, 2.0, 0.5, 0.996
+BG = (0.1, 0.2, 0.3)
+
+
+def _make(level, depth, seed, W, H, angle):
+    rng = np.random.default_rng(seed)
+    field = datasets.Field(seed, n_gauss=3, noise_octaves=1)
+    m = 2 ** level
+    blocks = {}
+    for z in range(m):
+        for y in range(m):
+            for x in range(m):
+                p = float(rng.uniform(0.3, 1.0))
+                ch = datasets.refined_subtree(depth, lambda l, locs: rng.random(locs.shape[0]) < p)
+                d = api.subtree_depth(ch, depth)
+                n2i = api.subtree_n2i(ch, d)
+                blocks[(x, y, z)] = datasets.Block(level, (x, y, z), d, n2i,
+                                                   datasets.sample(field, level, (x, y, z), d, n2i))
+    fmin, fmax = datasets.field_range(blocks.values())
+    tf = datasets.seeded_tf(seed, 256)
+    M = datasets.orbit_camera(angle)
+    w = api.Worker(0)
+    for b in blocks.values():
+        w.add_subtree(level, b.index, b.depth, b.n2i, b.sparse)
+    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+    return w, blocks, fmin, fmax, tf, M
+
+
+def _oracle_share(orc, blocks, level, bmin, bmax, M, W, H, fmin, fmax, tf):
+    """Oracle worker share with blocks of possibly different depths: the
+    oracle's frame routine takes one depth, so render block by block."""
+    vp = np.zeros((H, W), np.uint32)
+    eye = api.eye_from_view(M)
+    cmin, cmax = api.index_box_coords(level, bmin, bmax)
+    order = orc.b2f_macro(bmin, bmax, cmin, cmax, eye)
+    for idx in order:
+        b = blocks[tuple(int(v) for v in idx)]
+        live, po = orc.block_params(M, W, H, FOV, level, b.index, b.depth, fmin, fmax, THR, Q, CUT)
+        if live:
+            orc.render_block(po, orc.interpolate(b.sparse, b.n2i, b.depth), tf, vp)
+    r = orc.bound_rect(vp)
+    if r is None:
+        return None, None
+    x, y, w, h = r
+    return r, vp[y:y + h, x:x + w]
+
+
+def test_worker_scene_part_bit_exact(orc):
+    W, H = 220, 160
+    w, blocks, fmin, fmax, tf, M = _make(2, 4, 1, W, H, 35.0)
+    for bmin, bmax in [((0, 0, 0), (4, 4, 4)), ((1, 0, 2), (3, 4, 4)), ((3, 3, 3), (4, 4, 4))]:
+        say("worker part", bmin, bmax)
+        got = w.render_scene_part(2, bmin, bmax)
+        rect, bits = got
+        erect, ebits = _oracle_share(orc, blocks, 2, bmin, bmax, M, W, H, fmin, fmax, tf)
+        assert rect == erect
+        assert np.array_equal(bits, ebits)
+    w.close()
+
+
+def test_worker_empty_box_and_cancel(orc):
+    W, H = 64, 48
+    w, *_ = _make(1, 3, 2, W, H, 10.0)
+    rect, bits = w.render_scene_part(1, (1, 1, 1), (1, 2, 2))  # empty index box
+    assert rect == (0, 0, 0, 0) and bits.size == 0
+    assert w.render_scene_part(1, (0, 0, 0), (2, 2, 2), cancel=True) is None
+    with pytest.raises(RvError, match="Block does not exist"):
+        w.render_scene_part(2, (0, 0, 0), (1, 1, 1))
+    w.close()
+
+
+def _oracle_frame(orc, blocks, level, K, M, W, H, fmin, fmax, tf):
+    eye = api.eye_from_view(M)
+    boxes = orc.index_cube_split_sorted(level, K, eye)
+    parts = []
+    for b in boxes:
+        r, bits = _oracle_share(orc, blocks, level, b[:3], b[3:], M, W, H, fmin, fmax, tf)
+        parts.append((r, bits) if r is not None else ((0, 0, 0, 0), None))
+    def bbox(a, b):
+        xs = [a[0][0], a[0][0] + a[0][2], b[0][0], b[0][0] + b[0][2]]
+        ys = [a[0][1], a[0][1] + a[0][3], b[0][1], b[0][1] + b[0][3]]
+        return min(xs), min(ys), max(xs) - min(xs), max(ys) - min(ys)
+    layer = parts
+    while len(layer) > 1:
+        nxt = []
+        for i in range(1, len(layer), 2):
+            a, b = layer[i - 1], layer[i]
+            x, y, w_, h_ = bbox(a, b)
+            img = np.zeros((h_, w_), np.uint32)
+            if a[1] is not None:
+                orc.blend_image_part(img, a[1], a[0][:2], (x, y), True)
+            if b[1] is not None:
+                orc.blend_image_part(img, b[1], b[0][:2], (x, y), False)
+            nxt.append(((x, y, w_, h_), img))
+        if len(layer) % 2:
+            nxt.append(layer[-1])
+        layer = nxt
+    frame = np.full((H, W), orc.background_argb(*BG), np.uint32)
+    if layer[0][1] is not None:
+        orc.blend_image_part(frame, layer[0][1], layer[0][0][:2], (0, 0), False)
+    return frame
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 5, 8])
+def test_worker_compose_frame_bit_exact(orc, K):
+    W, H = 200, 150
+    w, blocks, fmin, fmax, tf, M = _make(2, 3, 7, W, H, 120.0)
+    say("compose frame K", K)
+    got = w.compose_frame(2, K)
+    exp = _oracle_frame(orc, blocks, 2, K, M, W, H, fmin, fmax, tf)
+    assert np.array_equal(got, exp)
+    w.close()
