@@ -1,17 +1,29 @@
 """Benchmark of the sparse->dense interpolation + ray-cast path on one MI355X.
 
-Workload (the GPU plain drop-in configuration): a block tree refined to a
-uniform level 3 (512 blocks), each block a fully refined depth-6 subtree
-(65^3 nodes, fp32), field = 3 seeded Gaussians sampled at the nodes.  One step
-= one frame of the reference's per-worker path (VsWorker_cu::renderScenePart):
-clear the 1920x1080 ARGB viewport, walk the blocks in BackToFrontOrder and for
-each block interpolate its brick from the sparse node values and ray-cast it
-(step 0.5 voxel = quality 2, early termination at alpha > 0.99), then find the
-non-transparent rectangle and download it.
+Headline workload (`value`): the GPU plain drop-in frame -- a block tree
+refined to a uniform level 3 (512 blocks), each block a fully refined depth-6
+subtree (65^3 nodes, fp32), field = 3 seeded Gaussians sampled at the nodes.
+One step = one frame of the reference's per-worker path
+(VsWorker_cu::renderScenePart) with its exact per-block semantics: clear the
+1920x1080 ARGB viewport, interpolate every block's brick from its sparse node
+values, ray-cast the blocks in BackToFrontOrder (step 0.5 voxel = quality 2,
+early termination at alpha > 0.99) with the per-block 8-bit composite, find
+the non-transparent rectangle and download it.  The drop-in worker does this
+in two launches (batched interpolation, then the whole share), bit-identical
+to one interpolation + one ray-cast launch per block; the per-block launch
+structure is timed too.  `value` = frames/s of that step.
 
-Prints one JSON line.  `value` is frames/s of that step; the line also carries
-the batched interpolation throughput (Gvoxels/s) with its HBM roofline, the
-render kernel figures, and a CPU baseline of the oracle on a bounded sample.
+The JSON line also carries:
+* interpolation of all 512 bricks in one launch: Gvoxels/s and the HBM
+  roofline of that kernel (`roofline`);
+* the render kernel: exact samples per frame S (reference sampling rule, no
+  early termination) and samples/s;
+* the batched persistent frame on the same data (interpolation + all blocks in
+  one launch, front-to-back in registers);
+* the 2000-block mixed-level configuration (levels 2-6, fill 5-100 %, 8
+  Gaussians + 3-octave noise): batched persistent frames over 64 cameras
+  replayed from one hipGraph vs the plain per-block path on the same data;
+* `cpu_baseline`: the CPU oracle on the box's host cores, bounded sample.
 """
 import argparse
 import json
@@ -27,102 +39,16 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
 SEED = 2024
+W, H = 1920, 1080
+FOV, THR, Q2, CUT2 = 30.0, 0.5, 2.0, 0.99
+ACC_CUT = 0.9999
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_config2(torch, seed=SEED, level=3, depth=6):
-    from revise_amd import api, datasets
-    field = datasets.Field(seed, n_gauss=3)
-    n2i = api.subtree_n2i(datasets.full_subtree(depth), depth)
-    N = n2i.shape[0]
-    m = 2 ** level
-    dev = "cuda"
-    n2i_d = torch.from_numpy(n2i.astype(np.int64)).to(dev).double()
-    c = torch.tensor(field.c, dtype=torch.float64, device=dev)
-    s = torch.tensor(field.s, dtype=torch.float64, device=dev)
-    sparse = torch.empty((m ** 3, N), dtype=torch.float32, device=dev)
-    idx = []
-    scale = 1.0 / 2 ** (level + depth)
-    b = 0
-    for z in range(m):
-        for y in range(m):
-            for x in range(m):
-                base = torch.tensor([x, y, z], dtype=torch.float64, device=dev) * 2 ** depth
-                u = (base[None, :] + n2i_d) * scale
-                f = torch.zeros(N, dtype=torch.float64, device=dev)
-                for k in range(c.shape[0]):
-                    d2 = ((u - c[k]) ** 2).sum(-1)
-                    f += torch.exp(-d2 / (2 * s[k] * s[k]))
-                sparse[b] = f.float()
-                idx.append((x, y, z))
-                b += 1
-    return n2i, sparse, idx
-
-
-class Frame:
-    """Config-2 frame state on the device."""
-
-    def __init__(self, torch, W=1920, H=1080, level=3, depth=6, seed=SEED):
-        from revise_amd import api, datasets
-        self.torch, self.api = torch, api
-        self.W, self.H, self.level, self.depth = W, H, level, depth
-        t0 = time.time()
-        self.n2i, self.sparse, self.idx = make_config2(torch, seed, level, depth)
-        self.nblocks, self.N = self.sparse.shape
-        self.V = api.brick_nodes(depth)
-        self.plan = torch.empty(self.V, dtype=torch.int32, device="cuda")
-        api.interp_plan_build(torch.from_numpy(self.n2i.view(np.int16)).cuda(), depth, self.plan)
-        self.bricks = torch.empty((self.nblocks, self.V), dtype=torch.float32, device="cuda")
-        self.fmin = np.float32(self.sparse.min().item())
-        self.fmax = np.float32(self.sparse.max().item())
-        self.tf_np = datasets.seeded_tf(seed, 256)
-        self.tf = torch.from_numpy(self.tf_np).cuda()
-        rng = np.random.default_rng(seed)
-        self.angle = float(rng.uniform(0.0, 360.0))
-        self.M = datasets.orbit_camera(self.angle)
-        eye = api.eye_from_view(self.M)
-        m = 2 ** level
-        cmin, cmax = api.index_box_coords(level, [0, 0, 0], [m, m, m])
-        self.order = [tuple(int(v) for v in o) for o in api.back_to_front([0] * 3, [m] * 3, cmin, cmax, eye)]
-        pos = {ix: i for i, ix in enumerate(self.idx)}
-        self.params = []
-        for ix in self.order:
-            live, p = api.block_params(self.M, W, H, 30.0, level, ix, depth, self.fmin, self.fmax,
-                                       0.5, 2.0, 0.99)
-            self.params.append((pos[ix], live, p))
-        self.vp = torch.empty((H, W), dtype=torch.int32, device="cuda")
-        self.batch = api.InterpBatch([(self.sparse[i], self.plan, self.bricks[i], depth)
-                                      for i in range(self.nblocks)])
-        torch.cuda.synchronize()
-        log("setup %.1fs: %d blocks x %d nodes, range [%g, %g], angle %.1f" % (
-            time.time() - t0, self.nblocks, self.N, self.fmin, self.fmax, self.angle))
-
-    def step(self):
-        """Plain drop-in frame: per block interpolate + ray-cast, then crop."""
-        api = self.api
-        api.clear_viewport(self.vp, self.W, self.H)
-        for b, live, p in self.params:
-            api.interp_brick_planned(self.sparse[b], self.plan, self.depth, self.bricks[b])
-            if live:
-                api.render_block_argb(p, self.bricks[b], self.tf, self.vp)
-        r = api.bound_rect(self.vp, self.W, self.H)
-        if r is not None:
-            x, y, w, h = r
-            return self.vp[y:y + h, x:x + w].cpu()
-        return None
-
-    def render_only(self):
-        api = self.api
-        api.clear_viewport(self.vp, self.W, self.H)
-        for b, live, p in self.params:
-            if live:
-                api.render_block_argb(p, self.bricks[b], self.tf, self.vp)
-
-
-def time_events(torch, fn, reps):
+def events_time(torch, fn, reps):
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     s.record()
@@ -133,76 +59,128 @@ def time_events(torch, fn, reps):
     return s.elapsed_time(e) / 1e3 / reps
 
 
-def samples_per_frame(fr, orc_mod=None):
-    """Exact count S = sum over blocks and pixels of (sampleCount-1), counted
-    with the reference rule on the CPU for a row subsample and scaled."""
-    return None
+class PlainFrame:
+    """Per-block drop-in path over a workload (one viewport)."""
+
+    def __init__(self, torch, wl, M, quality, cutoff, tf):
+        from revise_amd import api
+        self.api, self.torch = api, torch
+        self.wl, self.tf = wl, tf
+        keys = np.array([(b["level"],) + tuple(b["index"]) for b in wl.blocks], dtype=np.uint32)
+        order = api.mixed_back_to_front(keys, api.eye_from_view(M))
+        self.items = []
+        for i in order:
+            b = wl.blocks[int(i)]
+            live, p = api.block_params(M, W, H, FOV, b["level"], b["index"], b["depth"], wl.fmin, wl.fmax,
+                                       THR, quality, cutoff)
+            self.items.append((b, live, p))
+        self.vp = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        self.batch = None
+        self.share = None
+
+    def fused_step(self, crop=True):
+        """Drop-in worker frame in two launches (bit-identical to step())."""
+        api = self.api
+        if self.batch is None:
+            self.batch = api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"])
+                                          for b, _, _ in self.items])
+            self.share = api.ShareRenderer([(p, b["brick"]) for b, live, p in self.items if live],
+                                           self.tf, W, H)
+        api.clear_viewport(self.vp, W, H)
+        self.batch.run()
+        self.share.render(self.vp)
+        if crop:
+            r = api.bound_rect(self.vp, W, H)
+            if r is not None:
+                x, y, w, h = r
+                return self.vp[y:y + h, x:x + w].cpu()
+        return None
+
+    def step(self, interpolate=True, crop=True):
+        api = self.api
+        api.clear_viewport(self.vp, W, H)
+        for b, live, p in self.items:
+            if interpolate:
+                api.interp_brick_planned(b["sparse"], b["plan"], b["depth"], b["brick"])
+            if live:
+                api.render_block_argb(p, b["brick"], self.tf, self.vp)
+        if crop:
+            r = api.bound_rect(self.vp, W, H)
+            if r is not None:
+                x, y, w, h = r
+                return self.vp[y:y + h, x:x + w].cpu()
+        return None
+
+    def samples(self):
+        c = self.torch.zeros(1, dtype=self.torch.int64, device="cuda")
+        for b, live, p in self.items:
+            if live:
+                self.api.block_sample_count(p, c)
+        return int(c.item())
 
 
-def cpu_baseline(fr, budget_s=20.0):
-    """Oracle timed on the host: single-thread interpolation of whole bricks
-    (Gvoxels/s) and a multi-threaded frame on a band of viewport rows
-    (scaled to frames/s)."""
-    from oracle import oracle as orc
-    ncores = os.cpu_count() or 1
-    ncores = min(ncores, 16)
-    # interpolation: the reference interpolates one brick per call, single thread
-    n2i = fr.n2i
-    sp = fr.sparse[:4].cpu().numpy()
-    t0 = time.perf_counter()
-    k = 0
-    while True:
-        orc.interpolate(sp[k % 4], n2i, fr.depth)
-        k += 1
-        if time.perf_counter() - t0 > budget_s * 0.25 and k >= 2:
-            break
-    t_interp = (time.perf_counter() - t0) / k
-    gvox_1core = fr.V / t_interp / 1e9
-    # ray-cast: a band of rows through the middle of the frame, all blocks
-    bricks = fr.bricks.cpu().numpy()
-    band = 32
-    y0 = fr.H // 2 - band // 2
-    vp = np.zeros((fr.H, fr.W), np.uint32)
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        for b, live, p in fr.params:
-            if not live:
-                continue
-            # oracle params (its own setup) for the same block
-            lo, po = orc.block_params(fr.M, fr.W, fr.H, 30.0, fr.level, fr.idx[b], fr.depth, fr.fmin,
-                                      fr.fmax, 0.5, 2.0, 0.99)
-            _band_render(orc, po, bricks[b], fr.tf_np, vp, y0, y0 + band, ncores)
-        reps += 1
-        if time.perf_counter() - t0 > budget_s * 0.75 or reps >= 3:
-            break
-    t_band = (time.perf_counter() - t0) / reps
-    fps = 1.0 / (t_band * fr.H / band)
-    return {"value": fps, "unit": "frames/s", "cores": ncores, "kind": "port",
-            "sample": "oracle ray-cast of all 512 blocks over a %d-row band of the 1080p frame, "
-                      "scaled by 1080/%d; interpolation single-thread %.3f Gvox/s over whole bricks"
-                      % (band, band, gvox_1core),
-            "interp_gvox_s_1core": gvox_1core}
+def frame_plan(torch, wl, tf, quality, cutoff, tile=(0, 0)):
+    from revise_amd import api
+    return api.FramePlan(wl.blocks, W, H, FOV, wl.fmin, wl.fmax, THR, quality, cutoff, ACC_CUT, tf, tile)
 
 
-def _band_render(orc, po, brick, tf, vp, r0, r1, nthreads):
+def cpu_baseline(wl, M, tf_np, budget_s=20.0):
+    """Oracle on the host: whole-brick interpolation single-threaded
+    (Gvoxels/s) and the per-block ray-cast of all 512 blocks over a band of
+    rows with every host core (scaled to frames/s)."""
     import ctypes as C
     import threading
+    from oracle import oracle as orc
+    ncores = min(os.cpu_count() or 1, 16)
+    n2i = wl.n2i["full"]
+    sp = wl.sparse[:2].cpu().numpy()
+    t0 = time.perf_counter()
+    k = 0
+    while k < 2 or time.perf_counter() - t0 < budget_s * 0.2:
+        orc.interpolate(sp[k % 2], n2i, 6)
+        k += 1
+    t_interp = (time.perf_counter() - t0) / k
+    gvox_1core = (65 ** 3) / t_interp / 1e9
+    bricks = wl.bricks.cpu().numpy()
+    keys = np.array([(b["level"],) + tuple(b["index"]) for b in wl.blocks], dtype=np.uint32)
+    from revise_amd import api
+    order = api.mixed_back_to_front(keys, api.eye_from_view(M))
+    band = 24
+    y0 = H // 2 - band // 2
+    vp = np.zeros((H, W), np.uint32)
     L = orc.lib()
-    brick = np.ascontiguousarray(brick)
-    bp = brick.ctypes.data_as(C.POINTER(C.c_float))
-    tp = np.ascontiguousarray(tf).ctypes.data_as(C.POINTER(C.c_float))
+    tp = np.ascontiguousarray(tf_np).ctypes.data_as(C.POINTER(C.c_float))
     vpp = vp.ctypes.data_as(C.POINTER(C.c_uint32))
-    rows = list(range(r0, r1))
-    chunk = max(1, math.ceil(len(rows) / nthreads))
-    ths = []
-    for i in range(0, len(rows), chunk):
-        a, b = rows[i], rows[min(len(rows), i + chunk) - 1] + 1
-        th = threading.Thread(target=L.orc_render_block, args=(C.byref(po), bp, tp, tf.shape[0], vpp, a, b))
-        th.start()
-        ths.append(th)
-    for th in ths:
-        th.join()
+    t0 = time.perf_counter()
+    reps = 0
+    while reps < 1 or time.perf_counter() - t0 < budget_s * 0.5:
+        for i in order:
+            b = wl.blocks[int(i)]
+            live, po = orc.block_params(M, W, H, FOV, b["level"], b["index"], b["depth"], wl.fmin, wl.fmax,
+                                        THR, Q2, CUT2)
+            if not live or po.view_y > y0 + band or po.view_y + po.view_h < y0:
+                continue
+            bp = bricks[int(i)].ctypes.data_as(C.POINTER(C.c_float))
+            rows = list(range(y0, y0 + band))
+            chunk = max(1, math.ceil(band / ncores))
+            ths = []
+            for j in range(0, band, chunk):
+                th = threading.Thread(target=L.orc_render_block,
+                                      args=(C.byref(po), bp, tp, tf_np.shape[0], vpp, rows[j],
+                                            rows[min(band, j + chunk) - 1] + 1))
+                th.start()
+                ths.append(th)
+            for th in ths:
+                th.join()
+        reps += 1
+    t_band = (time.perf_counter() - t0) / reps
+    fps = 1.0 / (t_band * H / band)
+    return {"value": fps, "unit": "frames/s", "cores": ncores, "kind": "port",
+            "sample": "CPU oracle: per-block ray-cast of all 512 blocks over the middle %d rows of the "
+                      "1080p frame, %d threads, scaled by 1080/%d (interpolation is not in this "
+                      "figure: whole 65^3 bricks single-threaded run at %.3f Gvoxels/s)"
+                      % (band, ncores, band, gvox_1core),
+            "interp_gvox_s_1core": gvox_1core}
 
 
 def main():
@@ -211,47 +189,83 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cams", type=int, default=64)
+    ap.add_argument("--only-interp", action="store_true", help="time the batched interpolation only")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only; --gpus %d is not supported "
-              "(the multi-GPU sort-last exchange is design-only)" % args.gpus, file=sys.stderr)
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (the multi-GPU "
+              "sort-last exchange is a design, not built)" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
-    from revise_amd import lib
+    from revise_amd import api, configs, datasets, lib
     lib()
     torch.cuda.set_device(0)
-    fr = Frame(torch)
+    out = {}
 
+    # ---------------- config 2: 512 level-3 bricks, plain drop-in frame
+    t0 = time.time()
+    field2 = datasets.Field(SEED, n_gauss=3)
+    wl = configs.uniform(torch, field2, level=3, depth=6)
+    tf_np = datasets.seeded_tf(SEED, 256)
+    tf = torch.from_numpy(tf_np).cuda()
+    angle = float(np.random.default_rng(SEED).uniform(0.0, 360.0))
+    M = datasets.orbit_camera(angle)
+    plain = PlainFrame(torch, wl, M, Q2, CUT2, tf)
+    torch.cuda.synchronize()
+    log("config 2 setup %.1fs" % (time.time() - t0))
+    if args.only_interp:
+        batch = api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"]) for b in wl.blocks])
+        batch.run()
+        log("batched interp: %.3f ms" % (events_time(torch, batch.run, 10) * 1e3))
+        return
     for _ in range(args.warmup):
-        fr.step()
+        plain.fused_step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        fr.step()
+        plain.fused_step()
     torch.cuda.synchronize()
     t_step = (time.perf_counter() - t0) / args.steps
-    log("plain step %.3f ms" % (t_step * 1e3))
+    log("drop-in worker frame (fused) %.3f ms" % (t_step * 1e3))
+    plain.step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        plain.step()
+    torch.cuda.synchronize()
+    t_step_pb = (time.perf_counter() - t0) / 2
+    log("per-block launches frame %.3f ms" % (t_step_pb * 1e3))
+    t_share = events_time(torch, lambda: plain.share.render(plain.vp), 5)
 
-    # batched interpolation of all 512 bricks in one launch
-    fr.batch.run()
-    t_interp = time_events(torch, fr.batch.run, 20)
-    V_total = fr.nblocks * fr.V
-    N_total = fr.nblocks * fr.N
+    batch = api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"]) for b in wl.blocks])
+    batch.run()
+    t_interp = events_time(torch, batch.run, 20)
+    V_total, N_total = wl.dense_nodes, wl.sparse_nodes
     B_alg = 4 * V_total + 10 * N_total
-    interp_gbs = B_alg / t_interp
     log("batched interp %.3f ms, %.1f Gvox/s, %.0f GB/s" % (t_interp * 1e3, V_total / t_interp / 1e9,
-                                                          interp_gbs / 1e9))
-    # render only (bricks resident)
-    t_render = time_events(torch, fr.render_only, 5)
-    log("render-only frame %.3f ms" % (t_render * 1e3))
-    # per-block interp (plain path) share
-    def interp_plain():
-        for b, _, _ in fr.params:
-            fr.api.interp_brick_planned(fr.sparse[b], fr.plan, fr.depth, fr.bricks[b])
-    t_interp_plain = time_events(torch, interp_plain, 3)
+                                                          B_alg / t_interp / 1e9))
+    t_interp_plain = events_time(torch, lambda: [api.interp_brick_planned(b["sparse"], b["plan"], 6, b["brick"])
+                                                 for b in wl.blocks], 3)
+    t_render = events_time(torch, lambda: plain.step(interpolate=False, crop=False), 3)
+    S = plain.samples()
+    log("per-block render %.3f ms, share launch %.3f ms, S = %.3g samples/frame, %.3g samples/s (share)"
+        % (t_render * 1e3, t_share * 1e3, S, S / t_share))
 
-    out = {
+    fp = frame_plan(torch, wl, tf, Q2, CUT2)
+    fp.set_cameras(M[None])
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    tile = fp.autotune(argb, True, os.path.join(ROOT, "gpurun_out", "frame_tune.jsonl"))
+    fp.render(0, argb, None, True)
+    t_batched = events_time(torch, lambda: fp.render(0, argb, None, True), 5)
+    t_batched_res = events_time(torch, lambda: fp.render(0, argb, None, False), 5)
+    assert fp.status() == 0
+    log("batched frame %.3f ms (resident bricks %.3f ms), tile %s" % (t_batched * 1e3, t_batched_res * 1e3,
+                                                                     tile))
+    fp.close()
+
+    out.update({
         "metric": "sparse→dense Gvoxels/s; 1080p ray-cast frames/s; fwd+adjoint ms/frame",
         "value": 1.0 / t_step,
         "unit": "frames/s",
@@ -264,21 +278,68 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Gaussian field sampled at block-tree nodes)",
-        "config": {"workload": "plain drop-in frame: 512 level-3 blocks x 65^3 fp32 bricks "
-                               "(full fill), 1920x1080 ARGB, quality 2, alpha cutoff 0.99",
-                   "blocks": fr.nblocks, "brick_nodes": fr.V, "viewport": [fr.W, fr.H],
-                   "seed": SEED, "camera_angle_deg": fr.angle},
+        "config": {"workload": "plain drop-in frame: 512 level-3 blocks x 65^3 fp32 bricks (full fill), "
+                               "1920x1080 ARGB, quality 2, alpha cutoff 0.99, interpolate every brick + "
+                               "ray-cast in back-to-front order with per-block 8-bit composite, crop + "
+                               "download",
+                   "blocks": len(wl.blocks), "brick_nodes": 65 ** 3, "viewport": [W, H], "seed": SEED,
+                   "camera_angle_deg": angle},
+        "per_block_launches_frame_ms": t_step_pb * 1e3,
         "interp_gvox_s": V_total / t_interp / 1e9,
         "interp_ms_batched": t_interp * 1e3,
         "interp_ms_per_block_launches": t_interp_plain * 1e3,
-        "render_ms_frame": t_render * 1e3,
-        "render_fps": 1.0 / t_render,
-        "roofline": {"kernel": "interp_batch_kernel", "bound": "hbm", "achieved": interp_gbs / 1e9,
-                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": interp_gbs / HBM_PEAK,
-                     "traffic": None},
-    }
+        "render_share_ms": t_share * 1e3,
+        "render_per_block_launches_ms": t_render * 1e3,
+        "render_samples_per_frame": S,
+        "render_gsamples_s": S / t_share / 1e9,
+        "batched_frame_ms": t_batched * 1e3,
+        "batched_frame_fps": 1.0 / t_batched,
+        "batched_frame_resident_ms": t_batched_res * 1e3,
+        "batched_tile": list(tile),
+        "roofline": {"kernel": "interp_batch_kernel (512 bricks, one launch)", "bound": "hbm",
+                     "achieved": B_alg / t_interp / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": B_alg / t_interp / HBM_PEAK, "traffic": None,
+                     "alg_bytes_per_launch": B_alg},
+    })
+    del batch
+
+    # ---------------- config 3: 2000 mixed-level blocks, batched vs plain
+    if not args.no_mixed:
+        t0 = time.time()
+        field3 = datasets.Field(SEED + 3, n_gauss=8, noise_octaves=3)
+        wl3 = configs.mixed(torch, field3, nblocks=2000, depth=6, seed=SEED + 3)
+        cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in
+                         zip(np.random.default_rng(SEED + 3).uniform(0, 360, args.cams),
+                             np.random.default_rng(SEED + 4).uniform(-45, 45, args.cams))])
+        torch.cuda.synchronize()
+        log("config 3 setup %.1fs: %d blocks, %d sparse nodes" % (time.time() - t0, len(wl3.blocks),
+                                                                  wl3.sparse_nodes))
+        fp3 = frame_plan(torch, wl3, tf, Q2, CUT2)
+        fp3.set_cameras(cams)
+        argb3 = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        tile3 = fp3.autotune(argb3, True, os.path.join(ROOT, "gpurun_out", "frame_tune.jsonl"))
+        frames = torch.empty((args.cams, H, W), dtype=torch.int32, device="cuda")
+        fp3.graph_build(frames, H * W, True)
+        fp3.graph_launch()
+        torch.cuda.synchronize()
+        t_graph = events_time(torch, fp3.graph_launch, 2) / args.cams
+        assert fp3.status() == 0
+        log("config 3 batched: %.3f ms/frame over %d cameras (graph), tile %s" % (t_graph * 1e3, args.cams,
+                                                                                 tile3))
+        plain3 = PlainFrame(torch, wl3, cams[0], Q2, CUT2, tf)
+        plain3.step()
+        t_plain3 = events_time(torch, lambda: plain3.step(crop=False), 2)
+        log("config 3 plain per-block: %.3f ms/frame" % (t_plain3 * 1e3))
+        out["mixed2000"] = {"workload": "2000 blocks, levels 2-6, 65^3 fp32 bricks, fill 5-100%%, 8 Gaussians "
+                                        "+ 3-octave noise, 1920x1080, %d cameras" % args.cams,
+                            "batched_graph_fps": 1.0 / t_graph, "batched_ms_per_frame": t_graph * 1e3,
+                            "plain_per_block_fps": 1.0 / t_plain3, "speedup_vs_plain": t_plain3 / t_graph,
+                            "tile": list(tile3), "sparse_nodes": wl3.sparse_nodes}
+        fp3.close()
+        del frames, wl3
+
     if not args.no_cpu:
-        cb = cpu_baseline(fr, args.cpu_budget)
+        cb = cpu_baseline(wl, M, tf_np, args.cpu_budget)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
     print(json.dumps(out))
 

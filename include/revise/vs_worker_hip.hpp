@@ -64,7 +64,8 @@ public:
     FieldServiceHip(const FieldServiceHip&) = delete;
     FieldServiceHip& operator=(const FieldServiceHip&) = delete;
 
-    // Registers a subtree root; values are copied to the device.
+    // Registers a subtree root; values are copied to the device and the
+    // subtree's gather plan is encoded from its n2i (once per subtree).
     void addSubtree(unsigned level, const std::array<uint32_t, 3>& index, unsigned depth,
                     const uint16_t* n2i, uint64_t n, const float* sparse);
     void setFieldRange(float fmin, float fmax);
@@ -73,15 +74,22 @@ public:
     unsigned subtreeDepth(unsigned level, const std::array<uint32_t, 3>& index) const;
     // interpolateWith<DenseFieldInterpolatorHip>: interpolate the subtree of
     // (level, index) into d_dense (rv_brick_nodes(depth) floats) on `stream`;
-    // the gather plan is built once per subtree and cached.  Throws
+    // through the subtree's cached gather plan.  Throws
     // std::range_error for an absent block (CompressedBlockTree.hpp:77-78).
     void interpolate(unsigned level, const std::array<uint32_t, 3>& index, float* d_dense,
                      rv_stream_t stream);
+    // The interpolation job of a subtree into d_dense, for batched
+    // interpolation of many blocks in one launch.
+    rv_brick_job job(unsigned level, const std::array<uint32_t, 3>& index, float* d_dense,
+                     rv_stream_t stream);
+    // Bumped by addSubtree: cached batches of a previous field are stale.
+    uint64_t version() const { return m_version; }
 
 private:
     struct Subtree;
     std::map<std::array<uint32_t, 4>, std::unique_ptr<Subtree>> m_subtrees;
     float m_fmin = 0.f, m_fmax = 1.f;
+    uint64_t m_version = 0;
 };
 
 // The consumed fields of VsRenderSharedState / VsControllerInput
@@ -113,6 +121,9 @@ public:
     void renderDenseField(uint32_t* d_viewport, unsigned level, const std::array<uint32_t, 3>& index,
                           const std::function<bool()>& isCancelled);
     void markColorTableDirty() { m_tfDirty = true; }
+    void ensureColorTable();
+    const float* colorTable() const { return m_tf; }
+    uint32_t colorTableSize() const { return m_tfN; }
     rv_stream_t stream() const { return m_stream; }
 
 private:
@@ -144,10 +155,21 @@ public:
                                   const std::function<bool()>& isCancelled);
     // The last part on the device: pixels (w*h ARGB) and {x, y, w, h}.
     const uint32_t* devicePart(std::array<int, 4>& rect) const;
+    void renderFused(const SubtreeSetDescription& sub, const std::vector<uint32_t>& order,
+                     uint64_t count, const std::function<bool()>& isCancelled);
     bool lastCancelled() const { return m_cancelled; }
     rv_stream_t stream() const { return m_renderer.stream(); }
+    // fused (default): the share's bricks are interpolated in one batched
+    // launch and ray-cast in one launch (rv_render_share_argb), bit-identical
+    // to the per-block loop; per-block: one renderDenseField call per block,
+    // the reference's structure.  Cancellation is polled between blocks
+    // (per-block) or before each launch (fused).
+    void setFused(bool fused) { m_fused = fused; }
 
 private:
+    struct ShareCache;
+    std::unique_ptr<ShareCache> m_share;
+    bool m_fused = true;
     const RenderSharedState* m_state = nullptr;
     DenseFieldRendererHip m_renderer;
     uint32_t* m_viewport = nullptr;

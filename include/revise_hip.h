@@ -81,36 +81,45 @@ int rv_node_ordinals(const uint16_t* d_n2i, uint64_t n, uint32_t depth, uint32_t
 /* Number of nodes of a dense brick, (2^depth+1)^3. */
 uint64_t rv_brick_nodes(uint32_t depth);
 
-/* Gather plan of a subtree: d_plan[(2^depth+1)^3] receives, per dense node,
- * the index of its sparse value or 0xFFFFFFFF.  A plan depends only on the
- * subtree's n2i, so callers cache it per subtree (as the reference caches
- * BlockTreeNodes, BlockTreeFieldService.hpp:133-137).  Entries whose
- * coordinates fall outside the brick are ignored and counted into
- * *d_bad_count when it is not NULL (a device uint32 the caller zeroes). */
-int rv_interp_plan_build(const uint16_t* d_n2i, uint64_t n, uint32_t depth, uint32_t* d_plan,
-                         uint32_t* d_bad_count, rv_stream_t stream);
+/* Gather plan of a subtree, from its n2i (host array, n x 3).  The kernel
+ * completes a brick in tiles of at most 64 x 16 x 16 cells; the plan lists,
+ * per tile, the sparse nodes inside the tile's closed box in increasing
+ * sparse index with their offsets in the tile's LDS array, and the sparse
+ * index of each node of the brick's stride-16 lattice.  Its size is about
+ * 6 bytes per (node, tile) pair.  A plan depends only on the subtree's n2i,
+ * so callers cache it per subtree (as the reference caches BlockTreeNodes,
+ * BlockTreeFieldService.hpp:133-137).  RV_ERR_INVALID if a coordinate lies
+ * outside the (2^depth+1)^3 brick. */
+uint64_t rv_interp_plan_bytes(const uint16_t* n2i, uint64_t n, uint32_t depth);
+int rv_interp_plan_encode(const uint16_t* n2i, uint64_t n, uint32_t depth, void* plan,
+                          uint64_t bytes);
+/* Encode and upload (synchronous); free with rv_interp_plan_free. */
+int rv_interp_plan_build(const uint16_t* n2i, uint64_t n, uint32_t depth, void** d_plan,
+                         uint64_t* bytes);
+int rv_interp_plan_free(void* d_plan);
 
 /* Interpolate one brick from its sparse values through a plan.
  * Replaces DenseFieldInterpolator_cu_3::interpolate
  * (DenseFieldInterpolator_cu_3.cu:197-253; DenseFieldInterpolatorArr_cu_3.cu:197-257),
  * with the CPU semantics of DenseFieldInterpolator<3>::interpolate
- * (DenseFieldInterpolator.hpp:198-419), bit for bit. */
-int rv_interp_brick_planned(const float* d_sparse, const uint32_t* d_plan, uint32_t depth,
+ * (DenseFieldInterpolator.hpp:198-419), bit for bit.  depth must be the
+ * depth the plan was built for. */
+int rv_interp_brick_planned(const float* d_sparse, const void* d_plan, uint32_t depth,
                             float* d_dense, rv_stream_t stream);
 
-/* Plan + interpolate in one call (the plain drop-in for
+/* Plan + interpolate in one call, from the host n2i (the plain drop-in for
  * DenseFieldInterpolatorTpl_cu_3::interpolate(denseDev, sparse, nodes),
- * DenseFieldInterpolatorTpl_cu_3.hpp:103-107).  d_workspace: caller-owned
- * scratch of rv_brick_nodes(depth) uint32; if NULL the library allocates it
- * stream-ordered for the call. */
-int rv_interp_brick(const float* d_sparse, const uint16_t* d_n2i, uint64_t n, uint32_t depth,
-                    float* d_dense, uint32_t* d_workspace, rv_stream_t stream);
+ * DenseFieldInterpolatorTpl_cu_3.hpp:103-107, which uploads the n2i per
+ * call).  The plan is staged stream-ordered and the call synchronizes the
+ * stream before returning. */
+int rv_interp_brick(const float* d_sparse, const uint16_t* n2i, uint64_t n, uint32_t depth,
+                    float* d_dense, rv_stream_t stream);
 
 /* One brick of a batch.  dense16 != NULL stores fp16 instead of fp32
  * (exactly one of dense / dense16 is set). */
 typedef struct rv_brick_job {
     const float* sparse;
-    const uint32_t* plan;
+    const void* plan;
     float* dense;
     uint16_t* dense16;
     uint32_t depth;
@@ -177,6 +186,25 @@ int rv_clear_viewport(uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, rv_str
  * reference's normalized, linear, clamped 1-D texture. */
 int rv_render_block_argb(const rv_block_params* params, const float* d_brick, const float* d_tf,
                          uint32_t tf_n, uint32_t* d_viewport, rv_stream_t stream);
+
+/* Adds to *d_count (a device uint64) the number of samples the block's rays
+ * take without early termination: the sum over its view rectangle of
+ * (sampleCount - 1) (DenseFieldRenderer_cu.cu:171-175, renderArgb.hpp:40).
+ * Measurement helper: no reference counterpart. */
+int rv_block_sample_count(const rv_block_params* params, uint64_t* d_count, rv_stream_t stream);
+
+/* A whole worker share in one launch: blocks given back to front (params
+ * from rv_block_params_setup, bricks as device pointers), each ray-cast into
+ * the viewport with the same per-pixel operation as rv_render_block_argb, in
+ * that order -- the result equals one rv_render_block_argb call per block,
+ * bit for bit.  It replaces the per-block renderDenseField loop of
+ * VsWorker_cu::renderScenePart (VsWorker_cu.cpp:269-277).  d_workspace:
+ * caller-owned device scratch of rv_share_workspace_bytes(nblocks) bytes;
+ * the call synchronizes the stream once (after staging the block table). */
+uint64_t rv_share_workspace_bytes(uint32_t nblocks);
+int rv_render_share_argb(const rv_block_params* params, const float* const* bricks,
+                         uint32_t nblocks, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
+                         uint32_t vp_w, uint32_t vp_h, void* d_workspace, rv_stream_t stream);
 
 /* Piecewise-linear colour table from control points.  Replaces
  * s3vs::makeColorTransferVector (ColorTransferFunction.hpp:27-69).
@@ -262,7 +290,7 @@ typedef struct rv_frame_block {
     uint32_t index[3];       /* block index at that level */
     uint32_t depth;          /* brick depth */
     const float* sparse;     /* sparse values (for per-frame interpolation), may be NULL */
-    const uint32_t* plan;    /* gather plan (rv_interp_plan_build), may be NULL */
+    const void* plan;        /* gather plan (rv_interp_plan_build), may be NULL */
     float* brick;            /* fp32 brick, or */
     uint16_t* brick16;       /* fp16 brick; one kind for all blocks of a plan */
 } rv_frame_block;
@@ -356,6 +384,10 @@ int rv_worker_render_scene_part(rv_worker* w, uint32_t level, const uint32_t box
  * frame: vp_w*vp_h ARGB. */
 int rv_worker_compose_frame(rv_worker* w, uint32_t level, uint32_t split_count, uint32_t* frame,
                             const volatile int* cancel);
+/* fused != 0 (default): one batched interpolation + one share launch per
+ * renderScenePart (bit-identical to the per-block loop); 0: one
+ * interpolation + one ray-cast launch per block, like the reference. */
+int rv_worker_set_fused(rv_worker* w, int fused);
 void rv_worker_destroy(rv_worker* w);
 
 #ifdef __cplusplus

@@ -72,9 +72,13 @@ _SIGS = {
                                  C.POINTER(C.c_uint64)]),
     "rv_node_ordinals": (C.c_int, [_vp, C.c_uint64, C.c_uint32, _vp, _vp]),
     "rv_brick_nodes": (C.c_uint64, [C.c_uint32]),
-    "rv_interp_plan_build": (C.c_int, [_vp, C.c_uint64, C.c_uint32, _vp, _vp, _vp]),
+    "rv_interp_plan_bytes": (C.c_uint64, [_u16p, C.c_uint64, C.c_uint32]),
+    "rv_interp_plan_encode": (C.c_int, [_u16p, C.c_uint64, C.c_uint32, _vp, C.c_uint64]),
+    "rv_interp_plan_build": (C.c_int, [_u16p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_uint64)]),
+    "rv_interp_plan_free": (C.c_int, [_vp]),
     "rv_interp_brick_planned": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
-    "rv_interp_brick": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint32, _vp, _vp, _vp]),
+    "rv_interp_brick": (C.c_int, [_vp, _u16p, C.c_uint64, C.c_uint32, _vp, _vp]),
     "rv_interp_batch_create": (C.c_int, [C.POINTER(BrickJob), C.c_uint32, C.POINTER(C.c_void_p)]),
     "rv_interp_batch_run": (C.c_int, [_vp, _vp]),
     "rv_interp_batch_info": (C.c_int, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
@@ -84,6 +88,10 @@ _SIGS = {
                                         C.c_float, C.POINTER(BlockParams), C.POINTER(C.c_int)]),
     "rv_clear_viewport": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _vp]),
     "rv_render_block_argb": (C.c_int, [C.POINTER(BlockParams), _vp, _vp, C.c_uint32, _vp, _vp]),
+    "rv_block_sample_count": (C.c_int, [C.POINTER(BlockParams), _vp, _vp]),
+    "rv_share_workspace_bytes": (C.c_uint64, [C.c_uint32]),
+    "rv_render_share_argb": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p), C.c_uint32, _vp,
+                                       C.c_uint32, _vp, C.c_uint32, C.c_uint32, _vp, _vp]),
     "rv_color_transfer_vector": (C.c_int, [_f32p, _f32p, C.c_uint32, C.c_uint32, _f32p]),
     "rv_back_to_front": (C.c_int, [_u32p, _u32p, _f32p, _f32p, _f32p, _u32p, C.c_uint64,
                                    C.POINTER(C.c_uint64)]),
@@ -114,6 +122,7 @@ _SIGS = {
     "rv_worker_render_scene_part": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_int),
                                               C.POINTER(C.c_int32), _u32p, C.c_uint64]),
     "rv_worker_compose_frame": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _u32p, C.POINTER(C.c_int)]),
+    "rv_worker_set_fused": (C.c_int, [_vp, C.c_int]),
     "rv_worker_destroy": (None, [_vp]),
 }
 

@@ -115,9 +115,27 @@ def node_ordinals(d_n2i, depth, d_out, stream=None):
     check(lib().rv_node_ordinals(ptr(d_n2i), d_n2i.shape[0], depth, ptr(d_out), stream_ptr(stream)))
 
 
-def interp_plan_build(d_n2i, depth, d_plan, d_bad=None, stream=None):
-    check(lib().rv_interp_plan_build(ptr(d_n2i), d_n2i.shape[0], depth, ptr(d_plan),
-                                     ptr(d_bad) if d_bad is not None else None, stream_ptr(stream)))
+def interp_plan_bytes(n2i, depth):
+    n2i = _np(n2i, np.uint16)
+    return int(lib().rv_interp_plan_bytes(_p(n2i, C.c_uint16), n2i.shape[0], depth))
+
+
+def interp_plan_encode(n2i, depth):
+    """Host-encoded gather plan (numpy uint8)."""
+    n2i = _np(n2i, np.uint16)
+    nb = interp_plan_bytes(n2i, depth)
+    if nb == 0:
+        check(1)
+    out = np.zeros(nb, dtype=np.uint8)
+    check(lib().rv_interp_plan_encode(_p(n2i, C.c_uint16), n2i.shape[0], depth,
+                                      out.ctypes.data_as(C.c_void_p), nb))
+    return out
+
+
+def interp_plan(n2i, depth):
+    """Gather plan of a subtree as a torch uint8 device tensor."""
+    import torch
+    return torch.from_numpy(interp_plan_encode(n2i, depth)).cuda()
 
 
 def interp_brick_planned(d_sparse, d_plan, depth, d_dense, stream=None):
@@ -125,9 +143,10 @@ def interp_brick_planned(d_sparse, d_plan, depth, d_dense, stream=None):
                                         stream_ptr(stream)))
 
 
-def interp_brick(d_sparse, d_n2i, depth, d_dense, d_ws=None, stream=None):
-    check(lib().rv_interp_brick(ptr(d_sparse), ptr(d_n2i), d_n2i.shape[0], depth, ptr(d_dense),
-                                ptr(d_ws) if d_ws is not None else None, stream_ptr(stream)))
+def interp_brick(d_sparse, n2i, depth, d_dense, stream=None):
+    n2i = _np(n2i, np.uint16)
+    check(lib().rv_interp_brick(ptr(d_sparse), _p(n2i, C.c_uint16), n2i.shape[0], depth, ptr(d_dense),
+                                stream_ptr(stream)))
 
 
 class InterpBatch:
@@ -178,6 +197,30 @@ def clear_viewport(d_vp, w, h, stream=None):
 def render_block_argb(params, d_brick, d_tf, d_vp, stream=None):
     check(lib().rv_render_block_argb(C.byref(params), ptr(d_brick), ptr(d_tf), d_tf.shape[0],
                                      ptr(d_vp), stream_ptr(stream)))
+
+
+def block_sample_count(params, d_count, stream=None):
+    check(lib().rv_block_sample_count(C.byref(params), ptr(d_count), stream_ptr(stream)))
+
+
+class ShareRenderer:
+    """rv_render_share_argb over a fixed list of (params, brick) in
+    back-to-front order."""
+
+    def __init__(self, items, d_tf, W, H):
+        import torch
+        live = [(p, b) for p, b in items]
+        self.n = len(live)
+        self.params = (BlockParams * max(self.n, 1))(*[p for p, _ in live])
+        self.bricks = (C.c_void_p * max(self.n, 1))(*[b.data_ptr() for _, b in live])
+        self._keep = [b for _, b in live] + [d_tf]
+        self.tf, self.W, self.H = d_tf, W, H
+        nbytes = int(lib().rv_share_workspace_bytes(self.n))
+        self.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+
+    def render(self, d_vp, stream=None):
+        check(lib().rv_render_share_argb(self.params, self.bricks, self.n, ptr(self.tf), self.tf.shape[0],
+                                         ptr(d_vp), self.W, self.H, ptr(self.ws), stream_ptr(stream)))
 
 
 def bound_rect(d_pix, w, h, stream=None):
@@ -309,6 +352,9 @@ class Worker:
         check(rc)
         x, y, w, h = tuple(rect)
         return (x, y, w, h), bits[: w * h].reshape(h, w).copy()
+
+    def set_fused(self, fused):
+        check(lib().rv_worker_set_fused(self.h, int(fused)))
 
     def compose_frame(self, level, split_count):
         out = np.zeros((self.H, self.W), dtype=np.uint32)

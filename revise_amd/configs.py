@@ -1,0 +1,139 @@
+"""Device-resident synthetic workloads of the benchmark configurations.
+
+* uniform(level, depth): every block of a level, each a fully refined
+  depth-`depth` subtree (config "GPU plain drop-in": level 3, depth 6 ->
+  512 x 65^3 fp32, full fill).
+* mixed(nblocks, ...): disjoint blocks of levels 2..6 produced by seeded
+  splitting of the 64 level-2 blocks (splits follow the field gradient),
+  each a depth-6 subtree whose occupied part is a slab covering a seeded
+  fraction `fill` in [5%, 100%] of the block: there the subtree is fully
+  refined and carries field values; elsewhere it is refined only to depth 2
+  and its nodes are empty (1e30 -> NaN after interpolation).  Subtree shapes
+  are drawn from 16 fill levels, so 16 gather plans serve all blocks.
+Sparse values are sampled on the GPU in float64 and rounded to float32.
+"""
+import numpy as np
+
+from . import api, datasets
+
+NO_FIELD = float(np.float32(1e30))
+
+
+class Workload:
+    def __init__(self):
+        self.blocks = []      # dicts: level, index, depth, sparse, plan, brick
+        self.fmin = self.fmax = None
+        self.n2i = {}         # shape key -> host n2i
+        self.plans = {}       # shape key -> device plan
+        self.sparse_nodes = 0
+        self.dense_nodes = 0
+
+
+def _coords(torch, level, index, depth, n2i_dev):
+    scale = 1.0 / 2 ** (level + depth)
+    base = torch.tensor(index, dtype=torch.float64, device=n2i_dev.device) * 2 ** depth
+    return (base[None, :] + n2i_dev) * scale
+
+
+def uniform(torch, field, level=3, depth=6, brick_dtype=None):
+    brick_dtype = brick_dtype or torch.float32
+    w = Workload()
+    n2i = api.subtree_n2i(datasets.full_subtree(depth), depth)
+    w.n2i["full"] = n2i
+    plan = w.plans["full"] = api.interp_plan(n2i, depth)
+    n2i_dev = torch.from_numpy(n2i.astype(np.int64)).cuda().double()
+    m = 2 ** level
+    N, V = n2i.shape[0], api.brick_nodes(depth)
+    sparse = torch.empty((m ** 3, N), dtype=torch.float32, device="cuda")
+    bricks = torch.empty((m ** 3, V), dtype=brick_dtype, device="cuda")
+    b = 0
+    for z in range(m):
+        for y in range(m):
+            for x in range(m):
+                sparse[b] = field.eval_torch(_coords(torch, level, (x, y, z), depth, n2i_dev)).float()
+                w.blocks.append({"level": level, "index": (x, y, z), "depth": depth,
+                                 "sparse": sparse[b], "plan": plan, "brick": bricks[b],
+                                 "shape": "full"})
+                b += 1
+    w.sparse, w.bricks = sparse, bricks
+    w.fmin, w.fmax = np.float32(sparse.min().item()), np.float32(sparse.max().item())
+    w.sparse_nodes, w.dense_nodes = m ** 3 * N, m ** 3 * V
+    return w
+
+
+def slab_subtree(depth, fill, coarse_depth=2):
+    """Subtree refined fully where the block cell lies in z < fill, to
+    coarse_depth elsewhere."""
+    def refine(l, locs):
+        if l < coarse_depth:
+            return np.ones(locs.shape[0], dtype=bool)
+        zlo = locs[:, 2] / 2.0 ** l
+        return zlo < fill
+    return datasets.refined_subtree(depth, refine)
+
+
+def mixed_keys(field, nblocks=2000, seed=0, min_level=2, max_level=6):
+    rng = np.random.default_rng(seed)
+    keys = {(min_level, x, y, z) for z in range(4) for y in range(4) for x in range(4)}
+    grid = np.stack(np.meshgrid(*[np.linspace(0.05, 0.95, 24)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    g = field.grad_norm(grid)
+    gmax = g.max()
+    level_weight = {2: 1.0, 3: 0.5, 4: 0.35, 5: 0.25}
+    while len(keys) < nblocks:
+        cand = sorted(k for k in keys if k[0] < max_level)
+        cen = np.array([[(k[1] + 0.5) / 2 ** k[0], (k[2] + 0.5) / 2 ** k[0], (k[3] + 0.5) / 2 ** k[0]]
+                        for k in cand])
+        lev = np.array([k[0] for k in cand])
+        # split where the field varies; seeded sampling spreads the levels
+        wgt = (field.grad_norm(cen) / gmax + 0.05) * np.array([level_weight.get(int(l), 0.2) for l in lev])
+        take = min(len(cand), max(1, min(16, (nblocks - len(keys) + 6) // 7)))
+        for j in rng.choice(len(cand), size=take, replace=False, p=wgt / wgt.sum()):
+            L, x, y, z = cand[j]
+            keys.discard((L, x, y, z))
+            for k in range(8):
+                keys.add((L + 1, 2 * x + (k & 1), 2 * y + ((k >> 1) & 1), 2 * z + ((k >> 2) & 1)))
+    keys = sorted(keys)
+    while len(keys) > nblocks:  # sparse set: drop a few seeded blocks
+        keys.pop(int(rng.integers(len(keys))))
+    return keys
+
+
+def mixed(torch, field, nblocks=2000, depth=6, seed=0, n_fill=16, brick_dtype=None, keys=None):
+    brick_dtype = brick_dtype or torch.float32
+    rng = np.random.default_rng(seed + 1)
+    w = Workload()
+    keys = keys if keys is not None else mixed_keys(field, nblocks, seed)
+    fills = 0.05 + 0.95 * np.arange(n_fill) / (n_fill - 1)
+    shapes = {}
+    for k, f in enumerate(fills):
+        ch = slab_subtree(depth, f)
+        d = api.subtree_depth(ch, depth)
+        n2i = api.subtree_n2i(ch, d)
+        plan = api.interp_plan(n2i, d)
+        shapes[k] = (d, n2i, torch.from_numpy(n2i.astype(np.int64)).cuda().double(), plan, float(f))
+        w.n2i[k] = n2i
+        w.plans[k] = plan
+    V = api.brick_nodes(depth)
+    bricks = torch.empty((len(keys), V), dtype=brick_dtype, device="cuda")
+    lo, hi = np.inf, -np.inf
+    sparse_all = []
+    for i, (L, x, y, z) in enumerate(keys):
+        k = int(rng.integers(n_fill))
+        d, n2i, n2i_dev, plan, f = shapes[k]
+        u = _coords(torch, L, (x, y, z), d, n2i_dev)
+        v = field.eval_torch(u)
+        # occupied: node inside the slab z <= fill of the block
+        zloc = n2i_dev[:, 2] / 2 ** d
+        occ = zloc <= f + 1e-12
+        lo = min(lo, float(v[occ].min()))
+        hi = max(hi, float(v[occ].max()))
+        sp = torch.where(occ, v, torch.full_like(v, NO_FIELD)).float()
+        sparse_all.append(sp)
+        w.blocks.append({"level": L, "index": (x, y, z), "depth": d, "sparse": sp, "plan": plan,
+                         "brick": bricks[i], "shape": k, "fill": f})
+        w.sparse_nodes += n2i.shape[0]
+        w.dense_nodes += V
+    w.bricks = bricks
+    w.sparse = sparse_all
+    w.fmin, w.fmax = np.float32(lo), np.float32(hi)
+    return w

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
                     hi = mid;
             }
             const Job job = a.jobs[lo];
-            interp_tile(job, item - job.tile_start, A, lat);
+            interp_tile<kT>(job, item - job.tile_start, A, lat);
             __threadfence();
             __syncthreads();
             if (threadIdx.x == 0)
@@ -194,6 +194,71 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         }
         render_tile<BrickT>(a, item - a.n_interp, tfs, list, wave_cnt);
     }
+}
+
+// Worker share in one launch: the blocks (back to front) of one share are
+// ray-cast into the viewport with exactly the per-block operation of the
+// per-block renderer -- pixel = blendAbove(pixel, segment) through the 8-bit
+// ARGB round trip -- so the result equals one launch per block in that order,
+// bit for bit.  Tiles of 16 x 16 pixels; the blocks of a tile are compacted
+// from the view rectangles 256 at a time.
+constexpr uint32_t kShareT = 256;
+
+__global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* __restrict__ params,
+                                                        const uint4* __restrict__ rects,
+                                                        const void* const* __restrict__ bricks,
+                                                        uint32_t nblocks, const float4* __restrict__ tf,
+                                                        uint32_t tf_n, uint32_t* __restrict__ viewport,
+                                                        uint32_t W, uint32_t H)
+{
+    __shared__ float4 tfs[kMaxTf];
+    __shared__ uint32_t list[kShareT];
+    __shared__ uint32_t wave_cnt[kShareT / 64u];
+    for (uint32_t i = threadIdx.x; i < tf_n; i += kShareT)
+        tfs[i] = tf[i];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t x0 = blockIdx.x * 16u, y0 = blockIdx.y * 16u;
+    const uint32_t x1 = min(x0 + 16u, W), y1 = min(y0 + 16u, H);
+    const uint32_t vx = x0 + (tid & 15u), vy = y0 + (tid >> 4);
+    const bool inside = vx < W && vy < H;
+    const uint32_t pix = inside ? (H - 1u - vy) * W + vx : 0u;
+    uint32_t value = inside ? viewport[pix] : 0u;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
+    __syncthreads();
+    for (uint32_t c = 0; c < nblocks; c += kShareT) {
+        const uint32_t i = c + tid;
+        bool hit = false;
+        if (i < nblocks) {
+            const uint4 r = rects[i];
+            hit = r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0;
+        }
+        const uint64_t ball = __ballot(hit);
+        if (lane == 0)
+            wave_cnt[wave] = (uint32_t)__popcll(ball);
+        __syncthreads();
+        uint32_t off = 0, total = 0;
+        for (uint32_t w = 0; w < kShareT / 64u; ++w) {
+            off += w < wave ? wave_cnt[w] : 0u;
+            total += wave_cnt[w];
+        }
+        if (hit)
+            list[off + (uint32_t)__popcll(ball & ((1ull << lane) - 1ull))] = i;
+        __syncthreads();
+        for (uint32_t k = 0; k < total; ++k) {
+            const uint32_t blk = uniform(list[k]);
+            const rv_block_params p = params[blk];
+            if (inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
+                vy < p.view_y + p.view_h) {
+                const RaySeg r = setup_ray(p, vx, vy);
+                if (r.hit)
+                    value = blend_above_u32(
+                        value, march(p, reinterpret_cast<const float*>(bricks[blk]), tfs, (int)tf_n, r));
+            }
+        }
+        __syncthreads();
+    }
+    if (inside)
+        viewport[pix] = value;
 }
 
 __global__ void frame_reset_kernel(uint32_t* counters, uint32_t n)
@@ -328,7 +393,8 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
         RV_REQUIRE((b.brick16 != nullptr) == fp16 && (b.brick != nullptr) != fp16,
                    "rv_frame_plan_create: all blocks must use the same brick type");
         const Geom g = geom_of(b.depth);
-        jobs[i] = Job{b.sparse, b.plan, b.brick, reinterpret_cast<__half*>(b.brick16), b.depth,
+        jobs[i] = Job{b.sparse, static_cast<const uint8_t*>(b.plan), b.brick,
+                      reinterpret_cast<__half*>(b.brick16), b.depth,
                       (uint32_t)tiles};
         need[i] = g.ntiles;
         bricks[i] = fp16 ? (const void*)b.brick16 : (const void*)b.brick;
@@ -584,3 +650,49 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
 }
 
 extern "C" void rv_frame_plan_destroy(rv_frame_plan* P) { delete P; }
+
+extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
+{
+    return (uint64_t)nblocks * (sizeof(rv_block_params) + sizeof(uint4) + sizeof(void*)) + 256;
+}
+
+extern "C" int rv_render_share_argb(const rv_block_params* params, const float* const* bricks,
+                                    uint32_t nblocks, const float* d_tf, uint32_t tf_n,
+                                    uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                    void* d_workspace, rv_stream_t stream)
+{
+    RV_TRY
+    if (nblocks == 0)
+        return RV_OK;
+    RV_REQUIRE(params && bricks && d_tf && d_viewport && d_workspace,
+               "rv_render_share_argb: null pointer");
+    RV_REQUIRE(tf_n >= 1 && tf_n <= kMaxTf, "rv_render_share_argb: tf_n must be in [1, 1024]");
+    std::vector<uint8_t> host(rv_share_workspace_bytes(nblocks));
+    auto* hp = reinterpret_cast<rv_block_params*>(host.data());
+    auto* hr = reinterpret_cast<uint4*>(host.data() + sizeof(rv_block_params) * nblocks);
+    auto* hb = reinterpret_cast<const void**>(host.data() + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    for (uint32_t i = 0; i < nblocks; ++i) {
+        const rv_block_params& p = params[i];
+        RV_REQUIRE(p.vp_w == vp_w && p.vp_h == vp_h, "rv_render_share_argb: viewport mismatch");
+        RV_REQUIRE(p.tex_depth <= 10 && p.nodes == (1u << p.tex_depth) + 1u && p.fmax > p.fmin,
+                   "rv_render_share_argb: inconsistent block parameters");
+        RV_REQUIRE((uint64_t)p.view_x + p.view_w <= vp_w && (uint64_t)p.view_y + p.view_h <= vp_h,
+                   "rv_render_share_argb: view rectangle outside the viewport");
+        RV_REQUIRE(bricks[i] || p.view_w == 0 || p.view_h == 0, "rv_render_share_argb: null brick");
+        hp[i] = p;
+        hr[i] = make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h);
+        hb[i] = bricks[i];
+    }
+    uint8_t* d = static_cast<uint8_t*>(d_workspace);
+    RV_HIP(hipMemcpyAsync(d, host.data(), host.size() - 256, hipMemcpyHostToDevice, S(stream)));
+    // the host staging buffer must outlive the copy
+    RV_HIP(hipStreamSynchronize(S(stream)));
+    hipLaunchKernelGGL(share_kernel, dim3((vp_w + 15) / 16, (vp_h + 15) / 16), dim3(kShareT), 0,
+                       S(stream), reinterpret_cast<const rv_block_params*>(d),
+                       reinterpret_cast<const uint4*>(d + sizeof(rv_block_params) * nblocks),
+                       reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks),
+                       nblocks, reinterpret_cast<const float4*>(d_tf), tf_n, d_viewport, vp_w, vp_h);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+    RV_CATCH
+}

@@ -11,15 +11,18 @@
 // Decomposition.  A midpoint of a level-l cube depends only on values inside
 // that cube, and level-l midpoints never touch coarser lattice nodes.  So the
 // brick is cut into tiles of X x T x T cells (T = min(2^d, 16), X = min(2^d, 64),
-// x spans whole rows of the brick):
-//   1. every tile workgroup first completes the coarse levels (cubes larger
-//      than T) on the stride-T lattice of the whole brick -- at most 17^3
-//      nodes, in LDS -- and keeps the lattice values at its own corners;
-//   2. it gathers its own (X+1)(T+1)^2 nodes through the gather plan
-//      (plan[node] = sparse index or none) into LDS, sets its lattice nodes,
-//      and completes the remaining levels in LDS, three barrier-separated
-//      phases per level (edges, faces, cells);
-//   3. it stores the nodes it owns (half-open tile box, plus the brick's far
+// x spans whole rows of the brick), one workgroup each:
+//   1. a tile with a node not present in the sparse data first completes the
+//      coarse levels (cubes larger than T) on the stride-T lattice of the
+//      whole brick -- at most 17^3 nodes, in LDS -- and keeps the lattice
+//      values at its own corners;
+//   2. it scatters its sparse nodes into LDS: the gather plan (plan.cpp) lists
+//      them per tile in increasing sparse index with their LDS offsets, so
+//      every global read is coalesced and the permutation happens in LDS;
+//   3. it completes the remaining levels in LDS, three barrier-separated
+//      phases per level (edges, faces, cells); a tile whose every node is a
+//      sparse node skips 1 and 3;
+//   4. it stores the nodes it owns (half-open tile box, plus the brick's far
 //      faces), converting 1e30 to NaN.
 // Nodes on faces shared by two tiles are computed identically by both (their
 // operands lie on the shared face), so the tiles are independent.
@@ -33,24 +36,31 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <memory>
 #include <vector>
+
+extern "C" uint64_t rv_interp_plan_bytes(const uint16_t* n2i, uint64_t n, uint32_t depth);
+extern "C" int rv_interp_plan_encode(const uint16_t* n2i, uint64_t n, uint32_t depth, void* plan,
+                                     uint64_t bytes);
 
 namespace rv {
 
 namespace {
 
-__global__ __launch_bounds__(kThreads) void interp_single_kernel(Job job, uint32_t ntiles)
+constexpr uint32_t kInterpThreads = 512;
+
+__global__ __launch_bounds__(kInterpThreads) void interp_single_kernel(Job job, uint32_t ntiles)
 {
     __shared__ float A[kTileFloats];
     __shared__ float lat[32];
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= ntiles)
         return;
-    interp_tile(job, t, A, lat);
+    interp_tile<kInterpThreads>(job, t, A, lat);
 }
 
-__global__ __launch_bounds__(kThreads) void interp_batch_kernel(const Job* __restrict__ jobs,
-                                                                uint32_t njobs, uint32_t ntiles)
+__global__ __launch_bounds__(kInterpThreads) void interp_batch_kernel(const Job* __restrict__ jobs,
+                                                                      uint32_t njobs, uint32_t ntiles)
 {
     __shared__ float A[kTileFloats];
     __shared__ float lat[32];
@@ -67,7 +77,7 @@ __global__ __launch_bounds__(kThreads) void interp_batch_kernel(const Job* __res
             hi = mid;
     }
     const Job job = jobs[lo];
-    interp_tile(job, t - job.tile_start, A, lat);
+    interp_tile<kInterpThreads>(job, t - job.tile_start, A, lat);
 }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint64_t n, uint32_t value)
@@ -82,22 +92,6 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint64_t n, uint32_t
     const uint64_t i = tail + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i < n)
         dst[i] = value;
-}
-
-__global__ void plan_scatter_kernel(const uint16_t* __restrict__ n2i, uint64_t n, uint32_t depth,
-                                    uint32_t* __restrict__ plan, uint32_t* bad)
-{
-    const uint32_t npe = (1u << depth) + 1u;
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t x = n2i[3 * i], y = n2i[3 * i + 1], z = n2i[3 * i + 2];
-        if (x >= npe || y >= npe || z >= npe) {
-            if (bad)
-                atomicAdd(bad, 1u);
-            continue;
-        }
-        plan[ordinal3(x, y, z, depth)] = (uint32_t)i;
-    }
 }
 
 __global__ void ordinals_kernel(const uint16_t* __restrict__ n2i, uint64_t n, uint32_t depth,
@@ -164,55 +158,84 @@ extern "C" int rv_node_ordinals(const uint16_t* d_n2i, uint64_t n, uint32_t dept
     return RV_OK;
 }
 
-extern "C" int rv_interp_plan_build(const uint16_t* d_n2i, uint64_t n, uint32_t depth,
-                                    uint32_t* d_plan, uint32_t* d_bad_count, rv_stream_t stream)
+extern "C" int rv_interp_plan_build(const uint16_t* n2i, uint64_t n, uint32_t depth, void** d_plan,
+                                    uint64_t* bytes_out)
 {
+    RV_TRY
+    RV_REQUIRE(d_plan, "rv_interp_plan_build: null output");
+    *d_plan = nullptr;
     RV_REQUIRE(depth <= kMaxDepth, "rv_interp_plan_build: depth must be <= 8");
-    RV_REQUIRE(d_plan, "rv_interp_plan_build: null plan");
-    RV_REQUIRE(n == 0 || d_n2i, "rv_interp_plan_build: null n2i");
-    RV_REQUIRE(n <= 0xFFFFFFFEull, "rv_interp_plan_build: too many nodes");
-    const uint64_t V = rv_brick_nodes(depth);
-    int rc = fill_u32(d_plan, V, kNoIndex, S(stream));
+    const uint64_t bytes = rv_interp_plan_bytes(n2i, n, depth);
+    if (!bytes)
+        return RV_ERR_INVALID; // message set by the encoder
+    std::vector<uint8_t> host(bytes);
+    int rc = rv_interp_plan_encode(n2i, n, depth, host.data(), bytes);
     if (rc)
         return rc;
-    if (n) {
-        hipLaunchKernelGGL(plan_scatter_kernel, dim3(grid_for(n, 256)), dim3(256), 0, S(stream),
-                           d_n2i, n, depth, d_plan, d_bad_count);
-        RV_LAUNCH_CHECK();
+    void* d = nullptr;
+    RV_HIP(hipMalloc(&d, bytes));
+    hipError_t e = hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
     }
+    *d_plan = d;
+    if (bytes_out)
+        *bytes_out = bytes;
+    return RV_OK;
+    RV_CATCH
+}
+
+extern "C" int rv_interp_plan_free(void* d_plan)
+{
+    if (d_plan)
+        RV_HIP(hipFree(d_plan));
     return RV_OK;
 }
 
-extern "C" int rv_interp_brick_planned(const float* d_sparse, const uint32_t* d_plan,
-                                       uint32_t depth, float* d_dense, rv_stream_t stream)
+extern "C" int rv_interp_brick_planned(const float* d_sparse, const void* d_plan, uint32_t depth,
+                                       float* d_dense, rv_stream_t stream)
 {
     RV_REQUIRE(depth <= kMaxDepth, "rv_interp_brick_planned: depth must be <= 8");
     RV_REQUIRE(d_plan && d_dense, "rv_interp_brick_planned: null pointer");
-    Job job{d_sparse, d_plan, d_dense, nullptr, depth, 0};
+    Job job{d_sparse, static_cast<const uint8_t*>(d_plan), d_dense, nullptr, depth, 0};
     const Geom g = geom_of(depth);
-    hipLaunchKernelGGL(interp_single_kernel, dim3(round8(g.ntiles)), dim3(kThreads), 0, S(stream),
-                       job, g.ntiles);
+    hipLaunchKernelGGL(interp_single_kernel, dim3(round8(g.ntiles)), dim3(kInterpThreads), 0,
+                       S(stream), job, g.ntiles);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }
 
-extern "C" int rv_interp_brick(const float* d_sparse, const uint16_t* d_n2i, uint64_t n,
-                               uint32_t depth, float* d_dense, uint32_t* d_workspace,
-                               rv_stream_t stream)
+extern "C" int rv_interp_brick(const float* d_sparse, const uint16_t* n2i, uint64_t n,
+                               uint32_t depth, float* d_dense, rv_stream_t stream)
 {
+    RV_TRY
     RV_REQUIRE(depth <= kMaxDepth, "rv_interp_brick: depth must be <= 8");
-    uint32_t* ws = d_workspace;
-    if (!ws)
-        RV_HIP(hipMallocAsync((void**)&ws, rv_brick_nodes(depth) * 4, S(stream)));
-    int rc = rv_interp_plan_build(d_n2i, n, depth, ws, nullptr, stream);
-    if (!rc)
-        rc = rv_interp_brick_planned(d_sparse, ws, depth, d_dense, stream);
-    if (!d_workspace) {
-        hipError_t e = hipFreeAsync(ws, S(stream));
-        if (!rc && e != hipSuccess)
-            return hip_fail(e, "hipFreeAsync", __FILE__, __LINE__);
-    }
-    return rc;
+    const uint64_t bytes = rv_interp_plan_bytes(n2i, n, depth);
+    if (!bytes)
+        return RV_ERR_INVALID;
+    std::vector<uint8_t> host(bytes);
+    int rc = rv_interp_plan_encode(n2i, n, depth, host.data(), bytes);
+    if (rc)
+        return rc;
+    void* d = nullptr;
+    RV_HIP(hipMallocAsync(&d, bytes, S(stream)));
+    hipError_t e = hipMemcpyAsync(d, host.data(), bytes, hipMemcpyHostToDevice, S(stream));
+    if (e == hipSuccess)
+        rc = rv_interp_brick_planned(d_sparse, d, depth, d_dense, stream);
+    const hipError_t e2 = hipFreeAsync(d, S(stream));
+    // the host staging buffer must outlive the copy
+    const hipError_t e3 = hipStreamSynchronize(S(stream));
+    if (e != hipSuccess)
+        return hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+    if (rc)
+        return rc;
+    if (e2 != hipSuccess)
+        return hip_fail(e2, "hipFreeAsync", __FILE__, __LINE__);
+    if (e3 != hipSuccess)
+        return hip_fail(e3, "hipStreamSynchronize", __FILE__, __LINE__);
+    return RV_OK;
+    RV_CATCH
 }
 
 extern "C" int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count,
@@ -222,38 +245,33 @@ extern "C" int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count,
     RV_REQUIRE(out, "rv_interp_batch_create: null out");
     *out = nullptr;
     RV_REQUIRE(count == 0 || jobs, "rv_interp_batch_create: null jobs");
+    auto b = std::make_unique<rv_interp_batch>();
     std::vector<Job> host(count);
-    uint64_t tiles = 0, nodes = 0;
+    uint64_t tiles = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const rv_brick_job& j = jobs[i];
         RV_REQUIRE(j.depth <= kMaxDepth, "rv_interp_batch_create: depth must be <= 8");
         RV_REQUIRE(j.plan, "rv_interp_batch_create: null plan");
         RV_REQUIRE((j.dense != nullptr) != (j.dense16 != nullptr),
                    "rv_interp_batch_create: exactly one of dense/dense16 must be set");
-        host[i] = Job{j.sparse, j.plan, j.dense, reinterpret_cast<__half*>(j.dense16), j.depth,
-                      (uint32_t)tiles};
+        host[i] = Job{j.sparse, static_cast<const uint8_t*>(j.plan), j.dense,
+                      reinterpret_cast<__half*>(j.dense16), j.depth, (uint32_t)tiles};
         tiles += geom_of(j.depth).ntiles;
-        nodes += rv_brick_nodes(j.depth);
+        b->dense_nodes += rv_brick_nodes(j.depth);
     }
     RV_REQUIRE(tiles < 0x7FFFFFF0ull, "rv_interp_batch_create: too many tiles");
-    auto* b = new rv_interp_batch;
     b->njobs = count;
     b->ntiles = (uint32_t)tiles;
-    b->dense_nodes = nodes;
     if (count) {
-        hipError_t e = hipMalloc((void**)&b->d_jobs, sizeof(Job) * count);
-        if (e != hipSuccess) {
-            delete b;
-            return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
-        }
-        e = hipMemcpy(b->d_jobs, host.data(), sizeof(Job) * count, hipMemcpyHostToDevice);
+        RV_HIP(hipMalloc((void**)&b->d_jobs, sizeof(Job) * count));
+        const hipError_t e =
+            hipMemcpy(b->d_jobs, host.data(), sizeof(Job) * count, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             (void)hipFree(b->d_jobs);
-            delete b;
             return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
         }
     }
-    *out = b;
+    *out = b.release();
     return RV_OK;
     RV_CATCH
 }
@@ -263,8 +281,8 @@ extern "C" int rv_interp_batch_run(rv_interp_batch* b, rv_stream_t stream)
     RV_REQUIRE(b, "rv_interp_batch_run: null batch");
     if (b->ntiles == 0)
         return RV_OK;
-    hipLaunchKernelGGL(interp_batch_kernel, dim3(round8(b->ntiles)), dim3(kThreads), 0, S(stream),
-                       b->d_jobs, b->njobs, b->ntiles);
+    hipLaunchKernelGGL(interp_batch_kernel, dim3(round8(b->ntiles)), dim3(kInterpThreads), 0,
+                       S(stream), b->d_jobs, b->njobs, b->ntiles);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }

@@ -1,5 +1,6 @@
 // Device code of the brick interpolation tile, shared by the interpolation
-// kernels and the persistent frame kernel.  See interp.hip for the method.
+// kernels and the persistent frame kernel.  See interp.hip for the method and
+// plan.cpp for the gather plan layout.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -9,18 +10,20 @@
 
 namespace rv {
 
-
 constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
 constexpr float kNoField = 1e30f;
-constexpr uint32_t kThreads = 512;
 constexpr uint32_t kMaxDepth = 8;
-// Largest tile: 65 x 17 x 17 floats.  The coarse lattice (<= 17^3) reuses it.
+// LDS floats of the largest tile, 65 x 17 x 17 nodes; the coarse lattice of
+// the whole brick (at most 17^3 nodes) reuses the same array.
 constexpr uint32_t kTileFloats = 65u * 17u * 17u;
+constexpr uint32_t kPlanMagic = 0x31504e52u;
 
 struct Geom {
     uint32_t d, n, t, T, X, tiles_x, tiles_yz, ntiles, m;
 };
 
+// Tiles of at most 64 x 16 x 16 cells (x spans whole brick rows up to 64
+// cells); m = lattice cells per axis of the stride-T coarse lattice.
 __host__ __device__ inline Geom geom_of(uint32_t d)
 {
     Geom g;
@@ -32,7 +35,7 @@ __host__ __device__ inline Geom geom_of(uint32_t d)
     g.tiles_x = (1u << d) / g.X;
     g.tiles_yz = (1u << d) / g.T;
     g.ntiles = g.tiles_x * g.tiles_yz * g.tiles_yz;
-    g.m = (1u << d) >> g.t; // lattice cells per axis
+    g.m = (1u << d) >> g.t;
     return g;
 }
 
@@ -47,21 +50,24 @@ __device__ __forceinline__ bool is_nan(float v) { return v != v; }
 
 struct Job {
     const float* sparse;
-    const uint32_t* plan;
+    const uint8_t* plan;  // encoded gather plan (plan.cpp)
     float* dense;
     __half* dense16;
     uint32_t depth;
-    uint32_t tile_start;
+    uint32_t tile_start;  // first global tile of this job (batched launches)
 };
 
 // Complete the levels with cube sizes s_start, s_start/2, ..., 2 (in box units)
 // on a box of (X+1) x (T+1) x (T+1) values at A (x fastest).  Three phases per
-// level: nodes with one, two, three coordinates an odd multiple of h = s/2.
+// level: nodes with one, two, three coordinates an odd multiple of h = s/2,
+// each the mean of its neighbours at +-h along its odd axes, in the operand
+// order of DenseFieldInterpolator.hpp:257-370.
+template <uint32_t NT>
 __device__ inline void refine_box(float* A, uint32_t X, uint32_t T, uint32_t s_start)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t nwaves = kThreads / 64u;
+    const uint32_t nwaves = NT / 64u;
     const uint32_t rowlen = X + 1u;
     const uint32_t plane = rowlen * (T + 1u);
     for (uint32_t s = s_start; s >= 2u; s >>= 1) {
@@ -116,12 +122,32 @@ __device__ inline void refine_box(float* A, uint32_t X, uint32_t T, uint32_t s_s
     }
 }
 
-__device__ __forceinline__ float gather(const Job& job, uint32_t ord)
+struct PlanView {
+    uint32_t depth, ntiles, nlat, entries;
+    const uint32_t* tile_start;
+    const uint32_t* lat;
+    const uint32_t* idx;
+    const uint16_t* lds;
+};
+
+__device__ __forceinline__ PlanView plan_view(const uint8_t* P)
 {
-    const uint32_t s = __builtin_nontemporal_load(job.plan + ord);
-    return s == kNoIndex ? __builtin_nanf("") : job.sparse[s];
+    const uint32_t* h = reinterpret_cast<const uint32_t*>(P);
+    PlanView v;
+    v.depth = h[1];
+    v.ntiles = h[2];
+    v.nlat = h[3];
+    v.entries = h[4];
+    v.tile_start = h + 8;
+    v.lat = reinterpret_cast<const uint32_t*>(P + h[5]);
+    v.idx = reinterpret_cast<const uint32_t*>(P + h[6]);
+    v.lds = reinterpret_cast<const uint16_t*>(P + h[7]);
+    return v;
 }
 
+// One tile of a brick with NT threads (a multiple of 64); A holds
+// kTileFloats floats, lat 32 floats.
+template <uint32_t NT>
 __device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A, float* lat)
 {
     const Geom g = geom_of(job.depth);
@@ -132,52 +158,92 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A
     const uint32_t x0 = tx * X, y0 = ty * T, z0 = tz * T;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t nwaves = kThreads / 64u;
+    const uint32_t nwaves = NT / 64u;
     const uint32_t nlat_x = X / T + 1u;
-
-    // 1. coarse levels on the stride-T lattice of the whole brick
-    if (g.m >= 2u) {
-        const uint32_t mp = g.m + 1u;
-        const uint32_t total = mp * mp * mp;
-        for (uint32_t q = threadIdx.x; q < total; q += kThreads) {
-            const uint32_t i = q % mp, j = (q / mp) % mp, k = q / (mp * mp);
-            A[q] = gather(job, ordinal3(i * T, j * T, k * T, d));
-        }
-        __syncthreads();
-        refine_box(A, g.m, g.m, g.m);
-        const uint32_t nl = nlat_x * 4u;
-        if (threadIdx.x < nl) {
-            const uint32_t li = threadIdx.x % nlat_x, c = threadIdx.x / nlat_x;
-            const uint32_t I = x0 / T + li, J = y0 / T + (c & 1u), K = z0 / T + (c >> 1);
-            lat[threadIdx.x] = A[(K * mp + J) * mp + I];
-        }
-        __syncthreads();
-    }
-
-    // 2. gather the tile
     const uint32_t rowlen = X + 1u;
     const uint32_t tp1 = T + 1u;
-    const uint32_t nrows = tp1 * tp1;
-    for (uint32_t r = wave; r < nrows; r += nwaves) {
-        const uint32_t y = r % tp1, z = r / tp1;
-        const uint32_t base = ordinal3(x0, y0 + y, z0 + z, d);
-        float* dst = A + r * rowlen;
-        for (uint32_t x = lane; x < rowlen; x += 64u)
-            dst[x] = gather(job, base + x);
-    }
-    __syncthreads();
-    if (g.m >= 2u) {
-        const uint32_t nl = nlat_x * 4u;
-        if (threadIdx.x < nl) {
-            const uint32_t li = threadIdx.x % nlat_x, c = threadIdx.x / nlat_x;
-            const uint32_t y = (c & 1u) * T, z = (c >> 1) * T;
-            A[(z * tp1 + y) * rowlen + li * T] = lat[threadIdx.x];
+    const uint32_t tile_nodes = rowlen * tp1 * tp1;
+    const PlanView P = plan_view(job.plan);
+    const uint32_t e0 = P.tile_start[local_tile], e1 = P.tile_start[local_tile + 1];
+
+    // A "full" tile skips the interpolation; if a sparse value in it is NaN
+    // (undefined, so the reference interpolates it) the tile is redone as a
+    // partial one.
+    bool full = e1 - e0 == tile_nodes;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (!full) {
+            // 1. coarse levels on the stride-T lattice of the whole brick
+            if (g.m >= 2u) {
+                const uint32_t mp = g.m + 1u;
+                for (uint32_t q = threadIdx.x; q < P.nlat; q += NT) {
+                    const uint32_t s = P.lat[q];
+                    A[q] = s == kNoIndex ? __builtin_nanf("") : job.sparse[s];
+                }
+                __syncthreads();
+                refine_box<NT>(A, g.m, g.m, g.m);
+                const uint32_t nl = nlat_x * 4u;
+                if (threadIdx.x < nl) {
+                    const uint32_t li = threadIdx.x % nlat_x, c = threadIdx.x / nlat_x;
+                    const uint32_t I = x0 / T + li, J = y0 / T + (c & 1u), K = z0 / T + (c >> 1);
+                    lat[threadIdx.x] = A[(K * mp + J) * mp + I];
+                }
+                __syncthreads();
+            }
+            for (uint32_t q = threadIdx.x; q < tile_nodes; q += NT)
+                A[q] = __builtin_nanf("");
+            __syncthreads();
+        }
+
+        // 2. scatter the tile's sparse nodes into LDS: plan entries and sparse
+        // values are read in increasing sparse index (coalesced), kUnroll per
+        // thread in flight.
+        constexpr uint32_t kUnroll = 8;
+        bool saw_nan = false;
+        for (uint32_t k0 = e0 + threadIdx.x; k0 < e1; k0 += NT * kUnroll) {
+            uint32_t si[kUnroll];
+            uint32_t off[kUnroll];
+#pragma unroll
+            for (uint32_t j = 0; j < kUnroll; ++j) {
+                const uint32_t k = k0 + j * NT;
+                si[j] = kNoIndex;
+                off[j] = 0;
+                if (k < e1) {
+                    si[j] = __builtin_nontemporal_load(P.idx + k);
+                    off[j] = __builtin_nontemporal_load(P.lds + k);
+                }
+            }
+            float v[kUnroll];
+#pragma unroll
+            for (uint32_t j = 0; j < kUnroll; ++j)
+                v[j] = si[j] == kNoIndex ? 0.f : job.sparse[si[j]];
+#pragma unroll
+            for (uint32_t j = 0; j < kUnroll; ++j)
+                if (si[j] != kNoIndex) {
+                    A[off[j]] = v[j];
+                    saw_nan |= is_nan(v[j]);
+                }
+        }
+        if (full && __syncthreads_or(saw_nan)) {
+            full = false;
+            continue;
         }
         __syncthreads();
+        break;
     }
 
-    // 3. finer levels inside the tile
-    refine_box(A, X, T, T);
+    if (!full) {
+        if (g.m >= 2u) {
+            const uint32_t nl = nlat_x * 4u;
+            if (threadIdx.x < nl) {
+                const uint32_t li = threadIdx.x % nlat_x, c = threadIdx.x / nlat_x;
+                const uint32_t y = (c & 1u) * T, z = (c >> 1) * T;
+                A[(z * tp1 + y) * rowlen + li * T] = lat[threadIdx.x];
+            }
+            __syncthreads();
+        }
+        // 3. finer levels inside the tile
+        refine_box<NT>(A, X, T, T);
+    }
 
     // 4. store the owned half-open box (+ far faces of the brick)
     const uint32_t ox = (tx + 1u == g.tiles_x) ? rowlen : X;
@@ -207,6 +273,5 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t bid, uint32_t nblocks)
     const uint32_t per = nblocks >> 3; // nblocks is a multiple of 8
     return (bid & 7u) * per + (bid >> 3);
 }
-
 
 } // namespace rv

@@ -90,6 +90,25 @@ __global__ __launch_bounds__(256) void bound_rect_kernel(const uint32_t* __restr
     }
 }
 
+// S = sum over the view rectangle of (sampleCount - 1) for rays that hit
+// the block: the sample count of renderArgb.hpp:40 without early termination.
+__global__ __launch_bounds__(256) void sample_count_kernel(rv_block_params p,
+                                                           unsigned long long* out)
+{
+    const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+    const uint32_t ox = blockIdx.x * 16u + lx, oy = blockIdx.y * 16u + ly;
+    unsigned long long c = 0;
+    if (ox < p.view_w && oy < p.view_h) {
+        const RaySeg r = setup_ray(p, p.view_x + ox, p.view_y + oy);
+        if (r.hit && r.sample_count > 1)
+            c = (unsigned long long)(r.sample_count - 1);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        c += __shfl_down(c, off);
+    if ((threadIdx.x & 63u) == 0 && c)
+        atomicAdd(out, c);
+}
+
 __global__ void init_rect_kernel(int* out)
 {
     if (threadIdx.x == 0) {
@@ -174,6 +193,20 @@ extern "C" int rv_render_block_argb(const rv_block_params* params, const float* 
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
     hipLaunchKernelGGL(render_block_kernel, grid, dim3(256), tf_n * sizeof(float4), S(stream), p,
                        d_brick, d_tf, tf_n, d_viewport);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
+
+extern "C" int rv_block_sample_count(const rv_block_params* params, uint64_t* d_count,
+                                     rv_stream_t stream)
+{
+    RV_REQUIRE(params && d_count, "rv_block_sample_count: null pointer");
+    const rv_block_params& p = *params;
+    if (p.view_w == 0 || p.view_h == 0)
+        return RV_OK;
+    dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
+    hipLaunchKernelGGL(sample_count_kernel, grid, dim3(256), 0, S(stream), p,
+                       reinterpret_cast<unsigned long long*>(d_count));
     RV_LAUNCH_CHECK();
     return RV_OK;
 }

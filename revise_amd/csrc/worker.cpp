@@ -36,16 +36,13 @@ struct FieldServiceHip::Subtree {
     unsigned depth = 0;
     uint64_t n = 0;
     float* sparse = nullptr;
-    uint16_t* n2i = nullptr;
-    uint32_t* plan = nullptr; // built on first use (cached like the reference's n2i)
+    void* plan = nullptr; // gather plan, encoded from the n2i once per subtree
     ~Subtree()
     {
         if (sparse)
             (void)hipFree(sparse);
-        if (n2i)
-            (void)hipFree(n2i);
         if (plan)
-            (void)hipFree(plan);
+            (void)rv_interp_plan_free(plan);
     }
 };
 
@@ -63,10 +60,10 @@ void FieldServiceHip::addSubtree(unsigned level, const std::array<uint32_t, 3>& 
     if (n) {
         hipCheck(hipMalloc((void**)&st->sparse, 4 * n), "hipMalloc");
         hipCheck(hipMemcpy(st->sparse, sparse, 4 * n, hipMemcpyHostToDevice), "hipMemcpy");
-        hipCheck(hipMalloc((void**)&st->n2i, 6 * n), "hipMalloc");
-        hipCheck(hipMemcpy(st->n2i, n2i, 6 * n, hipMemcpyHostToDevice), "hipMemcpy");
     }
+    throwIfFailed(rv_interp_plan_build(n2i, n, depth, &st->plan, nullptr));
     m_subtrees[{level, index[0], index[1], index[2]}] = std::move(st);
+    ++m_version;
 }
 
 void FieldServiceHip::setFieldRange(float fmin, float fmax)
@@ -88,17 +85,29 @@ unsigned FieldServiceHip::subtreeDepth(unsigned level, const std::array<uint32_t
     return it->second->depth;
 }
 
+rv_brick_job FieldServiceHip::job(unsigned level, const std::array<uint32_t, 3>& i, float* d_dense,
+                                  rv_stream_t stream)
+{
+    auto it = m_subtrees.find({level, i[0], i[1], i[2]});
+    if (it == m_subtrees.end())
+        throw std::range_error("Block does not exist");
+    const Subtree& st = *it->second;
+    (void)stream;
+    rv_brick_job j{};
+    j.sparse = st.sparse;
+    j.plan = st.plan;
+    j.dense = d_dense;
+    j.depth = st.depth;
+    return j;
+}
+
 void FieldServiceHip::interpolate(unsigned level, const std::array<uint32_t, 3>& i, float* d_dense,
                                   rv_stream_t stream)
 {
     auto it = m_subtrees.find({level, i[0], i[1], i[2]});
     if (it == m_subtrees.end())
         throw std::range_error("Block does not exist");
-    Subtree& st = *it->second;
-    if (!st.plan) {
-        hipCheck(hipMalloc((void**)&st.plan, 4 * rv_brick_nodes(st.depth)), "hipMalloc");
-        throwIfFailed(rv_interp_plan_build(st.n2i, st.n, st.depth, st.plan, nullptr, stream));
-    }
+    const Subtree& st = *it->second;
     throwIfFailed(rv_interp_brick_planned(st.sparse, st.plan, st.depth, d_dense, stream));
 }
 
@@ -166,7 +175,26 @@ void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
         fs.interpolate(level, index, m_brick, m_stream);
         m_lastBlock = key;
     }
-    // makeColorTransferTexture (:573-589)
+    ensureColorTable();
+    if (isCancelled())
+        return;
+    const auto range = fs.fieldRange();
+    rv_block_params p;
+    int live = 0;
+    throwIfFailed(rv_block_params_setup(st.cameraTransform.data(), st.viewportSize[0],
+                                        st.viewportSize[1], st.fovY, level, index.data(), depth,
+                                        range[0], range[1], st.threshold, st.renderQuality,
+                                        st.alphaCutoff, &p, &live));
+    if (!live)
+        return;
+    throwIfFailed(rv_render_block_argb(&p, m_brick, m_tf, m_tfN, vp, m_stream));
+}
+
+// makeColorTransferTexture (DenseFieldRenderer_cu.cu:573-589): the table is
+// uploaded when it changed.
+void DenseFieldRendererHip::ensureColorTable()
+{
+    const RenderSharedState& st = *m_state;
     if (m_tfDirty) {
         const uint32_t n = (uint32_t)(st.colorTable.size() / 4);
         if (n == 0)
@@ -183,21 +211,30 @@ void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
                  "hipMemcpyAsync");
         m_tfDirty = false;
     }
-    if (isCancelled())
-        return;
-    const auto range = fs.fieldRange();
-    rv_block_params p;
-    int live = 0;
-    throwIfFailed(rv_block_params_setup(st.cameraTransform.data(), st.viewportSize[0],
-                                        st.viewportSize[1], st.fovY, level, index.data(), depth,
-                                        range[0], range[1], st.threshold, st.renderQuality,
-                                        st.alphaCutoff, &p, &live));
-    if (!live)
-        return;
-    throwIfFailed(rv_render_block_argb(&p, m_brick, m_tf, m_tfN, vp, m_stream));
 }
 
 // ---------------------------------------------------------------- worker
+
+// Device state of the fused share path for one (level, index box, field
+// version): one brick per block, the batched interpolation handle, the block
+// table workspace.
+struct VsWorkerHip::ShareCache {
+    std::array<uint32_t, 7> key{};
+    uint64_t version = ~0ull;
+    std::map<std::array<uint32_t, 3>, float*> bricks;
+    rv_interp_batch* batch = nullptr;
+    void* workspace = nullptr;
+    uint64_t workspace_bytes = 0;
+    ~ShareCache()
+    {
+        if (batch)
+            rv_interp_batch_destroy(batch);
+        for (auto& kv : bricks)
+            (void)hipFree(kv.second);
+        if (workspace)
+            (void)hipFree(workspace);
+    }
+};
 
 VsWorkerHip::VsWorkerHip() = default;
 
@@ -270,13 +307,17 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
     uint64_t got = 0;
     throwIfFailed(rv_back_to_front(sub.boxMin.data(), sub.boxMax.data(), cmin, cmax, eye,
                                    order.data(), count, &got));
-    for (uint64_t i = 0; i < got; ++i) {
-        if (isCancelled()) {
-            m_cancelled = true;
-            break;
+    if (m_fused) {
+        renderFused(sub, order, got, isCancelled);
+    } else {
+        for (uint64_t i = 0; i < got; ++i) {
+            if (isCancelled()) {
+                m_cancelled = true;
+                break;
+            }
+            const std::array<uint32_t, 3> idx{order[3 * i], order[3 * i + 1], order[3 * i + 2]};
+            m_renderer.renderDenseField(m_viewport, sub.level, idx, isCancelled);
         }
-        const std::array<uint32_t, 3> idx{order[3 * i], order[3 * i + 1], order[3 * i + 2]};
-        m_renderer.renderDenseField(m_viewport, sub.level, idx, isCancelled);
     }
     if (m_cancelled || isCancelled()) {
         m_cancelled = true;
@@ -310,6 +351,78 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
     hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
     m_partRect = {rect[0], rect[1], rect[2], rect[3]};
     return part;
+}
+
+void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vector<uint32_t>& order,
+                              uint64_t count, const std::function<bool()>& isCancelled)
+{
+    const RenderSharedState& st = *m_state;
+    FieldServiceHip& fs = *st.fieldSvc;
+    rv_stream_t s = stream();
+    if (!m_share)
+        m_share = std::make_unique<ShareCache>();
+    ShareCache& sc = *m_share;
+    const std::array<uint32_t, 7> key{sub.level,     sub.boxMin[0], sub.boxMin[1], sub.boxMin[2],
+                                      sub.boxMax[0], sub.boxMax[1], sub.boxMax[2]};
+    if (key != sc.key || sc.version != fs.version() || !sc.batch) {
+        if (sc.batch)
+            rv_interp_batch_destroy(sc.batch);
+        sc.batch = nullptr;
+        std::vector<rv_brick_job> jobs;
+        jobs.reserve(count);
+        for (uint64_t i = 0; i < count; ++i) {
+            const std::array<uint32_t, 3> idx{order[3 * i], order[3 * i + 1], order[3 * i + 2]};
+            const uint64_t nodes = rv_brick_nodes(fs.subtreeDepth(sub.level, idx));
+            float*& brick = sc.bricks[idx];
+            if (!brick)
+                hipCheck(hipMalloc((void**)&brick, 4 * nodes), "hipMalloc");
+            jobs.push_back(fs.job(sub.level, idx, brick, s));
+        }
+        throwIfFailed(rv_interp_batch_create(jobs.data(), (uint32_t)jobs.size(), &sc.batch));
+        sc.key = key;
+        sc.version = fs.version();
+    }
+    if (isCancelled()) {
+        m_cancelled = true;
+        return;
+    }
+    throwIfFailed(rv_interp_batch_run(sc.batch, s));
+    // colour table (as the per-block renderer keeps it)
+    m_renderer.ensureColorTable();
+    const auto range = fs.fieldRange();
+    std::vector<rv_block_params> params;
+    std::vector<const float*> bricks;
+    params.reserve(count);
+    bricks.reserve(count);
+    for (uint64_t i = 0; i < count; ++i) {
+        const std::array<uint32_t, 3> idx{order[3 * i], order[3 * i + 1], order[3 * i + 2]};
+        rv_block_params p;
+        int live = 0;
+        throwIfFailed(rv_block_params_setup(st.cameraTransform.data(), st.viewportSize[0],
+                                            st.viewportSize[1], st.fovY, sub.level, idx.data(),
+                                            fs.subtreeDepth(sub.level, idx), range[0], range[1],
+                                            st.threshold, st.renderQuality, st.alphaCutoff, &p, &live));
+        if (!live)
+            continue;
+        params.push_back(p);
+        bricks.push_back(sc.bricks[idx]);
+    }
+    const uint64_t need = rv_share_workspace_bytes((uint32_t)params.size());
+    if (need > sc.workspace_bytes) {
+        if (sc.workspace)
+            (void)hipFree(sc.workspace);
+        sc.workspace = nullptr;
+        hipCheck(hipMalloc(&sc.workspace, need), "hipMalloc");
+        sc.workspace_bytes = need;
+    }
+    if (isCancelled()) {
+        m_cancelled = true;
+        return;
+    }
+    throwIfFailed(rv_render_share_argb(params.data(), bricks.data(), (uint32_t)params.size(),
+                                       m_renderer.colorTable(), m_renderer.colorTableSize(),
+                                       m_viewport, st.viewportSize[0], st.viewportSize[1],
+                                       sc.workspace, s));
 }
 
 const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const
@@ -569,6 +682,16 @@ extern "C" int rv_worker_compose_frame(rv_worker* w, uint32_t level, uint32_t sp
     std::memcpy(frame, out.data(), 4 * out.size());
     return RV_OK;
     W_CATCH
+}
+
+extern "C" int rv_worker_set_fused(rv_worker* w, int fused)
+{
+    if (!w) {
+        rv::set_error("rv_worker_set_fused: null worker");
+        return RV_ERR_INVALID;
+    }
+    w->worker.setFused(fused != 0);
+    return RV_OK;
 }
 
 extern "C" void rv_worker_destroy(rv_worker* w) { delete w; }

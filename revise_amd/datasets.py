@@ -58,6 +58,31 @@ class Field:
             f += self.amp * v / (2 ** o)
         return f
 
+    def eval_torch(self, u):
+        """Same field on a torch float64 tensor (any device)."""
+        import torch
+        f = torch.zeros(u.shape[:-1], dtype=torch.float64, device=u.device)
+        for c, s in zip(self.c, self.s):
+            cc = torch.tensor(c, dtype=torch.float64, device=u.device)
+            f += torch.exp(-((u - cc) ** 2).sum(-1) / (2 * s * s))
+        for o, L in enumerate(self.lat):
+            Lt = torch.tensor(L, dtype=torch.float64, device=u.device)
+            cells = L.shape[0] - 1
+            x = u.clamp(0.0, 1.0) * cells
+            i = torch.clamp(torch.floor(x).long(), max=cells - 1)
+            t = x - i
+            t = t * t * (3 - 2 * t)
+            v = torch.zeros_like(f)
+            for dz in (0, 1):
+                for dy in (0, 1):
+                    for dx in (0, 1):
+                        w = ((t[..., 0] if dx else 1 - t[..., 0]) *
+                             (t[..., 1] if dy else 1 - t[..., 1]) *
+                             (t[..., 2] if dz else 1 - t[..., 2]))
+                        v = v + w * Lt[i[..., 0] + dx, i[..., 1] + dy, i[..., 2] + dz]
+            f += self.amp * v / (2 ** o)
+        return f
+
     def grad_norm(self, u, h=1e-4):
         g = np.zeros(np.asarray(u).shape[:-1])
         for a in range(3):

@@ -56,9 +56,7 @@ def _setup(blocks, W, H, fp16=False, acc_cut=2.0, seed=0):
     for b in blocks:
         key = (id(b.n2i), b.depth)
         if key not in plans:
-            pl = torch.empty(api.brick_nodes(b.depth), dtype=torch.int32, device="cuda")
-            api.interp_plan_build(to_dev(b.n2i.view(np.int16)), b.depth, pl)
-            plans[key] = pl
+            plans[key] = api.interp_plan(b.n2i, b.depth)
         brick = torch.empty(api.brick_nodes(b.depth), dtype=torch.float16 if fp16 else torch.float32,
                             device="cuda")
         dev_blocks.append({"level": b.level, "index": b.index, "depth": b.depth, "brick": brick,

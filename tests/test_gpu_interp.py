@@ -41,42 +41,38 @@ def test_interp_planned_bit_exact(orc, seed, depth, p, pnf):
     d, n2i, vals = _case(seed, depth, p, pnf)
     say("interp seed", seed, "depth", d, "nodes", n2i.shape[0])
     V = api.brick_nodes(d)
-    plan = torch.empty(V, dtype=torch.int32, device="cuda")
+    plan = api.interp_plan(n2i, d)
     dense = torch.empty(V, dtype=torch.float32, device="cuda")
-    bad = torch.zeros(1, dtype=torch.int32, device="cuda")
-    dn2i = to_dev(n2i.view(np.int16))
-    api.interp_plan_build(dn2i, d, plan, bad)
     api.interp_brick_planned(to_dev(vals), plan, d, dense)
     got = dense.cpu().numpy()
     exp = orc.interpolate(vals, n2i, d)
-    assert int(bad.item()) == 0
     assert bits_equal_nan(got, exp)
 
 
-def test_interp_unplanned_and_workspace(orc):
+def test_interp_unplanned(orc):
     import torch
     d, n2i, vals = _case(21, 5, 0.4, 0.02)
     V = api.brick_nodes(d)
     exp = orc.interpolate(vals, n2i, d)
-    dn2i, dv = to_dev(n2i.view(np.int16)), to_dev(vals)
     a = torch.empty(V, dtype=torch.float32, device="cuda")
-    api.interp_brick(dv, dn2i, d, a)  # library-allocated workspace
-    ws = torch.empty(V, dtype=torch.int32, device="cuda")
-    b = torch.empty(V, dtype=torch.float32, device="cuda")
-    api.interp_brick(dv, dn2i, d, b, ws)
+    api.interp_brick(to_dev(vals), n2i, d, a)  # plan staged inside the call
     assert bits_equal_nan(a.cpu().numpy(), exp)
-    assert bits_equal_nan(b.cpu().numpy(), exp)
 
 
-def test_interp_plan_counts_bad_coordinates():
+@pytest.mark.parametrize("depth", [3, 6, 8])
+def test_interp_full_tree_with_nan_values(orc, depth):
+    """A fully refined subtree whose sparse data holds NaN (undefined) values:
+    the reference interpolates those nodes like absent ones, so the kernel's
+    all-nodes-present fast path must fall back to the full method."""
     import torch
-    n2i = np.array([[0, 0, 0], [3, 0, 0], [0, 9, 0]], dtype=np.uint16)  # depth 2: max coord 4
-    plan = torch.empty(api.brick_nodes(2), dtype=torch.int32, device="cuda")
-    bad = torch.zeros(1, dtype=torch.int32, device="cuda")
-    api.interp_plan_build(to_dev(n2i.view(np.int16)), 2, plan, bad)
-    assert int(bad.item()) == 1
-    p = plan.cpu().numpy().view(np.uint32)
-    assert p[0] == 0 and p[3] == 1 and (p == 0xFFFFFFFF).sum() == p.size - 2
+    rng = np.random.default_rng(depth)
+    n2i = api.subtree_n2i(datasets.full_subtree(depth), depth)
+    vals = rng.normal(size=n2i.shape[0]).astype(np.float32)
+    vals[rng.random(vals.size) < 0.002] = np.nan
+    vals[rng.random(vals.size) < 0.001] = np.float32(1e30)
+    dense = torch.empty(api.brick_nodes(depth), dtype=torch.float32, device="cuda")
+    api.interp_brick_planned(to_dev(vals), api.interp_plan(n2i, depth), depth, dense)
+    assert bits_equal_nan(dense.cpu().numpy(), orc.interpolate(vals, n2i, depth))
 
 
 def test_interp_trilinear_field_reproduced():
@@ -95,7 +91,7 @@ def test_interp_trilinear_field_reproduced():
     vals = f(u[:, 0], u[:, 1], u[:, 2]).astype(np.float32)
     V = api.brick_nodes(d)
     dense = torch.empty(V, dtype=torch.float32, device="cuda")
-    api.interp_brick(to_dev(vals), to_dev(n2i.view(np.int16)), d, dense)
+    api.interp_brick(to_dev(vals), n2i, d, dense)
     n = 2 ** d + 1
     g = np.arange(n) / 2 ** d
     z, y, x = np.meshgrid(g, g, g, indexing="ij")
@@ -110,8 +106,7 @@ def test_interp_batch_mixed_depths_fp32_fp16(orc):
     jobs, outs, exps = [], [], []
     for i, (d, n2i, vals) in enumerate(cases):
         V = api.brick_nodes(d)
-        plan = torch.empty(V, dtype=torch.int32, device="cuda")
-        api.interp_plan_build(to_dev(n2i.view(np.int16)), d, plan)
+        plan = api.interp_plan(n2i, d)
         dt = torch.float16 if i % 2 else torch.float32
         dense = torch.empty(V, dtype=dt, device="cuda")
         jobs.append((to_dev(vals), plan, dense, d))
@@ -139,6 +134,6 @@ def test_interp_full_fill_is_scatter(orc):
     n2i = api.subtree_n2i(datasets.full_subtree(d), d)
     vals = np.arange(n2i.shape[0], dtype=np.float32)
     dense = torch.empty(api.brick_nodes(d), dtype=torch.float32, device="cuda")
-    api.interp_brick(to_dev(vals), to_dev(n2i.view(np.int16)), d, dense)
+    api.interp_brick(to_dev(vals), n2i, d, dense)
     out = dense.cpu().numpy()
     assert np.array_equal(out[orc.ordinals(n2i, d)], vals)

@@ -184,3 +184,43 @@ def test_bound_rect_cases(orc):
             img[ys, xs] = (rng.integers(1, 256, size=k).astype(np.uint32) << 24)
         img |= rng.integers(0, 2 ** 24, size=(H, W)).astype(np.uint32)  # colour without alpha
         assert api.bound_rect(to_dev(img.view(np.int32)), W, H) == orc.bound_rect(img)
+
+
+def test_share_kernel_equals_per_block_launches(orc):
+    """rv_render_share_argb (all blocks in one launch) against one
+    rv_render_block_argb launch per block, mixed brick depths, bit-exact."""
+    import torch
+    rng = np.random.default_rng(12)
+    field = datasets.Field(12, n_gauss=3, noise_octaves=1)
+    level, W, H = 2, 230, 170
+    m = 2 ** level
+    blocks = []
+    for z in range(m):
+        for y in range(m):
+            for x in range(m):
+                depth = int(rng.integers(2, 6))
+                ch = datasets.refined_subtree(depth, lambda l, locs: rng.random(locs.shape[0]) < 0.6)
+                d = api.subtree_depth(ch, depth)
+                n2i = api.subtree_n2i(ch, d)
+                blocks.append(datasets.Block(level, (x, y, z), d, n2i,
+                                             datasets.sample(field, level, (x, y, z), d, n2i)))
+    fmin, fmax = datasets.field_range(blocks)
+    tf = datasets.seeded_tf(12, 256)
+    d_tf = to_dev(tf)
+    M = datasets.orbit_camera(77.0, elevation_deg=-30.0)
+    eye = api.eye_from_view(M)
+    order = api.back_to_front([0] * 3, [m] * 3, [-5] * 3, [5] * 3, eye)
+    by = {b.index: b for b in blocks}
+    items = []
+    for idx in order:
+        b = by[tuple(int(v) for v in idx)]
+        live, p = api.block_params(M, W, H, 30.0, level, b.index, b.depth, fmin, fmax, 0.5, 2.0, 0.996)
+        if live:
+            items.append((p, to_dev(orc.interpolate(b.sparse, b.n2i, b.depth))))
+    init = rng.integers(0, 2 ** 32, size=(H, W), dtype=np.uint64).astype(np.uint32)
+    a = to_dev(init.view(np.int32))
+    for p, br in items:
+        api.render_block_argb(p, br, d_tf, a)
+    b_ = to_dev(init.view(np.int32))
+    api.ShareRenderer(items, d_tf, W, H).render(b_)
+    assert np.array_equal(a.cpu().numpy(), b_.cpu().numpy())

@@ -58,9 +58,11 @@ def _oracle_share(orc, blocks, level, bmin, bmax, M, W, H, fmin, fmax, tf):
     return r, vp[y:y + h, x:x + w]
 
 
-def test_worker_scene_part_bit_exact(orc):
+@pytest.mark.parametrize("fused", [True, False])
+def test_worker_scene_part_bit_exact(orc, fused):
     W, H = 220, 160
     w, blocks, fmin, fmax, tf, M = _make(2, 4, 1, W, H, 35.0)
+    w.set_fused(fused)
     for bmin, bmax in [((0, 0, 0), (4, 4, 4)), ((1, 0, 2), (3, 4, 4)), ((3, 3, 3), (4, 4, 4))]:
         say("worker part", bmin, bmax)
         got = w.render_scene_part(2, bmin, bmax)
@@ -71,9 +73,11 @@ def test_worker_scene_part_bit_exact(orc):
     w.close()
 
 
-def test_worker_empty_box_and_cancel(orc):
+@pytest.mark.parametrize("fused", [True, False])
+def test_worker_empty_box_and_cancel(orc, fused):
     W, H = 64, 48
     w, *_ = _make(1, 3, 2, W, H, 10.0)
+    w.set_fused(fused)
     rect, bits = w.render_scene_part(1, (1, 1, 1), (1, 2, 2))  # empty index box
     assert rect == (0, 0, 0, 0) and bits.size == 0
     assert w.render_scene_part(1, (0, 0, 0), (2, 2, 2), cancel=True) is None
@@ -114,10 +118,11 @@ def _oracle_frame(orc, blocks, level, K, M, W, H, fmin, fmax, tf):
     return frame
 
 
-@pytest.mark.parametrize("K", [1, 2, 3, 5, 8])
-def test_worker_compose_frame_bit_exact(orc, K):
+@pytest.mark.parametrize("K,fused", [(1, True), (2, True), (3, False), (5, True), (8, False), (8, True)])
+def test_worker_compose_frame_bit_exact(orc, K, fused):
     W, H = 200, 150
     w, blocks, fmin, fmax, tf, M = _make(2, 3, 7, W, H, 120.0)
+    w.set_fused(fused)
     say("compose frame K", K)
     got = w.compose_frame(2, K)
     exp = _oracle_frame(orc, blocks, 2, K, M, W, H, fmin, fmax, tf)

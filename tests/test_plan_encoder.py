@@ -1,0 +1,99 @@
+"""Gather plan encoder (host, C ABI): the per-tile lists the interpolation
+kernel scatters from.  Checked by decoding the plan and comparing with a
+brute-force tile membership of every n2i node; the kernel's GPU parity tests
+exercise the same plans end to end."""
+import numpy as np
+import pytest
+
+from revise_amd import api, datasets
+
+MAGIC = 0x31504E52
+
+
+def _geo(d):
+    t = min(d, 4)
+    T, X = 1 << t, (1 << d) if d < 6 else 64
+    return T, X, (1 << d) // X, (1 << d) // T, (1 << d) >> t
+
+
+def _decode(buf):
+    h = buf[:32].view(np.uint32)
+    magic, depth, ntiles, nlat, entries, lat_off, idx_off, lds_off = (int(v) for v in h)
+    start = buf[32:32 + 4 * (ntiles + 1)].view(np.uint32)
+    lat = buf[lat_off:lat_off + 4 * nlat].view(np.uint32)
+    idx = buf[idx_off:idx_off + 4 * entries].view(np.uint32)
+    lds = buf[lds_off:lds_off + 2 * entries].view(np.uint16)
+    return magic, depth, ntiles, start, lat, idx, lds
+
+
+def _tree(seed, depth, p):
+    rng = np.random.default_rng(seed)
+    ch = datasets.refined_subtree(depth, lambda l, locs: rng.random(locs.shape[0]) < p)
+    d = api.subtree_depth(ch, depth)
+    return d, api.subtree_n2i(ch, d)
+
+
+@pytest.mark.parametrize("seed,depth,p", [(0, 0, 0.0), (1, 2, 0.7), (2, 4, 0.5), (3, 6, 0.3),
+                                          (4, 6, 1.0), (5, 7, 0.2), (6, 8, 0.1)])
+def test_plan_matches_brute_force_tiles(seed, depth, p):
+    d, n2i = _tree(seed, depth, p)
+    buf = api.interp_plan_encode(n2i, d)
+    assert buf.size == api.interp_plan_bytes(n2i, d) and buf.size % 16 == 0
+    magic, depth_p, ntiles, start, lat, idx, lds = _decode(buf)
+    T, X, tx, tyz, m = _geo(d)
+    assert magic == MAGIC and depth_p == d and ntiles == tx * tyz * tyz
+    assert start[0] == 0 and np.all(np.diff(start.astype(np.int64)) >= 0)
+    rowlen, tp1 = X + 1, T + 1
+    c = n2i.astype(np.int64)
+    for t in range(ntiles):
+        ix, iyz = t % tx, t // tx
+        iy, iz = iyz % tyz, iyz // tyz
+        x0, y0, z0 = ix * X, iy * T, iz * T
+        inside = ((c[:, 0] >= x0) & (c[:, 0] <= x0 + X) & (c[:, 1] >= y0) & (c[:, 1] <= y0 + T) &
+                  (c[:, 2] >= z0) & (c[:, 2] <= z0 + T))
+        want = np.nonzero(inside)[0]
+        got = idx[start[t]:start[t + 1]]
+        assert np.array_equal(got, want)  # every node of the closed box, increasing index
+        lc = c[want] - np.array([x0, y0, z0])
+        assert np.array_equal(lds[start[t]:start[t + 1]], (lc[:, 2] * tp1 + lc[:, 1]) * rowlen + lc[:, 0])
+    if m >= 2:
+        mp = m + 1
+        assert lat.size == mp ** 3
+        want = np.full(mp ** 3, 0xFFFFFFFF, dtype=np.uint32)
+        on = np.all(c % T == 0, axis=1)
+        q = c[on] // T
+        want[(q[:, 2] * mp + q[:, 1]) * mp + q[:, 0]] = np.nonzero(on)[0]
+        assert np.array_equal(lat, want)
+    else:
+        assert lat.size == 0
+
+
+def test_plan_rejects_coordinates_outside_the_brick():
+    n2i = np.array([[0, 0, 0], [3, 0, 0], [0, 9, 0]], dtype=np.uint16)  # depth 2: max coordinate 4
+    assert api.interp_plan_bytes(n2i, 2) == 0
+    with pytest.raises(RuntimeError, match="outside the brick"):
+        api.interp_plan_encode(n2i, 2)
+
+
+def test_plan_rejects_depth_above_8():
+    n2i = np.zeros((1, 3), dtype=np.uint16)
+    with pytest.raises(RuntimeError, match="depth"):
+        api.interp_plan_encode(n2i, 9)
+
+
+def test_plan_of_empty_subtree():
+    buf = api.interp_plan_encode(np.zeros((0, 3), dtype=np.uint16), 6)
+    _, _, ntiles, start, lat, idx, _ = _decode(buf)
+    assert ntiles == 16 and np.all(start == 0) and idx.size == 0
+    assert lat.size == 125 and np.all(lat == 0xFFFFFFFF)
+
+
+def test_plan_keeps_last_of_duplicate_coordinates():
+    """The reference scatters sparse values in order, so a coordinate listed
+    twice ends with its last value; the plan lists only that occurrence."""
+    n2i = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 0], [2, 2, 2], [1, 0, 0]], dtype=np.uint16)
+    buf = api.interp_plan_encode(n2i, 1)
+    _, _, ntiles, start, _, idx, lds = _decode(buf)
+    assert ntiles == 1
+    assert idx.tolist() == [2, 3, 4]
+    assert lds.tolist() == [0, 26, 1]
