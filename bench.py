@@ -192,7 +192,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cams", type=int, default=64)
-    ap.add_argument("--only-interp", action="store_true", help="time the batched interpolation only")
+    ap.add_argument("--only-interp", action="store_true",
+                    help="time a 1 GiB device copy and the batched interpolation only (profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the multi-GPU "
@@ -216,8 +217,15 @@ def main():
     torch.cuda.synchronize()
     log("config 2 setup %.1fs" % (time.time() - t0))
     if args.only_interp:
+        # plain device copy of known size (1 GiB read + 1 GiB written): the
+        # calibration for the HBM byte counters of a profiler run
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
         batch = api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"]) for b in wl.blocks])
         batch.run()
+        log("copy 1 GiB: %.3f ms" % (events_time(torch, lambda: dst.copy_(src), 3) * 1e3))
         log("batched interp: %.3f ms" % (events_time(torch, batch.run, 10) * 1e3))
         return
     for _ in range(args.warmup):

@@ -153,7 +153,7 @@ template <class BrickT>
 __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
 {
     __shared__ __align__(16) float A[kTileFloats]; // interpolation tile; colour table + list for screen tiles
-    __shared__ float lat[32];
+    __shared__ float lat[64];
     __shared__ uint32_t wave_cnt[kT / 64u];
     __shared__ uint32_t item_s;
     float4* tfs = reinterpret_cast<float4*>(A);

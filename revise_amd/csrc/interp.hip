@@ -10,12 +10,13 @@
 //
 // Decomposition.  A midpoint of a level-l cube depends only on values inside
 // that cube, and level-l midpoints never touch coarser lattice nodes.  So the
-// brick is cut into tiles of X x T x T cells (T = min(2^d, 16), X = min(2^d, 64),
-// x spans whole rows of the brick), one workgroup each:
+// brick is cut into tiles of X x T x T cells (X = min(2^d, 64), so x spans
+// whole rows of the brick up to depth 6; T = 8 or 16, interp_geom.hpp), one
+// workgroup each:
 //   1. a tile with a node not present in the sparse data first completes the
 //      coarse levels (cubes larger than T) on the stride-T lattice of the
 //      whole brick -- at most 17^3 nodes, in LDS -- and keeps the lattice
-//      values at its own corners;
+//      values of its own box;
 //   2. it scatters its sparse nodes into LDS: the gather plan (plan.cpp) lists
 //      them per tile in increasing sparse index with their LDS offsets, so
 //      every global read is coalesced and the permutation happens in LDS;
@@ -27,8 +28,9 @@
 // Nodes on faces shared by two tiles are computed identically by both (their
 // operands lie on the shared face), so the tiles are independent.
 //
-// Memory layout: the dense brick is x fastest; a tile writes (T or T+1) full
-// rows of the brick per z-plane, i.e. contiguous runs of 16 x 260 B for d = 6.
+// Memory layout: the dense brick is x fastest; up to depth 6 a tile writes
+// (T or T+1) full rows of the brick per z-plane, one contiguous run, stored
+// lane-contiguously.
 
 #include "interp_device.hpp"
 #include "rv_internal.hpp"
@@ -47,23 +49,31 @@ namespace rv {
 
 namespace {
 
-constexpr uint32_t kInterpThreads = 512;
+// Workgroup size of the two LDS classes: 512 threads for tiles of up to
+// 65 x 17 x 17 nodes, RV_INTERP_SMALL_THREADS for tiles of up to 65 x 9 x 9.
+#ifndef RV_INTERP_SMALL_THREADS
+#define RV_INTERP_SMALL_THREADS 256
+#endif
+constexpr uint32_t kBigThreads = 512;
+constexpr uint32_t kSmallThreads = RV_INTERP_SMALL_THREADS;
 
-__global__ __launch_bounds__(kInterpThreads) void interp_single_kernel(Job job, uint32_t ntiles)
+template <uint32_t kFloats, uint32_t NT>
+__global__ __launch_bounds__(NT) void interp_single_kernel(Job job, uint32_t ntiles)
 {
-    __shared__ float A[kTileFloats];
-    __shared__ float lat[32];
+    __shared__ float A[kFloats];
+    __shared__ float lat[64];
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= ntiles)
         return;
-    interp_tile<kInterpThreads>(job, t, A, lat);
+    interp_tile<NT>(job, t, A, lat);
 }
 
-__global__ __launch_bounds__(kInterpThreads) void interp_batch_kernel(const Job* __restrict__ jobs,
-                                                                      uint32_t njobs, uint32_t ntiles)
+template <uint32_t kFloats, uint32_t NT>
+__global__ __launch_bounds__(NT) void interp_batch_kernel(const Job* __restrict__ jobs,
+                                                          uint32_t njobs, uint32_t ntiles)
 {
-    __shared__ float A[kTileFloats];
-    __shared__ float lat[32];
+    __shared__ float A[kFloats];
+    __shared__ float lat[64];
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= ntiles)
         return;
@@ -77,8 +87,10 @@ __global__ __launch_bounds__(kInterpThreads) void interp_batch_kernel(const Job*
             hi = mid;
     }
     const Job job = jobs[lo];
-    interp_tile<kInterpThreads>(job, t - job.tile_start, A, lat);
+    interp_tile<NT>(job, t - job.tile_start, A, lat);
 }
+
+bool big_class(uint32_t depth) { return tile_floats(geom_of(depth)) > kTileFloats8; }
 
 __global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint64_t n, uint32_t value)
 {
@@ -135,6 +147,7 @@ struct rv_interp_batch {
     uint32_t njobs = 0;
     uint32_t ntiles = 0;
     uint64_t dense_nodes = 0;
+    bool big = false; // some brick needs the 65 x 17 x 17-node LDS class
 };
 
 extern "C" uint64_t rv_brick_nodes(uint32_t depth)
@@ -200,8 +213,12 @@ extern "C" int rv_interp_brick_planned(const float* d_sparse, const void* d_plan
     RV_REQUIRE(d_plan && d_dense, "rv_interp_brick_planned: null pointer");
     Job job{d_sparse, static_cast<const uint8_t*>(d_plan), d_dense, nullptr, depth, 0};
     const Geom g = geom_of(depth);
-    hipLaunchKernelGGL(interp_single_kernel, dim3(round8(g.ntiles)), dim3(kInterpThreads), 0,
-                       S(stream), job, g.ntiles);
+    if (big_class(depth))
+        hipLaunchKernelGGL((interp_single_kernel<kTileFloats, kBigThreads>), dim3(round8(g.ntiles)),
+                           dim3(kBigThreads), 0, S(stream), job, g.ntiles);
+    else
+        hipLaunchKernelGGL((interp_single_kernel<kTileFloats8, kSmallThreads>),
+                           dim3(round8(g.ntiles)), dim3(kSmallThreads), 0, S(stream), job, g.ntiles);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }
@@ -257,6 +274,7 @@ extern "C" int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count,
         host[i] = Job{j.sparse, static_cast<const uint8_t*>(j.plan), j.dense,
                       reinterpret_cast<__half*>(j.dense16), j.depth, (uint32_t)tiles};
         tiles += geom_of(j.depth).ntiles;
+        b->big |= big_class(j.depth);
         b->dense_nodes += rv_brick_nodes(j.depth);
     }
     RV_REQUIRE(tiles < 0x7FFFFFF0ull, "rv_interp_batch_create: too many tiles");
@@ -281,8 +299,13 @@ extern "C" int rv_interp_batch_run(rv_interp_batch* b, rv_stream_t stream)
     RV_REQUIRE(b, "rv_interp_batch_run: null batch");
     if (b->ntiles == 0)
         return RV_OK;
-    hipLaunchKernelGGL(interp_batch_kernel, dim3(round8(b->ntiles)), dim3(kInterpThreads), 0,
-                       S(stream), b->d_jobs, b->njobs, b->ntiles);
+    if (b->big)
+        hipLaunchKernelGGL((interp_batch_kernel<kTileFloats, kBigThreads>), dim3(round8(b->ntiles)),
+                           dim3(kBigThreads), 0, S(stream), b->d_jobs, b->njobs, b->ntiles);
+    else
+        hipLaunchKernelGGL((interp_batch_kernel<kTileFloats8, kSmallThreads>),
+                           dim3(round8(b->ntiles)), dim3(kSmallThreads), 0, S(stream), b->d_jobs,
+                           b->njobs, b->ntiles);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }

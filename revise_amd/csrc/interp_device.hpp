@@ -8,36 +8,18 @@
 
 #include <cstdint>
 
+#include "interp_geom.hpp"
+
 namespace rv {
 
 constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
 constexpr float kNoField = 1e30f;
 constexpr uint32_t kMaxDepth = 8;
-// LDS floats of the largest tile, 65 x 17 x 17 nodes; the coarse lattice of
-// the whole brick (at most 17^3 nodes) reuses the same array.
+// LDS floats of the largest tile, 65 x 17 x 17 nodes (depth 8 always uses it).
 constexpr uint32_t kTileFloats = 65u * 17u * 17u;
-constexpr uint32_t kPlanMagic = 0x31504e52u;
-
-struct Geom {
-    uint32_t d, n, t, T, X, tiles_x, tiles_yz, ntiles, m;
-};
-
-// Tiles of at most 64 x 16 x 16 cells (x spans whole brick rows up to 64
-// cells); m = lattice cells per axis of the stride-T coarse lattice.
-__host__ __device__ inline Geom geom_of(uint32_t d)
-{
-    Geom g;
-    g.d = d;
-    g.n = (1u << d) + 1u;
-    g.t = d < 4u ? d : 4u;
-    g.T = 1u << g.t;
-    g.X = d < 6u ? (1u << d) : 64u;
-    g.tiles_x = (1u << d) / g.X;
-    g.tiles_yz = (1u << d) / g.T;
-    g.ntiles = g.tiles_x * g.tiles_yz * g.tiles_yz;
-    g.m = (1u << d) >> g.t;
-    return g;
-}
+// LDS floats of a 64 x 8 x 8-cell tile; its coarse lattice (at most 17^3
+// nodes, depth <= 7) fits as well.
+constexpr uint32_t kTileFloats8 = 65u * 9u * 9u;
 
 __device__ __forceinline__ uint32_t ordinal3(uint32_t x, uint32_t y, uint32_t z, uint32_t d)
 {
@@ -57,65 +39,106 @@ struct Job {
     uint32_t tile_start;  // first global tile of this job (batched launches)
 };
 
+// The odd-axis classes of one refinement phase of a level: phase 1 = edge
+// midpoints (one odd coordinate), 2 = face centres (two), 3 = cell centres
+// (three).  Class k of a phase has the odd-axis mask below (bit 0 = x, 1 = y,
+// 2 = z); its nodes form a cx x cy x cz grid, with qx, qy cubes per axis
+// (qy also along z).  Items of a phase are numbered class by class.
+__device__ __forceinline__ uint32_t phase_mask(uint32_t phase, uint32_t k)
+{
+    return phase == 1u ? (1u << k) : (phase == 2u ? (7u ^ (4u >> k)) : 7u);
+}
+
+__device__ __forceinline__ uint32_t class_size(uint32_t m, uint32_t qx, uint32_t qy)
+{
+    return ((m & 1u) ? qx : qx + 1u) * ((m & 2u) ? qy : qy + 1u) * ((m & 4u) ? qy : qy + 1u);
+}
+
+// Item q of a phase (classes end at end0 < end1 <= total): its node and the
+// value the reference assigns it if it is still undefined (operand order of
+// DenseFieldInterpolator.hpp:257-370).  Returns false when the node is
+// already defined.  All values stay in registers (no per-class tables, which
+// the compiler would place in scratch memory).
+__device__ __forceinline__ bool refine_item(float* A, uint32_t phase, uint32_t q, uint32_t qx,
+                                            uint32_t qy, uint32_t end0, uint32_t end1, uint32_t h,
+                                            uint32_t rowlen, uint32_t plane, float*& out,
+                                            float& value)
+{
+    const uint32_t k = q < end0 ? 0u : (q < end1 ? 1u : 2u);
+    const uint32_t m = phase_mask(phase, k);
+    const uint32_t base = k == 0u ? 0u : (k == 1u ? end0 : end1);
+    const uint32_t cx = (m & 1u) ? qx : qx + 1u;
+    const uint32_t cxy = cx * ((m & 2u) ? qy : qy + 1u);
+    // r / cxy and r2 / cx through fp32 reciprocals: the quotients are below
+    // 64 and (r + 0.5) / d stays 0.5 / d >= 1e-4 away from an integer, far
+    // above the rounding error (a few 1e-6), so the truncation is exact.
+    const uint32_t r = q - base;
+    const uint32_t iz = (uint32_t)(((float)r + 0.5f) * __builtin_amdgcn_rcpf((float)cxy));
+    const uint32_t r2 = r - iz * cxy;
+    const uint32_t iy = (uint32_t)(((float)r2 + 0.5f) * __builtin_amdgcn_rcpf((float)cx));
+    const uint32_t ix = r2 - iy * cx;
+    const uint32_t x = ((ix << 1) + (m & 1u)) * h;
+    const uint32_t y = ((iy << 1) + ((m >> 1) & 1u)) * h;
+    const uint32_t z = ((iz << 1) + (m >> 2)) * h;
+    float* p = A + z * plane + y * rowlen + x;
+    const int sx = (int)h, sy = (int)(h * rowlen), sz = (int)(h * plane);
+    const float cur = *p;
+    float v;
+    if (phase == 1u) {
+        const int st = (m & 1u) ? sx : ((m & 2u) ? sy : sz);
+        const float a = p[-st], b = p[st];
+        v = (a == kNoField || b == kNoField) ? kNoField : 0.5f * (a + b);
+    } else if (phase == 2u) {
+        // odd axes d1 < d2
+        const int st1 = (m & 1u) ? sx : sy;
+        const int st2 = (m & 4u) ? sz : sy;
+        const float s1 = p[-st1], e1 = p[st1];
+        const float s2 = p[-st2], e2 = p[st2];
+        v = (s1 == kNoField || e1 == kNoField || s2 == kNoField || e2 == kNoField)
+                ? kNoField
+                : 0.25f * (((s1 + e1) + s2) + e2);
+    } else {
+        const float s1 = p[-sx], e1 = p[sx];
+        const float s2 = p[-sy], e2 = p[sy];
+        const float s3 = p[-sz], e3 = p[sz];
+        v = (s1 == kNoField || e1 == kNoField || s2 == kNoField || e2 == kNoField ||
+             s3 == kNoField || e3 == kNoField)
+                ? kNoField
+                : (float)(1. / 6) * (((((s1 + e1) + s2) + e2) + s3) + e3);
+    }
+    out = p;
+    value = v;
+    return is_nan(cur);
+}
+
 // Complete the levels with cube sizes s_start, s_start/2, ..., 2 (in box units)
 // on a box of (X+1) x (T+1) x (T+1) values at A (x fastest).  Three phases per
-// level: nodes with one, two, three coordinates an odd multiple of h = s/2,
-// each the mean of its neighbours at +-h along its odd axes, in the operand
-// order of DenseFieldInterpolator.hpp:257-370.
+// level, separated by barriers; within a phase no node reads another node of
+// the same phase, so each thread handles two items at a time with all their
+// LDS reads issued before either store.
 template <uint32_t NT>
 __device__ inline void refine_box(float* A, uint32_t X, uint32_t T, uint32_t s_start)
 {
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t nwaves = NT / 64u;
     const uint32_t rowlen = X + 1u;
     const uint32_t plane = rowlen * (T + 1u);
     for (uint32_t s = s_start; s >= 2u; s >>= 1) {
         const uint32_t h = s >> 1;
-        const uint32_t ny = T / h + 1u;   // sub-lattice rows per axis (y and z)
-        const uint32_t nx = X / h + 1u;   // sub-lattice nodes per row
-        const uint32_t nrows = ny * ny;
-        const uint32_t sx = h, sy = h * rowlen, sz = h * plane;
         for (uint32_t phase = 1; phase <= 3; ++phase) {
-            for (uint32_t q = wave; q < nrows; q += nwaves) {
-                const uint32_t j = q % ny, k = q / ny;
-                const uint32_t oy = j & 1u, oz = k & 1u;
-                // rows whose y/z parities cannot produce this phase
-                if (oy + oz > phase || oy + oz + 1u < phase)
-                    continue;
-                float* row = A + (k * h) * plane + (j * h) * rowlen;
-                for (uint32_t i = lane; i < nx; i += 64u) {
-                    const uint32_t ox = i & 1u;
-                    if (ox + oy + oz != phase)
-                        continue;
-                    float* p = row + i * h;
-                    if (!is_nan(*p))
-                        continue;
-                    float v;
-                    if (phase == 1u) {
-                        const uint32_t st = ox ? sx : (oy ? sy : sz);
-                        const float a = p[-(int)st], b = p[st];
-                        v = (a == kNoField || b == kNoField) ? kNoField : 0.5f * (a + b);
-                    } else if (phase == 2u) {
-                        // odd axes d1 < d2
-                        const uint32_t st1 = ox ? sx : sy;
-                        const uint32_t st2 = oz ? sz : sy;
-                        const float s1 = p[-(int)st1], e1 = p[st1];
-                        const float s2 = p[-(int)st2], e2 = p[st2];
-                        v = (s1 == kNoField || e1 == kNoField || s2 == kNoField || e2 == kNoField)
-                                ? kNoField
-                                : 0.25f * (((s1 + e1) + s2) + e2);
-                    } else {
-                        const float s1 = p[-(int)sx], e1 = p[sx];
-                        const float s2 = p[-(int)sy], e2 = p[sy];
-                        const float s3 = p[-(int)sz], e3 = p[sz];
-                        v = (s1 == kNoField || e1 == kNoField || s2 == kNoField ||
-                             e2 == kNoField || s3 == kNoField || e3 == kNoField)
-                                ? kNoField
-                                : (float)(1. / 6) * (((((s1 + e1) + s2) + e2) + s3) + e3);
-                    }
-                    *p = v;
-                }
+            const uint32_t qx = X / s, qy = T / s;
+            const uint32_t end0 = class_size(phase_mask(phase, 0), qx, qy);
+            const uint32_t end1 = phase == 3u ? end0 : end0 + class_size(phase_mask(phase, 1), qx, qy);
+            const uint32_t total = phase == 3u ? end0 : end1 + class_size(phase_mask(phase, 2), qx, qy);
+            for (uint32_t q = threadIdx.x; q < total; q += 2u * NT) {
+                float *p0, *p1 = nullptr;
+                float v0, v1 = 0.f;
+                const bool w0 = refine_item(A, phase, q, qx, qy, end0, end1, h, rowlen, plane, p0, v0);
+                bool w1 = false;
+                if (q + NT < total)
+                    w1 = refine_item(A, phase, q + NT, qx, qy, end0, end1, h, rowlen, plane, p1, v1);
+                if (w0)
+                    *p0 = v0;
+                if (w1)
+                    *p1 = v1;
             }
             __syncthreads();
         }
@@ -138,7 +161,7 @@ __device__ __forceinline__ PlanView plan_view(const uint8_t* P)
     v.ntiles = h[2];
     v.nlat = h[3];
     v.entries = h[4];
-    v.tile_start = h + 8;
+    v.tile_start = h + kPlanHeader / 4u;
     v.lat = reinterpret_cast<const uint32_t*>(P + h[5]);
     v.idx = reinterpret_cast<const uint32_t*>(P + h[6]);
     v.lds = reinterpret_cast<const uint16_t*>(P + h[7]);
@@ -146,10 +169,11 @@ __device__ __forceinline__ PlanView plan_view(const uint8_t* P)
 }
 
 // One tile of a brick with NT threads (a multiple of 64); A holds
-// kTileFloats floats, lat 32 floats.
+// tile_floats(geom_of(depth)) floats, lat 64 floats.
 template <uint32_t NT>
 __device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A, float* lat)
 {
+    const PlanView P = plan_view(job.plan);
     const Geom g = geom_of(job.depth);
     const uint32_t d = g.d, X = g.X, T = g.T;
     const uint32_t tx = local_tile % g.tiles_x;
@@ -163,7 +187,6 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A
     const uint32_t rowlen = X + 1u;
     const uint32_t tp1 = T + 1u;
     const uint32_t tile_nodes = rowlen * tp1 * tp1;
-    const PlanView P = plan_view(job.plan);
     const uint32_t e0 = P.tile_start[local_tile], e1 = P.tile_start[local_tile + 1];
 
     // A "full" tile skips the interpolation; if a sparse value in it is NaN
@@ -249,6 +272,25 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A
     const uint32_t ox = (tx + 1u == g.tiles_x) ? rowlen : X;
     const uint32_t oy = (ty + 1u == g.tiles_yz) ? tp1 : T;
     const uint32_t oz = (tz + 1u == g.tiles_yz) ? tp1 : T;
+    if (g.tiles_x == 1u) {
+        // the tile spans whole brick rows: its rows of one z-plane are one
+        // contiguous run in the brick and in LDS
+        const uint32_t run = oy * rowlen;
+        for (uint32_t z = 0; z < oz; ++z) {
+            const uint32_t base = ordinal3(0, y0, z0 + z, d);
+            const float* src = A + z * tp1 * rowlen;
+            for (uint32_t q = threadIdx.x; q < run; q += NT) {
+                float v = src[q];
+                if (v == kNoField)
+                    v = __builtin_nanf("");
+                if (job.dense16)
+                    job.dense16[base + q] = __float2half(v);
+                else
+                    __builtin_nontemporal_store(v, job.dense + base + q);
+            }
+        }
+        return;
+    }
     const uint32_t orows = oy * oz;
     for (uint32_t r = wave; r < orows; r += nwaves) {
         const uint32_t y = r % oy, z = r / oy;

@@ -1,16 +1,17 @@
 // Gather plan of a subtree (host encoder).
 //
-// The interpolation kernel completes a brick in tiles of at most 64 x 16 x 16
-// cells (interp_device.hpp).  The plan lists, per tile, the sparse nodes that
-// lie in the tile's closed box, in increasing sparse index (so the kernel's
-// reads of sparse values and of the plan itself are coalesced), each with its
-// offset in the tile's LDS array; nodes on faces shared by tiles appear once
-// per tile.  It also lists, for the stride-16 lattice of the whole brick, the
+// The interpolation kernel completes a brick in tiles of X x T x T cells
+// (interp_geom.hpp).  The plan lists, per tile, the sparse nodes that lie in
+// the tile's closed box, in increasing sparse index (so the kernel's reads of
+// sparse values and of the plan itself are coalesced), each with its offset
+// in the tile's LDS array; nodes on faces shared by tiles appear once per
+// tile.  It also lists, for the stride-T lattice of the whole brick, the
 // sparse index of each lattice node (or none), for the coarse levels.
 //
 // Layout (little endian, all offsets in bytes from the plan start):
 //   u32 magic, u32 depth, u32 ntiles, u32 nlat,
 //   u32 entries, u32 lat_off, u32 idx_off, u32 ldsoff_off,
+//   u32 T, u32 X, u32 reserved[6]       (64-byte header)
 //   u32 tile_start[ntiles + 1]          entry range of each tile
 //   u32 lat_idx[nlat]                   lattice node -> sparse index / ~0
 //   u32 idx[entries]                    sparse index of each entry
@@ -18,6 +19,7 @@
 // A plan depends only on the subtree's n2i (like BlockTreeNodes, which the
 // reference caches per subtree, BlockTreeFieldService.hpp:133-137).
 #include "../../include/revise_hip.h"
+#include "interp_geom.hpp"
 
 #include <cstring>
 #include <string>
@@ -29,25 +31,8 @@ void set_error(const std::string& msg);
 
 namespace {
 
-constexpr uint32_t kMagic = 0x31504e52u; // "RNP1"
-
-struct Geo {
-    uint32_t d, X, T, tx, tyz, ntiles, m;
-};
-
-Geo geo(uint32_t d)
-{
-    Geo g;
-    g.d = d;
-    const uint32_t t = d < 4u ? d : 4u;
-    g.T = 1u << t;
-    g.X = d < 6u ? (1u << d) : 64u;
-    g.tx = (1u << d) / g.X;
-    g.tyz = (1u << d) / g.T;
-    g.ntiles = g.tx * g.tyz * g.tyz;
-    g.m = (1u << d) >> t;
-    return g;
-}
+using rv::Geom;
+using rv::geom_of;
 
 inline uint32_t align16(uint32_t v) { return (v + 15u) & ~15u; }
 
@@ -64,7 +49,7 @@ inline int tiles_of(uint32_t c, uint32_t e, uint32_t n, uint32_t* out)
 }
 
 struct Encoded {
-    Geo g;
+    Geom g;
     std::vector<uint32_t> start, lat, idx;
     std::vector<uint16_t> lds;
     uint32_t lat_off, idx_off, lds_off, bytes;
@@ -80,8 +65,8 @@ int encode(const uint16_t* n2i, uint64_t n, uint32_t depth, Encoded& e)
         rv::set_error("rv_interp_plan: too many nodes");
         return RV_ERR_INVALID;
     }
-    e.g = geo(depth);
-    const Geo& g = e.g;
+    e.g = geom_of(depth);
+    const Geom& g = e.g;
     const uint32_t npe = (1u << depth) + 1u;
     const uint32_t rowlen = g.X + 1u, tp1 = g.T + 1u;
     // A coordinate listed twice keeps its last sparse value, as the
@@ -107,12 +92,12 @@ int encode(const uint16_t* n2i, uint64_t n, uint32_t depth, Encoded& e)
             continue;
         const uint32_t x = n2i[3 * i], y = n2i[3 * i + 1], z = n2i[3 * i + 2];
         uint32_t ax[2], ay[2], az[2];
-        const int nx = tiles_of(x, g.X, g.tx, ax), ny = tiles_of(y, g.T, g.tyz, ay),
-                  nz = tiles_of(z, g.T, g.tyz, az);
+        const int nx = tiles_of(x, g.X, g.tiles_x, ax), ny = tiles_of(y, g.T, g.tiles_yz, ay),
+                  nz = tiles_of(z, g.T, g.tiles_yz, az);
         for (int a = 0; a < nz; ++a)
             for (int b = 0; b < ny; ++b)
                 for (int c = 0; c < nx; ++c)
-                    ++count[(az[a] * g.tyz + ay[b]) * g.tx + ax[c] + 1];
+                    ++count[(az[a] * g.tiles_yz + ay[b]) * g.tiles_x + ax[c] + 1];
     }
     e.start.assign(g.ntiles + 1, 0);
     for (uint32_t t = 0; t < g.ntiles; ++t)
@@ -127,12 +112,12 @@ int encode(const uint16_t* n2i, uint64_t n, uint32_t depth, Encoded& e)
             continue;
         const uint32_t x = n2i[3 * i], y = n2i[3 * i + 1], z = n2i[3 * i + 2];
         uint32_t ax[2], ay[2], az[2];
-        const int nx = tiles_of(x, g.X, g.tx, ax), ny = tiles_of(y, g.T, g.tyz, ay),
-                  nz = tiles_of(z, g.T, g.tyz, az);
+        const int nx = tiles_of(x, g.X, g.tiles_x, ax), ny = tiles_of(y, g.T, g.tiles_yz, ay),
+                  nz = tiles_of(z, g.T, g.tiles_yz, az);
         for (int a = 0; a < nz; ++a)
             for (int b = 0; b < ny; ++b)
                 for (int c = 0; c < nx; ++c) {
-                    const uint32_t t = (az[a] * g.tyz + ay[b]) * g.tx + ax[c];
+                    const uint32_t t = (az[a] * g.tiles_yz + ay[b]) * g.tiles_x + ax[c];
                     const uint32_t lx = x - ax[c] * g.X, ly = y - ay[b] * g.T, lz = z - az[a] * g.T;
                     const uint32_t k = cursor[t]++;
                     e.idx[k] = (uint32_t)i;
@@ -149,7 +134,7 @@ int encode(const uint16_t* n2i, uint64_t n, uint32_t depth, Encoded& e)
                 continue;
             e.lat[((z / g.T) * mp + y / g.T) * mp + x / g.T] = (uint32_t)i;
         }
-    const uint32_t head = 32u + 4u * (g.ntiles + 1u);
+    const uint32_t head = rv::kPlanHeader + 4u * (g.ntiles + 1u);
     e.lat_off = align16(head);
     e.idx_off = align16(e.lat_off + 4u * (uint32_t)e.lat.size());
     e.lds_off = align16(e.idx_off + 4u * entries);
@@ -184,10 +169,11 @@ extern "C" int rv_interp_plan_encode(const uint16_t* n2i, uint64_t n, uint32_t d
     }
     auto* p = static_cast<uint8_t*>(plan);
     std::memset(p, 0, e.bytes);
-    const uint32_t head[8] = {kMagic, depth, e.g.ntiles, (uint32_t)e.lat.size(),
-                              (uint32_t)e.idx.size(), e.lat_off, e.idx_off, e.lds_off};
+    const uint32_t head[10] = {rv::kPlanMagic, depth, e.g.ntiles, (uint32_t)e.lat.size(),
+                               (uint32_t)e.idx.size(), e.lat_off, e.idx_off, e.lds_off,
+                               e.g.T, e.g.X};
     std::memcpy(p, head, sizeof head);
-    std::memcpy(p + 32, e.start.data(), 4 * e.start.size());
+    std::memcpy(p + rv::kPlanHeader, e.start.data(), 4 * e.start.size());
     if (!e.lat.empty())
         std::memcpy(p + e.lat_off, e.lat.data(), 4 * e.lat.size());
     if (!e.idx.empty()) {

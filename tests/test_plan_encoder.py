@@ -7,23 +7,23 @@ import pytest
 
 from revise_amd import api, datasets
 
-MAGIC = 0x31504E52
+MAGIC = 0x32504E52
 
 
-def _geo(d):
-    t = min(d, 4)
-    T, X = 1 << t, (1 << d) if d < 6 else 64
-    return T, X, (1 << d) // X, (1 << d) // T, (1 << d) >> t
+def _geo(d, T):
+    X = (1 << d) if d < 6 else 64
+    return X, (1 << d) // X, (1 << d) // T, (1 << d) // T
 
 
 def _decode(buf):
-    h = buf[:32].view(np.uint32)
-    magic, depth, ntiles, nlat, entries, lat_off, idx_off, lds_off = (int(v) for v in h)
-    start = buf[32:32 + 4 * (ntiles + 1)].view(np.uint32)
+    h = buf[:64].view(np.uint32)
+    magic, depth, ntiles, nlat, entries, lat_off, idx_off, lds_off, T, X = (int(v) for v in h[:10])
+    assert T == min(1 << depth, T) and X == min(1 << depth, 64)
+    start = buf[64:64 + 4 * (ntiles + 1)].view(np.uint32)
     lat = buf[lat_off:lat_off + 4 * nlat].view(np.uint32)
     idx = buf[idx_off:idx_off + 4 * entries].view(np.uint32)
     lds = buf[lds_off:lds_off + 2 * entries].view(np.uint16)
-    return magic, depth, ntiles, start, lat, idx, lds
+    return magic, depth, ntiles, start, lat, idx, lds, T
 
 
 def _tree(seed, depth, p):
@@ -39,8 +39,10 @@ def test_plan_matches_brute_force_tiles(seed, depth, p):
     d, n2i = _tree(seed, depth, p)
     buf = api.interp_plan_encode(n2i, d)
     assert buf.size == api.interp_plan_bytes(n2i, d) and buf.size % 16 == 0
-    magic, depth_p, ntiles, start, lat, idx, lds = _decode(buf)
-    T, X, tx, tyz, m = _geo(d)
+    magic, depth_p, ntiles, start, lat, idx, lds, T = _decode(buf)
+    # the tile extent is 8 or 16 cells (build option), 16 at depth 8, at most 2^d
+    assert T == min(1 << d, 16 if d == 8 else T) and T in (1 << min(d, 3), 1 << min(d, 4))
+    X, tx, tyz, m = _geo(d, T)
     assert magic == MAGIC and depth_p == d and ntiles == tx * tyz * tyz
     assert start[0] == 0 and np.all(np.diff(start.astype(np.int64)) >= 0)
     rowlen, tp1 = X + 1, T + 1
@@ -83,9 +85,10 @@ def test_plan_rejects_depth_above_8():
 
 def test_plan_of_empty_subtree():
     buf = api.interp_plan_encode(np.zeros((0, 3), dtype=np.uint16), 6)
-    _, _, ntiles, start, lat, idx, _ = _decode(buf)
-    assert ntiles == 16 and np.all(start == 0) and idx.size == 0
-    assert lat.size == 125 and np.all(lat == 0xFFFFFFFF)
+    _, _, ntiles, start, lat, idx, _, T = _decode(buf)
+    m = 64 // T
+    assert ntiles == m * m and np.all(start == 0) and idx.size == 0
+    assert lat.size == (m + 1) ** 3 and np.all(lat == 0xFFFFFFFF)
 
 
 def test_plan_keeps_last_of_duplicate_coordinates():
@@ -93,7 +96,7 @@ def test_plan_keeps_last_of_duplicate_coordinates():
     twice ends with its last value; the plan lists only that occurrence."""
     n2i = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 0], [2, 2, 2], [1, 0, 0]], dtype=np.uint16)
     buf = api.interp_plan_encode(n2i, 1)
-    _, _, ntiles, start, _, idx, lds = _decode(buf)
+    _, _, ntiles, start, _, idx, lds, _ = _decode(buf)
     assert ntiles == 1
     assert idx.tolist() == [2, 3, 4]
     assert lds.tolist() == [0, 26, 1]
