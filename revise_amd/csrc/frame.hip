@@ -12,11 +12,16 @@
 // so the result is the reference's compositing of the same segments without
 // the per-block 8-bit viewport round trip.
 //
-// A screen tile that reaches a block whose brick is still being interpolated
-// waits on that block's tile counter (release/acquire at agent scope).  All
-// brick tiles are dequeued before any screen tile, and a workgroup holding a
-// brick tile never waits, so every wait ends; a wait is additionally bounded
-// (it sets an error flag and gives up after ~2^22 sleeps).
+// Bricks and screen tiles synchronize once per workgroup, not per tile: a
+// workgroup that interpolated brick tiles publishes them (device-scope fence
+// and one release add of its tile count) when it dequeues its first screen
+// tile or leaves; before its first screen tile it waits (acquire) until all
+// n_interp brick tiles are published.  On this chip every release/acquire at
+// agent scope writes back / invalidates the XCD's L2, so signalling per brick
+// tile costs more than the interpolation itself.  All brick tiles are
+// dequeued before any screen tile and a workgroup holding a brick tile never
+// waits, so every wait ends; a wait is additionally bounded (it sets an error
+// flag and gives up after ~2^22 sleeps).
 //
 // The blocks of a screen tile are found by testing the block view
 // rectangles (camera dependent, one uint4 per block) in front-to-back order,
@@ -44,14 +49,12 @@ int fill_u32(uint32_t* d, uint64_t n, uint32_t value, hipStream_t s);
 namespace {
 
 struct FrameArgs {
-    const Job* jobs;               // one interpolation job per block (block order)
-    uint32_t njobs;
+    const TileTask* tasks;         // brick tiles of all blocks
     uint32_t n_interp;             // brick tiles of this frame (0: bricks are resident)
     const rv_block_params* params; // per block
     const uint4* rects;            // per block view rectangle {x0, y0, x1, y1}, x1 = x0 if dead
     const uint32_t* f2b;           // block indices, front to back
     const void* const* bricks;     // per block, float or half
-    const uint32_t* need;          // per block: tiles of its brick
     uint32_t nblocks;
     const float4* tf;
     uint32_t tf_n;
@@ -59,7 +62,7 @@ struct FrameArgs {
     float acc_cutoff;
     uint32_t* argb;                // W*H, rows bottom-up like the reference viewport
     float4* rgba;                  // optional float output, same layout
-    uint32_t* counters;            // [0] queue, [1] error, [2 + b] tiles done of block b
+    uint32_t* counters;            // [0] queue, [1] error, [2] brick tiles published
 };
 
 constexpr uint32_t kT = 512;
@@ -107,21 +110,6 @@ __device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint
         __syncthreads();
         for (uint32_t k = 0; k < total; ++k) {
             const uint32_t blk = uniform(list[k]);
-            if (a.n_interp) {
-                if (tid == 0) {
-                    const uint32_t need = a.need[blk];
-                    uint32_t spins = 0;
-                    while (__hip_atomic_load(a.counters + 2 + blk, __ATOMIC_ACQUIRE,
-                                             __HIP_MEMORY_SCOPE_AGENT) < need) {
-                        __builtin_amdgcn_s_sleep(8);
-                        if (++spins > kSpinLimit) {
-                            atomicOr(a.counters + 1, 1u);
-                            break;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
             const rv_block_params p = a.params[blk];
             if (!opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
                 vy < p.view_y + p.view_h) {
@@ -149,6 +137,18 @@ __device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint
     }
 }
 
+// Make this workgroup's brick stores visible device-wide and add its tile
+// count to the published total.
+__device__ __forceinline__ void publish_tiles(uint32_t* published, uint32_t tiles)
+{
+    if (tiles == 0u) // uniform over the workgroup
+        return;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0)
+        __hip_atomic_fetch_add(published, tiles, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 template <class BrickT>
 __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
 {
@@ -160,6 +160,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
     uint32_t* list = reinterpret_cast<uint32_t*>(A) + 4u * kMaxTf;
     const uint32_t n_total = a.n_interp + a.ntx * a.nty;
     bool tf_loaded = false;
+    uint32_t my_tiles = 0; // brick tiles done by this workgroup, not yet published
     for (;;) {
         if (threadIdx.x == 0)
             item_s = atomicAdd(a.counters, 1u);
@@ -169,24 +170,28 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         if (item >= n_total)
             break;
         if (item < a.n_interp) {
-            uint32_t lo = 0, hi = a.njobs;
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (a.jobs[mid].tile_start <= item)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            const Job job = a.jobs[lo];
-            interp_tile<kT>(job, item - job.tile_start, A, lat);
-            __threadfence();
-            __syncthreads();
-            if (threadIdx.x == 0)
-                __hip_atomic_fetch_add(a.counters + 2 + lo, 1u, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+            const TileTask k = a.tasks[item];
+            interp_tile<kT>(k.job, k.local, k.e0, k.e1, A, lat);
+            ++my_tiles; // (the queue barrier above protects A before its next use)
             continue;
         }
         if (!tf_loaded) {
+            if (a.n_interp) {
+                publish_tiles(a.counters + 2, my_tiles);
+                my_tiles = 0;
+                if (threadIdx.x == 0) {
+                    uint32_t spins = 0;
+                    while (__hip_atomic_load(a.counters + 2, __ATOMIC_ACQUIRE,
+                                             __HIP_MEMORY_SCOPE_AGENT) < a.n_interp) {
+                        __builtin_amdgcn_s_sleep(8);
+                        if (++spins > kSpinLimit) {
+                            atomicOr(a.counters + 1, 1u);
+                            break;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
             for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT)
                 tfs[i] = a.tf[i];
             __syncthreads();
@@ -194,6 +199,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         }
         render_tile<BrickT>(a, item - a.n_interp, tfs, list, wave_cnt);
     }
+    publish_tiles(a.counters + 2, my_tiles);
 }
 
 // Worker share in one launch: the blocks (back to front) of one share are
@@ -278,9 +284,8 @@ struct rv_frame_plan {
     std::vector<rv_frame_block> blocks;
     rv_frame_desc desc{};
     bool fp16 = false;
-    Job* d_jobs = nullptr;
+    TileTask* d_tasks = nullptr;
     uint32_t n_interp = 0;
-    uint32_t* d_need = nullptr;
     const void** d_bricks = nullptr;
     uint32_t* d_counters = nullptr;
     uint32_t ncams = 0;
@@ -303,7 +308,7 @@ struct rv_frame_plan {
             (void)hipGraphDestroy(graph);
         if (capture_stream)
             (void)hipStreamDestroy(capture_stream);
-        for (void* p : {(void*)d_jobs, (void*)d_need, (void*)d_bricks, (void*)d_counters,
+        for (void* p : {(void*)d_tasks, (void*)d_bricks, (void*)d_counters,
                         (void*)d_params, (void*)d_rects, (void*)d_f2b})
             if (p)
                 (void)hipFree(p);
@@ -328,14 +333,12 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
                  hipStream_t s, uint32_t tw, uint32_t th)
 {
     FrameArgs a{};
-    a.jobs = P->d_jobs;
-    a.njobs = P->nblocks;
+    a.tasks = P->d_tasks;
     a.n_interp = interpolate ? P->n_interp : 0u;
     a.params = P->d_params + (size_t)cam * P->nblocks;
     a.rects = P->d_rects + (size_t)cam * P->nblocks;
     a.f2b = P->d_f2b + (size_t)cam * P->nblocks;
     a.bricks = P->d_bricks;
-    a.need = P->d_need;
     a.nblocks = P->nblocks;
     a.tf = reinterpret_cast<const float4*>(P->desc.d_tf);
     a.tf_n = P->desc.tf_n;
@@ -349,8 +352,7 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     a.argb = d_argb;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
     a.counters = P->d_counters;
-    hipLaunchKernelGGL(frame_reset_kernel, dim3(std::max(1u, (P->nblocks + 2 + 255) / 256)),
-                       dim3(256), 0, s, P->d_counters, P->nblocks + 2);
+    hipLaunchKernelGGL(frame_reset_kernel, dim3(1), dim3(64), 0, s, P->d_counters, 4u);
     RV_LAUNCH_CHECK();
     if (P->fp16)
         hipLaunchKernelGGL(frame_kernel<__half>, dim3(P->grid), dim3(kT), 0, s, a);
@@ -381,9 +383,7 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     P->desc = *desc;
     const bool fp16 = blocks[0].brick16 != nullptr;
     std::vector<Job> jobs(nblocks);
-    std::vector<uint32_t> need(nblocks);
     std::vector<const void*> bricks(nblocks);
-    uint64_t tiles = 0;
     for (uint32_t i = 0; i < nblocks; ++i) {
         const rv_frame_block& b = blocks[i];
         RV_REQUIRE(b.depth <= kMaxDepth, "rv_frame_plan_create: depth must be <= 8");
@@ -392,28 +392,27 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
             RV_REQUIRE(b.index[k] < (1u << b.level), "rv_frame_plan_create: index outside level");
         RV_REQUIRE((b.brick16 != nullptr) == fp16 && (b.brick != nullptr) != fp16,
                    "rv_frame_plan_create: all blocks must use the same brick type");
-        const Geom g = geom_of(b.depth);
         jobs[i] = Job{b.sparse, static_cast<const uint8_t*>(b.plan), b.brick,
-                      reinterpret_cast<__half*>(b.brick16), b.depth,
-                      (uint32_t)tiles};
-        need[i] = g.ntiles;
+                      reinterpret_cast<__half*>(b.brick16), b.depth};
         bricks[i] = fp16 ? (const void*)b.brick16 : (const void*)b.brick;
         if (!b.sparse || !b.plan)
             P->can_interp = false;
-        tiles += g.ntiles;
     }
-    RV_REQUIRE(tiles < 0x40000000ull, "rv_frame_plan_create: too many brick tiles");
     P->fp16 = fp16;
-    P->n_interp = (uint32_t)tiles;
     int rc;
-    if ((rc = upload((void**)&P->d_jobs, jobs.data(), sizeof(Job) * nblocks)))
-        return rc;
-    if ((rc = upload((void**)&P->d_need, need.data(), 4ull * nblocks)))
-        return rc;
+    if (P->can_interp) {
+        std::vector<TileTask> tasks;
+        if ((rc = build_tile_tasks(jobs.data(), nblocks, tasks)))
+            return rc;
+        RV_REQUIRE(tasks.size() < 0x40000000ull, "rv_frame_plan_create: too many brick tiles");
+        P->n_interp = (uint32_t)tasks.size();
+        if ((rc = upload((void**)&P->d_tasks, tasks.data(), sizeof(TileTask) * tasks.size())))
+            return rc;
+    }
     if ((rc = upload((void**)&P->d_bricks, bricks.data(), sizeof(void*) * nblocks)))
         return rc;
-    RV_HIP(hipMalloc((void**)&P->d_counters, 4ull * (nblocks + 2)));
-    RV_HIP(hipMemset(P->d_counters, 0, 4ull * (nblocks + 2)));
+    RV_HIP(hipMalloc((void**)&P->d_counters, 16));
+    RV_HIP(hipMemset(P->d_counters, 0, 16));
     // persistent grid: every workgroup resident at once
     int dev = 0, cus = 0, per_cu = 0;
     RV_HIP(hipGetDevice(&dev));

@@ -38,6 +38,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -65,29 +66,21 @@ __global__ __launch_bounds__(NT) void interp_single_kernel(Job job, uint32_t nti
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= ntiles)
         return;
-    interp_tile<NT>(job, t, A, lat);
+    const PlanView P = plan_view(job.plan);
+    interp_tile<NT>(job, t, P.tile_start[t], P.tile_start[t + 1], A, lat);
 }
 
 template <uint32_t kFloats, uint32_t NT>
-__global__ __launch_bounds__(NT) void interp_batch_kernel(const Job* __restrict__ jobs,
-                                                          uint32_t njobs, uint32_t ntiles)
+__global__ __launch_bounds__(NT) void interp_batch_kernel(const TileTask* __restrict__ tasks,
+                                                          uint32_t ntiles)
 {
     __shared__ float A[kFloats];
     __shared__ float lat[64];
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= ntiles)
         return;
-    // last job with tile_start <= t
-    uint32_t lo = 0, hi = njobs;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (jobs[mid].tile_start <= t)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    const Job job = jobs[lo];
-    interp_tile<NT>(job, t - job.tile_start, A, lat);
+    const TileTask k = tasks[t];
+    interp_tile<NT>(k.job, k.local, k.e0, k.e1, A, lat);
 }
 
 bool big_class(uint32_t depth) { return tile_floats(geom_of(depth)) > kTileFloats8; }
@@ -128,6 +121,41 @@ uint32_t round8(uint32_t v) { return (v + 7u) & ~7u; }
 
 } // namespace
 
+int build_tile_tasks(const Job* jobs, uint32_t njobs, std::vector<TileTask>& tasks)
+{
+    struct Table {
+        uint32_t depth;
+        std::vector<uint32_t> start;
+    };
+    std::map<const uint8_t*, Table> seen;
+    tasks.clear();
+    for (uint32_t i = 0; i < njobs; ++i) {
+        const Job& j = jobs[i];
+        const Geom g = geom_of(j.depth);
+        auto it = seen.find(j.plan);
+        if (it == seen.end()) {
+            uint32_t head[kPlanHeader / 4];
+            RV_HIP(hipMemcpy(head, j.plan, kPlanHeader, hipMemcpyDeviceToHost));
+            if (head[0] != kPlanMagic)
+                return fail(RV_ERR_INVALID, "interpolation job: plan is not an interpolation plan");
+            Table tb;
+            tb.depth = head[1];
+            if (tb.depth <= kMaxDepth && head[2] == geom_of(tb.depth).ntiles) {
+                tb.start.resize(head[2] + 1u);
+                RV_HIP(hipMemcpy(tb.start.data(), j.plan + kPlanHeader, 4ull * tb.start.size(),
+                                 hipMemcpyDeviceToHost));
+            }
+            it = seen.emplace(j.plan, std::move(tb)).first;
+        }
+        const Table& tb = it->second;
+        if (tb.depth != j.depth || tb.start.size() != g.ntiles + 1u)
+            return fail(RV_ERR_INVALID, "interpolation job: plan was built for another depth");
+        for (uint32_t t = 0; t < g.ntiles; ++t)
+            tasks.push_back(TileTask{j, t, tb.start[t], tb.start[t + 1]});
+    }
+    return RV_OK;
+}
+
 int fill_u32(uint32_t* d, uint64_t n, uint32_t value, hipStream_t s)
 {
     if (n == 0)
@@ -143,8 +171,7 @@ int fill_u32(uint32_t* d, uint64_t n, uint32_t value, hipStream_t s)
 using namespace rv;
 
 struct rv_interp_batch {
-    Job* d_jobs = nullptr;
-    uint32_t njobs = 0;
+    TileTask* d_tasks = nullptr;
     uint32_t ntiles = 0;
     uint64_t dense_nodes = 0;
     bool big = false; // some brick needs the 65 x 17 x 17-node LDS class
@@ -211,7 +238,7 @@ extern "C" int rv_interp_brick_planned(const float* d_sparse, const void* d_plan
 {
     RV_REQUIRE(depth <= kMaxDepth, "rv_interp_brick_planned: depth must be <= 8");
     RV_REQUIRE(d_plan && d_dense, "rv_interp_brick_planned: null pointer");
-    Job job{d_sparse, static_cast<const uint8_t*>(d_plan), d_dense, nullptr, depth, 0};
+    Job job{d_sparse, static_cast<const uint8_t*>(d_plan), d_dense, nullptr, depth};
     const Geom g = geom_of(depth);
     if (big_class(depth))
         hipLaunchKernelGGL((interp_single_kernel<kTileFloats, kBigThreads>), dim3(round8(g.ntiles)),
@@ -264,7 +291,6 @@ extern "C" int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count,
     RV_REQUIRE(count == 0 || jobs, "rv_interp_batch_create: null jobs");
     auto b = std::make_unique<rv_interp_batch>();
     std::vector<Job> host(count);
-    uint64_t tiles = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const rv_brick_job& j = jobs[i];
         RV_REQUIRE(j.depth <= kMaxDepth, "rv_interp_batch_create: depth must be <= 8");
@@ -272,20 +298,22 @@ extern "C" int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count,
         RV_REQUIRE((j.dense != nullptr) != (j.dense16 != nullptr),
                    "rv_interp_batch_create: exactly one of dense/dense16 must be set");
         host[i] = Job{j.sparse, static_cast<const uint8_t*>(j.plan), j.dense,
-                      reinterpret_cast<__half*>(j.dense16), j.depth, (uint32_t)tiles};
-        tiles += geom_of(j.depth).ntiles;
+                      reinterpret_cast<__half*>(j.dense16), j.depth};
         b->big |= big_class(j.depth);
         b->dense_nodes += rv_brick_nodes(j.depth);
     }
-    RV_REQUIRE(tiles < 0x7FFFFFF0ull, "rv_interp_batch_create: too many tiles");
-    b->njobs = count;
-    b->ntiles = (uint32_t)tiles;
-    if (count) {
-        RV_HIP(hipMalloc((void**)&b->d_jobs, sizeof(Job) * count));
-        const hipError_t e =
-            hipMemcpy(b->d_jobs, host.data(), sizeof(Job) * count, hipMemcpyHostToDevice);
+    std::vector<TileTask> tasks;
+    int rc = build_tile_tasks(host.data(), count, tasks);
+    if (rc)
+        return rc;
+    RV_REQUIRE(tasks.size() < 0x7FFFFFF0ull, "rv_interp_batch_create: too many tiles");
+    b->ntiles = (uint32_t)tasks.size();
+    if (!tasks.empty()) {
+        RV_HIP(hipMalloc((void**)&b->d_tasks, sizeof(TileTask) * tasks.size()));
+        const hipError_t e = hipMemcpy(b->d_tasks, tasks.data(), sizeof(TileTask) * tasks.size(),
+                                       hipMemcpyHostToDevice);
         if (e != hipSuccess) {
-            (void)hipFree(b->d_jobs);
+            (void)hipFree(b->d_tasks);
             return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
         }
     }
@@ -301,11 +329,11 @@ extern "C" int rv_interp_batch_run(rv_interp_batch* b, rv_stream_t stream)
         return RV_OK;
     if (b->big)
         hipLaunchKernelGGL((interp_batch_kernel<kTileFloats, kBigThreads>), dim3(round8(b->ntiles)),
-                           dim3(kBigThreads), 0, S(stream), b->d_jobs, b->njobs, b->ntiles);
+                           dim3(kBigThreads), 0, S(stream), b->d_tasks, b->ntiles);
     else
         hipLaunchKernelGGL((interp_batch_kernel<kTileFloats8, kSmallThreads>),
-                           dim3(round8(b->ntiles)), dim3(kSmallThreads), 0, S(stream), b->d_jobs,
-                           b->njobs, b->ntiles);
+                           dim3(round8(b->ntiles)), dim3(kSmallThreads), 0, S(stream), b->d_tasks,
+                           b->ntiles);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }
@@ -324,7 +352,7 @@ extern "C" void rv_interp_batch_destroy(rv_interp_batch* b)
 {
     if (!b)
         return;
-    if (b->d_jobs)
-        (void)hipFree(b->d_jobs);
+    if (b->d_tasks)
+        (void)hipFree(b->d_tasks);
     delete b;
 }

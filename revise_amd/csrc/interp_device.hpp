@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "interp_geom.hpp"
 
@@ -36,8 +37,21 @@ struct Job {
     float* dense;
     __half* dense16;
     uint32_t depth;
-    uint32_t tile_start;  // first global tile of this job (batched launches)
 };
+
+// One tile of a batched launch, with its plan entry range looked up on the
+// host when the batch is built: a workgroup reads this one record instead of
+// searching the job list and reading the plan's tile table.
+struct TileTask {
+    Job job;
+    uint32_t local; // tile index within its brick
+    uint32_t e0, e1; // plan entries [e0, e1) of the tile
+};
+
+// Builds the tile list of a batch, job after job (host).  Reads each distinct
+// plan's header and tile table back from the device once and checks the plan
+// against the job's depth (RV_ERR_INVALID on a mismatch).
+int build_tile_tasks(const Job* jobs, uint32_t njobs, std::vector<TileTask>& tasks);
 
 // The odd-axis classes of one refinement phase of a level: phase 1 = edge
 // midpoints (one odd coordinate), 2 = face centres (two), 3 = cell centres
@@ -169,9 +183,11 @@ __device__ __forceinline__ PlanView plan_view(const uint8_t* P)
 }
 
 // One tile of a brick with NT threads (a multiple of 64); A holds
-// tile_floats(geom_of(depth)) floats, lat 64 floats.
+// tile_floats(geom_of(depth)) floats, lat 64 floats; [e0, e1) is the tile's
+// plan entry range.
 template <uint32_t NT>
-__device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A, float* lat)
+__device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t e0, uint32_t e1,
+                                   float* A, float* lat)
 {
     const PlanView P = plan_view(job.plan);
     const Geom g = geom_of(job.depth);
@@ -187,7 +203,6 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A
     const uint32_t rowlen = X + 1u;
     const uint32_t tp1 = T + 1u;
     const uint32_t tile_nodes = rowlen * tp1 * tp1;
-    const uint32_t e0 = P.tile_start[local_tile], e1 = P.tile_start[local_tile + 1];
 
     // A "full" tile skips the interpolation; if a sparse value in it is NaN
     // (undefined, so the reference interpolates it) the tile is redone as a
@@ -219,7 +234,9 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A
 
         // 2. scatter the tile's sparse nodes into LDS: plan entries and sparse
         // values are read in increasing sparse index (coalesced), kUnroll per
-        // thread in flight.
+        // thread in flight.  The plan is per-subtree metadata read every
+        // frame (ordinary loads, cached); sparse values are read once per
+        // frame (streaming loads).
         constexpr uint32_t kUnroll = 8;
         bool saw_nan = false;
         for (uint32_t k0 = e0 + threadIdx.x; k0 < e1; k0 += NT * kUnroll) {
@@ -231,14 +248,14 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, float* A
                 si[j] = kNoIndex;
                 off[j] = 0;
                 if (k < e1) {
-                    si[j] = __builtin_nontemporal_load(P.idx + k);
-                    off[j] = __builtin_nontemporal_load(P.lds + k);
+                    si[j] = P.idx[k];
+                    off[j] = P.lds[k];
                 }
             }
             float v[kUnroll];
 #pragma unroll
             for (uint32_t j = 0; j < kUnroll; ++j)
-                v[j] = si[j] == kNoIndex ? 0.f : job.sparse[si[j]];
+                v[j] = si[j] == kNoIndex ? 0.f : __builtin_nontemporal_load(job.sparse + si[j]);
 #pragma unroll
             for (uint32_t j = 0; j < kUnroll; ++j)
                 if (si[j] != kNoIndex) {

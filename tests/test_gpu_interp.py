@@ -137,3 +137,18 @@ def test_interp_full_fill_is_scatter(orc):
     api.interp_brick(to_dev(vals), n2i, d, dense)
     out = dense.cpu().numpy()
     assert np.array_equal(out[orc.ordinals(n2i, d)], vals)
+
+
+def test_batch_rejects_plan_of_another_depth():
+    """A job whose plan was built for another depth is refused when the batch
+    is built (the kernel would otherwise index outside the brick)."""
+    import torch
+    from revise_amd import RvError
+    d, n2i, vals = _case(31, 5, 0.5)
+    plan = api.interp_plan(n2i, d)
+    dense = torch.empty(api.brick_nodes(d + 1), dtype=torch.float32, device="cuda")
+    with pytest.raises(RvError, match="another depth"):
+        api.InterpBatch([(to_dev(vals), plan, dense, d + 1)])
+    junk = torch.zeros(256, dtype=torch.uint8, device="cuda")
+    with pytest.raises(RvError, match="not an interpolation plan"):
+        api.InterpBatch([(to_dev(vals), junk, dense, d)])
