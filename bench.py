@@ -119,6 +119,20 @@ class PlainFrame:
         return int(c.item())
 
 
+def profiled_traffic(key="traffic_bytes_per_launch"):
+    """HBM bytes per launch of the batched interpolation, as measured by the
+    FETCH_SIZE / WRITE_SIZE counter runs summarised under profiles/ (None if
+    that summary is absent)."""
+    path = os.path.join(ROOT, "profiles", "interp_batch_512x65cubed_hbm_bytes.md")
+    try:
+        for line in open(path):
+            if line.startswith(key + ":"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def frame_plan(torch, wl, tf, quality, cutoff, tile=(0, 0)):
     from revise_amd import api
     return api.FramePlan(wl.blocks, W, H, FOV, wl.fmin, wl.fmax, THR, quality, cutoff, ACC_CUT, tf, tile)
@@ -227,6 +241,10 @@ def main():
         batch.run()
         log("copy 1 GiB: %.3f ms" % (events_time(torch, lambda: dst.copy_(src), 3) * 1e3))
         log("batched interp: %.3f ms" % (events_time(torch, batch.run, 10) * 1e3))
+        own = [b["plan"].clone() for b in wl.blocks]
+        batch_own = api.InterpBatch([(b["sparse"], pl, b["brick"], b["depth"]) for b, pl in zip(wl.blocks, own)])
+        batch_own.run()
+        log("batched interp, private plans: %.3f ms" % (events_time(torch, batch_own.run, 10) * 1e3))
         return
     for _ in range(args.warmup):
         plain.fused_step()
@@ -250,10 +268,19 @@ def main():
     batch = api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"]) for b in wl.blocks])
     batch.run()
     t_interp = events_time(torch, batch.run, 20)
+    # the same bricks with one private copy of the plan per brick: no plan
+    # reuse across bricks in the caches (every subtree of a real field has its
+    # own node list)
+    own_plans = [b["plan"].clone() for b in wl.blocks]
+    batch_own = api.InterpBatch([(b["sparse"], pl, b["brick"], b["depth"]) for b, pl in zip(wl.blocks, own_plans)])
+    batch_own.run()
+    t_interp_own = events_time(torch, batch_own.run, 20)
+    del batch_own, own_plans
     V_total, N_total = wl.dense_nodes, wl.sparse_nodes
     B_alg = 4 * V_total + 10 * N_total
     log("batched interp %.3f ms, %.1f Gvox/s, %.0f GB/s" % (t_interp * 1e3, V_total / t_interp / 1e9,
                                                           B_alg / t_interp / 1e9))
+    log("batched interp, private plans %.3f ms" % (t_interp_own * 1e3))
     t_interp_plain = events_time(torch, lambda: [api.interp_brick_planned(b["sparse"], b["plan"], 6, b["brick"])
                                                  for b in wl.blocks], 3)
     t_render = events_time(torch, lambda: plain.step(interpolate=False, crop=False), 3)
@@ -295,6 +322,8 @@ def main():
         "per_block_launches_frame_ms": t_step_pb * 1e3,
         "interp_gvox_s": V_total / t_interp / 1e9,
         "interp_ms_batched": t_interp * 1e3,
+        "interp_ms_batched_private_plans": t_interp_own * 1e3,
+        "interp_gvox_s_private_plans": V_total / t_interp_own / 1e9,
         "interp_ms_per_block_launches": t_interp_plain * 1e3,
         "render_share_ms": t_share * 1e3,
         "render_per_block_launches_ms": t_render * 1e3,
@@ -306,8 +335,18 @@ def main():
         "batched_tile": list(tile),
         "roofline": {"kernel": "interp_batch_kernel (512 bricks, one launch)", "bound": "hbm",
                      "achieved": B_alg / t_interp / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": B_alg / t_interp / HBM_PEAK, "traffic": None,
-                     "alg_bytes_per_launch": B_alg},
+                     "frac": B_alg / t_interp / HBM_PEAK, "traffic": profiled_traffic(),
+                     "alg_bytes_per_launch": B_alg,
+                     "traffic_private_plans": profiled_traffic("traffic_bytes_per_launch_private_plans"),
+                     "note": "achieved = algorithmic bytes (4 B per dense voxel written + 10 B per sparse "
+                             "node read) / kernel time; traffic = HBM bytes per launch from rocprofv3 "
+                             "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy; the 512 bricks share one "
+                             "plan, which stays in cache (private plans: interp_ms_batched_private_plans)"},
+        "render_kernel": {"kernel": "share_kernel (all 512 blocks, per-block 8-bit composite)",
+                          "bound": "valu/lds", "ms": t_share * 1e3,
+                          "gsamples_s": S / t_share / 1e9,
+                          "brick_bytes_per_frame": 4 * V_total,
+                          "brick_gb_s": 4 * V_total / t_share / 1e9},
     })
     del batch
 
