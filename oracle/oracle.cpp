@@ -1001,6 +1001,82 @@ extern "C" int orc_ray_segment(const orc_block_params* p, const float* brick, co
     return 1;
 }
 
+/* Derived per-block scalars of derive(): out = {texel, tcf, i_layer_count,
+ * opacity_factor}. */
+extern "C" void orc_block_derived(const orc_block_params* p, float* out4)
+{
+    Derived d = derive(p);
+    out4[0] = d.texel;
+    out4[1] = d.tcf;
+    out4[2] = d.i_layer_count;
+    out4[3] = d.opacity_factor;
+}
+
+/* The samples march() composites for one pixel and block -- value not NaN,
+ * up to and including the one that ends the segment -- with the 8 corner
+ * indices of each sample's trilinear fetch (order 000,100,010,110,001,101,
+ * 011,111; x fastest) and its weights fx, fy, fz.  Test support for the
+ * adjoint: these fp32 decisions fix the set of samples a gradient is taken
+ * over.  Returns the number of samples (-1 when above max_samples, 0 when
+ * the ray misses) and the segment's alpha in *seg_alpha. */
+extern "C" int orc_ray_samples(const orc_block_params* p, const float* brick, const float* tf,
+                               uint32_t tf_n, uint32_t vx, uint32_t vy, uint32_t max_samples,
+                               uint32_t* corners, float* weights, float* seg_alpha)
+{
+    Derived d = derive(p);
+    *seg_alpha = 0.f;
+    if (vx < p->view_x || vx >= p->view_x + p->view_w || vy < p->view_y ||
+        vy >= p->view_y + p->view_h)
+        return 0;
+    Ray r = setup_ray(p, d, vx, vy);
+    if (!r.hit)
+        return 0;
+    const float range = p->fmax - p->fmin;
+    const uint32_t n = d.n;
+    float acc = 0.f;
+    int count = 0;
+    for (int i = 1; i < r.sampleCount; ++i, r.pos = add(r.pos, r.step)) {
+        V3 tc = mul(sub(r.pos, d.origin), d.tcf);
+        float v = fetch3(brick, n, tc);
+        if (std::isnan(v))
+            continue;
+        float v01 = (v - p->fmin) / range;
+        float c[4];
+        fetch_tf(tf, tf_n, v01, c);
+        c[3] = 1.f - orc_powf(1.f - c[3] * d.opacity_factor, d.i_layer_count);
+        if ((uint32_t)count >= max_samples)
+            return -1;
+        /* the taps of fetch3 */
+        float t[3] = {tc.x - 0.5f, tc.y - 0.5f, tc.z - 0.5f};
+        int i0[3], i1[3];
+        for (int a = 0; a < 3; ++a) {
+            float q = t[a];
+            if (!(q > -1.f))
+                q = -1.f;
+            if (q > (float)n)
+                q = (float)n;
+            float fl = std::floor(q);
+            weights[3 * count + a] = q - fl;
+            int k = (int)fl;
+            i0[a] = clampi(k, 0, (int)n - 1);
+            i1[a] = clampi(k + 1, 0, (int)n - 1);
+        }
+        for (int k = 0; k < 8; ++k) {
+            const int x = (k & 1) ? i1[0] : i0[0];
+            const int y = (k & 2) ? i1[1] : i0[1];
+            const int z = (k & 4) ? i1[2] : i0[2];
+            corners[8 * count + k] = (uint32_t)(((size_t)z * n + y) * n + x);
+        }
+        ++count;
+        float da = c[3] * (1.f - acc);
+        acc = acc + da;
+        if (acc > p->alpha_cutoff)
+            break;
+    }
+    *seg_alpha = acc;
+    return count;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Crop, blend, compose.                                                     */
 /* ------------------------------------------------------------------------ */

@@ -106,6 +106,15 @@ void orc_render_block(const orc_block_params* p, const float* brick, const float
 int orc_ray_segment(const orc_block_params* p, const float* brick, const float* tf,
                     uint32_t tf_n, uint32_t vx, uint32_t vy, float* rgba);
 
+/* Derived scalars of a block: {texel, tcf, i_layer_count, opacity_factor}. */
+void orc_block_derived(const orc_block_params* p, float* out4);
+
+/* Kept samples of one ray segment with their trilinear corners and weights
+ * (test support for the adjoint's gradient checks). */
+int orc_ray_samples(const orc_block_params* p, const float* brick, const float* tf,
+                    uint32_t tf_n, uint32_t vx, uint32_t vy, uint32_t max_samples,
+                    uint32_t* corners, float* weights, float* seg_alpha);
+
 /* Reference pow replacement shared by the oracle and the HIP kernels (see
  * the definition for the method).  Exposed for its error-bound test. */
 float orc_powf(float b, float e);

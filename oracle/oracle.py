@@ -82,6 +82,10 @@ def lib():
         L.orc_ray_segment.restype = C.c_int
         L.orc_ray_segment.argtypes = [C.POINTER(BlockParams), f32p, f32p, C.c_uint32, C.c_uint32,
                                       C.c_uint32, f32p]
+        L.orc_block_derived.argtypes = [C.POINTER(BlockParams), f32p]
+        L.orc_ray_samples.restype = C.c_int
+        L.orc_ray_samples.argtypes = [C.POINTER(BlockParams), f32p, f32p, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, C.c_uint32, u32p, f32p, C.POINTER(C.c_float)]
         L.orc_powf.restype = C.c_float
         L.orc_powf.argtypes = [C.c_float, C.c_float]
         L.orc_bound_rect.restype = C.c_int
@@ -211,6 +215,26 @@ def ray_segment(params, brick, tf, vx, vy):
     hit = lib().orc_ray_segment(C.byref(params), _p(brick, C.c_float), _p(tf, C.c_float), tf.shape[0],
                                 vx, vy, _p(out, C.c_float))
     return hit, out
+
+
+def block_derived(params):
+    """{texel, tcf, i_layer_count, opacity_factor} of a block (float32)."""
+    out = np.zeros(4, dtype=np.float32)
+    lib().orc_block_derived(C.byref(params), _p(out, C.c_float))
+    return out
+
+
+def ray_samples(params, brick, tf, vx, vy, max_samples=8192):
+    """Kept samples of one pixel's segment of one block: (corners (k, 8)
+    uint32, weights (k, 3) float32, segment alpha)."""
+    corners = np.zeros((max_samples, 8), dtype=np.uint32)
+    weights = np.zeros((max_samples, 3), dtype=np.float32)
+    a = C.c_float()
+    k = lib().orc_ray_samples(C.byref(params), _p(brick, C.c_float), _p(tf, C.c_float), tf.shape[0], vx, vy,
+                              max_samples, _p(corners, C.c_uint32), _p(weights, C.c_float), C.byref(a))
+    if k < 0:
+        raise ValueError("more than %d samples" % max_samples)
+    return corners[:k], weights[:k], np.float32(a.value)
 
 
 def powf(b, e):

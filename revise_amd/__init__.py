@@ -115,6 +115,8 @@ _SIGS = {
     "rv_frame_status": (C.c_int, [_vp, _u32p]),
     "rv_frame_get_tile": (C.c_int, [_vp, _u32p, _u32p]),
     "rv_frame_autotune": (C.c_int, [_vp, _vp, C.c_int, C.c_char_p, _vp, _u32p, _u32p]),
+    "rv_frame_set_grad_bricks": (C.c_int, [_vp, _vp]),
+    "rv_frame_adjoint": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp]),
     "rv_frame_plan_destroy": (None, [_vp]),
     "rv_worker_create": (C.c_int, [C.c_uint32, C.POINTER(C.c_void_p)]),
     "rv_worker_add_subtree": (C.c_int, [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p, C.c_uint64, _f32p]),

@@ -294,6 +294,20 @@ class FramePlan:
         check(lib().rv_frame_get_tile(self.h, C.byref(w), C.byref(h)))
         return w.value, h.value
 
+    def set_grad_bricks(self, grads):
+        """Per-block fp32 gradient arrays (torch tensors, brick layout), or None."""
+        if grads is None:
+            check(lib().rv_frame_set_grad_bricks(self.h, None))
+            self._grads = None
+            return
+        arr = (C.c_void_p * len(grads))(*[g.data_ptr() for g in grads])
+        check(lib().rv_frame_set_grad_bricks(self.h, arr))
+        self._grads = list(grads)
+
+    def adjoint(self, cam, d_rgba, d_grad_rgba, d_grad_tf, stream=None):
+        check(lib().rv_frame_adjoint(self.h, cam, ptr(d_rgba), ptr(d_grad_rgba), ptr(d_grad_tf),
+                                     stream_ptr(stream)))
+
     def autotune(self, d_argb, interpolate=True, cache_path=None, stream=None):
         w, h = C.c_uint32(), C.c_uint32()
         check(lib().rv_frame_autotune(self.h, ptr(d_argb), int(interpolate),

@@ -27,6 +27,7 @@
 // rectangles (camera dependent, one uint4 per block) in front-to-back order,
 // 512 at a time, and compacting the hits into LDS with wave ballots.
 
+#include "frame_plan.hpp"
 #include "interp_device.hpp"
 #include "order.hpp"
 #include "render_common.hpp"
@@ -65,11 +66,7 @@ struct FrameArgs {
     uint32_t* counters;            // [0] queue, [1] error, [2] brick tiles published
 };
 
-constexpr uint32_t kT = 512;
-constexpr uint32_t kMaxTf = 1024;
 constexpr uint32_t kSpinLimit = 1u << 22;
-
-__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 template <class BrickT>
 __device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint32_t* list,
@@ -83,51 +80,20 @@ __device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint
     const bool inside = vx < a.W && vy < a.H;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     bool opaque = !inside;
-    const uint32_t lane = tid & 63u, wave = tid >> 6;
-    bool finished = false;
-    for (uint32_t c = 0; c < a.nblocks && !finished; c += kT) {
-        // compact the blocks of this chunk whose rectangles meet the tile
-        const uint32_t i = c + tid;
-        uint32_t b = 0;
-        bool hit = false;
-        if (i < a.nblocks) {
-            b = a.f2b[i];
-            const uint4 r = a.rects[b];
-            hit = r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0;
-        }
-        const uint64_t ball = __ballot(hit);
-        if (lane == 0)
-            wave_cnt[wave] = (uint32_t)__popcll(ball);
-        __syncthreads();
-        uint32_t off = 0, total = 0;
-        for (uint32_t w = 0; w < kT / 64u; ++w) {
-            const uint32_t n = wave_cnt[w];
-            off += w < wave ? n : 0u;
-            total += n;
-        }
-        if (hit)
-            list[off + (uint32_t)__popcll(ball & ((1ull << lane) - 1ull))] = b;
-        __syncthreads();
-        for (uint32_t k = 0; k < total; ++k) {
-            const uint32_t blk = uniform(list[k]);
-            const rv_block_params p = a.params[blk];
-            if (!opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
-                vy < p.view_y + p.view_h) {
-                const RaySeg r = setup_ray(p, vx, vy);
-                if (r.hit) {
-                    const float4 seg =
-                        march(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), tfs, (int)a.tf_n, r);
-                    blend_behind(acc, seg);
-                    opaque = acc.w > a.acc_cutoff;
-                }
-            }
-            if (__syncthreads_and(opaque)) {
-                finished = true;
-                break;
+    for_tile_blocks(a.rects, a.f2b, a.nblocks, x0, y0, x1, y1, list, wave_cnt, [&](uint32_t blk) {
+        const rv_block_params p = a.params[blk];
+        if (!opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
+            vy < p.view_y + p.view_h) {
+            const RaySeg r = setup_ray(p, vx, vy);
+            if (r.hit) {
+                const float4 seg =
+                    march(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), tfs, (int)a.tf_n, r);
+                blend_behind(acc, seg);
+                opaque = acc.w > a.acc_cutoff;
             }
         }
-        __syncthreads(); // list is reused by the next chunk
-    }
+        return opaque;
+    });
     if (inside) {
         const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
         a.argb[pix] = ((unsigned)(acc.w * 255.999f) << 24) + ((unsigned)(acc.x * 255.999f) << 16) +
@@ -279,45 +245,7 @@ __global__ void frame_reset_kernel(uint32_t* counters, uint32_t n)
 
 using namespace rv;
 
-struct rv_frame_plan {
-    uint32_t nblocks = 0;
-    std::vector<rv_frame_block> blocks;
-    rv_frame_desc desc{};
-    bool fp16 = false;
-    TileTask* d_tasks = nullptr;
-    uint32_t n_interp = 0;
-    const void** d_bricks = nullptr;
-    uint32_t* d_counters = nullptr;
-    uint32_t ncams = 0;
-    std::vector<float> views;
-    rv_block_params* d_params = nullptr;
-    uint4* d_rects = nullptr;
-    uint32_t* d_f2b = nullptr;
-    uint32_t grid = 0;
-    uint32_t tw = 32, th = 16;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    hipStream_t capture_stream = nullptr;
-    bool can_interp = true;
-
-    ~rv_frame_plan()
-    {
-        if (graph_exec)
-            (void)hipGraphExecDestroy(graph_exec);
-        if (graph)
-            (void)hipGraphDestroy(graph);
-        if (capture_stream)
-            (void)hipStreamDestroy(capture_stream);
-        for (void* p : {(void*)d_tasks, (void*)d_bricks, (void*)d_counters,
-                        (void*)d_params, (void*)d_rects, (void*)d_f2b})
-            if (p)
-                (void)hipFree(p);
-    }
-};
-
-namespace {
-
-int upload(void** dst, const void* src, size_t bytes)
+int rv::frame_upload(void** dst, const void* src, size_t bytes)
 {
     if (*dst)
         (void)hipFree(*dst);
@@ -328,6 +256,8 @@ int upload(void** dst, const void* src, size_t bytes)
     RV_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
     return RV_OK;
 }
+
+namespace {
 
 int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba, int interpolate,
                  hipStream_t s, uint32_t tw, uint32_t th)
@@ -406,10 +336,10 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
             return rc;
         RV_REQUIRE(tasks.size() < 0x40000000ull, "rv_frame_plan_create: too many brick tiles");
         P->n_interp = (uint32_t)tasks.size();
-        if ((rc = upload((void**)&P->d_tasks, tasks.data(), sizeof(TileTask) * tasks.size())))
+        if ((rc = frame_upload((void**)&P->d_tasks, tasks.data(), sizeof(TileTask) * tasks.size())))
             return rc;
     }
-    if ((rc = upload((void**)&P->d_bricks, bricks.data(), sizeof(void*) * nblocks)))
+    if ((rc = frame_upload((void**)&P->d_bricks, bricks.data(), sizeof(void*) * nblocks)))
         return rc;
     RV_HIP(hipMalloc((void**)&P->d_counters, 16));
     RV_HIP(hipMemset(P->d_counters, 0, 16));
@@ -477,11 +407,11 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
             f2b[(size_t)c * nb + i] = order[nb - 1 - i];
     }
     int rc;
-    if ((rc = upload((void**)&P->d_params, params.data(), sizeof(rv_block_params) * params.size())))
+    if ((rc = frame_upload((void**)&P->d_params, params.data(), sizeof(rv_block_params) * params.size())))
         return rc;
-    if ((rc = upload((void**)&P->d_rects, rects.data(), sizeof(uint4) * rects.size())))
+    if ((rc = frame_upload((void**)&P->d_rects, rects.data(), sizeof(uint4) * rects.size())))
         return rc;
-    if ((rc = upload((void**)&P->d_f2b, f2b.data(), 4ull * f2b.size())))
+    if ((rc = frame_upload((void**)&P->d_f2b, f2b.data(), 4ull * f2b.size())))
         return rc;
     P->ncams = ncams;
     P->views.assign(views16, views16 + 16ull * ncams);

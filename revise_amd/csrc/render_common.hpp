@@ -234,26 +234,46 @@ __device__ __forceinline__ uint32_t blend_above_u32(uint32_t argb, const float4&
 }
 
 // March one block segment from rgba = 0 (renderArgb.hpp:39-51).
+// The march is latency bound (each sample waits for its 8 brick loads), so
+// the positions and brick fetches of kBatch consecutive samples are issued
+// together and then classified and composited in order; a segment that
+// terminates early discards the rest of its batch.  Positions advance by the
+// same sequential fp32 additions as the reference loop, so every composited
+// sample is bit-identical to the one-at-a-time march.
+constexpr int kBatch = 4;
+
 template <class T>
 __device__ __forceinline__ float4 march(const rv_block_params& p, const T* __restrict__ brick,
                                         const float4* tf, int tf_n, RaySeg r)
 {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     const int n = (int)p.nodes;
-    for (int i = 1; i < r.sample_count; ++i) {
-        const float tx = (r.px - p.tex_origin[0]) * p.tcf;
-        const float ty = (r.py - p.tex_origin[1]) * p.tcf;
-        const float tz = (r.pz - p.tex_origin[2]) * p.tcf;
-        const float v = fetch3(brick, n, tx, ty, tz);
-        if (v == v) {
-            const float4 c = classify(p, tf, tf_n, v);
-            blend_behind(acc, c);
-            if (acc.w > p.alpha_cutoff)
-                break;
+    for (int i = 1; i < r.sample_count; i += kBatch) {
+        float v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            v[k] = __builtin_nanf("");
+            if (i + k < r.sample_count) {
+                const float tx = (r.px - p.tex_origin[0]) * p.tcf;
+                const float ty = (r.py - p.tex_origin[1]) * p.tcf;
+                const float tz = (r.pz - p.tex_origin[2]) * p.tcf;
+                v[k] = fetch3(brick, n, tx, ty, tz);
+                r.px = r.px + r.sx;
+                r.py = r.py + r.sy;
+                r.pz = r.pz + r.sz;
+            }
         }
-        r.px = r.px + r.sx;
-        r.py = r.py + r.sy;
-        r.pz = r.pz + r.sz;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < kBatch; ++k) {
+            if (!done && v[k] == v[k]) {
+                const float4 c = classify(p, tf, tf_n, v[k]);
+                blend_behind(acc, c);
+                done = acc.w > p.alpha_cutoff;
+            }
+        }
+        if (done)
+            break;
     }
     return acc;
 }
