@@ -207,6 +207,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cams", type=int, default=64)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only-render", action="store_true",
+                    help="run the render kernels only: share kernel x3, batched frame (resident "
+                         "bricks, 32x16 tiles) x3 at 1920x1080, then x3 at 512x512 (profiler runs)")
     ap.add_argument("--only-interp", action="store_true",
                     help="time a 1 GiB device copy and the batched interpolation only (profiler runs)")
     args = ap.parse_args()
@@ -246,6 +249,26 @@ def main():
         batch_own = api.InterpBatch([(b["sparse"], pl, b["brick"], b["depth"]) for b, pl in zip(wl.blocks, own)])
         batch_own.run()
         log("batched interp, private plans: %.3f ms" % (events_time(torch, batch_own.run, 10) * 1e3))
+        return
+    if args.only_render:
+        plain.fused_step()
+        for _ in range(3):
+            plain.share.render(plain.vp)
+        fpr = frame_plan(torch, wl, tf, Q2, CUT2, tile=(32, 16))
+        fpr.set_cameras(M[None])
+        argb_r = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        for _ in range(3):
+            fpr.render(0, argb_r, None, False)
+        fpr.close()
+        fps = api.FramePlan(wl.blocks, 512, 512, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf, (32, 16))
+        fps.set_cameras(M[None])
+        argb_s = torch.empty((512, 512), dtype=torch.int32, device="cuda")
+        for _ in range(3):
+            fps.render(0, argb_s, None, False)
+        torch.cuda.synchronize()
+        log("share %.3f ms, frame 1080p %.3f ms" % (events_time(torch, lambda: plain.share.render(plain.vp), 3) * 1e3,
+                                                     events_time(torch, lambda: fps.render(0, argb_s, None, False), 3) * 1e3))
+        fps.close()
         return
     for _ in range(args.warmup):
         plain.fused_step()
@@ -414,6 +437,8 @@ def main():
             gb4.zero_()
             fp4.adjoint(0, rgba4, gI, gtf4)
 
+        t_fwd4_plain = events_time(torch, fwd4, 10)
+        fp4.set_tape(True)  # the forward of a training step records the adjoint's segment states
         fwd4()
         adj4()
         t_fwd4 = events_time(torch, fwd4, 10)
@@ -427,15 +452,17 @@ def main():
             fp4.adjoint(0, rgba4, gI, gtf4)
 
         t_fa4_tf = events_time(torch, lambda: (fwd4(), adj4_tf()), 5)
-        log("config 4: forward %.3f ms, forward+adjoint %.3f ms (ratio %.2f), adjoint %.3f ms, "
-            "colour-table-only forward+adjoint %.3f ms" % (t_fwd4 * 1e3, t_fa4 * 1e3, t_fa4 / t_fwd4,
-                                                           t_adj4 * 1e3, t_fa4_tf * 1e3))
+        log("config 4: forward %.3f ms (with tape %.3f), forward+adjoint %.3f ms (ratio %.2f), adjoint "
+            "%.3f ms, colour-table-only forward+adjoint %.3f ms" % (t_fwd4_plain * 1e3, t_fwd4 * 1e3,
+                                                                    t_fa4 * 1e3, t_fa4 / t_fwd4_plain,
+                                                                    t_adj4 * 1e3, t_fa4_tf * 1e3))
         out["adjoint512"] = {
             "workload": "512 level-3 blocks x 65^3 fp32 (resident), 512x512, L = 1/2 |image - target|^2 "
                         "with the target rendered from the colour table + seeded N(0,0.05); gradients "
                         "w.r.t. the 256x4 table and all 140.6M field values (zeroed every step)",
-            "forward_ms": t_fwd4 * 1e3, "fwd_adjoint_ms": t_fa4 * 1e3, "adjoint_ms": t_adj4 * 1e3,
-            "ratio_fwd_adjoint_over_fwd": t_fa4 / t_fwd4,
+            "forward_ms": t_fwd4_plain * 1e3, "forward_with_tape_ms": t_fwd4 * 1e3,
+            "fwd_adjoint_ms": t_fa4 * 1e3, "adjoint_ms": t_adj4 * 1e3,
+            "ratio_fwd_adjoint_over_fwd": t_fa4 / t_fwd4_plain,
             "tf_only_fwd_adjoint_ms": t_fa4_tf * 1e3}
         fp4.close()
         fp4t.close()

@@ -336,14 +336,20 @@ int rv_frame_autotune(rv_frame_plan* plan, uint32_t* d_argb, int interpolate,
 void rv_frame_plan_destroy(rv_frame_plan* plan);
 
 /* Adjoint ray-cast (no reference counterpart: the reference renders only).
- * Gradients of an image loss L with respect to the colour table and the
- * dense field values, for the frame that rv_frame_render(cam, ..., d_rgba)
- * produced from resident bricks: given d_rgba (that forward image, vp_w*vp_h
- * float4) and d_grad_rgba (dL/d image, same layout), adds dL/d table to
+ * rv_frame_set_tape(plan, 1) makes every later rv_frame_render with a float
+ * image record, per pixel and per block its ray reaches, the pixel's state in
+ * front of the block (premultiplied colour and transmittance, 16 bytes per
+ * pixel of each block's view rectangle; the plan owns the buffer, sized for
+ * the largest camera).  rv_frame_adjoint then gives gradients of an image
+ * loss L with respect to the colour table and the dense field values for the
+ * camera last rendered that way: given d_rgba (that forward image, vp_w*vp_h
+ * float4) and d_grad_rgba (dL/d image, same layout), it adds dL/d table to
  * d_grad_tf (tf_n x 4 floats) and dL/d field to the per-block gradient arrays
  * set with rv_frame_set_grad_bricks (fp32, brick layout; not set = colour
- * table only).  Samples are recomputed per ray, none are stored.  Outputs
- * accumulate: the caller zeroes them.  Calls on one plan must not overlap. */
+ * table only).  Samples are recomputed per ray segment, none are stored.
+ * Outputs accumulate: the caller zeroes them.  Calls on one plan must not
+ * overlap. */
+int rv_frame_set_tape(rv_frame_plan* plan, int enable);
 int rv_frame_set_grad_bricks(rv_frame_plan* plan, float* const* d_grad_bricks);
 int rv_frame_adjoint(rv_frame_plan* plan, uint32_t cam, const float* d_rgba,
                      const float* d_grad_rgba, float* d_grad_tf, rv_stream_t stream);

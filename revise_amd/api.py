@@ -294,6 +294,10 @@ class FramePlan:
         check(lib().rv_frame_get_tile(self.h, C.byref(w), C.byref(h)))
         return w.value, h.value
 
+    def set_tape(self, enable=True):
+        """Record the adjoint's per-segment states in later float-image renders."""
+        check(lib().rv_frame_set_tape(self.h, int(enable)))
+
     def set_grad_bricks(self, grads):
         """Per-block fp32 gradient arrays (torch tensors, brick layout), or None."""
         if grads is None:

@@ -26,10 +26,20 @@
 // linearly interpolated table entry, so da/dt.w = ilc of (1 - a) / (1 - t.w of);
 // t = (1 - f) tf[i0] + f tf[i1] at x = v01 N - 0.5, dt/dv = (tf[i1] - tf[i0]) N
 // / (fmax - fmin) inside the table's range and 0 where x is clamped; v is
-// trilinear in the 8 brick corners.  Colour-table gradients accumulate in LDS
-// per workgroup and are added to the output once per workgroup; field
-// gradients are added to the per-block gradient arrays with float atomics,
-// merged in registers while consecutive samples share a cell.
+// trilinear in the 8 brick corners.
+//
+// Order of work.  The forward records, for every pixel and every block its
+// ray reaches, the pixel's state in front of the block (premultiplied colour
+// and transmittance, 16 bytes; rv_frame_set_tape).  The adjoint then works
+// block by block: one workgroup takes a 16 x 16 tile of one block's view
+// rectangle and re-marches only that block's segments, starting from the
+// recorded state, so all rays of a workgroup are in one block at a time.
+// Colour-table terms are summed in registers while a ray's samples
+// interpolate the same two table entries, then into an LDS copy of the
+// table, added to the output once per workgroup.  Field terms are summed per
+// cell in registers (keeping the 4 corners shared with the next cell when
+// the ray steps to a face neighbour), then into an LDS window per slab of the
+// block, which is added to memory with coalesced atomics (adjoint_item).
 
 #include "frame_plan.hpp"
 
@@ -38,17 +48,20 @@
 namespace rv {
 namespace {
 
+constexpr int kSlab = 8;              // cells per slab along the slab axis
+constexpr uint32_t kWinCap = 16384;   // floats of the LDS gradient window
+
 struct AdjArgs {
     const rv_block_params* params;
-    const uint4* rects;
-    const uint32_t* f2b;
     const void* const* bricks;
     float* const* grad_bricks; // NULL: no field gradient
-    uint32_t nblocks;
+    const uint32_t* items;     // (block, tile) pairs of this camera
+    uint32_t nitems;
+    const uint64_t* rec_off;   // per block: its rectangle's offset in the tape
+    const float4* tape;        // per (block, pixel): state in front of the block, T < 0: none
     const float4* tf;
     uint32_t tf_n;
-    uint32_t W, H, tw, th, ntx, nty;
-    float acc_cutoff;
+    uint32_t W, H;
     const float4* rgba;        // forward image, rows bottom-up
     const float4* grad_rgba;   // dL/dimage, same layout
     float4* grad_tf;           // tf_n entries, accumulated
@@ -56,255 +69,446 @@ struct AdjArgs {
 
 // Per-pixel state of the re-march.
 struct AdjPixel {
-    float acc_w;           // pixel alpha, with the forward's exact operations
-    float T;               // transmittance in front of the current block
+    float T;               // transmittance in front of the current sample's block
     float Px, Py, Pz;      // premultiplied colour of the samples so far
     float Fx, Fy, Fz, Tend;
     float gPx, gPy, gPz, gA;
+};
+
+// Colour-table gradient accumulator of the current pair of table entries.
+struct TfAcc {
+    int i0, i1;  // i0 < 0: empty
+    float4 g0, g1;
+};
+
+__device__ __forceinline__ void tf_flush(TfAcc& t, float4* gtf)
+{
+    if (t.i0 < 0)
+        return;
+    atomicAdd(&gtf[t.i0].x, t.g0.x);
+    atomicAdd(&gtf[t.i0].y, t.g0.y);
+    atomicAdd(&gtf[t.i0].z, t.g0.z);
+    atomicAdd(&gtf[t.i0].w, t.g0.w);
+    atomicAdd(&gtf[t.i1].x, t.g1.x);
+    atomicAdd(&gtf[t.i1].y, t.g1.y);
+    atomicAdd(&gtf[t.i1].z, t.g1.z);
+    atomicAdd(&gtf[t.i1].w, t.g1.w);
+    t.i0 = -1;
+}
+
+// The LDS gradient window of the current slab: node ranges along x, y, z
+// (origin and extent), uniform over the workgroup; ok = false: no window.
+struct Window {
+    int o[3], e[3];
+    bool ok;
 };
 
 // Field-gradient accumulator of the current cell (8 corners).
 struct CellAcc {
     uint32_t key;  // corner-000 index * 8 + clamp pattern, ~0 = empty
     uint32_t a, dx, dy, dz;
+    int x0, y0, z0; // node of corner 000
     float g[8];
 };
 
-__device__ __forceinline__ void cell_flush(CellAcc& c, float* __restrict__ gb)
+// Adds the cell's 8 corner terms to the window where the corner lies inside
+// it (LDS float atomics), else to the brick's gradient in memory.
+__device__ __forceinline__ void cell_flush(CellAcc& c, const Window& w, float* win,
+                                           float* __restrict__ gb)
 {
     if (c.key == 0xFFFFFFFFu)
         return;
     const uint32_t off[8] = {0u, c.dx, c.dy, c.dy + c.dx, c.dz, c.dz + c.dx, c.dz + c.dy,
                              c.dz + c.dy + c.dx};
+    const int ddx = c.dx ? 1 : 0, ddy = c.dy ? 1 : 0, ddz = c.dz ? 1 : 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-        if (c.g[k] != 0.f)
+    for (int k = 0; k < 8; ++k) {
+        if (c.g[k] == 0.f)
+            continue;
+        const int lx = c.x0 + ((k & 1) ? ddx : 0) - w.o[0];
+        const int ly = c.y0 + ((k & 2) ? ddy : 0) - w.o[1];
+        const int lz = c.z0 + ((k & 4) ? ddz : 0) - w.o[2];
+        if (w.ok && (unsigned)lx < (unsigned)w.e[0] && (unsigned)ly < (unsigned)w.e[1] &&
+            (unsigned)lz < (unsigned)w.e[2])
+            atomicAdd(win + (lz * w.e[1] + ly) * w.e[0] + lx, c.g[k]);
+        else
             unsafeAtomicAdd(gb + c.a + off[k], c.g[k]);
+    }
     c.key = 0xFFFFFFFFu;
 }
 
+// Moves the accumulator to the cell of taps t.  A ray usually steps to a face
+// neighbour, which shares 4 corners with the current cell: those terms stay
+// in registers and only the 4 corners left behind are flushed.  A step of
+// +-1 in the corner index is a step along x only if both cells are unclamped
+// in x (dx != 0), likewise +-n along y (dy != 0) and +-n^2 along z (dz !=
+// 0); every other move flushes all 8.
+__device__ __forceinline__ void cell_move(CellAcc& c, uint32_t key, const Taps& t, uint32_t n,
+                                          const Window& w, float* win, float* __restrict__ gb)
+{
+    float ng[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (c.key != 0xFFFFFFFFu && t.dx == c.dx && t.dy == c.dy && t.dz == c.dz) {
+        const int32_t d = (int32_t)(t.a - c.a);
+        const int32_t nn = (int32_t)(n * n);
+        float* g = c.g;
+        if (t.dx && d == 1) {
+            ng[0] = g[1]; ng[2] = g[3]; ng[4] = g[5]; ng[6] = g[7];
+            g[1] = g[3] = g[5] = g[7] = 0.f;
+        } else if (t.dx && d == -1) {
+            ng[1] = g[0]; ng[3] = g[2]; ng[5] = g[4]; ng[7] = g[6];
+            g[0] = g[2] = g[4] = g[6] = 0.f;
+        } else if (t.dy && d == (int32_t)n) {
+            ng[0] = g[2]; ng[1] = g[3]; ng[4] = g[6]; ng[5] = g[7];
+            g[2] = g[3] = g[6] = g[7] = 0.f;
+        } else if (t.dy && d == -(int32_t)n) {
+            ng[2] = g[0]; ng[3] = g[1]; ng[6] = g[4]; ng[7] = g[5];
+            g[0] = g[1] = g[4] = g[5] = 0.f;
+        } else if (t.dz && d == nn) {
+            ng[0] = g[4]; ng[1] = g[5]; ng[2] = g[6]; ng[3] = g[7];
+            g[4] = g[5] = g[6] = g[7] = 0.f;
+        } else if (t.dz && d == -nn) {
+            ng[4] = g[0]; ng[5] = g[1]; ng[6] = g[2]; ng[7] = g[3];
+            g[0] = g[1] = g[2] = g[3] = 0.f;
+        }
+    }
+    cell_flush(c, w, win, gb);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        c.g[k] = ng[k];
+    c.key = key;
+    c.a = t.a;
+    c.dx = t.dx;
+    c.dy = t.dy;
+    c.dz = t.dz;
+    c.x0 = t.x0;
+    c.y0 = t.y0;
+    c.z0 = t.z0;
+}
+
+// Per-ray state of the re-march of one segment.
+struct SegRay {
+    RaySeg r;
+    int i;          // next sample
+    float seg_w;    // the forward's segment alpha (for its termination)
+    float Tseg;     // transmittance through the block so far
+    bool ended;
+    TfAcc tacc;
+    CellAcc cell;
+};
+
+// The samples of one ray segment from sample sg.i on, up to the slab end:
+// along axis A (0 x, 1 y, 2 z) a sample whose cell index passes `lim`
+// (> lim when up, < lim otherwise) is left for the next slab; A < 0: no
+// slab limit.
 template <class T>
-__device__ void adjoint_segment(const rv_block_params& p, const T* __restrict__ brick,
-                                float* __restrict__ gbrick, const float4* tfs, float4* gtf,
-                                int tf_n, RaySeg r, AdjPixel& st)
+__device__ __forceinline__ void march_adjoint(const rv_block_params& p, const T* __restrict__ brick,
+                                              float* __restrict__ gbrick, const float4* tfs,
+                                              float4* gtf, int tf_n, AdjPixel& st, SegRay& sg,
+                                              int A, bool up, int lim, const Window& w, float* win)
 {
     const int n = (int)p.nodes;
     const float span = p.fmax - p.fmin;
-    float seg_w = 0.f, Tseg = 1.f;
-    CellAcc cell;
-    cell.key = 0xFFFFFFFFu;
-    // Samples are fetched two at a time (positions by the forward's
-    // sequential additions), then processed in order.
-    constexpr int K = 2;
-    bool ended = false;
-    for (int i0s = 1; i0s < r.sample_count && !ended; i0s += K) {
-        uint32_t ka[K], kdx[K], kdy[K], kdz[K];
-        float kfx[K], kfy[K], kfz[K], kv[K][8];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            ka[k] = 0u;
-            kdx[k] = kdy[k] = kdz[k] = 0u;
-            kfx[k] = kfy[k] = kfz[k] = 0.f;
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-                kv[k][c] = __builtin_nanf("");
-            if (i0s + k < r.sample_count) {
-                const float tx = (r.px - p.tex_origin[0]) * p.tcf;
-                const float ty = (r.py - p.tex_origin[1]) * p.tcf;
-                const float tz = (r.pz - p.tex_origin[2]) * p.tcf;
-                int x0, x1, y0, y1, z0, z1;
-                axis_taps(tx, n, x0, x1, kfx[k]);
-                axis_taps(ty, n, y0, y1, kfy[k]);
-                axis_taps(tz, n, z0, z1, kfz[k]);
-                const uint32_t a = ((uint32_t)z0 * n + y0) * n + x0;
-                const uint32_t dx = (uint32_t)(x1 - x0);
-                const uint32_t dy = (uint32_t)(y1 - y0) * n;
-                const uint32_t dz = (uint32_t)(z1 - z0) * n * n;
-                ka[k] = a;
-                kdx[k] = dx;
-                kdy[k] = dy;
-                kdz[k] = dz;
-                kv[k][0] = (float)brick[a];
-                kv[k][1] = (float)brick[a + dx];
-                kv[k][2] = (float)brick[a + dy];
-                kv[k][3] = (float)brick[a + dy + dx];
-                kv[k][4] = (float)brick[a + dz];
-                kv[k][5] = (float)brick[a + dz + dx];
-                kv[k][6] = (float)brick[a + dz + dy];
-                kv[k][7] = (float)brick[a + dz + dy + dx];
-                r.px = r.px + r.sx;
-                r.py = r.py + r.sy;
-                r.pz = r.pz + r.sz;
-            }
+    while (!sg.ended && sg.i < sg.r.sample_count) {
+        const float tx = (sg.r.px - p.tex_origin[0]) * p.tcf;
+        const float ty = (sg.r.py - p.tex_origin[1]) * p.tcf;
+        const float tz = (sg.r.pz - p.tex_origin[2]) * p.tcf;
+        if (A >= 0) {
+            int c0, c1;
+            float fdum;
+            axis_taps(A == 0 ? tx : (A == 1 ? ty : tz), n, c0, c1, fdum);
+            if (up ? c0 > lim : c0 < lim)
+                break; // the next slab's
         }
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (ended)
-                break;
-            const float fx = kfx[k], fy = kfy[k], fz = kfz[k];
-            // the trilinear operations of fetch3
-            const float c00 = lerpf(kv[k][0], kv[k][1], fx);
-            const float c10 = lerpf(kv[k][2], kv[k][3], fx);
-            const float c01 = lerpf(kv[k][4], kv[k][5], fx);
-            const float c11 = lerpf(kv[k][6], kv[k][7], fx);
-            const float c0 = lerpf(c00, c10, fy);
-            const float c1 = lerpf(c01, c11, fy);
-            const float v = lerpf(c0, c1, fz);
-            if (!(v == v))
-                continue; // NaN sample, or past the segment's end
-            const uint32_t a = ka[k], dx = kdx[k], dy = kdy[k], dz = kdz[k];
-            // classify, the operations of classify / fetch_tf
-            const float v01 = (v - p.fmin) / span;
-            const float xraw = v01 * (float)tf_n - 0.5f;
-            float x = xraw;
-            if (!(x > -1.f))
-                x = -1.f;
-            if (x > (float)tf_n)
-                x = (float)tf_n;
-            const float fl = floorf(x);
-            const float f = x - fl;
-            const int ti = (int)fl;
-            const int i0 = clampi(ti, 0, tf_n - 1), i1 = clampi(ti + 1, 0, tf_n - 1);
-            const float4 A0 = tfs[i0], A1 = tfs[i1];
-            const float cx = lerpf(A0.x, A1.x, f), cy = lerpf(A0.y, A1.y, f),
-                        cz = lerpf(A0.z, A1.z, f), tw = lerpf(A0.w, A1.w, f);
-            const float base = 1.f - tw * p.opacity_factor;
-            const float alpha = 1.f - pow01(base, p.i_layer_count);
+        sg.r.px = sg.r.px + sg.r.sx;
+        sg.r.py = sg.r.py + sg.r.sy;
+        sg.r.pz = sg.r.pz + sg.r.sz;
+        ++sg.i;
+        float v8[8];
+        const Taps t = fetch_corners(brick, n, tx, ty, tz, v8);
+        const float v = trilinear(v8, t.fx, t.fy, t.fz); // the operations of fetch3
+        if (!(v == v))
+            continue;
+        // classify, the operations of classify / fetch_tf
+        const float v01 = (v - p.fmin) / span;
+        const float xraw = v01 * (float)tf_n - 0.5f;
+        float x = xraw;
+        if (!(x > -1.f))
+            x = -1.f;
+        if (x > (float)tf_n)
+            x = (float)tf_n;
+        const float fl = floorf(x);
+        const float f = x - fl;
+        const int ti = (int)fl;
+        const int i0 = clampi(ti, 0, tf_n - 1), i1 = clampi(ti + 1, 0, tf_n - 1);
+        const float4 A0 = tfs[i0], A1 = tfs[i1];
+        const float cx = lerpf(A0.x, A1.x, f), cy = lerpf(A0.y, A1.y, f),
+                    cz = lerpf(A0.z, A1.z, f), tw = lerpf(A0.w, A1.w, f);
+        const float base = 1.f - tw * p.opacity_factor;
+        const float alpha = 1.f - pow01(base, p.i_layer_count);
 
-            const float Ts = st.T * Tseg;
-            const float wgt = Ts * alpha;
-            st.Px = st.Px + wgt * cx;
-            st.Py = st.Py + wgt * cy;
-            st.Pz = st.Pz + wgt * cz;
-            const float Tn = Ts * (1.f - alpha);
-            const bool behind = Tn > 1e-30f;
-            const float inv = behind ? 1.f / Tn : 0.f;
-            const float Qx = (st.Fx - st.Px) * inv, Qy = (st.Fy - st.Py) * inv,
-                        Qz = (st.Fz - st.Pz) * inv;
-            const float ratio = behind ? st.Tend * inv : 1.f;
-            const float g_alpha =
-                Ts * (((st.gPx * cx + st.gPy * cy) + st.gPz * cz) -
-                      ((st.gPx * Qx + st.gPy * Qy) + st.gPz * Qz) + st.gA * ratio);
-            const float gcx = st.gPx * wgt, gcy = st.gPy * wgt, gcz = st.gPz * wgt;
-            const float gtw = base > 0.f
-                                  ? g_alpha * (p.i_layer_count * p.opacity_factor * (1.f - alpha) / base)
-                                  : 0.f;
-            // colour table
-            const float w0 = 1.f - f;
-            atomicAdd(&gtf[i0].x, w0 * gcx);
-            atomicAdd(&gtf[i0].y, w0 * gcy);
-            atomicAdd(&gtf[i0].z, w0 * gcz);
-            atomicAdd(&gtf[i0].w, w0 * gtw);
-            atomicAdd(&gtf[i1].x, f * gcx);
-            atomicAdd(&gtf[i1].y, f * gcy);
-            atomicAdd(&gtf[i1].z, f * gcz);
-            atomicAdd(&gtf[i1].w, f * gtw);
-            // field
-            if (gbrick && xraw > -1.f && xraw <= (float)tf_n) {
-                const float dxdv = (float)tf_n / span;
-                const float gv = dxdv * ((((A1.x - A0.x) * gcx + (A1.y - A0.y) * gcy) +
-                                          (A1.z - A0.z) * gcz) + (A1.w - A0.w) * gtw);
-                if (gv != 0.f) {
-                    const uint32_t key = a * 8u + (dx ? 1u : 0u) + (dy ? 2u : 0u) + (dz ? 4u : 0u);
-                    if (key != cell.key) {
-                        cell_flush(cell, gbrick);
-                        cell.key = key;
-                        cell.a = a;
-                        cell.dx = dx;
-                        cell.dy = dy;
-                        cell.dz = dz;
-#pragma unroll
-                        for (int c = 0; c < 8; ++c)
-                            cell.g[c] = 0.f;
-                    }
-                    const float ux = 1.f - fx, uy = 1.f - fy, uz = 1.f - fz;
-                    const float gz0 = gv * uz, gz1 = gv * fz;
-                    const float g00 = gz0 * uy, g10 = gz0 * fy, g01 = gz1 * uy, g11 = gz1 * fy;
-                    cell.g[0] += g00 * ux;
-                    cell.g[1] += g00 * fx;
-                    cell.g[2] += g10 * ux;
-                    cell.g[3] += g10 * fx;
-                    cell.g[4] += g01 * ux;
-                    cell.g[5] += g01 * fx;
-                    cell.g[6] += g11 * ux;
-                    cell.g[7] += g11 * fx;
-                }
-            }
-            Tseg = Tseg * (1.f - alpha);
-            // the forward's segment alpha (blend_behind), for its termination
-            const float da = alpha * (1.f - seg_w);
-            seg_w = seg_w + da;
-            if (seg_w > p.alpha_cutoff)
-                ended = true;
+        const float Ts = st.T * sg.Tseg;
+        const float wgt = Ts * alpha;
+        st.Px = st.Px + wgt * cx;
+        st.Py = st.Py + wgt * cy;
+        st.Pz = st.Pz + wgt * cz;
+        const float Tn = Ts * (1.f - alpha);
+        const bool behind = Tn > 1e-30f;
+        const float inv = behind ? 1.f / Tn : 0.f;
+        const float Qx = (st.Fx - st.Px) * inv, Qy = (st.Fy - st.Py) * inv,
+                    Qz = (st.Fz - st.Pz) * inv;
+        const float ratio = behind ? st.Tend * inv : 1.f;
+        const float g_alpha = Ts * (((st.gPx * cx + st.gPy * cy) + st.gPz * cz) -
+                                    ((st.gPx * Qx + st.gPy * Qy) + st.gPz * Qz) + st.gA * ratio);
+        const float gcx = st.gPx * wgt, gcy = st.gPy * wgt, gcz = st.gPz * wgt;
+        const float gtw = base > 0.f
+                              ? g_alpha * (p.i_layer_count * p.opacity_factor * (1.f - alpha) / base)
+                              : 0.f;
+        // colour table
+        TfAcc& ta = sg.tacc;
+        const float w0 = 1.f - f;
+        if (i0 != ta.i0 || i1 != ta.i1) {
+            tf_flush(ta, gtf);
+            ta.i0 = i0;
+            ta.i1 = i1;
+            ta.g0 = make_float4(0.f, 0.f, 0.f, 0.f);
+            ta.g1 = make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        ta.g0.x += w0 * gcx;
+        ta.g0.y += w0 * gcy;
+        ta.g0.z += w0 * gcz;
+        ta.g0.w += w0 * gtw;
+        ta.g1.x += f * gcx;
+        ta.g1.y += f * gcy;
+        ta.g1.z += f * gcz;
+        ta.g1.w += f * gtw;
+        // field
+        if (gbrick && xraw > -1.f && xraw <= (float)tf_n) {
+            const float gv = ((float)tf_n / span) *
+                             ((((A1.x - A0.x) * gcx + (A1.y - A0.y) * gcy) + (A1.z - A0.z) * gcz) +
+                              (A1.w - A0.w) * gtw);
+            if (gv != 0.f) {
+                CellAcc& cell = sg.cell;
+                const uint32_t key =
+                    t.a * 8u + (t.dx ? 1u : 0u) + (t.dy ? 2u : 0u) + (t.dz ? 4u : 0u);
+                if (key != cell.key)
+                    cell_move(cell, key, t, (uint32_t)n, w, win, gbrick);
+                const float ux = 1.f - t.fx, uy = 1.f - t.fy, uz = 1.f - t.fz;
+                const float gz0 = gv * uz, gz1 = gv * t.fz;
+                const float g00 = gz0 * uy, g10 = gz0 * t.fy, g01 = gz1 * uy, g11 = gz1 * t.fy;
+                cell.g[0] += g00 * ux;
+                cell.g[1] += g00 * t.fx;
+                cell.g[2] += g10 * ux;
+                cell.g[3] += g10 * t.fx;
+                cell.g[4] += g01 * ux;
+                cell.g[5] += g01 * t.fx;
+                cell.g[6] += g11 * ux;
+                cell.g[7] += g11 * t.fx;
+            }
+        }
+        sg.Tseg = sg.Tseg * (1.f - alpha);
+        const float da = alpha * (1.f - sg.seg_w);
+        sg.seg_w = sg.seg_w + da;
+        if (sg.seg_w > p.alpha_cutoff)
+            sg.ended = true;
     }
-    if (gbrick)
-        cell_flush(cell, gbrick);
-    const float da = seg_w * (1.f - st.acc_w);
-    st.acc_w = st.acc_w + da;
-    st.T = st.T * Tseg;
 }
 
-template <class BrickT>
-__global__ __launch_bounds__(kT) void adjoint_kernel(AdjArgs a)
+// One work item (one block, one 16 x 16 tile of its view rectangle) for every
+// thread of the workgroup; active: this thread's pixel has a segment in the
+// block.  With field gradients the block is walked in slabs of kSlab cells
+// along the axis the rays advance along most: all rays pause at each slab's
+// far side, their field terms go to an LDS window covering the slab's part of
+// their frustum, and the window is added to the brick's gradient in x-fastest
+// order, so a wave's global atomics fall into a few cache lines (scattered
+// float atomics cost one L2 request per lane).  A window above kWinCap
+// floats, or rays advancing both ways along the axis, fall back to direct
+// atomics.  Barriers are uniform over the workgroup.
+template <class T>
+__device__ void adjoint_item(const rv_block_params& p, const T* __restrict__ brick,
+                             float* __restrict__ gbrick, const float4* tfs, float4* gtf, int tf_n,
+                             bool active, const RaySeg& r, AdjPixel& st, float* win, int* red)
 {
-    __shared__ float4 tfs[kMaxTf];
-    __shared__ float4 gtf[kMaxTf];
-    __shared__ uint32_t list[kT];
-    __shared__ uint32_t wave_cnt[kT / 64u];
-    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT) {
+    const int n = (int)p.nodes;
+    SegRay sg;
+    sg.r = r;
+    sg.i = 1;
+    sg.seg_w = 0.f;
+    sg.Tseg = 1.f;
+    sg.ended = !active;
+    sg.tacc.i0 = -1;
+    sg.cell.key = 0xFFFFFFFFu;
+    Window w;
+    w.ok = false;
+    w.o[0] = w.o[1] = w.o[2] = 0;
+    w.e[0] = w.e[1] = w.e[2] = 0;
+    // slab axis A (uniform): the axis the view follows most; U, V the others
+    const float ax = fabsf(p.n[0]), ay = fabsf(p.n[1]), az = fabsf(p.n[2]);
+    const int A = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    const int U = A == 0 ? 1 : 0, V = 3 - A - U;
+    bool up = true, slabs = false;
+    if (gbrick) {
+        const float sA = A == 0 ? r.sx : (A == 1 ? r.sy : r.sz);
+        if (threadIdx.x == 0) {
+            red[0] = 0;
+            red[1] = 0;
+        }
+        __syncthreads();
+        if (active) {
+            if (sA > 0.f)
+                red[0] = 1;
+            else
+                red[1] = 1;
+        }
+        __syncthreads();
+        up = red[0] != 0;
+        slabs = (red[0] != 0) != (red[1] != 0);
+        __syncthreads(); // red is rewritten below
+    }
+    const int ns = slabs ? (n + kSlab - 1) / kSlab : 1;
+    for (int k = 0; k < ns; ++k) {
+        const int s = up ? k : ns - 1 - k;
+        const int lo = s * kSlab;
+        const int hi = min(lo + kSlab - 1, n - 1); // cells lo..hi along A
+        if (slabs) {
+            // window bounds: the part of each ray's remaining samples in this
+            // slab, from its position and step in texel units, widened by one
+            // node for the fp32 accumulation of positions
+            if (threadIdx.x == 0) {
+                red[2] = 0x7fffffff;
+                red[3] = -1;
+                red[4] = 0x7fffffff;
+                red[5] = -1;
+            }
+            __syncthreads();
+            if (!sg.ended && sg.i < sg.r.sample_count) {
+                const float tx = (sg.r.px - p.tex_origin[0]) * p.tcf;
+                const float ty = (sg.r.py - p.tex_origin[1]) * p.tcf;
+                const float tz = (sg.r.pz - p.tex_origin[2]) * p.tcf;
+                const float dx = sg.r.sx * p.tcf, dy = sg.r.sy * p.tcf, dz = sg.r.sz * p.tcf;
+                const float tA = A == 0 ? tx : (A == 1 ? ty : tz), dA = A == 0 ? dx : (A == 1 ? dy : dz);
+                const float tU = U == 0 ? tx : ty, dU = U == 0 ? dx : dy;
+                const float tV = V == 1 ? ty : tz, dV = V == 1 ? dy : dz;
+                const float rem = (float)(sg.r.sample_count - 1 - sg.i);
+                float la = ((float)lo + 0.5f - tA) / dA;
+                float lb = ((float)hi + 1.5f - tA) / dA;
+                if (la > lb) {
+                    const float q = la;
+                    la = lb;
+                    lb = q;
+                }
+                la = fmaxf(la, 0.f);
+                lb = fminf(lb, rem);
+                if (la <= lb) {
+                    const float ua = tU + la * dU, ub = tU + lb * dU;
+                    const float va = tV + la * dV, vb = tV + lb * dV;
+                    atomicMin(&red[2], max((int)floorf(fminf(ua, ub) - 0.5f) - 1, 0));
+                    atomicMax(&red[3], min((int)floorf(fmaxf(ua, ub) - 0.5f) + 2, n - 1));
+                    atomicMin(&red[4], max((int)floorf(fminf(va, vb) - 0.5f) - 1, 0));
+                    atomicMax(&red[5], min((int)floorf(fmaxf(va, vb) - 0.5f) + 2, n - 1));
+                }
+            }
+            __syncthreads();
+            const int nl = min(hi + 1, n - 1) - lo + 1;
+            const int nu = red[3] - red[2] + 1, nv = red[5] - red[4] + 1;
+            w.ok = red[3] >= red[2] && red[5] >= red[4] && nu * nv * nl <= (int)kWinCap;
+            if (w.ok) {
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    w.o[j] = j == A ? lo : (j == U ? red[2] : red[4]);
+                    w.e[j] = j == A ? nl : (j == U ? nu : nv);
+                }
+                const int size = nu * nv * nl;
+                for (int j = threadIdx.x; j < size; j += (int)blockDim.x)
+                    win[j] = 0.f;
+            }
+            __syncthreads();
+        }
+        march_adjoint(p, brick, gbrick, tfs, gtf, tf_n, st, sg, slabs ? A : -1, up, up ? hi : lo, w,
+                      win);
+        if (gbrick)
+            cell_flush(sg.cell, w, win, gbrick); // the slab's last cell
+        if (!slabs)
+            break;
+        __syncthreads();
+        if (w.ok) {
+            // the window, x fastest, to the brick's gradient
+            const int ex = w.e[0], exy = w.e[0] * w.e[1], size = exy * w.e[2];
+            for (int j = threadIdx.x; j < size; j += (int)blockDim.x) {
+                const float g = win[j];
+                if (g == 0.f)
+                    continue;
+                const int lz = j / exy, rxy = j - lz * exy;
+                const int ly = rxy / ex, lx = rxy - ly * ex;
+                unsafeAtomicAdd(gbrick + ((uint32_t)(w.o[2] + lz) * n + (uint32_t)(w.o[1] + ly)) * n +
+                                    (uint32_t)(w.o[0] + lx),
+                                g);
+            }
+        }
+        __syncthreads();
+    }
+    tf_flush(sg.tacc, gtf);
+}
+
+// Persistent grid; work items (block, tile) in order, tiles of a block
+// consecutive.
+template <class BrickT>
+__global__ __launch_bounds__(kRT) void adjoint_kernel(AdjArgs a)
+{
+    extern __shared__ float4 dyn[]; // colour table, its gradient, gradient window
+    float4* tfs = dyn;
+    float4* gtf = dyn + a.tf_n;
+    float* win = reinterpret_cast<float*>(dyn + 2 * a.tf_n);
+    __shared__ int red[8];
+    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT) {
         tfs[i] = a.tf[i];
         gtf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
     __syncthreads();
-    const uint32_t ntiles = a.ntx * a.nty;
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const uint32_t tid = threadIdx.x;
-        const uint32_t tx = tile % a.ntx, ty = tile / a.ntx;
-        const uint32_t x0 = tx * a.tw, y0 = ty * a.th;
-        const uint32_t x1 = min(x0 + a.tw, a.W), y1 = min(y0 + a.th, a.H);
-        const uint32_t vx = x0 + tid % a.tw, vy = y0 + tid / a.tw;
-        const bool inside = vx < a.W && vy < a.H;
+    for (uint32_t item = blockIdx.x; item < a.nitems; item += gridDim.x) {
+        const uint32_t blk = a.items[2 * item], tile = a.items[2 * item + 1];
+        const rv_block_params p = a.params[blk];
+        const uint32_t ntx = (p.view_w + 15u) >> 4;
+        uint32_t px, py;
+        tile_pixel(threadIdx.x, 16u, 16u, px, py);
+        const uint32_t lx = (tile % ntx) * 16u + px, ly = (tile / ntx) * 16u + py;
         AdjPixel st{};
-        bool done = true;
-        if (inside) {
-            const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
-            const float4 I = a.rgba[pix];
-            const float4 g = a.grad_rgba[pix];
-            st.acc_w = 0.f;
-            st.T = 1.f;
-            st.Fx = I.x * I.w;
-            st.Fy = I.y * I.w;
-            st.Fz = I.z * I.w;
-            st.Tend = 1.f - I.w;
-            if (I.w > 0.f) {
-                st.gPx = g.x / I.w;
-                st.gPy = g.y / I.w;
-                st.gPz = g.z / I.w;
-                st.gA = g.w - ((g.x * I.x + g.y * I.y) + g.z * I.z) / I.w;
-            } else {
-                st.gA = g.w;
+        RaySeg r{};
+        bool active = false;
+        if (lx < p.view_w && ly < p.view_h) {
+            const float4 rec = a.tape[a.rec_off[blk] + ly * p.view_w + lx];
+            if (rec.w >= 0.f) { // the pixel's ray reaches this block
+                const uint32_t vx = p.view_x + lx, vy = p.view_y + ly;
+                const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
+                const float4 I = a.rgba[pix];
+                const float4 g = a.grad_rgba[pix];
+                st.T = rec.w;
+                st.Px = rec.x;
+                st.Py = rec.y;
+                st.Pz = rec.z;
+                st.Fx = I.x * I.w;
+                st.Fy = I.y * I.w;
+                st.Fz = I.z * I.w;
+                st.Tend = 1.f - I.w;
+                if (I.w > 0.f) {
+                    st.gPx = g.x / I.w;
+                    st.gPy = g.y / I.w;
+                    st.gPz = g.z / I.w;
+                    st.gA = g.w - ((g.x * I.x + g.y * I.y) + g.z * I.z) / I.w;
+                } else {
+                    st.gA = g.w;
+                }
+                if (st.gPx != 0.f || st.gPy != 0.f || st.gPz != 0.f || st.gA != 0.f) {
+                    r = setup_ray(p, vx, vy);
+                    active = r.hit;
+                }
             }
-            done = st.gPx == 0.f && st.gPy == 0.f && st.gPz == 0.f && st.gA == 0.f;
         }
-        for_tile_blocks(a.rects, a.f2b, a.nblocks, x0, y0, x1, y1, list, wave_cnt,
-                        [&](uint32_t blk) {
-                            const rv_block_params p = a.params[blk];
-                            if (!done && vx >= p.view_x && vx < p.view_x + p.view_w &&
-                                vy >= p.view_y && vy < p.view_y + p.view_h) {
-                                const RaySeg r = setup_ray(p, vx, vy);
-                                if (r.hit) {
-                                    adjoint_segment(p, reinterpret_cast<const BrickT*>(a.bricks[blk]),
-                                                    a.grad_bricks ? a.grad_bricks[blk] : nullptr,
-                                                    tfs, gtf, (int)a.tf_n, r, st);
-                                    done = st.acc_w > a.acc_cutoff;
-                                }
-                            }
-                            return done;
-                        });
+        adjoint_item(p, reinterpret_cast<const BrickT*>(a.bricks[blk]),
+                     a.grad_bricks ? a.grad_bricks[blk] : nullptr, tfs, gtf, (int)a.tf_n, active, r,
+                     st, win, red);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT) {
+    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT) {
         const float4 g = gtf[i];
         float* out = reinterpret_cast<float*>(a.grad_tf + i);
         if (g.x != 0.f)
@@ -322,6 +526,8 @@ __global__ __launch_bounds__(kT) void adjoint_kernel(AdjArgs a)
 } // namespace rv
 
 using namespace rv;
+
+static size_t adjoint_lds(uint32_t tf_n) { return 32ull * tf_n + 4ull * kWinCap; }
 
 extern "C" int rv_frame_set_grad_bricks(rv_frame_plan* P, float* const* d_grad_bricks)
 {
@@ -350,36 +556,41 @@ extern "C" int rv_frame_adjoint(rv_frame_plan* P, uint32_t cam, const float* d_r
         RV_HIP(hipGetDevice(&dev));
         RV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         if (P->fp16)
-            RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<__half>, kT, 0));
+            RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<__half>, kRT,
+                                                                adjoint_lds(P->desc.tf_n)));
         else
-            RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<float>, kT, 0));
+            RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<float>, kRT,
+                                                                adjoint_lds(P->desc.tf_n)));
         RV_REQUIRE(per_cu >= 1, "rv_frame_adjoint: kernel cannot be resident");
         P->adj_grid = (uint32_t)(cus * per_cu);
     }
+    RV_REQUIRE(P->tape_on && P->tape_cam == (int64_t)cam,
+               "rv_frame_adjoint: render this camera with the tape on (rv_frame_set_tape) and a float "
+               "image first");
     AdjArgs a{};
     a.params = P->d_params + (size_t)cam * P->nblocks;
-    a.rects = P->d_rects + (size_t)cam * P->nblocks;
-    a.f2b = P->d_f2b + (size_t)cam * P->nblocks;
     a.bricks = P->d_bricks;
     a.grad_bricks = P->d_grad_bricks;
-    a.nblocks = P->nblocks;
+    a.items = P->d_items + 2 * P->item_off[cam];
+    a.nitems = (uint32_t)(P->item_off[cam + 1] - P->item_off[cam]);
+    a.rec_off = P->d_rec_off + (size_t)cam * P->nblocks;
+    a.tape = P->d_tape;
     a.tf = reinterpret_cast<const float4*>(P->desc.d_tf);
     a.tf_n = P->desc.tf_n;
     a.W = P->desc.vp_w;
     a.H = P->desc.vp_h;
-    a.tw = P->tw;
-    a.th = P->th;
-    a.ntx = (a.W + a.tw - 1) / a.tw;
-    a.nty = (a.H + a.th - 1) / a.th;
-    a.acc_cutoff = P->desc.acc_cutoff;
     a.rgba = reinterpret_cast<const float4*>(d_rgba);
     a.grad_rgba = reinterpret_cast<const float4*>(d_grad_rgba);
     a.grad_tf = reinterpret_cast<float4*>(d_grad_tf);
-    const uint32_t grid = std::min(P->adj_grid, a.ntx * a.nty);
+    if (a.nitems == 0)
+        return RV_OK;
+    const uint32_t grid = P->adj_grid;
     if (P->fp16)
-        hipLaunchKernelGGL(adjoint_kernel<__half>, dim3(grid), dim3(kT), 0, S(stream), a);
+        hipLaunchKernelGGL(adjoint_kernel<__half>, dim3(grid), dim3(kRT), adjoint_lds(a.tf_n),
+                           S(stream), a);
     else
-        hipLaunchKernelGGL(adjoint_kernel<float>, dim3(grid), dim3(kT), 0, S(stream), a);
+        hipLaunchKernelGGL(adjoint_kernel<float>, dim3(grid), dim3(kRT), adjoint_lds(a.tf_n),
+                           S(stream), a);
     RV_LAUNCH_CHECK();
     return RV_OK;
     RV_CATCH

@@ -64,11 +64,13 @@ struct FrameArgs {
     uint32_t* argb;                // W*H, rows bottom-up like the reference viewport
     float4* rgba;                  // optional float output, same layout
     uint32_t* counters;            // [0] queue, [1] error, [2] brick tiles published
+    float4* tape;                  // adjoint record per (block, pixel of its rectangle), or NULL
+    const uint64_t* rec_off;       // per block: offset of its rectangle in the tape
 };
 
 constexpr uint32_t kSpinLimit = 1u << 22;
 
-template <class BrickT>
+template <class BrickT, uint32_t NT>
 __device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint32_t* list,
                             uint32_t* wave_cnt)
 {
@@ -76,18 +78,31 @@ __device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint
     const uint32_t tx = tile % a.ntx, ty = tile / a.ntx;
     const uint32_t x0 = tx * a.tw, y0 = ty * a.th;
     const uint32_t x1 = min(x0 + a.tw, a.W), y1 = min(y0 + a.th, a.H);
-    const uint32_t vx = x0 + tid % a.tw, vy = y0 + tid / a.tw;
+    uint32_t px, py;
+    tile_pixel(tid, a.tw, a.th, px, py);
+    const uint32_t vx = x0 + px, vy = y0 + py;
     const bool inside = vx < a.W && vy < a.H;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float trans = 1.f; // product of (1 - alpha) in front of the current block (tape only)
     bool opaque = !inside;
-    for_tile_blocks(a.rects, a.f2b, a.nblocks, x0, y0, x1, y1, list, wave_cnt, [&](uint32_t blk) {
+    for_tile_blocks<NT>(a.rects, a.f2b, a.nblocks, x0, y0, x1, y1, list, wave_cnt, [&](uint32_t blk) {
         const rv_block_params p = a.params[blk];
         if (!opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
             vy < p.view_y + p.view_h) {
             const RaySeg r = setup_ray(p, vx, vy);
             if (r.hit) {
-                const float4 seg =
-                    march(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), tfs, (int)a.tf_n, r);
+                const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);
+                float4 seg;
+                if (a.tape) {
+                    // the pixel's state in front of this block, for the adjoint
+                    a.tape[a.rec_off[blk] + (vy - p.view_y) * p.view_w + (vx - p.view_x)] =
+                        make_float4(acc.x * acc.w, acc.y * acc.w, acc.z * acc.w, trans);
+                    float ts;
+                    seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
+                    trans = trans * ts;
+                } else {
+                    seg = march(p, brick, tfs, (int)a.tf_n, r);
+                }
                 blend_behind(acc, seg);
                 opaque = acc.w > a.acc_cutoff;
             }
@@ -163,9 +178,35 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
             __syncthreads();
             tf_loaded = true;
         }
-        render_tile<BrickT>(a, item - a.n_interp, tfs, list, wave_cnt);
+        render_tile<BrickT, kT>(a, item - a.n_interp, tfs, list, wave_cnt);
     }
     publish_tiles(a.counters + 2, my_tiles);
+}
+
+// Frames of resident bricks: screen tiles only, 16 x 16 pixels per
+// workgroup of kRT threads, no interpolation LDS, so more workgroups per CU
+// and more tiles for small viewports.  Tiles are taken from the queue
+// counter like the brick-and-screen kernel.
+template <class BrickT>
+__global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
+{
+    __shared__ float4 tfs[kMaxTf];
+    __shared__ uint32_t list[kRT];
+    __shared__ uint32_t wave_cnt[kRT / 64u];
+    __shared__ uint32_t item_s;
+    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT)
+        tfs[i] = a.tf[i];
+    const uint32_t ntiles = a.ntx * a.nty;
+    for (;;) {
+        if (threadIdx.x == 0)
+            item_s = atomicAdd(a.counters, 1u);
+        __syncthreads();
+        const uint32_t item = item_s;
+        __syncthreads();
+        if (item >= ntiles)
+            break;
+        render_tile<BrickT, kRT>(a, item, tfs, list, wave_cnt);
+    }
 }
 
 // Worker share in one launch: the blocks (back to front) of one share are
@@ -191,7 +232,9 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
     const uint32_t tid = threadIdx.x;
     const uint32_t x0 = blockIdx.x * 16u, y0 = blockIdx.y * 16u;
     const uint32_t x1 = min(x0 + 16u, W), y1 = min(y0 + 16u, H);
-    const uint32_t vx = x0 + (tid & 15u), vy = y0 + (tid >> 4);
+    uint32_t px, py;
+    tile_pixel(tid, 16u, 16u, px, py);
+    const uint32_t vx = x0 + px, vy = y0 + py;
     const bool inside = vx < W && vy < H;
     const uint32_t pix = inside ? (H - 1u - vy) * W + vx : 0u;
     uint32_t value = inside ? viewport[pix] : 0u;
@@ -274,6 +317,10 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     a.tf_n = P->desc.tf_n;
     a.W = P->desc.vp_w;
     a.H = P->desc.vp_h;
+    if (!interpolate) {
+        tw = 16;
+        th = 16;
+    }
     a.tw = tw;
     a.th = th;
     a.ntx = (a.W + tw - 1) / tw;
@@ -282,9 +329,24 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     a.argb = d_argb;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
     a.counters = P->d_counters;
+    if (P->tape_on && d_rgba) {
+        // records of blocks a pixel never reaches keep T = -1
+        const uint64_t nrec = P->rec_total[cam];
+        int rc = fill_u32(reinterpret_cast<uint32_t*>(P->d_tape), 4 * nrec, 0xBF800000u, s);
+        if (rc)
+            return rc;
+        a.tape = P->d_tape;
+        a.rec_off = P->d_rec_off + (size_t)cam * P->nblocks;
+        P->tape_cam = cam;
+    }
     hipLaunchKernelGGL(frame_reset_kernel, dim3(1), dim3(64), 0, s, P->d_counters, 4u);
     RV_LAUNCH_CHECK();
-    if (P->fp16)
+    if (!interpolate) {
+        if (P->fp16)
+            hipLaunchKernelGGL(render_frame_kernel<__half>, dim3(P->render_grid), dim3(kRT), 0, s, a);
+        else
+            hipLaunchKernelGGL(render_frame_kernel<float>, dim3(P->render_grid), dim3(kRT), 0, s, a);
+    } else if (P->fp16)
         hipLaunchKernelGGL(frame_kernel<__half>, dim3(P->grid), dim3(kT), 0, s, a);
     else
         hipLaunchKernelGGL(frame_kernel<float>, dim3(P->grid), dim3(kT), 0, s, a);
@@ -353,6 +415,12 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<float>, kT, 0));
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->grid = (uint32_t)(cus * per_cu);
+    if (fp16)
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<__half>, kRT, 0));
+    else
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<float>, kRT, 0));
+    RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
+    P->render_grid = (uint32_t)(cus * per_cu);
     if (desc->tile_w && desc->tile_h) {
         RV_REQUIRE(desc->tile_w * desc->tile_h == kT, "rv_frame_plan_create: tile_w*tile_h must be 512");
         P->tw = desc->tile_w;
@@ -413,8 +481,65 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
         return rc;
     if ((rc = frame_upload((void**)&P->d_f2b, f2b.data(), 4ull * f2b.size())))
         return rc;
+    // adjoint tables: tape offsets of the view rectangles, (block, tile) pairs
+    std::vector<uint64_t> rec_off((size_t)ncams * nb);
+    std::vector<uint32_t> items;
+    P->rec_total.assign(ncams, 0);
+    P->item_off.assign(ncams + 1, 0);
+    for (uint32_t c = 0; c < ncams; ++c) {
+        uint64_t off = 0;
+        for (uint32_t i = 0; i < nb; ++i) {
+            const uint4 r = rects[(size_t)c * nb + i];
+            rec_off[(size_t)c * nb + i] = off;
+            const uint64_t w = r.z - r.x, h = r.w - r.y;
+            off += w * h;
+            const uint32_t tx = (uint32_t)((w + 15) / 16), ty = (uint32_t)((h + 15) / 16);
+            for (uint32_t t = 0; t < tx * ty; ++t) {
+                items.push_back(i);
+                items.push_back(t);
+            }
+        }
+        P->rec_total[c] = off;
+        P->item_off[c + 1] = items.size() / 2;
+    }
+    if ((rc = frame_upload((void**)&P->d_rec_off, rec_off.data(), 8ull * rec_off.size())))
+        return rc;
+    if ((rc = frame_upload((void**)&P->d_items, items.data(), 4ull * items.size())))
+        return rc;
+    P->tape_cam = -1;
     P->ncams = ncams;
+    if (P->tape_on) {
+        const int rc2 = rv_frame_set_tape(P, 1);
+        if (rc2)
+            return rc2;
+    }
     P->views.assign(views16, views16 + 16ull * ncams);
+    return RV_OK;
+    RV_CATCH
+}
+
+extern "C" int rv_frame_set_tape(rv_frame_plan* P, int enable)
+{
+    RV_TRY
+    RV_REQUIRE(P, "rv_frame_set_tape: null plan");
+    P->tape_cam = -1;
+    if (!enable) {
+        P->tape_on = false;
+        return RV_OK;
+    }
+    RV_REQUIRE(P->ncams > 0, "rv_frame_set_tape: set cameras first");
+    uint64_t need = 1;
+    for (uint64_t v : P->rec_total)
+        need = std::max(need, v);
+    if (need > P->tape_cap) {
+        if (P->d_tape)
+            RV_HIP(hipFree(P->d_tape));
+        P->d_tape = nullptr;
+        P->tape_cap = 0;
+        RV_HIP(hipMalloc((void**)&P->d_tape, sizeof(float4) * need));
+        P->tape_cap = need;
+    }
+    P->tape_on = true;
     return RV_OK;
     RV_CATCH
 }

@@ -3,7 +3,7 @@
 // sub-image blending for gfx950.
 //
 // The block kernel runs one thread per pixel of the block's view rectangle in
-// 16 x 16 pixel workgroups (one wave = 16 x 4 pixels, so a wave's rays are a
+// 16 x 16 pixel workgroups (one wave = 8 x 8 pixels, so a wave's rays are a
 // compact bundle through the brick and share L1/L2 lines of the brick).  The
 // colour table is staged in LDS as float4.  Every arithmetic step follows the
 // reference's operand order (see render_common.hpp), so the 8-bit viewport is
@@ -30,7 +30,8 @@ __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
     for (uint32_t i = threadIdx.x; i < tf_n; i += 256u)
         tfs[i] = reinterpret_cast<const float4*>(tf)[i];
     __syncthreads();
-    const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+    uint32_t lx, ly;
+    tile_pixel(threadIdx.x, 16u, 16u, lx, ly);
     const uint32_t ox = blockIdx.x * 16u + lx, oy = blockIdx.y * 16u + ly;
     if (ox >= p.view_w || oy >= p.view_h)
         return;
@@ -95,7 +96,8 @@ __global__ __launch_bounds__(256) void bound_rect_kernel(const uint32_t* __restr
 __global__ __launch_bounds__(256) void sample_count_kernel(rv_block_params p,
                                                            unsigned long long* out)
 {
-    const uint32_t lx = threadIdx.x & 15u, ly = threadIdx.x >> 4;
+    uint32_t lx, ly;
+    tile_pixel(threadIdx.x, 16u, 16u, lx, ly);
     const uint32_t ox = blockIdx.x * 16u + lx, oy = blockIdx.y * 16u + ly;
     unsigned long long c = 0;
     if (ox < p.view_w && oy < p.view_h) {

@@ -6,6 +6,7 @@
 
 #include "../../include/revise_hip.h"
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 namespace rv {
@@ -83,31 +84,102 @@ __device__ __forceinline__ void axis_taps(float t, int n, int& i0, int& i1, floa
     i1 = clampi(i + 1, 0, n - 1);
 }
 
+// The two x-neighbours brick[i], brick[i+1] in one 8-byte (fp32) or 4-byte
+// (fp16) load: the brick is x fastest, so the corners (x0, x0+1) of a cell
+// are adjacent, and a wave's corner loads touch half as many cache lines as
+// with one load per corner.
+__device__ __forceinline__ void load_pair(const float* __restrict__ b, uint32_t i, float& lo, float& hi)
+{
+    float2 v;
+    __builtin_memcpy(&v, b + i, sizeof v);
+    lo = v.x;
+    hi = v.y;
+}
+
+__device__ __forceinline__ void load_pair(const __half* __restrict__ b, uint32_t i, float& lo, float& hi)
+{
+    __half2 v;
+    __builtin_memcpy(&v, b + i, sizeof v);
+    lo = __low2float(v);
+    hi = __high2float(v);
+}
+
+// The 8 corner values of the trilinear fetch at texel coordinates (tx, ty,
+// tz): taps and weights of axis_taps, corner c = bit0 x, bit1 y, bit2 z, and
+// the index arithmetic the adjoint scatters through (a + {0, dx} + {0, dy} +
+// {0, dz}).  x-pairs are loaded at bx = min(x0, n-2) (n >= 2) and the two
+// corners picked from the pair, so no load leaves the brick.
+struct Taps {
+    uint32_t a, dx, dy, dz;
+    float fx, fy, fz;
+    int x0, y0, z0; // node of corner 000
+};
+
+template <class T>
+__device__ __forceinline__ Taps fetch_corners(const T* __restrict__ brick, int n, float tx, float ty,
+                                              float tz, float v[8])
+{
+    int x0, x1, y0, y1, z0, z1;
+    Taps t;
+    axis_taps(tx, n, x0, x1, t.fx);
+    axis_taps(ty, n, y0, y1, t.fy);
+    axis_taps(tz, n, z0, z1, t.fz);
+    t.x0 = x0;
+    t.y0 = y0;
+    t.z0 = z0;
+    t.a = ((uint32_t)z0 * n + y0) * n + x0;
+    t.dx = (uint32_t)(x1 - x0);
+    t.dy = (uint32_t)(y1 - y0) * n;
+    t.dz = (uint32_t)(z1 - z0) * n * n;
+    const int bx = x0 < n - 2 ? x0 : n - 2;
+    const uint32_t row = t.a - (uint32_t)x0 + (uint32_t)bx;
+    const bool l0 = x0 == bx, l1 = x1 == bx;
+    const uint32_t off[4] = {0u, t.dy, t.dz, t.dz + t.dy};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float p0, p1;
+        load_pair(brick, row + off[k], p0, p1);
+        v[2 * k] = l0 ? p0 : p1;
+        v[2 * k + 1] = l1 ? p0 : p1;
+    }
+    return t;
+}
+
 // Exact fp32 trilinear fetch (the tex3D of RenderFuncInput.hpp:46-49).
+__device__ __forceinline__ float trilinear(const float v[8], float fx, float fy, float fz)
+{
+    const float c00 = lerpf(v[0], v[1], fx);
+    const float c10 = lerpf(v[2], v[3], fx);
+    const float c01 = lerpf(v[4], v[5], fx);
+    const float c11 = lerpf(v[6], v[7], fx);
+    const float c0 = lerpf(c00, c10, fy);
+    const float c1 = lerpf(c01, c11, fy);
+    return lerpf(c0, c1, fz);
+}
+
 template <class T>
 __device__ __forceinline__ float fetch3(const T* __restrict__ brick, int n, float tx, float ty,
                                         float tz)
 {
-    int x0, x1, y0, y1, z0, z1;
-    float fx, fy, fz;
-    axis_taps(tx, n, x0, x1, fx);
-    axis_taps(ty, n, y0, y1, fy);
-    axis_taps(tz, n, z0, z1, fz);
-    const uint32_t a = ((uint32_t)z0 * n + y0) * n + x0;
-    const uint32_t dx = (uint32_t)(x1 - x0);
-    const uint32_t dy = (uint32_t)(y1 - y0) * n;
-    const uint32_t dz = (uint32_t)(z1 - z0) * n * n;
-    const float v000 = (float)brick[a], v100 = (float)brick[a + dx];
-    const float v010 = (float)brick[a + dy], v110 = (float)brick[a + dy + dx];
-    const float v001 = (float)brick[a + dz], v101 = (float)brick[a + dz + dx];
-    const float v011 = (float)brick[a + dz + dy], v111 = (float)brick[a + dz + dy + dx];
-    const float c00 = lerpf(v000, v100, fx);
-    const float c10 = lerpf(v010, v110, fx);
-    const float c01 = lerpf(v001, v101, fx);
-    const float c11 = lerpf(v011, v111, fx);
-    const float c0 = lerpf(c00, c10, fy);
-    const float c1 = lerpf(c01, c11, fy);
-    return lerpf(c0, c1, fz);
+    float v[8];
+    const Taps t = fetch_corners(brick, n, tx, ty, tz, v);
+    return trilinear(v, t.fx, t.fy, t.fz);
+}
+
+// Pixel of thread tid in a tw x th screen tile: each wave covers an 8 x 8
+// square when both extents are multiples of 8 (a compact ray bundle, fewer
+// brick cache lines per load), else rows of the tile.
+__device__ __forceinline__ void tile_pixel(uint32_t tid, uint32_t tw, uint32_t th, uint32_t& px,
+                                           uint32_t& py)
+{
+    if ((tw & 7u) == 0u && (th & 7u) == 0u) {
+        const uint32_t w = tid >> 6, l = tid & 63u, sq = tw >> 3;
+        px = (w % sq) * 8u + (l & 7u);
+        py = (w / sq) * 8u + (l >> 3);
+    } else {
+        px = tid % tw;
+        py = tid / tw;
+    }
 }
 
 // Normalized, linear, clamped 1-D fetch of the colour table (tex1D,
@@ -242,11 +314,15 @@ __device__ __forceinline__ uint32_t blend_above_u32(uint32_t argb, const float4&
 // sample is bit-identical to the one-at-a-time march.
 constexpr int kBatch = 4;
 
-template <class T>
-__device__ __forceinline__ float4 march(const rv_block_params& p, const T* __restrict__ brick,
-                                        const float4* tf, int tf_n, RaySeg r)
+// kTrans: also return in *trans the product of (1 - alpha) over the
+// composited samples (the segment's transmittance, as a product rather than
+// 1 - alpha, for the adjoint's record of the pixel state at each block).
+template <class T, bool kTrans>
+__device__ __forceinline__ float4 march_impl(const rv_block_params& p, const T* __restrict__ brick,
+                                             const float4* tf, int tf_n, RaySeg r, float* trans)
 {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float tr = 1.f;
     const int n = (int)p.nodes;
     for (int i = 1; i < r.sample_count; i += kBatch) {
         float v[kBatch];
@@ -269,13 +345,24 @@ __device__ __forceinline__ float4 march(const rv_block_params& p, const T* __res
             if (!done && v[k] == v[k]) {
                 const float4 c = classify(p, tf, tf_n, v[k]);
                 blend_behind(acc, c);
+                if (kTrans)
+                    tr = tr * (1.f - c.w);
                 done = acc.w > p.alpha_cutoff;
             }
         }
         if (done)
             break;
     }
+    if (kTrans)
+        *trans = tr;
     return acc;
+}
+
+template <class T>
+__device__ __forceinline__ float4 march(const rv_block_params& p, const T* __restrict__ brick,
+                                        const float4* tf, int tf_n, RaySeg r)
+{
+    return march_impl<T, false>(p, brick, tf, tf_n, r, nullptr);
 }
 
 } // namespace rv

@@ -95,6 +95,7 @@ def test_adjoint_matches_float64_reference(orc, level, depth, W, H, crop):
     bricks = wl.bricks  # (nblocks, V) fp32, the plan's bricks
     fp = _plan(wl, W, H, tf.detach())
     fp.set_cameras(M[None])
+    fp.set_tape(True)
     # target image: same field, perturbed colour table
     tft = to_dev(tf_target)
     fpt = _plan(wl, W, H, tft)
@@ -192,6 +193,27 @@ def test_adjoint_matches_float64_reference(orc, level, depth, W, H, crop):
     fpt.close()
 
 
+def test_adjoint_requires_taped_render():
+    """The adjoint refuses a camera that was not rendered with the tape on."""
+    import torch
+    from revise_amd import RvError
+    wl, tf_np, tf_target, M = _scene(1, 3, 32, 32, seed=3)
+    fp = _plan(wl, 32, 32, to_dev(tf_np))
+    fp.set_cameras(np.stack([M, M]))
+    rgba = torch.zeros((32, 32, 4), dtype=torch.float32, device="cuda")
+    g = torch.zeros_like(rgba)
+    gtf = torch.zeros((256, 4), dtype=torch.float32, device="cuda")
+    with pytest.raises(RvError, match="tape"):
+        fp.adjoint(0, rgba, g, gtf)
+    fp.set_tape(True)
+    fp.render(0, torch.empty((32, 32), dtype=torch.int32, device="cuda"), rgba, interpolate=False)
+    with pytest.raises(RvError, match="tape"):
+        fp.adjoint(1, rgba, g, gtf)
+    fp.adjoint(0, rgba, g, gtf)
+    torch.cuda.synchronize()
+    fp.close()
+
+
 def test_adjoint_accumulates_and_tf_only(orc):
     """Outputs accumulate over calls; without gradient arrays only the colour
     table gradient is produced, and it is the same.  Float atomics add in a
@@ -202,6 +224,7 @@ def test_adjoint_accumulates_and_tf_only(orc):
     tf = to_dev(tf_np)
     fp = _plan(wl, 96, 96, tf)
     fp.set_cameras(M[None])
+    fp.set_tape(True)
     rgba = torch.empty((96, 96, 4), dtype=torch.float32, device="cuda")
     fp.render(0, torch.empty((96, 96), dtype=torch.int32, device="cuda"), rgba, interpolate=False)
     g = torch.randn((96, 96, 4), dtype=torch.float32, device="cuda", generator=torch.Generator("cuda").manual_seed(3))
