@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cams", type=int, default=64)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="512x512 forward with tape x3 and adjoint x3 (profiler runs)")
     ap.add_argument("--only-render", action="store_true",
                     help="run the render kernels only: share kernel x3, batched frame (resident "
                          "bricks, 32x16 tiles) x3 at 1920x1080, then x3 at 512x512 (profiler runs)")
@@ -250,6 +252,22 @@ def main():
         batch_own.run()
         log("batched interp, private plans: %.3f ms" % (events_time(torch, batch_own.run, 10) * 1e3))
         return
+    if args.only_adjoint:
+        plain.fused_step()  # interpolates the bricks the frame plan reads
+        fpa = api.FramePlan(wl.blocks, 512, 512, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf)
+        fpa.set_cameras(M[None])
+        fpa.set_tape(True)
+        argb_a = torch.empty((512, 512), dtype=torch.int32, device="cuda")
+        rgba_a = torch.empty((512, 512, 4), dtype=torch.float32, device="cuda")
+        gb_a = torch.zeros_like(wl.bricks)
+        fpa.set_grad_bricks([gb_a[i] for i in range(gb_a.shape[0])])
+        gtf_a = torch.zeros((tf_np.shape[0], 4), dtype=torch.float32, device="cuda")
+        for _ in range(3):
+            fpa.render(0, argb_a, rgba_a, False)
+            fpa.adjoint(0, rgba_a, rgba_a * 0.5, gtf_a)
+        torch.cuda.synchronize()
+        fpa.close()
+        return
     if args.only_render:
         plain.fused_step()
         for _ in range(3):
@@ -266,7 +284,7 @@ def main():
         for _ in range(3):
             fps.render(0, argb_s, None, False)
         torch.cuda.synchronize()
-        log("share %.3f ms, frame 1080p %.3f ms" % (events_time(torch, lambda: plain.share.render(plain.vp), 3) * 1e3,
+        log("share %.3f ms, frame 512x512 %.3f ms" % (events_time(torch, lambda: plain.share.render(plain.vp), 3) * 1e3,
                                                      events_time(torch, lambda: fps.render(0, argb_s, None, False), 3) * 1e3))
         fps.close()
         return
