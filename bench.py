@@ -211,7 +211,7 @@ def main():
                     help="512x512 forward with tape x3 and adjoint x3 (profiler runs)")
     ap.add_argument("--only-render", action="store_true",
                     help="run the render kernels only: share kernel x3, batched frame (resident "
-                         "bricks, 32x16 tiles) x3 at 1920x1080, then x3 at 512x512 (profiler runs)")
+                         "bricks, 8x8 bins) x3 at 1920x1080, then x3 at 512x512 (profiler runs)")
     ap.add_argument("--only-interp", action="store_true",
                     help="time a 1 GiB device copy and the batched interpolation only (profiler runs)")
     args = ap.parse_args()
@@ -272,13 +272,13 @@ def main():
         plain.fused_step()
         for _ in range(3):
             plain.share.render(plain.vp)
-        fpr = frame_plan(torch, wl, tf, Q2, CUT2, tile=(32, 16))
+        fpr = frame_plan(torch, wl, tf, Q2, CUT2, tile=(8, 8))
         fpr.set_cameras(M[None])
         argb_r = torch.empty((H, W), dtype=torch.int32, device="cuda")
         for _ in range(3):
             fpr.render(0, argb_r, None, False)
         fpr.close()
-        fps = api.FramePlan(wl.blocks, 512, 512, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf, (32, 16))
+        fps = api.FramePlan(wl.blocks, 512, 512, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf, (8, 8))
         fps.set_cameras(M[None])
         argb_s = torch.empty((512, 512), dtype=torch.int32, device="cuda")
         for _ in range(3):

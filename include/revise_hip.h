@@ -280,10 +280,12 @@ uint32_t rv_background_argb(float r, float g, float b);
  * renderKernel launch, per block -- by one launch per frame over all blocks of
  * one GPU's share, of uneven levels, composited front to back in registers
  * (blendBehind, cuda_blend.hpp:24-33) instead of through the 8-bit viewport.
- * The handle owns device copies of the block table, per-camera parameters,
- * block orders (rv_mixed_back_to_front) and view rectangles, sync counters,
- * and an optional captured graph; bricks, sparse values, plans, colour table
- * and outputs stay caller-owned. */
+ * The screen is cut into bins of one wave footprint (tile_w x tile_h = 64
+ * pixels); per camera, each bin lists the blocks whose view rectangles meet
+ * it, front to back (rv_mixed_back_to_front order), and one wave composites
+ * one bin.  The handle owns device copies of the block table, per-camera
+ * parameters, bin lists, sync counters and an optional captured graph;
+ * bricks, sparse values, plans, colour table and outputs stay caller-owned. */
 
 typedef struct rv_frame_block {
     uint32_t level;          /* block (subtree root) level */
@@ -303,7 +305,8 @@ typedef struct rv_frame_desc {
     float acc_cutoff;        /* stop compositing a pixel once its alpha exceeds this */
     const float* d_tf;       /* tf_n x 4 colour table on the device */
     uint32_t tf_n;           /* 1..1024 */
-    uint32_t tile_w, tile_h; /* screen tile, tile_w*tile_h == 512; 0,0 = 32x16 */
+    uint32_t tile_w, tile_h; /* wave footprint of a screen bin: tile_w*tile_h == 64,
+                                tile_w a power of two; 0,0 = 8x8 */
 } rv_frame_desc;
 
 typedef struct rv_frame_plan rv_frame_plan;
@@ -311,8 +314,9 @@ typedef struct rv_frame_plan rv_frame_plan;
 int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nblocks, const rv_frame_desc* desc,
                          rv_frame_plan** out);
 /* Cameras (row-major world->eye transforms, ncams x 16 floats): per camera the
- * host computes every block's parameters (rv_block_params_setup) and the
- * front-to-back order, and uploads them. */
+ * host computes every block's parameters (rv_block_params_setup), the
+ * front-to-back order and the bin lists (on up to 16 threads), and uploads
+ * them.  Synchronous. */
 int rv_frame_plan_set_cameras(rv_frame_plan* plan, const float* views16, uint32_t ncams);
 /* One frame for camera `cam`: a counter-reset kernel and one persistent
  * launch.  interpolate != 0 rebuilds every brick from its sparse values
@@ -328,8 +332,9 @@ int rv_frame_graph_launch(rv_frame_plan* plan, rv_stream_t stream);
 /* Sticky error flags of the last frames (1: a brick wait gave up); synchronous. */
 int rv_frame_status(rv_frame_plan* plan, uint32_t* error_flags);
 int rv_frame_get_tile(const rv_frame_plan* plan, uint32_t* tile_w, uint32_t* tile_h);
-/* Pick the screen-tile shape by timing candidates on camera 0; the choice is
- * cached (one JSON object per line) in cache_path when it is not NULL. */
+/* Pick the bins' wave footprint (8x8, 16x4, 4x16, 32x2) by timing candidates
+ * on camera 0; the choice is cached (one JSON object per line) in cache_path
+ * when it is not NULL. */
 int rv_frame_autotune(rv_frame_plan* plan, uint32_t* d_argb, int interpolate,
                       const char* cache_path, rv_stream_t stream, uint32_t* tile_w,
                       uint32_t* tile_h);

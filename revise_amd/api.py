@@ -239,29 +239,42 @@ def fill_u32(d, n, value, stream=None):
     check(lib().rv_fill_u32(ptr(d), n, value, stream_ptr(stream)))
 
 
+# numpy layout of rv_frame_block (for shares of millions of blocks)
+FRAME_BLOCK_DTYPE = np.dtype({"names": ["level", "index", "depth", "sparse", "plan", "brick", "brick16"],
+                              "formats": ["<u4", ("<u4", 3), "<u4", "<u8", "<u8", "<u8", "<u8"],
+                              "offsets": [0, 4, 16, 24, 32, 40, 48], "itemsize": 56})
+assert FRAME_BLOCK_DTYPE.itemsize == C.sizeof(FrameBlock)
+
+
 class FramePlan:
     """Batched persistent frame (rv_frame_plan).  blocks: list of dicts with
     level, index, depth, brick (torch fp32/fp16 tensor), optional sparse and
-    plan tensors."""
+    plan tensors; or a numpy array of FRAME_BLOCK_DTYPE holding device
+    addresses (the caller keeps the memory alive)."""
 
     def __init__(self, blocks, vp_w, vp_h, fov, fmin, fmax, thr_rel, quality, alpha_cutoff,
                  acc_cutoff, d_tf, tile=(0, 0)):
         import torch
-        arr = (FrameBlock * len(blocks))()
         self._keep = [d_tf]
-        for i, b in enumerate(blocks):
-            arr[i].level = b["level"]
-            for k in range(3):
-                arr[i].index[k] = b["index"][k]
-            arr[i].depth = b["depth"]
-            if b.get("sparse") is not None:
-                arr[i].sparse = b["sparse"].data_ptr()
-                arr[i].plan = b["plan"].data_ptr()
-            if b["brick"].dtype == torch.float16:
-                arr[i].brick16 = b["brick"].data_ptr()
-            else:
-                arr[i].brick = b["brick"].data_ptr()
-            self._keep.append(b)
+        if isinstance(blocks, np.ndarray):
+            assert blocks.dtype == FRAME_BLOCK_DTYPE and blocks.flags.c_contiguous
+            arr = C.cast(blocks.ctypes.data, C.POINTER(FrameBlock))
+            self._keep.append(blocks)
+        else:
+            arr = (FrameBlock * len(blocks))()
+            for i, b in enumerate(blocks):
+                arr[i].level = b["level"]
+                for k in range(3):
+                    arr[i].index[k] = b["index"][k]
+                arr[i].depth = b["depth"]
+                if b.get("sparse") is not None:
+                    arr[i].sparse = b["sparse"].data_ptr()
+                    arr[i].plan = b["plan"].data_ptr()
+                if b["brick"].dtype == torch.float16:
+                    arr[i].brick16 = b["brick"].data_ptr()
+                else:
+                    arr[i].brick = b["brick"].data_ptr()
+                self._keep.append(b)
         d = FrameDesc(vp_w, vp_h, fov, fmin, fmax, thr_rel, quality, alpha_cutoff, acc_cutoff,
                       d_tf.data_ptr(), d_tf.shape[0], tile[0], tile[1])
         h = C.c_void_p()

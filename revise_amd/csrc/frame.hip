@@ -1,31 +1,33 @@
 // Batched persistent frame: interpolation of every brick and ray-casting of
 // the whole frame in one launch per frame, graph-capturable.
 //
-// Work items, handed out in order by one atomic queue counter:
-//   [0, n_interp)           brick tiles (the interpolation tile of interp.hip);
-//   [n_interp, n_interp+nt) screen tiles of tw x th pixels (tw*th = 512).
-// A screen tile composites, for each of its pixels, the per-block ray
-// segments of every block whose view rectangle covers it, front to back, in
-// registers (blendBehind of the reference, cuda_blend.hpp:24-33), and stops as
-// soon as every pixel of the tile is opaque.  Each block segment is sampled
-// and classified exactly like the per-block renderer (renderArgb.hpp:30-55),
-// so the result is the reference's compositing of the same segments without
-// the per-block 8-bit viewport round trip.
+// Work is handed out by two atomic queue counters:
+//   counters[0]: brick tiles (the interpolation tile of interp.hip), taken
+//                by whole workgroups;
+//   counters[3]: screen bins of bw x bh = 64 pixels, taken by single waves.
+// A bin's wave composites, for each of its pixels, the per-block ray
+// segments of the blocks listed for the bin (every block whose view
+// rectangle meets it), front to back, in registers (blendBehind of the
+// reference, cuda_blend.hpp:24-33), and leaves the list once every lane's
+// pixel is opaque (a wave ballot).  Each block segment is sampled and
+// classified exactly like the per-block renderer (renderArgb.hpp:30-55), so
+// the result is the reference's compositing of the same segments without
+// the per-block 8-bit viewport round trip.  Waves never wait for each other
+// inside the screen phase.
 //
-// Bricks and screen tiles synchronize once per workgroup, not per tile: a
-// workgroup that interpolated brick tiles publishes them (device-scope fence
-// and one release add of its tile count) when it dequeues its first screen
-// tile or leaves; before its first screen tile it waits (acquire) until all
-// n_interp brick tiles are published.  On this chip every release/acquire at
-// agent scope writes back / invalidates the XCD's L2, so signalling per brick
-// tile costs more than the interpolation itself.  All brick tiles are
-// dequeued before any screen tile and a workgroup holding a brick tile never
-// waits, so every wait ends; a wait is additionally bounded (it sets an error
-// flag and gives up after ~2^22 sleeps).
+// Bricks and screen bins synchronize once per workgroup: a workgroup that
+// interpolated brick tiles publishes them (device-scope fence and one
+// release add of its tile count) when the brick queue runs dry; it then
+// waits (acquire) until all n_interp brick tiles are published.  On this
+// chip every release/acquire at agent scope writes back / invalidates the
+// XCD's L2, so signalling per brick tile costs more than the interpolation
+// itself.  A workgroup waits only after the brick queue is empty, and a
+// workgroup holding a brick tile never waits, so every wait ends; a wait is
+// additionally bounded (it sets an error flag and gives up after ~2^22
+// sleeps).
 //
-// The blocks of a screen tile are found by testing the block view
-// rectangles (camera dependent, one uint4 per block) in front-to-back order,
-// 512 at a time, and compacting the hits into LDS with wave ballots.
+// The bins' block lists are built on the host per camera (counting sort of
+// the view rectangles over the bins, in front-to-back order).
 
 #include "frame_plan.hpp"
 #include "interp_device.hpp"
@@ -41,6 +43,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace rv {
@@ -53,39 +56,37 @@ struct FrameArgs {
     const TileTask* tasks;         // brick tiles of all blocks
     uint32_t n_interp;             // brick tiles of this frame (0: bricks are resident)
     const rv_block_params* params; // per block
-    const uint4* rects;            // per block view rectangle {x0, y0, x1, y1}, x1 = x0 if dead
-    const uint32_t* f2b;           // block indices, front to back
+    const uint32_t* bin_off;       // per bin: its first entry in bin_list (nbins + 1)
+    const uint32_t* bin_list;      // per bin: block indices, front to back
     const void* const* bricks;     // per block, float or half
-    uint32_t nblocks;
     const float4* tf;
     uint32_t tf_n;
-    uint32_t W, H, tw, th, ntx, nty;
+    uint32_t W, H;
+    uint32_t bw_log2, bw, bh, nbx, nbins; // wave footprint bw x bh = 64
     float acc_cutoff;
     uint32_t* argb;                // W*H, rows bottom-up like the reference viewport
     float4* rgba;                  // optional float output, same layout
-    uint32_t* counters;            // [0] queue, [1] error, [2] brick tiles published
+    uint32_t* counters;            // [0] brick queue, [1] error, [2] brick tiles published, [3] bin queue
     float4* tape;                  // adjoint record per (block, pixel of its rectangle), or NULL
     const uint64_t* rec_off;       // per block: offset of its rectangle in the tape
 };
 
 constexpr uint32_t kSpinLimit = 1u << 22;
 
-template <class BrickT, uint32_t NT>
-__device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint32_t* list,
-                            uint32_t* wave_cnt)
+// One wave: the pixels of screen bin `bin`.
+template <class BrickT>
+__device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs)
 {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t tx = tile % a.ntx, ty = tile / a.ntx;
-    const uint32_t x0 = tx * a.tw, y0 = ty * a.th;
-    const uint32_t x1 = min(x0 + a.tw, a.W), y1 = min(y0 + a.th, a.H);
-    uint32_t px, py;
-    tile_pixel(tid, a.tw, a.th, px, py);
-    const uint32_t vx = x0 + px, vy = y0 + py;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t bx = bin % a.nbx, by = bin / a.nbx;
+    const uint32_t vx = bx * a.bw + (lane & (a.bw - 1u)), vy = by * a.bh + (lane >> a.bw_log2);
     const bool inside = vx < a.W && vy < a.H;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     float trans = 1.f; // product of (1 - alpha) in front of the current block (tape only)
     bool opaque = !inside;
-    for_tile_blocks<NT>(a.rects, a.f2b, a.nblocks, x0, y0, x1, y1, list, wave_cnt, [&](uint32_t blk) {
+    const uint32_t k1 = a.bin_off[bin + 1];
+    for (uint32_t k = a.bin_off[bin]; k < k1; ++k) {
+        const uint32_t blk = uniform(a.bin_list[k]);
         const rv_block_params p = a.params[blk];
         if (!opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
             vy < p.view_y + p.view_h) {
@@ -107,14 +108,30 @@ __device__ void render_tile(const FrameArgs& a, uint32_t tile, float4* tfs, uint
                 opaque = acc.w > a.acc_cutoff;
             }
         }
-        return opaque;
-    });
+        if (__all(opaque))
+            break;
+    }
     if (inside) {
         const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
         a.argb[pix] = ((unsigned)(acc.w * 255.999f) << 24) + ((unsigned)(acc.x * 255.999f) << 16) +
                       ((unsigned)(acc.y * 255.999f) << 8) + (unsigned)(acc.z * 255.999f);
         if (a.rgba)
             a.rgba[pix] = acc;
+    }
+}
+
+// Screen phase of a wave: take bins from the bin queue until it is empty.
+template <class BrickT>
+__device__ void render_bins(const FrameArgs& a, const float4* tfs)
+{
+    for (;;) {
+        uint32_t bin = 0;
+        if ((threadIdx.x & 63u) == 0u)
+            bin = atomicAdd(a.counters + 3, 1u);
+        bin = uniform(bin);
+        if (bin >= a.nbins)
+            break;
+        render_bin<BrickT>(a, bin, tfs);
     }
 }
 
@@ -133,80 +150,54 @@ __device__ __forceinline__ void publish_tiles(uint32_t* published, uint32_t tile
 template <class BrickT>
 __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
 {
-    __shared__ __align__(16) float A[kTileFloats]; // interpolation tile; colour table + list for screen tiles
+    __shared__ __align__(16) float A[kTileFloats]; // interpolation tile, then the colour table
     __shared__ float lat[64];
-    __shared__ uint32_t wave_cnt[kT / 64u];
     __shared__ uint32_t item_s;
     float4* tfs = reinterpret_cast<float4*>(A);
-    uint32_t* list = reinterpret_cast<uint32_t*>(A) + 4u * kMaxTf;
-    const uint32_t n_total = a.n_interp + a.ntx * a.nty;
-    bool tf_loaded = false;
-    uint32_t my_tiles = 0; // brick tiles done by this workgroup, not yet published
+    uint32_t my_tiles = 0; // brick tiles done by this workgroup
     for (;;) {
         if (threadIdx.x == 0)
             item_s = atomicAdd(a.counters, 1u);
         __syncthreads();
         const uint32_t item = item_s;
         __syncthreads();
-        if (item >= n_total)
+        if (item >= a.n_interp)
             break;
-        if (item < a.n_interp) {
-            const TileTask k = a.tasks[item];
-            interp_tile<kT>(k.job, k.local, k.e0, k.e1, A, lat);
-            ++my_tiles; // (the queue barrier above protects A before its next use)
-            continue;
-        }
-        if (!tf_loaded) {
-            if (a.n_interp) {
-                publish_tiles(a.counters + 2, my_tiles);
-                my_tiles = 0;
-                if (threadIdx.x == 0) {
-                    uint32_t spins = 0;
-                    while (__hip_atomic_load(a.counters + 2, __ATOMIC_ACQUIRE,
-                                             __HIP_MEMORY_SCOPE_AGENT) < a.n_interp) {
-                        __builtin_amdgcn_s_sleep(8);
-                        if (++spins > kSpinLimit) {
-                            atomicOr(a.counters + 1, 1u);
-                            break;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-            for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT)
-                tfs[i] = a.tf[i];
-            __syncthreads();
-            tf_loaded = true;
-        }
-        render_tile<BrickT, kT>(a, item - a.n_interp, tfs, list, wave_cnt);
+        const TileTask k = a.tasks[item];
+        interp_tile<kT>(k.job, k.local, k.e0, k.e1, A, lat);
+        ++my_tiles; // (the queue barrier above protects A before its next use)
     }
-    publish_tiles(a.counters + 2, my_tiles);
+    if (a.n_interp) {
+        publish_tiles(a.counters + 2, my_tiles);
+        if (threadIdx.x == 0) {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(a.counters + 2, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+                   a.n_interp) {
+                __builtin_amdgcn_s_sleep(8);
+                if (++spins > kSpinLimit) {
+                    atomicOr(a.counters + 1, 1u);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT)
+        tfs[i] = a.tf[i];
+    __syncthreads();
+    render_bins<BrickT>(a, tfs);
 }
 
-// Frames of resident bricks: screen tiles only, 16 x 16 pixels per
-// workgroup of kRT threads, no interpolation LDS, so more workgroups per CU
-// and more tiles for small viewports.  Tiles are taken from the queue
-// counter like the brick-and-screen kernel.
+// Frames of resident bricks: screen bins only, no interpolation LDS, so
+// more waves per CU.
 template <class BrickT>
 __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 {
-    __shared__ float4 tfs[kMaxTf];
-    __shared__ uint32_t list[kRT];
-    __shared__ uint32_t wave_cnt[kRT / 64u];
-    __shared__ uint32_t item_s;
+    extern __shared__ float4 tfs_dyn[]; // tf_n entries
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT)
-        tfs[i] = a.tf[i];
-    const uint32_t ntiles = a.ntx * a.nty;
-    for (;;) {
-        if (threadIdx.x == 0)
-            item_s = atomicAdd(a.counters, 1u);
-        __syncthreads();
-        const uint32_t item = item_s;
-        __syncthreads();
-        if (item >= ntiles)
-            break;
-        render_tile<BrickT, kRT>(a, item, tfs, list, wave_cnt);
-    }
+        tfs_dyn[i] = a.tf[i];
+    __syncthreads();
+    render_bins<BrickT>(a, tfs_dyn);
 }
 
 // Worker share in one launch: the blocks (back to front) of one share are
@@ -300,31 +291,141 @@ int rv::frame_upload(void** dst, const void* src, size_t bytes)
     return RV_OK;
 }
 
+int rv::parallel_for(uint64_t n, const std::function<int(uint64_t, uint64_t)>& fn)
+{
+    const uint64_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const uint64_t nt = std::min<uint64_t>(hw, n);
+    if (nt <= 1)
+        return n ? fn(0, n) : RV_OK;
+    std::vector<int> rc(nt, RV_OK);
+    std::vector<std::string> err(nt);
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (uint64_t t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            rc[t] = fn(n * t / nt, n * (t + 1) / nt);
+            if (rc[t])
+                err[t] = rv_last_error(); // the message is thread-local
+        });
+    for (auto& x : th)
+        x.join();
+    for (uint64_t t = 0; t < nt; ++t)
+        if (rc[t]) {
+            set_error(err[t]);
+            return rc[t];
+        }
+    return RV_OK;
+}
+
 namespace {
+
+// Screen bins of the wave footprint bw x bh for every camera of the plan:
+// a counting sort of the blocks' view rectangles over the bins, in
+// front-to-back order, so each bin's list is front to back.
+int build_bins(rv_frame_plan* P, uint32_t bw, uint32_t bh, FrameBins** out)
+{
+    auto B = std::make_unique<FrameBins>();
+    B->bw = bw;
+    B->bh = bh;
+    B->nbx = (P->desc.vp_w + bw - 1) / bw;
+    B->nby = (P->desc.vp_h + bh - 1) / bh;
+    const uint64_t nbins = (uint64_t)B->nbx * B->nby, nb = P->nblocks;
+    std::vector<std::vector<uint32_t>> off(P->ncams), list(P->ncams);
+    int rc = parallel_for(P->ncams, [&](uint64_t c0, uint64_t c1) -> int {
+        for (uint64_t c = c0; c < c1; ++c) {
+            const uint4* rects = P->h_rects.data() + c * nb;
+            const uint32_t* f2b = P->h_f2b.data() + c * nb;
+            std::vector<uint32_t>& o = off[c];
+            o.assign(nbins + 1, 0u);
+            for (uint64_t i = 0; i < nb; ++i) {
+                const uint4 r = rects[i];
+                if (r.x >= r.z || r.y >= r.w)
+                    continue;
+                for (uint32_t y = r.y / bh; y <= (r.w - 1) / bh; ++y)
+                    for (uint32_t x = r.x / bw; x <= (r.z - 1) / bw; ++x)
+                        ++o[(uint64_t)y * B->nbx + x + 1];
+            }
+            for (uint64_t k = 0; k < nbins; ++k)
+                o[k + 1] += o[k];
+            if (o[nbins] >= 0x80000000u)
+                return fail(RV_ERR_INVALID, "rv_frame: too many (block, bin) pairs for one camera");
+            std::vector<uint32_t> cur(o.begin(), o.end() - 1);
+            std::vector<uint32_t>& l = list[c];
+            l.resize(o[nbins]);
+            for (uint64_t i = 0; i < nb; ++i) {
+                const uint32_t b = f2b[i];
+                const uint4 r = rects[b];
+                if (r.x >= r.z || r.y >= r.w)
+                    continue;
+                for (uint32_t y = r.y / bh; y <= (r.w - 1) / bh; ++y)
+                    for (uint32_t x = r.x / bw; x <= (r.z - 1) / bw; ++x)
+                        l[cur[(uint64_t)y * B->nbx + x]++] = b;
+            }
+        }
+        return RV_OK;
+    });
+    if (rc)
+        return rc;
+    uint64_t total = 0;
+    B->list_base.resize(P->ncams);
+    for (uint32_t c = 0; c < P->ncams; ++c) {
+        B->list_base[c] = total;
+        total += list[c].size();
+    }
+    RV_HIP(hipMalloc((void**)&B->d_off, 4ull * (nbins + 1) * P->ncams));
+    RV_HIP(hipMalloc((void**)&B->d_list, 4ull * std::max<uint64_t>(total, 1)));
+    for (uint32_t c = 0; c < P->ncams; ++c) {
+        RV_HIP(hipMemcpy(B->d_off + (nbins + 1) * c, off[c].data(), 4ull * (nbins + 1),
+                         hipMemcpyHostToDevice));
+        if (!list[c].empty())
+            RV_HIP(hipMemcpy(B->d_list + B->list_base[c], list[c].data(), 4ull * list[c].size(),
+                             hipMemcpyHostToDevice));
+    }
+    P->bins.push_back(std::move(B));
+    *out = P->bins.back().get();
+    return RV_OK;
+}
+
+// The bins of footprint bw x bh, built on first use.
+int get_bins(rv_frame_plan* P, uint32_t bw, uint32_t bh, FrameBins** out)
+{
+    for (auto& b : P->bins)
+        if (b->bw == bw && b->bh == bh) {
+            *out = b.get();
+            return RV_OK;
+        }
+    return build_bins(P, bw, bh, out);
+}
+
+bool valid_footprint(uint32_t w, uint32_t h)
+{
+    return w * h == 64u && w >= 1u && (w & (w - 1u)) == 0u;
+}
 
 int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba, int interpolate,
                  hipStream_t s, uint32_t tw, uint32_t th)
 {
+    FrameBins* B = nullptr;
+    int rc = get_bins(P, tw, th, &B);
+    if (rc)
+        return rc;
     FrameArgs a{};
     a.tasks = P->d_tasks;
     a.n_interp = interpolate ? P->n_interp : 0u;
     a.params = P->d_params + (size_t)cam * P->nblocks;
-    a.rects = P->d_rects + (size_t)cam * P->nblocks;
-    a.f2b = P->d_f2b + (size_t)cam * P->nblocks;
+    const uint64_t nbins = (uint64_t)B->nbx * B->nby;
+    a.bin_off = B->d_off + (nbins + 1) * cam;
+    a.bin_list = B->d_list + B->list_base[cam];
     a.bricks = P->d_bricks;
-    a.nblocks = P->nblocks;
     a.tf = reinterpret_cast<const float4*>(P->desc.d_tf);
     a.tf_n = P->desc.tf_n;
     a.W = P->desc.vp_w;
     a.H = P->desc.vp_h;
-    if (!interpolate) {
-        tw = 16;
-        th = 16;
-    }
-    a.tw = tw;
-    a.th = th;
-    a.ntx = (a.W + tw - 1) / tw;
-    a.nty = (a.H + th - 1) / th;
+    a.bw = tw;
+    a.bh = th;
+    a.bw_log2 = (uint32_t)__builtin_ctz(tw);
+    a.nbx = B->nbx;
+    a.nbins = (uint32_t)nbins;
     a.acc_cutoff = P->desc.acc_cutoff;
     a.argb = d_argb;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
@@ -332,7 +433,7 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     if (P->tape_on && d_rgba) {
         // records of blocks a pixel never reaches keep T = -1
         const uint64_t nrec = P->rec_total[cam];
-        int rc = fill_u32(reinterpret_cast<uint32_t*>(P->d_tape), 4 * nrec, 0xBF800000u, s);
+        rc = fill_u32(reinterpret_cast<uint32_t*>(P->d_tape), 4 * nrec, 0xBF800000u, s);
         if (rc)
             return rc;
         a.tape = P->d_tape;
@@ -342,10 +443,11 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     hipLaunchKernelGGL(frame_reset_kernel, dim3(1), dim3(64), 0, s, P->d_counters, 4u);
     RV_LAUNCH_CHECK();
     if (!interpolate) {
+        const size_t lds = sizeof(float4) * a.tf_n;
         if (P->fp16)
-            hipLaunchKernelGGL(render_frame_kernel<__half>, dim3(P->render_grid), dim3(kRT), 0, s, a);
+            hipLaunchKernelGGL(render_frame_kernel<__half>, dim3(P->render_grid), dim3(kRT), lds, s, a);
         else
-            hipLaunchKernelGGL(render_frame_kernel<float>, dim3(P->render_grid), dim3(kRT), 0, s, a);
+            hipLaunchKernelGGL(render_frame_kernel<float>, dim3(P->render_grid), dim3(kRT), lds, s, a);
     } else if (P->fp16)
         hipLaunchKernelGGL(frame_kernel<__half>, dim3(P->grid), dim3(kT), 0, s, a);
     else
@@ -416,13 +518,16 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->grid = (uint32_t)(cus * per_cu);
     if (fp16)
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<__half>, kRT, 0));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<__half>, kRT,
+                                                            sizeof(float4) * desc->tf_n));
     else
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<float>, kRT, 0));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<float>, kRT,
+                                                            sizeof(float4) * desc->tf_n));
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->render_grid = (uint32_t)(cus * per_cu);
     if (desc->tile_w && desc->tile_h) {
-        RV_REQUIRE(desc->tile_w * desc->tile_h == kT, "rv_frame_plan_create: tile_w*tile_h must be 512");
+        RV_REQUIRE(valid_footprint(desc->tile_w, desc->tile_h),
+                   "rv_frame_plan_create: tile_w*tile_h must be 64 with tile_w a power of two");
         P->tw = desc->tile_w;
         P->th = desc->tile_h;
     }
@@ -443,10 +548,14 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
         (void)hipGraphDestroy(P->graph);
         P->graph = nullptr;
     }
+    P->bins.clear();
+    P->ncams = 0;
     const uint32_t nb = P->nblocks;
     std::vector<rv_block_params> params((size_t)ncams * nb);
-    std::vector<uint4> rects((size_t)ncams * nb);
-    std::vector<uint32_t> f2b((size_t)ncams * nb);
+    std::vector<uint4>& rects = P->h_rects;
+    std::vector<uint32_t>& f2b = P->h_f2b;
+    rects.assign((size_t)ncams * nb, make_uint4(0, 0, 0, 0));
+    f2b.assign((size_t)ncams * nb, 0u);
     std::vector<uint32_t> keys(4ull * nb);
     for (uint32_t i = 0; i < nb; ++i) {
         keys[4 * i] = P->blocks[i].level;
@@ -454,32 +563,36 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
             keys[4 * i + 1 + k] = P->blocks[i].index[k];
     }
     const rv_frame_desc& d = P->desc;
-    std::vector<uint32_t> order;
-    for (uint32_t c = 0; c < ncams; ++c) {
-        const float* M = views16 + 16ull * c;
-        for (uint32_t i = 0; i < nb; ++i) {
-            rv_block_params& p = params[(size_t)c * nb + i];
-            const int live = block_params(M, d.vp_w, d.vp_h, d.fov_y_deg, P->blocks[i].level,
-                                          P->blocks[i].index, P->blocks[i].depth, d.fmin, d.fmax,
-                                          d.threshold_rel, d.quality, d.alpha_cutoff, p);
-            rects[(size_t)c * nb + i] =
-                live ? make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h)
-                     : make_uint4(0, 0, 0, 0);
+    int rc = parallel_for((uint64_t)ncams * nb, [&](uint64_t j0, uint64_t j1) -> int {
+        for (uint64_t j = j0; j < j1; ++j) {
+            const uint64_t c = j / nb, i = j % nb;
+            rv_block_params& p = params[j];
+            const int live = block_params(views16 + 16 * c, d.vp_w, d.vp_h, d.fov_y_deg,
+                                          P->blocks[i].level, P->blocks[i].index, P->blocks[i].depth,
+                                          d.fmin, d.fmax, d.threshold_rel, d.quality, d.alpha_cutoff, p);
+            rects[j] = live ? make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h)
+                            : make_uint4(0, 0, 0, 0);
         }
-        float eye[3];
-        eye_from_view(M, eye);
-        int rc = mixed_back_to_front(keys.data(), nb, eye, order);
-        if (rc)
-            return rc;
-        for (uint32_t i = 0; i < nb; ++i)
-            f2b[(size_t)c * nb + i] = order[nb - 1 - i];
-    }
-    int rc;
+        return RV_OK;
+    });
+    if (rc)
+        return rc;
+    rc = parallel_for(ncams, [&](uint64_t c0, uint64_t c1) -> int {
+        std::vector<uint32_t> order;
+        for (uint64_t c = c0; c < c1; ++c) {
+            float eye[3];
+            eye_from_view(views16 + 16 * c, eye);
+            const int r = mixed_back_to_front(keys.data(), nb, eye, order);
+            if (r)
+                return r;
+            for (uint32_t i = 0; i < nb; ++i)
+                f2b[c * nb + i] = order[nb - 1 - i];
+        }
+        return RV_OK;
+    });
+    if (rc)
+        return rc;
     if ((rc = frame_upload((void**)&P->d_params, params.data(), sizeof(rv_block_params) * params.size())))
-        return rc;
-    if ((rc = frame_upload((void**)&P->d_rects, rects.data(), sizeof(uint4) * rects.size())))
-        return rc;
-    if ((rc = frame_upload((void**)&P->d_f2b, f2b.data(), 4ull * f2b.size())))
         return rc;
     // adjoint tables: tape offsets of the view rectangles, (block, tile) pairs
     std::vector<uint64_t> rec_off((size_t)ncams * nb);
@@ -508,6 +621,9 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
         return rc;
     P->tape_cam = -1;
     P->ncams = ncams;
+    FrameBins* B = nullptr;
+    if ((rc = get_bins(P, P->tw, P->th, &B)))
+        return rc;
     if (P->tape_on) {
         const int rc2 = rv_frame_set_tape(P, 1);
         if (rc2)
@@ -572,6 +688,10 @@ extern "C" int rv_frame_graph_build(rv_frame_plan* P, uint32_t* d_argb, uint64_t
     if (!P->capture_stream)
         RV_HIP(hipStreamCreateWithFlags(&P->capture_stream, hipStreamNonBlocking));
     hipStream_t s = P->capture_stream;
+    FrameBins* B = nullptr; // built before the capture (host work and copies)
+    int brc = get_bins(P, P->tw, P->th, &B);
+    if (brc)
+        return brc;
     RV_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = RV_OK;
     for (uint32_t c = 0; c < P->ncams && !rc; ++c)
@@ -614,8 +734,9 @@ extern "C" int rv_frame_get_tile(const rv_frame_plan* P, uint32_t* tile_w, uint3
     return RV_OK;
 }
 
-// Autotuner: time every candidate screen-tile shape on the plan's camera 0
-// (hip events, median of 3 after a warm-up) and keep the fastest.  Results
+// Autotuner: time every candidate wave footprint of the screen bins on the
+// plan's camera 0 (hip events, median of 3 after a warm-up) and keep the
+// fastest; the bins of the other footprints are released.  Results
 // are cached in a small JSON file keyed by device name and workload
 // signature, so later runs reuse the choice without timing.
 extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpolate,
@@ -625,13 +746,13 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     RV_TRY
     RV_REQUIRE(P && d_argb && P->ncams > 0, "rv_frame_autotune: bad arguments");
     RV_REQUIRE(!interpolate || P->can_interp, "rv_frame_autotune: blocks have no sparse values/plans");
-    static const uint32_t cand[][2] = {{32, 16}, {16, 32}, {64, 8}, {128, 4}, {8, 64}};
+    static const uint32_t cand[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}};
     int dev = 0;
     RV_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     RV_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream sig;
-    sig << prop.gcnArchName << "|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
+    sig << prop.gcnArchName << "|bins|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
         << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth;
     const std::string key = sig.str();
     if (cache_path && *cache_path) {
@@ -645,7 +766,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
             unsigned w = 0, h = 0;
             const size_t t = line.find("\"tile\": [");
             if (t != std::string::npos && std::sscanf(line.c_str() + t + 9, "%u, %u", &w, &h) == 2 &&
-                w * h == kT) {
+                valid_footprint(w, h)) {
                 P->tw = w;
                 P->th = h;
                 if (tile_w)
@@ -690,6 +811,9 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     (void)hipEventDestroy(e1);
     P->tw = bw;
     P->th = bh;
+    P->bins.erase(std::remove_if(P->bins.begin(), P->bins.end(),
+                                 [&](const std::unique_ptr<FrameBins>& b) { return b->bw != bw || b->bh != bh; }),
+                  P->bins.end());
     if (tile_w)
         *tile_w = bw;
     if (tile_h)

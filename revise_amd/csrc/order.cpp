@@ -188,6 +188,44 @@ int mixed_back_to_front(const uint32_t* blocks, uint64_t n, const float* eye,
     order.clear();
     if (n == 0)
         return RV_OK;
+    // Blocks all of one level L <= 8 (the root node is then "uniform"): the
+    // same BackToFront3 walk of the cube's index box at level L, with a dense
+    // index table in place of the hash maps.
+    bool one_level = blocks[0] <= 8;
+    for (uint64_t i = 1; i < n && one_level; ++i)
+        one_level = blocks[4 * i] == blocks[0];
+    if (one_level) {
+        const uint32_t L = blocks[0], m = 1u << L;
+        std::vector<uint32_t> at((size_t)m * m * m, ~0u);
+        for (uint64_t i = 0; i < n; ++i) {
+            const uint32_t x = blocks[4 * i + 1], y = blocks[4 * i + 2], z = blocks[4 * i + 3];
+            if (x >= m || y >= m || z >= m) {
+                set_error("rv_mixed_back_to_front: block index outside its level");
+                return RV_ERR_INVALID;
+            }
+            uint32_t& slot = at[((size_t)z * m + y) * m + x];
+            if (slot != ~0u) {
+                set_error("rv_mixed_back_to_front: duplicate block");
+                return RV_ERR_INVALID;
+            }
+            slot = (uint32_t)i;
+        }
+        const uint32_t bmin[3] = {0, 0, 0}, bmax[3] = {m, m, m};
+        float cmin[3], cmax[3];
+        index_box_coords(L, bmin, bmax, cmin, cmax);
+        AxisSplit sp[3];
+        for (int a = 0; a < 3; ++a)
+            sp[a] = UniformSplitter(bmin[a], bmax[a], cmin[a], cmax[a]).split(eye[a]);
+        BackToFront3 it3(sp);
+        uint32_t p[3];
+        order.reserve(n);
+        while (it3.next(p)) {
+            const uint32_t b = at[((size_t)p[2] * m + p[1]) * m + p[0]];
+            if (b != ~0u)
+                order.push_back(b);
+        }
+        return RV_OK;
+    }
     auto key = [](uint32_t level, uint32_t x, uint32_t y, uint32_t z) {
         return ((uint64_t)level << 60) | ((uint64_t)x << 40) | ((uint64_t)y << 20) | (uint64_t)z;
     };

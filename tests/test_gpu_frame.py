@@ -46,7 +46,7 @@ def _mixed_blocks(seed, depth=3, n_splits=12):
     return blocks
 
 
-def _setup(blocks, W, H, fp16=False, acc_cut=2.0, seed=0):
+def _setup(blocks, W, H, fp16=False, acc_cut=2.0, seed=0, tile=(0, 0)):
     import torch
     fmin, fmax = datasets.field_range(blocks)
     tf = datasets.seeded_tf(seed, 256)
@@ -61,7 +61,7 @@ def _setup(blocks, W, H, fp16=False, acc_cut=2.0, seed=0):
                             device="cuda")
         dev_blocks.append({"level": b.level, "index": b.index, "depth": b.depth, "brick": brick,
                            "sparse": to_dev(b.sparse), "plan": plans[key]})
-    fp = api.FramePlan(dev_blocks, W, H, FOV, fmin, fmax, THR, Q, CUT, acc_cut, d_tf)
+    fp = api.FramePlan(dev_blocks, W, H, FOV, fmin, fmax, THR, Q, CUT, acc_cut, d_tf, tile)
     return fp, tf, fmin, fmax, dev_blocks
 
 
@@ -159,7 +159,38 @@ def test_frame_autotune_picks_valid_tile(tmp_path):
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
     cache = str(tmp_path / "tune.jsonl")
     w, h = fp.autotune(argb, True, cache)
-    assert w * h == 512 and (w, h) == fp.tile()
+    assert w * h == 64 and (w, h) == fp.tile()
     w2, h2 = fp.autotune(argb, True, cache)  # served from the cache
     assert (w2, h2) == (w, h)
     assert open(cache).read().count("\n") == 1
+
+
+@pytest.mark.parametrize("tile", [(8, 8), (16, 4), (4, 16), (32, 2)])
+def test_frame_small_blocks_every_footprint(orc, tile):
+    """A sparse level-4 share of depth-2 bricks: blocks a few pixels wide, many
+    per bin; every wave footprint gives the oracle's frame."""
+    import torch
+    rng = np.random.default_rng(11)
+    field = datasets.Field(11, n_gauss=3, noise_octaves=1)
+    n2i = api.subtree_n2i(datasets.full_subtree(2), 2)
+    keys = [(x, y, z) for z in range(16) for y in range(16) for x in range(16) if rng.random() < 0.3]
+    blocks = [datasets.Block(4, k, 2, n2i, datasets.sample(field, 4, k, 2, n2i)) for k in keys]
+    W, H = 150, 110
+    fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.999, seed=11, tile=tile)
+    assert fp.tile() == tile
+    cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((20.0, 10.0), (200.0, -30.0))])
+    fp.set_cameras(cams)
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    for c in range(2):
+        say("small blocks", len(blocks), "tile", tile, "cam", c)
+        fp.render(c, argb, rgba, interpolate=True)
+        torch.cuda.synchronize()
+        assert fp.status() == 0
+        exp = _oracle_frame(orc, blocks, bricks, cams[c], W, H, fmin, fmax, tf, 0.999)
+        assert np.array_equal(rgba.cpu().numpy(), exp)
+        assert (exp[..., 3] > 0).mean() > 0.2
+        fp.render(c, argb, rgba, interpolate=False)
+        torch.cuda.synchronize()
+        assert np.array_equal(rgba.cpu().numpy(), exp)
