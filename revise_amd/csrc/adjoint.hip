@@ -31,15 +31,16 @@
 // Order of work.  The forward records, for every pixel and every block its
 // ray reaches, the pixel's state in front of the block (premultiplied colour
 // and transmittance, 16 bytes; rv_frame_set_tape).  The adjoint then works
-// block by block: one workgroup takes a 16 x 16 tile of one block's view
-// rectangle and re-marches only that block's segments, starting from the
-// recorded state, so all rays of a workgroup are in one block at a time.
-// Colour-table terms are summed in registers while a ray's samples
-// interpolate the same two table entries, then into an LDS copy of the
-// table, added to the output once per workgroup.  Field terms are summed per
-// cell in registers (keeping the 4 corners shared with the next cell when
-// the ray steps to a face neighbour), then into an LDS window per slab of the
-// block, which is added to memory with coalesced atomics (adjoint_item).
+// block by block: one wave takes an 8 x 8 tile of one block's view rectangle
+// and re-marches only that block's segments, starting from the recorded
+// state, so all rays of a wave are in one block at a time, and waves never
+// wait for each other.  Colour-table terms are summed in registers while a
+// ray's samples interpolate the same two table entries, then into an LDS copy
+// of the table shared by the workgroup, added to the output once per
+// workgroup.  Field terms are summed per cell in registers (keeping the 4
+// corners shared with the next cell when the ray steps to a face neighbour),
+// then into the wave's LDS window over one slab of the block, which is added
+// to memory with coalesced atomics (adjoint_item).
 
 #include "frame_plan.hpp"
 
@@ -48,8 +49,31 @@
 namespace rv {
 namespace {
 
-constexpr int kSlab = 8;              // cells per slab along the slab axis
-constexpr uint32_t kWinCap = 16384;   // floats of the LDS gradient window
+constexpr int kSlab = 8;                // cells per slab along the slab axis (at most)
+constexpr uint32_t kWinWave = 4096;     // floats of a wave's LDS gradient window
+constexpr uint32_t kAdjWaves = kRT / 64u;
+
+__device__ __forceinline__ int wave_min(int v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = min(v, __shfl_xor(v, o));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+    for (int o = 32; o > 0; o >>= 1)
+        v = max(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Orders the LDS accesses of one wave's lanes (a wave's LDS operations
+// execute in program order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
 
 struct AdjArgs {
     const rv_block_params* params;
@@ -313,22 +337,25 @@ __device__ __forceinline__ void march_adjoint(const rv_block_params& p, const T*
     }
 }
 
-// One work item (one block, one 16 x 16 tile of its view rectangle) for every
-// thread of the workgroup; active: this thread's pixel has a segment in the
-// block.  With field gradients the block is walked in slabs of kSlab cells
-// along the axis the rays advance along most: all rays pause at each slab's
-// far side, their field terms go to an LDS window covering the slab's part of
-// their frustum, and the window is added to the brick's gradient in x-fastest
-// order, so a wave's global atomics fall into a few cache lines (scattered
-// float atomics cost one L2 request per lane).  A window above kWinCap
-// floats, or rays advancing both ways along the axis, fall back to direct
-// atomics.  Barriers are uniform over the workgroup.
+// One work item (one block, one 8 x 8 tile of its view rectangle) for the
+// lanes of one wave; active: this lane's pixel has a segment in the block.
+// With field gradients the block is walked in slabs of kSlab cells along the
+// axis the rays advance along most: the wave's rays pause at each slab's far
+// side, their field terms go to the wave's LDS window covering the slab's
+// part of their frustum, and the window is added to the brick's gradient in
+// x-fastest order, so the global atomics of a wave fall into a few cache
+// lines (scattered float atomics cost one L2 request per lane).  A window
+// above kWinWave floats, or rays advancing both ways along the axis, fall
+// back to direct atomics.  The window is zero between items: every flush
+// clears what it adds.  All decisions are wave-uniform (ballots and
+// butterfly reductions).
 template <class T>
 __device__ void adjoint_item(const rv_block_params& p, const T* __restrict__ brick,
                              float* __restrict__ gbrick, const float4* tfs, float4* gtf, int tf_n,
-                             bool active, const RaySeg& r, AdjPixel& st, float* win, int* red)
+                             bool active, const RaySeg& r, AdjPixel& st, float* win)
 {
     const int n = (int)p.nodes;
+    const uint32_t lane = threadIdx.x & 63u;
     SegRay sg;
     sg.r = r;
     sg.i = 1;
@@ -348,80 +375,76 @@ __device__ void adjoint_item(const rv_block_params& p, const T* __restrict__ bri
     bool up = true, slabs = false;
     if (gbrick) {
         const float sA = A == 0 ? r.sx : (A == 1 ? r.sy : r.sz);
-        if (threadIdx.x == 0) {
-            red[0] = 0;
-            red[1] = 0;
-        }
-        __syncthreads();
-        if (active) {
-            if (sA > 0.f)
-                red[0] = 1;
-            else
-                red[1] = 1;
-        }
-        __syncthreads();
-        up = red[0] != 0;
-        slabs = (red[0] != 0) != (red[1] != 0);
-        __syncthreads(); // red is rewritten below
+        const bool any_up = __ballot(active && sA > 0.f) != 0ull;
+        const bool any_down = __ballot(active && !(sA > 0.f)) != 0ull;
+        up = any_up;
+        slabs = any_up != any_down;
     }
-    const int ns = slabs ? (n + kSlab - 1) / kSlab : 1;
-    for (int k = 0; k < ns; ++k) {
-        const int s = up ? k : ns - 1 - k;
-        const int lo = s * kSlab;
-        const int hi = min(lo + kSlab - 1, n - 1); // cells lo..hi along A
+    // slabs of at most kSlab cells, walked in the rays' direction along A;
+    // a slab whose window would not fit is halved until it does (or is one
+    // cell thick)
+    int cur = up ? 0 : n - 1; // next cell index along A
+    for (;;) {
+        if (slabs && __ballot(!sg.ended && sg.i < sg.r.sample_count) == 0ull)
+            break; // every ray of the wave has left the block
+        int lo = up ? cur : max(cur - kSlab + 1, 0);
+        int hi = up ? min(cur + kSlab - 1, n - 1) : cur; // cells lo..hi along A
         if (slabs) {
-            // window bounds: the part of each ray's remaining samples in this
-            // slab, from its position and step in texel units, widened by one
-            // node for the fp32 accumulation of positions
-            if (threadIdx.x == 0) {
-                red[2] = 0x7fffffff;
-                red[3] = -1;
-                red[4] = 0x7fffffff;
-                red[5] = -1;
-            }
-            __syncthreads();
-            if (!sg.ended && sg.i < sg.r.sample_count) {
-                const float tx = (sg.r.px - p.tex_origin[0]) * p.tcf;
-                const float ty = (sg.r.py - p.tex_origin[1]) * p.tcf;
-                const float tz = (sg.r.pz - p.tex_origin[2]) * p.tcf;
-                const float dx = sg.r.sx * p.tcf, dy = sg.r.sy * p.tcf, dz = sg.r.sz * p.tcf;
-                const float tA = A == 0 ? tx : (A == 1 ? ty : tz), dA = A == 0 ? dx : (A == 1 ? dy : dz);
-                const float tU = U == 0 ? tx : ty, dU = U == 0 ? dx : dy;
-                const float tV = V == 1 ? ty : tz, dV = V == 1 ? dy : dz;
-                const float rem = (float)(sg.r.sample_count - 1 - sg.i);
-                float la = ((float)lo + 0.5f - tA) / dA;
-                float lb = ((float)hi + 1.5f - tA) / dA;
-                if (la > lb) {
-                    const float q = la;
-                    la = lb;
-                    lb = q;
+            for (;;) {
+                // window bounds: the part of each ray's remaining samples in
+                // this slab, from its position and step in texel units,
+                // widened by one node for the fp32 accumulation of positions
+                int umin = 0x7fffffff, umax = -1, vmin = 0x7fffffff, vmax = -1;
+                if (!sg.ended && sg.i < sg.r.sample_count) {
+                    const float tx = (sg.r.px - p.tex_origin[0]) * p.tcf;
+                    const float ty = (sg.r.py - p.tex_origin[1]) * p.tcf;
+                    const float tz = (sg.r.pz - p.tex_origin[2]) * p.tcf;
+                    const float dx = sg.r.sx * p.tcf, dy = sg.r.sy * p.tcf, dz = sg.r.sz * p.tcf;
+                    const float tA = A == 0 ? tx : (A == 1 ? ty : tz), dA = A == 0 ? dx : (A == 1 ? dy : dz);
+                    const float tU = U == 0 ? tx : ty, dU = U == 0 ? dx : dy;
+                    const float tV = V == 1 ? ty : tz, dV = V == 1 ? dy : dz;
+                    const float rem = (float)(sg.r.sample_count - 1 - sg.i);
+                    float la = ((float)lo + 0.5f - tA) / dA;
+                    float lb = ((float)hi + 1.5f - tA) / dA;
+                    if (la > lb) {
+                        const float q = la;
+                        la = lb;
+                        lb = q;
+                    }
+                    la = fmaxf(la, 0.f);
+                    lb = fminf(lb, rem);
+                    if (la <= lb) {
+                        const float ua = tU + la * dU, ub = tU + lb * dU;
+                        const float va = tV + la * dV, vb = tV + lb * dV;
+                        umin = max((int)floorf(fminf(ua, ub) - 0.5f) - 1, 0);
+                        umax = min((int)floorf(fmaxf(ua, ub) - 0.5f) + 2, n - 1);
+                        vmin = max((int)floorf(fminf(va, vb) - 0.5f) - 1, 0);
+                        vmax = min((int)floorf(fmaxf(va, vb) - 0.5f) + 2, n - 1);
+                    }
                 }
-                la = fmaxf(la, 0.f);
-                lb = fminf(lb, rem);
-                if (la <= lb) {
-                    const float ua = tU + la * dU, ub = tU + lb * dU;
-                    const float va = tV + la * dV, vb = tV + lb * dV;
-                    atomicMin(&red[2], max((int)floorf(fminf(ua, ub) - 0.5f) - 1, 0));
-                    atomicMax(&red[3], min((int)floorf(fmaxf(ua, ub) - 0.5f) + 2, n - 1));
-                    atomicMin(&red[4], max((int)floorf(fminf(va, vb) - 0.5f) - 1, 0));
-                    atomicMax(&red[5], min((int)floorf(fmaxf(va, vb) - 0.5f) + 2, n - 1));
-                }
-            }
-            __syncthreads();
-            const int nl = min(hi + 1, n - 1) - lo + 1;
-            const int nu = red[3] - red[2] + 1, nv = red[5] - red[4] + 1;
-            w.ok = red[3] >= red[2] && red[5] >= red[4] && nu * nv * nl <= (int)kWinCap;
-            if (w.ok) {
+                umin = wave_min(umin);
+                umax = wave_max(umax);
+                vmin = wave_min(vmin);
+                vmax = wave_max(vmax);
+                const int nl = min(hi + 1, n - 1) - lo + 1;
+                const int nu = umax - umin + 1, nv = vmax - vmin + 1;
+                w.ok = umax >= umin && vmax >= vmin && nu * nv * nl <= (int)kWinWave;
+                if (w.ok) {
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    w.o[j] = j == A ? lo : (j == U ? red[2] : red[4]);
-                    w.e[j] = j == A ? nl : (j == U ? nu : nv);
+                    for (int j = 0; j < 3; ++j) {
+                        w.o[j] = j == A ? lo : (j == U ? umin : vmin);
+                        w.e[j] = j == A ? nl : (j == U ? nu : nv);
+                    }
                 }
-                const int size = nu * nv * nl;
-                for (int j = threadIdx.x; j < size; j += (int)blockDim.x)
-                    win[j] = 0.f;
+                if (w.ok || hi == lo || umax < umin || vmax < vmin)
+                    break;
+                // too wide: a thinner slab
+                const int k = (hi - lo + 1) >> 1;
+                if (up)
+                    hi = lo + k - 1;
+                else
+                    lo = hi - k + 1;
             }
-            __syncthreads();
         }
         march_adjoint(p, brick, gbrick, tfs, gtf, tf_n, st, sg, slabs ? A : -1, up, up ? hi : lo, w,
                       win);
@@ -429,48 +452,56 @@ __device__ void adjoint_item(const rv_block_params& p, const T* __restrict__ bri
             cell_flush(sg.cell, w, win, gbrick); // the slab's last cell
         if (!slabs)
             break;
-        __syncthreads();
+        cur = up ? hi + 1 : lo - 1;
         if (w.ok) {
-            // the window, x fastest, to the brick's gradient
+            // the window, x fastest, to the brick's gradient; cleared behind
+            wave_sync();
             const int ex = w.e[0], exy = w.e[0] * w.e[1], size = exy * w.e[2];
-            for (int j = threadIdx.x; j < size; j += (int)blockDim.x) {
+            for (int j = (int)lane; j < size; j += 64) {
                 const float g = win[j];
                 if (g == 0.f)
                     continue;
+                win[j] = 0.f;
                 const int lz = j / exy, rxy = j - lz * exy;
                 const int ly = rxy / ex, lx = rxy - ly * ex;
                 unsafeAtomicAdd(gbrick + ((uint32_t)(w.o[2] + lz) * n + (uint32_t)(w.o[1] + ly)) * n +
                                     (uint32_t)(w.o[0] + lx),
                                 g);
             }
+            wave_sync();
         }
-        __syncthreads();
+        if (cur < 0 || cur > n - 1)
+            break;
     }
     tf_flush(sg.tacc, gtf);
 }
 
 // Persistent grid; work items (block, tile) in order, tiles of a block
-// consecutive.
+// consecutive; wave v of workgroup g takes items g*4+v, then every
+// (grid waves)-th after it, so the waves of a workgroup work on neighbouring
+// tiles of one block.
 template <class BrickT>
 __global__ __launch_bounds__(kRT) void adjoint_kernel(AdjArgs a)
 {
-    extern __shared__ float4 dyn[]; // colour table, its gradient, gradient window
+    extern __shared__ float4 dyn[]; // colour table, its gradient, the waves' gradient windows
     float4* tfs = dyn;
     float4* gtf = dyn + a.tf_n;
-    float* win = reinterpret_cast<float*>(dyn + 2 * a.tf_n);
-    __shared__ int red[8];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    float* win = reinterpret_cast<float*>(dyn + 2 * a.tf_n) + wave * kWinWave;
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT) {
         tfs[i] = a.tf[i];
         gtf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    if (a.grad_bricks)
+        for (uint32_t j = lane; j < kWinWave; j += 64u)
+            win[j] = 0.f;
     __syncthreads();
-    for (uint32_t item = blockIdx.x; item < a.nitems; item += gridDim.x) {
-        const uint32_t blk = a.items[2 * item], tile = a.items[2 * item + 1];
+    const uint32_t stride = gridDim.x * kAdjWaves;
+    for (uint32_t item = blockIdx.x * kAdjWaves + wave; item < a.nitems; item += stride) {
+        const uint32_t blk = uniform(a.items[2 * item]), tile = uniform(a.items[2 * item + 1]);
         const rv_block_params p = a.params[blk];
-        const uint32_t ntx = (p.view_w + 15u) >> 4;
-        uint32_t px, py;
-        tile_pixel(threadIdx.x, 16u, 16u, px, py);
-        const uint32_t lx = (tile % ntx) * 16u + px, ly = (tile / ntx) * 16u + py;
+        const uint32_t ntx = (p.view_w + 7u) >> 3;
+        const uint32_t lx = (tile % ntx) * 8u + (lane & 7u), ly = (tile / ntx) * 8u + (lane >> 3);
         AdjPixel st{};
         RaySeg r{};
         bool active = false;
@@ -505,7 +536,7 @@ __global__ __launch_bounds__(kRT) void adjoint_kernel(AdjArgs a)
         }
         adjoint_item(p, reinterpret_cast<const BrickT*>(a.bricks[blk]),
                      a.grad_bricks ? a.grad_bricks[blk] : nullptr, tfs, gtf, (int)a.tf_n, active, r,
-                     st, win, red);
+                     st, win);
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT) {
@@ -527,7 +558,7 @@ __global__ __launch_bounds__(kRT) void adjoint_kernel(AdjArgs a)
 
 using namespace rv;
 
-static size_t adjoint_lds(uint32_t tf_n) { return 32ull * tf_n + 4ull * kWinCap; }
+static size_t adjoint_lds(uint32_t tf_n) { return 32ull * tf_n + 4ull * kWinWave * kAdjWaves; }
 
 extern "C" int rv_frame_set_grad_bricks(rv_frame_plan* P, float* const* d_grad_bricks)
 {

@@ -63,6 +63,7 @@ struct FrameArgs {
     uint32_t tf_n;
     uint32_t W, H;
     uint32_t bw_log2, bw, bh, nbx, nbins; // wave footprint bw x bh = 64
+    uint32_t render_waves;         // waves per workgroup that take screen bins (frame_kernel)
     float acc_cutoff;
     uint32_t* argb;                // W*H, rows bottom-up like the reference viewport
     float4* rgba;                  // optional float output, same layout
@@ -185,7 +186,8 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT)
         tfs[i] = a.tf[i];
     __syncthreads();
-    render_bins<BrickT>(a, tfs);
+    if ((threadIdx.x >> 6) < a.render_waves) // fewer concurrent rays can be faster (vector L1)
+        render_bins<BrickT>(a, tfs);
 }
 
 // Frames of resident bricks: screen bins only, no interpolation LDS, so
@@ -204,9 +206,17 @@ __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 // ray-cast into the viewport with exactly the per-block operation of the
 // per-block renderer -- pixel = blendAbove(pixel, segment) through the 8-bit
 // ARGB round trip -- so the result equals one launch per block in that order,
-// bit for bit.  Tiles of 16 x 16 pixels; the blocks of a tile are compacted
-// from the view rectangles 256 at a time.
+// bit for bit.  Each wave owns an 8 x 8 pixel square (a workgroup a 16 x 16
+// tile) and finds its blocks itself: the view rectangles are tested 64 at a
+// time and the hits of each ballot visited in order, with no workgroup
+// barrier after the colour table is loaded.  The dynamic LDS is padded so
+// that at most kShareWgsPerCu workgroups share a CU: fewer concurrent rays
+// keep more of each brick neighbourhood in the vector L1.
 constexpr uint32_t kShareT = 256;
+#ifndef RV_SHARE_WGS_PER_CU
+#define RV_SHARE_WGS_PER_CU 4
+#endif
+constexpr uint32_t kShareWgsPerCu = RV_SHARE_WGS_PER_CU;
 
 __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* __restrict__ params,
                                                         const uint4* __restrict__ rects,
@@ -215,53 +225,39 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         uint32_t tf_n, uint32_t* __restrict__ viewport,
                                                         uint32_t W, uint32_t H)
 {
-    __shared__ float4 tfs[kMaxTf];
-    __shared__ uint32_t list[kShareT];
-    __shared__ uint32_t wave_cnt[kShareT / 64u];
+    extern __shared__ float4 tfs_share[]; // tf_n entries (and padding)
     for (uint32_t i = threadIdx.x; i < tf_n; i += kShareT)
-        tfs[i] = tf[i];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t x0 = blockIdx.x * 16u, y0 = blockIdx.y * 16u;
-    const uint32_t x1 = min(x0 + 16u, W), y1 = min(y0 + 16u, H);
-    uint32_t px, py;
-    tile_pixel(tid, 16u, 16u, px, py);
-    const uint32_t vx = x0 + px, vy = y0 + py;
+        tfs_share[i] = tf[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t x0 = blockIdx.x * 16u + (wave & 1u) * 8u, y0 = blockIdx.y * 16u + (wave >> 1) * 8u;
+    if (x0 >= W || y0 >= H)
+        return; // the whole wave is outside the viewport
+    const uint32_t x1 = min(x0 + 8u, W), y1 = min(y0 + 8u, H);
+    const uint32_t vx = x0 + (lane & 7u), vy = y0 + (lane >> 3);
     const bool inside = vx < W && vy < H;
     const uint32_t pix = inside ? (H - 1u - vy) * W + vx : 0u;
     uint32_t value = inside ? viewport[pix] : 0u;
-    const uint32_t lane = tid & 63u, wave = tid >> 6;
-    __syncthreads();
-    for (uint32_t c = 0; c < nblocks; c += kShareT) {
-        const uint32_t i = c + tid;
+    for (uint32_t c = 0; c < nblocks; c += 64u) {
+        const uint32_t i = c + lane;
         bool hit = false;
         if (i < nblocks) {
             const uint4 r = rects[i];
             hit = r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0;
         }
-        const uint64_t ball = __ballot(hit);
-        if (lane == 0)
-            wave_cnt[wave] = (uint32_t)__popcll(ball);
-        __syncthreads();
-        uint32_t off = 0, total = 0;
-        for (uint32_t w = 0; w < kShareT / 64u; ++w) {
-            off += w < wave ? wave_cnt[w] : 0u;
-            total += wave_cnt[w];
-        }
-        if (hit)
-            list[off + (uint32_t)__popcll(ball & ((1ull << lane) - 1ull))] = i;
-        __syncthreads();
-        for (uint32_t k = 0; k < total; ++k) {
-            const uint32_t blk = uniform(list[k]);
+        uint64_t m = __ballot(hit);
+        while (m) {
+            const uint32_t blk = c + (uint32_t)__builtin_ctzll(m);
+            m &= m - 1ull;
             const rv_block_params p = params[blk];
             if (inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
                 vy < p.view_y + p.view_h) {
                 const RaySeg r = setup_ray(p, vx, vy);
                 if (r.hit)
                     value = blend_above_u32(
-                        value, march(p, reinterpret_cast<const float*>(bricks[blk]), tfs, (int)tf_n, r));
+                        value, march(p, reinterpret_cast<const float*>(bricks[blk]), tfs_share, (int)tf_n, r));
             }
         }
-        __syncthreads();
     }
     if (inside)
         viewport[pix] = value;
@@ -426,6 +422,7 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     a.bw_log2 = (uint32_t)__builtin_ctz(tw);
     a.nbx = B->nbx;
     a.nbins = (uint32_t)nbins;
+    a.render_waves = P->frame_rw;
     a.acc_cutoff = P->desc.acc_cutoff;
     a.argb = d_argb;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
@@ -516,6 +513,8 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     else
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<float>, kT, 0));
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
+    P->cus = (uint32_t)cus;
+    P->grid_max_wgs = (uint32_t)per_cu;
     P->grid = (uint32_t)(cus * per_cu);
     if (fp16)
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<__half>, kRT,
@@ -524,7 +523,8 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<float>, kRT,
                                                             sizeof(float4) * desc->tf_n));
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
-    P->render_grid = (uint32_t)(cus * per_cu);
+    P->render_max_wgs = (uint32_t)per_cu;
+    P->render_grid = (uint32_t)(cus * std::min(per_cu, 4)); // 16 waves per CU until tuned
     if (desc->tile_w && desc->tile_h) {
         RV_REQUIRE(valid_footprint(desc->tile_w, desc->tile_h),
                    "rv_frame_plan_create: tile_w*tile_h must be 64 with tile_w a power of two");
@@ -594,7 +594,7 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
         return rc;
     if ((rc = frame_upload((void**)&P->d_params, params.data(), sizeof(rv_block_params) * params.size())))
         return rc;
-    // adjoint tables: tape offsets of the view rectangles, (block, tile) pairs
+    // adjoint tables: tape offsets of the view rectangles, (block, 8 x 8 tile) pairs
     std::vector<uint64_t> rec_off((size_t)ncams * nb);
     std::vector<uint32_t> items;
     P->rec_total.assign(ncams, 0);
@@ -606,7 +606,7 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
             rec_off[(size_t)c * nb + i] = off;
             const uint64_t w = r.z - r.x, h = r.w - r.y;
             off += w * h;
-            const uint32_t tx = (uint32_t)((w + 15) / 16), ty = (uint32_t)((h + 15) / 16);
+            const uint32_t tx = (uint32_t)((w + 7) / 8), ty = (uint32_t)((h + 7) / 8);
             for (uint32_t t = 0; t < tx * ty; ++t) {
                 items.push_back(i);
                 items.push_back(t);
@@ -752,7 +752,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     hipDeviceProp_t prop;
     RV_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream sig;
-    sig << prop.gcnArchName << "|bins|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
+    sig << prop.gcnArchName << "|bins2|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
         << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth;
     const std::string key = sig.str();
     if (cache_path && *cache_path) {
@@ -763,12 +763,21 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
             const std::string k = "\"key\": \"" + key + "\"";
             if (line.find(k) == std::string::npos)
                 continue;
-            unsigned w = 0, h = 0;
+            unsigned w = 0, h = 0, g = 0;
             const size_t t = line.find("\"tile\": [");
+            const size_t gp = line.find("\"wgs_per_cu\": ");
             if (t != std::string::npos && std::sscanf(line.c_str() + t + 9, "%u, %u", &w, &h) == 2 &&
-                valid_footprint(w, h)) {
+                valid_footprint(w, h) && gp != std::string::npos &&
+                std::sscanf(line.c_str() + gp + 14, "%u", &g) == 1 && g >= 1 &&
+                g <= (interpolate ? P->grid_max_wgs : P->render_max_wgs)) {
+                unsigned rw = 0;
+                const size_t rp = line.find("\"render_waves\": ");
+                if (interpolate && rp != std::string::npos &&
+                    std::sscanf(line.c_str() + rp + 16, "%u", &rw) == 1 && rw >= 1 && rw <= kT / 64u)
+                    P->frame_rw = rw;
                 P->tw = w;
                 P->th = h;
+                (interpolate ? P->grid : P->render_grid) = P->cus * g;
                 if (tile_w)
                     *tile_w = w;
                 if (tile_h)
@@ -783,8 +792,24 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     RV_HIP(hipEventCreate(&e1));
     float best = 1e30f;
     uint32_t bw = P->tw, bh = P->th;
+    uint32_t& grid = interpolate ? P->grid : P->render_grid;
+    const uint32_t gmax = interpolate ? P->grid_max_wgs : P->render_max_wgs;
+    uint32_t bg = grid / P->cus;
+    std::vector<uint32_t> wgs;
+    for (uint32_t g : {1u, 2u, 3u, 4u, 6u, 8u, gmax})
+        if (g <= gmax && std::find(wgs.begin(), wgs.end(), g) == wgs.end())
+            wgs.push_back(g);
+    std::vector<uint32_t> rws = {kT / 64u};
+    if (interpolate)
+        rws = {4u, 6u, kT / 64u};
+    uint32_t brw = P->frame_rw;
     std::ostringstream times;
-    for (const auto& c : cand) {
+    for (const auto& c : cand)
+    for (uint32_t g : wgs)
+    for (uint32_t rw : rws) {
+        grid = P->cus * g;
+        if (interpolate)
+            P->frame_rw = rw;
         float t[3];
         int rc = launch_frame(P, 0, d_argb, nullptr, interpolate, s, c[0], c[1]);
         for (int r = 0; r < 3 && !rc; ++r) {
@@ -800,13 +825,21 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
             return rc;
         }
         std::sort(t, t + 3);
-        times << (times.tellp() > 0 ? ", " : "") << "\"" << c[0] << "x" << c[1] << "\": " << t[1];
+        times << (times.tellp() > 0 ? ", " : "") << "\"" << c[0] << "x" << c[1] << "/" << g;
+        if (interpolate)
+            times << "/" << rw;
+        times << "\": " << t[1];
         if (t[1] < best) {
             best = t[1];
             bw = c[0];
             bh = c[1];
+            bg = g;
+            brw = rw;
         }
     }
+    grid = P->cus * bg;
+    if (interpolate)
+        P->frame_rw = brw;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     P->tw = bw;
@@ -820,7 +853,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
         *tile_h = bh;
     if (cache_path && *cache_path) {
         std::ofstream f(cache_path, std::ios::app);
-        f << "{\"key\": \"" << key << "\", \"tile\": [" << bw << ", " << bh << "], \"ms\": {"
+        f << "{\"key\": \"" << key << "\", \"tile\": [" << bw << ", " << bh
+          << "], \"wgs_per_cu\": " << bg << ", \"render_waves\": " << P->frame_rw << ", \"ms\": {"
           << times.str() << "}}\n";
     }
     return RV_OK;
@@ -865,7 +899,11 @@ extern "C" int rv_render_share_argb(const rv_block_params* params, const float* 
     RV_HIP(hipMemcpyAsync(d, host.data(), host.size() - 256, hipMemcpyHostToDevice, S(stream)));
     // the host staging buffer must outlive the copy
     RV_HIP(hipStreamSynchronize(S(stream)));
-    hipLaunchKernelGGL(share_kernel, dim3((vp_w + 15) / 16, (vp_h + 15) / 16), dim3(kShareT), 0,
+    // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
+    // LDS per workgroup at most)
+    const size_t lds = std::max<size_t>(sizeof(float4) * tf_n,
+                                        std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u));
+    hipLaunchKernelGGL(share_kernel, dim3((vp_w + 15) / 16, (vp_h + 15) / 16), dim3(kShareT), lds,
                        S(stream), reinterpret_cast<const rv_block_params*>(d),
                        reinterpret_cast<const uint4*>(d + sizeof(rv_block_params) * nblocks),
                        reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks),

@@ -69,8 +69,10 @@ struct rv_frame_plan {
     float** d_grad_bricks = nullptr; // per block fp32 field gradient (adjoint), or NULL
     uint32_t adj_grid = 0;
     uint32_t render_grid = 0; // resident-brick frames
+    uint32_t cus = 0, grid_max_wgs = 0, render_max_wgs = 0; // CUs, resident workgroups per CU
+    uint32_t frame_rw = rv::kT / 64u; // waves per frame_kernel workgroup that render
     // adjoint tape: per camera, per block, the offset of the block's view
-    // rectangle in the tape (float4 per pixel); the blocks' 16 x 16 tiles
+    // rectangle in the tape (float4 per pixel); the blocks' 8 x 8 tiles
     // as (block, tile) pairs, grouped by block
     std::vector<uint64_t> rec_total, item_off;
     uint64_t* d_rec_off = nullptr;
