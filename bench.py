@@ -197,6 +197,99 @@ def cpu_baseline(wl, M, tf_np, budget_s=20.0):
             "interp_gvox_s_1core": gvox_1core}
 
 
+def large_share_bench(torch, args, tf_np, log):
+    """Large single-GPU share: level 8 (256^3 block grid), the 12% of blocks
+    with the largest centre values, each a full depth-4 subtree (17^3 fp32
+    sparse nodes -> 17^3 fp16 brick); field: 32 seeded Gaussians + 4 octaves
+    of value noise.  Interpolation of every brick in one launch; batched
+    resident-brick frames at 3840x2160; forward + colour-table adjoint over 8
+    seeded views (transfer-function fitting)."""
+    from revise_amd import api, configs, datasets
+    t0 = time.time()
+    ls = configs.large_share(torch, SEED + 5, level=8, depth=4, occupancy=0.12, n_gauss=32, octaves=4)
+    torch.cuda.synchronize()
+    nb = ls.keys.shape[0]
+    log("config 5 setup %.1fs: %d blocks, %.3g dense voxels, field [%g, %g]" % (
+        time.time() - t0, nb, ls.dense_nodes, ls.fmin, ls.fmax))
+    ib = api.InterpBatch(ls.brick_jobs())
+    ib.run()
+    torch.cuda.synchronize()
+    t_i = events_time(torch, ib.run, 5)
+    V, N = ls.dense_nodes, ls.sparse_nodes
+    B_alg = 2 * V + 10 * N
+    # full subtrees: every brick value is its node's sparse value (scatter)
+    nn = 2 ** ls.depth + 1
+    fi = torch.from_numpy(((ls.n2i[:, 2].astype(np.int64) * nn + ls.n2i[:, 1]) * nn + ls.n2i[:, 0])).cuda()
+    Nn = ls.n2i.shape[0]
+    for c0 in range(0, nb, 1 << 18):
+        c1 = min(nb, c0 + (1 << 18))
+        assert torch.equal(ls.bricks[c0:c1, :nn ** 3][:, fi], ls.sparse[c0:c1, :Nn].half()), "interp"
+    log("config 5 interp %.3f ms, %.1f Gvox/s, %.0f GB/s (fp16 bricks)" % (t_i * 1e3, V / t_i / 1e9,
+                                                                          B_alg / t_i / 1e9))
+    W5, H5 = 3840, 2160
+    rng = np.random.default_rng(SEED + 5)
+    views = np.stack([datasets.orbit_camera(float(a), elevation_deg=float(e))
+                      for a, e in zip(rng.uniform(0, 360, 8), rng.uniform(-30, 30, 8))])
+    tf_buf = torch.from_numpy(tf_np).cuda()
+    t0 = time.time()
+    fp = api.FramePlan(ls.frame_blocks(), W5, H5, FOV, ls.fmin, ls.fmax, THR, Q2, CUT2, ACC_CUT, tf_buf)
+    fp.set_cameras(views)
+    t_setup = time.time() - t0
+    argb = torch.empty((H5, W5), dtype=torch.int32, device="cuda")
+    tile = fp.autotune(argb, False, os.path.join(ROOT, "gpurun_out", "frame_tune.jsonl"))
+    fp.render(0, argb, None, False)
+    t_f = events_time(torch, lambda: fp.render(0, argb, None, False), 5)
+    t_fi = events_time(torch, lambda: fp.render(0, argb, None, True), 3)
+    assert fp.status() == 0
+    # the same frame from the interpolating launch and from resident bricks
+    a1 = argb.clone()
+    fp.render(0, argb, None, False)
+    assert torch.equal(a1, argb)
+    log("config 5 frame 3840x2160: %.3f ms resident, %.3f ms with interpolation (camera setup %.1fs, tile %s)"
+        % (t_f * 1e3, t_fi * 1e3, t_setup, tile))
+    # forward + colour-table adjoint over the 8 views; targets from the
+    # perturbed table N(0, 0.05)
+    tf_t = np.clip(tf_np + rng.normal(0.0, 0.05, size=tf_np.shape), 0.0, 1.0).astype(np.float32)
+    rgba = torch.empty((H5, W5, 4), dtype=torch.float32, device="cuda")
+    targets = torch.empty((8, H5, W5, 4), dtype=torch.float32, device="cuda")
+    tf_buf.copy_(torch.from_numpy(tf_t))
+    for v in range(8):
+        fp.render(v, argb, targets[v], False)
+    tf_buf.copy_(torch.from_numpy(tf_np))
+    gtf = torch.zeros((tf_np.shape[0], 4), dtype=torch.float32, device="cuda")
+
+    def fwd8():
+        for v in range(8):
+            fp.render(v, argb, rgba, False)
+
+    def fwd_adj8():
+        gtf.zero_()
+        for v in range(8):
+            fp.render(v, argb, rgba, False)
+            fp.adjoint(v, rgba, rgba - targets[v], gtf)
+
+    t_fw = events_time(torch, fwd8, 2)
+    fp.set_tape(True)
+    fwd_adj8()
+    t_fa = events_time(torch, fwd_adj8, 2)
+    assert bool(torch.isfinite(gtf).all()) and float(gtf.abs().max()) > 0
+    log("config 5 8 views: forward %.3f ms, forward+adjoint (colour table) %.3f ms, ratio %.2f"
+        % (t_fw * 1e3, t_fa * 1e3, t_fa / t_fw))
+    res = {
+        "workload": "level 8 (256^3 block grid), %d blocks (12%%) x 17^3 fp16 bricks = %.3g voxels, "
+                    "32 Gaussians + 4-octave noise; 3840x2160; 8 seeded views" % (nb, V),
+        "blocks": nb, "dense_voxels": V, "sparse_nodes": N,
+        "interp_ms": t_i * 1e3, "interp_gvox_s": V / t_i / 1e9, "interp_gb_s": B_alg / t_i / 1e9,
+        "interp_frac_of_hbm_peak": B_alg / t_i / 8e12,
+        "frame_4k_ms": t_f * 1e3, "frame_4k_fps": 1.0 / t_f, "frame_4k_with_interp_ms": t_fi * 1e3,
+        "camera_setup_s_8_views": t_setup, "tile": list(tile),
+        "fwd_8views_ms": t_fw * 1e3, "fwd_adjoint_8views_ms": t_fa * 1e3, "ratio": t_fa / t_fw}
+    fp.close()
+    del ib, ls, targets
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +300,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--cams", type=int, default=64)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-large", action="store_true", help="skip the level-8 fp16 share")
+    ap.add_argument("--only-large", action="store_true", help="run the level-8 fp16 share only")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="512x512 forward with tape x3 and adjoint x3 (profiler runs)")
     ap.add_argument("--only-render", action="store_true",
@@ -224,6 +319,9 @@ def main():
     lib()
     torch.cuda.set_device(0)
     out = {}
+    if args.only_large:
+        print(json.dumps(large_share_bench(torch, args, datasets.seeded_tf(SEED, 256), log)))
+        return
 
     # ---------------- config 2: 512 level-3 bricks, plain drop-in frame
     t0 = time.time()
@@ -488,6 +586,9 @@ def main():
         fp4.close()
         fp4t.close()
         del gb4
+
+    if not args.no_large:
+        out["large_share"] = large_share_bench(torch, args, tf_np, log)
 
     if not args.no_cpu:
         cb = cpu_baseline(wl, M, tf_np, args.cpu_budget)

@@ -149,14 +149,31 @@ def interp_brick(d_sparse, n2i, depth, d_dense, stream=None):
                                 stream_ptr(stream)))
 
 
+# numpy layout of rv_brick_job (for batches of millions of bricks)
+BRICK_JOB_DTYPE = np.dtype({"names": ["sparse", "plan", "dense", "dense16", "depth", "reserved"],
+                            "formats": ["<u8", "<u8", "<u8", "<u8", "<u4", "<u4"],
+                            "offsets": [0, 8, 16, 24, 32, 36], "itemsize": 40})
+assert BRICK_JOB_DTYPE.itemsize == C.sizeof(BrickJob)
+
+
 class InterpBatch:
     """Batched interpolation handle; jobs: list of (sparse, plan, dense, depth)
-    torch tensors (dense may be float32 or float16)."""
+    torch tensors (dense may be float32 or float16), or a numpy array of
+    BRICK_JOB_DTYPE holding device addresses (the caller keeps the memory
+    alive)."""
 
     def __init__(self, jobs):
         import torch
-        arr = (BrickJob * max(len(jobs), 1))()
         self._keep = []
+        if isinstance(jobs, np.ndarray):
+            assert jobs.dtype == BRICK_JOB_DTYPE and jobs.flags.c_contiguous
+            self._keep.append(jobs)
+            h = C.c_void_p()
+            check(lib().rv_interp_batch_create(C.cast(jobs.ctypes.data, C.POINTER(BrickJob)), len(jobs),
+                                               C.byref(h)))
+            self.h = h
+            return
+        arr = (BrickJob * max(len(jobs), 1))()
         for i, (sp, pl, de, depth) in enumerate(jobs):
             arr[i].sparse = sp.data_ptr()
             arr[i].plan = pl.data_ptr()

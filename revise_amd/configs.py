@@ -137,3 +137,92 @@ def mixed(torch, field, nblocks=2000, depth=6, seed=0, n_fill=16, brick_dtype=No
     w.sparse = sparse_all
     w.fmin, w.fmax = np.float32(lo), np.float32(hi)
     return w
+
+
+class LargeShare:
+    """One GPU's share of a deep uniform level: keys (B, 3) int64 block
+    indices (x, y, z), sparse (B, N) fp32 node values in n2i order (rows
+    padded to sparse_stride), bricks (B, brick_stride) fp16 (or fp32), one
+    shared gather plan (every block is a full subtree)."""
+
+    def frame_blocks(self, with_sparse=True):
+        """numpy array of api.FRAME_BLOCK_DTYPE (device addresses)."""
+        B = self.keys.shape[0]
+        a = np.zeros(B, dtype=api.FRAME_BLOCK_DTYPE)
+        a["level"] = self.level
+        a["index"] = self.keys.astype(np.uint32)
+        a["depth"] = self.depth
+        rows = np.arange(B, dtype=np.uint64)
+        if with_sparse:
+            a["sparse"] = np.uint64(self.sparse.data_ptr()) + rows * np.uint64(4 * self.sparse_stride)
+            a["plan"] = self.plan.data_ptr()
+        es = self.bricks.element_size()
+        base = np.uint64(self.bricks.data_ptr()) + rows * np.uint64(es * self.brick_stride)
+        if es == 2:
+            a["brick16"] = base
+        else:
+            a["brick"] = base
+        return a
+
+    def brick_jobs(self):
+        """numpy array of api.BRICK_JOB_DTYPE for api.InterpBatch."""
+        B = self.keys.shape[0]
+        a = np.zeros(B, dtype=api.BRICK_JOB_DTYPE)
+        rows = np.arange(B, dtype=np.uint64)
+        a["sparse"] = np.uint64(self.sparse.data_ptr()) + rows * np.uint64(4 * self.sparse_stride)
+        a["plan"] = self.plan.data_ptr()
+        es = self.bricks.element_size()
+        base = np.uint64(self.bricks.data_ptr()) + rows * np.uint64(es * self.brick_stride)
+        a["dense16" if es == 2 else "dense"] = base
+        a["depth"] = self.depth
+        return a
+
+
+def large_share(torch, seed, level=8, depth=4, occupancy=0.12, n_gauss=32, octaves=4,
+                brick_dtype=None, chunk=32768, device="cuda"):
+    """The blocks of one level (8^level of them) whose centre values are the
+    largest `occupancy` fraction of the field, each a fully refined
+    depth-`depth` subtree; field: SeparableField (n_gauss Gaussians plus
+    `octaves` of value noise).  Rows are padded to 16-byte multiples."""
+    brick_dtype = brick_dtype or torch.float16
+    field = datasets.SeparableField(seed, n_gauss=n_gauss, noise_octaves=octaves)
+    m = 2 ** level
+    dev = device
+    # centre value of every block of the level, z slab by z slab
+    centre = torch.empty(m ** 3, dtype=torch.float32, device=dev)
+    mid = torch.tensor([2 ** (depth - 1)], dtype=torch.int64, device=dev)
+    ar = torch.arange(m, dtype=torch.int64, device=dev)
+    zc = max(1, (1 << 20) // (m * m))
+    for z0 in range(0, m, zc):
+        z1 = min(m, z0 + zc)
+        zz, yy, xx = torch.meshgrid(torch.arange(z0, z1, device=dev), ar, ar, indexing="ij")
+        idx = torch.stack([xx.reshape(-1), yy.reshape(-1), zz.reshape(-1)], 1)
+        centre[z0 * m * m:z1 * m * m] = field.blocks(torch, level, depth, idx, mid).reshape(-1).float()
+    nsel = int(round(occupancy * m ** 3))
+    sel = torch.sort(torch.argsort(centre, descending=True)[:nsel]).values  # linear z-major index
+    del centre
+    keys = torch.stack([sel % m, (sel // m) % m, sel // (m * m)], 1)
+    n2i = api.subtree_n2i(datasets.full_subtree(depth), depth)
+    nn = 2 ** depth + 1
+    N, V = n2i.shape[0], api.brick_nodes(depth)
+    fi = torch.from_numpy(((n2i[:, 2].astype(np.int64) * nn + n2i[:, 1]) * nn + n2i[:, 0])).to(dev)
+    w = LargeShare()
+    w.level, w.depth, w.field = level, depth, field
+    w.n2i = n2i
+    w.plan = torch.from_numpy(api.interp_plan_encode(n2i, depth)).to(dev)
+    w.sparse_stride = (N + 3) // 4 * 4
+    w.brick_stride = (V + 7) // 8 * 8
+    w.sparse = torch.empty((nsel, w.sparse_stride), dtype=torch.float32, device=dev)
+    w.bricks = torch.empty((nsel, w.brick_stride), dtype=brick_dtype, device=dev)
+    lo, hi = float("inf"), float("-inf")
+    for c0 in range(0, nsel, chunk):
+        c1 = min(nsel, c0 + chunk)
+        v = field.blocks(torch, level, depth, keys[c0:c1]).reshape(c1 - c0, nn ** 3)[:, fi].float()
+        w.sparse[c0:c1, :N] = v
+        lo = min(lo, float(v.min()))
+        hi = max(hi, float(v.max()))
+    w.sparse[:, N:] = 0.0
+    w.keys = keys.cpu().numpy()
+    w.fmin, w.fmax = np.float32(lo), np.float32(hi)
+    w.sparse_nodes, w.dense_nodes = nsel * N, nsel * V
+    return w

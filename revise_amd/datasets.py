@@ -92,6 +92,70 @@ class Field:
         return np.sqrt(g)
 
 
+class SeparableField:
+    """Sum of seeded isotropic Gaussians plus value noise, evaluated on the
+    node lattice of a level (unit cube, 2^(level+depth) cells per axis) with
+    each term factored per axis, for shares of millions of blocks.  A
+    Gaussian is exp(-(x-cx)^2/2s^2) exp(-(y-cy)^2/2s^2) exp(-(z-cz)^2/2s^2);
+    an octave of value noise is the trilinear, smoothstep-weighted
+    interpolation of a seeded (4*2^o + 1)^3 lattice in [-1, 1] at amplitude
+    amp / 2^o.  Float64 on the device, rounded to float32 by the caller."""
+
+    def __init__(self, seed, n_gauss=32, centre=(0.2, 0.8), sigma=(0.05, 0.15), noise_octaves=4,
+                 noise_amp=0.1):
+        rng = np.random.default_rng(seed)
+        self.c = rng.uniform(centre[0], centre[1], size=(n_gauss, 3))
+        self.s = rng.uniform(sigma[0], sigma[1], size=n_gauss)
+        self.amp = noise_amp
+        self.lat = [rng.uniform(-1.0, 1.0, size=(4 * 2 ** o + 1,) * 3) for o in range(noise_octaves)]
+
+    def blocks(self, torch, level, depth, idx, nodes=None):
+        """Values at the nodes of blocks idx ((B, 3) int64 tensor, x y z) of
+        the level: (B, m, m, m) float64 indexed [z, y, x], m = 2^depth + 1.
+        nodes: optional node offsets per axis (default 0..2^depth)."""
+        dev = idx.device
+        g = float(2 ** (level + depth))
+        loc = torch.arange(2 ** depth + 1, device=dev, dtype=torch.int64) if nodes is None else nodes
+        m = loc.shape[0]
+        gi = idx[:, :, None] * (2 ** depth) + loc[None, None, :]      # (B, 3, m) global node index
+        u = gi.double() / g
+        c = torch.tensor(self.c, dtype=torch.float64, device=dev)       # (K, 3)
+        s = torch.tensor(self.s, dtype=torch.float64, device=dev)
+        e = torch.exp(-(u[None] - c[:, None, :, None]) ** 2 / (2.0 * s[:, None, None, None] ** 2))  # (K,B,3,m)
+        ex, ey, ez = e[:, :, 0], e[:, :, 1], e[:, :, 2]                # (K, B, m)
+        B = idx.shape[0]
+        K = c.shape[0]
+        zy = (ez[:, :, :, None] * ey[:, :, None, :]).reshape(K, B, m * m).permute(1, 2, 0).contiguous()
+        f = torch.bmm(zy, ex.permute(1, 0, 2).contiguous()).reshape(B, m, m, m)
+        for o, L in enumerate(self.lat):
+            cells = L.shape[0] - 1
+            x = u * cells                                               # (B, 3, m)
+            i0 = torch.clamp(torch.floor(x).long(), max=cells - 1)
+            t = x - i0
+            w1 = t * t * (3.0 - 2.0 * t)
+            base = i0.min(dim=2).values                                 # (B, 3): first cell of the block
+            rel = i0 - base[:, :, None]                                 # cell of a node, from the first
+            # lattice points a block can touch: the cells its 2^depth node
+            # steps can span, plus the far corner of the last one
+            P = -(-(int(loc.max()) - int(loc.min())) * cells // int(g)) + 2
+            assert int(rel.max()) + 1 < P
+            W = torch.zeros((B, 3, m, P), dtype=torch.float64, device=dev)
+            W.scatter_add_(3, rel[..., None], (1.0 - w1)[..., None])
+            W.scatter_add_(3, rel[..., None] + 1, w1[..., None])
+            Lt = torch.tensor(L, dtype=torch.float64, device=dev)       # [x, y, z]
+            rp = torch.arange(P, device=dev)
+            ix = torch.clamp(base[:, 0, None] + rp, max=cells)          # (B, P)
+            iy = torch.clamp(base[:, 1, None] + rp, max=cells)
+            iz = torch.clamp(base[:, 2, None] + rp, max=cells)
+            lb = Lt[ix[:, :, None, None], iy[:, None, :, None], iz[:, None, None, :]]  # (B, Px, Py, Pz)
+            # contract x, y, z with the per-axis weights -> (B, z, y, x)
+            v = torch.einsum("bijk,bxi->bxjk", lb, W[:, 0])
+            v = torch.einsum("bxjk,byj->bxyk", v, W[:, 1])
+            v = torch.einsum("bxyk,bzk->bzyx", v, W[:, 2])
+            f = f + (self.amp / 2 ** o) * v
+        return f
+
+
 # ---------------------------------------------------------------- trees
 
 def full_subtree(depth):

@@ -1,0 +1,173 @@
+"""Large single-GPU share: the blocks of one level with the largest centre
+values (12%), each a full depth-4 subtree (17^3 sparse nodes, fp16 brick),
+field = seeded Gaussians + value noise (configs.large_share).
+
+* reduced size (level 5: 3932 blocks, 384x216, two cameras): the batched
+  frame, resident and interpolating, equals the oracle's float composite of
+  the fp16 bricks bit for bit;
+* colour-table adjoint (level 4, 256x144, 48x48 crop): equals torch autograd
+  of the float64 reference of the same forward within 1e-4 relative;
+* full size (level 8: about 2.0 M blocks, 3840x2160), through properties
+  that do not depend on size: every brick equals its nodes' sparse values
+  rounded to fp16 (a full subtree makes the interpolation a scatter), the
+  interpolating frame equals the resident one, two wave footprints give the
+  same frame, and the colour-table gradient of several views is the sum of
+  the per-view gradients (to float-atomic rounding)."""
+import numpy as np
+import pytest
+
+from revise_amd import api, configs, datasets
+from tests.adjoint_ref import SampleTable
+from tests.gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+FOV, THR, Q, CUT, ACC = 30.0, 0.5, 2.0, 0.99, 0.9999
+
+
+def _share(level, seed):
+    import torch
+    ls = configs.large_share(torch, seed, level=level, depth=4, occupancy=0.12, n_gauss=32, octaves=4)
+    api.InterpBatch(ls.brick_jobs()).run()
+    torch.cuda.synchronize()
+    return ls
+
+
+def _scatter_check(torch, ls):
+    nn = 17
+    fi = torch.from_numpy(((ls.n2i[:, 2].astype(np.int64) * nn + ls.n2i[:, 1]) * nn + ls.n2i[:, 0])).cuda()
+    N = ls.n2i.shape[0]
+    nb = ls.keys.shape[0]
+    for c0 in range(0, nb, 1 << 18):
+        c1 = min(nb, c0 + (1 << 18))
+        assert torch.equal(ls.bricks[c0:c1, :nn ** 3][:, fi], ls.sparse[c0:c1, :N].half())
+
+
+def test_large_share_reduced_matches_oracle(orc):
+    import torch
+    ls = _share(5, seed=51)
+    nb = ls.keys.shape[0]
+    say("level-5 share", nb, "blocks")
+    assert nb == round(0.12 * 32 ** 3)
+    _scatter_check(torch, ls)
+    W, H = 384, 216
+    tf_np = datasets.seeded_tf(51, 256)
+    tf = torch.from_numpy(tf_np).cuda()
+    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf)
+    cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((35.0, 15.0), (220.0, -25.0))])
+    fp.set_cameras(cams)
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    bricks = ls.bricks[:, :17 ** 3].float().cpu().numpy()
+    keys = np.concatenate([np.full((nb, 1), 5, np.uint32), ls.keys.astype(np.uint32)], 1)
+    for c in range(2):
+        order = api.mixed_back_to_front(keys, api.eye_from_view(cams[c]))
+        lst = [(5, tuple(int(v) for v in ls.keys[i]), 4, bricks[i]) for i in order]
+        exp = orc.render_float_composite(cams[c], W, H, FOV, lst, ls.fmin, ls.fmax, THR, Q, CUT, tf_np, ACC)
+        assert (exp[..., 3] > 0).mean() > 0.03
+        for interp in (False, True):
+            say("level-5 frame cam", c, "interpolate", interp)
+            fp.render(c, argb, rgba, interpolate=interp)
+            torch.cuda.synchronize()
+            assert fp.status() == 0
+            assert np.array_equal(rgba.cpu().numpy(), exp)
+    fp.close()
+
+
+def test_large_share_tf_adjoint_matches_float64(orc):
+    import torch
+    ls = _share(4, seed=44)
+    W, H = 256, 144
+    crop = (104, 48, 48, 48)
+    tf_np = datasets.seeded_tf(44, 256)
+    rng = np.random.default_rng(45)
+    tf_t = np.clip(tf_np + rng.normal(0.0, 0.05, size=tf_np.shape), 0.0, 1.0).astype(np.float32)
+    M = datasets.orbit_camera(120.0, elevation_deg=20.0)
+    tf = torch.from_numpy(tf_np).cuda()
+    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf)
+    fp.set_cameras(M[None])
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    target = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    tf.copy_(torch.from_numpy(tf_t))
+    fp.render(0, argb, target, interpolate=False)
+    tf.copy_(torch.from_numpy(tf_np))
+    fp.set_tape(True)
+    rgba = torch.empty_like(target)
+    fp.render(0, argb, rgba, interpolate=False)
+    x0, y0, cw, ch = crop
+    mask = torch.zeros((H, W, 1), dtype=torch.float32, device="cuda")
+    for j in range(ch):
+        mask[H - 1 - (y0 + j), x0:x0 + cw] = 1.0
+    gI = (rgba - target) * mask * mask
+    gtf = torch.zeros((256, 4), dtype=torch.float32, device="cuda")
+    fp.adjoint(0, rgba, gI, gtf)
+    torch.cuda.synchronize()
+    g_tf = gtf.double().cpu().numpy()
+    # float64 reference over the crop's samples
+    blocks = [{"level": 4, "index": tuple(int(v) for v in k), "depth": 4} for k in ls.keys]
+    bricks_np = ls.bricks[:, :17 ** 3].float().cpu().numpy()
+    table = SampleTable(orc, blocks, bricks_np, M, W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf_np, crop)
+    say("samples", table.pix.size, "blocks touched", table.touched.size)
+    assert table.pix.size > 1000
+    tf64 = torch.tensor(tf_np, dtype=torch.float64, device="cuda", requires_grad=True)
+    b64 = torch.tensor(bricks_np, dtype=torch.float64, device="cuda")
+    img = table.forward(torch, tf64, b64, "cuda")
+    rows = [H - 1 - (y0 + j) for j in range(ch)]
+    tgt = target[rows][:, x0:x0 + cw].double()
+    got = rgba[rows][:, x0:x0 + cw].double()
+    assert float((got - img.detach()).abs().max()) < 2e-3
+    (0.5 * ((img - tgt) ** 2).sum()).backward()
+    r_tf = tf64.grad.cpu().numpy()
+    say("tf grad rel err", np.linalg.norm(g_tf - r_tf) / np.linalg.norm(r_tf))
+    assert np.linalg.norm(r_tf) > 0
+    assert np.linalg.norm(g_tf - r_tf) <= 1e-4 * np.linalg.norm(r_tf)
+    fp.close()
+
+
+def test_large_share_full_size_properties():
+    import torch
+    say("level-8 share: generating")
+    ls = _share(8, seed=2029)
+    nb = ls.keys.shape[0]
+    say("level-8 share", nb, "blocks")
+    assert nb == round(0.12 * 256 ** 3)
+    _scatter_check(torch, ls)
+    W, H = 3840, 2160
+    tf_np = datasets.seeded_tf(2029, 256)
+    tf = torch.from_numpy(tf_np).cuda()
+    rng = np.random.default_rng(2029)
+    views = np.stack([datasets.orbit_camera(float(a), elevation_deg=float(e))
+                      for a, e in zip(rng.uniform(0, 360, 3), rng.uniform(-30, 30, 3))])
+    frames = {}
+    for tile in ((8, 8), (16, 4)):
+        fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf, tile)
+        fp.set_cameras(views)
+        argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        say("level-8 frame, tile", tile)
+        fp.render(0, argb, None, interpolate=False)
+        frames[tile] = argb.clone()
+        if tile == (8, 8):
+            fp.render(0, argb, None, interpolate=True)
+            torch.cuda.synchronize()
+            assert fp.status() == 0
+            assert torch.equal(argb, frames[tile])
+            assert float((argb != 0).float().mean()) > 0.02
+            # colour-table gradients: several views at once = sum of each
+            fp.set_tape(True)
+            rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+            gsum = torch.zeros((256, 4), dtype=torch.float32, device="cuda")
+            each = []
+            for v in range(3):
+                fp.render(v, argb, rgba, interpolate=False)
+                g = torch.zeros_like(gsum)
+                gI = rgba * 0.5 - 0.25
+                fp.adjoint(v, rgba, gI, g)
+                fp.adjoint(v, rgba, gI, gsum)
+                each.append(g)
+            torch.cuda.synchronize()
+            tot = sum(each)
+            scale = float(tot.abs().max())
+            assert scale > 0 and bool(torch.isfinite(gsum).all())
+            assert float((gsum - tot).abs().max()) <= 1e-4 * scale
+        fp.close()
+    assert torch.equal(frames[(8, 8)], frames[(16, 4)])
