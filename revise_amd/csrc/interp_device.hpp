@@ -286,6 +286,40 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
     }
 
     // 4. store the owned half-open box (+ far faces of the brick)
+    if (g.tiles_x == 1u && g.tiles_yz == 1u) {
+        // the tile is the whole brick: LDS and brick layouts coincide, one
+        // contiguous run (fp16: two values per 4-byte store when aligned)
+        if (job.dense16 && (reinterpret_cast<uintptr_t>(job.dense16) & 3u) == 0u) {
+            const uint32_t pairs = tile_nodes >> 1;
+            __half2* out2 = reinterpret_cast<__half2*>(job.dense16);
+            const float2* A2 = reinterpret_cast<const float2*>(A);
+            for (uint32_t q = threadIdx.x; q < pairs; q += NT) {
+                float2 v = A2[q];
+                if (v.x == kNoField)
+                    v.x = __builtin_nanf("");
+                if (v.y == kNoField)
+                    v.y = __builtin_nanf("");
+                out2[q] = __halves2half2(__float2half(v.x), __float2half(v.y));
+            }
+            if ((tile_nodes & 1u) && threadIdx.x == 0) {
+                float v = A[tile_nodes - 1u];
+                if (v == kNoField)
+                    v = __builtin_nanf("");
+                job.dense16[tile_nodes - 1u] = __float2half(v);
+            }
+            return;
+        }
+        for (uint32_t q = threadIdx.x; q < tile_nodes; q += NT) {
+            float v = A[q];
+            if (v == kNoField)
+                v = __builtin_nanf("");
+            if (job.dense16)
+                job.dense16[q] = __float2half(v);
+            else
+                __builtin_nontemporal_store(v, job.dense + q);
+        }
+        return;
+    }
     const uint32_t ox = (tx + 1u == g.tiles_x) ? rowlen : X;
     const uint32_t oy = (ty + 1u == g.tiles_yz) ? tp1 : T;
     const uint32_t oz = (tz + 1u == g.tiles_yz) ? tp1 : T;

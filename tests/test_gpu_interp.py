@@ -152,3 +152,24 @@ def test_batch_rejects_plan_of_another_depth():
     junk = torch.zeros(256, dtype=torch.uint8, device="cuda")
     with pytest.raises(RvError, match="not an interpolation plan"):
         api.InterpBatch([(to_dev(vals), junk, dense, d)])
+
+
+@pytest.mark.parametrize("depth,p,offset", [(4, 1.0, 0), (4, 1.0, 1), (3, 0.6, 1), (2, 0.8, 0), (4, 0.5, 3)])
+def test_interp_fp16_whole_brick_tile_alignment(orc, depth, p, offset):
+    """Bricks of depth <= 4 are one tile, stored as one run: fp16 pairs when
+    the brick is 4-byte aligned, single values otherwise (offset in fp16
+    elements into a larger allocation)."""
+    import torch
+    d, n2i, vals = _case(200 + 10 * depth + offset, depth, p, 0.02)
+    V = api.brick_nodes(d)
+    buf = torch.full((V + 8,), -7.0, dtype=torch.float16, device="cuda")
+    dense = buf[offset:offset + V]
+    plan = api.interp_plan(n2i, d)
+    b = api.InterpBatch([(to_dev(vals), plan, dense, d)])
+    b.run()
+    torch.cuda.synchronize()
+    got = buf.cpu().numpy().astype(np.float32)
+    e = orc.interpolate(vals, n2i, d).astype(np.float16).astype(np.float32)
+    assert bits_equal_nan(got[offset:offset + V], e)
+    assert (got[:offset] == -7.0).all() and (got[offset + V:] == -7.0).all()
+    b.close()
