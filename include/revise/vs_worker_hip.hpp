@@ -104,6 +104,12 @@ struct RenderSharedState {
     float alphaCutoff = 0.996f;
     std::array<float, 3> backgroundColor{0.f, 0.f, 0.f};
     std::vector<float> colorTable; // N x 4 RGBA (makeColorTransferVector output)
+    // VsControllerInput::fieldMode / fieldAllParam().isosurfaceLevel() /
+    // isosurfaceOpacity() (VsControllerInput.hpp:208-251): a RV_FIELD_* mode;
+    // the isosurface level is relative to the field range like threshold.
+    uint32_t fieldMode = RV_FIELD_ARGB;
+    float isosurfaceLevel = 0.5f;
+    float isosurfaceOpacity = 1.f;
 };
 
 class DenseFieldRendererHip {
@@ -159,6 +165,13 @@ public:
                      uint64_t count, const std::function<bool()>& isCancelled);
     bool lastCancelled() const { return m_cancelled; }
     rv_stream_t stream() const { return m_renderer.stream(); }
+    // The device colour table of the current state (uploaded when it changed).
+    const float* colorTable()
+    {
+        m_renderer.ensureColorTable();
+        return m_renderer.colorTable();
+    }
+    uint32_t colorTableSize() const { return m_renderer.colorTableSize(); }
     // fused (default): the share's bricks are interpolated in one batched
     // launch and ray-cast in one launch (rv_render_share_argb), bit-identical
     // to the per-block loop; per-block: one renderDenseField call per block,

@@ -187,6 +187,54 @@ int rv_clear_viewport(uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, rv_str
 int rv_render_block_argb(const rv_block_params* params, const float* d_brick, const float* d_tf,
                          uint32_t tf_n, uint32_t* d_viewport, rv_stream_t stream);
 
+/* Field modes: s3vs::FieldMode (FieldMode.hpp:26-51), the same numbering.
+ * Argb, ArgbLight, Isosurface and MaxIntensityProjection are rendered. */
+enum {
+    RV_FIELD_ISOSURFACE = 0,
+    RV_FIELD_ISOSURFACES = 1,
+    RV_FIELD_MAX_INTENSITY_PROJECTION = 2,
+    RV_FIELD_ARGB = 3,
+    RV_FIELD_ARGB_LIGHT = 4,
+    RV_FIELD_DOMAIN_VOXELS = 5,
+    RV_FIELD_VALUE_ON_ISOSURFACE = 6,
+    RV_FIELD_VALUE_ON_ISOSURFACES = 7
+};
+
+/* Ray-cast one block in a field mode.  Replaces renderKernel<fieldMode>
+ * (DenseFieldRenderer_cu.cu:62-215) with renderArgb (renderArgb.hpp:30-55),
+ * renderArgbLight (renderArgbLight.hpp:29-63, lighting of
+ * cuda_lighting.hpp:27-95 with the light of DenseFieldRenderer_cu.cu:440-445),
+ * renderIsosurface (renderIsosurface.hpp:31-61; the level is
+ * params->threshold, i.e. set up with threshold_rel = the isosurface level)
+ * and renderMaxIntensityProjection (renderMaxIntensityProjection.hpp:31-48;
+ * the viewport holds float bits, cleared to NaN by rv_clear_viewport_mode).
+ * iso_opacity scales the isosurface colour's alpha (isosurfaceOpacity).
+ * Every segment stops at params->alpha_cutoff (the reference: 0.996). */
+int rv_render_block(const rv_block_params* params, uint32_t field_mode, float iso_opacity,
+                    const float* d_brick, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
+                    rv_stream_t stream);
+
+/* clearViewport of a field mode (DenseFieldRenderer_cu.cu:268-277): 0, or NaN
+ * bits for MaxIntensityProjection. */
+int rv_clear_viewport_mode(uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, uint32_t field_mode,
+                           rv_stream_t stream);
+
+/* blendImagePartMaxIntensity (BlendImagePart.hpp:148-191) on device float
+ * images: the larger value wins, NaN is empty; arguments as
+ * rv_blend_image_part. */
+int rv_blend_image_part_max_intensity(float* d_dst, int32_t dw, int32_t dh, int32_t dst_off_x,
+                                      int32_t dst_off_y, const float* d_src, int32_t sw,
+                                      int32_t sh, int32_t src_x, int32_t src_y, int just_copy,
+                                      rv_stream_t stream);
+
+/* makeMaxIntensityProjection (VsRenderer.cpp:542-565): the non-NaN values of
+ * a float part at (src_x, src_y) become ARGB8 colours of the nctv-entry
+ * colour vector d_ctv (s3vs::calcColor, ColorTransferFunction.hpp:83-94) in
+ * the dw x dh frame d_dst. */
+int rv_mip_colorize(uint32_t* d_dst, int32_t dw, int32_t dh, const float* d_src, int32_t sw,
+                    int32_t sh, int32_t src_x, int32_t src_y, const float* d_ctv, uint32_t nctv,
+                    rv_stream_t stream);
+
 /* Adds to *d_count (a device uint64) the number of samples the block's rays
  * take without early termination: the sum over its view rectangle of
  * (sampleCount - 1) (DenseFieldRenderer_cu.cu:171-175, renderArgb.hpp:40).
@@ -205,6 +253,12 @@ uint64_t rv_share_workspace_bytes(uint32_t nblocks);
 int rv_render_share_argb(const rv_block_params* params, const float* const* bricks,
                          uint32_t nblocks, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
                          uint32_t vp_w, uint32_t vp_h, void* d_workspace, rv_stream_t stream);
+/* The same in a field mode of rv_render_block: the result equals one
+ * rv_render_block call per block, in the given order. */
+int rv_render_share(const rv_block_params* params, const float* const* bricks, uint32_t nblocks,
+                    uint32_t field_mode, float iso_opacity, const float* d_tf, uint32_t tf_n,
+                    uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, void* d_workspace,
+                    rv_stream_t stream);
 
 /* Piecewise-linear colour table from control points.  Replaces
  * s3vs::makeColorTransferVector (ColorTransferFunction.hpp:27-69).
@@ -340,6 +394,12 @@ int rv_frame_get_tile(const rv_frame_plan* plan, uint32_t* tile_w, uint32_t* til
 int rv_frame_autotune(rv_frame_plan* plan, uint32_t* d_argb, int interpolate,
                       const char* cache_path, rv_stream_t stream, uint32_t* tile_w,
                       uint32_t* tile_h);
+/* Field mode of later frames (default RV_FIELD_ARGB): Argb, ArgbLight and
+ * Isosurface segments are composited front to back (blendBehind); in
+ * MaxIntensityProjection the frame's d_argb receives the float bits of the
+ * largest normalised value over the blocks (NaN: none), the order of blocks
+ * being immaterial.  The tape and the adjoint are Argb only. */
+int rv_frame_set_field_mode(rv_frame_plan* plan, uint32_t field_mode, float iso_opacity);
 void rv_frame_plan_destroy(rv_frame_plan* plan);
 
 /* Adjoint ray-cast (no reference counterpart: the reference renders only).
@@ -382,6 +442,9 @@ typedef struct rv_render_state {
     float background[3];     /* backgroundColor */
     const float* tf;         /* host colour table, tf_n x 4 */
     uint32_t tf_n;
+    uint32_t field_mode;     /* RV_FIELD_* (fieldMode); note 0 is Isosurface */
+    float iso_level_rel;     /* isosurfaceLevel, relative to the field range */
+    float iso_opacity;       /* isosurfaceOpacity */
 } rv_render_state;
 
 /* VsWorkerInterface::initialize: selects GPU thread_id % device count

@@ -966,6 +966,149 @@ uint32_t blend_above_u32(uint32_t argb, const float* rgba)
            ((unsigned)(ex[1] * 255.999f) << 8) + (unsigned)(ex[2] * 255.999f);
 }
 
+/* ---- field modes (s3vs::FieldMode numbering, FieldMode.hpp:26-51) ---- */
+enum { kIso = 0, kMip = 2, kArgb = 3, kArgbLight = 4 };
+
+/* cuda_vec_op.hpp:48-52 */
+V3 normalize3(V3 a)
+{
+    float il = 1.f / std::sqrt(dot(a, a));
+    return {a.x * il, a.y * il, a.z * il};
+}
+
+/* clamp.hpp:24-34: max(0, min(1, x)), a NaN operand ignored */
+float clamp01(float x) { return std::fmax(0.f, std::fmin(1.f, x)); }
+
+/* computeFragColor + blinn, cuda_lighting.hpp:27-95, with the one light of
+ * DenseFieldRenderer_cu.cu:440-445 (at pos + n + e2 h; ambient 0, diffuse 1,
+ * specular 0.1); Ka = 1, Ks = 0.5, shininess 100. */
+void frag_color(const orc_block_params* p, const Derived& d, const float* brick,
+                const float* diffuse, V3 surf, V3 tc, float* out)
+{
+    const float del = 0.1f;
+    V3 a{fetch3(brick, d.n, {tc.x - del, tc.y, tc.z}), fetch3(brick, d.n, {tc.x, tc.y - del, tc.z}),
+         fetch3(brick, d.n, {tc.x, tc.y, tc.z - del})};
+    V3 b{fetch3(brick, d.n, {tc.x + del, tc.y, tc.z}), fetch3(brick, d.n, {tc.x, tc.y + del, tc.z}),
+         fetch3(brick, d.n, {tc.x, tc.y, tc.z + del})};
+    V3 N = normalize3(sub(a, b));
+    V3 eye{p->pos[0], p->pos[1], p->pos[2]};
+    V3 light = add(add(eye, V3{p->n[0], p->n[1], p->n[2]}), mul(V3{p->e2[0], p->e2[1], p->e2[2]}, p->h));
+    V3 V = normalize3(sub(eye, surf));
+    V3 L = normalize3(sub(light, surf));
+    if (dot(L, N) < 0.f)
+        N = {-N.x, -N.y, -N.z};
+    float diff = std::fmax(dot(L, N), 0.f);
+    V3 H = normalize3(add(L, V));
+    float spec = diff > 0.f ? orc_powf(std::fmax(dot(H, N), 0.f), 100.f) : 0.f;
+    float ks = 0.5f * 0.1f;
+    for (int k = 0; k < 3; ++k) {
+        float c = (1.f * 0.f + (diffuse[k] * 1.f) * diff) + ks * spec;
+        out[k] = clamp01(0.f + c);
+    }
+    out[3] = diffuse[3];
+}
+
+void blend_behind4(float* acc, const float* c)
+{
+    float da = c[3] * (1.f - acc[3]);
+    float a = acc[3] + da;
+    if (a > 0.f) {
+        float ka = acc[3] / a, kb = da / a;
+        for (int k = 0; k < 3; ++k)
+            acc[k] = acc[k] * ka + c[k] * kb;
+    }
+    acc[3] = a;
+}
+
+/* renderArgbLight loop, renderArgbLight.hpp:29-63 */
+void march_light(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
+                 uint32_t tf_n, Ray& r, float* rgba)
+{
+    rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
+    const float range = p->fmax - p->fmin;
+    for (int i = 1; i < r.sampleCount; ++i, r.pos = add(r.pos, r.step)) {
+        V3 tc = mul(sub(r.pos, d.origin), d.tcf);
+        float v = fetch3(brick, d.n, tc);
+        if (std::isnan(v))
+            continue;
+        float c[4];
+        fetch_tf(tf, tf_n, (v - p->fmin) / range, c);
+        c[3] = 1.f - orc_powf(1.f - c[3] * d.opacity_factor, d.i_layer_count);
+        if (c[3] >= 0.00001f) {
+            float col[4];
+            frag_color(p, d, brick, c, r.pos, tc, col);
+            blend_behind4(rgba, col);
+            if (rgba[3] > p->alpha_cutoff)
+                break;
+        }
+    }
+}
+
+/* renderIsosurface loop, renderIsosurface.hpp:31-61: the surface point is the
+ * second sample's position plus step * (level - v1) / (v2 - v1). */
+void march_iso(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
+               uint32_t tf_n, float iso_opacity, Ray& r, float* rgba)
+{
+    rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
+    float v1 = fetch3(brick, d.n, mul(sub(r.pos, d.origin), d.tcf));
+    r.pos = add(r.pos, r.step);
+    const float lv = p->threshold;
+    float diffuse[4];
+    fetch_tf(tf, tf_n, (lv - p->fmin) / (p->fmax - p->fmin), diffuse);
+    diffuse[3] = diffuse[3] * iso_opacity;
+    for (int i = 1; i < r.sampleCount; ++i, r.pos = add(r.pos, r.step)) {
+        float v2 = fetch3(brick, d.n, mul(sub(r.pos, d.origin), d.tcf));
+        if (!std::isnan(v1) && !std::isnan(v2) && (v1 - lv) * (v2 - lv) < 0.f) {
+            float t = (lv - v1) / (v2 - v1);
+            V3 x = add(r.pos, mul(r.step, t));
+            float col[4];
+            frag_color(p, d, brick, diffuse, x, mul(sub(x, d.origin), d.tcf), col);
+            blend_behind4(rgba, col);
+            if (rgba[3] > p->alpha_cutoff)
+                break;
+        }
+        v1 = v2;
+    }
+}
+
+/* renderMaxIntensityProjection.hpp:31-37 */
+float march_mip(const orc_block_params* p, const Derived& d, const float* brick, Ray& r)
+{
+    float m = p->fmin - (p->fmax - p->fmin);
+    for (int i = 1; i < r.sampleCount; ++i, r.pos = add(r.pos, r.step)) {
+        float v = fetch3(brick, d.n, mul(sub(r.pos, d.origin), d.tcf));
+        if (v >= p->threshold && m < v)
+            m = v;
+    }
+    return m;
+}
+
+/* renderMaxIntensityProjection.hpp:39-48: float bits in the viewport */
+uint32_t mip_merge(uint32_t prev_bits, float m, const orc_block_params* p)
+{
+    if (m < p->fmin)
+        return prev_bits;
+    float t = (m - p->fmin) / (p->fmax - p->fmin);
+    float prev;
+    std::memcpy(&prev, &prev_bits, 4);
+    if (!std::isnan(prev) && prev > t)
+        return prev_bits;
+    uint32_t bits;
+    std::memcpy(&bits, &t, 4);
+    return bits;
+}
+
+void segment_mode(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
+                  uint32_t tf_n, uint32_t mode, float iso_opacity, Ray& r, float* rgba)
+{
+    if (mode == kArgbLight)
+        march_light(p, d, brick, tf, tf_n, r, rgba);
+    else if (mode == kIso)
+        march_iso(p, d, brick, tf, tf_n, iso_opacity, r, rgba);
+    else
+        march(p, d, brick, tf, tf_n, r, rgba);
+}
+
 } // namespace
 
 extern "C" void orc_render_block(const orc_block_params* p, const float* brick, const float* tf,
@@ -984,6 +1127,81 @@ extern "C" void orc_render_block(const orc_block_params* p, const float* brick, 
             viewport[r.pixel] = blend_above_u32(viewport[r.pixel], rgba);
         }
     }
+}
+
+/* renderKernel<fieldMode> for one block in a field mode (Argb, ArgbLight,
+ * Isosurface, MaxIntensityProjection), DenseFieldRenderer_cu.cu:62-215. */
+extern "C" void orc_render_block_mode(const orc_block_params* p, uint32_t mode, float iso_opacity,
+                                      const float* brick, const float* tf, uint32_t tf_n,
+                                      uint32_t* viewport, uint32_t row0, uint32_t row1)
+{
+    Derived d = derive(p);
+    for (uint32_t vy = p->view_y; vy < p->view_y + p->view_h; ++vy) {
+        if (vy < row0 || vy >= row1)
+            continue;
+        for (uint32_t vx = p->view_x; vx < p->view_x + p->view_w; ++vx) {
+            Ray r = setup_ray(p, d, vx, vy);
+            if (!r.hit)
+                continue;
+            if (mode == kMip) {
+                viewport[r.pixel] = mip_merge(viewport[r.pixel], march_mip(p, d, brick, r), p);
+                continue;
+            }
+            float rgba[4];
+            segment_mode(p, d, brick, tf, tf_n, mode, iso_opacity, r, rgba);
+            viewport[r.pixel] = blend_above_u32(viewport[r.pixel], rgba);
+        }
+    }
+}
+
+/* blendImagePartMaxIntensity, BlendImagePart.hpp:148-191 */
+extern "C" void orc_blend_image_part_max_intensity(float* dst, int32_t dw, const float* src,
+                                                   int32_t sw, int32_t sh, int32_t dx, int32_t dy,
+                                                   int just_copy)
+{
+    for (int32_t y = 0; y < sh; ++y)
+        for (int32_t x = 0; x < sw; ++x) {
+            float s = src[(size_t)y * sw + x];
+            float& dv = dst[(size_t)dw * (dy + y) + dx + x];
+            if (just_copy) {
+                dv = s;
+                continue;
+            }
+            if (std::isnan(s))
+                continue;
+            if (std::isnan(dv) || dv < s)
+                dv = s;
+        }
+}
+
+/* makeMaxIntensityProjection, VsRenderer.cpp:542-565, with s3vs::calcColor,
+ * ColorTransferFunction.hpp:83-94, and floatRgbaToUint32Argb */
+extern "C" void orc_mip_colorize(uint32_t* dst, int32_t dw, const float* src, int32_t sw, int32_t sh,
+                                 int32_t dx, int32_t dy, const float* ctv, uint32_t nctv)
+{
+    for (int32_t y = 0; y < sh; ++y)
+        for (int32_t x = 0; x < sw; ++x) {
+            float s = src[(size_t)y * sw + x];
+            if (std::isnan(s))
+                continue;
+            float c[4];
+            if (s <= 0.f || nctv == 1u) {
+                for (int k = 0; k < 4; ++k)
+                    c[k] = ctv[k];
+            } else if (s >= 1.f) {
+                for (int k = 0; k < 4; ++k)
+                    c[k] = ctv[4 * (nctv - 1) + k];
+            } else {
+                float par = s * (float)(nctv - 1);
+                size_t idx = (size_t)par;
+                float t = par - (float)idx;
+                for (int k = 0; k < 4; ++k)
+                    c[k] = ctv[4 * idx + k] * (1 - t) + ctv[4 * (idx + 1) + k] * t;
+            }
+            dst[(size_t)dw * (dy + y) + dx + x] =
+                ((unsigned)(c[3] * 255.999f) << 24) + ((unsigned)(c[0] * 255.999f) << 16) +
+                ((unsigned)(c[1] * 255.999f) << 8) + (unsigned)(c[2] * 255.999f);
+        }
 }
 
 extern "C" int orc_ray_segment(const orc_block_params* p, const float* brick, const float* tf,
@@ -1280,12 +1498,13 @@ extern "C" void orc_render_share(const orc_frame_desc* d, const float* const* br
     });
 }
 
-extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, uint32_t vp_h,
-                                           float fov_y_deg, const orc_block_ref* blocks,
-                                           uint32_t nblocks, float fmin, float fmax,
-                                           float threshold_rel, float quality, float alpha_cutoff,
-                                           float acc_cutoff, const float* tf, uint32_t tf_n,
-                                           float* out, uint32_t nthreads)
+extern "C" void orc_render_float_composite_mode(const float* view16, uint32_t vp_w, uint32_t vp_h,
+                                                float fov_y_deg, const orc_block_ref* blocks,
+                                                uint32_t nblocks, float fmin, float fmax,
+                                                float threshold_rel, float quality,
+                                                float alpha_cutoff, float acc_cutoff,
+                                                const float* tf, uint32_t tf_n, uint32_t mode,
+                                                float iso_opacity, float* out, uint32_t nthreads)
 {
     std::vector<orc_block_params> params(nblocks);
     std::vector<Derived> der(nblocks);
@@ -1301,6 +1520,25 @@ extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, u
             uint32_t vy = vp_h - 1 - row;
             for (uint32_t vx = 0; vx < vp_w; ++vx) {
                 float acc[4] = {0, 0, 0, 0};
+                if (mode == kMip) {
+                    /* the largest normalised value over the blocks */
+                    uint32_t bits = 0x7fc00000u;
+                    for (uint32_t bi = 0; bi < nblocks; ++bi) {
+                        const orc_block_params* p = &params[bi];
+                        if (!live[bi] || vx < p->view_x || vx >= p->view_x + p->view_w ||
+                            vy < p->view_y || vy >= p->view_y + p->view_h)
+                            continue;
+                        Ray r = setup_ray(p, der[bi], vx, vy);
+                        if (r.hit)
+                            bits = mip_merge(bits, march_mip(p, der[bi], blocks[bi].brick, r), p);
+                    }
+                    float t;
+                    std::memcpy(&t, &bits, 4);
+                    float* o = out + 4 * ((size_t)row * vp_w + vx);
+                    for (int k = 0; k < 4; ++k)
+                        o[k] = t;
+                    continue;
+                }
                 /* front to back: the reverse of the back-to-front list */
                 for (uint32_t bi = nblocks; bi-- > 0;) {
                     if (acc[3] > acc_cutoff)
@@ -1315,7 +1553,7 @@ extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, u
                     if (!r.hit)
                         continue;
                     float seg[4];
-                    march(p, der[bi], blocks[bi].brick, tf, tf_n, r, seg);
+                    segment_mode(p, der[bi], blocks[bi].brick, tf, tf_n, mode, iso_opacity, r, seg);
                     /* acc (in front) composited over seg: blendBehind(acc, seg) */
                     float da = seg[3] * (1.f - acc[3]);
                     float a = acc[3] + da;
@@ -1332,4 +1570,16 @@ extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, u
             }
         }
     });
+}
+
+extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, uint32_t vp_h,
+                                           float fov_y_deg, const orc_block_ref* blocks,
+                                           uint32_t nblocks, float fmin, float fmax,
+                                           float threshold_rel, float quality, float alpha_cutoff,
+                                           float acc_cutoff, const float* tf, uint32_t tf_n,
+                                           float* out, uint32_t nthreads)
+{
+    orc_render_float_composite_mode(view16, vp_w, vp_h, fov_y_deg, blocks, nblocks, fmin, fmax,
+                                    threshold_rel, quality, alpha_cutoff, acc_cutoff, tf, tf_n, kArgb,
+                                    1.f, out, nthreads);
 }

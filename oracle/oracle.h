@@ -101,6 +101,26 @@ int orc_block_params_setup(const float* view16, uint32_t vp_w, uint32_t vp_h, fl
 void orc_render_block(const orc_block_params* p, const float* brick, const float* tf,
                       uint32_t tf_n, uint32_t* viewport, uint32_t row0, uint32_t row1);
 
+/* renderKernel<fieldMode> over the view rectangle in a field mode (s3vs
+ * FieldMode numbering: 0 Isosurface, 2 MaxIntensityProjection, 3 Argb, 4
+ * ArgbLight): renderIsosurface.hpp:31-61, renderMaxIntensityProjection.hpp:
+ * 31-48, renderArgb.hpp:30-55, renderArgbLight.hpp:29-63 with
+ * cuda_lighting.hpp:27-95 and the light of DenseFieldRenderer_cu.cu:440-445.
+ * MaxIntensityProjection keeps float bits in the viewport (NaN: none). */
+void orc_render_block_mode(const orc_block_params* p, uint32_t mode, float iso_opacity,
+                           const float* brick, const float* tf, uint32_t tf_n, uint32_t* viewport,
+                           uint32_t row0, uint32_t row1);
+
+/* blendImagePartMaxIntensity (BlendImagePart.hpp:148-191) of a float part at
+ * (dx, dy) of a dw-wide float image. */
+void orc_blend_image_part_max_intensity(float* dst, int32_t dw, const float* src, int32_t sw,
+                                        int32_t sh, int32_t dx, int32_t dy, int just_copy);
+
+/* makeMaxIntensityProjection (VsRenderer.cpp:542-565): non-NaN values of a
+ * float part become ARGB8 colours of the nctv x 4 colour vector. */
+void orc_mip_colorize(uint32_t* dst, int32_t dw, const float* src, int32_t sw, int32_t sh,
+                      int32_t dx, int32_t dy, const float* ctv, uint32_t nctv);
+
 /* Per-ray float RGBA segment of one block (no viewport blend): the loop of
  * renderArgb.hpp:40-51.  Returns 1 if the pixel is covered, writes rgba[4]. */
 int orc_ray_segment(const orc_block_params* p, const float* brick, const float* tf,
@@ -176,6 +196,15 @@ void orc_render_float_composite(const float* view16, uint32_t vp_w, uint32_t vp_
                                 float fmax, float threshold_rel, float quality, float alpha_cutoff,
                                 float acc_cutoff, const float* tf, uint32_t tf_n, float* out_rgba,
                                 uint32_t nthreads);
+/* The same in a field mode: Argb, ArgbLight and Isosurface segments
+ * composited front to back; MaxIntensityProjection writes the float of the
+ * largest normalised value (NaN: none) to all 4 channels. */
+void orc_render_float_composite_mode(const float* view16, uint32_t vp_w, uint32_t vp_h,
+                                     float fov_y_deg, const orc_block_ref* blocks_b2f,
+                                     uint32_t nblocks, float fmin, float fmax, float threshold_rel,
+                                     float quality, float alpha_cutoff, float acc_cutoff,
+                                     const float* tf, uint32_t tf_n, uint32_t mode,
+                                     float iso_opacity, float* out_rgba, uint32_t nthreads);
 
 #ifdef __cplusplus
 }

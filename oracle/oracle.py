@@ -100,6 +100,17 @@ def lib():
                                                  C.POINTER(BlockRef), C.c_uint32, C.c_float, C.c_float,
                                                  C.c_float, C.c_float, C.c_float, C.c_float, f32p,
                                                  C.c_uint32, f32p, C.c_uint32]
+        L.orc_render_block_mode.argtypes = [C.POINTER(BlockParams), C.c_uint32, C.c_float, f32p, f32p,
+                                            C.c_uint32, u32p, C.c_uint32, C.c_uint32]
+        L.orc_blend_image_part_max_intensity.argtypes = [f32p, C.c_int32, f32p, C.c_int32, C.c_int32,
+                                                         C.c_int32, C.c_int32, C.c_int]
+        L.orc_mip_colorize.argtypes = [u32p, C.c_int32, f32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       f32p, C.c_uint32]
+        L.orc_render_float_composite_mode.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_float,
+                                                      C.POINTER(BlockRef), C.c_uint32, C.c_float,
+                                                      C.c_float, C.c_float, C.c_float, C.c_float,
+                                                      C.c_float, f32p, C.c_uint32, C.c_uint32, C.c_float,
+                                                      f32p, C.c_uint32]
         _lib = L
     return _lib
 
@@ -210,6 +221,32 @@ def render_block(params, brick, tf, viewport):
                            _p(viewport, C.c_uint32), 0, params.vp_h)
 
 
+def render_block_mode(params, mode, iso_opacity, brick, tf, viewport):
+    """Field modes: 0 Isosurface, 2 MaxIntensityProjection (float bits), 3 Argb, 4 ArgbLight."""
+    brick = np.ascontiguousarray(brick, dtype=np.float32)
+    tf = np.ascontiguousarray(tf, dtype=np.float32)
+    assert viewport.dtype == np.uint32 and viewport.flags.c_contiguous
+    lib().orc_render_block_mode(C.byref(params), mode, iso_opacity, _p(brick, C.c_float),
+                                _p(tf, C.c_float), tf.shape[0], _p(viewport, C.c_uint32), 0, params.vp_h)
+
+
+def blend_image_part_max_intensity(dst, src, dx, dy, just_copy=False):
+    """dst, src: 2-D float32 arrays (rows of the image); in place on dst."""
+    assert dst.dtype == np.float32 and src.dtype == np.float32 and dst.flags.c_contiguous
+    src = np.ascontiguousarray(src)
+    lib().orc_blend_image_part_max_intensity(_p(dst, C.c_float), dst.shape[1], _p(src, C.c_float),
+                                             src.shape[1], src.shape[0], dx, dy, int(just_copy))
+
+
+def mip_colorize(dst, src, dx, dy, ctv):
+    """dst: 2-D uint32 frame, src: 2-D float32 part at (dx, dy), ctv: n x 4."""
+    assert dst.dtype == np.uint32 and dst.flags.c_contiguous
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    ctv = np.ascontiguousarray(ctv, dtype=np.float32)
+    lib().orc_mip_colorize(_p(dst, C.c_uint32), dst.shape[1], _p(src, C.c_float), src.shape[1],
+                           src.shape[0], dx, dy, _p(ctv, C.c_float), ctv.shape[0])
+
+
 def ray_segment(params, brick, tf, vx, vy):
     out = np.zeros(4, dtype=np.float32)
     hit = lib().orc_ray_segment(C.byref(params), _p(brick, C.c_float), _p(tf, C.c_float), tf.shape[0],
@@ -290,7 +327,7 @@ def render_share(view16, vp_w, vp_h, fov, level, box_min, box_max, tex_depth, fm
 
 
 def render_float_composite(view16, vp_w, vp_h, fov, blocks_b2f, fmin, fmax, thr_rel, quality,
-                           alpha_cutoff, tf, acc_cutoff=2.0, nthreads=None):
+                           alpha_cutoff, tf, acc_cutoff=2.0, nthreads=None, mode=3, iso_opacity=1.0):
     """blocks_b2f: list of (level, (ix,iy,iz), tex_depth, brick) in back-to-front order."""
     v = np.ascontiguousarray(view16, dtype=np.float32).reshape(16)
     tf = np.ascontiguousarray(tf, dtype=np.float32)
@@ -305,7 +342,8 @@ def render_float_composite(view16, vp_w, vp_h, fov, blocks_b2f, fmin, fmax, thr_
         arr[i].tex_depth = td
         arr[i].brick = _p(b, C.c_float)
     out = np.zeros((vp_h, vp_w, 4), dtype=np.float32)
-    lib().orc_render_float_composite(_p(v, C.c_float), vp_w, vp_h, fov, arr, len(blocks_b2f), fmin,
-                                     fmax, thr_rel, quality, alpha_cutoff, acc_cutoff, _p(tf, C.c_float),
-                                     tf.shape[0], _p(out, C.c_float), nthreads or os.cpu_count())
+    lib().orc_render_float_composite_mode(_p(v, C.c_float), vp_w, vp_h, fov, arr, len(blocks_b2f), fmin,
+                                          fmax, thr_rel, quality, alpha_cutoff, acc_cutoff,
+                                          _p(tf, C.c_float), tf.shape[0], mode, iso_opacity,
+                                          _p(out, C.c_float), nthreads or os.cpu_count())
     return out

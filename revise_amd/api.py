@@ -216,6 +216,31 @@ def render_block_argb(params, d_brick, d_tf, d_vp, stream=None):
                                      ptr(d_vp), stream_ptr(stream)))
 
 
+# s3vs::FieldMode (FieldMode.hpp:26-51)
+FIELD_ISOSURFACE, FIELD_ISOSURFACES, FIELD_MIP, FIELD_ARGB, FIELD_ARGB_LIGHT = 0, 1, 2, 3, 4
+
+
+def render_block(params, mode, iso_opacity, d_brick, d_tf, d_vp, stream=None):
+    check(lib().rv_render_block(C.byref(params), mode, iso_opacity, ptr(d_brick), ptr(d_tf), d_tf.shape[0],
+                                ptr(d_vp), stream_ptr(stream)))
+
+
+def clear_viewport_mode(d_vp, w, h, mode, stream=None):
+    check(lib().rv_clear_viewport_mode(ptr(d_vp), w, h, mode, stream_ptr(stream)))
+
+
+def blend_image_part_max_intensity(d_dst, dw, dh, dst_off, d_src, sw, sh, src_xy, just_copy=False,
+                                   stream=None):
+    check(lib().rv_blend_image_part_max_intensity(ptr(d_dst), dw, dh, dst_off[0], dst_off[1], ptr(d_src), sw,
+                                                  sh, src_xy[0], src_xy[1], int(just_copy),
+                                                  stream_ptr(stream)))
+
+
+def mip_colorize(d_dst, dw, dh, d_src, sw, sh, src_xy, d_ctv, stream=None):
+    check(lib().rv_mip_colorize(ptr(d_dst), dw, dh, ptr(d_src), sw, sh, src_xy[0], src_xy[1], ptr(d_ctv),
+                                d_ctv.shape[0], stream_ptr(stream)))
+
+
 def block_sample_count(params, d_count, stream=None):
     check(lib().rv_block_sample_count(C.byref(params), ptr(d_count), stream_ptr(stream)))
 
@@ -234,6 +259,11 @@ class ShareRenderer:
         self.tf, self.W, self.H = d_tf, W, H
         nbytes = int(lib().rv_share_workspace_bytes(self.n))
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+
+    def render_mode(self, d_vp, mode, iso_opacity=1.0, stream=None):
+        check(lib().rv_render_share(self.params, self.bricks, self.n, mode, iso_opacity, ptr(self.tf),
+                                    self.tf.shape[0], ptr(d_vp), self.W, self.H, ptr(self.ws),
+                                    stream_ptr(stream)))
 
     def render(self, d_vp, stream=None):
         check(lib().rv_render_share_argb(self.params, self.bricks, self.n, ptr(self.tf), self.tf.shape[0],
@@ -324,6 +354,9 @@ class FramePlan:
         check(lib().rv_frame_get_tile(self.h, C.byref(w), C.byref(h)))
         return w.value, h.value
 
+    def set_field_mode(self, mode, iso_opacity=1.0):
+        check(lib().rv_frame_set_field_mode(self.h, mode, iso_opacity))
+
     def set_tape(self, enable=True):
         """Record the adjoint's per-segment states in later float-image renders."""
         check(lib().rv_frame_set_tape(self.h, int(enable)))
@@ -379,11 +412,12 @@ class Worker:
                                           n2i.shape[0], _p(sp, C.c_float)))
 
     def set_state(self, view16, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin, fmax, background,
-                  tf):
+                  tf, field_mode=FIELD_ARGB, iso_level_rel=0.5, iso_opacity=1.0):
         self._view = _np(view16, np.float32).reshape(16)
         self._tf = _np(tf, np.float32)
         st = RenderState(self._view.ctypes.data, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin,
-                         fmax, (C.c_float * 3)(*background), self._tf.ctypes.data, self._tf.shape[0])
+                         fmax, (C.c_float * 3)(*background), self._tf.ctypes.data, self._tf.shape[0],
+                         field_mode, iso_level_rel, iso_opacity)
         check(lib().rv_worker_set_state(self.h, C.byref(st)))
         self.W, self.H = vp_w, vp_h
 

@@ -64,6 +64,7 @@ struct FrameArgs {
     uint32_t W, H;
     uint32_t bw_log2, bw, bh, nbx, nbins; // wave footprint bw x bh = 64
     uint32_t render_waves;         // waves per workgroup that take screen bins (frame_kernel)
+    float iso_opacity;             // Isosurface mode: alpha scale of the level's colour
     float acc_cutoff;
     uint32_t* argb;                // W*H, rows bottom-up like the reference viewport
     float4* rgba;                  // optional float output, same layout
@@ -74,18 +75,41 @@ struct FrameArgs {
 
 constexpr uint32_t kSpinLimit = 1u << 22;
 
-// One wave: the pixels of screen bin `bin`.
-template <class BrickT>
+// One wave: the pixels of screen bin `bin`, in field mode kMode.
+template <class BrickT, uint32_t kMode>
 __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t bx = bin % a.nbx, by = bin / a.nbx;
     const uint32_t vx = bx * a.bw + (lane & (a.bw - 1u)), vy = by * a.bh + (lane >> a.bw_log2);
     const bool inside = vx < a.W && vy < a.H;
+    const uint32_t k1 = a.bin_off[bin + 1];
+    if (kMode == kModeMip) {
+        // the largest normalised value over the blocks: order does not matter
+        uint32_t bits = 0x7fc00000u; // NaN: none yet
+        for (uint32_t k = a.bin_off[bin]; k < k1; ++k) {
+            const uint32_t blk = uniform(a.bin_list[k]);
+            const rv_block_params p = a.params[blk];
+            if (inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
+                vy < p.view_y + p.view_h) {
+                const RaySeg r = setup_ray(p, vx, vy);
+                if (r.hit)
+                    bits = mip_merge(bits, march_mip(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), r), p);
+            }
+        }
+        if (inside) {
+            const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
+            a.argb[pix] = bits;
+            if (a.rgba) {
+                const float t = __uint_as_float(bits);
+                a.rgba[pix] = make_float4(t, t, t, t);
+            }
+        }
+        return;
+    }
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     float trans = 1.f; // product of (1 - alpha) in front of the current block (tape only)
     bool opaque = !inside;
-    const uint32_t k1 = a.bin_off[bin + 1];
     for (uint32_t k = a.bin_off[bin]; k < k1; ++k) {
         const uint32_t blk = uniform(a.bin_list[k]);
         const rv_block_params p = a.params[blk];
@@ -95,7 +119,7 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs)
             if (r.hit) {
                 const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);
                 float4 seg;
-                if (a.tape) {
+                if (kMode == kModeArgb && a.tape) {
                     // the pixel's state in front of this block, for the adjoint
                     a.tape[a.rec_off[blk] + (vy - p.view_y) * p.view_w + (vx - p.view_x)] =
                         make_float4(acc.x * acc.w, acc.y * acc.w, acc.z * acc.w, trans);
@@ -103,7 +127,7 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs)
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
                 } else {
-                    seg = march(p, brick, tfs, (int)a.tf_n, r);
+                    seg = segment<kMode>(p, brick, tfs, (int)a.tf_n, a.iso_opacity, r);
                 }
                 blend_behind(acc, seg);
                 opaque = acc.w > a.acc_cutoff;
@@ -122,7 +146,7 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs)
 }
 
 // Screen phase of a wave: take bins from the bin queue until it is empty.
-template <class BrickT>
+template <class BrickT, uint32_t kMode>
 __device__ void render_bins(const FrameArgs& a, const float4* tfs)
 {
     for (;;) {
@@ -132,7 +156,7 @@ __device__ void render_bins(const FrameArgs& a, const float4* tfs)
         bin = uniform(bin);
         if (bin >= a.nbins)
             break;
-        render_bin<BrickT>(a, bin, tfs);
+        render_bin<BrickT, kMode>(a, bin, tfs);
     }
 }
 
@@ -148,7 +172,7 @@ __device__ __forceinline__ void publish_tiles(uint32_t* published, uint32_t tile
         __hip_atomic_fetch_add(published, tiles, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <class BrickT>
+template <class BrickT, uint32_t kMode>
 __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
 {
     __shared__ __align__(16) float A[kTileFloats]; // interpolation tile, then the colour table
@@ -187,19 +211,19 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         tfs[i] = a.tf[i];
     __syncthreads();
     if ((threadIdx.x >> 6) < a.render_waves) // fewer concurrent rays can be faster (vector L1)
-        render_bins<BrickT>(a, tfs);
+        render_bins<BrickT, kMode>(a, tfs);
 }
 
 // Frames of resident bricks: screen bins only, no interpolation LDS, so
 // more waves per CU.
-template <class BrickT>
+template <class BrickT, uint32_t kMode>
 __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 {
     extern __shared__ float4 tfs_dyn[]; // tf_n entries
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT)
         tfs_dyn[i] = a.tf[i];
     __syncthreads();
-    render_bins<BrickT>(a, tfs_dyn);
+    render_bins<BrickT, kMode>(a, tfs_dyn);
 }
 
 // Worker share in one launch: the blocks (back to front) of one share are
@@ -218,11 +242,13 @@ constexpr uint32_t kShareT = 256;
 #endif
 constexpr uint32_t kShareWgsPerCu = RV_SHARE_WGS_PER_CU;
 
+template <uint32_t kMode>
 __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* __restrict__ params,
                                                         const uint4* __restrict__ rects,
                                                         const void* const* __restrict__ bricks,
                                                         uint32_t nblocks, const float4* __restrict__ tf,
-                                                        uint32_t tf_n, uint32_t* __restrict__ viewport,
+                                                        uint32_t tf_n, float iso_opacity,
+                                                        uint32_t* __restrict__ viewport,
                                                         uint32_t W, uint32_t H)
 {
     extern __shared__ float4 tfs_share[]; // tf_n entries (and padding)
@@ -253,9 +279,14 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
             if (inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
                 vy < p.view_y + p.view_h) {
                 const RaySeg r = setup_ray(p, vx, vy);
-                if (r.hit)
-                    value = blend_above_u32(
-                        value, march(p, reinterpret_cast<const float*>(bricks[blk]), tfs_share, (int)tf_n, r));
+                if (r.hit) {
+                    const float* brick = reinterpret_cast<const float*>(bricks[blk]);
+                    if (kMode == kModeMip)
+                        value = mip_merge(value, march_mip(p, brick, r), p);
+                    else
+                        value = blend_above_u32(
+                            value, segment<kMode>(p, brick, tfs_share, (int)tf_n, iso_opacity, r));
+                }
             }
         }
     }
@@ -393,6 +424,36 @@ int get_bins(rv_frame_plan* P, uint32_t bw, uint32_t bh, FrameBins** out)
     return build_bins(P, bw, bh, out);
 }
 
+template <uint32_t kMode>
+void launch_mode(const rv_frame_plan* P, const FrameArgs& a, int interpolate, uint32_t grid,
+                 hipStream_t s)
+{
+    if (!interpolate) {
+        const size_t lds = sizeof(float4) * a.tf_n;
+        if (P->fp16)
+            hipLaunchKernelGGL((render_frame_kernel<__half, kMode>), dim3(P->render_grid), dim3(kRT),
+                               lds, s, a);
+        else
+            hipLaunchKernelGGL((render_frame_kernel<float, kMode>), dim3(P->render_grid), dim3(kRT),
+                               lds, s, a);
+    } else if (P->fp16) {
+        hipLaunchKernelGGL((frame_kernel<__half, kMode>), dim3(grid), dim3(kT), 0, s, a);
+    } else {
+        hipLaunchKernelGGL((frame_kernel<float, kMode>), dim3(grid), dim3(kT), 0, s, a);
+    }
+}
+
+// Resident workgroups per CU of the interpolating kernel in a field mode.
+template <uint32_t kMode>
+int frame_max_wgs(bool fp16, int* per_cu)
+{
+    if (fp16)
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<__half, kMode>, kT, 0));
+    else
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<float, kMode>, kT, 0));
+    return RV_OK;
+}
+
 bool valid_footprint(uint32_t w, uint32_t h)
 {
     return w * h == 64u && w >= 1u && (w & (w - 1u)) == 0u;
@@ -427,7 +488,7 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     a.argb = d_argb;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
     a.counters = P->d_counters;
-    if (P->tape_on && d_rgba) {
+    if (P->tape_on && d_rgba && P->field_mode == RV_FIELD_ARGB) {
         // records of blocks a pixel never reaches keep T = -1
         const uint64_t nrec = P->rec_total[cam];
         rc = fill_u32(reinterpret_cast<uint32_t*>(P->d_tape), 4 * nrec, 0xBF800000u, s);
@@ -437,18 +498,26 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
         a.rec_off = P->d_rec_off + (size_t)cam * P->nblocks;
         P->tape_cam = cam;
     }
+    a.iso_opacity = P->iso_opacity;
     hipLaunchKernelGGL(frame_reset_kernel, dim3(1), dim3(64), 0, s, P->d_counters, 4u);
     RV_LAUNCH_CHECK();
-    if (!interpolate) {
-        const size_t lds = sizeof(float4) * a.tf_n;
-        if (P->fp16)
-            hipLaunchKernelGGL(render_frame_kernel<__half>, dim3(P->render_grid), dim3(kRT), lds, s, a);
-        else
-            hipLaunchKernelGGL(render_frame_kernel<float>, dim3(P->render_grid), dim3(kRT), lds, s, a);
-    } else if (P->fp16)
-        hipLaunchKernelGGL(frame_kernel<__half>, dim3(P->grid), dim3(kT), 0, s, a);
-    else
-        hipLaunchKernelGGL(frame_kernel<float>, dim3(P->grid), dim3(kT), 0, s, a);
+    // the interpolating kernel waits for its bricks: its grid never exceeds
+    // what the mode's kernel keeps resident
+    const uint32_t grid = std::min(P->grid, P->cus * P->mode_max_wgs);
+    switch (P->field_mode) {
+    case RV_FIELD_ARGB_LIGHT:
+        launch_mode<kModeArgbLight>(P, a, interpolate, grid, s);
+        break;
+    case RV_FIELD_ISOSURFACE:
+        launch_mode<kModeIsosurface>(P, a, interpolate, grid, s);
+        break;
+    case RV_FIELD_MAX_INTENSITY_PROJECTION:
+        launch_mode<kModeMip>(P, a, interpolate, grid, s);
+        break;
+    default:
+        launch_mode<kModeArgb>(P, a, interpolate, grid, s);
+        break;
+    }
     RV_LAUNCH_CHECK();
     return RV_OK;
 }
@@ -509,19 +578,20 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     RV_HIP(hipGetDevice(&dev));
     RV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     if (fp16)
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<__half>, kT, 0));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<__half, kModeArgb>, kT, 0));
     else
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<float>, kT, 0));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<float, kModeArgb>, kT, 0));
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->cus = (uint32_t)cus;
     P->grid_max_wgs = (uint32_t)per_cu;
+    P->mode_max_wgs = (uint32_t)per_cu;
     P->grid = (uint32_t)(cus * per_cu);
     if (fp16)
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<__half>, kRT,
-                                                            sizeof(float4) * desc->tf_n));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<__half, kModeArgb>,
+                                                            kRT, sizeof(float4) * desc->tf_n));
     else
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<float>, kRT,
-                                                            sizeof(float4) * desc->tf_n));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<float, kModeArgb>,
+                                                            kRT, sizeof(float4) * desc->tf_n));
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->render_max_wgs = (uint32_t)per_cu;
     P->render_grid = (uint32_t)(cus * std::min(per_cu, 4)); // 16 waves per CU until tuned
@@ -753,7 +823,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     RV_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream sig;
     sig << prop.gcnArchName << "|bins2|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
-        << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth;
+        << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth
+        << "|mode" << P->field_mode;
     const std::string key = sig.str();
     if (cache_path && *cache_path) {
         std::ifstream f(cache_path);
@@ -861,6 +932,47 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     RV_CATCH
 }
 
+extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, float iso_opacity)
+{
+    RV_TRY
+    RV_REQUIRE(P, "rv_frame_set_field_mode: null plan");
+    int per_cu = 0, rc;
+    switch (field_mode) {
+    case RV_FIELD_ARGB:
+        rc = frame_max_wgs<kModeArgb>(P->fp16, &per_cu);
+        break;
+    case RV_FIELD_ARGB_LIGHT:
+        rc = frame_max_wgs<kModeArgbLight>(P->fp16, &per_cu);
+        break;
+    case RV_FIELD_ISOSURFACE:
+        rc = frame_max_wgs<kModeIsosurface>(P->fp16, &per_cu);
+        break;
+    case RV_FIELD_MAX_INTENSITY_PROJECTION:
+        rc = frame_max_wgs<kModeMip>(P->fp16, &per_cu);
+        break;
+    default:
+        return fail(RV_ERR_INVALID, "rv_frame_set_field_mode: field mode not supported (Argb, "
+                                    "ArgbLight, Isosurface, MaxIntensityProjection)");
+    }
+    if (rc)
+        return rc;
+    RV_REQUIRE(per_cu >= 1, "rv_frame_set_field_mode: frame kernel cannot be resident");
+    if (P->graph_exec) { // a captured graph holds the previous mode's kernels
+        (void)hipGraphExecDestroy(P->graph_exec);
+        P->graph_exec = nullptr;
+    }
+    if (P->graph) {
+        (void)hipGraphDestroy(P->graph);
+        P->graph = nullptr;
+    }
+    P->field_mode = field_mode;
+    P->iso_opacity = iso_opacity;
+    P->mode_max_wgs = (uint32_t)per_cu;
+    P->tape_cam = -1;
+    return RV_OK;
+    RV_CATCH
+}
+
 extern "C" void rv_frame_plan_destroy(rv_frame_plan* P) { delete P; }
 
 extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
@@ -868,29 +980,33 @@ extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
     return (uint64_t)nblocks * (sizeof(rv_block_params) + sizeof(uint4) + sizeof(void*)) + 256;
 }
 
-extern "C" int rv_render_share_argb(const rv_block_params* params, const float* const* bricks,
-                                    uint32_t nblocks, const float* d_tf, uint32_t tf_n,
-                                    uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
-                                    void* d_workspace, rv_stream_t stream)
+extern "C" int rv_render_share(const rv_block_params* params, const float* const* bricks,
+                               uint32_t nblocks, uint32_t field_mode, float iso_opacity,
+                               const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
+                               uint32_t vp_w, uint32_t vp_h, void* d_workspace, rv_stream_t stream)
 {
     RV_TRY
+    RV_REQUIRE(field_mode == RV_FIELD_ARGB || field_mode == RV_FIELD_ARGB_LIGHT ||
+                   field_mode == RV_FIELD_ISOSURFACE || field_mode == RV_FIELD_MAX_INTENSITY_PROJECTION,
+               "rv_render_share: field mode not supported (Argb, ArgbLight, Isosurface, "
+               "MaxIntensityProjection)");
     if (nblocks == 0)
         return RV_OK;
     RV_REQUIRE(params && bricks && d_tf && d_viewport && d_workspace,
-               "rv_render_share_argb: null pointer");
-    RV_REQUIRE(tf_n >= 1 && tf_n <= kMaxTf, "rv_render_share_argb: tf_n must be in [1, 1024]");
+               "rv_render_share: null pointer");
+    RV_REQUIRE(tf_n >= 1 && tf_n <= kMaxTf, "rv_render_share: tf_n must be in [1, 1024]");
     std::vector<uint8_t> host(rv_share_workspace_bytes(nblocks));
     auto* hp = reinterpret_cast<rv_block_params*>(host.data());
     auto* hr = reinterpret_cast<uint4*>(host.data() + sizeof(rv_block_params) * nblocks);
     auto* hb = reinterpret_cast<const void**>(host.data() + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
     for (uint32_t i = 0; i < nblocks; ++i) {
         const rv_block_params& p = params[i];
-        RV_REQUIRE(p.vp_w == vp_w && p.vp_h == vp_h, "rv_render_share_argb: viewport mismatch");
+        RV_REQUIRE(p.vp_w == vp_w && p.vp_h == vp_h, "rv_render_share: viewport mismatch");
         RV_REQUIRE(p.tex_depth <= 10 && p.nodes == (1u << p.tex_depth) + 1u && p.fmax > p.fmin,
-                   "rv_render_share_argb: inconsistent block parameters");
+                   "rv_render_share: inconsistent block parameters");
         RV_REQUIRE((uint64_t)p.view_x + p.view_w <= vp_w && (uint64_t)p.view_y + p.view_h <= vp_h,
-                   "rv_render_share_argb: view rectangle outside the viewport");
-        RV_REQUIRE(bricks[i] || p.view_w == 0 || p.view_h == 0, "rv_render_share_argb: null brick");
+                   "rv_render_share: view rectangle outside the viewport");
+        RV_REQUIRE(bricks[i] || p.view_w == 0 || p.view_h == 0, "rv_render_share: null brick");
         hp[i] = p;
         hr[i] = make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h);
         hb[i] = bricks[i];
@@ -903,12 +1019,40 @@ extern "C" int rv_render_share_argb(const rv_block_params* params, const float* 
     // LDS per workgroup at most)
     const size_t lds = std::max<size_t>(sizeof(float4) * tf_n,
                                         std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u));
-    hipLaunchKernelGGL(share_kernel, dim3((vp_w + 15) / 16, (vp_h + 15) / 16), dim3(kShareT), lds,
-                       S(stream), reinterpret_cast<const rv_block_params*>(d),
-                       reinterpret_cast<const uint4*>(d + sizeof(rv_block_params) * nblocks),
-                       reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks),
-                       nblocks, reinterpret_cast<const float4*>(d_tf), tf_n, d_viewport, vp_w, vp_h);
+    const auto* dp = reinterpret_cast<const rv_block_params*>(d);
+    const auto* dr = reinterpret_cast<const uint4*>(d + sizeof(rv_block_params) * nblocks);
+    const auto* db =
+        reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
+    const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);
+    switch (field_mode) {
+    case RV_FIELD_ARGB:
+        hipLaunchKernelGGL(share_kernel<kModeArgb>, grid, dim3(kShareT), lds, S(stream), dp, dr, db,
+                           nblocks, tf4, tf_n, iso_opacity, d_viewport, vp_w, vp_h);
+        break;
+    case RV_FIELD_ARGB_LIGHT:
+        hipLaunchKernelGGL(share_kernel<kModeArgbLight>, grid, dim3(kShareT), lds, S(stream), dp, dr,
+                           db, nblocks, tf4, tf_n, iso_opacity, d_viewport, vp_w, vp_h);
+        break;
+    case RV_FIELD_ISOSURFACE:
+        hipLaunchKernelGGL(share_kernel<kModeIsosurface>, grid, dim3(kShareT), lds, S(stream), dp, dr,
+                           db, nblocks, tf4, tf_n, iso_opacity, d_viewport, vp_w, vp_h);
+        break;
+    default:
+        hipLaunchKernelGGL(share_kernel<kModeMip>, grid, dim3(kShareT), lds, S(stream), dp, dr, db,
+                           nblocks, tf4, tf_n, iso_opacity, d_viewport, vp_w, vp_h);
+        break;
+    }
     RV_LAUNCH_CHECK();
     return RV_OK;
     RV_CATCH
+}
+
+extern "C" int rv_render_share_argb(const rv_block_params* params, const float* const* bricks,
+                                    uint32_t nblocks, const float* d_tf, uint32_t tf_n,
+                                    uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                    void* d_workspace, rv_stream_t stream)
+{
+    return rv_render_share(params, bricks, nblocks, RV_FIELD_ARGB, 1.f, d_tf, tf_n, d_viewport, vp_w,
+                           vp_h, d_workspace, stream);
 }

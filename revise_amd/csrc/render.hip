@@ -1,6 +1,7 @@
-// Plain per-block Argb ray-cast (the drop-in for renderKernel<FieldMode::Argb>,
-// DenseFieldRenderer_cu.cu:62-215), viewport clear, crop rectangle and 8-bit
-// sub-image blending for gfx950.
+// Plain per-block ray-cast (the drop-in for renderKernel<fieldMode>,
+// DenseFieldRenderer_cu.cu:62-215, in the Argb, ArgbLight, Isosurface and
+// MaxIntensityProjection modes), viewport clear, crop rectangle, 8-bit
+// sub-image blending and the max-intensity compositing for gfx950.
 //
 // The block kernel runs one thread per pixel of the block's view rectangle in
 // 16 x 16 pixel workgroups (one wave = 8 x 8 pixels, so a wave's rays are a
@@ -20,10 +21,11 @@ namespace {
 
 constexpr uint32_t kTfMax = 4096;
 
+template <uint32_t kMode>
 __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
                                                            const float* __restrict__ brick,
                                                            const float* __restrict__ tf,
-                                                           uint32_t tf_n,
+                                                           uint32_t tf_n, float iso_opacity,
                                                            uint32_t* __restrict__ viewport)
 {
     extern __shared__ float4 tfs[];
@@ -39,8 +41,12 @@ __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
     const RaySeg r = setup_ray(p, vx, vy);
     if (!r.hit)
         return;
-    const float4 rgba = march(p, brick, tfs, (int)tf_n, r);
     const uint32_t pix = (p.vp_h - 1u - vy) * p.vp_w + vx;
+    if (kMode == kModeMip) {
+        viewport[pix] = mip_merge(viewport[pix], march_mip(p, brick, r), p);
+        return;
+    }
+    const float4 rgba = segment<kMode>(p, brick, tfs, (int)tf_n, iso_opacity, r);
     viewport[pix] = blend_above_u32(viewport[pix], rgba);
 }
 
@@ -160,10 +166,167 @@ __global__ __launch_bounds__(256) void blend_part_kernel(uint32_t* __restrict__ 
     d = out;
 }
 
+// blendImagePartMaxIntensity (BlendImagePart.hpp:148-191): float pixels,
+// NaN = none; the larger value wins.
+__global__ __launch_bounds__(256) void blend_part_mip_kernel(float* __restrict__ dst, int dw, int dx,
+                                                             int dy, const float* __restrict__ src,
+                                                             int sw, int sh, int just_copy)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= sw || y >= sh)
+        return;
+    const float s = src[(size_t)y * sw + x];
+    float& d = dst[(size_t)dw * (dy + y) + dx + x];
+    if (just_copy) {
+        d = s;
+        return;
+    }
+    if (!(s == s))
+        return;
+    const float dv = d;
+    if (!(dv == dv) || dv < s)
+        d = s;
+}
+
+// makeMaxIntensityProjection (VsRenderer.cpp:542-565): each non-NaN value
+// becomes calcColor of the colour vector (ColorTransferFunction.hpp:83-94),
+// encoded ARGB8; NaN pixels keep what dst holds.
+__global__ __launch_bounds__(256) void mip_colorize_kernel(uint32_t* __restrict__ dst, int dw, int dx,
+                                                           int dy, const float* __restrict__ src,
+                                                           int sw, int sh,
+                                                           const float4* __restrict__ ctv,
+                                                           uint32_t nctv)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= sw || y >= sh)
+        return;
+    const float s = src[(size_t)y * sw + x];
+    if (!(s == s))
+        return;
+    float4 c;
+    if (s <= 0.f || nctv == 1u) {
+        c = ctv[0];
+    } else if (s >= 1.f) {
+        c = ctv[nctv - 1u];
+    } else {
+        const float par = s * (float)(nctv - 1u);
+        const uint32_t idx = (uint32_t)par;
+        const float t = par - (float)idx;
+        const float4 a = ctv[idx], b = ctv[idx + 1u];
+        c = make_float4(a.x * (1.f - t) + b.x * t, a.y * (1.f - t) + b.y * t,
+                        a.z * (1.f - t) + b.z * t, a.w * (1.f - t) + b.w * t);
+    }
+    dst[(size_t)dw * (dy + y) + dx + x] =
+        ((unsigned)(c.w * 255.999f) << 24) + ((unsigned)(c.x * 255.999f) << 16) +
+        ((unsigned)(c.y * 255.999f) << 8) + (unsigned)(c.z * 255.999f);
+}
+
 } // namespace
 } // namespace rv
 
 using namespace rv;
+
+// The entry points of the field modes share their argument checks.
+static int check_block(const rv_block_params* params, const float* d_brick, const float* d_tf,
+                       uint32_t tf_n, const uint32_t* d_viewport, const char* who)
+{
+    if (!(params && d_brick && d_tf && d_viewport))
+        return fail(RV_ERR_INVALID, std::string(who) + ": null pointer");
+    if (!(tf_n >= 1 && tf_n <= kTfMax))
+        return fail(RV_ERR_INVALID, std::string(who) + ": tf_n must be in [1, 4096]");
+    const rv_block_params& p = *params;
+    if (!(p.tex_depth <= 10 && p.nodes == (1u << p.tex_depth) + 1u))
+        return fail(RV_ERR_INVALID,
+                    std::string(who) + ": inconsistent tex_depth/nodes (use rv_block_params_setup)");
+    if (!(p.fmax > p.fmin))
+        return fail(RV_ERR_INVALID, std::string(who) + ": field range must be non-empty");
+    if (!((uint64_t)p.view_x + p.view_w <= p.vp_w && (uint64_t)p.view_y + p.view_h <= p.vp_h))
+        return fail(RV_ERR_INVALID, std::string(who) + ": view rectangle outside the viewport");
+    return RV_OK;
+}
+
+extern "C" int rv_render_block(const rv_block_params* params, uint32_t field_mode, float iso_opacity,
+                               const float* d_brick, const float* d_tf, uint32_t tf_n,
+                               uint32_t* d_viewport, rv_stream_t stream)
+{
+    int rc = check_block(params, d_brick, d_tf, tf_n, d_viewport, "rv_render_block");
+    if (rc)
+        return rc;
+    const rv_block_params& p = *params;
+    if (p.view_w == 0 || p.view_h == 0)
+        return RV_OK;
+    dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
+    const size_t lds = tf_n * sizeof(float4);
+    switch (field_mode) {
+    case RV_FIELD_ARGB:
+        hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), lds, S(stream), p, d_brick,
+                           d_tf, tf_n, iso_opacity, d_viewport);
+        break;
+    case RV_FIELD_ARGB_LIGHT:
+        hipLaunchKernelGGL(render_block_kernel<kModeArgbLight>, grid, dim3(256), lds, S(stream), p,
+                           d_brick, d_tf, tf_n, iso_opacity, d_viewport);
+        break;
+    case RV_FIELD_ISOSURFACE:
+        hipLaunchKernelGGL(render_block_kernel<kModeIsosurface>, grid, dim3(256), lds, S(stream), p,
+                           d_brick, d_tf, tf_n, iso_opacity, d_viewport);
+        break;
+    case RV_FIELD_MAX_INTENSITY_PROJECTION:
+        hipLaunchKernelGGL(render_block_kernel<kModeMip>, grid, dim3(256), lds, S(stream), p, d_brick,
+                           d_tf, tf_n, iso_opacity, d_viewport);
+        break;
+    default:
+        return fail(RV_ERR_INVALID, "rv_render_block: field mode not supported (Argb, ArgbLight, "
+                                    "Isosurface, MaxIntensityProjection)");
+    }
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
+
+extern "C" int rv_clear_viewport_mode(uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                      uint32_t field_mode, rv_stream_t stream)
+{
+    RV_REQUIRE(d_viewport || (uint64_t)vp_w * vp_h == 0, "rv_clear_viewport_mode: null viewport");
+    // DenseFieldRenderer_cu.cu:268-277: NaN bits for max intensity projection
+    const uint32_t v = field_mode == RV_FIELD_MAX_INTENSITY_PROJECTION ? 0x7fc00000u : 0u;
+    return fill_u32(d_viewport, (uint64_t)vp_w * vp_h, v, S(stream));
+}
+
+extern "C" int rv_blend_image_part_max_intensity(float* d_dst, int32_t dw, int32_t dh,
+                                                 int32_t dst_off_x, int32_t dst_off_y,
+                                                 const float* d_src, int32_t sw, int32_t sh,
+                                                 int32_t src_x, int32_t src_y, int just_copy,
+                                                 rv_stream_t stream)
+{
+    if (sw <= 0 || sh <= 0)
+        return RV_OK;
+    RV_REQUIRE(d_dst && d_src, "rv_blend_image_part_max_intensity: null pointer");
+    const int dx = src_x - dst_off_x, dy = src_y - dst_off_y;
+    RV_REQUIRE(dx >= 0 && dy >= 0 && dx + sw <= dw && dy + sh <= dh,
+               "rv_blend_image_part_max_intensity: part outside destination");
+    dim3 grid((sw + 63) / 64, (sh + 3) / 4);
+    hipLaunchKernelGGL(blend_part_mip_kernel, grid, dim3(256), 0, S(stream), d_dst, dw, dx, dy, d_src,
+                       sw, sh, just_copy);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
+
+extern "C" int rv_mip_colorize(uint32_t* d_dst, int32_t dw, int32_t dh, const float* d_src,
+                               int32_t sw, int32_t sh, int32_t src_x, int32_t src_y,
+                               const float* d_ctv, uint32_t nctv, rv_stream_t stream)
+{
+    if (sw <= 0 || sh <= 0)
+        return RV_OK;
+    RV_REQUIRE(d_dst && d_src && d_ctv && nctv >= 1, "rv_mip_colorize: null pointer or empty vector");
+    RV_REQUIRE(src_x >= 0 && src_y >= 0 && src_x + sw <= dw && src_y + sh <= dh,
+               "rv_mip_colorize: part outside destination");
+    dim3 grid((sw + 63) / 64, (sh + 3) / 4);
+    hipLaunchKernelGGL(mip_colorize_kernel, grid, dim3(256), 0, S(stream), d_dst, dw, src_x, src_y,
+                       d_src, sw, sh, reinterpret_cast<const float4*>(d_ctv), nctv);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
 
 extern "C" int rv_clear_viewport(uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
                                  rv_stream_t stream)
@@ -193,8 +356,8 @@ extern "C" int rv_render_block_argb(const rv_block_params* params, const float* 
     if (p.view_w == 0 || p.view_h == 0)
         return RV_OK;
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
-    hipLaunchKernelGGL(render_block_kernel, grid, dim3(256), tf_n * sizeof(float4), S(stream), p,
-                       d_brick, d_tf, tf_n, d_viewport);
+    hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), tf_n * sizeof(float4),
+                       S(stream), p, d_brick, d_tf, tf_n, 1.f, d_viewport);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }

@@ -365,4 +365,181 @@ __device__ __forceinline__ float4 march(const rv_block_params& p, const T* __res
     return march_impl<T, false>(p, brick, tf, tf_n, r, nullptr);
 }
 
+// ---------------------------------------------------------------- field modes
+// s3vs::FieldMode (FieldMode.hpp:26-51), the same numbering.
+constexpr uint32_t kModeIsosurface = 0, kModeIsosurfaces = 1, kModeMip = 2, kModeArgb = 3,
+                   kModeArgbLight = 4;
+
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// cuda_vec_op.hpp:48-52
+__device__ __forceinline__ F3 normalize3(F3 a)
+{
+    const float il = 1.f / sqrtf(dot3(a, a));
+    return f3(a.x * il, a.y * il, a.z * il);
+}
+// clamp.hpp:24-34 (max/min ignore a NaN operand)
+__device__ __forceinline__ float clamp01(float x) { return fmaxf(0.f, fminf(1.f, x)); }
+
+// The one light of DenseFieldRenderer_cu.cu:440-445: at eye + n + e2 h,
+// ambient 0, diffuse 1, specular 0.1.
+__device__ __forceinline__ F3 light_pos(const rv_block_params& p)
+{
+    return (f3(p.pos[0], p.pos[1], p.pos[2]) + f3(p.n[0], p.n[1], p.n[2])) +
+           f3(p.e2[0], p.e2[1], p.e2[2]) * p.h;
+}
+
+// computeFragColor + blinn (cuda_lighting.hpp:27-95) for one light: the
+// normal is the central difference of the field at +-0.1 texel, lit double
+// sided; Ka = 1, Ks = 0.5, shininess 100.
+template <class T>
+__device__ inline float4 frag_color(const rv_block_params& p, const T* __restrict__ brick,
+                                    const float4& diffuse, F3 surf, F3 tc)
+{
+    const int n = (int)p.nodes;
+    constexpr float d = 0.1f;
+    F3 a, b;
+    a.x = fetch3(brick, n, tc.x - d, tc.y, tc.z);
+    b.x = fetch3(brick, n, tc.x + d, tc.y, tc.z);
+    a.y = fetch3(brick, n, tc.x, tc.y - d, tc.z);
+    b.y = fetch3(brick, n, tc.x, tc.y + d, tc.z);
+    a.z = fetch3(brick, n, tc.x, tc.y, tc.z - d);
+    b.z = fetch3(brick, n, tc.x, tc.y, tc.z + d);
+    F3 N = normalize3(a - b);
+    const F3 V = normalize3(f3(p.pos[0], p.pos[1], p.pos[2]) - surf);
+    const F3 L = normalize3(light_pos(p) - surf);
+    if (dot3(L, N) < 0.f)
+        N = f3(-N.x, -N.y, -N.z);
+    const float diff = fmaxf(dot3(L, N), 0.f);
+    const F3 H = normalize3(L + V);
+    const float spec = diff > 0.f ? pow01(fmaxf(dot3(H, N), 0.f), 100.f) : 0.f;
+    // Ka * ambient + Kd * diffuse * diff + Ks * specular * spec, per channel
+    const float ks = 0.5f * 0.1f;
+    const float cx = (1.f * 0.f + (diffuse.x * 1.f) * diff) + ks * spec;
+    const float cy = (1.f * 0.f + (diffuse.y * 1.f) * diff) + ks * spec;
+    const float cz = (1.f * 0.f + (diffuse.z * 1.f) * diff) + ks * spec;
+    return make_float4(clamp01(0.f + cx), clamp01(0.f + cy), clamp01(0.f + cz), diffuse.w);
+}
+
+// renderArgbLight loop (renderArgbLight.hpp:29-63): Argb classification,
+// samples above alpha 1e-5 shaded, composited front to back.
+template <class T>
+__device__ inline float4 march_light(const rv_block_params& p, const T* __restrict__ brick,
+                                     const float4* tf, int tf_n, RaySeg r)
+{
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int n = (int)p.nodes;
+    for (int i = 1; i < r.sample_count; ++i) {
+        const F3 pos = f3(r.px, r.py, r.pz);
+        const F3 tc = f3((r.px - p.tex_origin[0]) * p.tcf, (r.py - p.tex_origin[1]) * p.tcf,
+                         (r.pz - p.tex_origin[2]) * p.tcf);
+        r.px = r.px + r.sx;
+        r.py = r.py + r.sy;
+        r.pz = r.pz + r.sz;
+        const float v = fetch3(brick, n, tc.x, tc.y, tc.z);
+        if (!(v == v))
+            continue;
+        const float4 c = classify(p, tf, tf_n, v);
+        if (c.w >= 0.00001f) {
+            blend_behind(acc, frag_color(p, brick, c, pos, tc));
+            if (acc.w > p.alpha_cutoff)
+                break;
+        }
+    }
+    return acc;
+}
+
+// renderIsosurface loop (renderIsosurface.hpp:31-61): where consecutive
+// samples v1, v2 straddle the level (p.threshold), the surface point is
+// placed at the second sample's position plus step * (level - v1) / (v2 - v1),
+// as the reference does, and shaded with the table colour of the level, its
+// alpha scaled by iso_opacity.
+template <class T>
+__device__ inline float4 march_iso(const rv_block_params& p, const T* __restrict__ brick,
+                                   const float4* tf, int tf_n, float iso_opacity, RaySeg r)
+{
+    const int n = (int)p.nodes;
+    auto value_at = [&](float x, float y, float z) {
+        return fetch3(brick, n, (x - p.tex_origin[0]) * p.tcf, (y - p.tex_origin[1]) * p.tcf,
+                      (z - p.tex_origin[2]) * p.tcf);
+    };
+    float v1 = value_at(r.px, r.py, r.pz);
+    r.px = r.px + r.sx;
+    r.py = r.py + r.sy;
+    r.pz = r.pz + r.sz;
+    const float lv = p.threshold;
+    float4 diffuse = fetch_tf(tf, tf_n, (lv - p.fmin) / (p.fmax - p.fmin));
+    diffuse.w = diffuse.w * iso_opacity;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i = 1; i < r.sample_count; ++i) {
+        const float v2 = value_at(r.px, r.py, r.pz);
+        if (v1 == v1 && v2 == v2 && (v1 - lv) * (v2 - lv) < 0.f) {
+            const float t = (lv - v1) / (v2 - v1);
+            const F3 x = f3(r.px, r.py, r.pz) + f3(r.sx, r.sy, r.sz) * t;
+            const F3 tc = f3((x.x - p.tex_origin[0]) * p.tcf, (x.y - p.tex_origin[1]) * p.tcf,
+                             (x.z - p.tex_origin[2]) * p.tcf);
+            blend_behind(acc, frag_color(p, brick, diffuse, x, tc));
+            if (acc.w > p.alpha_cutoff)
+                break;
+        }
+        v1 = v2;
+        r.px = r.px + r.sx;
+        r.py = r.py + r.sy;
+        r.pz = r.pz + r.sz;
+    }
+    return acc;
+}
+
+// renderMaxIntensityProjection loop (renderMaxIntensityProjection.hpp:31-37):
+// the largest sample at or above p.threshold, or fmin - (fmax - fmin) when
+// there is none.
+template <class T>
+__device__ inline float march_mip(const rv_block_params& p, const T* __restrict__ brick, RaySeg r)
+{
+    const int n = (int)p.nodes;
+    float m = p.fmin - (p.fmax - p.fmin);
+    for (int i = 1; i < r.sample_count; ++i) {
+        const float v = fetch3(brick, n, (r.px - p.tex_origin[0]) * p.tcf,
+                               (r.py - p.tex_origin[1]) * p.tcf, (r.pz - p.tex_origin[2]) * p.tcf);
+        r.px = r.px + r.sx;
+        r.py = r.py + r.sy;
+        r.pz = r.pz + r.sz;
+        if (v >= p.threshold && m < v)
+            m = v;
+    }
+    return m;
+}
+
+// The viewport update of renderMaxIntensityProjection.hpp:39-48: the pixel
+// holds float bits of the normalised maximum (NaN: none yet).
+__device__ __forceinline__ uint32_t mip_merge(uint32_t prev_bits, float m, const rv_block_params& p)
+{
+    if (m < p.fmin)
+        return prev_bits;
+    const float t = (m - p.fmin) / (p.fmax - p.fmin);
+    const float prev = __uint_as_float(prev_bits);
+    if (prev == prev && prev > t)
+        return prev_bits;
+    return __float_as_uint(t);
+}
+
+// One block segment of a pixel in field mode kMode, as a colour to blend
+// (Argb, ArgbLight, Isosurface).
+template <uint32_t kMode, class T>
+__device__ __forceinline__ float4 segment(const rv_block_params& p, const T* __restrict__ brick,
+                                          const float4* tf, int tf_n, float iso_opacity, RaySeg r)
+{
+    if (kMode == kModeArgbLight)
+        return march_light(p, brick, tf, tf_n, r);
+    if (kMode == kModeIsosurface)
+        return march_iso(p, brick, tf, tf_n, iso_opacity, r);
+    return march(p, brick, tf, tf_n, r);
+}
+
 } // namespace rv

@@ -15,6 +15,15 @@ void set_error(const std::string& msg);
 
 namespace revise {
 
+namespace {
+// The presentation threshold of a field mode (DenseFieldRenderer_cu.cu:418-433):
+// the isosurface level for Isosurface, else the threshold (both relative).
+float presentationLevel(const RenderSharedState& st)
+{
+    return st.fieldMode == RV_FIELD_ISOSURFACE ? st.isosurfaceLevel : st.threshold;
+}
+} // namespace
+
 void throwIfFailed(int status)
 {
     if (status != RV_OK)
@@ -141,8 +150,8 @@ void DenseFieldRendererHip::setRenderSharedState(const RenderSharedState* s)
 
 void DenseFieldRendererHip::clearViewport(uint32_t* d_viewport) const
 {
-    throwIfFailed(rv_clear_viewport(d_viewport, m_state->viewportSize[0], m_state->viewportSize[1],
-                                    m_stream));
+    throwIfFailed(rv_clear_viewport_mode(d_viewport, m_state->viewportSize[0],
+                                         m_state->viewportSize[1], m_state->fieldMode, m_stream));
 }
 
 void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
@@ -183,11 +192,12 @@ void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
     int live = 0;
     throwIfFailed(rv_block_params_setup(st.cameraTransform.data(), st.viewportSize[0],
                                         st.viewportSize[1], st.fovY, level, index.data(), depth,
-                                        range[0], range[1], st.threshold, st.renderQuality,
+                                        range[0], range[1], presentationLevel(st), st.renderQuality,
                                         st.alphaCutoff, &p, &live));
     if (!live)
         return;
-    throwIfFailed(rv_render_block_argb(&p, m_brick, m_tf, m_tfN, vp, m_stream));
+    throwIfFailed(
+        rv_render_block(&p, st.fieldMode, st.isosurfaceOpacity, m_brick, m_tf, m_tfN, vp, m_stream));
 }
 
 // makeColorTransferTexture (DenseFieldRenderer_cu.cu:573-589): the table is
@@ -401,7 +411,8 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         throwIfFailed(rv_block_params_setup(st.cameraTransform.data(), st.viewportSize[0],
                                             st.viewportSize[1], st.fovY, sub.level, idx.data(),
                                             fs.subtreeDepth(sub.level, idx), range[0], range[1],
-                                            st.threshold, st.renderQuality, st.alphaCutoff, &p, &live));
+                                            presentationLevel(st), st.renderQuality, st.alphaCutoff,
+                                            &p, &live));
         if (!live)
             continue;
         params.push_back(p);
@@ -419,10 +430,10 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         m_cancelled = true;
         return;
     }
-    throwIfFailed(rv_render_share_argb(params.data(), bricks.data(), (uint32_t)params.size(),
-                                       m_renderer.colorTable(), m_renderer.colorTableSize(),
-                                       m_viewport, st.viewportSize[0], st.viewportSize[1],
-                                       sc.workspace, s));
+    throwIfFailed(rv_render_share(params.data(), bricks.data(), (uint32_t)params.size(), st.fieldMode,
+                                  st.isosurfaceOpacity, m_renderer.colorTable(),
+                                  m_renderer.colorTableSize(), m_viewport, st.viewportSize[0],
+                                  st.viewportSize[1], sc.workspace, s));
 }
 
 const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const
@@ -463,6 +474,7 @@ std::vector<uint32_t> composeFrame(VsWorkerHip& worker, unsigned level, unsigned
 {
     const RenderSharedState& st = *worker.renderSharedState();
     const uint32_t W = st.viewportSize[0], H = st.viewportSize[1];
+    const bool mip = st.fieldMode == RV_FIELD_MAX_INTENSITY_PROJECTION;
     float eye[3];
     throwIfFailed(rv_eye_from_view(st.cameraTransform.data(), eye));
     std::vector<uint32_t> boxes(6ull * (splitCount + 8));
@@ -514,13 +526,21 @@ std::vector<uint32_t> composeFrame(VsWorkerHip& worker, unsigned level, unsigned
             m.w = x1 - x0;
             m.h = y1 - y0;
             m.pix = bufs.alloc((uint64_t)m.w * m.h);
-            throwIfFailed(rv_fill_u32(m.pix, (uint64_t)m.w * m.h, 0u, worker.stream()));
+            // max intensity parts hold float bits, NaN = none (VsRenderer.cpp:440-445)
+            throwIfFailed(rv_fill_u32(m.pix, (uint64_t)m.w * m.h, mip ? 0x7fc00000u : 0u, worker.stream()));
+            auto blend = [&](const DevPart& q, int copy) {
+                if (mip)
+                    throwIfFailed(rv_blend_image_part_max_intensity(
+                        reinterpret_cast<float*>(m.pix), m.w, m.h, m.x, m.y,
+                        reinterpret_cast<const float*>(q.pix), q.w, q.h, q.x, q.y, copy, worker.stream()));
+                else
+                    throwIfFailed(rv_blend_image_part(m.pix, m.w, m.h, m.x, m.y, q.pix, q.w, q.h, q.x,
+                                                      q.y, copy, worker.stream()));
+            };
             if (a.pix)
-                throwIfFailed(rv_blend_image_part(m.pix, m.w, m.h, m.x, m.y, a.pix, a.w, a.h, a.x,
-                                                  a.y, 1, worker.stream()));
+                blend(a, 1);
             if (b.pix)
-                throwIfFailed(rv_blend_image_part(m.pix, m.w, m.h, m.x, m.y, b.pix, b.w, b.h, b.x,
-                                                  b.y, 0, worker.stream()));
+                blend(b, 0);
             next.push_back(m);
         }
         if (layer.size() % 2)
@@ -533,9 +553,15 @@ std::vector<uint32_t> composeFrame(VsWorkerHip& worker, unsigned level, unsigned
                               rv_background_argb(st.backgroundColor[0], st.backgroundColor[1],
                                                  st.backgroundColor[2]),
                               worker.stream()));
-    if (!layer.empty() && layer[0].pix)
-        throwIfFailed(rv_blend_image_part(frame, W, H, 0, 0, layer[0].pix, layer[0].w, layer[0].h,
-                                          layer[0].x, layer[0].y, 0, worker.stream()));
+    if (!layer.empty() && layer[0].pix) {
+        if (mip) // makeMaxIntensityProjection (VsRenderer.cpp:542-565) with the colour table
+            throwIfFailed(rv_mip_colorize(frame, W, H, reinterpret_cast<const float*>(layer[0].pix),
+                                          layer[0].w, layer[0].h, layer[0].x, layer[0].y,
+                                          worker.colorTable(), worker.colorTableSize(), worker.stream()));
+        else
+            throwIfFailed(rv_blend_image_part(frame, W, H, 0, 0, layer[0].pix, layer[0].w, layer[0].h,
+                                              layer[0].x, layer[0].y, 0, worker.stream()));
+    }
     std::vector<uint32_t> out((size_t)W * H);
     hipCheck(hipMemcpyAsync(out.data(), frame, 4ull * W * H, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
     hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -623,6 +649,9 @@ extern "C" int rv_worker_set_state(rv_worker* w, const rv_render_state* s)
     st.alphaCutoff = s->alpha_cutoff;
     st.backgroundColor = {s->background[0], s->background[1], s->background[2]};
     st.colorTable.assign(s->tf, s->tf + 4ull * s->tf_n);
+    st.fieldMode = s->field_mode;
+    st.isosurfaceLevel = s->iso_level_rel;
+    st.isosurfaceOpacity = s->iso_opacity;
     w->field.setFieldRange(s->fmin, s->fmax);
     w->worker.setRenderSharedState(&st);
     w->worker.updateState(1u << 7);

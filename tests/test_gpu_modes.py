@@ -1,0 +1,188 @@
+"""Field modes besides Argb (ArgbLight, Isosurface, MaxIntensityProjection)
+on the GPU against the oracle, bit for bit:
+
+* per block (rv_render_block, the drop-in for renderKernel<fieldMode>), over a
+  viewport holding earlier blocks' pixels, with NaN (no-field) regions in the
+  bricks;
+* a whole share in one launch (rv_render_share) equals the per-block calls;
+* the batched frame (rv_frame_set_field_mode) equals the oracle's float
+  composite of the same segments, resident and interpolating;
+* the max-intensity part blend and colouring (rv_blend_image_part_max_intensity,
+  rv_mip_colorize)."""
+import numpy as np
+import pytest
+
+from revise_amd import api, datasets
+from tests.gpu_util import say, to_dev
+
+pytestmark = pytest.mark.gpu
+
+FOV = 30.0
+MODES = [(api.FIELD_ARGB_LIGHT, 0.5, 1.0), (api.FIELD_ISOSURFACE, 0.45, 0.7), (api.FIELD_MIP, 0.3, 1.0)]
+
+
+def _scene(level, depth, seed):
+    field = datasets.Field(seed, n_gauss=3, noise_octaves=1)
+    n2i = api.subtree_n2i(datasets.full_subtree(depth), depth)
+    blocks = []
+    m = 2 ** level
+    for z in range(m):
+        for y in range(m):
+            for x in range(m):
+                # a no-field ball in some blocks: NaN after interpolation
+                occ = (lambda u, c=np.array([x + 0.3, y + 0.6, z + 0.5]) / m:
+                       ((u - c) ** 2).sum(-1) > (0.25 / m) ** 2)
+                blocks.append(datasets.Block(level, (x, y, z), depth, n2i,
+                                             datasets.sample(field, level, (x, y, z), depth, n2i, occ)))
+    return blocks
+
+
+def _init_viewport(mode, W, H, seed):
+    rng = np.random.default_rng(seed)
+    if mode == api.FIELD_MIP:
+        v = rng.uniform(0.0, 0.6, size=(H, W)).astype(np.float32)
+        v[rng.random((H, W)) < 0.5] = np.nan
+        return v.view(np.uint32)
+    return rng.integers(0, 2 ** 32, size=(H, W), dtype=np.uint64).astype(np.uint32)
+
+
+@pytest.mark.parametrize("mode,thr,iso", MODES)
+def test_render_block_modes_bit_exact(orc, mode, thr, iso):
+    import torch
+    W, H = 160, 120
+    blocks = _scene(1, 4, seed=21)
+    bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    assert any(np.isnan(br).any() for br in bricks)
+    fmin, fmax = datasets.field_range(blocks)
+    tf = datasets.seeded_tf(21, 256)
+    d_tf = to_dev(tf)
+    M = datasets.orbit_camera(40.0, elevation_deg=15.0)
+    init = _init_viewport(mode, W, H, 5)
+    d_vp = to_dev(init.view(np.int32))
+    exp = init.copy()
+    order = api.mixed_back_to_front(np.array([(1,) + b.index for b in blocks], np.uint32), api.eye_from_view(M))
+    n = 0
+    for i in order:
+        b, brick = blocks[int(i)], bricks[int(i)]
+        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        assert live == lo
+        if not live:
+            continue
+        say("mode", mode, "block", b.index)
+        api.render_block(p, mode, iso, to_dev(brick), d_tf, d_vp)
+        orc.render_block_mode(po, mode, iso, brick, tf, exp.reshape(-1))
+        n += 1
+    got = d_vp.cpu().numpy().view(np.uint32)
+    assert n == 8
+    assert not np.array_equal(exp, init)  # the blocks drew something
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("mode,thr,iso", MODES)
+def test_render_share_modes_equal_per_block(orc, mode, thr, iso):
+    import torch
+    W, H = 150, 110
+    blocks = _scene(2, 3, seed=22)
+    bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    fmin, fmax = datasets.field_range(blocks)
+    tf = datasets.seeded_tf(22, 256)
+    d_tf = to_dev(tf)
+    M = datasets.orbit_camera(250.0, elevation_deg=-20.0)
+    order = api.mixed_back_to_front(np.array([(2,) + b.index for b in blocks], np.uint32), api.eye_from_view(M))
+    items, exp = [], None
+    d_bricks = [to_dev(br) for br in bricks]
+    vp_seq = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    api.clear_viewport_mode(vp_seq, W, H, mode)
+    exp = vp_seq.cpu().numpy().view(np.uint32).copy()
+    for i in order:
+        b = blocks[int(i)]
+        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        if live:
+            items.append((p, d_bricks[int(i)]))
+            api.render_block(p, mode, iso, d_bricks[int(i)], d_tf, vp_seq)
+            orc.render_block_mode(po, mode, iso, bricks[int(i)], tf, exp.reshape(-1))
+    share = api.ShareRenderer(items, d_tf, W, H)
+    vp = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    api.clear_viewport_mode(vp, W, H, mode)
+    share.render_mode(vp, mode, iso)
+    torch.cuda.synchronize()
+    got = vp.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, vp_seq.cpu().numpy().view(np.uint32))
+    assert np.array_equal(got, exp)
+    if mode == api.FIELD_MIP:
+        assert (~np.isnan(got.view(np.float32))).any()
+    else:
+        assert (got >> 24).any()
+
+
+@pytest.mark.parametrize("mode,thr,iso", MODES)
+def test_frame_modes_match_oracle(orc, mode, thr, iso):
+    import torch
+    W, H = 144, 108
+    blocks = _scene(1, 4, seed=23)
+    fmin, fmax = datasets.field_range(blocks)
+    tf = datasets.seeded_tf(23, 256)
+    d_tf = to_dev(tf)
+    plan = api.interp_plan(blocks[0].n2i, 4)
+    dev = [{"level": b.level, "index": b.index, "depth": b.depth, "sparse": to_dev(b.sparse), "plan": plan,
+            "brick": torch.empty(api.brick_nodes(4), dtype=torch.float32, device="cuda")} for b in blocks]
+    fp = api.FramePlan(dev, W, H, FOV, fmin, fmax, thr, 2.0, 0.996, 0.9999, d_tf)
+    fp.set_field_mode(mode, iso)
+    cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((60.0, 10.0), (300.0, -35.0))])
+    fp.set_cameras(cams)
+    bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    keys = np.array([(1,) + b.index for b in blocks], np.uint32)
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    for c in range(2):
+        order = api.mixed_back_to_front(keys, api.eye_from_view(cams[c]))
+        lst = [(1, blocks[i].index, 4, bricks[i]) for i in order]
+        exp = orc.render_float_composite(cams[c], W, H, FOV, lst, fmin, fmax, thr, 2.0, 0.996, tf, 0.9999,
+                                         mode=mode, iso_opacity=iso)
+        for interp in (True, False):
+            say("frame mode", mode, "cam", c, "interpolate", interp)
+            fp.render(c, argb, rgba, interpolate=interp)
+            torch.cuda.synchronize()
+            assert fp.status() == 0
+            got = rgba.cpu().numpy()
+            assert np.array_equal(got, exp, equal_nan=True)
+            if mode == api.FIELD_MIP:
+                bits = argb.cpu().numpy().view(np.uint32)
+                assert np.array_equal(bits.view(np.float32), exp[..., 0], equal_nan=True)
+                assert (~np.isnan(exp[..., 0])).mean() > 0.02
+            else:
+                assert (exp[..., 3] > 0).mean() > 0.02
+    # the tape and adjoint are Argb only
+    fp.set_tape(True)
+    fp.render(0, argb, rgba, interpolate=False)
+    from revise_amd import RvError
+    with pytest.raises(RvError, match="tape"):
+        fp.adjoint(0, rgba, rgba, torch.zeros((256, 4), dtype=torch.float32, device="cuda"))
+    fp.close()
+
+
+def test_max_intensity_compose_kernels(orc):
+    import torch
+    rng = np.random.default_rng(9)
+    dw, dh = 40, 30
+    dst = rng.uniform(0, 1, size=(dh, dw)).astype(np.float32)
+    dst[rng.random((dh, dw)) < 0.4] = np.nan
+    src = rng.uniform(0, 1, size=(12, 17)).astype(np.float32)
+    src[rng.random(src.shape) < 0.3] = np.nan
+    for just_copy in (False, True):
+        d_dst = to_dev(dst)
+        api.blend_image_part_max_intensity(d_dst, dw, dh, (2, 3), to_dev(src), 17, 12, (7, 9), just_copy)
+        exp = dst.copy()
+        orc.blend_image_part_max_intensity(exp, src, 5, 6, just_copy)
+        assert np.array_equal(d_dst.cpu().numpy(), exp, equal_nan=True)
+    ctv = rng.uniform(0, 1, size=(512, 4)).astype(np.float32)
+    src2 = rng.uniform(-0.2, 1.2, size=(12, 17)).astype(np.float32)
+    src2[rng.random(src2.shape) < 0.2] = np.nan
+    frame = rng.integers(0, 2 ** 32, size=(dh, dw), dtype=np.uint64).astype(np.uint32)
+    d_frame = to_dev(frame.view(np.int32))
+    api.mip_colorize(d_frame, dw, dh, to_dev(src2), 17, 12, (7, 9), to_dev(ctv))
+    exp = frame.copy()
+    orc.mip_colorize(exp, src2, 7, 9, ctv)
+    assert np.array_equal(d_frame.cpu().numpy().view(np.uint32), exp)

@@ -128,3 +128,75 @@ def test_worker_compose_frame_bit_exact(orc, K, fused):
     exp = _oracle_frame(orc, blocks, 2, K, M, W, H, fmin, fmax, tf)
     assert np.array_equal(got, exp)
     w.close()
+
+
+def _oracle_share_mode(orc, blocks, level, bmin, bmax, M, W, H, fmin, fmax, tf, mode, lvl, iso):
+    init = 0x7FC00000 if mode == api.FIELD_MIP else 0
+    vp = np.full((H, W), init, np.uint32)
+    eye = api.eye_from_view(M)
+    cmin, cmax = api.index_box_coords(level, bmin, bmax)
+    thr = lvl if mode == api.FIELD_ISOSURFACE else THR
+    for idx in orc.b2f_macro(bmin, bmax, cmin, cmax, eye):
+        b = blocks[tuple(int(v) for v in idx)]
+        live, po = orc.block_params(M, W, H, FOV, level, b.index, b.depth, fmin, fmax, thr, Q, CUT)
+        if live:
+            orc.render_block_mode(po, mode, iso, orc.interpolate(b.sparse, b.n2i, b.depth), tf, vp.reshape(-1))
+    r = orc.bound_rect(vp)
+    if r is None:
+        return None, None
+    x, y, w, h = r
+    return r, vp[y:y + h, x:x + w]
+
+
+@pytest.mark.parametrize("mode,fused", [(api.FIELD_ARGB_LIGHT, True), (api.FIELD_ISOSURFACE, False),
+                                        (api.FIELD_ISOSURFACE, True), (api.FIELD_MIP, True),
+                                        (api.FIELD_MIP, False)])
+def test_worker_field_modes_bit_exact(orc, mode, fused):
+    """renderScenePart and the composed frame in the other field modes: the
+    isosurface level replaces the threshold; max intensity parts are float
+    images (NaN: none) merged by the larger value and coloured with the
+    colour table."""
+    W, H = 180, 130
+    w, blocks, fmin, fmax, tf, M = _make(2, 3, 11, W, H, 70.0)
+    lvl, iso = 0.42, 0.8
+    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, field_mode=mode, iso_level_rel=lvl,
+                iso_opacity=iso)
+    w.set_fused(fused)
+    say("worker mode", mode, "fused", fused)
+    rect, bits = w.render_scene_part(2, (0, 0, 0), (4, 4, 4))
+    erect, ebits = _oracle_share_mode(orc, blocks, 2, (0, 0, 0), (4, 4, 4), M, W, H, fmin, fmax, tf, mode,
+                                      lvl, iso)
+    assert rect == erect
+    assert np.array_equal(bits, ebits)
+    if mode != api.FIELD_MIP:
+        w.close()
+        return
+    # composed frame of 3 boxes: max-intensity assembly, colouring over the background
+    got = w.compose_frame(2, 3)
+    eye = api.eye_from_view(M)
+    parts = []
+    for b in orc.index_cube_split_sorted(2, 3, eye):
+        r, pb = _oracle_share_mode(orc, blocks, 2, b[:3], b[3:], M, W, H, fmin, fmax, tf, mode, lvl, iso)
+        parts.append((r, pb) if r is not None else ((0, 0, 0, 0), None))
+    layer = parts
+    while len(layer) > 1:
+        nxt = []
+        for i in range(1, len(layer), 2):
+            a, b = layer[i - 1], layer[i]
+            xs = [a[0][0], a[0][0] + a[0][2], b[0][0], b[0][0] + b[0][2]]
+            ys = [a[0][1], a[0][1] + a[0][3], b[0][1], b[0][1] + b[0][3]]
+            x, y = min(xs), min(ys)
+            img = np.full((max(ys) - y, max(xs) - x), np.nan, np.float32)
+            for q, copy in ((a, True), (b, False)):
+                if q[1] is not None:
+                    orc.blend_image_part_max_intensity(img, q[1].view(np.float32), q[0][0] - x, q[0][1] - y, copy)
+            nxt.append(((x, y, img.shape[1], img.shape[0]), img.view(np.uint32)))
+        if len(layer) % 2:
+            nxt.append(layer[-1])
+        layer = nxt
+    frame = np.full((H, W), orc.background_argb(*BG), np.uint32)
+    if layer[0][1] is not None:
+        orc.mip_colorize(frame, layer[0][1].view(np.float32), layer[0][0][0], layer[0][0][1], tf)
+    assert np.array_equal(got, frame)
+    assert (got != orc.background_argb(*BG)).any()
+    w.close()

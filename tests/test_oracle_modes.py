@@ -1,0 +1,209 @@
+"""The oracle's field modes (Isosurface, MaxIntensityProjection, ArgbLight)
+pinned by closed forms, independent of the oracle's own code:
+
+* fields linear in one texel axis, whose trilinear value and central
+  difference are exact, so the surface normal is known analytically;
+* the single-sample segment (quality chosen so that the block's centre ray
+  takes exactly one sample) of ArgbLight and the single crossing of an
+  isosurface, whose colours follow from the lighting model evaluated in
+  float64 at the analytic position and normal (the reference's light, Blinn
+  terms, double-sided normal, and its isosurface point placed one step past
+  the interpolated crossing);
+* the maximum of a monotone field along a ray (the last sample) and the
+  threshold rule of MaxIntensityProjection;
+* hand-worked cases of the max-intensity part blend and of calcColor.
+Outputs are 8-bit (+-1 LSB) or float (1e-5 relative)."""
+import math
+
+import numpy as np
+import pytest
+
+from revise_amd import datasets
+
+FOV = 30.0
+ISO, MIP, ARGB, LIGHT = 0, 2, 3, 4
+
+
+def _params(orc, W, H, depth, thr_rel, quality, angle=30.0, elev=20.0, cutoff=0.996):
+    M = datasets.orbit_camera(angle, elevation_deg=elev)
+    live, p = orc.block_params(M, W, H, FOV, 0, (0, 0, 0), depth, 0.0, 1.0, thr_rel, quality, cutoff)
+    assert live
+    return p
+
+
+def _ray64(p, vx, vy):
+    """Float64 ray setup of DenseFieldRenderer_cu.cu:66-181."""
+    x = (-0.5 + (vx + 0.5) / p.vp_w) * p.w
+    y = (-0.5 + (vy + 0.5) / p.vp_h) * p.h
+    n, e1, e2 = (np.array(v[:], dtype=np.float64) for v in (p.n, p.e1, p.e2))
+    eye = np.array(p.pos[:], dtype=np.float64)
+    c = np.array(p.center[:], dtype=np.float64)
+    h = float(p.half)
+    r = n + e1 * x + e2 * y
+    te, tx = 1.0, 0.0
+    for a in range(3):
+        if abs(r[a]) <= 1e-10:
+            continue
+        o1, o2 = (1, 2) if a == 0 else ((0, 2) if a == 1 else (0, 1))
+        for side in range(2):
+            face = c[a] - h if side == 0 else c[a] + h
+            tau = (face - eye[a]) / r[a]
+            pt = eye + r * tau
+            if abs(pt[o1] - c[o1]) <= h and abs(pt[o2] - c[o2]) <= h:
+                enter = (r[a] > 0) if side == 0 else not (r[a] > 0)
+                if enter:
+                    te = tau
+                else:
+                    tx = tau
+    te = max(te, 1.0)
+    nn = 2 ** p.tex_depth + 1
+    texel = 2 * h / nn
+    ratio = (tx - te) * (r @ r) / (texel * np.abs(r).sum())
+    org = c - (h + 0.5 * texel)
+    tcf = nn / (2 * (h + 0.5 * texel))
+    return dict(r=r, eye=eye, te=te, tx=tx, ratio=ratio, org=org, tcf=tcf, n=nn, texel=texel)
+
+
+def _light64(p, diffuse, surf, normal):
+    """Blinn of cuda_lighting.hpp:27-95 with the light of
+    DenseFieldRenderer_cu.cu:440-445, float64."""
+    eye = np.array(p.pos[:], dtype=np.float64)
+    light = eye + np.array(p.n[:]) + np.array(p.e2[:]) * p.h
+    unit = lambda v: v / np.linalg.norm(v)
+    N = unit(normal)
+    V = unit(eye - surf)
+    L = unit(light - surf)
+    if L @ N < 0:
+        N = -N
+    diff = max(L @ N, 0.0)
+    Hh = unit(L + V)
+    spec = max(Hh @ N, 0.0) ** 100 if diff > 0 else 0.0
+    rgb = np.clip(np.asarray(diffuse[:3]) * diff + 0.5 * 0.1 * spec, 0.0, 1.0)
+    return rgb
+
+
+def _argb8(rgba):
+    q = lambda v: int(v * 255.999)
+    return np.array([q(rgba[3]), q(rgba[0]), q(rgba[1]), q(rgba[2])])
+
+
+def _unpack(v):
+    v = int(v)
+    return np.array([v >> 24, (v >> 16) & 255, (v >> 8) & 255, v & 255])
+
+
+def _linear_brick(n, axis, a0, slope):
+    """Brick values a0 + slope * node index along `axis` (0 x, 1 y, 2 z)."""
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    idx = (x, y, z)[axis]
+    return (a0 + slope * idx).astype(np.float32).reshape(-1)
+
+
+@pytest.mark.parametrize("axis,slope,angle,elev", [(2, 0.1, 30.0, 20.0), (0, -0.08, 30.0, 20.0),
+                                                   (1, 0.05, 30.0, 20.0), (2, 0.1, 0.0, 0.0)])
+def test_argb_light_single_sample_closed_form(orc, axis, slope, angle, elev):
+    """(The last case faces the surface to the camera: the specular term adds
+    about 13 LSB.)"""
+    W, H, depth = 64, 48, 3
+    p0 = _params(orc, W, H, depth, 0.5, 2.0, angle, elev)
+    vx, vy = p0.view_x + p0.view_w // 2, p0.view_y + p0.view_h // 2
+    g = _ray64(p0, vx, vy)
+    p = _params(orc, W, H, depth, 0.5, 2.5 / g["ratio"], angle, elev)  # one sample: sampleCount 2
+    n = g["n"]
+    brick = _linear_brick(n, axis, 0.3, slope)
+    tf = np.tile(np.array([[0.6, 0.3, 0.9, 0.5]], np.float32), (16, 1))
+    vp = np.zeros(W * H, np.uint32)
+    orc.render_block_mode(p, LIGHT, 1.0, brick, tf, vp)
+    got = _unpack(vp[(H - 1 - vy) * W + vx])
+    # the one sample at the entry point; the field is linear in texel units
+    surf = g["eye"] + g["r"] * g["te"]
+    grad = np.zeros(3)
+    grad[axis] = slope
+    rgb = _light64(p, tf[0], surf, -grad)  # central difference a - b = -0.2 grad
+    ilc = 100.0 / (2 ** depth * p.quality)
+    of = (0.5 - 0.0) / 1.0
+    alpha = 1 - (1 - 0.5 * of) ** ilc
+    exp = _argb8(np.concatenate([rgb, [alpha]]))
+    assert np.abs(got - exp).max() <= 1, (got, exp)
+    assert got[0] > 0
+
+
+@pytest.mark.parametrize("opacity", [1.0, 0.6])
+def test_isosurface_single_crossing_closed_form(orc, opacity):
+    W, H, depth = 64, 48, 3
+    p = _params(orc, W, H, depth, 0.47, 2.0)  # level = 0.47 (fmin 0, fmax 1)
+    vx, vy = p.view_x + p.view_w // 2, p.view_y + p.view_h // 2
+    g = _ray64(p, vx, vy)
+    n = g["n"]
+    slope = 1.0 / (n - 1)
+    brick = _linear_brick(n, 2, 0.0, slope)  # v = (tz - 0.5) / (n - 1)
+    tf = np.tile(np.array([[0.2, 0.8, 0.4, 0.7]], np.float32), (16, 1))
+    vp = np.zeros(W * H, np.uint32)
+    orc.render_block_mode(p, ISO, opacity, brick, tf, vp)
+    got = _unpack(vp[(H - 1 - vy) * W + vx])
+    # samples s_i = entry + i step, i = 0 .. count-1; value at texel z
+    count = int(p.quality * g["ratio"])
+    step = g["r"] * ((g["tx"] - g["te"]) / count)
+    pos = [g["eye"] + g["r"] * g["te"] + step * i for i in range(count)]
+    val = lambda q: min(max((q[2] - g["org"][2]) * g["tcf"] - 0.5, 0.0), n - 1.0) * slope
+    lv = 0.47
+    hits = [i for i in range(1, count) if (val(pos[i - 1]) - lv) * (val(pos[i]) - lv) < 0]
+    assert len(hits) == 1
+    i = hits[0]
+    t = (lv - val(pos[i - 1])) / (val(pos[i]) - val(pos[i - 1]))
+    assert 1e-3 < t < 1 - 1e-3
+    surf = pos[i] + step * t  # the reference's point: one step past the crossing
+    rgb = _light64(p, tf[0], surf, np.array([0.0, 0.0, -slope]))
+    exp = _argb8(np.concatenate([rgb, [0.7 * opacity]]))
+    assert np.abs(got - exp).max() <= 1, (got, exp)
+
+
+def test_max_intensity_monotone_field_and_threshold(orc):
+    W, H, depth = 64, 48, 3
+    for thr, expect_none in ((0.2, False), (0.999, True)):
+        p = _params(orc, W, H, depth, thr, 2.0, angle=0.0, elev=0.0)  # looking along -z
+        vx, vy = p.view_x + p.view_w // 2, p.view_y + p.view_h // 2
+        g = _ray64(p, vx, vy)
+        n = g["n"]
+        brick = _linear_brick(n, 2, 0.9, -0.8 / (n - 1))  # decreasing in z: largest at the far z... 
+        vp = np.full(W * H, 0x7FC00000, np.uint32)
+        orc.render_block_mode(p, MIP, 1.0, brick, np.zeros((4, 4), np.float32), vp)
+        got = vp[(H - 1 - vy) * W + vx].view(np.float32)
+        count = int(p.quality * g["ratio"])
+        step = g["r"] * ((g["tx"] - g["te"]) / count)
+        vals = []
+        for i in range(count - 1):  # samples i = 1 .. count-1 at entry + (i-1) step
+            q = g["eye"] + g["r"] * g["te"] + step * i
+            tz = min(max((q[2] - g["org"][2]) * g["tcf"] - 0.5, 0.0), n - 1.0)
+            vals.append(0.9 - 0.8 / (n - 1) * tz)
+        m = max(v for v in vals)
+        if expect_none:
+            assert np.isnan(got)
+        else:
+            assert abs(float(got) - m) <= 1e-5 * m, (got, m)
+            # monotone along the ray: the maximum is at one end of the segment
+            assert m in (vals[0], vals[-1])
+
+
+def test_max_intensity_blend_cases(orc):
+    nan = np.float32(np.nan)
+    dst = np.array([[nan, 0.5, 0.5, nan, 0.2]], np.float32)
+    src = np.array([[0.3, 0.7, 0.1, nan, nan]], np.float32)
+    d = dst.copy()
+    orc.blend_image_part_max_intensity(d, src, 0, 0)
+    assert np.array_equal(d, np.array([[0.3, 0.7, 0.5, nan, 0.2]], np.float32), equal_nan=True)
+    d = dst.copy()
+    orc.blend_image_part_max_intensity(d, src, 0, 0, just_copy=True)
+    assert np.array_equal(d, src, equal_nan=True)
+
+
+def test_mip_colorize_calc_color_cases(orc):
+    ctv = np.array([[0, 0, 0, 0], [1, 0.5, 0.25, 1], [0.2, 0.4, 0.6, 0.8]], np.float32)
+    nan = np.float32(np.nan)
+    src = np.array([[-0.5, 0.0, 0.5, 1.0, 2.0, 0.25, nan]], np.float32)
+    dst = np.full((1, 7), 7, np.uint32)
+    orc.mip_colorize(dst, src, 0, 0, ctv)
+    enc = lambda c: (int(c[3] * 255.999) << 24) + (int(c[0] * 255.999) << 16) + (int(c[1] * 255.999) << 8) + int(c[2] * 255.999)
+    mid = (ctv[0] + ctv[1]) * np.float32(0.5)
+    exp = [enc(ctv[0]), enc(ctv[0]), enc(ctv[1]), enc(ctv[2]), enc(ctv[2]), enc(mid), 7]
+    assert list(dst[0]) == exp
