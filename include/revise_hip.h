@@ -69,6 +69,15 @@ int rv_subtree_depth(const uint32_t* children, uint32_t nnodes, uint32_t max_dep
 int rv_subtree_n2i(const uint32_t* children, uint32_t nnodes, uint32_t depth, uint16_t* n2i,
                    uint64_t cap, uint64_t* count);
 
+/* The same list built on the device (depth <= 8): the replacement of the
+ * host DFS and hash lookups of BlockTreeNodes (BlockTreeNodes.hpp:166-181)
+ * by level-parallel expansion, pre-order positions from subtree sizes, and
+ * first sightings on a dense lattice.  d_children: nnodes x 8 uint32 on the
+ * device (0 = no child, node 0 the root); d_n2i: cap x 3 uint16 on the
+ * device.  Synchronous; same result and errors as rv_subtree_n2i. */
+int rv_subtree_n2i_device(const uint32_t* d_children, uint32_t nnodes, uint32_t depth,
+                          uint16_t* d_n2i, uint64_t cap, uint64_t* count, rv_stream_t stream);
+
 /* Node -> dense ordinal map on the device: d_out[i] = vertexIndexToOrdinal(n2i[i], depth).
  * Replaces I3::vertexIndexToOrdinal (DenseFieldInterpolator_cu_3.cu:24-30). */
 int rv_node_ordinals(const uint16_t* d_n2i, uint64_t n, uint32_t depth, uint32_t* d_out,

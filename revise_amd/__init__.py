@@ -90,6 +90,8 @@ _SIGS = {
     "rv_clear_viewport": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _vp]),
     "rv_render_block_argb": (C.c_int, [C.POINTER(BlockParams), _vp, _vp, C.c_uint32, _vp, _vp]),
     "rv_block_sample_count": (C.c_int, [C.POINTER(BlockParams), _vp, _vp]),
+    "rv_subtree_n2i_device": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _vp, C.c_uint64,
+                                        C.POINTER(C.c_uint64), _vp]),
     "rv_render_block": (C.c_int, [C.POINTER(BlockParams), C.c_uint32, C.c_float, _vp, _vp, C.c_uint32,
                                   _vp, _vp]),
     "rv_clear_viewport_mode": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _vp]),

@@ -216,6 +216,15 @@ def render_block_argb(params, d_brick, d_tf, d_vp, stream=None):
                                      ptr(d_vp), stream_ptr(stream)))
 
 
+def subtree_n2i_device(d_children, nnodes, depth, d_out, stream=None):
+    """n2i built on the device into d_out (cap x 3 uint16 tensor); returns
+    the node count."""
+    cnt = C.c_uint64()
+    check(lib().rv_subtree_n2i_device(ptr(d_children), nnodes, depth, ptr(d_out), d_out.shape[0],
+                                      C.byref(cnt), stream_ptr(stream)))
+    return cnt.value
+
+
 # s3vs::FieldMode (FieldMode.hpp:26-51)
 FIELD_ISOSURFACE, FIELD_ISOSURFACES, FIELD_MIP, FIELD_ARGB, FIELD_ARGB_LIGHT = 0, 1, 2, 3, 4
 

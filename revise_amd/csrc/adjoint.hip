@@ -277,7 +277,9 @@ __device__ __forceinline__ void march_adjoint(const rv_block_params& p, const T*
         st.Pz = st.Pz + wgt * cz;
         const float Tn = Ts * (1.f - alpha);
         const bool behind = Tn > 1e-30f;
-        const float inv = behind ? 1.f / Tn : 0.f;
+        // gradient-only quantities use the hardware reciprocal (1 ulp); the
+        // classification above keeps the forward's exact operations
+        const float inv = behind ? __builtin_amdgcn_rcpf(Tn) : 0.f;
         const float Qx = (st.Fx - st.Px) * inv, Qy = (st.Fy - st.Py) * inv,
                     Qz = (st.Fz - st.Pz) * inv;
         const float ratio = behind ? st.Tend * inv : 1.f;
@@ -285,7 +287,8 @@ __device__ __forceinline__ void march_adjoint(const rv_block_params& p, const T*
                                     ((st.gPx * Qx + st.gPy * Qy) + st.gPz * Qz) + st.gA * ratio);
         const float gcx = st.gPx * wgt, gcy = st.gPy * wgt, gcz = st.gPz * wgt;
         const float gtw = base > 0.f
-                              ? g_alpha * (p.i_layer_count * p.opacity_factor * (1.f - alpha) / base)
+                              ? g_alpha * (p.i_layer_count * p.opacity_factor * (1.f - alpha) *
+                                           __builtin_amdgcn_rcpf(base))
                               : 0.f;
         // colour table
         TfAcc& ta = sg.tacc;
