@@ -173,3 +173,39 @@ def test_interp_fp16_whole_brick_tile_alignment(orc, depth, p, offset):
     assert bits_equal_nan(got[offset:offset + V], e)
     assert (got[:offset] == -7.0).all() and (got[offset + V:] == -7.0).all()
     b.close()
+
+
+def test_interp_batch_full_small_bricks(orc):
+    """A batch of 3000 whole, fully refined bricks of depths 0-4 (each one
+    tile, stored as one run), fp16 and fp32, some holding NaN values (the
+    tile is then interpolated as a partial one, as the reference
+    interpolates undefined nodes) or 1e30: the oracle's bricks."""
+    import torch
+    rng = np.random.default_rng(77)
+    jobs, outs, exps = [], [], []
+    plans = {d: (api.subtree_n2i(datasets.full_subtree(d), d)) for d in range(5)}
+    dev_plans = {d: api.interp_plan(plans[d], d) for d in range(5)}
+    for i in range(3000):
+        d = int(rng.integers(0, 5))
+        n2i = plans[d]
+        vals = rng.normal(size=n2i.shape[0]).astype(np.float32)
+        if i % 97 == 0:
+            vals[rng.random(n2i.shape[0]) < 0.05] = np.nan
+        if i % 89 == 0:
+            vals[rng.random(n2i.shape[0]) < 0.05] = np.float32(1e30)
+        dt = torch.float16 if i % 2 else torch.float32
+        dense = torch.empty(api.brick_nodes(d), dtype=dt, device="cuda")
+        jobs.append((to_dev(vals), dev_plans[d], dense, d))
+        outs.append(dense)
+        exps.append((vals, n2i, d))
+    b = api.InterpBatch(jobs)
+    b.run()
+    torch.cuda.synchronize()
+    for o, (vals, n2i, d) in zip(outs, exps):
+        e = orc.interpolate(vals, n2i, d)
+        got = o.cpu().numpy()
+        if got.dtype == np.float16:
+            assert bits_equal_nan(got.astype(np.float32), e.astype(np.float16).astype(np.float32))
+        else:
+            assert bits_equal_nan(got, e)
+    b.close()
