@@ -110,6 +110,7 @@ struct RenderSharedState {
     uint32_t fieldMode = RV_FIELD_ARGB;
     float isosurfaceLevel = 0.5f;
     float isosurfaceOpacity = 1.f;
+    std::vector<float> isosurfaceLevels; // fieldAllParam().isosurfaceLevels() (Isosurfaces)
 };
 
 class DenseFieldRendererHip {

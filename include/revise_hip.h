@@ -197,7 +197,8 @@ int rv_render_block_argb(const rv_block_params* params, const float* d_brick, co
                          uint32_t tf_n, uint32_t* d_viewport, rv_stream_t stream);
 
 /* Field modes: s3vs::FieldMode (FieldMode.hpp:26-51), the same numbering.
- * Argb, ArgbLight, Isosurface and MaxIntensityProjection are rendered. */
+ * All but the secondary-field modes (ValueOnIsosurface, ValueOnIsosurfaces)
+ * are rendered. */
 enum {
     RV_FIELD_ISOSURFACE = 0,
     RV_FIELD_ISOSURFACES = 1,
@@ -215,13 +216,18 @@ enum {
  * cuda_lighting.hpp:27-95 with the light of DenseFieldRenderer_cu.cu:440-445),
  * renderIsosurface (renderIsosurface.hpp:31-61; the level is
  * params->threshold, i.e. set up with threshold_rel = the isosurface level)
- * and renderMaxIntensityProjection (renderMaxIntensityProjection.hpp:31-48;
+ * renderIsosurfaces (renderIsosurfaces.hpp:31-103; levels_rel: level_count
+ * isosurfaceLevels relative to the field range, made absolute, sorted and
+ * deduplicated, at most 50 kept as makeLevels does,
+ * DenseFieldRenderer_cu.cu:597-612; none: the block's pixels become 0),
+ * renderDomainVoxels (renderDomainVoxels.hpp:30-54) and
+ * renderMaxIntensityProjection (renderMaxIntensityProjection.hpp:31-48;
  * the viewport holds float bits, cleared to NaN by rv_clear_viewport_mode).
  * iso_opacity scales the isosurface colour's alpha (isosurfaceOpacity).
  * Every segment stops at params->alpha_cutoff (the reference: 0.996). */
 int rv_render_block(const rv_block_params* params, uint32_t field_mode, float iso_opacity,
-                    const float* d_brick, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
-                    rv_stream_t stream);
+                    const float* levels_rel, uint32_t level_count, const float* d_brick,
+                    const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, rv_stream_t stream);
 
 /* clearViewport of a field mode (DenseFieldRenderer_cu.cu:268-277): 0, or NaN
  * bits for MaxIntensityProjection. */
@@ -265,9 +271,9 @@ int rv_render_share_argb(const rv_block_params* params, const float* const* bric
 /* The same in a field mode of rv_render_block: the result equals one
  * rv_render_block call per block, in the given order. */
 int rv_render_share(const rv_block_params* params, const float* const* bricks, uint32_t nblocks,
-                    uint32_t field_mode, float iso_opacity, const float* d_tf, uint32_t tf_n,
-                    uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, void* d_workspace,
-                    rv_stream_t stream);
+                    uint32_t field_mode, float iso_opacity, const float* levels_rel,
+                    uint32_t level_count, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
+                    uint32_t vp_w, uint32_t vp_h, void* d_workspace, rv_stream_t stream);
 
 /* Piecewise-linear colour table from control points.  Replaces
  * s3vs::makeColorTransferVector (ColorTransferFunction.hpp:27-69).
@@ -403,12 +409,14 @@ int rv_frame_get_tile(const rv_frame_plan* plan, uint32_t* tile_w, uint32_t* til
 int rv_frame_autotune(rv_frame_plan* plan, uint32_t* d_argb, int interpolate,
                       const char* cache_path, rv_stream_t stream, uint32_t* tile_w,
                       uint32_t* tile_h);
-/* Field mode of later frames (default RV_FIELD_ARGB): Argb, ArgbLight and
- * Isosurface segments are composited front to back (blendBehind); in
+/* Field mode of later frames (default RV_FIELD_ARGB; levels as in
+ * rv_render_block): Argb, ArgbLight, Isosurface, Isosurfaces and
+ * DomainVoxels segments are composited front to back (blendBehind); in
  * MaxIntensityProjection the frame's d_argb receives the float bits of the
  * largest normalised value over the blocks (NaN: none), the order of blocks
  * being immaterial.  The tape and the adjoint are Argb only. */
-int rv_frame_set_field_mode(rv_frame_plan* plan, uint32_t field_mode, float iso_opacity);
+int rv_frame_set_field_mode(rv_frame_plan* plan, uint32_t field_mode, float iso_opacity,
+                            const float* levels_rel, uint32_t level_count);
 void rv_frame_plan_destroy(rv_frame_plan* plan);
 
 /* Adjoint ray-cast (no reference counterpart: the reference renders only).
@@ -454,6 +462,8 @@ typedef struct rv_render_state {
     uint32_t field_mode;     /* RV_FIELD_* (fieldMode); note 0 is Isosurface */
     float iso_level_rel;     /* isosurfaceLevel, relative to the field range */
     float iso_opacity;       /* isosurfaceOpacity */
+    const float* iso_levels_rel; /* isosurfaceLevels (Isosurfaces), relative */
+    uint32_t iso_level_count;
 } rv_render_state;
 
 /* VsWorkerInterface::initialize: selects GPU thread_id % device count

@@ -967,7 +967,40 @@ uint32_t blend_above_u32(uint32_t argb, const float* rgba)
 }
 
 /* ---- field modes (s3vs::FieldMode numbering, FieldMode.hpp:26-51) ---- */
-enum { kIso = 0, kMip = 2, kArgb = 3, kArgbLight = 4 };
+enum { kIso = 0, kIsos = 1, kMip = 2, kArgb = 3, kArgbLight = 4, kDomain = 5 };
+
+/* makeLevels, DenseFieldRenderer_cu.cu:597-612: relToAbsFieldValue of each
+ * level, sorted, duplicates removed, at most 50 (MaxLevels) kept. */
+std::vector<float> make_levels(float fmin, float fmax, const float* rel, uint32_t n)
+{
+    std::vector<float> v(n);
+    for (uint32_t i = 0; i < n; ++i)
+        v[i] = fmin + (fmax - fmin) * rel[i];
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    if (v.size() > 50)
+        v.resize(50);
+    return v;
+}
+
+/* findInterval, cuda_findInterval.hpp:28-44 */
+int find_interval(float v, const std::vector<float>& lv)
+{
+    const int n = (int)lv.size();
+    if (v < lv[0])
+        return 0;
+    if (v >= lv[n - 1])
+        return n;
+    int i1 = 0, i2 = n;
+    while (i2 - i1 > 1) {
+        int i = (i1 + i2) >> 1;
+        if (v < lv[i])
+            i2 = i;
+        else
+            i1 = i;
+    }
+    return i2;
+}
 
 /* cuda_vec_op.hpp:48-52 */
 V3 normalize3(V3 a)
@@ -1098,13 +1131,94 @@ uint32_t mip_merge(uint32_t prev_bits, float m, const orc_block_params* p)
     return bits;
 }
 
+/* renderIsosurfaces loop, renderIsosurfaces.hpp:31-103 (levels non-empty) */
+void march_isos(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
+                uint32_t tf_n, float iso_opacity, const std::vector<float>& lv, Ray& r, float* rgba)
+{
+    rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
+    const int n = (int)lv.size();
+    float v1 = fetch3(brick, d.n, mul(sub(r.pos, d.origin), d.tcf));
+    r.pos = add(r.pos, r.step);
+    int iint = std::isnan(v1) ? -1 : find_interval(v1, lv);
+    for (int i = 1; i < r.sampleCount; ++i, r.pos = add(r.pos, r.step)) {
+        float v2 = fetch3(brick, d.n, mul(sub(r.pos, d.origin), d.tcf));
+        if (std::isnan(v2)) {
+            iint = -1;
+            v1 = NAN;
+            continue;
+        }
+        if (std::isnan(v1)) {
+            v1 = v2;
+            iint = find_interval(v1, lv);
+            continue;
+        }
+        int il, dl;
+        if (v2 < v1) {
+            if (iint == 0) {
+                v1 = v2;
+                continue;
+            }
+            il = iint - 1;
+            dl = -1;
+        } else {
+            if (iint == n) {
+                v1 = v2;
+                continue;
+            }
+            il = iint;
+            dl = 1;
+        }
+        while (il >= 0 && il < n) {
+            float level = lv[il];
+            if (!((v1 - level) * (v2 - level) < 0.f))
+                break;
+            float diffuse[4];
+            fetch_tf(tf, tf_n, (level - p->fmin) / (p->fmax - p->fmin), diffuse);
+            diffuse[3] = diffuse[3] * iso_opacity;
+            float t = (level - v1) / (v2 - v1);
+            V3 x = add(r.pos, mul(r.step, t));
+            float col[4];
+            frag_color(p, d, brick, diffuse, x, mul(sub(x, d.origin), d.tcf), col);
+            blend_behind4(rgba, col);
+            iint += dl;
+            il += dl;
+        }
+        if (rgba[3] > p->alpha_cutoff)
+            break;
+        v1 = v2;
+    }
+}
+
+/* renderDomainVoxels loop, renderDomainVoxels.hpp:30-54 */
+void march_domain(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
+                  uint32_t tf_n, Ray& r, float* rgba)
+{
+    rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
+    float diffuse[4];
+    fetch_tf(tf, tf_n, 0.5f, diffuse);
+    diffuse[3] = 1.f - orc_powf(1.f - diffuse[3] * d.opacity_factor, d.i_layer_count);
+    for (int i = 1; i < r.sampleCount; ++i, r.pos = add(r.pos, r.step)) {
+        float v = fetch3(brick, d.n, mul(sub(r.pos, d.origin), d.tcf));
+        if (!std::isnan(v)) {
+            blend_behind4(rgba, diffuse);
+            if (rgba[3] > p->alpha_cutoff)
+                break;
+        }
+    }
+}
+
 void segment_mode(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
-                  uint32_t tf_n, uint32_t mode, float iso_opacity, Ray& r, float* rgba)
+                  uint32_t tf_n, uint32_t mode, float iso_opacity, const std::vector<float>& lv,
+                  Ray& r, float* rgba)
 {
     if (mode == kArgbLight)
         march_light(p, d, brick, tf, tf_n, r, rgba);
     else if (mode == kIso)
         march_iso(p, d, brick, tf, tf_n, iso_opacity, r, rgba);
+    else if (mode == kIsos)
+        march_isos(p, d, brick, tf, tf_n, iso_opacity, lv, r, rgba);
+    else if (mode == kDomain)
+        march_domain(p, d, brick, tf, tf_n, r, rgba);
     else
         march(p, d, brick, tf, tf_n, r, rgba);
 }
@@ -1132,10 +1246,12 @@ extern "C" void orc_render_block(const orc_block_params* p, const float* brick, 
 /* renderKernel<fieldMode> for one block in a field mode (Argb, ArgbLight,
  * Isosurface, MaxIntensityProjection), DenseFieldRenderer_cu.cu:62-215. */
 extern "C" void orc_render_block_mode(const orc_block_params* p, uint32_t mode, float iso_opacity,
+                                      const float* levels_rel, uint32_t level_count,
                                       const float* brick, const float* tf, uint32_t tf_n,
                                       uint32_t* viewport, uint32_t row0, uint32_t row1)
 {
     Derived d = derive(p);
+    const std::vector<float> lv = make_levels(p->fmin, p->fmax, levels_rel, level_count);
     for (uint32_t vy = p->view_y; vy < p->view_y + p->view_h; ++vy) {
         if (vy < row0 || vy >= row1)
             continue;
@@ -1147,8 +1263,12 @@ extern "C" void orc_render_block_mode(const orc_block_params* p, uint32_t mode, 
                 viewport[r.pixel] = mip_merge(viewport[r.pixel], march_mip(p, d, brick, r), p);
                 continue;
             }
+            if (mode == kIsos && lv.empty()) {
+                viewport[r.pixel] = 0u; /* renderIsosurfaces.hpp:33-34 */
+                continue;
+            }
             float rgba[4];
-            segment_mode(p, d, brick, tf, tf_n, mode, iso_opacity, r, rgba);
+            segment_mode(p, d, brick, tf, tf_n, mode, iso_opacity, lv, r, rgba);
             viewport[r.pixel] = blend_above_u32(viewport[r.pixel], rgba);
         }
     }
@@ -1504,8 +1624,10 @@ extern "C" void orc_render_float_composite_mode(const float* view16, uint32_t vp
                                                 float threshold_rel, float quality,
                                                 float alpha_cutoff, float acc_cutoff,
                                                 const float* tf, uint32_t tf_n, uint32_t mode,
-                                                float iso_opacity, float* out, uint32_t nthreads)
+                                                float iso_opacity, const float* levels_rel,
+                                                uint32_t level_count, float* out, uint32_t nthreads)
 {
+    const std::vector<float> lv = make_levels(fmin, fmax, levels_rel, level_count);
     std::vector<orc_block_params> params(nblocks);
     std::vector<Derived> der(nblocks);
     std::vector<int> live(nblocks);
@@ -1553,7 +1675,9 @@ extern "C" void orc_render_float_composite_mode(const float* view16, uint32_t vp
                     if (!r.hit)
                         continue;
                     float seg[4];
-                    segment_mode(p, der[bi], blocks[bi].brick, tf, tf_n, mode, iso_opacity, r, seg);
+                    if (mode == kIsos && lv.empty())
+                        continue; /* no levels: nothing drawn */
+                    segment_mode(p, der[bi], blocks[bi].brick, tf, tf_n, mode, iso_opacity, lv, r, seg);
                     /* acc (in front) composited over seg: blendBehind(acc, seg) */
                     float da = seg[3] * (1.f - acc[3]);
                     float a = acc[3] + da;
@@ -1581,5 +1705,5 @@ extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, u
 {
     orc_render_float_composite_mode(view16, vp_w, vp_h, fov_y_deg, blocks, nblocks, fmin, fmax,
                                     threshold_rel, quality, alpha_cutoff, acc_cutoff, tf, tf_n, kArgb,
-                                    1.f, out, nthreads);
+                                    1.f, nullptr, 0, out, nthreads);
 }

@@ -102,14 +102,17 @@ void orc_render_block(const orc_block_params* p, const float* brick, const float
                       uint32_t tf_n, uint32_t* viewport, uint32_t row0, uint32_t row1);
 
 /* renderKernel<fieldMode> over the view rectangle in a field mode (s3vs
- * FieldMode numbering: 0 Isosurface, 2 MaxIntensityProjection, 3 Argb, 4
- * ArgbLight): renderIsosurface.hpp:31-61, renderMaxIntensityProjection.hpp:
+ * FieldMode numbering: 0 Isosurface, 1 Isosurfaces (levels relative, made
+ * absolute/sorted/unique as makeLevels), 2 MaxIntensityProjection, 3 Argb,
+ * 4 ArgbLight, 5 DomainVoxels; renderIsosurfaces.hpp:31-103,
+ * renderDomainVoxels.hpp:30-54): renderIsosurface.hpp:31-61, renderMaxIntensityProjection.hpp:
  * 31-48, renderArgb.hpp:30-55, renderArgbLight.hpp:29-63 with
  * cuda_lighting.hpp:27-95 and the light of DenseFieldRenderer_cu.cu:440-445.
  * MaxIntensityProjection keeps float bits in the viewport (NaN: none). */
 void orc_render_block_mode(const orc_block_params* p, uint32_t mode, float iso_opacity,
-                           const float* brick, const float* tf, uint32_t tf_n, uint32_t* viewport,
-                           uint32_t row0, uint32_t row1);
+                           const float* levels_rel, uint32_t level_count, const float* brick,
+                           const float* tf, uint32_t tf_n, uint32_t* viewport, uint32_t row0,
+                           uint32_t row1);
 
 /* blendImagePartMaxIntensity (BlendImagePart.hpp:148-191) of a float part at
  * (dx, dy) of a dw-wide float image. */
@@ -204,7 +207,8 @@ void orc_render_float_composite_mode(const float* view16, uint32_t vp_w, uint32_
                                      uint32_t nblocks, float fmin, float fmax, float threshold_rel,
                                      float quality, float alpha_cutoff, float acc_cutoff,
                                      const float* tf, uint32_t tf_n, uint32_t mode,
-                                     float iso_opacity, float* out_rgba, uint32_t nthreads);
+                                     float iso_opacity, const float* levels_rel,
+                                     uint32_t level_count, float* out_rgba, uint32_t nthreads);
 
 #ifdef __cplusplus
 }

@@ -100,8 +100,8 @@ def lib():
                                                  C.POINTER(BlockRef), C.c_uint32, C.c_float, C.c_float,
                                                  C.c_float, C.c_float, C.c_float, C.c_float, f32p,
                                                  C.c_uint32, f32p, C.c_uint32]
-        L.orc_render_block_mode.argtypes = [C.POINTER(BlockParams), C.c_uint32, C.c_float, f32p, f32p,
-                                            C.c_uint32, u32p, C.c_uint32, C.c_uint32]
+        L.orc_render_block_mode.argtypes = [C.POINTER(BlockParams), C.c_uint32, C.c_float, f32p, C.c_uint32,
+                                            f32p, f32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32]
         L.orc_blend_image_part_max_intensity.argtypes = [f32p, C.c_int32, f32p, C.c_int32, C.c_int32,
                                                          C.c_int32, C.c_int32, C.c_int]
         L.orc_mip_colorize.argtypes = [u32p, C.c_int32, f32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -110,7 +110,7 @@ def lib():
                                                       C.POINTER(BlockRef), C.c_uint32, C.c_float,
                                                       C.c_float, C.c_float, C.c_float, C.c_float,
                                                       C.c_float, f32p, C.c_uint32, C.c_uint32, C.c_float,
-                                                      f32p, C.c_uint32]
+                                                      f32p, C.c_uint32, f32p, C.c_uint32]
         _lib = L
     return _lib
 
@@ -221,13 +221,17 @@ def render_block(params, brick, tf, viewport):
                            _p(viewport, C.c_uint32), 0, params.vp_h)
 
 
-def render_block_mode(params, mode, iso_opacity, brick, tf, viewport):
-    """Field modes: 0 Isosurface, 2 MaxIntensityProjection (float bits), 3 Argb, 4 ArgbLight."""
+def render_block_mode(params, mode, iso_opacity, brick, tf, viewport, levels=None):
+    """Field modes: 0 Isosurface, 1 Isosurfaces (levels relative), 2
+    MaxIntensityProjection (float bits), 3 Argb, 4 ArgbLight, 5 DomainVoxels."""
     brick = np.ascontiguousarray(brick, dtype=np.float32)
     tf = np.ascontiguousarray(tf, dtype=np.float32)
+    lv = np.ascontiguousarray(levels if levels is not None else [0.0], dtype=np.float32)
+    nlv = 0 if levels is None else len(levels)
     assert viewport.dtype == np.uint32 and viewport.flags.c_contiguous
-    lib().orc_render_block_mode(C.byref(params), mode, iso_opacity, _p(brick, C.c_float),
-                                _p(tf, C.c_float), tf.shape[0], _p(viewport, C.c_uint32), 0, params.vp_h)
+    lib().orc_render_block_mode(C.byref(params), mode, iso_opacity, _p(lv, C.c_float), nlv,
+                                _p(brick, C.c_float), _p(tf, C.c_float), tf.shape[0],
+                                _p(viewport, C.c_uint32), 0, params.vp_h)
 
 
 def blend_image_part_max_intensity(dst, src, dx, dy, just_copy=False):
@@ -327,11 +331,14 @@ def render_share(view16, vp_w, vp_h, fov, level, box_min, box_max, tex_depth, fm
 
 
 def render_float_composite(view16, vp_w, vp_h, fov, blocks_b2f, fmin, fmax, thr_rel, quality,
-                           alpha_cutoff, tf, acc_cutoff=2.0, nthreads=None, mode=3, iso_opacity=1.0):
+                           alpha_cutoff, tf, acc_cutoff=2.0, nthreads=None, mode=3, iso_opacity=1.0,
+                           levels=None):
     """blocks_b2f: list of (level, (ix,iy,iz), tex_depth, brick) in back-to-front order."""
     v = np.ascontiguousarray(view16, dtype=np.float32).reshape(16)
     tf = np.ascontiguousarray(tf, dtype=np.float32)
     keep = []
+    lv = np.ascontiguousarray(levels if levels is not None else [0.0], dtype=np.float32)
+    nlv = 0 if levels is None else len(levels)
     arr = (BlockRef * len(blocks_b2f))()
     for i, (lvl, idx, td, brick) in enumerate(blocks_b2f):
         b = np.ascontiguousarray(brick, dtype=np.float32)
@@ -345,5 +352,5 @@ def render_float_composite(view16, vp_w, vp_h, fov, blocks_b2f, fmin, fmax, thr_
     lib().orc_render_float_composite_mode(_p(v, C.c_float), vp_w, vp_h, fov, arr, len(blocks_b2f), fmin,
                                           fmax, thr_rel, quality, alpha_cutoff, acc_cutoff,
                                           _p(tf, C.c_float), tf.shape[0], mode, iso_opacity,
-                                          _p(out, C.c_float), nthreads or os.cpu_count())
+                                          _p(lv, C.c_float), nlv, _p(out, C.c_float), nthreads or os.cpu_count())
     return out

@@ -51,7 +51,8 @@ class RenderState(C.Structure):
                 ("fov_y_deg", C.c_float), ("quality", C.c_float), ("threshold_rel", C.c_float),
                 ("alpha_cutoff", C.c_float), ("fmin", C.c_float), ("fmax", C.c_float),
                 ("background", C.c_float * 3), ("tf", C.c_void_p), ("tf_n", C.c_uint32),
-                ("field_mode", C.c_uint32), ("iso_level_rel", C.c_float), ("iso_opacity", C.c_float)]
+                ("field_mode", C.c_uint32), ("iso_level_rel", C.c_float), ("iso_opacity", C.c_float),
+                ("iso_levels_rel", C.c_void_p), ("iso_level_count", C.c_uint32)]
 
 
 class BrickJob(C.Structure):
@@ -92,8 +93,8 @@ _SIGS = {
     "rv_block_sample_count": (C.c_int, [C.POINTER(BlockParams), _vp, _vp]),
     "rv_subtree_n2i_device": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _vp, C.c_uint64,
                                         C.POINTER(C.c_uint64), _vp]),
-    "rv_render_block": (C.c_int, [C.POINTER(BlockParams), C.c_uint32, C.c_float, _vp, _vp, C.c_uint32,
-                                  _vp, _vp]),
+    "rv_render_block": (C.c_int, [C.POINTER(BlockParams), C.c_uint32, C.c_float, _f32p, C.c_uint32, _vp,
+                                  _vp, C.c_uint32, _vp, _vp]),
     "rv_clear_viewport_mode": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, _vp]),
     "rv_blend_image_part_max_intensity": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _vp,
                                                     C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int,
@@ -101,7 +102,8 @@ _SIGS = {
     "rv_mip_colorize": (C.c_int, [_vp, C.c_int32, C.c_int32, _vp, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_int32, _vp, C.c_uint32, _vp]),
     "rv_render_share": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32,
-                                  C.c_float, _vp, C.c_uint32, _vp, C.c_uint32, C.c_uint32, _vp, _vp]),
+                                  C.c_float, _f32p, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32,
+                                  C.c_uint32, _vp, _vp]),
     "rv_share_workspace_bytes": (C.c_uint64, [C.c_uint32]),
     "rv_render_share_argb": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p), C.c_uint32, _vp,
                                        C.c_uint32, _vp, C.c_uint32, C.c_uint32, _vp, _vp]),
@@ -129,7 +131,7 @@ _SIGS = {
     "rv_frame_get_tile": (C.c_int, [_vp, _u32p, _u32p]),
     "rv_frame_autotune": (C.c_int, [_vp, _vp, C.c_int, C.c_char_p, _vp, _u32p, _u32p]),
     "rv_frame_set_tape": (C.c_int, [_vp, C.c_int]),
-    "rv_frame_set_field_mode": (C.c_int, [_vp, C.c_uint32, C.c_float]),
+    "rv_frame_set_field_mode": (C.c_int, [_vp, C.c_uint32, C.c_float, _f32p, C.c_uint32]),
     "rv_frame_set_grad_bricks": (C.c_int, [_vp, _vp]),
     "rv_frame_adjoint": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp]),
     "rv_frame_plan_destroy": (None, [_vp]),

@@ -226,12 +226,18 @@ def subtree_n2i_device(d_children, nnodes, depth, d_out, stream=None):
 
 
 # s3vs::FieldMode (FieldMode.hpp:26-51)
-FIELD_ISOSURFACE, FIELD_ISOSURFACES, FIELD_MIP, FIELD_ARGB, FIELD_ARGB_LIGHT = 0, 1, 2, 3, 4
+FIELD_ISOSURFACE, FIELD_ISOSURFACES, FIELD_MIP, FIELD_ARGB, FIELD_ARGB_LIGHT, FIELD_DOMAIN_VOXELS = 0, 1, 2, 3, 4, 5
 
 
-def render_block(params, mode, iso_opacity, d_brick, d_tf, d_vp, stream=None):
-    check(lib().rv_render_block(C.byref(params), mode, iso_opacity, ptr(d_brick), ptr(d_tf), d_tf.shape[0],
-                                ptr(d_vp), stream_ptr(stream)))
+def _levels(levels):
+    lv = _np(levels if levels is not None else [], np.float32).reshape(-1)
+    return lv, (_p(lv, C.c_float) if lv.size else None), lv.size
+
+
+def render_block(params, mode, iso_opacity, d_brick, d_tf, d_vp, levels=None, stream=None):
+    lv, lp, ln = _levels(levels)
+    check(lib().rv_render_block(C.byref(params), mode, iso_opacity, lp, ln, ptr(d_brick), ptr(d_tf),
+                                d_tf.shape[0], ptr(d_vp), stream_ptr(stream)))
 
 
 def clear_viewport_mode(d_vp, w, h, mode, stream=None):
@@ -269,8 +275,9 @@ class ShareRenderer:
         nbytes = int(lib().rv_share_workspace_bytes(self.n))
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
 
-    def render_mode(self, d_vp, mode, iso_opacity=1.0, stream=None):
-        check(lib().rv_render_share(self.params, self.bricks, self.n, mode, iso_opacity, ptr(self.tf),
+    def render_mode(self, d_vp, mode, iso_opacity=1.0, levels=None, stream=None):
+        lv, lp, ln = _levels(levels)
+        check(lib().rv_render_share(self.params, self.bricks, self.n, mode, iso_opacity, lp, ln, ptr(self.tf),
                                     self.tf.shape[0], ptr(d_vp), self.W, self.H, ptr(self.ws),
                                     stream_ptr(stream)))
 
@@ -363,8 +370,9 @@ class FramePlan:
         check(lib().rv_frame_get_tile(self.h, C.byref(w), C.byref(h)))
         return w.value, h.value
 
-    def set_field_mode(self, mode, iso_opacity=1.0):
-        check(lib().rv_frame_set_field_mode(self.h, mode, iso_opacity))
+    def set_field_mode(self, mode, iso_opacity=1.0, levels=None):
+        lv, lp, ln = _levels(levels)
+        check(lib().rv_frame_set_field_mode(self.h, mode, iso_opacity, lp, ln))
 
     def set_tape(self, enable=True):
         """Record the adjoint's per-segment states in later float-image renders."""
@@ -421,12 +429,14 @@ class Worker:
                                           n2i.shape[0], _p(sp, C.c_float)))
 
     def set_state(self, view16, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin, fmax, background,
-                  tf, field_mode=FIELD_ARGB, iso_level_rel=0.5, iso_opacity=1.0):
+                  tf, field_mode=FIELD_ARGB, iso_level_rel=0.5, iso_opacity=1.0, iso_levels=None):
         self._view = _np(view16, np.float32).reshape(16)
         self._tf = _np(tf, np.float32)
+        self._levels = _np(iso_levels if iso_levels is not None else [], np.float32).reshape(-1)
         st = RenderState(self._view.ctypes.data, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin,
                          fmax, (C.c_float * 3)(*background), self._tf.ctypes.data, self._tf.shape[0],
-                         field_mode, iso_level_rel, iso_opacity)
+                         field_mode, iso_level_rel, iso_opacity,
+                         self._levels.ctypes.data if self._levels.size else None, self._levels.size)
         check(lib().rv_worker_set_state(self.h, C.byref(st)))
         self.W, self.H = vp_w, vp_h
 

@@ -64,7 +64,8 @@ struct FrameArgs {
     uint32_t W, H;
     uint32_t bw_log2, bw, bh, nbx, nbins; // wave footprint bw x bh = 64
     uint32_t render_waves;         // waves per workgroup that take screen bins (frame_kernel)
-    float iso_opacity;             // Isosurface mode: alpha scale of the level's colour
+    float iso_opacity;             // Isosurface(s) modes: alpha scale of the level's colour
+    LevelsArg lev;                 // Isosurfaces mode: the levels
     float acc_cutoff;
     uint32_t* argb;                // W*H, rows bottom-up like the reference viewport
     float4* rgba;                  // optional float output, same layout
@@ -77,7 +78,7 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 
 // One wave: the pixels of screen bin `bin`, in field mode kMode.
 template <class BrickT, uint32_t kMode>
-__device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs)
+__device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, const float* lvs)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t bx = bin % a.nbx, by = bin / a.nbx;
@@ -126,8 +127,10 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs)
                     float ts;
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
+                } else if (kMode == kModeIsosurfaces && a.lev.n == 0) {
+                    seg = make_float4(0.f, 0.f, 0.f, 0.f); // no levels: nothing to draw
                 } else {
-                    seg = segment<kMode>(p, brick, tfs, (int)a.tf_n, a.iso_opacity, r);
+                    seg = segment<kMode>(p, brick, tfs, (int)a.tf_n, a.iso_opacity, r, lvs, a.lev.n);
                 }
                 blend_behind(acc, seg);
                 opaque = acc.w > a.acc_cutoff;
@@ -147,7 +150,7 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs)
 
 // Screen phase of a wave: take bins from the bin queue until it is empty.
 template <class BrickT, uint32_t kMode>
-__device__ void render_bins(const FrameArgs& a, const float4* tfs)
+__device__ void render_bins(const FrameArgs& a, const float4* tfs, const float* lvs)
 {
     for (;;) {
         uint32_t bin = 0;
@@ -156,7 +159,7 @@ __device__ void render_bins(const FrameArgs& a, const float4* tfs)
         bin = uniform(bin);
         if (bin >= a.nbins)
             break;
-        render_bin<BrickT, kMode>(a, bin, tfs);
+        render_bin<BrickT, kMode>(a, bin, tfs, lvs);
     }
 }
 
@@ -177,6 +180,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
 {
     __shared__ __align__(16) float A[kTileFloats]; // interpolation tile, then the colour table
     __shared__ float lat[64];
+    __shared__ float lvs[kMaxLevels];
     __shared__ uint32_t item_s;
     float4* tfs = reinterpret_cast<float4*>(A);
     uint32_t my_tiles = 0; // brick tiles done by this workgroup
@@ -209,9 +213,11 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
     }
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT)
         tfs[i] = a.tf[i];
+    if (kMode == kModeIsosurfaces && threadIdx.x < a.lev.n)
+        lvs[threadIdx.x] = a.lev.v[threadIdx.x];
     __syncthreads();
     if ((threadIdx.x >> 6) < a.render_waves) // fewer concurrent rays can be faster (vector L1)
-        render_bins<BrickT, kMode>(a, tfs);
+        render_bins<BrickT, kMode>(a, tfs, lvs);
 }
 
 // Frames of resident bricks: screen bins only, no interpolation LDS, so
@@ -220,10 +226,13 @@ template <class BrickT, uint32_t kMode>
 __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 {
     extern __shared__ float4 tfs_dyn[]; // tf_n entries
+    __shared__ float lvs[kMaxLevels];
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT)
         tfs_dyn[i] = a.tf[i];
+    if (kMode == kModeIsosurfaces && threadIdx.x < a.lev.n)
+        lvs[threadIdx.x] = a.lev.v[threadIdx.x];
     __syncthreads();
-    render_bins<BrickT, kMode>(a, tfs_dyn);
+    render_bins<BrickT, kMode>(a, tfs_dyn, lvs);
 }
 
 // Worker share in one launch: the blocks (back to front) of one share are
@@ -247,13 +256,16 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         const uint4* __restrict__ rects,
                                                         const void* const* __restrict__ bricks,
                                                         uint32_t nblocks, const float4* __restrict__ tf,
-                                                        uint32_t tf_n, float iso_opacity,
+                                                        uint32_t tf_n, float iso_opacity, LevelsArg lev,
                                                         uint32_t* __restrict__ viewport,
                                                         uint32_t W, uint32_t H)
 {
     extern __shared__ float4 tfs_share[]; // tf_n entries (and padding)
+    __shared__ float lvs[kMaxLevels];
     for (uint32_t i = threadIdx.x; i < tf_n; i += kShareT)
         tfs_share[i] = tf[i];
+    if (kMode == kModeIsosurfaces && threadIdx.x < lev.n)
+        lvs[threadIdx.x] = lev.v[threadIdx.x];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t x0 = blockIdx.x * 16u + (wave & 1u) * 8u, y0 = blockIdx.y * 16u + (wave >> 1) * 8u;
@@ -283,9 +295,11 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                     const float* brick = reinterpret_cast<const float*>(bricks[blk]);
                     if (kMode == kModeMip)
                         value = mip_merge(value, march_mip(p, brick, r), p);
+                    else if (kMode == kModeIsosurfaces && lev.n == 0)
+                        value = 0u; // renderIsosurfaces.hpp:33-34
                     else
-                        value = blend_above_u32(
-                            value, segment<kMode>(p, brick, tfs_share, (int)tf_n, iso_opacity, r));
+                        value = blend_above_u32(value, segment<kMode>(p, brick, tfs_share, (int)tf_n,
+                                                                      iso_opacity, r, lvs, lev.n));
                 }
             }
         }
@@ -499,6 +513,7 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
         P->tape_cam = cam;
     }
     a.iso_opacity = P->iso_opacity;
+    a.lev = P->levels;
     hipLaunchKernelGGL(frame_reset_kernel, dim3(1), dim3(64), 0, s, P->d_counters, 4u);
     RV_LAUNCH_CHECK();
     // the interpolating kernel waits for its bricks: its grid never exceeds
@@ -513,6 +528,12 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
         break;
     case RV_FIELD_MAX_INTENSITY_PROJECTION:
         launch_mode<kModeMip>(P, a, interpolate, grid, s);
+        break;
+    case RV_FIELD_ISOSURFACES:
+        launch_mode<kModeIsosurfaces>(P, a, interpolate, grid, s);
+        break;
+    case RV_FIELD_DOMAIN_VOXELS:
+        launch_mode<kModeDomainVoxels>(P, a, interpolate, grid, s);
         break;
     default:
         launch_mode<kModeArgb>(P, a, interpolate, grid, s);
@@ -932,10 +953,12 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     RV_CATCH
 }
 
-extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, float iso_opacity)
+extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, float iso_opacity,
+                                       const float* levels_rel, uint32_t level_count)
 {
     RV_TRY
     RV_REQUIRE(P, "rv_frame_set_field_mode: null plan");
+    RV_REQUIRE(levels_rel || level_count == 0, "rv_frame_set_field_mode: null levels");
     int per_cu = 0, rc;
     switch (field_mode) {
     case RV_FIELD_ARGB:
@@ -950,9 +973,15 @@ extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, fl
     case RV_FIELD_MAX_INTENSITY_PROJECTION:
         rc = frame_max_wgs<kModeMip>(P->fp16, &per_cu);
         break;
+    case RV_FIELD_ISOSURFACES:
+        rc = frame_max_wgs<kModeIsosurfaces>(P->fp16, &per_cu);
+        break;
+    case RV_FIELD_DOMAIN_VOXELS:
+        rc = frame_max_wgs<kModeDomainVoxels>(P->fp16, &per_cu);
+        break;
     default:
-        return fail(RV_ERR_INVALID, "rv_frame_set_field_mode: field mode not supported (Argb, "
-                                    "ArgbLight, Isosurface, MaxIntensityProjection)");
+        return fail(RV_ERR_INVALID, "rv_frame_set_field_mode: field mode not supported (the "
+                                    "secondary-field modes ValueOnIsosurface(s) are not built)");
     }
     if (rc)
         return rc;
@@ -967,6 +996,7 @@ extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, fl
     }
     P->field_mode = field_mode;
     P->iso_opacity = iso_opacity;
+    P->levels = make_levels(P->desc.fmin, P->desc.fmax, levels_rel, level_count);
     P->mode_max_wgs = (uint32_t)per_cu;
     P->tape_cam = -1;
     return RV_OK;
@@ -982,14 +1012,15 @@ extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
 
 extern "C" int rv_render_share(const rv_block_params* params, const float* const* bricks,
                                uint32_t nblocks, uint32_t field_mode, float iso_opacity,
-                               const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
-                               uint32_t vp_w, uint32_t vp_h, void* d_workspace, rv_stream_t stream)
+                               const float* levels_rel, uint32_t level_count, const float* d_tf,
+                               uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                               void* d_workspace, rv_stream_t stream)
 {
     RV_TRY
-    RV_REQUIRE(field_mode == RV_FIELD_ARGB || field_mode == RV_FIELD_ARGB_LIGHT ||
-                   field_mode == RV_FIELD_ISOSURFACE || field_mode == RV_FIELD_MAX_INTENSITY_PROJECTION,
-               "rv_render_share: field mode not supported (Argb, ArgbLight, Isosurface, "
-               "MaxIntensityProjection)");
+    RV_REQUIRE(field_mode <= RV_FIELD_DOMAIN_VOXELS,
+               "rv_render_share: field mode not supported (the secondary-field modes "
+               "ValueOnIsosurface(s) are not built)");
+    RV_REQUIRE(levels_rel || level_count == 0, "rv_render_share: null levels");
     if (nblocks == 0)
         return RV_OK;
     RV_REQUIRE(params && bricks && d_tf && d_viewport && d_workspace,
@@ -1007,6 +1038,8 @@ extern "C" int rv_render_share(const rv_block_params* params, const float* const
         RV_REQUIRE((uint64_t)p.view_x + p.view_w <= vp_w && (uint64_t)p.view_y + p.view_h <= vp_h,
                    "rv_render_share: view rectangle outside the viewport");
         RV_REQUIRE(bricks[i] || p.view_w == 0 || p.view_h == 0, "rv_render_share: null brick");
+        RV_REQUIRE(p.fmin == params[0].fmin && p.fmax == params[0].fmax,
+                   "rv_render_share: the blocks of a share have one field range");
         hp[i] = p;
         hr[i] = make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h);
         hb[i] = bricks[i];
@@ -1025,22 +1058,32 @@ extern "C" int rv_render_share(const rv_block_params* params, const float* const
         reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
     const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
     const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);
+    // the levels are relative to the field's range, the same for every block
+    const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
     switch (field_mode) {
     case RV_FIELD_ARGB:
         hipLaunchKernelGGL(share_kernel<kModeArgb>, grid, dim3(kShareT), lds, S(stream), dp, dr, db,
-                           nblocks, tf4, tf_n, iso_opacity, d_viewport, vp_w, vp_h);
+                           nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
         break;
     case RV_FIELD_ARGB_LIGHT:
         hipLaunchKernelGGL(share_kernel<kModeArgbLight>, grid, dim3(kShareT), lds, S(stream), dp, dr,
-                           db, nblocks, tf4, tf_n, iso_opacity, d_viewport, vp_w, vp_h);
+                           db, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
         break;
     case RV_FIELD_ISOSURFACE:
         hipLaunchKernelGGL(share_kernel<kModeIsosurface>, grid, dim3(kShareT), lds, S(stream), dp, dr,
-                           db, nblocks, tf4, tf_n, iso_opacity, d_viewport, vp_w, vp_h);
+                           db, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
+        break;
+    case RV_FIELD_ISOSURFACES:
+        hipLaunchKernelGGL(share_kernel<kModeIsosurfaces>, grid, dim3(kShareT), lds, S(stream), dp, dr,
+                           db, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
+        break;
+    case RV_FIELD_DOMAIN_VOXELS:
+        hipLaunchKernelGGL(share_kernel<kModeDomainVoxels>, grid, dim3(kShareT), lds, S(stream), dp, dr,
+                           db, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
         break;
     default:
         hipLaunchKernelGGL(share_kernel<kModeMip>, grid, dim3(kShareT), lds, S(stream), dp, dr, db,
-                           nblocks, tf4, tf_n, iso_opacity, d_viewport, vp_w, vp_h);
+                           nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
         break;
     }
     RV_LAUNCH_CHECK();
@@ -1053,6 +1096,6 @@ extern "C" int rv_render_share_argb(const rv_block_params* params, const float* 
                                     uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
                                     void* d_workspace, rv_stream_t stream)
 {
-    return rv_render_share(params, bricks, nblocks, RV_FIELD_ARGB, 1.f, d_tf, tf_n, d_viewport, vp_w,
-                           vp_h, d_workspace, stream);
+    return rv_render_share(params, bricks, nblocks, RV_FIELD_ARGB, 1.f, nullptr, 0, d_tf, tf_n,
+                           d_viewport, vp_w, vp_h, d_workspace, stream);
 }

@@ -73,6 +73,7 @@ struct rv_frame_plan {
     uint32_t frame_rw = rv::kT / 64u; // waves per frame_kernel workgroup that render
     uint32_t field_mode = RV_FIELD_ARGB;
     float iso_opacity = 1.f;
+    rv::LevelsArg levels{};
     uint32_t mode_max_wgs = 0; // resident interpolating workgroups per CU in field_mode
     // adjoint tape: per camera, per block, the offset of the block's view
     // rectangle in the tape (float4 per pixel); the blocks' 8 x 8 tiles

@@ -1,6 +1,6 @@
 // Plain per-block ray-cast (the drop-in for renderKernel<fieldMode>,
-// DenseFieldRenderer_cu.cu:62-215, in the Argb, ArgbLight, Isosurface and
-// MaxIntensityProjection modes), viewport clear, crop rectangle, 8-bit
+// DenseFieldRenderer_cu.cu:62-215, in the Argb, ArgbLight, Isosurface,
+// Isosurfaces, DomainVoxels and MaxIntensityProjection modes), viewport clear, crop rectangle, 8-bit
 // sub-image blending and the max-intensity compositing for gfx950.
 //
 // The block kernel runs one thread per pixel of the block's view rectangle in
@@ -26,11 +26,15 @@ __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
                                                            const float* __restrict__ brick,
                                                            const float* __restrict__ tf,
                                                            uint32_t tf_n, float iso_opacity,
+                                                           LevelsArg lev,
                                                            uint32_t* __restrict__ viewport)
 {
     extern __shared__ float4 tfs[];
+    __shared__ float lvs[kMaxLevels];
     for (uint32_t i = threadIdx.x; i < tf_n; i += 256u)
         tfs[i] = reinterpret_cast<const float4*>(tf)[i];
+    if (kMode == kModeIsosurfaces && threadIdx.x < lev.n)
+        lvs[threadIdx.x] = lev.v[threadIdx.x];
     __syncthreads();
     uint32_t lx, ly;
     tile_pixel(threadIdx.x, 16u, 16u, lx, ly);
@@ -46,7 +50,11 @@ __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
         viewport[pix] = mip_merge(viewport[pix], march_mip(p, brick, r), p);
         return;
     }
-    const float4 rgba = segment<kMode>(p, brick, tfs, (int)tf_n, iso_opacity, r);
+    if (kMode == kModeIsosurfaces && lev.n == 0) {
+        viewport[pix] = 0u; // renderIsosurfaces.hpp:33-34: no levels, the pixel is 0
+        return;
+    }
+    const float4 rgba = segment<kMode>(p, brick, tfs, (int)tf_n, iso_opacity, r, lvs, lev.n);
     viewport[pix] = blend_above_u32(viewport[pix], rgba);
 }
 
@@ -248,37 +256,48 @@ static int check_block(const rv_block_params* params, const float* d_brick, cons
 }
 
 extern "C" int rv_render_block(const rv_block_params* params, uint32_t field_mode, float iso_opacity,
-                               const float* d_brick, const float* d_tf, uint32_t tf_n,
-                               uint32_t* d_viewport, rv_stream_t stream)
+                               const float* levels_rel, uint32_t level_count, const float* d_brick,
+                               const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
+                               rv_stream_t stream)
 {
     int rc = check_block(params, d_brick, d_tf, tf_n, d_viewport, "rv_render_block");
     if (rc)
         return rc;
+    RV_REQUIRE(levels_rel || level_count == 0, "rv_render_block: null levels");
     const rv_block_params& p = *params;
     if (p.view_w == 0 || p.view_h == 0)
         return RV_OK;
+    const LevelsArg lev = make_levels(p.fmin, p.fmax, levels_rel, level_count);
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
     const size_t lds = tf_n * sizeof(float4);
     switch (field_mode) {
     case RV_FIELD_ARGB:
         hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), lds, S(stream), p, d_brick,
-                           d_tf, tf_n, iso_opacity, d_viewport);
+                           d_tf, tf_n, iso_opacity, lev, d_viewport);
         break;
     case RV_FIELD_ARGB_LIGHT:
         hipLaunchKernelGGL(render_block_kernel<kModeArgbLight>, grid, dim3(256), lds, S(stream), p,
-                           d_brick, d_tf, tf_n, iso_opacity, d_viewport);
+                           d_brick, d_tf, tf_n, iso_opacity, lev, d_viewport);
         break;
     case RV_FIELD_ISOSURFACE:
         hipLaunchKernelGGL(render_block_kernel<kModeIsosurface>, grid, dim3(256), lds, S(stream), p,
-                           d_brick, d_tf, tf_n, iso_opacity, d_viewport);
+                           d_brick, d_tf, tf_n, iso_opacity, lev, d_viewport);
+        break;
+    case RV_FIELD_ISOSURFACES:
+        hipLaunchKernelGGL(render_block_kernel<kModeIsosurfaces>, grid, dim3(256), lds, S(stream), p,
+                           d_brick, d_tf, tf_n, iso_opacity, lev, d_viewport);
+        break;
+    case RV_FIELD_DOMAIN_VOXELS:
+        hipLaunchKernelGGL(render_block_kernel<kModeDomainVoxels>, grid, dim3(256), lds, S(stream), p,
+                           d_brick, d_tf, tf_n, iso_opacity, lev, d_viewport);
         break;
     case RV_FIELD_MAX_INTENSITY_PROJECTION:
         hipLaunchKernelGGL(render_block_kernel<kModeMip>, grid, dim3(256), lds, S(stream), p, d_brick,
-                           d_tf, tf_n, iso_opacity, d_viewport);
+                           d_tf, tf_n, iso_opacity, lev, d_viewport);
         break;
     default:
-        return fail(RV_ERR_INVALID, "rv_render_block: field mode not supported (Argb, ArgbLight, "
-                                    "Isosurface, MaxIntensityProjection)");
+        return fail(RV_ERR_INVALID, "rv_render_block: field mode not supported (the secondary-field "
+                                    "modes ValueOnIsosurface(s) are not built)");
     }
     RV_LAUNCH_CHECK();
     return RV_OK;
@@ -357,7 +376,7 @@ extern "C" int rv_render_block_argb(const rv_block_params* params, const float* 
         return RV_OK;
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
     hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), tf_n * sizeof(float4),
-                       S(stream), p, d_brick, d_tf, tf_n, 1.f, d_viewport);
+                       S(stream), p, d_brick, d_tf, tf_n, 1.f, LevelsArg{}, d_viewport);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }

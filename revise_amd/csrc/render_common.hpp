@@ -368,7 +368,37 @@ __device__ __forceinline__ float4 march(const rv_block_params& p, const T* __res
 // ---------------------------------------------------------------- field modes
 // s3vs::FieldMode (FieldMode.hpp:26-51), the same numbering.
 constexpr uint32_t kModeIsosurface = 0, kModeIsosurfaces = 1, kModeMip = 2, kModeArgb = 3,
-                   kModeArgbLight = 4;
+                   kModeArgbLight = 4, kModeDomainVoxels = 5;
+
+// Isosurface levels of the Isosurfaces mode: absolute values, sorted,
+// distinct, at most kMaxLevels (RenderKernelPresentationParam::MaxLevels,
+// RenderKernelParam.hpp:75-84; makeLevels, DenseFieldRenderer_cu.cu:597-612).
+constexpr uint32_t kMaxLevels = 50;
+struct LevelsArg {
+    uint32_t n;
+    float v[kMaxLevels];
+};
+
+// makeLevels on the host: relative levels to absolute (relToAbsFieldValue,
+// as the threshold), sorted, duplicates dropped, at most kMaxLevels kept.
+inline LevelsArg make_levels(float fmin, float fmax, const float* rel, uint32_t count)
+{
+    LevelsArg L{};
+    float buf[256];
+    uint32_t m = 0;
+    for (uint32_t i = 0; i < count && m < 256; ++i)
+        buf[m++] = fmin + (fmax - fmin) * rel[i];
+    for (uint32_t i = 1; i < m; ++i) // insertion sort (few levels)
+        for (uint32_t j = i; j > 0 && buf[j] < buf[j - 1]; --j) {
+            const float t = buf[j];
+            buf[j] = buf[j - 1];
+            buf[j - 1] = t;
+        }
+    for (uint32_t i = 0; i < m && L.n < kMaxLevels; ++i)
+        if (L.n == 0 || buf[i] != L.v[L.n - 1])
+            L.v[L.n++] = buf[i];
+    return L;
+}
 
 struct F3 {
     float x, y, z;
@@ -529,16 +559,138 @@ __device__ __forceinline__ uint32_t mip_merge(uint32_t prev_bits, float m, const
     return __float_as_uint(t);
 }
 
+// findInterval (cuda_findInterval.hpp:28-44): 0 <= i <= n with v between
+// levels[i-1] and levels[i]; v must not be NaN.
+__device__ __forceinline__ int find_interval(float v, const float* lv, uint32_t n)
+{
+    if (v < lv[0])
+        return 0;
+    if (v >= lv[n - 1])
+        return (int)n;
+    int i1 = 0, i2 = (int)n;
+    while (i2 - i1 > 1) {
+        const int i = (i1 + i2) >> 1;
+        if (v < lv[i])
+            i2 = i;
+        else
+            i1 = i;
+    }
+    return i2;
+}
+
+// renderIsosurfaces loop (renderIsosurfaces.hpp:31-103): every level crossed
+// between consecutive samples is shaded at its point (placed as in
+// march_iso) with the table colour of the level, in the order the ray
+// crosses them.  lv: n >= 1 sorted distinct levels (the caller handles n = 0).
+template <class T>
+__device__ inline float4 march_isos(const rv_block_params& p, const T* __restrict__ brick,
+                                    const float4* tf, int tf_n, float iso_opacity, const float* lv,
+                                    uint32_t nlv, RaySeg r)
+{
+    const int n = (int)p.nodes;
+    auto value_at = [&](float x, float y, float z) {
+        return fetch3(brick, n, (x - p.tex_origin[0]) * p.tcf, (y - p.tex_origin[1]) * p.tcf,
+                      (z - p.tex_origin[2]) * p.tcf);
+    };
+    float v1 = value_at(r.px, r.py, r.pz);
+    r.px = r.px + r.sx;
+    r.py = r.py + r.sy;
+    r.pz = r.pz + r.sz;
+    int iint = v1 == v1 ? find_interval(v1, lv, nlv) : -1;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i = 1; i < r.sample_count; ++i) {
+        const float v2 = value_at(r.px, r.py, r.pz);
+        const float px = r.px, py = r.py, pz = r.pz;
+        r.px = r.px + r.sx;
+        r.py = r.py + r.sy;
+        r.pz = r.pz + r.sz;
+        if (!(v2 == v2)) {
+            iint = -1;
+            v1 = __builtin_nanf("");
+            continue;
+        }
+        if (!(v1 == v1)) {
+            v1 = v2;
+            iint = find_interval(v1, lv, nlv);
+            continue;
+        }
+        int il, d;
+        if (v2 < v1) {
+            if (iint == 0) {
+                v1 = v2;
+                continue;
+            }
+            il = iint - 1;
+            d = -1;
+        } else {
+            if (iint == (int)nlv) {
+                v1 = v2;
+                continue;
+            }
+            il = iint;
+            d = 1;
+        }
+        while (il >= 0 && il < (int)nlv) {
+            const float level = lv[il];
+            if (!((v1 - level) * (v2 - level) < 0.f))
+                break;
+            float4 diffuse = fetch_tf(tf, tf_n, (level - p.fmin) / (p.fmax - p.fmin));
+            diffuse.w = diffuse.w * iso_opacity;
+            const float t = (level - v1) / (v2 - v1);
+            const F3 x = f3(px, py, pz) + f3(r.sx, r.sy, r.sz) * t;
+            const F3 tc = f3((x.x - p.tex_origin[0]) * p.tcf, (x.y - p.tex_origin[1]) * p.tcf,
+                             (x.z - p.tex_origin[2]) * p.tcf);
+            blend_behind(acc, frag_color(p, brick, diffuse, x, tc));
+            iint += d;
+            il += d;
+        }
+        if (acc.w > p.alpha_cutoff)
+            break;
+        v1 = v2;
+    }
+    return acc;
+}
+
+// renderDomainVoxels loop (renderDomainVoxels.hpp:30-54): every sample where
+// the field is defined adds the table colour at 0.5, opacity-corrected.
+template <class T>
+__device__ inline float4 march_domain(const rv_block_params& p, const T* __restrict__ brick,
+                                      const float4* tf, int tf_n, RaySeg r)
+{
+    const int n = (int)p.nodes;
+    float4 diffuse = fetch_tf(tf, tf_n, 0.5f);
+    diffuse.w = 1.f - pow01(1.f - diffuse.w * p.opacity_factor, p.i_layer_count);
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int i = 1; i < r.sample_count; ++i) {
+        const float v = fetch3(brick, n, (r.px - p.tex_origin[0]) * p.tcf,
+                               (r.py - p.tex_origin[1]) * p.tcf, (r.pz - p.tex_origin[2]) * p.tcf);
+        r.px = r.px + r.sx;
+        r.py = r.py + r.sy;
+        r.pz = r.pz + r.sz;
+        if (v == v) {
+            blend_behind(acc, diffuse);
+            if (acc.w > p.alpha_cutoff)
+                break;
+        }
+    }
+    return acc;
+}
+
 // One block segment of a pixel in field mode kMode, as a colour to blend
-// (Argb, ArgbLight, Isosurface).
+// (Argb, ArgbLight, Isosurface, Isosurfaces with nlv >= 1, DomainVoxels).
 template <uint32_t kMode, class T>
 __device__ __forceinline__ float4 segment(const rv_block_params& p, const T* __restrict__ brick,
-                                          const float4* tf, int tf_n, float iso_opacity, RaySeg r)
+                                          const float4* tf, int tf_n, float iso_opacity, RaySeg r,
+                                          const float* lv = nullptr, uint32_t nlv = 0)
 {
     if (kMode == kModeArgbLight)
         return march_light(p, brick, tf, tf_n, r);
     if (kMode == kModeIsosurface)
         return march_iso(p, brick, tf, tf_n, iso_opacity, r);
+    if (kMode == kModeIsosurfaces)
+        return march_isos(p, brick, tf, tf_n, iso_opacity, lv, nlv, r);
+    if (kMode == kModeDomainVoxels)
+        return march_domain(p, brick, tf, tf_n, r);
     return march(p, brick, tf, tf_n, r);
 }
 

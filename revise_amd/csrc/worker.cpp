@@ -196,8 +196,9 @@ void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
                                         st.alphaCutoff, &p, &live));
     if (!live)
         return;
-    throwIfFailed(
-        rv_render_block(&p, st.fieldMode, st.isosurfaceOpacity, m_brick, m_tf, m_tfN, vp, m_stream));
+    throwIfFailed(rv_render_block(&p, st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
+                                  (uint32_t)st.isosurfaceLevels.size(), m_brick, m_tf, m_tfN, vp,
+                                  m_stream));
 }
 
 // makeColorTransferTexture (DenseFieldRenderer_cu.cu:573-589): the table is
@@ -431,7 +432,8 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         return;
     }
     throwIfFailed(rv_render_share(params.data(), bricks.data(), (uint32_t)params.size(), st.fieldMode,
-                                  st.isosurfaceOpacity, m_renderer.colorTable(),
+                                  st.isosurfaceOpacity, st.isosurfaceLevels.data(),
+                                  (uint32_t)st.isosurfaceLevels.size(), m_renderer.colorTable(),
                                   m_renderer.colorTableSize(), m_viewport, st.viewportSize[0],
                                   st.viewportSize[1], sc.workspace, s));
 }
@@ -652,6 +654,11 @@ extern "C" int rv_worker_set_state(rv_worker* w, const rv_render_state* s)
     st.fieldMode = s->field_mode;
     st.isosurfaceLevel = s->iso_level_rel;
     st.isosurfaceOpacity = s->iso_opacity;
+    if (s->iso_level_count && !s->iso_levels_rel) {
+        rv::set_error("rv_worker_set_state: null isosurface levels");
+        return RV_ERR_INVALID;
+    }
+    st.isosurfaceLevels.assign(s->iso_levels_rel, s->iso_levels_rel + s->iso_level_count);
     w->field.setFieldRange(s->fmin, s->fmax);
     w->worker.setRenderSharedState(&st);
     w->worker.updateState(1u << 7);

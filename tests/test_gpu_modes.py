@@ -18,7 +18,10 @@ from tests.gpu_util import say, to_dev
 pytestmark = pytest.mark.gpu
 
 FOV = 30.0
-MODES = [(api.FIELD_ARGB_LIGHT, 0.5, 1.0), (api.FIELD_ISOSURFACE, 0.45, 0.7), (api.FIELD_MIP, 0.3, 1.0)]
+LEVELS = [0.55, 0.3, 0.42, 0.3]
+MODES = [(api.FIELD_ARGB_LIGHT, 0.5, 1.0, None), (api.FIELD_ISOSURFACE, 0.45, 0.7, None),
+         (api.FIELD_MIP, 0.3, 1.0, None), (api.FIELD_ISOSURFACES, 0.5, 0.6, LEVELS),
+         (api.FIELD_DOMAIN_VOXELS, 0.35, 1.0, None)]
 
 
 def _scene(level, depth, seed):
@@ -46,8 +49,8 @@ def _init_viewport(mode, W, H, seed):
     return rng.integers(0, 2 ** 32, size=(H, W), dtype=np.uint64).astype(np.uint32)
 
 
-@pytest.mark.parametrize("mode,thr,iso", MODES)
-def test_render_block_modes_bit_exact(orc, mode, thr, iso):
+@pytest.mark.parametrize("mode,thr,iso,levels", MODES)
+def test_render_block_modes_bit_exact(orc, mode, thr, iso, levels):
     import torch
     W, H = 160, 120
     blocks = _scene(1, 4, seed=21)
@@ -70,8 +73,8 @@ def test_render_block_modes_bit_exact(orc, mode, thr, iso):
         if not live:
             continue
         say("mode", mode, "block", b.index)
-        api.render_block(p, mode, iso, to_dev(brick), d_tf, d_vp)
-        orc.render_block_mode(po, mode, iso, brick, tf, exp.reshape(-1))
+        api.render_block(p, mode, iso, to_dev(brick), d_tf, d_vp, levels=levels)
+        orc.render_block_mode(po, mode, iso, brick, tf, exp.reshape(-1), levels=levels)
         n += 1
     got = d_vp.cpu().numpy().view(np.uint32)
     assert n == 8
@@ -79,8 +82,8 @@ def test_render_block_modes_bit_exact(orc, mode, thr, iso):
     assert np.array_equal(got, exp)
 
 
-@pytest.mark.parametrize("mode,thr,iso", MODES)
-def test_render_share_modes_equal_per_block(orc, mode, thr, iso):
+@pytest.mark.parametrize("mode,thr,iso,levels", MODES)
+def test_render_share_modes_equal_per_block(orc, mode, thr, iso, levels):
     import torch
     W, H = 150, 110
     blocks = _scene(2, 3, seed=22)
@@ -101,12 +104,12 @@ def test_render_share_modes_equal_per_block(orc, mode, thr, iso):
         lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
         if live:
             items.append((p, d_bricks[int(i)]))
-            api.render_block(p, mode, iso, d_bricks[int(i)], d_tf, vp_seq)
-            orc.render_block_mode(po, mode, iso, bricks[int(i)], tf, exp.reshape(-1))
+            api.render_block(p, mode, iso, d_bricks[int(i)], d_tf, vp_seq, levels=levels)
+            orc.render_block_mode(po, mode, iso, bricks[int(i)], tf, exp.reshape(-1), levels=levels)
     share = api.ShareRenderer(items, d_tf, W, H)
     vp = torch.empty((H, W), dtype=torch.int32, device="cuda")
     api.clear_viewport_mode(vp, W, H, mode)
-    share.render_mode(vp, mode, iso)
+    share.render_mode(vp, mode, iso, levels=levels)
     torch.cuda.synchronize()
     got = vp.cpu().numpy().view(np.uint32)
     assert np.array_equal(got, vp_seq.cpu().numpy().view(np.uint32))
@@ -117,8 +120,8 @@ def test_render_share_modes_equal_per_block(orc, mode, thr, iso):
         assert (got >> 24).any()
 
 
-@pytest.mark.parametrize("mode,thr,iso", MODES)
-def test_frame_modes_match_oracle(orc, mode, thr, iso):
+@pytest.mark.parametrize("mode,thr,iso,levels", MODES)
+def test_frame_modes_match_oracle(orc, mode, thr, iso, levels):
     import torch
     W, H = 144, 108
     blocks = _scene(1, 4, seed=23)
@@ -129,7 +132,7 @@ def test_frame_modes_match_oracle(orc, mode, thr, iso):
     dev = [{"level": b.level, "index": b.index, "depth": b.depth, "sparse": to_dev(b.sparse), "plan": plan,
             "brick": torch.empty(api.brick_nodes(4), dtype=torch.float32, device="cuda")} for b in blocks]
     fp = api.FramePlan(dev, W, H, FOV, fmin, fmax, thr, 2.0, 0.996, 0.9999, d_tf)
-    fp.set_field_mode(mode, iso)
+    fp.set_field_mode(mode, iso, levels=levels)
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((60.0, 10.0), (300.0, -35.0))])
     fp.set_cameras(cams)
     bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
@@ -140,7 +143,7 @@ def test_frame_modes_match_oracle(orc, mode, thr, iso):
         order = api.mixed_back_to_front(keys, api.eye_from_view(cams[c]))
         lst = [(1, blocks[i].index, 4, bricks[i]) for i in order]
         exp = orc.render_float_composite(cams[c], W, H, FOV, lst, fmin, fmax, thr, 2.0, 0.996, tf, 0.9999,
-                                         mode=mode, iso_opacity=iso)
+                                         mode=mode, iso_opacity=iso, levels=levels)
         for interp in (True, False):
             say("frame mode", mode, "cam", c, "interpolate", interp)
             fp.render(c, argb, rgba, interpolate=interp)
@@ -186,3 +189,28 @@ def test_max_intensity_compose_kernels(orc):
     exp = frame.copy()
     orc.mip_colorize(exp, src2, 7, 9, ctv)
     assert np.array_equal(d_frame.cpu().numpy().view(np.uint32), exp)
+
+
+def test_isosurfaces_without_levels(orc):
+    """No levels: every pixel a block covers becomes 0 (per block and share);
+    the frame draws nothing."""
+    import torch
+    W, H = 96, 72
+    blocks = _scene(1, 3, seed=24)
+    bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    fmin, fmax = datasets.field_range(blocks)
+    tf = datasets.seeded_tf(24, 64)
+    d_tf = to_dev(tf)
+    M = datasets.orbit_camera(10.0)
+    init = _init_viewport(api.FIELD_ARGB, W, H, 3)
+    d_vp = to_dev(init.view(np.int32))
+    exp = init.copy()
+    for b, br in zip(blocks, bricks):
+        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, 0.5, 2.0, 0.996)
+        lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, 0.5, 2.0, 0.996)
+        if live:
+            api.render_block(p, api.FIELD_ISOSURFACES, 1.0, to_dev(br), d_tf, d_vp, levels=[])
+            orc.render_block_mode(po, api.FIELD_ISOSURFACES, 1.0, br, tf, exp.reshape(-1), levels=[])
+    got = d_vp.cpu().numpy().view(np.uint32)
+    assert np.array_equal(got, exp)
+    assert (got == 0).sum() > (init == 0).sum()

@@ -130,7 +130,7 @@ def test_worker_compose_frame_bit_exact(orc, K, fused):
     w.close()
 
 
-def _oracle_share_mode(orc, blocks, level, bmin, bmax, M, W, H, fmin, fmax, tf, mode, lvl, iso):
+def _oracle_share_mode(orc, blocks, level, bmin, bmax, M, W, H, fmin, fmax, tf, mode, lvl, iso, levels=None):
     init = 0x7FC00000 if mode == api.FIELD_MIP else 0
     vp = np.full((H, W), init, np.uint32)
     eye = api.eye_from_view(M)
@@ -140,7 +140,8 @@ def _oracle_share_mode(orc, blocks, level, bmin, bmax, M, W, H, fmin, fmax, tf, 
         b = blocks[tuple(int(v) for v in idx)]
         live, po = orc.block_params(M, W, H, FOV, level, b.index, b.depth, fmin, fmax, thr, Q, CUT)
         if live:
-            orc.render_block_mode(po, mode, iso, orc.interpolate(b.sparse, b.n2i, b.depth), tf, vp.reshape(-1))
+            orc.render_block_mode(po, mode, iso, orc.interpolate(b.sparse, b.n2i, b.depth), tf, vp.reshape(-1),
+                                  levels=levels)
     r = orc.bound_rect(vp)
     if r is None:
         return None, None
@@ -150,7 +151,8 @@ def _oracle_share_mode(orc, blocks, level, bmin, bmax, M, W, H, fmin, fmax, tf, 
 
 @pytest.mark.parametrize("mode,fused", [(api.FIELD_ARGB_LIGHT, True), (api.FIELD_ISOSURFACE, False),
                                         (api.FIELD_ISOSURFACE, True), (api.FIELD_MIP, True),
-                                        (api.FIELD_MIP, False)])
+                                        (api.FIELD_MIP, False), (api.FIELD_ISOSURFACES, True),
+                                        (api.FIELD_ISOSURFACES, False), (api.FIELD_DOMAIN_VOXELS, True)])
 def test_worker_field_modes_bit_exact(orc, mode, fused):
     """renderScenePart and the composed frame in the other field modes: the
     isosurface level replaces the threshold; max intensity parts are float
@@ -159,13 +161,14 @@ def test_worker_field_modes_bit_exact(orc, mode, fused):
     W, H = 180, 130
     w, blocks, fmin, fmax, tf, M = _make(2, 3, 11, W, H, 70.0)
     lvl, iso = 0.42, 0.8
+    levels = [0.6, 0.35, 0.45] if mode == api.FIELD_ISOSURFACES else None
     w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, field_mode=mode, iso_level_rel=lvl,
-                iso_opacity=iso)
+                iso_opacity=iso, iso_levels=levels)
     w.set_fused(fused)
     say("worker mode", mode, "fused", fused)
     rect, bits = w.render_scene_part(2, (0, 0, 0), (4, 4, 4))
     erect, ebits = _oracle_share_mode(orc, blocks, 2, (0, 0, 0), (4, 4, 4), M, W, H, fmin, fmax, tf, mode,
-                                      lvl, iso)
+                                      lvl, iso, levels)
     assert rect == erect
     assert np.array_equal(bits, ebits)
     if mode != api.FIELD_MIP:

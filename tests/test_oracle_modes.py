@@ -207,3 +207,72 @@ def test_mip_colorize_calc_color_cases(orc):
     mid = (ctv[0] + ctv[1]) * np.float32(0.5)
     exp = [enc(ctv[0]), enc(ctv[0]), enc(ctv[1]), enc(ctv[2]), enc(ctv[2]), enc(mid), 7]
     assert list(dst[0]) == exp
+
+
+def test_domain_voxels_constant_colour_closed_form(orc):
+    """Every defined sample adds the table colour at 0.5, opacity-corrected:
+    after k samples alpha = 1 - (1 - a')^k and the colour stays the table's;
+    an undefined brick draws nothing."""
+    W, H, depth = 64, 48, 3
+    p = _params(orc, W, H, depth, 0.4, 2.0)
+    vx, vy = p.view_x + p.view_w // 2, p.view_y + p.view_h // 2
+    g = _ray64(p, vx, vy)
+    n = g["n"]
+    tf = np.tile(np.array([[0.25, 0.5, 0.75, 0.3]], np.float32), (8, 1))
+    vp = np.zeros(W * H, np.uint32)
+    orc.render_block_mode(p, 5, 1.0, _linear_brick(n, 0, 0.1, 0.01), tf, vp)
+    got = _unpack(vp[(H - 1 - vy) * W + vx])
+    k = int(p.quality * g["ratio"]) - 1
+    ilc = 100.0 / (2 ** depth * p.quality)
+    a1 = 1 - (1 - 0.3 * 0.4) ** ilc
+    a = 1 - (1 - a1) ** k if 1 - (1 - a1) ** k <= 0.996 else None
+    if a is None:  # early exit: the first sample count passing the cutoff
+        kk = next(i for i in range(1, k + 1) if 1 - (1 - a1) ** i > 0.996)
+        a = 1 - (1 - a1) ** kk
+    exp = _argb8(np.array([0.25, 0.5, 0.75, a]))
+    assert np.abs(got - exp).max() <= 1, (got, exp)
+    vp2 = np.zeros(W * H, np.uint32)
+    orc.render_block_mode(p, 5, 1.0, np.full(n ** 3, np.nan, np.float32), tf, vp2)
+    assert not vp2.any()
+
+
+def test_isosurfaces_two_levels_closed_form(orc):
+    """Two levels crossed by a z-linear field: two shaded points composited
+    front to back (the same table colour, alpha scaled by the opacity);
+    repeated levels count once; no levels make the block's pixels 0."""
+    W, H, depth = 64, 48, 3
+    p = _params(orc, W, H, depth, 0.5, 2.0)
+    vx, vy = p.view_x + p.view_w // 2, p.view_y + p.view_h // 2
+    g = _ray64(p, vx, vy)
+    n = g["n"]
+    slope = 1.0 / (n - 1)
+    brick = _linear_brick(n, 2, 0.0, slope)
+    tf = np.tile(np.array([[0.9, 0.3, 0.1, 0.6]], np.float32), (16, 1))
+    levels = [0.31, 0.62, 0.31]
+    vp = np.zeros(W * H, np.uint32)
+    orc.render_block_mode(p, 1, 0.5, brick, tf, vp, levels=levels)
+    got = _unpack(vp[(H - 1 - vy) * W + vx])
+    count = int(p.quality * g["ratio"])
+    step = g["r"] * ((g["tx"] - g["te"]) / count)
+    pos = [g["eye"] + g["r"] * g["te"] + step * i for i in range(count)]
+    val = lambda q: min(max((q[2] - g["org"][2]) * g["tcf"] - 0.5, 0.0), n - 1.0) * slope
+    acc = np.zeros(4)
+    hits = 0
+    for i in range(1, count):
+        v1, v2 = val(pos[i - 1]), val(pos[i])
+        for lv in (sorted({0.31, 0.62}) if v2 >= v1 else sorted({0.31, 0.62}, reverse=True)):
+            if (v1 - lv) * (v2 - lv) < 0:
+                t = (lv - v1) / (v2 - v1)
+                rgb = _light64(p, tf[0], pos[i] + step * t, np.array([0.0, 0.0, -slope]))
+                c = np.concatenate([rgb, [0.6 * 0.5]])
+                da = c[3] * (1 - acc[3])
+                a = acc[3] + da
+                acc[:3] = (acc[:3] * acc[3] + c[:3] * da) / a
+                acc[3] = a
+                hits += 1
+    assert hits == 2
+    exp = _argb8(acc)
+    assert np.abs(got - exp).max() <= 1, (got, exp)
+    vp0 = np.full(W * H, 0x12345678, np.uint32)
+    orc.render_block_mode(p, 1, 0.5, brick, tf, vp0, levels=[])
+    assert vp0[(H - 1 - vy) * W + vx] == 0
