@@ -498,6 +498,80 @@ int rv_worker_compose_frame(rv_worker* w, uint32_t level, uint32_t split_count, 
 int rv_worker_set_fused(rv_worker* w, int fused);
 void rv_worker_destroy(rv_worker* w);
 
+/* ------------------------------------------------------------------------
+ * Dataset files (host only: these calls make no device call)
+ * ------------------------------------------------------------------------
+ * A dataset is <base>.s3dmm-meta (the block tree and its subtree roots,
+ * Metadata<3>, Metadata.hpp:182-214, 401-499), one <base>.s3dmm-field#<name>
+ * per field (per subtree: float min, max, then its values in n2i order,
+ * BlockTreeFieldProvider.hpp:124-168, 216-262) and the text file
+ * <base>.s3dmm-fields (BlockTreeFieldService.hpp:177-228).  Time-dependent
+ * datasets use <dir>/<stem>/<stem>_tNNNNNN<ext> as the base of frame NNNNNN
+ * (filename_util.hpp:36-48). */
+
+typedef struct rv_dataset rv_dataset;
+
+typedef struct rv_dataset_info {
+    uint32_t format_version;    /* 1, or 0 for files without the header */
+    uint32_t max_subtree_depth; /* maxSubtreeDepth */
+    uint32_t max_full_level;    /* maxFullLevel (0 in version 0) */
+    uint32_t level_count;       /* levels of the subtree maps */
+    float bc_min[3], bc_size;   /* bounding cube of the block tree */
+    uint64_t node_count;        /* blocks in the tree */
+    uint64_t subtree_count;
+    uint64_t field_file_size;   /* bytes of each field file */
+} rv_dataset_info;
+
+typedef struct rv_subtree_info {
+    uint32_t level, location[3]; /* the root block */
+    uint32_t node;               /* subtreeBlockTreePos: the root's node index in the tree */
+    uint32_t depth;              /* subtreeDepth (maxSubtreeDepth in version 0) */
+    uint64_t values_pos;         /* subtreeValuesPos: byte offset in the field files */
+} rv_subtree_info;
+
+typedef struct rv_field_info {
+    char name[64];
+    float min, max;             /* fieldRange */
+    int32_t animated;
+} rv_field_info;
+
+/* Metadata<3>(std::istream&) (Metadata.hpp:182-214): reads the whole file and
+ * checks it (header, tree node count, every subtree root present in the
+ * tree).  RV_ERR_INVALID for a malformed or truncated file. */
+int rv_dataset_open(const char* meta_path, rv_dataset** out);
+void rv_dataset_close(rv_dataset* ds);
+int rv_dataset_get_info(const rv_dataset* ds, rv_dataset_info* info);
+/* Subtree i, in the order of their values in the field files: the order of
+ * Metadata::generate's walkSubtrees (Metadata.hpp:408-433). */
+int rv_dataset_subtree(const rv_dataset* ds, uint64_t i, rv_subtree_info* out);
+/* subtreeData(level, blockIndex) (Metadata.hpp:242-248): the subtree whose
+ * root is this block; RV_ERR_INVALID if none. */
+int rv_dataset_find_subtree(const rv_dataset* ds, uint32_t level, const uint32_t loc[3], uint64_t* i);
+/* closestSubtree (Metadata.hpp:216-233): the subtree containing the block,
+ * found by climbing from (level, block) to the root. */
+int rv_dataset_closest_subtree(const rv_dataset* ds, uint32_t level, const uint32_t block[3],
+                               uint64_t* i);
+/* The subtree's blocks down to its depth, as the children table that
+ * rv_subtree_n2i and rv_subtree_n2i_device take (nnodes x 8, node 0 = the
+ * root; BlockTreeNodes(blockTree, root, subtreeDepth), Metadata.hpp:254-267).
+ * *nnodes is set also when cap_nodes is too small (then RV_ERR_INVALID). */
+int rv_dataset_subtree_children(const rv_dataset* ds, uint64_t i, uint32_t* children,
+                                uint64_t cap_nodes, uint64_t* nnodes);
+/* BlockTreeFieldProvider::fieldValues (BlockTreeFieldProvider.hpp:156-168):
+ * range = the subtree's (min, max), values = n sparse values, n = the n2i
+ * count of the subtree. */
+int rv_dataset_read_field(const rv_dataset* ds, const char* field_path, uint64_t i, float range[2],
+                          float* values, uint64_t n);
+/* Register every subtree rooted at levels level_min..level_max with the
+ * worker (rv_dataset_subtree_children -> rv_subtree_n2i -> the field file ->
+ * rv_worker_add_subtree); *loaded = subtrees registered. */
+int rv_dataset_load_worker(const rv_dataset* ds, const char* field_path, rv_worker* w,
+                           uint32_t level_min, uint32_t level_max, uint64_t* loaded);
+/* The field info file: frame count and per field (name, min, max, animated);
+ * *count is set also when cap is too small (then RV_ERR_INVALID). */
+int rv_fields_info_read(const char* path, uint32_t* time_steps, rv_field_info* fields, uint32_t cap,
+                        uint32_t* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,23 @@ class BrickJob(C.Structure):
                 ("dense16", C.c_void_p), ("depth", C.c_uint32), ("reserved", C.c_uint32)]
 
 
+class DatasetInfo(C.Structure):
+    _fields_ = [("format_version", C.c_uint32), ("max_subtree_depth", C.c_uint32),
+                ("max_full_level", C.c_uint32), ("level_count", C.c_uint32),
+                ("bc_min", C.c_float * 3), ("bc_size", C.c_float), ("node_count", C.c_uint64),
+                ("subtree_count", C.c_uint64), ("field_file_size", C.c_uint64)]
+
+
+class SubtreeInfo(C.Structure):
+    _fields_ = [("level", C.c_uint32), ("location", C.c_uint32 * 3), ("node", C.c_uint32),
+                ("depth", C.c_uint32), ("values_pos", C.c_uint64)]
+
+
+class FieldInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("min", C.c_float), ("max", C.c_float),
+                ("animated", C.c_int32)]
+
+
 _u32p = C.POINTER(C.c_uint32)
 _u16p = C.POINTER(C.c_uint16)
 _f32p = C.POINTER(C.c_float)
@@ -143,6 +160,18 @@ _SIGS = {
     "rv_worker_compose_frame": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _u32p, C.POINTER(C.c_int)]),
     "rv_worker_set_fused": (C.c_int, [_vp, C.c_int]),
     "rv_worker_destroy": (None, [_vp]),
+    "rv_dataset_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "rv_dataset_close": (None, [_vp]),
+    "rv_dataset_get_info": (C.c_int, [_vp, C.POINTER(DatasetInfo)]),
+    "rv_dataset_subtree": (C.c_int, [_vp, C.c_uint64, C.POINTER(SubtreeInfo)]),
+    "rv_dataset_find_subtree": (C.c_int, [_vp, C.c_uint32, _u32p, C.POINTER(C.c_uint64)]),
+    "rv_dataset_closest_subtree": (C.c_int, [_vp, C.c_uint32, _u32p, C.POINTER(C.c_uint64)]),
+    "rv_dataset_subtree_children": (C.c_int, [_vp, C.c_uint64, _u32p, C.c_uint64,
+                                              C.POINTER(C.c_uint64)]),
+    "rv_dataset_read_field": (C.c_int, [_vp, C.c_char_p, C.c_uint64, _f32p, _f32p, C.c_uint64]),
+    "rv_dataset_load_worker": (C.c_int, [_vp, C.c_char_p, _vp, C.c_uint32, C.c_uint32,
+                                         C.POINTER(C.c_uint64)]),
+    "rv_fields_info_read": (C.c_int, [C.c_char_p, _u32p, C.POINTER(FieldInfo), C.c_uint32, _u32p]),
 }
 
 _lib = None

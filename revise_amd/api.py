@@ -4,8 +4,8 @@ import ctypes as C
 
 import numpy as np
 
-from . import (BlockParams, BrickJob, FrameBlock, FrameDesc, RenderState, RvError, check, lib, ptr,
-               stream_ptr)
+from . import (BlockParams, BrickJob, DatasetInfo, FieldInfo, FrameBlock, FrameDesc, RenderState,
+               RvError, SubtreeInfo, check, lib, ptr, stream_ptr)
 
 
 def _np(a, dt):
@@ -466,6 +466,106 @@ class Worker:
     def close(self):
         if self.h:
             lib().rv_worker_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- dataset files ------------------------------------------------------
+
+def dataset_file_names(base, field=None, frame=0, has_time_steps=False):
+    """(meta, field file, field info) paths of a dataset: <base>.s3dmm-meta,
+    <base>.s3dmm-field#<field> and <base>.s3dmm-fields; with time steps the
+    frame's base is <dir>/<stem>/<stem>_tNNNNNN<ext> and the info file
+    <dir>/<stem>/<stem><ext>.s3dmm-fields (filename_util.hpp:36-48,
+    BlockTreeFieldService.hpp:125-131, 177-191)."""
+    import os
+    if has_time_steps:
+        d, name = os.path.split(base)
+        stem, ext = os.path.splitext(name)
+        fbase = os.path.join(d, stem, "%s_t%06d%s" % (stem, frame, ext))
+        info = os.path.join(d, stem, stem + ext + ".s3dmm-fields")
+    else:
+        fbase, info = base, base + ".s3dmm-fields"
+    return (fbase + ".s3dmm-meta", None if field is None else fbase + ".s3dmm-field#" + field, info)
+
+
+def read_fields_info(path):
+    """(frame count, [(name, min, max, animated)]) of a .s3dmm-fields file."""
+    steps, count = C.c_uint32(), C.c_uint32()
+    rc = lib().rv_fields_info_read(path.encode(), C.byref(steps), None, 0, C.byref(count))
+    if count.value == 0:
+        check(rc)
+    arr = (FieldInfo * max(count.value, 1))()
+    check(lib().rv_fields_info_read(path.encode(), C.byref(steps), arr, count.value, C.byref(count)))
+    return steps.value, [(f.name.decode(), f.min, f.max, bool(f.animated)) for f in arr[:count.value]]
+
+
+class Dataset:
+    """rv_dataset: a .s3dmm-meta file (block tree + subtree roots)."""
+
+    def __init__(self, meta_path):
+        h = C.c_void_p()
+        check(lib().rv_dataset_open(meta_path.encode(), C.byref(h)))
+        self.h = h
+        info = DatasetInfo()
+        check(lib().rv_dataset_get_info(self.h, C.byref(info)))
+        self.info = {f: (tuple(getattr(info, f)) if f == "bc_min" else getattr(info, f))
+                     for f, _ in DatasetInfo._fields_}
+
+    def __len__(self):
+        return self.info["subtree_count"]
+
+    def subtree(self, i):
+        s = SubtreeInfo()
+        check(lib().rv_dataset_subtree(self.h, i, C.byref(s)))
+        return dict(level=s.level, location=tuple(s.location), node=s.node, depth=s.depth,
+                    values_pos=s.values_pos)
+
+    def find_subtree(self, level, loc):
+        i = C.c_uint64()
+        l3 = _np(loc, np.uint32)
+        check(lib().rv_dataset_find_subtree(self.h, level, _p(l3, C.c_uint32), C.byref(i)))
+        return i.value
+
+    def closest_subtree(self, level, block):
+        i = C.c_uint64()
+        b3 = _np(block, np.uint32)
+        check(lib().rv_dataset_closest_subtree(self.h, level, _p(b3, C.c_uint32), C.byref(i)))
+        return i.value
+
+    def children(self, i):
+        n = C.c_uint64()
+        lib().rv_dataset_subtree_children(self.h, i, None, 0, C.byref(n))
+        if n.value == 0:
+            check(lib().rv_dataset_subtree_children(self.h, i, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 8), dtype=np.uint32)
+        check(lib().rv_dataset_subtree_children(self.h, i, _p(out, C.c_uint32), n.value, C.byref(n)))
+        return out
+
+    def n2i(self, i):
+        return subtree_n2i(self.children(i), self.subtree(i)["depth"])
+
+    def read_field(self, field_path, i, n):
+        rng = np.zeros(2, dtype=np.float32)
+        vals = np.zeros(n, dtype=np.float32)
+        check(lib().rv_dataset_read_field(self.h, field_path.encode(), i, _p(rng, C.c_float),
+                                          _p(vals, C.c_float), n))
+        return rng, vals
+
+    def load_worker(self, worker, field_path, level_min, level_max):
+        n = C.c_uint64()
+        check(lib().rv_dataset_load_worker(self.h, field_path.encode(), worker.h, level_min, level_max,
+                                           C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self.h:
+            lib().rv_dataset_close(self.h)
             self.h = None
 
     def __del__(self):
