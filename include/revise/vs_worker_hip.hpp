@@ -70,6 +70,13 @@ public:
                     const uint16_t* n2i, uint64_t n, const float* sparse);
     void setFieldRange(float fmin, float fmax);
     std::array<float, 2> fieldRange() const { return {m_fmin, m_fmax}; }
+    // The secondary field of the ValueOnIsosurface(s) modes
+    // (RenderSharedState::secondaryFieldIndex, DenseFieldRenderer_cu.cu:288-307):
+    // values of a registered subtree, in its n2i order (copied), and the
+    // field's range.
+    void addSecondary(unsigned level, const std::array<uint32_t, 3>& index, const float* sparse);
+    void setSecondaryRange(float fmin, float fmax);
+    std::array<float, 2> secondaryRange() const { return {m_smin, m_smax}; }
     bool hasSubtree(unsigned level, const std::array<uint32_t, 3>& index) const;
     unsigned subtreeDepth(unsigned level, const std::array<uint32_t, 3>& index) const;
     // interpolateWith<DenseFieldInterpolatorHip>: interpolate the subtree of
@@ -81,7 +88,10 @@ public:
     // The interpolation job of a subtree into d_dense, for batched
     // interpolation of many blocks in one launch.
     rv_brick_job job(unsigned level, const std::array<uint32_t, 3>& index, float* d_dense,
-                     rv_stream_t stream);
+                     rv_stream_t stream, bool secondary = false);
+    // The same for the secondary field (throws if the subtree has none).
+    void interpolateSecondary(unsigned level, const std::array<uint32_t, 3>& index, float* d_dense,
+                              rv_stream_t stream);
     // Bumped by addSubtree: cached batches of a previous field are stale.
     uint64_t version() const { return m_version; }
 
@@ -89,6 +99,7 @@ private:
     struct Subtree;
     std::map<std::array<uint32_t, 4>, std::unique_ptr<Subtree>> m_subtrees;
     float m_fmin = 0.f, m_fmax = 1.f;
+    float m_smin = 0.f, m_smax = 1.f;
     uint64_t m_version = 0;
 };
 
@@ -142,6 +153,8 @@ private:
     float* m_brick = nullptr;
     uint64_t m_brickNodes = 0;
     std::array<uint32_t, 4> m_lastBlock{~0u, 0, 0, 0};
+    float* m_brick2 = nullptr; // secondary field brick (ValueOnIsosurface(s))
+    std::array<uint32_t, 4> m_lastBlock2{~0u, 0, 0, 0};
 };
 
 class VsWorkerHip {

@@ -228,6 +228,18 @@ enum {
 int rv_render_block(const rv_block_params* params, uint32_t field_mode, float iso_opacity,
                     const float* levels_rel, uint32_t level_count, const float* d_brick,
                     const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, rv_stream_t stream);
+/* rv_render_block with the secondary field, which the modes
+ * ValueOnIsosurface (renderValueOnIsosurface.hpp:32-62) and
+ * ValueOnIsosurfaces (renderValueOnIsosurfaces.hpp:33-109) take: the
+ * primary field's isosurface(s), as in Isosurface(s), each surface point
+ * coloured by the table at (v2 - smin) / (smax - smin), v2 the secondary
+ * field there (d_brick2: the block's brick of it, the primary brick's
+ * layout; [smin, smax] its range, secondaryField.minValue/maxValue), alpha
+ * times iso_opacity.  Other modes ignore d_brick2. */
+int rv_render_block_secondary(const rv_block_params* params, uint32_t field_mode, float iso_opacity,
+                              const float* levels_rel, uint32_t level_count, const float* d_brick,
+                              const float* d_brick2, float smin, float smax, const float* d_tf,
+                              uint32_t tf_n, uint32_t* d_viewport, rv_stream_t stream);
 
 /* clearViewport of a field mode (DenseFieldRenderer_cu.cu:268-277): 0, or NaN
  * bits for MaxIntensityProjection. */
@@ -274,6 +286,14 @@ int rv_render_share(const rv_block_params* params, const float* const* bricks, u
                     uint32_t field_mode, float iso_opacity, const float* levels_rel,
                     uint32_t level_count, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
                     uint32_t vp_w, uint32_t vp_h, void* d_workspace, rv_stream_t stream);
+/* ... and with the secondary field (bricks2: per block; the modes of
+ * rv_render_block_secondary). */
+int rv_render_share_secondary(const rv_block_params* params, const float* const* bricks,
+                              const float* const* bricks2, uint32_t nblocks, uint32_t field_mode,
+                              float iso_opacity, const float* levels_rel, uint32_t level_count,
+                              float smin, float smax, const float* d_tf, uint32_t tf_n,
+                              uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, void* d_workspace,
+                              rv_stream_t stream);
 
 /* Piecewise-linear colour table from control points.  Replaces
  * s3vs::makeColorTransferVector (ColorTransferFunction.hpp:27-69).
@@ -464,6 +484,7 @@ typedef struct rv_render_state {
     float iso_opacity;       /* isosurfaceOpacity */
     const float* iso_levels_rel; /* isosurfaceLevels (Isosurfaces), relative */
     uint32_t iso_level_count;
+    float smin, smax;        /* the secondary field's range (ValueOnIsosurface(s)) */
 } rv_render_state;
 
 /* VsWorkerInterface::initialize: selects GPU thread_id % device count
@@ -474,6 +495,12 @@ int rv_worker_create(uint32_t thread_id, rv_worker** out);
  * are host arrays, copied. */
 int rv_worker_add_subtree(rv_worker* w, uint32_t level, const uint32_t index[3], uint32_t depth,
                           const uint16_t* n2i, uint64_t n, const float* sparse);
+/* The secondary field's values of a registered subtree (same n2i order and
+ * count), which the ValueOnIsosurface(s) modes colour by: the reference's
+ * field service values of secondaryFieldIndex (DenseFieldRenderer_cu.cu:
+ * 288-307).  Copied. */
+int rv_worker_add_subtree_secondary(rv_worker* w, uint32_t level, const uint32_t index[3],
+                                    const float* sparse);
 /* setRenderSharedState + updateState (VsWorkerInterface.hpp:38-111). */
 int rv_worker_set_state(rv_worker* w, const rv_render_state* state);
 /* renderScenePart (VsWorker_cu.cpp:225-313): clear, BackToFrontOrder over the
@@ -567,6 +594,10 @@ int rv_dataset_read_field(const rv_dataset* ds, const char* field_path, uint64_t
  * rv_worker_add_subtree); *loaded = subtrees registered. */
 int rv_dataset_load_worker(const rv_dataset* ds, const char* field_path, rv_worker* w,
                            uint32_t level_min, uint32_t level_max, uint64_t* loaded);
+/* The same for a secondary field (rv_worker_add_subtree_secondary) of
+ * subtrees already registered. */
+int rv_dataset_load_worker_secondary(const rv_dataset* ds, const char* field_path, rv_worker* w,
+                                     uint32_t level_min, uint32_t level_max, uint64_t* loaded);
 /* The field info file: frame count and per field (name, min, max, animated);
  * *count is set also when cap is too small (then RV_ERR_INVALID). */
 int rv_fields_info_read(const char* path, uint32_t* time_steps, rv_field_info* fields, uint32_t cap,

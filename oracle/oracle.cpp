@@ -967,7 +967,16 @@ uint32_t blend_above_u32(uint32_t argb, const float* rgba)
 }
 
 /* ---- field modes (s3vs::FieldMode numbering, FieldMode.hpp:26-51) ---- */
-enum { kIso = 0, kIsos = 1, kMip = 2, kArgb = 3, kArgbLight = 4, kDomain = 5 };
+enum { kIso = 0, kIsos = 1, kMip = 2, kArgb = 3, kArgbLight = 4, kDomain = 5, kValIso = 6, kValIsos = 7 };
+
+/* The secondary field of ValueOnIsosurface(s): the block's brick of it and
+ * the field's range; brick == nullptr for the other modes. */
+struct Second {
+    const float* brick;
+    float smin, smax;
+};
+
+bool multi_level(uint32_t mode) { return mode == kIsos || mode == kValIsos; }
 
 /* makeLevels, DenseFieldRenderer_cu.cu:597-612: relToAbsFieldValue of each
  * level, sorted, duplicates removed, at most 50 (MaxLevels) kept. */
@@ -1077,10 +1086,23 @@ void march_light(const orc_block_params* p, const Derived& d, const float* brick
     }
 }
 
+/* renderValueOnIsosurface(s): the diffuse colour at a surface point is the
+ * table colour of the secondary field there, normalised by its range, alpha
+ * times the isosurface opacity (renderValueOnIsosurface.hpp:44-48,
+ * renderValueOnIsosurfaces.hpp:77-81). */
+void value_diffuse(const Second& s, const Derived& d, const float* tf, uint32_t tf_n,
+                   float iso_opacity, V3 tc, float* diffuse)
+{
+    float v = fetch3(s.brick, d.n, tc);
+    fetch_tf(tf, tf_n, (v - s.smin) / (s.smax - s.smin), diffuse);
+    diffuse[3] = diffuse[3] * iso_opacity;
+}
+
 /* renderIsosurface loop, renderIsosurface.hpp:31-61: the surface point is the
- * second sample's position plus step * (level - v1) / (v2 - v1). */
+ * second sample's position plus step * (level - v1) / (v2 - v1).  With a
+ * secondary field: renderValueOnIsosurface, renderValueOnIsosurface.hpp:32-62. */
 void march_iso(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
-               uint32_t tf_n, float iso_opacity, Ray& r, float* rgba)
+               uint32_t tf_n, float iso_opacity, Ray& r, float* rgba, const Second& sec)
 {
     rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
     float v1 = fetch3(brick, d.n, mul(sub(r.pos, d.origin), d.tcf));
@@ -1094,6 +1116,8 @@ void march_iso(const orc_block_params* p, const Derived& d, const float* brick, 
         if (!std::isnan(v1) && !std::isnan(v2) && (v1 - lv) * (v2 - lv) < 0.f) {
             float t = (lv - v1) / (v2 - v1);
             V3 x = add(r.pos, mul(r.step, t));
+            if (sec.brick)
+                value_diffuse(sec, d, tf, tf_n, iso_opacity, mul(sub(x, d.origin), d.tcf), diffuse);
             float col[4];
             frag_color(p, d, brick, diffuse, x, mul(sub(x, d.origin), d.tcf), col);
             blend_behind4(rgba, col);
@@ -1131,9 +1155,12 @@ uint32_t mip_merge(uint32_t prev_bits, float m, const orc_block_params* p)
     return bits;
 }
 
-/* renderIsosurfaces loop, renderIsosurfaces.hpp:31-103 (levels non-empty) */
+/* renderIsosurfaces loop, renderIsosurfaces.hpp:31-103 (levels non-empty);
+ * with a secondary field renderValueOnIsosurfaces,
+ * renderValueOnIsosurfaces.hpp:33-109. */
 void march_isos(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
-                uint32_t tf_n, float iso_opacity, const std::vector<float>& lv, Ray& r, float* rgba)
+                uint32_t tf_n, float iso_opacity, const std::vector<float>& lv, Ray& r, float* rgba,
+                const Second& sec)
 {
     rgba[0] = rgba[1] = rgba[2] = rgba[3] = 0.f;
     const int n = (int)lv.size();
@@ -1172,11 +1199,15 @@ void march_isos(const orc_block_params* p, const Derived& d, const float* brick,
             float level = lv[il];
             if (!((v1 - level) * (v2 - level) < 0.f))
                 break;
-            float diffuse[4];
-            fetch_tf(tf, tf_n, (level - p->fmin) / (p->fmax - p->fmin), diffuse);
-            diffuse[3] = diffuse[3] * iso_opacity;
             float t = (level - v1) / (v2 - v1);
             V3 x = add(r.pos, mul(r.step, t));
+            float diffuse[4];
+            if (sec.brick) {
+                value_diffuse(sec, d, tf, tf_n, iso_opacity, mul(sub(x, d.origin), d.tcf), diffuse);
+            } else {
+                fetch_tf(tf, tf_n, (level - p->fmin) / (p->fmax - p->fmin), diffuse);
+                diffuse[3] = diffuse[3] * iso_opacity;
+            }
             float col[4];
             frag_color(p, d, brick, diffuse, x, mul(sub(x, d.origin), d.tcf), col);
             blend_behind4(rgba, col);
@@ -1209,14 +1240,15 @@ void march_domain(const orc_block_params* p, const Derived& d, const float* bric
 
 void segment_mode(const orc_block_params* p, const Derived& d, const float* brick, const float* tf,
                   uint32_t tf_n, uint32_t mode, float iso_opacity, const std::vector<float>& lv,
-                  Ray& r, float* rgba)
+                  Ray& r, float* rgba, const Second& sec = Second{nullptr, 0.f, 1.f})
 {
+    const Second none{nullptr, 0.f, 1.f};
     if (mode == kArgbLight)
         march_light(p, d, brick, tf, tf_n, r, rgba);
-    else if (mode == kIso)
-        march_iso(p, d, brick, tf, tf_n, iso_opacity, r, rgba);
-    else if (mode == kIsos)
-        march_isos(p, d, brick, tf, tf_n, iso_opacity, lv, r, rgba);
+    else if (mode == kIso || mode == kValIso)
+        march_iso(p, d, brick, tf, tf_n, iso_opacity, r, rgba, mode == kValIso ? sec : none);
+    else if (mode == kIsos || mode == kValIsos)
+        march_isos(p, d, brick, tf, tf_n, iso_opacity, lv, r, rgba, mode == kValIsos ? sec : none);
     else if (mode == kDomain)
         march_domain(p, d, brick, tf, tf_n, r, rgba);
     else
@@ -1245,11 +1277,13 @@ extern "C" void orc_render_block(const orc_block_params* p, const float* brick, 
 
 /* renderKernel<fieldMode> for one block in a field mode (Argb, ArgbLight,
  * Isosurface, MaxIntensityProjection), DenseFieldRenderer_cu.cu:62-215. */
-extern "C" void orc_render_block_mode(const orc_block_params* p, uint32_t mode, float iso_opacity,
-                                      const float* levels_rel, uint32_t level_count,
-                                      const float* brick, const float* tf, uint32_t tf_n,
-                                      uint32_t* viewport, uint32_t row0, uint32_t row1)
+extern "C" void orc_render_block_mode2(const orc_block_params* p, uint32_t mode, float iso_opacity,
+                                       const float* levels_rel, uint32_t level_count,
+                                       const float* brick, const float* brick2, float smin,
+                                       float smax, const float* tf, uint32_t tf_n,
+                                       uint32_t* viewport, uint32_t row0, uint32_t row1)
 {
+    const Second sec{brick2, smin, smax};
     Derived d = derive(p);
     const std::vector<float> lv = make_levels(p->fmin, p->fmax, levels_rel, level_count);
     for (uint32_t vy = p->view_y; vy < p->view_y + p->view_h; ++vy) {
@@ -1263,15 +1297,24 @@ extern "C" void orc_render_block_mode(const orc_block_params* p, uint32_t mode, 
                 viewport[r.pixel] = mip_merge(viewport[r.pixel], march_mip(p, d, brick, r), p);
                 continue;
             }
-            if (mode == kIsos && lv.empty()) {
+            if (multi_level(mode) && lv.empty()) {
                 viewport[r.pixel] = 0u; /* renderIsosurfaces.hpp:33-34 */
                 continue;
             }
             float rgba[4];
-            segment_mode(p, d, brick, tf, tf_n, mode, iso_opacity, lv, r, rgba);
+            segment_mode(p, d, brick, tf, tf_n, mode, iso_opacity, lv, r, rgba, sec);
             viewport[r.pixel] = blend_above_u32(viewport[r.pixel], rgba);
         }
     }
+}
+
+extern "C" void orc_render_block_mode(const orc_block_params* p, uint32_t mode, float iso_opacity,
+                                      const float* levels_rel, uint32_t level_count,
+                                      const float* brick, const float* tf, uint32_t tf_n,
+                                      uint32_t* viewport, uint32_t row0, uint32_t row1)
+{
+    orc_render_block_mode2(p, mode, iso_opacity, levels_rel, level_count, brick, nullptr, 0.f, 1.f, tf,
+                           tf_n, viewport, row0, row1);
 }
 
 /* blendImagePartMaxIntensity, BlendImagePart.hpp:148-191 */

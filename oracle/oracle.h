@@ -113,6 +113,12 @@ void orc_render_block_mode(const orc_block_params* p, uint32_t mode, float iso_o
                            const float* levels_rel, uint32_t level_count, const float* brick,
                            const float* tf, uint32_t tf_n, uint32_t* viewport, uint32_t row0,
                            uint32_t row1);
+/* The same with the secondary field of modes 6 ValueOnIsosurface and 7
+ * ValueOnIsosurfaces (brick2: the block's brick of it, [smin, smax] its range). */
+void orc_render_block_mode2(const orc_block_params* p, uint32_t mode, float iso_opacity,
+                            const float* levels_rel, uint32_t level_count, const float* brick,
+                            const float* brick2, float smin, float smax, const float* tf,
+                            uint32_t tf_n, uint32_t* viewport, uint32_t row0, uint32_t row1);
 
 /* blendImagePartMaxIntensity (BlendImagePart.hpp:148-191) of a float part at
  * (dx, dy) of a dw-wide float image. */

@@ -102,6 +102,9 @@ def lib():
                                                  C.c_uint32, f32p, C.c_uint32]
         L.orc_render_block_mode.argtypes = [C.POINTER(BlockParams), C.c_uint32, C.c_float, f32p, C.c_uint32,
                                             f32p, f32p, C.c_uint32, u32p, C.c_uint32, C.c_uint32]
+        L.orc_render_block_mode2.argtypes = [C.POINTER(BlockParams), C.c_uint32, C.c_float, f32p,
+                                             C.c_uint32, f32p, f32p, C.c_float, C.c_float, f32p, C.c_uint32,
+                                             u32p, C.c_uint32, C.c_uint32]
         L.orc_blend_image_part_max_intensity.argtypes = [f32p, C.c_int32, f32p, C.c_int32, C.c_int32,
                                                          C.c_int32, C.c_int32, C.c_int]
         L.orc_mip_colorize.argtypes = [u32p, C.c_int32, f32p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -221,14 +224,25 @@ def render_block(params, brick, tf, viewport):
                            _p(viewport, C.c_uint32), 0, params.vp_h)
 
 
-def render_block_mode(params, mode, iso_opacity, brick, tf, viewport, levels=None):
+def render_block_mode(params, mode, iso_opacity, brick, tf, viewport, levels=None, brick2=None,
+                      srange=(0.0, 1.0)):
     """Field modes: 0 Isosurface, 1 Isosurfaces (levels relative), 2
-    MaxIntensityProjection (float bits), 3 Argb, 4 ArgbLight, 5 DomainVoxels."""
+    MaxIntensityProjection (float bits), 3 Argb, 4 ArgbLight, 5 DomainVoxels,
+    6 ValueOnIsosurface and 7 ValueOnIsosurfaces (brick2: the secondary
+    field's brick, srange its range)."""
     brick = np.ascontiguousarray(brick, dtype=np.float32)
     tf = np.ascontiguousarray(tf, dtype=np.float32)
     lv = np.ascontiguousarray(levels if levels is not None else [0.0], dtype=np.float32)
     nlv = 0 if levels is None else len(levels)
     assert viewport.dtype == np.uint32 and viewport.flags.c_contiguous
+    if mode in (6, 7):
+        b2 = np.ascontiguousarray(brick2, dtype=np.float32)
+        assert b2.shape == brick.shape
+        lib().orc_render_block_mode2(C.byref(params), mode, iso_opacity, _p(lv, C.c_float), nlv,
+                                     _p(brick, C.c_float), _p(b2, C.c_float), srange[0], srange[1],
+                                     _p(tf, C.c_float), tf.shape[0], _p(viewport, C.c_uint32), 0,
+                                     params.vp_h)
+        return
     lib().orc_render_block_mode(C.byref(params), mode, iso_opacity, _p(lv, C.c_float), nlv,
                                 _p(brick, C.c_float), _p(tf, C.c_float), tf.shape[0],
                                 _p(viewport, C.c_uint32), 0, params.vp_h)

@@ -52,7 +52,8 @@ class RenderState(C.Structure):
                 ("alpha_cutoff", C.c_float), ("fmin", C.c_float), ("fmax", C.c_float),
                 ("background", C.c_float * 3), ("tf", C.c_void_p), ("tf_n", C.c_uint32),
                 ("field_mode", C.c_uint32), ("iso_level_rel", C.c_float), ("iso_opacity", C.c_float),
-                ("iso_levels_rel", C.c_void_p), ("iso_level_count", C.c_uint32)]
+                ("iso_levels_rel", C.c_void_p), ("iso_level_count", C.c_uint32),
+                ("smin", C.c_float), ("smax", C.c_float)]
 
 
 class BrickJob(C.Structure):
@@ -122,6 +123,13 @@ _SIGS = {
                                   C.c_float, _f32p, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32,
                                   C.c_uint32, _vp, _vp]),
     "rv_share_workspace_bytes": (C.c_uint64, [C.c_uint32]),
+    "rv_render_block_secondary": (C.c_int, [C.POINTER(BlockParams), C.c_uint32, C.c_float, _f32p,
+                                            C.c_uint32, _vp, _vp, C.c_float, C.c_float, _vp, C.c_uint32,
+                                            _vp, _vp]),
+    "rv_render_share_secondary": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p),
+                                            C.POINTER(C.c_void_p), C.c_uint32, C.c_uint32, C.c_float,
+                                            _f32p, C.c_uint32, C.c_float, C.c_float, _vp, C.c_uint32, _vp,
+                                            C.c_uint32, C.c_uint32, _vp, _vp]),
     "rv_render_share_argb": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p), C.c_uint32, _vp,
                                        C.c_uint32, _vp, C.c_uint32, C.c_uint32, _vp, _vp]),
     "rv_color_transfer_vector": (C.c_int, [_f32p, _f32p, C.c_uint32, C.c_uint32, _f32p]),
@@ -154,6 +162,7 @@ _SIGS = {
     "rv_frame_plan_destroy": (None, [_vp]),
     "rv_worker_create": (C.c_int, [C.c_uint32, C.POINTER(C.c_void_p)]),
     "rv_worker_add_subtree": (C.c_int, [_vp, C.c_uint32, _u32p, C.c_uint32, _u16p, C.c_uint64, _f32p]),
+    "rv_worker_add_subtree_secondary": (C.c_int, [_vp, C.c_uint32, _u32p, _f32p]),
     "rv_worker_set_state": (C.c_int, [_vp, C.POINTER(RenderState)]),
     "rv_worker_render_scene_part": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_int),
                                               C.POINTER(C.c_int32), _u32p, C.c_uint64]),
@@ -171,6 +180,8 @@ _SIGS = {
     "rv_dataset_read_field": (C.c_int, [_vp, C.c_char_p, C.c_uint64, _f32p, _f32p, C.c_uint64]),
     "rv_dataset_load_worker": (C.c_int, [_vp, C.c_char_p, _vp, C.c_uint32, C.c_uint32,
                                          C.POINTER(C.c_uint64)]),
+    "rv_dataset_load_worker_secondary": (C.c_int, [_vp, C.c_char_p, _vp, C.c_uint32, C.c_uint32,
+                                                   C.POINTER(C.c_uint64)]),
     "rv_fields_info_read": (C.c_int, [C.c_char_p, _u32p, C.POINTER(FieldInfo), C.c_uint32, _u32p]),
 }
 

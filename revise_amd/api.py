@@ -227,6 +227,7 @@ def subtree_n2i_device(d_children, nnodes, depth, d_out, stream=None):
 
 # s3vs::FieldMode (FieldMode.hpp:26-51)
 FIELD_ISOSURFACE, FIELD_ISOSURFACES, FIELD_MIP, FIELD_ARGB, FIELD_ARGB_LIGHT, FIELD_DOMAIN_VOXELS = 0, 1, 2, 3, 4, 5
+FIELD_VALUE_ON_ISOSURFACE, FIELD_VALUE_ON_ISOSURFACES = 6, 7
 
 
 def _levels(levels):
@@ -234,8 +235,16 @@ def _levels(levels):
     return lv, (_p(lv, C.c_float) if lv.size else None), lv.size
 
 
-def render_block(params, mode, iso_opacity, d_brick, d_tf, d_vp, levels=None, stream=None):
+def render_block(params, mode, iso_opacity, d_brick, d_tf, d_vp, levels=None, stream=None, d_brick2=None,
+                 srange=None):
+    """rv_render_block; with d_brick2 (and srange = the secondary field's
+    range) rv_render_block_secondary, for the ValueOnIsosurface(s) modes."""
     lv, lp, ln = _levels(levels)
+    if d_brick2 is not None:
+        check(lib().rv_render_block_secondary(C.byref(params), mode, iso_opacity, lp, ln, ptr(d_brick),
+                                              ptr(d_brick2), srange[0], srange[1], ptr(d_tf), d_tf.shape[0],
+                                              ptr(d_vp), stream_ptr(stream)))
+        return
     check(lib().rv_render_block(C.byref(params), mode, iso_opacity, lp, ln, ptr(d_brick), ptr(d_tf),
                                 d_tf.shape[0], ptr(d_vp), stream_ptr(stream)))
 
@@ -264,19 +273,29 @@ class ShareRenderer:
     """rv_render_share_argb over a fixed list of (params, brick) in
     back-to-front order."""
 
-    def __init__(self, items, d_tf, W, H):
+    def __init__(self, items, d_tf, W, H, bricks2=None):
         import torch
         live = [(p, b) for p, b in items]
         self.n = len(live)
         self.params = (BlockParams * max(self.n, 1))(*[p for p, _ in live])
         self.bricks = (C.c_void_p * max(self.n, 1))(*[b.data_ptr() for _, b in live])
-        self._keep = [b for _, b in live] + [d_tf]
+        self.bricks2 = None
+        if bricks2 is not None:
+            assert len(bricks2) == self.n
+            self.bricks2 = (C.c_void_p * max(self.n, 1))(*[b.data_ptr() for b in bricks2])
+        self._keep = [b for _, b in live] + [d_tf] + list(bricks2 or [])
         self.tf, self.W, self.H = d_tf, W, H
         nbytes = int(lib().rv_share_workspace_bytes(self.n))
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
 
-    def render_mode(self, d_vp, mode, iso_opacity=1.0, levels=None, stream=None):
+    def render_mode(self, d_vp, mode, iso_opacity=1.0, levels=None, stream=None, srange=None):
         lv, lp, ln = _levels(levels)
+        if self.bricks2 is not None:
+            check(lib().rv_render_share_secondary(self.params, self.bricks, self.bricks2, self.n, mode,
+                                                  iso_opacity, lp, ln, srange[0], srange[1], ptr(self.tf),
+                                                  self.tf.shape[0], ptr(d_vp), self.W, self.H, ptr(self.ws),
+                                                  stream_ptr(stream)))
+            return
         check(lib().rv_render_share(self.params, self.bricks, self.n, mode, iso_opacity, lp, ln, ptr(self.tf),
                                     self.tf.shape[0], ptr(d_vp), self.W, self.H, ptr(self.ws),
                                     stream_ptr(stream)))
@@ -428,15 +447,23 @@ class Worker:
         check(lib().rv_worker_add_subtree(self.h, level, _p(idx, C.c_uint32), depth, _p(n2i, C.c_uint16),
                                           n2i.shape[0], _p(sp, C.c_float)))
 
+    def add_subtree_secondary(self, level, index, sparse):
+        """Secondary field values of a registered subtree (ValueOnIsosurface(s))."""
+        idx = _np(index, np.uint32)
+        sp = _np(sparse, np.float32)
+        check(lib().rv_worker_add_subtree_secondary(self.h, level, _p(idx, C.c_uint32), _p(sp, C.c_float)))
+
     def set_state(self, view16, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin, fmax, background,
-                  tf, field_mode=FIELD_ARGB, iso_level_rel=0.5, iso_opacity=1.0, iso_levels=None):
+                  tf, field_mode=FIELD_ARGB, iso_level_rel=0.5, iso_opacity=1.0, iso_levels=None,
+                  secondary_range=(0.0, 1.0)):
         self._view = _np(view16, np.float32).reshape(16)
         self._tf = _np(tf, np.float32)
         self._levels = _np(iso_levels if iso_levels is not None else [], np.float32).reshape(-1)
         st = RenderState(self._view.ctypes.data, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin,
                          fmax, (C.c_float * 3)(*background), self._tf.ctypes.data, self._tf.shape[0],
                          field_mode, iso_level_rel, iso_opacity,
-                         self._levels.ctypes.data if self._levels.size else None, self._levels.size)
+                         self._levels.ctypes.data if self._levels.size else None, self._levels.size,
+                         secondary_range[0], secondary_range[1])
         check(lib().rv_worker_set_state(self.h, C.byref(st)))
         self.W, self.H = vp_w, vp_h
 
@@ -557,10 +584,13 @@ class Dataset:
                                           _p(vals, C.c_float), n))
         return rng, vals
 
-    def load_worker(self, worker, field_path, level_min, level_max):
+    def load_worker(self, worker, field_path, level_min, level_max, secondary=False):
+        """Register the subtrees of levels level_min..level_max with the
+        worker, values from field_path; secondary: as the secondary field of
+        subtrees already registered."""
         n = C.c_uint64()
-        check(lib().rv_dataset_load_worker(self.h, field_path.encode(), worker.h, level_min, level_max,
-                                           C.byref(n)))
+        f = lib().rv_dataset_load_worker_secondary if secondary else lib().rv_dataset_load_worker
+        check(f(self.h, field_path.encode(), worker.h, level_min, level_max, C.byref(n)))
         return n.value
 
     def close(self):

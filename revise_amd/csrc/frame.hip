@@ -255,6 +255,8 @@ template <uint32_t kMode>
 __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* __restrict__ params,
                                                         const uint4* __restrict__ rects,
                                                         const void* const* __restrict__ bricks,
+                                                        const void* const* __restrict__ bricks2,
+                                                        float smin, float smax,
                                                         uint32_t nblocks, const float4* __restrict__ tf,
                                                         uint32_t tf_n, float iso_opacity, LevelsArg lev,
                                                         uint32_t* __restrict__ viewport,
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
     __shared__ float lvs[kMaxLevels];
     for (uint32_t i = threadIdx.x; i < tf_n; i += kShareT)
         tfs_share[i] = tf[i];
-    if (kMode == kModeIsosurfaces && threadIdx.x < lev.n)
+    if (multi_level(kMode) && threadIdx.x < lev.n)
         lvs[threadIdx.x] = lev.v[threadIdx.x];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -293,13 +295,16 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                 const RaySeg r = setup_ray(p, vx, vy);
                 if (r.hit) {
                     const float* brick = reinterpret_cast<const float*>(bricks[blk]);
+                    Secondary<float> sec{nullptr, smin, smax};
+                    if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces)
+                        sec.brick = reinterpret_cast<const float*>(bricks2[blk]);
                     if (kMode == kModeMip)
                         value = mip_merge(value, march_mip(p, brick, r), p);
-                    else if (kMode == kModeIsosurfaces && lev.n == 0)
+                    else if (multi_level(kMode) && lev.n == 0)
                         value = 0u; // renderIsosurfaces.hpp:33-34
                     else
                         value = blend_above_u32(value, segment<kMode>(p, brick, tfs_share, (int)tf_n,
-                                                                      iso_opacity, r, lvs, lev.n));
+                                                                      iso_opacity, r, lvs, lev.n, sec));
                 }
             }
         }
@@ -979,9 +984,13 @@ extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, fl
     case RV_FIELD_DOMAIN_VOXELS:
         rc = frame_max_wgs<kModeDomainVoxels>(P->fp16, &per_cu);
         break;
+    case RV_FIELD_VALUE_ON_ISOSURFACE:
+    case RV_FIELD_VALUE_ON_ISOSURFACES:
+        return fail(RV_ERR_UNSUPPORTED, "rv_frame_set_field_mode: the frame plan holds one field; "
+                                        "the ValueOnIsosurface(s) modes render through "
+                                        "rv_render_block_secondary / rv_render_share_secondary");
     default:
-        return fail(RV_ERR_INVALID, "rv_frame_set_field_mode: field mode not supported (the "
-                                    "secondary-field modes ValueOnIsosurface(s) are not built)");
+        return fail(RV_ERR_INVALID, "rv_frame_set_field_mode: unknown field mode");
     }
     if (rc)
         return rc;
@@ -1007,19 +1016,24 @@ extern "C" void rv_frame_plan_destroy(rv_frame_plan* P) { delete P; }
 
 extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
 {
-    return (uint64_t)nblocks * (sizeof(rv_block_params) + sizeof(uint4) + sizeof(void*)) + 256;
+    return (uint64_t)nblocks * (sizeof(rv_block_params) + sizeof(uint4) + 2 * sizeof(void*)) + 256;
 }
 
-extern "C" int rv_render_share(const rv_block_params* params, const float* const* bricks,
-                               uint32_t nblocks, uint32_t field_mode, float iso_opacity,
-                               const float* levels_rel, uint32_t level_count, const float* d_tf,
-                               uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
-                               void* d_workspace, rv_stream_t stream)
+extern "C" int rv_render_share_secondary(const rv_block_params* params, const float* const* bricks,
+                                         const float* const* bricks2, uint32_t nblocks,
+                                         uint32_t field_mode, float iso_opacity,
+                                         const float* levels_rel, uint32_t level_count, float smin,
+                                         float smax, const float* d_tf, uint32_t tf_n,
+                                         uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                         void* d_workspace, rv_stream_t stream)
 {
     RV_TRY
-    RV_REQUIRE(field_mode <= RV_FIELD_DOMAIN_VOXELS,
-               "rv_render_share: field mode not supported (the secondary-field modes "
-               "ValueOnIsosurface(s) are not built)");
+    RV_REQUIRE(field_mode <= RV_FIELD_VALUE_ON_ISOSURFACES, "rv_render_share: unknown field mode");
+    const bool value = field_mode == RV_FIELD_VALUE_ON_ISOSURFACE ||
+                       field_mode == RV_FIELD_VALUE_ON_ISOSURFACES;
+    RV_REQUIRE(!value || (bricks2 && smax > smin),
+               "rv_render_share: the ValueOnIsosurface(s) modes need the secondary field's bricks "
+               "and a non-empty range");
     RV_REQUIRE(levels_rel || level_count == 0, "rv_render_share: null levels");
     if (nblocks == 0)
         return RV_OK;
@@ -1038,11 +1052,14 @@ extern "C" int rv_render_share(const rv_block_params* params, const float* const
         RV_REQUIRE((uint64_t)p.view_x + p.view_w <= vp_w && (uint64_t)p.view_y + p.view_h <= vp_h,
                    "rv_render_share: view rectangle outside the viewport");
         RV_REQUIRE(bricks[i] || p.view_w == 0 || p.view_h == 0, "rv_render_share: null brick");
+        RV_REQUIRE(!value || bricks2[i] || p.view_w == 0 || p.view_h == 0,
+                   "rv_render_share: null secondary brick");
         RV_REQUIRE(p.fmin == params[0].fmin && p.fmax == params[0].fmax,
                    "rv_render_share: the blocks of a share have one field range");
         hp[i] = p;
         hr[i] = make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h);
         hb[i] = bricks[i];
+        hb[nblocks + i] = value ? bricks2[i] : nullptr;
     }
     uint8_t* d = static_cast<uint8_t*>(d_workspace);
     RV_HIP(hipMemcpyAsync(d, host.data(), host.size() - 256, hipMemcpyHostToDevice, S(stream)));
@@ -1060,35 +1077,35 @@ extern "C" int rv_render_share(const rv_block_params* params, const float* const
     const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);
     // the levels are relative to the field's range, the same for every block
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
+    const auto* db2 = db + nblocks;
+#define RV_SHARE_LAUNCH(M)                                                                         \
+    hipLaunchKernelGGL(share_kernel<M>, grid, dim3(kShareT), lds, S(stream), dp, dr, db, db2, smin, smax, \
+                       nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h)
     switch (field_mode) {
-    case RV_FIELD_ARGB:
-        hipLaunchKernelGGL(share_kernel<kModeArgb>, grid, dim3(kShareT), lds, S(stream), dp, dr, db,
-                           nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
-        break;
-    case RV_FIELD_ARGB_LIGHT:
-        hipLaunchKernelGGL(share_kernel<kModeArgbLight>, grid, dim3(kShareT), lds, S(stream), dp, dr,
-                           db, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
-        break;
-    case RV_FIELD_ISOSURFACE:
-        hipLaunchKernelGGL(share_kernel<kModeIsosurface>, grid, dim3(kShareT), lds, S(stream), dp, dr,
-                           db, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
-        break;
-    case RV_FIELD_ISOSURFACES:
-        hipLaunchKernelGGL(share_kernel<kModeIsosurfaces>, grid, dim3(kShareT), lds, S(stream), dp, dr,
-                           db, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
-        break;
-    case RV_FIELD_DOMAIN_VOXELS:
-        hipLaunchKernelGGL(share_kernel<kModeDomainVoxels>, grid, dim3(kShareT), lds, S(stream), dp, dr,
-                           db, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
-        break;
-    default:
-        hipLaunchKernelGGL(share_kernel<kModeMip>, grid, dim3(kShareT), lds, S(stream), dp, dr, db,
-                           nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h);
-        break;
+    case RV_FIELD_ARGB: RV_SHARE_LAUNCH(kModeArgb); break;
+    case RV_FIELD_ARGB_LIGHT: RV_SHARE_LAUNCH(kModeArgbLight); break;
+    case RV_FIELD_ISOSURFACE: RV_SHARE_LAUNCH(kModeIsosurface); break;
+    case RV_FIELD_ISOSURFACES: RV_SHARE_LAUNCH(kModeIsosurfaces); break;
+    case RV_FIELD_DOMAIN_VOXELS: RV_SHARE_LAUNCH(kModeDomainVoxels); break;
+    case RV_FIELD_VALUE_ON_ISOSURFACE: RV_SHARE_LAUNCH(kModeValueOnIsosurface); break;
+    case RV_FIELD_VALUE_ON_ISOSURFACES: RV_SHARE_LAUNCH(kModeValueOnIsosurfaces); break;
+    default: RV_SHARE_LAUNCH(kModeMip); break;
     }
+#undef RV_SHARE_LAUNCH
     RV_LAUNCH_CHECK();
     return RV_OK;
     RV_CATCH
+}
+
+extern "C" int rv_render_share(const rv_block_params* params, const float* const* bricks,
+                               uint32_t nblocks, uint32_t field_mode, float iso_opacity,
+                               const float* levels_rel, uint32_t level_count, const float* d_tf,
+                               uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                               void* d_workspace, rv_stream_t stream)
+{
+    return rv_render_share_secondary(params, bricks, nullptr, nblocks, field_mode, iso_opacity,
+                                     levels_rel, level_count, 0.f, 1.f, d_tf, tf_n, d_viewport, vp_w,
+                                     vp_h, d_workspace, stream);
 }
 
 extern "C" int rv_render_share_argb(const rv_block_params* params, const float* const* bricks,

@@ -24,6 +24,7 @@ constexpr uint32_t kTfMax = 4096;
 template <uint32_t kMode>
 __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
                                                            const float* __restrict__ brick,
+                                                           Secondary<float> sec,
                                                            const float* __restrict__ tf,
                                                            uint32_t tf_n, float iso_opacity,
                                                            LevelsArg lev,
@@ -33,7 +34,7 @@ __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
     __shared__ float lvs[kMaxLevels];
     for (uint32_t i = threadIdx.x; i < tf_n; i += 256u)
         tfs[i] = reinterpret_cast<const float4*>(tf)[i];
-    if (kMode == kModeIsosurfaces && threadIdx.x < lev.n)
+    if (multi_level(kMode) && threadIdx.x < lev.n)
         lvs[threadIdx.x] = lev.v[threadIdx.x];
     __syncthreads();
     uint32_t lx, ly;
@@ -50,11 +51,13 @@ __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
         viewport[pix] = mip_merge(viewport[pix], march_mip(p, brick, r), p);
         return;
     }
-    if (kMode == kModeIsosurfaces && lev.n == 0) {
-        viewport[pix] = 0u; // renderIsosurfaces.hpp:33-34: no levels, the pixel is 0
+    if (multi_level(kMode) && lev.n == 0) {
+        // renderIsosurfaces.hpp:33-34, renderValueOnIsosurfaces.hpp:35-36:
+        // no levels, the pixel is 0
+        viewport[pix] = 0u;
         return;
     }
-    const float4 rgba = segment<kMode>(p, brick, tfs, (int)tf_n, iso_opacity, r, lvs, lev.n);
+    const float4 rgba = segment<kMode>(p, brick, tfs, (int)tf_n, iso_opacity, r, lvs, lev.n, sec);
     viewport[pix] = blend_above_u32(viewport[pix], rgba);
 }
 
@@ -255,52 +258,55 @@ static int check_block(const rv_block_params* params, const float* d_brick, cons
     return RV_OK;
 }
 
-extern "C" int rv_render_block(const rv_block_params* params, uint32_t field_mode, float iso_opacity,
-                               const float* levels_rel, uint32_t level_count, const float* d_brick,
-                               const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
-                               rv_stream_t stream)
+extern "C" int rv_render_block_secondary(const rv_block_params* params, uint32_t field_mode,
+                                         float iso_opacity, const float* levels_rel,
+                                         uint32_t level_count, const float* d_brick,
+                                         const float* d_brick2, float smin, float smax,
+                                         const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
+                                         rv_stream_t stream)
 {
     int rc = check_block(params, d_brick, d_tf, tf_n, d_viewport, "rv_render_block");
     if (rc)
         return rc;
     RV_REQUIRE(levels_rel || level_count == 0, "rv_render_block: null levels");
+    const bool value = field_mode == RV_FIELD_VALUE_ON_ISOSURFACE ||
+                       field_mode == RV_FIELD_VALUE_ON_ISOSURFACES;
+    RV_REQUIRE(!value || d_brick2, "rv_render_block: the ValueOnIsosurface(s) modes need the "
+                                   "secondary field's brick");
+    RV_REQUIRE(!value || smax > smin, "rv_render_block: the secondary field range must be non-empty");
     const rv_block_params& p = *params;
     if (p.view_w == 0 || p.view_h == 0)
         return RV_OK;
     const LevelsArg lev = make_levels(p.fmin, p.fmax, levels_rel, level_count);
+    const Secondary<float> sec{value ? d_brick2 : nullptr, smin, smax};
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
     const size_t lds = tf_n * sizeof(float4);
+#define RV_BLOCK_LAUNCH(M)                                                                         \
+    hipLaunchKernelGGL(render_block_kernel<M>, grid, dim3(256), lds, S(stream), p, d_brick, sec, d_tf, \
+                       tf_n, iso_opacity, lev, d_viewport)
     switch (field_mode) {
-    case RV_FIELD_ARGB:
-        hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), lds, S(stream), p, d_brick,
-                           d_tf, tf_n, iso_opacity, lev, d_viewport);
-        break;
-    case RV_FIELD_ARGB_LIGHT:
-        hipLaunchKernelGGL(render_block_kernel<kModeArgbLight>, grid, dim3(256), lds, S(stream), p,
-                           d_brick, d_tf, tf_n, iso_opacity, lev, d_viewport);
-        break;
-    case RV_FIELD_ISOSURFACE:
-        hipLaunchKernelGGL(render_block_kernel<kModeIsosurface>, grid, dim3(256), lds, S(stream), p,
-                           d_brick, d_tf, tf_n, iso_opacity, lev, d_viewport);
-        break;
-    case RV_FIELD_ISOSURFACES:
-        hipLaunchKernelGGL(render_block_kernel<kModeIsosurfaces>, grid, dim3(256), lds, S(stream), p,
-                           d_brick, d_tf, tf_n, iso_opacity, lev, d_viewport);
-        break;
-    case RV_FIELD_DOMAIN_VOXELS:
-        hipLaunchKernelGGL(render_block_kernel<kModeDomainVoxels>, grid, dim3(256), lds, S(stream), p,
-                           d_brick, d_tf, tf_n, iso_opacity, lev, d_viewport);
-        break;
-    case RV_FIELD_MAX_INTENSITY_PROJECTION:
-        hipLaunchKernelGGL(render_block_kernel<kModeMip>, grid, dim3(256), lds, S(stream), p, d_brick,
-                           d_tf, tf_n, iso_opacity, lev, d_viewport);
-        break;
-    default:
-        return fail(RV_ERR_INVALID, "rv_render_block: field mode not supported (the secondary-field "
-                                    "modes ValueOnIsosurface(s) are not built)");
+    case RV_FIELD_ARGB: RV_BLOCK_LAUNCH(kModeArgb); break;
+    case RV_FIELD_ARGB_LIGHT: RV_BLOCK_LAUNCH(kModeArgbLight); break;
+    case RV_FIELD_ISOSURFACE: RV_BLOCK_LAUNCH(kModeIsosurface); break;
+    case RV_FIELD_ISOSURFACES: RV_BLOCK_LAUNCH(kModeIsosurfaces); break;
+    case RV_FIELD_DOMAIN_VOXELS: RV_BLOCK_LAUNCH(kModeDomainVoxels); break;
+    case RV_FIELD_MAX_INTENSITY_PROJECTION: RV_BLOCK_LAUNCH(kModeMip); break;
+    case RV_FIELD_VALUE_ON_ISOSURFACE: RV_BLOCK_LAUNCH(kModeValueOnIsosurface); break;
+    case RV_FIELD_VALUE_ON_ISOSURFACES: RV_BLOCK_LAUNCH(kModeValueOnIsosurfaces); break;
+    default: return fail(RV_ERR_INVALID, "rv_render_block: unknown field mode");
     }
+#undef RV_BLOCK_LAUNCH
     RV_LAUNCH_CHECK();
     return RV_OK;
+}
+
+extern "C" int rv_render_block(const rv_block_params* params, uint32_t field_mode, float iso_opacity,
+                               const float* levels_rel, uint32_t level_count, const float* d_brick,
+                               const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
+                               rv_stream_t stream)
+{
+    return rv_render_block_secondary(params, field_mode, iso_opacity, levels_rel, level_count, d_brick,
+                                     nullptr, 0.f, 1.f, d_tf, tf_n, d_viewport, stream);
 }
 
 extern "C" int rv_clear_viewport_mode(uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
@@ -376,7 +382,8 @@ extern "C" int rv_render_block_argb(const rv_block_params* params, const float* 
         return RV_OK;
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
     hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), tf_n * sizeof(float4),
-                       S(stream), p, d_brick, d_tf, tf_n, 1.f, LevelsArg{}, d_viewport);
+                       S(stream), p, d_brick, Secondary<float>{nullptr, 0.f, 1.f}, d_tf, tf_n, 1.f,
+                       LevelsArg{}, d_viewport);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }

@@ -368,7 +368,22 @@ __device__ __forceinline__ float4 march(const rv_block_params& p, const T* __res
 // ---------------------------------------------------------------- field modes
 // s3vs::FieldMode (FieldMode.hpp:26-51), the same numbering.
 constexpr uint32_t kModeIsosurface = 0, kModeIsosurfaces = 1, kModeMip = 2, kModeArgb = 3,
-                   kModeArgbLight = 4, kModeDomainVoxels = 5;
+                   kModeArgbLight = 4, kModeDomainVoxels = 5, kModeValueOnIsosurface = 6,
+                   kModeValueOnIsosurfaces = 7;
+
+// The secondary field of ValueOnIsosurface(s): the block's brick of it (the
+// primary brick's layout) and the field's range (RenderKernelParam.hpp:103,
+// DenseFieldRenderer_cu.cu:288-335).
+template <class T> struct Secondary {
+    const T* brick;
+    float smin, smax;
+};
+
+// Levels are staged for these modes.
+__host__ __device__ constexpr bool multi_level(uint32_t mode)
+{
+    return mode == kModeIsosurfaces || mode == kModeValueOnIsosurfaces;
+}
 
 // Isosurface levels of the Isosurfaces mode: absolute values, sorted,
 // distinct, at most kMaxLevels (RenderKernelPresentationParam::MaxLevels,
@@ -485,14 +500,30 @@ __device__ inline float4 march_light(const rv_block_params& p, const T* __restri
     return acc;
 }
 
+// Diffuse colour of a surface point of ValueOnIsosurface(s): the table
+// colour of the secondary field's value there, normalised by its range, its
+// alpha scaled by iso_opacity (renderValueOnIsosurface.hpp:44-48).
+template <class T>
+__device__ __forceinline__ float4 value_diffuse(const Secondary<T>& s, int n, const float4* tf, int tf_n,
+                                                float iso_opacity, F3 tc)
+{
+    const float v = fetch3(s.brick, n, tc.x, tc.y, tc.z);
+    float4 d = fetch_tf(tf, tf_n, (v - s.smin) / (s.smax - s.smin));
+    d.w = d.w * iso_opacity;
+    return d;
+}
+
 // renderIsosurface loop (renderIsosurface.hpp:31-61): where consecutive
 // samples v1, v2 straddle the level (p.threshold), the surface point is
 // placed at the second sample's position plus step * (level - v1) / (v2 - v1),
 // as the reference does, and shaded with the table colour of the level, its
-// alpha scaled by iso_opacity.
-template <class T>
+// alpha scaled by iso_opacity.  kValue: ValueOnIsosurface
+// (renderValueOnIsosurface.hpp:32-62), the colour of the secondary field at
+// the point instead.
+template <bool kValue, class T>
 __device__ inline float4 march_iso(const rv_block_params& p, const T* __restrict__ brick,
-                                   const float4* tf, int tf_n, float iso_opacity, RaySeg r)
+                                   const float4* tf, int tf_n, float iso_opacity, RaySeg r,
+                                   Secondary<T> sec)
 {
     const int n = (int)p.nodes;
     auto value_at = [&](float x, float y, float z) {
@@ -504,8 +535,11 @@ __device__ inline float4 march_iso(const rv_block_params& p, const T* __restrict
     r.py = r.py + r.sy;
     r.pz = r.pz + r.sz;
     const float lv = p.threshold;
-    float4 diffuse = fetch_tf(tf, tf_n, (lv - p.fmin) / (p.fmax - p.fmin));
-    diffuse.w = diffuse.w * iso_opacity;
+    float4 diffuse = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!kValue) {
+        diffuse = fetch_tf(tf, tf_n, (lv - p.fmin) / (p.fmax - p.fmin));
+        diffuse.w = diffuse.w * iso_opacity;
+    }
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int i = 1; i < r.sample_count; ++i) {
         const float v2 = value_at(r.px, r.py, r.pz);
@@ -514,7 +548,8 @@ __device__ inline float4 march_iso(const rv_block_params& p, const T* __restrict
             const F3 x = f3(r.px, r.py, r.pz) + f3(r.sx, r.sy, r.sz) * t;
             const F3 tc = f3((x.x - p.tex_origin[0]) * p.tcf, (x.y - p.tex_origin[1]) * p.tcf,
                              (x.z - p.tex_origin[2]) * p.tcf);
-            blend_behind(acc, frag_color(p, brick, diffuse, x, tc));
+            const float4 dif = kValue ? value_diffuse(sec, n, tf, tf_n, iso_opacity, tc) : diffuse;
+            blend_behind(acc, frag_color(p, brick, dif, x, tc));
             if (acc.w > p.alpha_cutoff)
                 break;
         }
@@ -582,10 +617,12 @@ __device__ __forceinline__ int find_interval(float v, const float* lv, uint32_t 
 // between consecutive samples is shaded at its point (placed as in
 // march_iso) with the table colour of the level, in the order the ray
 // crosses them.  lv: n >= 1 sorted distinct levels (the caller handles n = 0).
-template <class T>
+// kValue: ValueOnIsosurfaces (renderValueOnIsosurfaces.hpp:33-109), each
+// point coloured by the secondary field there.
+template <bool kValue, class T>
 __device__ inline float4 march_isos(const rv_block_params& p, const T* __restrict__ brick,
                                     const float4* tf, int tf_n, float iso_opacity, const float* lv,
-                                    uint32_t nlv, RaySeg r)
+                                    uint32_t nlv, RaySeg r, Secondary<T> sec)
 {
     const int n = (int)p.nodes;
     auto value_at = [&](float x, float y, float z) {
@@ -634,12 +671,17 @@ __device__ inline float4 march_isos(const rv_block_params& p, const T* __restric
             const float level = lv[il];
             if (!((v1 - level) * (v2 - level) < 0.f))
                 break;
-            float4 diffuse = fetch_tf(tf, tf_n, (level - p.fmin) / (p.fmax - p.fmin));
-            diffuse.w = diffuse.w * iso_opacity;
             const float t = (level - v1) / (v2 - v1);
             const F3 x = f3(px, py, pz) + f3(r.sx, r.sy, r.sz) * t;
             const F3 tc = f3((x.x - p.tex_origin[0]) * p.tcf, (x.y - p.tex_origin[1]) * p.tcf,
                              (x.z - p.tex_origin[2]) * p.tcf);
+            float4 diffuse;
+            if (kValue) {
+                diffuse = value_diffuse(sec, n, tf, tf_n, iso_opacity, tc);
+            } else {
+                diffuse = fetch_tf(tf, tf_n, (level - p.fmin) / (p.fmax - p.fmin));
+                diffuse.w = diffuse.w * iso_opacity;
+            }
             blend_behind(acc, frag_color(p, brick, diffuse, x, tc));
             iint += d;
             il += d;
@@ -677,18 +719,22 @@ __device__ inline float4 march_domain(const rv_block_params& p, const T* __restr
 }
 
 // One block segment of a pixel in field mode kMode, as a colour to blend
-// (Argb, ArgbLight, Isosurface, Isosurfaces with nlv >= 1, DomainVoxels).
+// (Argb, ArgbLight, Isosurface(s) and ValueOnIsosurface(s) with nlv >= 1
+// levels where they take levels, DomainVoxels).  sec: the secondary field
+// of the ValueOnIsosurface(s) modes.
 template <uint32_t kMode, class T>
 __device__ __forceinline__ float4 segment(const rv_block_params& p, const T* __restrict__ brick,
                                           const float4* tf, int tf_n, float iso_opacity, RaySeg r,
-                                          const float* lv = nullptr, uint32_t nlv = 0)
+                                          const float* lv = nullptr, uint32_t nlv = 0,
+                                          Secondary<T> sec = Secondary<T>{nullptr, 0.f, 1.f})
 {
     if (kMode == kModeArgbLight)
         return march_light(p, brick, tf, tf_n, r);
-    if (kMode == kModeIsosurface)
-        return march_iso(p, brick, tf, tf_n, iso_opacity, r);
-    if (kMode == kModeIsosurfaces)
-        return march_isos(p, brick, tf, tf_n, iso_opacity, lv, nlv, r);
+    if (kMode == kModeIsosurface || kMode == kModeValueOnIsosurface)
+        return march_iso<kMode == kModeValueOnIsosurface>(p, brick, tf, tf_n, iso_opacity, r, sec);
+    if (kMode == kModeIsosurfaces || kMode == kModeValueOnIsosurfaces)
+        return march_isos<kMode == kModeValueOnIsosurfaces>(p, brick, tf, tf_n, iso_opacity, lv, nlv,
+                                                            r, sec);
     if (kMode == kModeDomainVoxels)
         return march_domain(p, brick, tf, tf_n, r);
     return march(p, brick, tf, tf_n, r);

@@ -409,8 +409,10 @@ extern "C" int rv_dataset_read_field(const rv_dataset* ds, const char* field_pat
     IO_CATCH
 }
 
-extern "C" int rv_dataset_load_worker(const rv_dataset* ds, const char* field_path, rv_worker* w,
-                                      uint32_t level_min, uint32_t level_max, uint64_t* loaded)
+namespace {
+
+int load_worker(const rv_dataset* ds, const char* field_path, rv_worker* w, uint32_t level_min,
+                uint32_t level_max, uint64_t* loaded, bool secondary)
 {
     IO_TRY
     IO_REQUIRE(ds && field_path && w, "rv_dataset_load_worker: null pointer");
@@ -439,7 +441,9 @@ extern "C" int rv_dataset_load_worker(const rv_dataset* ds, const char* field_pa
         rc = rv_dataset_read_field(ds, field_path, i, range, values.data(), n);
         if (rc != RV_OK)
             return rc;
-        rc = rv_worker_add_subtree(w, s.level, s.location, s.depth, n2i.data(), n, values.data());
+        rc = secondary ? rv_worker_add_subtree_secondary(w, s.level, s.location, values.data())
+                       : rv_worker_add_subtree(w, s.level, s.location, s.depth, n2i.data(), n,
+                                               values.data());
         if (rc != RV_OK)
             return rc;
         ++count;
@@ -448,6 +452,21 @@ extern "C" int rv_dataset_load_worker(const rv_dataset* ds, const char* field_pa
         *loaded = count;
     return RV_OK;
     IO_CATCH
+}
+
+} // namespace
+
+extern "C" int rv_dataset_load_worker(const rv_dataset* ds, const char* field_path, rv_worker* w,
+                                      uint32_t level_min, uint32_t level_max, uint64_t* loaded)
+{
+    return load_worker(ds, field_path, w, level_min, level_max, loaded, false);
+}
+
+extern "C" int rv_dataset_load_worker_secondary(const rv_dataset* ds, const char* field_path,
+                                                rv_worker* w, uint32_t level_min, uint32_t level_max,
+                                                uint64_t* loaded)
+{
+    return load_worker(ds, field_path, w, level_min, level_max, loaded, true);
 }
 
 // BlockTreeFieldService::readFieldInfoFile (BlockTreeFieldService.hpp:177-228):

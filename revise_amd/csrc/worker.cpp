@@ -20,7 +20,15 @@ namespace {
 // the isosurface level for Isosurface, else the threshold (both relative).
 float presentationLevel(const RenderSharedState& st)
 {
-    return st.fieldMode == RV_FIELD_ISOSURFACE ? st.isosurfaceLevel : st.threshold;
+    return st.fieldMode == RV_FIELD_ISOSURFACE || st.fieldMode == RV_FIELD_VALUE_ON_ISOSURFACE
+               ? st.isosurfaceLevel
+               : st.threshold;
+}
+
+// The modes that colour by the secondary field (DenseFieldRenderer_cu.cu:290-292).
+bool hasSecondaryField(const RenderSharedState& st)
+{
+    return st.fieldMode == RV_FIELD_VALUE_ON_ISOSURFACE || st.fieldMode == RV_FIELD_VALUE_ON_ISOSURFACES;
 }
 } // namespace
 
@@ -45,11 +53,14 @@ struct FieldServiceHip::Subtree {
     unsigned depth = 0;
     uint64_t n = 0;
     float* sparse = nullptr;
+    float* sparse2 = nullptr; // secondary field values, or null
     void* plan = nullptr; // gather plan, encoded from the n2i once per subtree
     ~Subtree()
     {
         if (sparse)
             (void)hipFree(sparse);
+        if (sparse2)
+            (void)hipFree(sparse2);
         if (plan)
             (void)rv_interp_plan_free(plan);
     }
@@ -81,6 +92,26 @@ void FieldServiceHip::setFieldRange(float fmin, float fmax)
     m_fmax = fmax;
 }
 
+void FieldServiceHip::addSecondary(unsigned level, const std::array<uint32_t, 3>& i, const float* sparse)
+{
+    auto it = m_subtrees.find({level, i[0], i[1], i[2]});
+    if (it == m_subtrees.end())
+        throw HipError(RV_ERR_INVALID, "addSecondary: no such subtree (register it first)");
+    Subtree& st = *it->second;
+    if (st.n) {
+        if (!st.sparse2)
+            hipCheck(hipMalloc((void**)&st.sparse2, 4 * st.n), "hipMalloc");
+        hipCheck(hipMemcpy(st.sparse2, sparse, 4 * st.n, hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    ++m_version;
+}
+
+void FieldServiceHip::setSecondaryRange(float fmin, float fmax)
+{
+    m_smin = fmin;
+    m_smax = fmax;
+}
+
 bool FieldServiceHip::hasSubtree(unsigned level, const std::array<uint32_t, 3>& i) const
 {
     return m_subtrees.count({level, i[0], i[1], i[2]}) != 0;
@@ -95,15 +126,17 @@ unsigned FieldServiceHip::subtreeDepth(unsigned level, const std::array<uint32_t
 }
 
 rv_brick_job FieldServiceHip::job(unsigned level, const std::array<uint32_t, 3>& i, float* d_dense,
-                                  rv_stream_t stream)
+                                  rv_stream_t stream, bool secondary)
 {
     auto it = m_subtrees.find({level, i[0], i[1], i[2]});
     if (it == m_subtrees.end())
         throw std::range_error("Block does not exist");
     const Subtree& st = *it->second;
     (void)stream;
+    if (secondary && st.n && !st.sparse2)
+        throw HipError(RV_ERR_INVALID, "renderScenePart: a block has no secondary field values");
     rv_brick_job j{};
-    j.sparse = st.sparse;
+    j.sparse = secondary ? st.sparse2 : st.sparse;
     j.plan = st.plan;
     j.dense = d_dense;
     j.depth = st.depth;
@@ -118,6 +151,18 @@ void FieldServiceHip::interpolate(unsigned level, const std::array<uint32_t, 3>&
         throw std::range_error("Block does not exist");
     const Subtree& st = *it->second;
     throwIfFailed(rv_interp_brick_planned(st.sparse, st.plan, st.depth, d_dense, stream));
+}
+
+void FieldServiceHip::interpolateSecondary(unsigned level, const std::array<uint32_t, 3>& i,
+                                           float* d_dense, rv_stream_t stream)
+{
+    auto it = m_subtrees.find({level, i[0], i[1], i[2]});
+    if (it == m_subtrees.end())
+        throw std::range_error("Block does not exist");
+    const Subtree& st = *it->second;
+    if (st.n && !st.sparse2)
+        throw HipError(RV_ERR_INVALID, "renderDenseField: the block has no secondary field values");
+    throwIfFailed(rv_interp_brick_planned(st.sparse2, st.plan, st.depth, d_dense, stream));
 }
 
 // ---------------------------------------------------------------- renderer
@@ -135,6 +180,8 @@ DenseFieldRendererHip::~DenseFieldRendererHip()
         (void)hipFree(m_tf);
     if (m_brick)
         (void)hipFree(m_brick);
+    if (m_brick2)
+        (void)hipFree(m_brick2);
     if (m_stream)
         (void)hipStreamDestroy(hs(m_stream));
 }
@@ -145,6 +192,7 @@ void DenseFieldRendererHip::setRenderSharedState(const RenderSharedState* s)
         m_state = s;
         m_tfDirty = true;
         m_lastBlock = {~0u, 0, 0, 0};
+        m_lastBlock2 = {~0u, 0, 0, 0};
     }
 }
 
@@ -176,13 +224,23 @@ void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
             (void)hipFree(m_brick);
         m_brick = nullptr;
         hipCheck(hipMalloc((void**)&m_brick, 4 * nodes), "hipMalloc");
+        if (m_brick2)
+            (void)hipFree(m_brick2);
+        m_brick2 = nullptr;
+        hipCheck(hipMalloc((void**)&m_brick2, 4 * nodes), "hipMalloc");
         m_brickNodes = nodes;
         m_lastBlock = {~0u, 0, 0, 0};
+        m_lastBlock2 = {~0u, 0, 0, 0};
     }
     const std::array<uint32_t, 4> key{level, index[0], index[1], index[2]};
     if (key != m_lastBlock) {
         fs.interpolate(level, index, m_brick, m_stream);
         m_lastBlock = key;
+    }
+    const bool second = hasSecondaryField(st);
+    if (second && key != m_lastBlock2) {
+        fs.interpolateSecondary(level, index, m_brick2, m_stream);
+        m_lastBlock2 = key;
     }
     ensureColorTable();
     if (isCancelled())
@@ -196,9 +254,12 @@ void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
                                         st.alphaCutoff, &p, &live));
     if (!live)
         return;
-    throwIfFailed(rv_render_block(&p, st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
-                                  (uint32_t)st.isosurfaceLevels.size(), m_brick, m_tf, m_tfN, vp,
-                                  m_stream));
+    const auto srange = fs.secondaryRange();
+    throwIfFailed(rv_render_block_secondary(&p, st.fieldMode, st.isosurfaceOpacity,
+                                            st.isosurfaceLevels.data(),
+                                            (uint32_t)st.isosurfaceLevels.size(), m_brick,
+                                            second ? m_brick2 : nullptr, srange[0], srange[1], m_tf,
+                                            m_tfN, vp, m_stream));
 }
 
 // makeColorTransferTexture (DenseFieldRenderer_cu.cu:573-589): the table is
@@ -230,9 +291,10 @@ void DenseFieldRendererHip::ensureColorTable()
 // version): one brick per block, the batched interpolation handle, the block
 // table workspace.
 struct VsWorkerHip::ShareCache {
-    std::array<uint32_t, 7> key{};
+    std::array<uint32_t, 8> key{};
     uint64_t version = ~0ull;
     std::map<std::array<uint32_t, 3>, float*> bricks;
+    std::map<std::array<uint32_t, 3>, float*> bricks2; // secondary field (ValueOnIsosurface(s))
     rv_interp_batch* batch = nullptr;
     void* workspace = nullptr;
     uint64_t workspace_bytes = 0;
@@ -241,6 +303,8 @@ struct VsWorkerHip::ShareCache {
         if (batch)
             rv_interp_batch_destroy(batch);
         for (auto& kv : bricks)
+            (void)hipFree(kv.second);
+        for (auto& kv : bricks2)
             (void)hipFree(kv.second);
         if (workspace)
             (void)hipFree(workspace);
@@ -373,8 +437,9 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
     if (!m_share)
         m_share = std::make_unique<ShareCache>();
     ShareCache& sc = *m_share;
-    const std::array<uint32_t, 7> key{sub.level,     sub.boxMin[0], sub.boxMin[1], sub.boxMin[2],
-                                      sub.boxMax[0], sub.boxMax[1], sub.boxMax[2]};
+    const bool second = hasSecondaryField(st);
+    const std::array<uint32_t, 8> key{sub.level,     sub.boxMin[0], sub.boxMin[1], sub.boxMin[2],
+                                      sub.boxMax[0], sub.boxMax[1], sub.boxMax[2], second ? 1u : 0u};
     if (key != sc.key || sc.version != fs.version() || !sc.batch) {
         if (sc.batch)
             rv_interp_batch_destroy(sc.batch);
@@ -388,6 +453,12 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
             if (!brick)
                 hipCheck(hipMalloc((void**)&brick, 4 * nodes), "hipMalloc");
             jobs.push_back(fs.job(sub.level, idx, brick, s));
+            if (second) {
+                float*& brick2 = sc.bricks2[idx];
+                if (!brick2)
+                    hipCheck(hipMalloc((void**)&brick2, 4 * nodes), "hipMalloc");
+                jobs.push_back(fs.job(sub.level, idx, brick2, s, true));
+            }
         }
         throwIfFailed(rv_interp_batch_create(jobs.data(), (uint32_t)jobs.size(), &sc.batch));
         sc.key = key;
@@ -402,7 +473,7 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
     m_renderer.ensureColorTable();
     const auto range = fs.fieldRange();
     std::vector<rv_block_params> params;
-    std::vector<const float*> bricks;
+    std::vector<const float*> bricks, bricks2;
     params.reserve(count);
     bricks.reserve(count);
     for (uint64_t i = 0; i < count; ++i) {
@@ -418,6 +489,8 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
             continue;
         params.push_back(p);
         bricks.push_back(sc.bricks[idx]);
+        if (second)
+            bricks2.push_back(sc.bricks2[idx]);
     }
     const uint64_t need = rv_share_workspace_bytes((uint32_t)params.size());
     if (need > sc.workspace_bytes) {
@@ -431,11 +504,13 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         m_cancelled = true;
         return;
     }
-    throwIfFailed(rv_render_share(params.data(), bricks.data(), (uint32_t)params.size(), st.fieldMode,
-                                  st.isosurfaceOpacity, st.isosurfaceLevels.data(),
-                                  (uint32_t)st.isosurfaceLevels.size(), m_renderer.colorTable(),
-                                  m_renderer.colorTableSize(), m_viewport, st.viewportSize[0],
-                                  st.viewportSize[1], sc.workspace, s));
+    const auto srange = fs.secondaryRange();
+    throwIfFailed(rv_render_share_secondary(
+        params.data(), bricks.data(), second ? bricks2.data() : nullptr, (uint32_t)params.size(),
+        st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
+        (uint32_t)st.isosurfaceLevels.size(), srange[0], srange[1], m_renderer.colorTable(),
+        m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1],
+        sc.workspace, s));
 }
 
 const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const
@@ -635,6 +710,19 @@ extern "C" int rv_worker_add_subtree(rv_worker* w, uint32_t level, const uint32_
     W_CATCH
 }
 
+extern "C" int rv_worker_add_subtree_secondary(rv_worker* w, uint32_t level, const uint32_t index[3],
+                                               const float* sparse)
+{
+    W_TRY
+    if (!w || !index || !sparse) {
+        rv::set_error("rv_worker_add_subtree_secondary: bad arguments");
+        return RV_ERR_INVALID;
+    }
+    w->field.addSecondary(level, {index[0], index[1], index[2]}, sparse);
+    return RV_OK;
+    W_CATCH
+}
+
 extern "C" int rv_worker_set_state(rv_worker* w, const rv_render_state* s)
 {
     W_TRY
@@ -659,7 +747,13 @@ extern "C" int rv_worker_set_state(rv_worker* w, const rv_render_state* s)
         return RV_ERR_INVALID;
     }
     st.isosurfaceLevels.assign(s->iso_levels_rel, s->iso_levels_rel + s->iso_level_count);
+    if ((s->field_mode == RV_FIELD_VALUE_ON_ISOSURFACE || s->field_mode == RV_FIELD_VALUE_ON_ISOSURFACES) &&
+        !(s->smax > s->smin)) {
+        rv::set_error("rv_worker_set_state: the secondary field range must be non-empty");
+        return RV_ERR_INVALID;
+    }
     w->field.setFieldRange(s->fmin, s->fmax);
+    w->field.setSecondaryRange(s->smin, s->smax);
     w->worker.setRenderSharedState(&st);
     w->worker.updateState(1u << 7);
     return RV_OK;

@@ -15,7 +15,7 @@ from tests.test_gpu_worker import BG, CUT, FOV, Q, THR, _oracle_share
 pytestmark = pytest.mark.gpu
 
 
-def _dataset(tmp_path, seed, level=2, max_level=5, msd=3):
+def _dataset(tmp_path, seed, level=2, max_level=5, msd=3, second=None):
     rng = np.random.default_rng(seed)
     m = 1 << level
     blocks = []
@@ -38,6 +38,13 @@ def _dataset(tmp_path, seed, level=2, max_level=5, msd=3):
         f.write(meta)
     with open(field_p, "wb") as f:
         f.write(fmt.field_bytes(subs, size, lambda s: vals[s["values_pos"]]))
+    if second is not None:
+        f2 = datasets.Field(second, n_gauss=4, noise_octaves=2)
+        vals2 = {s["values_pos"]: datasets.sample(f2, s["level"], s["location"], s["depth"], s["n2i"])
+                 for s in subs}
+        with open(api.dataset_file_names(base, "g")[1], "wb") as f:
+            f.write(fmt.field_bytes(subs, size, lambda s: vals2[s["values_pos"]]))
+        sec = {s["location"]: vals2[s["values_pos"]] for s in subs if s["level"] == level}
     lv_blocks = {s["location"]: datasets.Block(level, s["location"], s["depth"], s["n2i"],
                                                 vals[s["values_pos"]])
                  for s in subs if s["level"] == level}
@@ -45,6 +52,8 @@ def _dataset(tmp_path, seed, level=2, max_level=5, msd=3):
     fmin, fmax = datasets.field_range(lv_blocks.values())
     with open(info_p, "w") as f:
         f.write(fmt.fields_info_text(1, [("f", fmin, fmax, False)]))
+    if second is not None:
+        return meta_p, field_p, info_p, lv_blocks, sec
     return meta_p, field_p, info_p, lv_blocks
 
 
@@ -69,5 +78,44 @@ def test_worker_from_files_bit_exact(orc, tmp_path, fused):
         erect, ebits = _oracle_share(orc, blocks, level, bmin, bmax, M, W, H, fmin, fmax, tf)
         assert rect == erect
         assert np.array_equal(bits, ebits)
+    w.close()
+    ds.close()
+
+
+def test_worker_secondary_field_from_files(orc, tmp_path):
+    """ValueOnIsosurfaces with the secondary field loaded from its own field
+    file (rv_dataset_load_worker_secondary)."""
+    level, W, H = 2, 180, 140
+    meta_p, field_p, _, blocks, sec = _dataset(tmp_path, 8, second=9)
+    fmin, fmax = datasets.field_range(blocks.values())
+    lo = min(float(v.min()) for v in sec.values())
+    hi = max(float(v.max()) for v in sec.values())
+    ds = api.Dataset(meta_p)
+    w = api.Worker(0)
+    assert ds.load_worker(w, field_p, level, level) == len(blocks)
+    assert ds.load_worker(w, api.dataset_file_names(meta_p[:-len(".s3dmm-meta")], "g")[1], level, level,
+                          secondary=True) == len(blocks)
+    tf = datasets.seeded_tf(8, 256)
+    M = datasets.orbit_camera(200.0, elevation_deg=-15.0)
+    levels = [0.35, 0.6]
+    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, field_mode=api.FIELD_VALUE_ON_ISOSURFACES,
+                iso_opacity=0.7, iso_levels=levels, secondary_range=(lo, hi))
+    say("worker from files, secondary field")
+    rect, bits = w.render_scene_part(level, (0, 0, 0), (4, 4, 4))
+    vp = np.zeros((H, W), np.uint32)
+    eye = api.eye_from_view(M)
+    cmin, cmax = api.index_box_coords(level, (0, 0, 0), (4, 4, 4))
+    for idx in orc.b2f_macro((0, 0, 0), (4, 4, 4), cmin, cmax, eye):
+        key = tuple(int(v) for v in idx)
+        b = blocks[key]
+        live, po = orc.block_params(M, W, H, FOV, level, b.index, b.depth, fmin, fmax, THR, Q, CUT)
+        if live:
+            orc.render_block_mode(po, api.FIELD_VALUE_ON_ISOSURFACES, 0.7, orc.interpolate(b.sparse, b.n2i, b.depth),
+                                  tf, vp.reshape(-1), levels=levels,
+                                  brick2=orc.interpolate(sec[key], b.n2i, b.depth), srange=(lo, hi))
+    r = orc.bound_rect(vp)
+    assert r is not None and rect == r
+    x, y, ww, hh = r
+    assert np.array_equal(bits, vp[y:y + hh, x:x + ww])
     w.close()
     ds.close()

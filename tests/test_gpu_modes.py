@@ -214,3 +214,73 @@ def test_isosurfaces_without_levels(orc):
     got = d_vp.cpu().numpy().view(np.uint32)
     assert np.array_equal(got, exp)
     assert (got == 0).sum() > (init == 0).sum()
+
+
+VALUE_MODES = [(api.FIELD_VALUE_ON_ISOSURFACE, 0.45, 0.7, None),
+               (api.FIELD_VALUE_ON_ISOSURFACES, 0.5, 0.6, LEVELS)]
+
+
+def _secondary(blocks, seed):
+    """A second field over the same blocks (same n2i), with its own no-field ball."""
+    field = datasets.Field(seed, n_gauss=4, noise_octaves=2)
+    out = []
+    for b in blocks:
+        m = 2 ** b.level
+        c = np.array([b.index[0] + 0.7, b.index[1] + 0.4, b.index[2] + 0.5]) / m
+        occ = lambda u, c=c: ((u - c) ** 2).sum(-1) > (0.2 / m) ** 2
+        out.append(datasets.Block(b.level, b.index, b.depth, b.n2i,
+                                  datasets.sample(field, b.level, b.index, b.depth, b.n2i, occ)))
+    return out
+
+
+@pytest.mark.parametrize("mode,thr,iso,levels", VALUE_MODES)
+def test_value_modes_block_and_share_bit_exact(orc, mode, thr, iso, levels):
+    """ValueOnIsosurface(s): per block (rv_render_block_secondary) over a
+    viewport holding earlier pixels, and a whole share
+    (rv_render_share_secondary), against the oracle, bit for bit."""
+    import torch
+    W, H = 160, 120
+    blocks = _scene(2, 3, seed=24)
+    sec = _secondary(blocks, 77)
+    bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    bricks2 = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in sec]
+    assert any(np.isnan(br).any() for br in bricks2)
+    fmin, fmax = datasets.field_range(blocks)
+    smin, smax = datasets.field_range(sec)
+    tf = datasets.seeded_tf(24, 256)
+    d_tf = to_dev(tf)
+    M = datasets.orbit_camera(130.0, elevation_deg=20.0)
+    init = _init_viewport(mode, W, H, 6)
+    d_vp = to_dev(init.view(np.int32))
+    exp = init.copy()
+    order = api.mixed_back_to_front(np.array([(2,) + b.index for b in blocks], np.uint32), api.eye_from_view(M))
+    d_b = [to_dev(br) for br in bricks]
+    d_b2 = [to_dev(br) for br in bricks2]
+    items, items2 = [], []
+    for i in order:
+        i = int(i)
+        b = blocks[i]
+        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        assert live == lo
+        if not live:
+            continue
+        say("value mode", mode, "block", b.index)
+        api.render_block(p, mode, iso, d_b[i], d_tf, d_vp, levels=levels, d_brick2=d_b2[i], srange=(smin, smax))
+        orc.render_block_mode(po, mode, iso, bricks[i], tf, exp.reshape(-1), levels=levels, brick2=bricks2[i],
+                              srange=(smin, smax))
+        items.append((p, d_b[i]))
+        items2.append(d_b2[i])
+    got = d_vp.cpu().numpy().view(np.uint32)
+    assert not np.array_equal(exp, init)
+    assert np.array_equal(got, exp)
+    # the share, from a cleared viewport, equals the per-block calls from one
+    share = api.ShareRenderer(items, d_tf, W, H, bricks2=items2)
+    vp = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    share.render_mode(vp, mode, iso, levels=levels, srange=(smin, smax))
+    vp_seq = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    for (p, br), br2 in zip(items, items2):
+        api.render_block(p, mode, iso, br, d_tf, vp_seq, levels=levels, d_brick2=br2, srange=(smin, smax))
+    torch.cuda.synchronize()
+    assert (vp.cpu().numpy().view(np.uint32) >> 24).any()
+    assert torch.equal(vp, vp_seq)

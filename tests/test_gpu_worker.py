@@ -203,3 +203,48 @@ def test_worker_field_modes_bit_exact(orc, mode, fused):
     assert np.array_equal(got, frame)
     assert (got != orc.background_argb(*BG)).any()
     w.close()
+
+
+@pytest.mark.parametrize("mode,fused", [(api.FIELD_VALUE_ON_ISOSURFACE, True),
+                                        (api.FIELD_VALUE_ON_ISOSURFACE, False),
+                                        (api.FIELD_VALUE_ON_ISOSURFACES, True),
+                                        (api.FIELD_VALUE_ON_ISOSURFACES, False)])
+def test_worker_value_modes_bit_exact(orc, mode, fused):
+    """renderScenePart in ValueOnIsosurface(s): each block's secondary brick
+    interpolated beside the primary one (per block, or both in the batched
+    interpolation), the isosurface(s) of the primary field coloured by the
+    secondary's value."""
+    W, H = 170, 120
+    w, blocks, fmin, fmax, tf, M = _make(2, 3, 12, W, H, 110.0)
+    field2 = datasets.Field(99, n_gauss=4, noise_octaves=2)
+    sec = {}
+    for key, b in blocks.items():
+        sec[key] = datasets.sample(field2, 2, b.index, b.depth, b.n2i)
+        w.add_subtree_secondary(2, b.index, sec[key])
+    lo = min(float(v.min()) for v in sec.values())
+    hi = max(float(v.max()) for v in sec.values())
+    lvl, iso = 0.47, 0.75
+    levels = [0.3, 0.5, 0.68] if mode == api.FIELD_VALUE_ON_ISOSURFACES else None
+    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, field_mode=mode, iso_level_rel=lvl,
+                iso_opacity=iso, iso_levels=levels, secondary_range=(lo, hi))
+    w.set_fused(fused)
+    say("worker value mode", mode, "fused", fused)
+    for bmin, bmax in [((0, 0, 0), (4, 4, 4)), ((1, 0, 1), (3, 4, 4))]:
+        rect, bits = w.render_scene_part(2, bmin, bmax)
+        vp = np.zeros((H, W), np.uint32)
+        eye = api.eye_from_view(M)
+        cmin, cmax = api.index_box_coords(2, bmin, bmax)
+        thr = lvl if mode == api.FIELD_VALUE_ON_ISOSURFACE else THR
+        for idx in orc.b2f_macro(bmin, bmax, cmin, cmax, eye):
+            key = tuple(int(v) for v in idx)
+            b = blocks[key]
+            live, po = orc.block_params(M, W, H, FOV, 2, b.index, b.depth, fmin, fmax, thr, Q, CUT)
+            if live:
+                orc.render_block_mode(po, mode, iso, orc.interpolate(b.sparse, b.n2i, b.depth), tf,
+                                      vp.reshape(-1), levels=levels,
+                                      brick2=orc.interpolate(sec[key], b.n2i, b.depth), srange=(lo, hi))
+        r = orc.bound_rect(vp)
+        assert r is not None and rect == r
+        x, y, ww, hh = r
+        assert np.array_equal(bits, vp[y:y + hh, x:x + ww])
+    w.close()

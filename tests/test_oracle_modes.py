@@ -276,3 +276,91 @@ def test_isosurfaces_two_levels_closed_form(orc):
     vp0 = np.full(W * H, 0x12345678, np.uint32)
     orc.render_block_mode(p, 1, 0.5, brick, tf, vp0, levels=[])
     assert vp0[(H - 1 - vy) * W + vx] == 0
+
+
+VALUE_ISO, VALUE_ISOS = 6, 7
+
+
+def _tf64(tf, u):
+    """The colour table lookup in float64 (linear filter, clamped ends)."""
+    N = tf.shape[0]
+    x = min(max(u * N - 0.5, -1.0), float(N))
+    i = math.floor(x)
+    f = x - i
+    a, b = tf[min(max(i, 0), N - 1)], tf[min(max(i + 1, 0), N - 1)]
+    return (1 - f) * a.astype(np.float64) + f * b.astype(np.float64)
+
+
+@pytest.mark.parametrize("mode,levels", [(VALUE_ISO, None), (VALUE_ISOS, [0.31, 0.62])])
+def test_value_on_isosurface_closed_form(orc, mode, levels):
+    """A z-linear primary field and an x-linear secondary one: each surface
+    point of the primary takes the table colour of the secondary's value at
+    that point (float64: the point one step past the interpolated crossing,
+    the linear secondary there, the filtered table), lit with the primary's
+    normal.  The same colour taken at the sample before the point would be
+    at least 3 LSB off, so the test sees where the secondary is read."""
+    W, H, depth = 64, 48, 2
+    p = _params(orc, W, H, depth, 0.47, 2.0)
+    vx, vy = p.view_x + p.view_w // 2, p.view_y + p.view_h // 2
+    g = _ray64(p, vx, vy)
+    n = g["n"]
+    slope = 1.0 / (n - 1)
+    brick = _linear_brick(n, 2, 0.0, slope)          # primary: (tz - 0.5) / (n - 1)
+    brick2 = _linear_brick(n, 0, 0.0, slope)         # secondary: (tx - 0.5) / (n - 1)
+    smin, smax = 0.1, 0.9
+    k = np.arange(16, dtype=np.float32) / 15
+    tf = np.stack([k, 1 - k, np.full(16, 0.5, np.float32), np.full(16, 0.8, np.float32)], 1)
+    vp = np.zeros(W * H, np.uint32)
+    orc.render_block_mode(p, mode, 0.7, brick, tf, vp, levels=levels, brick2=brick2, srange=(smin, smax))
+    got = _unpack(vp[(H - 1 - vy) * W + vx])
+    count = int(p.quality * g["ratio"])
+    step = g["r"] * ((g["tx"] - g["te"]) / count)
+    pos = [g["eye"] + g["r"] * g["te"] + step * i for i in range(count)]
+    tex = lambda q, a: min(max((q[a] - g["org"][a]) * g["tcf"] - 0.5, 0.0), n - 1.0) * slope
+    lvs = [0.47] if levels is None else sorted(set(levels))
+
+    def expect(at_sample):
+        acc = np.zeros(4)
+        hits = 0
+        for i in range(1, count):
+            v1, v2 = tex(pos[i - 1], 2), tex(pos[i], 2)
+            for lv in (lvs if v2 >= v1 else lvs[::-1]):
+                if (v1 - lv) * (v2 - lv) < 0:
+                    t = (lv - v1) / (v2 - v1)
+                    assert 1e-3 < t < 1 - 1e-3
+                    surf = pos[i] + step * t
+                    where = pos[i] if at_sample else surf
+                    dif = _tf64(tf, (tex(where, 0) - smin) / (smax - smin))
+                    rgb = _light64(p, dif, surf, np.array([0.0, 0.0, -slope]))
+                    c = np.concatenate([rgb, [dif[3] * 0.7]])
+                    da = c[3] * (1 - acc[3])
+                    a = acc[3] + da
+                    acc[:3] = (acc[:3] * acc[3] + c[:3] * da) / a
+                    acc[3] = a
+                    hits += 1
+        assert hits == len(lvs)
+        return _argb8(acc)
+    exp = expect(False)
+    assert np.abs(got - exp).max() <= 1, (got, exp)
+    assert np.abs(expect(True) - exp).max() >= 3
+
+
+@pytest.mark.parametrize("mode,prim,levels", [(VALUE_ISO, ISO, None), (VALUE_ISOS, 1, [0.2, 0.55, 0.8])])
+def test_value_on_isosurface_constant_table_equals_isosurface(orc, mode, prim, levels):
+    """With one colour in the table the secondary field cannot change a
+    pixel: ValueOnIsosurface(s) equals Isosurface(s), bit for bit, over a
+    whole block of a smooth random field."""
+    W, H, depth = 96, 72, 4
+    p = _params(orc, W, H, depth, 0.5, 2.0, angle=40.0, elev=25.0)
+    n = 2 ** depth + 1
+    rng = np.random.default_rng(3)
+    z, y, x = np.meshgrid(*(np.linspace(0, 1, n),) * 3, indexing="ij")
+    brick = (0.5 + 0.3 * np.sin(3 * x + 1) * np.cos(2 * y) + 0.2 * z).astype(np.float32).reshape(-1)
+    brick2 = rng.random(n ** 3).astype(np.float32)
+    tf = np.tile(np.array([[0.3, 0.6, 0.9, 0.4]], np.float32), (32, 1))
+    a = np.zeros(W * H, np.uint32)
+    b = np.zeros(W * H, np.uint32)
+    orc.render_block_mode(p, prim, 0.8, brick, tf, a, levels=levels)
+    orc.render_block_mode(p, mode, 0.8, brick, tf, b, levels=levels, brick2=brick2, srange=(0.0, 1.0))
+    assert np.count_nonzero(a) > 100
+    assert np.array_equal(a, b)
