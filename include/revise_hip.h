@@ -421,11 +421,20 @@ int rv_frame_graph_launch(rv_frame_plan* plan, rv_stream_t stream);
 /* Sticky error flags of the last frames (1: a brick wait gave up); synchronous. */
 int rv_frame_status(rv_frame_plan* plan, uint32_t* error_flags);
 int rv_frame_get_tile(const rv_frame_plan* plan, uint32_t* tile_w, uint32_t* tile_h);
-/* Pick the bins' wave footprint (8x8, 16x4, 4x16, 32x2) and the persistent
+/* How a wave walks the blocks of its screen bin.  BINS: the wave marches
+ * each block of the bin's list with the lanes whose pixels the block
+ * covers.  LANES (Argb mode): each lane walks its own pixel's blocks, so a
+ * wave marches different blocks at once -- faster when blocks are a few
+ * pixels wide.  Both give identical images (and adjoint records). */
+enum { RV_FRAME_SCHEDULE_BINS = 0, RV_FRAME_SCHEDULE_LANES = 1 };
+int rv_frame_set_schedule(rv_frame_plan* plan, uint32_t schedule);
+int rv_frame_get_schedule(const rv_frame_plan* plan, uint32_t* schedule);
+/* Pick the bins' wave footprint (8x8, 16x4, 4x16, 32x2), the persistent
  * grid (resident workgroups per CU) of the mode's kernel (interpolate or
- * resident bricks) by timing candidates on camera 0; the choice is cached
- * (one JSON object per line) in cache_path when it is not NULL.  The
- * footprint applies to both modes, the grid to the tuned mode. */
+ * resident bricks) and, in Argb mode, the schedule, by timing candidates
+ * on camera 0; the choice is cached (one JSON object per line) in
+ * cache_path when it is not NULL.  The footprint and schedule apply to
+ * both modes, the grid to the tuned mode. */
 int rv_frame_autotune(rv_frame_plan* plan, uint32_t* d_argb, int interpolate,
                       const char* cache_path, rv_stream_t stream, uint32_t* tile_w,
                       uint32_t* tile_h);

@@ -228,6 +228,7 @@ def subtree_n2i_device(d_children, nnodes, depth, d_out, stream=None):
 # s3vs::FieldMode (FieldMode.hpp:26-51)
 FIELD_ISOSURFACE, FIELD_ISOSURFACES, FIELD_MIP, FIELD_ARGB, FIELD_ARGB_LIGHT, FIELD_DOMAIN_VOXELS = 0, 1, 2, 3, 4, 5
 FIELD_VALUE_ON_ISOSURFACE, FIELD_VALUE_ON_ISOSURFACES = 6, 7
+SCHEDULE_BINS, SCHEDULE_LANES = 0, 1
 
 
 def _levels(levels):
@@ -388,6 +389,16 @@ class FramePlan:
         w, h = C.c_uint32(), C.c_uint32()
         check(lib().rv_frame_get_tile(self.h, C.byref(w), C.byref(h)))
         return w.value, h.value
+
+    def set_schedule(self, schedule):
+        """SCHEDULE_BINS (the wave marches each block of its bin) or
+        SCHEDULE_LANES (each lane walks its own pixel's blocks; Argb)."""
+        check(lib().rv_frame_set_schedule(self.h, schedule))
+
+    def schedule(self):
+        v = C.c_uint32()
+        check(lib().rv_frame_get_schedule(self.h, C.byref(v)))
+        return v.value
 
     def set_field_mode(self, mode, iso_opacity=1.0, levels=None):
         lv, lp, ln = _levels(levels)

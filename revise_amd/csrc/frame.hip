@@ -148,8 +148,77 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
     }
 }
 
+// The same bin with each lane walking its own pixel's blocks (Argb): a lane
+// scans the bin's front-to-back list for the next block whose view
+// rectangle holds its pixel and marches it with that block's parameters, so
+// the lanes of a wave march different blocks at once.  Every pixel composes
+// the same segments in the same order as render_bin, so the results are
+// identical; the gain is for blocks a few pixels wide, which cover a small
+// part of a bin (the wave-wide walk then marches few of its lanes).
+template <class BrickT>
+__device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4* tfs)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t bx = bin % a.nbx, by = bin / a.nbx;
+    const uint32_t vx = bx * a.bw + (lane & (a.bw - 1u)), vy = by * a.bh + (lane >> a.bw_log2);
+    const bool inside = vx < a.W && vy < a.H;
+    const uint32_t k1 = a.bin_off[bin + 1];
+    uint32_t k = a.bin_off[bin];
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    float trans = 1.f;
+    bool live = inside; // the lane may still meet blocks
+    for (;;) {
+        uint32_t blk = 0;
+        bool found = false;
+        while (live && k < k1) {
+            const uint32_t b = a.bin_list[k];
+            ++k;
+            // view_x, view_y, view_w, view_h
+            const uint32_t* vr = reinterpret_cast<const uint32_t*>(a.params + b) + 2;
+            const uint32_t x0 = vr[0], y0 = vr[1], w = vr[2], h = vr[3];
+            if (vx >= x0 && vx < x0 + w && vy >= y0 && vy < y0 + h) {
+                blk = b;
+                found = true;
+                break;
+            }
+        }
+        if (!found)
+            live = false;
+        if (!__any(found))
+            break;
+        if (found) {
+            const rv_block_params p = a.params[blk];
+            const RaySeg r = setup_ray(p, vx, vy);
+            if (r.hit) {
+                const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);
+                float4 seg;
+                if (a.tape) {
+                    a.tape[a.rec_off[blk] + (vy - p.view_y) * p.view_w + (vx - p.view_x)] =
+                        make_float4(acc.x * acc.w, acc.y * acc.w, acc.z * acc.w, trans);
+                    float ts;
+                    seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
+                    trans = trans * ts;
+                } else {
+                    seg = march_impl<BrickT, false>(p, brick, tfs, (int)a.tf_n, r, nullptr);
+                }
+                blend_behind(acc, seg);
+                if (acc.w > a.acc_cutoff)
+                    live = false;
+            }
+        }
+    }
+    if (inside) {
+        const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
+        a.argb[pix] = ((unsigned)(acc.w * 255.999f) << 24) + ((unsigned)(acc.x * 255.999f) << 16) +
+                      ((unsigned)(acc.y * 255.999f) << 8) + (unsigned)(acc.z * 255.999f);
+        if (a.rgba)
+            a.rgba[pix] = acc;
+    }
+}
+
 // Screen phase of a wave: take bins from the bin queue until it is empty.
-template <class BrickT, uint32_t kMode>
+// kLanes: the per-lane walk (Argb only).
+template <class BrickT, uint32_t kMode, bool kLanes>
 __device__ void render_bins(const FrameArgs& a, const float4* tfs, const float* lvs)
 {
     for (;;) {
@@ -159,7 +228,10 @@ __device__ void render_bins(const FrameArgs& a, const float4* tfs, const float* 
         bin = uniform(bin);
         if (bin >= a.nbins)
             break;
-        render_bin<BrickT, kMode>(a, bin, tfs, lvs);
+        if constexpr (kLanes)
+            render_bin_lanes<BrickT>(a, bin, tfs);
+        else
+            render_bin<BrickT, kMode>(a, bin, tfs, lvs);
     }
 }
 
@@ -175,7 +247,7 @@ __device__ __forceinline__ void publish_tiles(uint32_t* published, uint32_t tile
         __hip_atomic_fetch_add(published, tiles, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <class BrickT, uint32_t kMode>
+template <class BrickT, uint32_t kMode, bool kLanes>
 __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
 {
     __shared__ __align__(16) float A[kTileFloats]; // interpolation tile, then the colour table
@@ -217,12 +289,12 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         lvs[threadIdx.x] = a.lev.v[threadIdx.x];
     __syncthreads();
     if ((threadIdx.x >> 6) < a.render_waves) // fewer concurrent rays can be faster (vector L1)
-        render_bins<BrickT, kMode>(a, tfs, lvs);
+        render_bins<BrickT, kMode, kLanes>(a, tfs, lvs);
 }
 
 // Frames of resident bricks: screen bins only, no interpolation LDS, so
 // more waves per CU.
-template <class BrickT, uint32_t kMode>
+template <class BrickT, uint32_t kMode, bool kLanes>
 __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 {
     extern __shared__ float4 tfs_dyn[]; // tf_n entries
@@ -232,7 +304,7 @@ __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
     if (kMode == kModeIsosurfaces && threadIdx.x < a.lev.n)
         lvs[threadIdx.x] = a.lev.v[threadIdx.x];
     __syncthreads();
-    render_bins<BrickT, kMode>(a, tfs_dyn, lvs);
+    render_bins<BrickT, kMode, kLanes>(a, tfs_dyn, lvs);
 }
 
 // Worker share in one launch: the blocks (back to front) of one share are
@@ -443,33 +515,48 @@ int get_bins(rv_frame_plan* P, uint32_t bw, uint32_t bh, FrameBins** out)
     return build_bins(P, bw, bh, out);
 }
 
-template <uint32_t kMode>
-void launch_mode(const rv_frame_plan* P, const FrameArgs& a, int interpolate, uint32_t grid,
-                 hipStream_t s)
+template <uint32_t kMode, bool kLanes>
+void launch_sched(const rv_frame_plan* P, const FrameArgs& a, int interpolate, uint32_t grid,
+                  hipStream_t s)
 {
     if (!interpolate) {
         const size_t lds = sizeof(float4) * a.tf_n;
         if (P->fp16)
-            hipLaunchKernelGGL((render_frame_kernel<__half, kMode>), dim3(P->render_grid), dim3(kRT),
-                               lds, s, a);
+            hipLaunchKernelGGL((render_frame_kernel<__half, kMode, kLanes>), dim3(P->render_grid),
+                               dim3(kRT), lds, s, a);
         else
-            hipLaunchKernelGGL((render_frame_kernel<float, kMode>), dim3(P->render_grid), dim3(kRT),
-                               lds, s, a);
+            hipLaunchKernelGGL((render_frame_kernel<float, kMode, kLanes>), dim3(P->render_grid),
+                               dim3(kRT), lds, s, a);
     } else if (P->fp16) {
-        hipLaunchKernelGGL((frame_kernel<__half, kMode>), dim3(grid), dim3(kT), 0, s, a);
+        hipLaunchKernelGGL((frame_kernel<__half, kMode, kLanes>), dim3(grid), dim3(kT), 0, s, a);
     } else {
-        hipLaunchKernelGGL((frame_kernel<float, kMode>), dim3(grid), dim3(kT), 0, s, a);
+        hipLaunchKernelGGL((frame_kernel<float, kMode, kLanes>), dim3(grid), dim3(kT), 0, s, a);
     }
 }
 
-// Resident workgroups per CU of the interpolating kernel in a field mode.
 template <uint32_t kMode>
+void launch_mode(const rv_frame_plan* P, const FrameArgs& a, int interpolate, uint32_t grid,
+                 hipStream_t s)
+{
+    if constexpr (kMode == kModeArgb) {
+        if (P->schedule == RV_FRAME_SCHEDULE_LANES) {
+            launch_sched<kMode, true>(P, a, interpolate, grid, s);
+            return;
+        }
+    }
+    launch_sched<kMode, false>(P, a, interpolate, grid, s);
+}
+
+// Resident workgroups per CU of the interpolating kernel in a field mode.
+template <uint32_t kMode, bool kLanes = false>
 int frame_max_wgs(bool fp16, int* per_cu)
 {
     if (fp16)
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<__half, kMode>, kT, 0));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<__half, kMode, kLanes>,
+                                                            kT, 0));
     else
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<float, kMode>, kT, 0));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<float, kMode, kLanes>,
+                                                            kT, 0));
     return RV_OK;
 }
 
@@ -523,7 +610,10 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     RV_LAUNCH_CHECK();
     // the interpolating kernel waits for its bricks: its grid never exceeds
     // what the mode's kernel keeps resident
-    const uint32_t grid = std::min(P->grid, P->cus * P->mode_max_wgs);
+    uint32_t cap = P->mode_max_wgs;
+    if (P->schedule == RV_FRAME_SCHEDULE_LANES && P->field_mode == RV_FIELD_ARGB)
+        cap = std::min(cap, P->lanes_max_wgs);
+    const uint32_t grid = std::min(P->grid, P->cus * cap);
     switch (P->field_mode) {
     case RV_FIELD_ARGB_LIGHT:
         launch_mode<kModeArgbLight>(P, a, interpolate, grid, s);
@@ -603,21 +693,26 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     int dev = 0, cus = 0, per_cu = 0;
     RV_HIP(hipGetDevice(&dev));
     RV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    if (fp16)
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<__half, kModeArgb>, kT, 0));
-    else
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, frame_kernel<float, kModeArgb>, kT, 0));
+    int lanes_per_cu = 0;
+    int orc = frame_max_wgs<kModeArgb, true>(fp16, &lanes_per_cu);
+    if (orc)
+        return orc;
+    RV_REQUIRE(lanes_per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
+    P->lanes_max_wgs = (uint32_t)lanes_per_cu;
+    orc = frame_max_wgs<kModeArgb, false>(fp16, &per_cu);
+    if (orc)
+        return orc;
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->cus = (uint32_t)cus;
     P->grid_max_wgs = (uint32_t)per_cu;
     P->mode_max_wgs = (uint32_t)per_cu;
     P->grid = (uint32_t)(cus * per_cu);
     if (fp16)
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<__half, kModeArgb>,
-                                                            kRT, sizeof(float4) * desc->tf_n));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, render_frame_kernel<__half, kModeArgb, false>, kRT, sizeof(float4) * desc->tf_n));
     else
-        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, render_frame_kernel<float, kModeArgb>,
-                                                            kRT, sizeof(float4) * desc->tf_n));
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, render_frame_kernel<float, kModeArgb, false>, kRT, sizeof(float4) * desc->tf_n));
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->render_max_wgs = (uint32_t)per_cu;
     P->render_grid = (uint32_t)(cus * std::min(per_cu, 4)); // 16 waves per CU until tuned
@@ -830,6 +925,33 @@ extern "C" int rv_frame_get_tile(const rv_frame_plan* P, uint32_t* tile_w, uint3
     return RV_OK;
 }
 
+extern "C" int rv_frame_set_schedule(rv_frame_plan* P, uint32_t schedule)
+{
+    RV_REQUIRE(P, "rv_frame_set_schedule: null plan");
+    RV_REQUIRE(schedule == RV_FRAME_SCHEDULE_BINS || schedule == RV_FRAME_SCHEDULE_LANES,
+               "rv_frame_set_schedule: unknown schedule");
+    if (schedule != P->schedule) {
+        // a captured graph holds the previous schedule's kernels
+        if (P->graph_exec) {
+            (void)hipGraphExecDestroy(P->graph_exec);
+            P->graph_exec = nullptr;
+        }
+        if (P->graph) {
+            (void)hipGraphDestroy(P->graph);
+            P->graph = nullptr;
+        }
+        P->schedule = schedule;
+    }
+    return RV_OK;
+}
+
+extern "C" int rv_frame_get_schedule(const rv_frame_plan* P, uint32_t* schedule)
+{
+    RV_REQUIRE(P && schedule, "rv_frame_get_schedule: null pointer");
+    *schedule = P->schedule;
+    return RV_OK;
+}
+
 // Autotuner: time every candidate wave footprint of the screen bins on the
 // plan's camera 0 (hip events, median of 3 after a warm-up) and keep the
 // fastest; the bins of the other footprints are released.  Results
@@ -848,7 +970,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     hipDeviceProp_t prop;
     RV_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream sig;
-    sig << prop.gcnArchName << "|bins2|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
+    sig << prop.gcnArchName << "|bins3|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
         << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth
         << "|mode" << P->field_mode;
     const std::string key = sig.str();
@@ -867,7 +989,12 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
                 valid_footprint(w, h) && gp != std::string::npos &&
                 std::sscanf(line.c_str() + gp + 14, "%u", &g) == 1 && g >= 1 &&
                 g <= (interpolate ? P->grid_max_wgs : P->render_max_wgs)) {
-                unsigned rw = 0;
+                unsigned rw = 0, sch = 0;
+                const size_t sp = line.find("\"schedule\": ");
+                if (sp != std::string::npos && std::sscanf(line.c_str() + sp + 12, "%u", &sch) == 1 &&
+                    (sch == RV_FRAME_SCHEDULE_BINS ||
+                     (sch == RV_FRAME_SCHEDULE_LANES && P->field_mode == RV_FIELD_ARGB)))
+                    (void)rv_frame_set_schedule(P, sch);
                 const size_t rp = line.find("\"render_waves\": ");
                 if (interpolate && rp != std::string::npos &&
                     std::sscanf(line.c_str() + rp + 16, "%u", &rw) == 1 && rw >= 1 && rw <= kT / 64u)
@@ -899,14 +1026,19 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     std::vector<uint32_t> rws = {kT / 64u};
     if (interpolate)
         rws = {4u, 6u, kT / 64u};
-    uint32_t brw = P->frame_rw;
+    std::vector<uint32_t> scheds = {RV_FRAME_SCHEDULE_BINS};
+    if (P->field_mode == RV_FIELD_ARGB)
+        scheds.push_back(RV_FRAME_SCHEDULE_LANES);
+    uint32_t brw = P->frame_rw, bsched = P->schedule;
     std::ostringstream times;
+    for (uint32_t sch : scheds)
     for (const auto& c : cand)
     for (uint32_t g : wgs)
     for (uint32_t rw : rws) {
         grid = P->cus * g;
         if (interpolate)
             P->frame_rw = rw;
+        (void)rv_frame_set_schedule(P, sch);
         float t[3];
         int rc = launch_frame(P, 0, d_argb, nullptr, interpolate, s, c[0], c[1]);
         for (int r = 0; r < 3 && !rc; ++r) {
@@ -922,7 +1054,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
             return rc;
         }
         std::sort(t, t + 3);
-        times << (times.tellp() > 0 ? ", " : "") << "\"" << c[0] << "x" << c[1] << "/" << g;
+        times << (times.tellp() > 0 ? ", " : "") << "\"" << (sch == RV_FRAME_SCHEDULE_LANES ? "L" : "B")
+              << c[0] << "x" << c[1] << "/" << g;
         if (interpolate)
             times << "/" << rw;
         times << "\": " << t[1];
@@ -932,11 +1065,13 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
             bh = c[1];
             bg = g;
             brw = rw;
+            bsched = sch;
         }
     }
     grid = P->cus * bg;
     if (interpolate)
         P->frame_rw = brw;
+    (void)rv_frame_set_schedule(P, bsched);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     P->tw = bw;
@@ -951,7 +1086,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     if (cache_path && *cache_path) {
         std::ofstream f(cache_path, std::ios::app);
         f << "{\"key\": \"" << key << "\", \"tile\": [" << bw << ", " << bh
-          << "], \"wgs_per_cu\": " << bg << ", \"render_waves\": " << P->frame_rw << ", \"ms\": {"
+          << "], \"wgs_per_cu\": " << bg << ", \"render_waves\": " << P->frame_rw
+          << ", \"schedule\": " << P->schedule << ", \"ms\": {"
           << times.str() << "}}\n";
     }
     return RV_OK;

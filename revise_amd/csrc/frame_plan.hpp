@@ -75,6 +75,8 @@ struct rv_frame_plan {
     float iso_opacity = 1.f;
     rv::LevelsArg levels{};
     uint32_t mode_max_wgs = 0; // resident interpolating workgroups per CU in field_mode
+    uint32_t schedule = RV_FRAME_SCHEDULE_BINS; // how a wave walks its screen bin (Argb)
+    uint32_t lanes_max_wgs = 0; // resident interpolating workgroups per CU, per-lane walk
     // adjoint tape: per camera, per block, the offset of the block's view
     // rectangle in the tape (float4 per pixel); the blocks' 8 x 8 tiles
     // as (block, tile) pairs, grouped by block

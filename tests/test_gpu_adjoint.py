@@ -245,3 +245,36 @@ def test_adjoint_accumulates_and_tf_only(orc):
     torch.cuda.synchronize()
     assert float((g3 - g1).abs().max()) <= 1e-4 * scale
     fp.close()
+
+
+def test_adjoint_after_lanes_schedule(orc):
+    """The per-lane walk records the same adjoint states: its taped float
+    image equals the bin walk's bit for bit, and the gradients agree to the
+    rounding of the float atomics."""
+    import torch
+    wl, tf_np, tf_target, M = _scene(2, 4, 96, 96, seed=12)
+    tf = to_dev(tf_np)
+    fp = _plan(wl, 96, 96, tf)
+    fp.set_cameras(M[None])
+    fp.set_tape(True)
+    g = torch.randn((96, 96, 4), dtype=torch.float32, device="cuda",
+                    generator=torch.Generator("cuda").manual_seed(5))
+    out = {}
+    for sched in (api.SCHEDULE_BINS, api.SCHEDULE_LANES):
+        say("adjoint after schedule", sched)
+        fp.set_schedule(sched)
+        rgba = torch.empty((96, 96, 4), dtype=torch.float32, device="cuda")
+        fp.render(0, torch.empty((96, 96), dtype=torch.int32, device="cuda"), rgba, interpolate=False)
+        gt = torch.zeros((256, 4), dtype=torch.float32, device="cuda")
+        grads = [torch.zeros_like(wl.bricks[i]) for i in range(len(wl.blocks))]
+        fp.set_grad_bricks(grads)
+        fp.adjoint(0, rgba, g, gt)
+        torch.cuda.synchronize()
+        out[sched] = (rgba.clone(), gt, torch.stack(grads))
+    a, b = out[api.SCHEDULE_BINS], out[api.SCHEDULE_LANES]
+    assert torch.equal(a[0], b[0])
+    for x, y in ((a[1], b[1]), (a[2], b[2])):
+        scale = float(x.abs().max())
+        assert scale > 0
+        assert float((x - y).abs().max()) <= 1e-4 * scale
+    fp.close()

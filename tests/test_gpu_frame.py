@@ -72,18 +72,24 @@ def _oracle_frame(orc, blocks, bricks_np, M, W, H, fmin, fmax, tf, acc_cut):
     return orc.render_float_composite(M, W, H, FOV, lst, fmin, fmax, THR, Q, CUT, tf, acc_cut)
 
 
+SCHEDULES = [api.SCHEDULE_BINS, api.SCHEDULE_LANES]
+
+
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("fp16,acc_cut", [(False, 2.0), (False, 0.9999), (True, 0.9999)])
-def test_frame_uniform_matches_oracle(orc, fp16, acc_cut):
+def test_frame_uniform_matches_oracle(orc, fp16, acc_cut, schedule):
     import torch
     blocks = _uniform_blocks(2, 4, seed=5)
     W, H = 200, 150
     fp, tf, fmin, fmax, dev = _setup(blocks, W, H, fp16, acc_cut, seed=5)
+    fp.set_schedule(schedule)
+    assert fp.schedule() == schedule
     cams = np.stack([datasets.orbit_camera(a) for a in (15.0, 140.0)])
     fp.set_cameras(cams)
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
     rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
     for c in range(2):
-        say("frame uniform cam", c, "fp16", fp16)
+        say("frame uniform cam", c, "fp16", fp16, "schedule", schedule)
         fp.render(c, argb, rgba, interpolate=True)
         torch.cuda.synchronize()
         assert fp.status() == 0
@@ -99,12 +105,14 @@ def test_frame_uniform_matches_oracle(orc, fp16, acc_cut):
         assert (a >> 24).any()
 
 
-def test_frame_mixed_levels_matches_oracle(orc):
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_frame_mixed_levels_matches_oracle(orc, schedule):
     import torch
     blocks = _mixed_blocks(seed=8)
-    say("mixed blocks", len(blocks))
+    say("mixed blocks", len(blocks), "schedule", schedule)
     W, H = 180, 140
     fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=8)
+    fp.set_schedule(schedule)
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((70.0, 25.0), (250.0, -40.0))])
     fp.set_cameras(cams)
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
@@ -122,11 +130,13 @@ def test_frame_mixed_levels_matches_oracle(orc):
         assert np.array_equal(rgba.cpu().numpy(), exp)
 
 
-def test_frame_graph_replays_match_eager(orc):
+@pytest.mark.parametrize("schedule", SCHEDULES)
+def test_frame_graph_replays_match_eager(orc, schedule):
     import torch
     blocks = _uniform_blocks(2, 4, seed=6)
     W, H = 160, 120
     fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=6)
+    fp.set_schedule(schedule)
     cams = np.stack([datasets.orbit_camera(a) for a in (0.0, 45.0, 90.0)])
     fp.set_cameras(cams)
     eager = []
@@ -160,15 +170,19 @@ def test_frame_autotune_picks_valid_tile(tmp_path):
     cache = str(tmp_path / "tune.jsonl")
     w, h = fp.autotune(argb, True, cache)
     assert w * h == 64 and (w, h) == fp.tile()
+    sched = fp.schedule()
+    assert sched in SCHEDULES
     w2, h2 = fp.autotune(argb, True, cache)  # served from the cache
-    assert (w2, h2) == (w, h)
-    assert open(cache).read().count("\n") == 1
+    assert (w2, h2) == (w, h) and fp.schedule() == sched
+    text = open(cache).read()
+    assert text.count("\n") == 1 and '"schedule": %d' % sched in text
 
 
+@pytest.mark.parametrize("schedule", SCHEDULES)
 @pytest.mark.parametrize("tile", [(8, 8), (16, 4), (4, 16), (32, 2)])
-def test_frame_small_blocks_every_footprint(orc, tile):
+def test_frame_small_blocks_every_footprint(orc, tile, schedule):
     """A sparse level-4 share of depth-2 bricks: blocks a few pixels wide, many
-    per bin; every wave footprint gives the oracle's frame."""
+    per bin; every wave footprint and both schedules give the oracle's frame."""
     import torch
     rng = np.random.default_rng(11)
     field = datasets.Field(11, n_gauss=3, noise_octaves=1)
@@ -178,13 +192,14 @@ def test_frame_small_blocks_every_footprint(orc, tile):
     W, H = 150, 110
     fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.999, seed=11, tile=tile)
     assert fp.tile() == tile
+    fp.set_schedule(schedule)
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((20.0, 10.0), (200.0, -30.0))])
     fp.set_cameras(cams)
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
     rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
     bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
     for c in range(2):
-        say("small blocks", len(blocks), "tile", tile, "cam", c)
+        say("small blocks", len(blocks), "tile", tile, "schedule", schedule, "cam", c)
         fp.render(c, argb, rgba, interpolate=True)
         torch.cuda.synchronize()
         assert fp.status() == 0
