@@ -1014,11 +1014,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     hipEvent_t e0, e1;
     RV_HIP(hipEventCreate(&e0));
     RV_HIP(hipEventCreate(&e1));
-    float best = 1e30f;
-    uint32_t bw = P->tw, bh = P->th;
     uint32_t& grid = interpolate ? P->grid : P->render_grid;
     const uint32_t gmax = interpolate ? P->grid_max_wgs : P->render_max_wgs;
-    uint32_t bg = grid / P->cus;
     std::vector<uint32_t> wgs;
     for (uint32_t g : {1u, 2u, 3u, 4u, 6u, 8u, gmax})
         if (g <= gmax && std::find(wgs.begin(), wgs.end(), g) == wgs.end())
@@ -1029,49 +1026,88 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     std::vector<uint32_t> scheds = {RV_FRAME_SCHEDULE_BINS};
     if (P->field_mode == RV_FIELD_ARGB)
         scheds.push_back(RV_FRAME_SCHEDULE_LANES);
-    uint32_t brw = P->frame_rw, bsched = P->schedule;
     std::ostringstream times;
-    for (uint32_t sch : scheds)
-    for (const auto& c : cand)
-    for (uint32_t g : wgs)
-    for (uint32_t rw : rws) {
+    // median of 3 timed frames of camera cam in a configuration
+    auto time_cfg = [&](uint32_t sch, uint32_t tw, uint32_t th, uint32_t g, uint32_t rw, uint32_t cam,
+                        float* ms) -> int {
         grid = P->cus * g;
         if (interpolate)
             P->frame_rw = rw;
         (void)rv_frame_set_schedule(P, sch);
         float t[3];
-        int rc = launch_frame(P, 0, d_argb, nullptr, interpolate, s, c[0], c[1]);
+        int rc = launch_frame(P, cam, d_argb, nullptr, interpolate, s, tw, th);
         for (int r = 0; r < 3 && !rc; ++r) {
             (void)hipEventRecord(e0, s);
-            rc = launch_frame(P, 0, d_argb, nullptr, interpolate, s, c[0], c[1]);
+            rc = launch_frame(P, cam, d_argb, nullptr, interpolate, s, tw, th);
             (void)hipEventRecord(e1, s);
             (void)hipEventSynchronize(e1);
             (void)hipEventElapsedTime(&t[r], e0, e1);
         }
+        std::sort(t, t + 3);
+        *ms = t[1];
+        return rc;
+    };
+    struct Choice {
+        float ms = 1e30f;
+        uint32_t w = 8, h = 8, g = 1, rw = kT / 64u;
+    } pick[2];
+    for (uint32_t sch : scheds)
+    for (const auto& c : cand)
+    for (uint32_t g : wgs)
+    for (uint32_t rw : rws) {
+        float ms = 0.f;
+        const int rc = time_cfg(sch, c[0], c[1], g, rw, 0, &ms);
         if (rc) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             return rc;
         }
-        std::sort(t, t + 3);
         times << (times.tellp() > 0 ? ", " : "") << "\"" << (sch == RV_FRAME_SCHEDULE_LANES ? "L" : "B")
               << c[0] << "x" << c[1] << "/" << g;
         if (interpolate)
             times << "/" << rw;
-        times << "\": " << t[1];
-        if (t[1] < best) {
-            best = t[1];
-            bw = c[0];
-            bh = c[1];
-            bg = g;
-            brw = rw;
-            bsched = sch;
-        }
+        times << "\": " << ms;
+        if (ms < pick[sch].ms)
+            pick[sch] = Choice{ms, c[0], c[1], g, rw};
     }
-    grid = P->cus * bg;
+    // The schedules' best configurations on camera 0 are compared again over
+    // up to 4 cameras spread through the plan's set: how much a per-lane walk
+    // gains depends on the view.
+    uint32_t bsched = pick[RV_FRAME_SCHEDULE_LANES].ms < pick[RV_FRAME_SCHEDULE_BINS].ms
+                          ? RV_FRAME_SCHEDULE_LANES
+                          : RV_FRAME_SCHEDULE_BINS;
+    if (scheds.size() == 2 && P->ncams > 1) {
+        std::vector<uint32_t> cams;
+        for (uint32_t q = 0; q < 4; ++q) {
+            const uint32_t cam = (uint32_t)((uint64_t)P->ncams * q / 4);
+            if (std::find(cams.begin(), cams.end(), cam) == cams.end())
+                cams.push_back(cam);
+        }
+        float tot[2] = {0.f, 0.f};
+        for (uint32_t sch : scheds)
+            for (uint32_t cam : cams) {
+                float ms = 0.f;
+                const Choice& c = pick[sch];
+                const int rc = time_cfg(sch, c.w, c.h, c.g, c.rw, cam, &ms);
+                if (rc) {
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    return rc;
+                }
+                tot[sch] += ms;
+            }
+        times << ", \"B over " << cams.size() << " cameras\": " << tot[0] << ", \"L over "
+              << cams.size() << " cameras\": " << tot[1];
+        bsched = tot[RV_FRAME_SCHEDULE_LANES] < tot[RV_FRAME_SCHEDULE_BINS] ? RV_FRAME_SCHEDULE_LANES
+                                                                            : RV_FRAME_SCHEDULE_BINS;
+    }
+    const Choice& bc = pick[bsched];
+    const uint32_t bw = bc.w, bh = bc.h;
+    grid = P->cus * bc.g;
     if (interpolate)
-        P->frame_rw = brw;
+        P->frame_rw = bc.rw;
     (void)rv_frame_set_schedule(P, bsched);
+    const uint32_t bg = bc.g;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     P->tw = bw;
