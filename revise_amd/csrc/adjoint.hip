@@ -89,6 +89,9 @@ struct AdjArgs {
     const float4* rgba;        // forward image, rows bottom-up
     const float4* grad_rgba;   // dL/dimage, same layout
     float4* grad_tf;           // tf_n entries, accumulated
+    const uint32_t* rec_chunk; // per 64 records: the block of the first (colour-table-only kernel)
+    uint64_t nrec;             // records of this camera
+    uint32_t nblocks;
 };
 
 // Per-pixel state of the re-march.
@@ -479,6 +482,99 @@ __device__ void adjoint_item(const rv_block_params& p, const T* __restrict__ bri
     tf_flush(sg.tacc, gtf);
 }
 
+// The re-march's start for the record rec of block p, pixel (lx, ly) of its
+// view rectangle: the state in front of the block (tape), the forward image
+// and dL/dimage at the pixel; false when the ray never reaches the block,
+// carries no gradient, or misses it.
+__device__ __forceinline__ bool pixel_start(const AdjArgs& a, const rv_block_params& p, uint64_t recidx,
+                                            uint32_t lx, uint32_t ly, AdjPixel& st, RaySeg& r)
+{
+    const float4 rec = a.tape[recidx];
+    if (!(rec.w >= 0.f)) // the pixel's ray does not reach this block
+        return false;
+    const uint32_t vx = p.view_x + lx, vy = p.view_y + ly;
+    const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
+    const float4 I = a.rgba[pix];
+    const float4 g = a.grad_rgba[pix];
+    st.T = rec.w;
+    st.Px = rec.x;
+    st.Py = rec.y;
+    st.Pz = rec.z;
+    st.Fx = I.x * I.w;
+    st.Fy = I.y * I.w;
+    st.Fz = I.z * I.w;
+    st.Tend = 1.f - I.w;
+    if (I.w > 0.f) {
+        st.gPx = g.x / I.w;
+        st.gPy = g.y / I.w;
+        st.gPz = g.z / I.w;
+        st.gA = g.w - ((g.x * I.x + g.y * I.y) + g.z * I.z) / I.w;
+    } else {
+        st.gA = g.w;
+    }
+    if (st.gPx == 0.f && st.gPy == 0.f && st.gPz == 0.f && st.gA == 0.f)
+        return false;
+    r = setup_ray(p, vx, vy);
+    return r.hit;
+}
+
+// Colour-table gradient only (no field gradient arrays): one lane per tape
+// record (block, pixel of its view rectangle), records in tape order, so a
+// wave's lanes are one block's consecutive pixels, or the pixels of a few
+// consecutive blocks, and small blocks fill the wave.  Each lane re-marches
+// its segment with its own block's parameters.
+template <class BrickT>
+__global__ __launch_bounds__(kRT) void adjoint_records_kernel(AdjArgs a)
+{
+    extern __shared__ float4 dyn[]; // colour table and its gradient
+    float4* tfs = dyn;
+    float4* gtf = dyn + a.tf_n;
+    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT) {
+        tfs[i] = a.tf[i];
+        gtf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * kRT;
+    Window w;
+    w.ok = false;
+    w.o[0] = w.o[1] = w.o[2] = 0;
+    w.e[0] = w.e[1] = w.e[2] = 0;
+    for (uint64_t rid = (uint64_t)blockIdx.x * kRT + threadIdx.x; rid < a.nrec; rid += stride) {
+        uint32_t blk = a.rec_chunk[rid >> 6];
+        while (blk + 1u < a.nblocks && a.rec_off[blk + 1u] <= rid)
+            ++blk;
+        const rv_block_params p = a.params[blk];
+        const uint32_t li = (uint32_t)(rid - a.rec_off[blk]);
+        const uint32_t ly = li / p.view_w, lx = li - ly * p.view_w;
+        AdjPixel st{};
+        SegRay sg;
+        if (!pixel_start(a, p, rid, lx, ly, st, sg.r))
+            continue;
+        sg.i = 1;
+        sg.seg_w = 0.f;
+        sg.Tseg = 1.f;
+        sg.ended = false;
+        sg.tacc.i0 = -1;
+        sg.cell.key = 0xFFFFFFFFu;
+        march_adjoint(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), nullptr, tfs, gtf,
+                      (int)a.tf_n, st, sg, -1, true, 0, w, nullptr);
+        tf_flush(sg.tacc, gtf);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT) {
+        const float4 g = gtf[i];
+        float* out = reinterpret_cast<float*>(a.grad_tf + i);
+        if (g.x != 0.f)
+            unsafeAtomicAdd(out + 0, g.x);
+        if (g.y != 0.f)
+            unsafeAtomicAdd(out + 1, g.y);
+        if (g.z != 0.f)
+            unsafeAtomicAdd(out + 2, g.z);
+        if (g.w != 0.f)
+            unsafeAtomicAdd(out + 3, g.w);
+    }
+}
+
 // Persistent grid; work items (block, tile) in order, tiles of a block
 // consecutive; wave v of workgroup g takes items g*4+v, then every
 // (grid waves)-th after it, so the waves of a workgroup work on neighbouring
@@ -508,35 +604,8 @@ __global__ __launch_bounds__(kRT) void adjoint_kernel(AdjArgs a)
         AdjPixel st{};
         RaySeg r{};
         bool active = false;
-        if (lx < p.view_w && ly < p.view_h) {
-            const float4 rec = a.tape[a.rec_off[blk] + ly * p.view_w + lx];
-            if (rec.w >= 0.f) { // the pixel's ray reaches this block
-                const uint32_t vx = p.view_x + lx, vy = p.view_y + ly;
-                const uint32_t pix = (a.H - 1u - vy) * a.W + vx;
-                const float4 I = a.rgba[pix];
-                const float4 g = a.grad_rgba[pix];
-                st.T = rec.w;
-                st.Px = rec.x;
-                st.Py = rec.y;
-                st.Pz = rec.z;
-                st.Fx = I.x * I.w;
-                st.Fy = I.y * I.w;
-                st.Fz = I.z * I.w;
-                st.Tend = 1.f - I.w;
-                if (I.w > 0.f) {
-                    st.gPx = g.x / I.w;
-                    st.gPy = g.y / I.w;
-                    st.gPz = g.z / I.w;
-                    st.gA = g.w - ((g.x * I.x + g.y * I.y) + g.z * I.z) / I.w;
-                } else {
-                    st.gA = g.w;
-                }
-                if (st.gPx != 0.f || st.gPy != 0.f || st.gPz != 0.f || st.gA != 0.f) {
-                    r = setup_ray(p, vx, vy);
-                    active = r.hit;
-                }
-            }
-        }
+        if (lx < p.view_w && ly < p.view_h)
+            active = pixel_start(a, p, a.rec_off[blk] + ly * p.view_w + lx, lx, ly, st, r);
         adjoint_item(p, reinterpret_cast<const BrickT*>(a.bricks[blk]),
                      a.grad_bricks ? a.grad_bricks[blk] : nullptr, tfs, gtf, (int)a.tf_n, active, r,
                      st, win);
@@ -618,6 +687,37 @@ extern "C" int rv_frame_adjoint(rv_frame_plan* P, uint32_t cam, const float* d_r
     a.grad_tf = reinterpret_cast<float4*>(d_grad_tf);
     if (a.nitems == 0)
         return RV_OK;
+    // Colour-table-only gradients of blocks a few pixels wide: their 8 x 8
+    // tiles leave most lanes of a wave idle, so one lane per record instead
+    const uint64_t nrec = P->rec_total[cam];
+    if (!a.grad_bricks && nrec < 48ull * a.nitems) {
+        a.rec_chunk = P->d_rec_chunk + P->chunk_off[cam];
+        a.nrec = nrec;
+        a.nblocks = P->nblocks;
+        if (!P->rec_grid) {
+            int dev = 0, cus = 0, per_cu = 0;
+            RV_HIP(hipGetDevice(&dev));
+            RV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            if (P->fp16)
+                RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_records_kernel<__half>,
+                                                                    kRT, 32ull * P->desc.tf_n));
+            else
+                RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_records_kernel<float>,
+                                                                    kRT, 32ull * P->desc.tf_n));
+            RV_REQUIRE(per_cu >= 1, "rv_frame_adjoint: kernel cannot be resident");
+            P->rec_grid = (uint32_t)(cus * per_cu);
+        }
+        const uint64_t need = (a.nrec + kRT - 1) / kRT;
+        const uint32_t rgrid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(P->rec_grid, need));
+        if (P->fp16)
+            hipLaunchKernelGGL(adjoint_records_kernel<__half>, dim3(rgrid), dim3(kRT), 32ull * a.tf_n,
+                               S(stream), a);
+        else
+            hipLaunchKernelGGL(adjoint_records_kernel<float>, dim3(rgrid), dim3(kRT), 32ull * a.tf_n,
+                               S(stream), a);
+        RV_LAUNCH_CHECK();
+        return RV_OK;
+    }
     const uint32_t grid = P->adj_grid;
     if (P->fp16)
         hipLaunchKernelGGL(adjoint_kernel<__half>, dim3(grid), dim3(kRT), adjoint_lds(a.tf_n),

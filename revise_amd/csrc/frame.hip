@@ -787,15 +787,19 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
         return rc;
     // adjoint tables: tape offsets of the view rectangles, (block, 8 x 8 tile) pairs
     std::vector<uint64_t> rec_off((size_t)ncams * nb);
-    std::vector<uint32_t> items;
+    std::vector<uint32_t> items, chunks;
     P->rec_total.assign(ncams, 0);
     P->item_off.assign(ncams + 1, 0);
+    P->chunk_off.assign(ncams + 1, 0);
     for (uint32_t c = 0; c < ncams; ++c) {
         uint64_t off = 0;
         for (uint32_t i = 0; i < nb; ++i) {
             const uint4 r = rects[(size_t)c * nb + i];
             rec_off[(size_t)c * nb + i] = off;
             const uint64_t w = r.z - r.x, h = r.w - r.y;
+            // the 64-record chunks whose first record is this block's
+            for (uint64_t j = (off + 63) / 64; j * 64 < off + w * h; ++j)
+                chunks.push_back(i);
             off += w * h;
             const uint32_t tx = (uint32_t)((w + 7) / 8), ty = (uint32_t)((h + 7) / 8);
             for (uint32_t t = 0; t < tx * ty; ++t) {
@@ -805,8 +809,11 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
         }
         P->rec_total[c] = off;
         P->item_off[c + 1] = items.size() / 2;
+        P->chunk_off[c + 1] = chunks.size();
     }
     if ((rc = frame_upload((void**)&P->d_rec_off, rec_off.data(), 8ull * rec_off.size())))
+        return rc;
+    if ((rc = frame_upload((void**)&P->d_rec_chunk, chunks.data(), 4ull * chunks.size())))
         return rc;
     if ((rc = frame_upload((void**)&P->d_items, items.data(), 4ull * items.size())))
         return rc;

@@ -68,6 +68,7 @@ struct rv_frame_plan {
     bool can_interp = true;
     float** d_grad_bricks = nullptr; // per block fp32 field gradient (adjoint), or NULL
     uint32_t adj_grid = 0;
+    uint32_t rec_grid = 0;    // colour-table-only adjoint (one lane per record)
     uint32_t render_grid = 0; // resident-brick frames
     uint32_t cus = 0, grid_max_wgs = 0, render_max_wgs = 0; // CUs, resident workgroups per CU
     uint32_t frame_rw = rv::kT / 64u; // waves per frame_kernel workgroup that render
@@ -80,9 +81,10 @@ struct rv_frame_plan {
     // adjoint tape: per camera, per block, the offset of the block's view
     // rectangle in the tape (float4 per pixel); the blocks' 8 x 8 tiles
     // as (block, tile) pairs, grouped by block
-    std::vector<uint64_t> rec_total, item_off;
+    std::vector<uint64_t> rec_total, item_off, chunk_off;
     uint64_t* d_rec_off = nullptr;
     uint32_t* d_items = nullptr;
+    uint32_t* d_rec_chunk = nullptr; // per camera, per 64 records: the block of the chunk's first
     float4* d_tape = nullptr;
     uint64_t tape_cap = 0;
     bool tape_on = false;
@@ -98,7 +100,7 @@ struct rv_frame_plan {
             (void)hipStreamDestroy(capture_stream);
         for (void* p : {(void*)d_tasks, (void*)d_bricks, (void*)d_counters,
                         (void*)d_params, (void*)d_grad_bricks,
-                        (void*)d_rec_off, (void*)d_items, (void*)d_tape})
+                        (void*)d_rec_off, (void*)d_items, (void*)d_rec_chunk, (void*)d_tape})
             if (p)
                 (void)hipFree(p);
     }

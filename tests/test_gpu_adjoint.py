@@ -278,3 +278,35 @@ def test_adjoint_after_lanes_schedule(orc):
         assert scale > 0
         assert float((x - y).abs().max()) <= 1e-4 * scale
     fp.close()
+
+
+def test_tf_adjoint_of_small_blocks_one_lane_per_record(orc):
+    """Blocks a few pixels wide (level 4 at 96x72: the 8 x 8 tiles of their
+    view rectangles are mostly empty) take the colour-table-only adjoint with
+    one lane per tape record; its table gradient equals the tile kernel's
+    (run with field gradients on) to the rounding of the float atomics."""
+    import torch
+    W, H = 96, 72
+    wl, tf_np, tf_target, M = _scene(4, 2, W, H, seed=21)
+    tf = to_dev(tf_np)
+    fp = _plan(wl, W, H, tf)
+    fp.set_cameras(np.stack([M, datasets.orbit_camera(100.0, elevation_deg=-20.0)]))
+    fp.set_tape(True)
+    for cam in range(2):
+        say("tf adjoint small blocks cam", cam)
+        rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+        fp.render(cam, torch.empty((H, W), dtype=torch.int32, device="cuda"), rgba, interpolate=False)
+        g = torch.randn((H, W, 4), dtype=torch.float32, device="cuda",
+                        generator=torch.Generator("cuda").manual_seed(7 + cam))
+        fp.set_grad_bricks(None)
+        g_rec = torch.zeros((256, 4), dtype=torch.float32, device="cuda")
+        fp.adjoint(cam, rgba, g, g_rec)
+        grads = [torch.zeros_like(wl.bricks[i]) for i in range(len(wl.blocks))]
+        fp.set_grad_bricks(grads)
+        g_tile = torch.zeros((256, 4), dtype=torch.float32, device="cuda")
+        fp.adjoint(cam, rgba, g, g_tile)
+        torch.cuda.synchronize()
+        scale = float(g_tile.abs().max())
+        assert scale > 0
+        assert float((g_rec - g_tile).abs().max()) <= 1e-4 * scale
+    fp.close()
