@@ -215,4 +215,24 @@ private:
 std::vector<uint32_t> composeFrame(VsWorkerHip& worker, unsigned level, unsigned splitCount,
                                    const std::function<bool()>& isCancelled);
 
+// VsControllerFrameOutputHeader (VsControllerFrameOutputHeader.hpp:30-50),
+// with the same layout: 24 bytes, the level padded to 8.
+struct FrameOutputHeader {
+    uint64_t frameNumber = ~0ull; // ~0: no frame
+    uint64_t durationMs = 0;
+    uint32_t level = 0;
+    uint32_t pad = 0;
+};
+
+// VsRenderer::continueRendering (VsRenderer.cpp:1061-1166): the progressive
+// level-of-detail loop over levels levelMin..levelMax while not cancelled.
+// Each level's composed frame goes to writeFrame(header, pixels) with the
+// next frame number (then incremented) and, when measureTime, the level's
+// wall time in ms (VsRenderer.cpp:1138-1152).  A level cancelled midway
+// writes nothing.  Returns the number of frames written.
+unsigned renderProgressive(VsWorkerHip& worker, unsigned levelMin, unsigned levelMax,
+                           unsigned splitCount, const std::function<bool()>& isCancelled,
+                           uint64_t& frameNumber, bool measureTime,
+                           const std::function<void(const FrameOutputHeader&, const uint32_t*)>& writeFrame);
+
 } // namespace revise

@@ -528,6 +528,34 @@ int rv_worker_render_scene_part(rv_worker* w, uint32_t level, const uint32_t box
  * frame: vp_w*vp_h ARGB. */
 int rv_worker_compose_frame(rv_worker* w, uint32_t level, uint32_t split_count, uint32_t* frame,
                             const volatile int* cancel);
+/* The block a frame output shared memory holds (VsControllerFrameOutputRW,
+ * VsControllerFrameOutputRW.hpp:41-75): this header
+ * (VsControllerFrameOutputHeader, VsControllerFrameOutputHeader.hpp:30-50;
+ * 24 bytes), then vp_w*vp_h ARGB pixels. */
+typedef struct rv_frame_output_header {
+    uint64_t frame_number; /* ~0: no frame */
+    uint64_t duration_ms;  /* rendering time of the frame (when measured) */
+    uint32_t level;        /* level of detail of the frame */
+    uint32_t pad;
+} rv_frame_output_header;
+/* VsRenderer::continueRendering (VsRenderer.cpp:1061-1166): the progressive
+ * level-of-detail loop.  For each level from level_min to level_max while
+ * *cancel is 0, the level's frame (rv_worker_compose_frame with split_count
+ * boxes) is written to `out` (writeFrame): the header with *frame_number
+ * (then incremented), the level's wall time in ms if measure_time, and the
+ * level, followed by the pixels.  A level cancelled midway writes nothing.
+ * out_size must hold the header and vp_w*vp_h*4 bytes.  The writes are
+ * plain copies: a caller sharing `out` between processes provides the lock
+ * (the reference's SharedMemory write accessor).  *frames_written: frames
+ * written. */
+int rv_worker_render_progressive(rv_worker* w, uint32_t level_min, uint32_t level_max,
+                                 uint32_t split_count, const volatile int* cancel, void* out,
+                                 uint64_t out_size, uint64_t* frame_number, int measure_time,
+                                 uint32_t* frames_written);
+/* readFrame (VsControllerFrameOutputRW.hpp:41-60) of such a block; pixels
+ * may be NULL (header only). */
+int rv_frame_output_read(const void* in, uint64_t in_size, uint32_t vp_w, uint32_t vp_h,
+                         rv_frame_output_header* header, uint32_t* pixels);
 /* fused != 0 (default): one batched interpolation + one share launch per
  * renderScenePart (bit-identical to the per-block loop); 0: one
  * interpolation + one ray-cast launch per block, like the reference. */

@@ -73,6 +73,11 @@ class SubtreeInfo(C.Structure):
                 ("depth", C.c_uint32), ("values_pos", C.c_uint64)]
 
 
+class FrameOutputHeader(C.Structure):
+    _fields_ = [("frame_number", C.c_uint64), ("duration_ms", C.c_uint64), ("level", C.c_uint32),
+                ("pad", C.c_uint32)]
+
+
 class FieldInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("min", C.c_float), ("max", C.c_float),
                 ("animated", C.c_int32)]
@@ -170,6 +175,10 @@ _SIGS = {
                                               C.POINTER(C.c_int32), _u32p, C.c_uint64]),
     "rv_worker_compose_frame": (C.c_int, [_vp, C.c_uint32, C.c_uint32, _u32p, C.POINTER(C.c_int)]),
     "rv_worker_set_fused": (C.c_int, [_vp, C.c_int]),
+    "rv_worker_render_progressive": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_int),
+                                               _vp, C.c_uint64, C.POINTER(C.c_uint64), C.c_int, _u32p]),
+    "rv_frame_output_read": (C.c_int, [_vp, C.c_uint64, C.c_uint32, C.c_uint32,
+                                       C.POINTER(FrameOutputHeader), _u32p]),
     "rv_worker_destroy": (None, [_vp]),
     "rv_dataset_open": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "rv_dataset_close": (None, [_vp]),

@@ -4,8 +4,8 @@ import ctypes as C
 
 import numpy as np
 
-from . import (BlockParams, BrickJob, DatasetInfo, FieldInfo, FrameBlock, FrameDesc, RenderState,
-               RvError, SubtreeInfo, check, lib, ptr, stream_ptr)
+from . import (BlockParams, BrickJob, DatasetInfo, FieldInfo, FrameBlock, FrameDesc, FrameOutputHeader,
+               RenderState, RvError, SubtreeInfo, check, lib, ptr, stream_ptr)
 
 
 def _np(a, dt):
@@ -495,6 +495,21 @@ class Worker:
     def set_fused(self, fused):
         check(lib().rv_worker_set_fused(self.h, int(fused)))
 
+    def render_progressive(self, level_min, level_max, split_count, out=None, frame_number=0,
+                           measure_time=False, cancel=False):
+        """rv_worker_render_progressive into the frame output block `out`
+        (a uint8 array of FRAME_OUTPUT_HEADER_BYTES + 4*W*H bytes, made if
+        None).  Returns (frames written, next frame number, out)."""
+        if out is None:
+            out = np.zeros(FRAME_OUTPUT_HEADER_BYTES + 4 * self.W * self.H, dtype=np.uint8)
+        fn = C.c_uint64(frame_number)
+        nw = C.c_uint32()
+        flag = C.c_int(1 if cancel else 0)
+        check(lib().rv_worker_render_progressive(self.h, level_min, level_max, split_count, C.byref(flag),
+                                                 out.ctypes.data, out.nbytes, C.byref(fn), int(measure_time),
+                                                 C.byref(nw)))
+        return nw.value, fn.value, out
+
     def compose_frame(self, level, split_count):
         out = np.zeros((self.H, self.W), dtype=np.uint32)
         flag = C.c_int(0)
@@ -511,6 +526,20 @@ class Worker:
             self.close()
         except Exception:
             pass
+
+
+# ---- frame output block -------------------------------------------------
+
+FRAME_OUTPUT_HEADER_BYTES = 24
+
+
+def read_frame_output(buf, vp_w, vp_h):
+    """(header dict, vp_h x vp_w ARGB pixels) of a frame output block."""
+    h = FrameOutputHeader()
+    pix = np.zeros((vp_h, vp_w), dtype=np.uint32)
+    check(lib().rv_frame_output_read(buf.ctypes.data, buf.nbytes, vp_w, vp_h, C.byref(h),
+                                     _p(pix, C.c_uint32)))
+    return dict(frame_number=h.frame_number, duration_ms=h.duration_ms, level=h.level), pix
 
 
 # ---- dataset files ------------------------------------------------------

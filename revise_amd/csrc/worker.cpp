@@ -6,6 +6,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -645,6 +647,30 @@ std::vector<uint32_t> composeFrame(VsWorkerHip& worker, unsigned level, unsigned
     return out;
 }
 
+unsigned renderProgressive(VsWorkerHip& worker, unsigned levelMin, unsigned levelMax,
+                           unsigned splitCount, const std::function<bool()>& isCancelled,
+                           uint64_t& frameNumber, bool measureTime,
+                           const std::function<void(const FrameOutputHeader&, const uint32_t*)>& writeFrame)
+{
+    unsigned written = 0;
+    for (unsigned level = levelMin; level <= levelMax && !isCancelled(); ++level) {
+        const auto t0 = std::chrono::steady_clock::now();
+        const std::vector<uint32_t> frame = composeFrame(worker, level, splitCount, isCancelled);
+        if (frame.empty() || isCancelled())
+            break;
+        FrameOutputHeader h;
+        h.frameNumber = frameNumber++;
+        if (measureTime) {
+            const std::chrono::duration<double, std::milli> ms = std::chrono::steady_clock::now() - t0;
+            h.durationMs = (uint64_t)std::llround(ms.count());
+        }
+        h.level = level;
+        writeFrame(h, frame.data());
+        ++written;
+    }
+    return written;
+}
+
 } // namespace revise
 
 // ---------------------------------------------------------------- C ABI
@@ -812,6 +838,53 @@ extern "C" int rv_worker_compose_frame(rv_worker* w, uint32_t level, uint32_t sp
     std::memcpy(frame, out.data(), 4 * out.size());
     return RV_OK;
     W_CATCH
+}
+
+static_assert(sizeof(revise::FrameOutputHeader) == sizeof(rv_frame_output_header),
+              "frame output header layout");
+
+extern "C" int rv_worker_render_progressive(rv_worker* w, uint32_t level_min, uint32_t level_max,
+                                            uint32_t split_count, const volatile int* cancel, void* out,
+                                            uint64_t out_size, uint64_t* frame_number, int measure_time,
+                                            uint32_t* frames_written)
+{
+    W_TRY
+    if (!w || !out || !frame_number || split_count == 0 || level_min > level_max) {
+        rv::set_error("rv_worker_render_progressive: bad arguments");
+        return RV_ERR_INVALID;
+    }
+    const uint64_t px = (uint64_t)w->state.viewportSize[0] * w->state.viewportSize[1];
+    if (out_size < sizeof(rv_frame_output_header) + 4 * px) {
+        rv::set_error("rv_worker_render_progressive: the output block cannot hold a frame");
+        return RV_ERR_INVALID;
+    }
+    unsigned char* dst = static_cast<unsigned char*>(out);
+    const unsigned n = revise::renderProgressive(
+        w->worker, level_min, level_max, split_count, [cancel] { return cancel && *cancel != 0; },
+        *frame_number, measure_time != 0,
+        [&](const revise::FrameOutputHeader& h, const uint32_t* pixels) {
+            std::memcpy(dst, &h, sizeof h);
+            std::memcpy(dst + sizeof h, pixels, 4 * px);
+        });
+    if (frames_written)
+        *frames_written = n;
+    return RV_OK;
+    W_CATCH
+}
+
+extern "C" int rv_frame_output_read(const void* in, uint64_t in_size, uint32_t vp_w, uint32_t vp_h,
+                                    rv_frame_output_header* header, uint32_t* pixels)
+{
+    const uint64_t px = (uint64_t)vp_w * vp_h;
+    if (!in || !header || in_size < sizeof(rv_frame_output_header) + (pixels ? 4 * px : 0)) {
+        rv::set_error("rv_frame_output_read: bad arguments");
+        return RV_ERR_INVALID;
+    }
+    const unsigned char* src = static_cast<const unsigned char*>(in);
+    std::memcpy(header, src, sizeof *header);
+    if (pixels)
+        std::memcpy(pixels, src + sizeof *header, 4 * px);
+    return RV_OK;
 }
 
 extern "C" int rv_worker_set_fused(rv_worker* w, int fused)

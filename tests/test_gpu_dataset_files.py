@@ -119,3 +119,35 @@ def test_worker_secondary_field_from_files(orc, tmp_path):
     assert np.array_equal(bits, vp[y:y + hh, x:x + ww])
     w.close()
     ds.close()
+
+
+def test_progressive_levels_from_files(orc, tmp_path):
+    """The level-of-detail loop over a dataset read from files: levels 0..2
+    (maxFullLevel 2, so every block of each level is a subtree root), each
+    level's frame equals the composed frame of that level, the frame output
+    block holds the last one with its number and level; a cancelled loop
+    writes nothing."""
+    meta_p, field_p, _, blocks = _dataset(tmp_path, 13)
+    fmin, fmax = datasets.field_range(blocks.values())
+    ds = api.Dataset(meta_p)
+    w = api.Worker(0)
+    assert ds.load_worker(w, field_p, 0, 2) == 1 + 8 + 64
+    W, H = 128, 96
+    tf = datasets.seeded_tf(13, 256)
+    w.set_state(datasets.orbit_camera(75.0), W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+    frames = [w.compose_frame(level, 3) for level in range(3)]
+    for level in range(3):
+        say("progressive level", level)
+        n, fn, out = w.render_progressive(level, level, 3, frame_number=10 + level)
+        hdr, pix = api.read_frame_output(out, W, H)
+        assert (n, fn) == (1, 11 + level)
+        assert hdr["frame_number"] == 10 + level and hdr["level"] == level and hdr["duration_ms"] == 0
+        assert np.array_equal(pix, frames[level])
+    n, fn, out = w.render_progressive(0, 2, 3, frame_number=100, measure_time=True)
+    hdr, pix = api.read_frame_output(out, W, H)
+    assert (n, fn) == (3, 103) and hdr["frame_number"] == 102 and hdr["level"] == 2
+    assert np.array_equal(pix, frames[2])
+    n, fn, out = w.render_progressive(0, 2, 3, frame_number=7, cancel=True)
+    assert (n, fn) == (0, 7) and not out.any()
+    w.close()
+    ds.close()
