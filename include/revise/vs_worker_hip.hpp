@@ -171,8 +171,11 @@ public:
     // kept on the device (devicePart) for on-device compositing.
     RgbaImagePart renderScenePart(const SubtreeSetDescription& subtrees,
                                   const std::atomic<bool>& isCancelled);
+    // hostDst != nullptr: the part's pixels go straight to hostDst (hostCap
+    // pixels available; pinned memory copies fastest) and bits stays empty.
     RgbaImagePart renderScenePart(const SubtreeSetDescription& subtrees,
-                                  const std::function<bool()>& isCancelled);
+                                  const std::function<bool()>& isCancelled,
+                                  uint32_t* hostDst = nullptr, uint64_t hostCap = 0);
     // The last part on the device: pixels (w*h ARGB) and {x, y, w, h}.
     const uint32_t* devicePart(std::array<int, 4>& rect) const;
     void renderFused(const SubtreeSetDescription& sub, const std::vector<uint32_t>& order,

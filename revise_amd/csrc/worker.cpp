@@ -354,7 +354,8 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
 }
 
 RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
-                                           const std::function<bool()>& isCancelled)
+                                           const std::function<bool()>& isCancelled,
+                                           uint32_t* hostDst, uint64_t hostCap)
 {
     RgbaImagePart part;
     m_partRect = {0, 0, 0, 0};
@@ -422,9 +423,16 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
              "hipMemcpy2DAsync");
     part.origin = {rect[0], rect[1]};
     part.size = {rect[2], rect[3]};
-    part.bits.resize(4 * n);
-    hipCheck(hipMemcpyAsync(part.bits.data(), m_part, 4 * n, hipMemcpyDeviceToHost, hs(stream())),
-             "hipMemcpyAsync");
+    if (hostDst) {
+        if (n > hostCap)
+            throw HipError(RV_ERR_INVALID, "renderScenePart: output capacity too small");
+        hipCheck(hipMemcpyAsync(hostDst, m_part, 4 * n, hipMemcpyDeviceToHost, hs(stream())),
+                 "hipMemcpyAsync");
+    } else {
+        part.bits.resize(4 * n);
+        hipCheck(hipMemcpyAsync(part.bits.data(), m_part, 4 * n, hipMemcpyDeviceToHost, hs(stream())),
+                 "hipMemcpyAsync");
+    }
     hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
     m_partRect = {rect[0], rect[1], rect[2], rect[3]};
     return part;
@@ -801,7 +809,9 @@ extern "C" int rv_worker_render_scene_part(rv_worker* w, uint32_t level, const u
         sub.boxMin[a] = bmin[a];
         sub.boxMax[a] = bmax[a];
     }
-    auto part = w->worker.renderScenePart(sub, [cancel] { return cancel && *cancel != 0; });
+    // the part goes straight into the caller's buffer
+    auto part = w->worker.renderScenePart(sub, [cancel] { return cancel && *cancel != 0; }, bits,
+                                          bits ? cap : 0);
     rect[0] = part.origin[0];
     rect[1] = part.origin[1];
     rect[2] = part.size[0];
@@ -810,13 +820,10 @@ extern "C" int rv_worker_render_scene_part(rv_worker* w, uint32_t level, const u
         rv::set_error("rv_worker_render_scene_part: cancelled");
         return RV_ERR_CANCELLED;
     }
-    const uint64_t n = part.bits.size() / 4;
-    if (n > cap || (n && !bits)) {
+    if (!bits && (uint64_t)part.size[0] * part.size[1] > 0) {
         rv::set_error("rv_worker_render_scene_part: output capacity too small");
         return RV_ERR_INVALID;
     }
-    if (n)
-        std::memcpy(bits, part.bits.data(), part.bits.size());
     return RV_OK;
     W_CATCH
 }
