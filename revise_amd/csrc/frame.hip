@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         if (item >= a.n_interp)
             break;
         const TileTask k = a.tasks[item];
-        interp_tile<kT>(k.job, k.local, k.e0, k.e1, A, lat);
+        interp_tile<kT>(k.job, k.local, k.e0, k.e_own, k.e1, A, lat);
         ++my_tiles; // (the queue barrier above protects A before its next use)
     }
     if (a.n_interp) {

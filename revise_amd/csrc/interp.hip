@@ -67,7 +67,7 @@ __global__ __launch_bounds__(NT) void interp_single_kernel(Job job, uint32_t nti
     if (t >= ntiles)
         return;
     const PlanView P = plan_view(job.plan);
-    interp_tile<NT>(job, t, P.tile_start[t], P.tile_start[t + 1], A, lat);
+    interp_tile<NT>(job, t, P.tile_start[t], P.own_end[t], P.tile_start[t + 1], A, lat);
 }
 
 template <uint32_t kFloats, uint32_t NT>
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(NT) void interp_batch_kernel(const TileTask* __rest
     if (t >= ntiles)
         return;
     const TileTask k = tasks[t];
-    interp_tile<NT>(k.job, k.local, k.e0, k.e1, A, lat);
+    interp_tile<NT>(k.job, k.local, k.e0, k.e_own, k.e1, A, lat);
 }
 
 bool big_class(uint32_t depth) { return tile_floats(geom_of(depth)) > kTileFloats8; }
@@ -125,7 +125,7 @@ int build_tile_tasks(const Job* jobs, uint32_t njobs, std::vector<TileTask>& tas
 {
     struct Table {
         uint32_t depth;
-        std::vector<uint32_t> start;
+        std::vector<uint32_t> start, own;
     };
     std::map<const uint8_t*, Table> seen;
     tasks.clear();
@@ -144,6 +144,12 @@ int build_tile_tasks(const Job* jobs, uint32_t njobs, std::vector<TileTask>& tas
                 tb.start.resize(head[2] + 1u);
                 RV_HIP(hipMemcpy(tb.start.data(), j.plan + kPlanHeader, 4ull * tb.start.size(),
                                  hipMemcpyDeviceToHost));
+                tb.own.resize(head[2]);
+                RV_HIP(hipMemcpy(tb.own.data(), j.plan + head[10], 4ull * tb.own.size(),
+                                 hipMemcpyDeviceToHost));
+                for (uint32_t t = 0; t < head[2]; ++t)
+                    if (tb.own[t] < tb.start[t] || tb.own[t] > tb.start[t + 1])
+                        return fail(RV_ERR_INVALID, "interpolation job: corrupt plan");
             }
             it = seen.emplace(j.plan, std::move(tb)).first;
         }
@@ -151,7 +157,7 @@ int build_tile_tasks(const Job* jobs, uint32_t njobs, std::vector<TileTask>& tas
         if (tb.depth != j.depth || tb.start.size() != g.ntiles + 1u)
             return fail(RV_ERR_INVALID, "interpolation job: plan was built for another depth");
         for (uint32_t t = 0; t < g.ntiles; ++t)
-            tasks.push_back(TileTask{j, t, tb.start[t], tb.start[t + 1]});
+            tasks.push_back(TileTask{j, t, tb.start[t], tb.start[t + 1], tb.own[t], 0u});
     }
     return RV_OK;
 }

@@ -44,8 +44,10 @@ struct Job {
 // searching the job list and reading the plan's tile table.
 struct TileTask {
     Job job;
-    uint32_t local; // tile index within its brick
+    uint32_t local;  // tile index within its brick
     uint32_t e0, e1; // plan entries [e0, e1) of the tile
+    uint32_t e_own;  // [e0, e_own): the nodes the tile stores
+    uint32_t pad;
 };
 
 // Builds the tile list of a batch, job after job (host).  Reads each distinct
@@ -162,6 +164,7 @@ __device__ inline void refine_box(float* A, uint32_t X, uint32_t T, uint32_t s_s
 struct PlanView {
     uint32_t depth, ntiles, nlat, entries;
     const uint32_t* tile_start;
+    const uint32_t* own_end;
     const uint32_t* lat;
     const uint32_t* idx;
     const uint16_t* lds;
@@ -176,6 +179,7 @@ __device__ __forceinline__ PlanView plan_view(const uint8_t* P)
     v.nlat = h[3];
     v.entries = h[4];
     v.tile_start = h + kPlanHeader / 4u;
+    v.own_end = reinterpret_cast<const uint32_t*>(P + h[10]);
     v.lat = reinterpret_cast<const uint32_t*>(P + h[5]);
     v.idx = reinterpret_cast<const uint32_t*>(P + h[6]);
     v.lds = reinterpret_cast<const uint16_t*>(P + h[7]);
@@ -184,10 +188,10 @@ __device__ __forceinline__ PlanView plan_view(const uint8_t* P)
 
 // One tile of a brick with NT threads (a multiple of 64); A holds
 // tile_floats(geom_of(depth)) floats, lat 64 floats; [e0, e1) is the tile's
-// plan entry range.
+// plan entry range, [e0, e_own) the nodes it stores.
 template <uint32_t NT>
-__device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t e0, uint32_t e1,
-                                   float* A, float* lat)
+__device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t e0, uint32_t e_own,
+                                   uint32_t e1, float* A, float* lat)
 {
     const PlanView P = plan_view(job.plan);
     const Geom g = geom_of(job.depth);
@@ -204,9 +208,9 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
     const uint32_t tp1 = T + 1u;
     const uint32_t tile_nodes = rowlen * tp1 * tp1;
 
-    // A "full" tile skips the interpolation; if a sparse value in it is NaN
-    // (undefined, so the reference interpolates it) the tile is redone as a
-    // partial one.
+    // A "full" tile skips the interpolation and scatters only the nodes it
+    // stores; if one of them is NaN (undefined, so the reference interpolates
+    // it) the tile is redone as a partial one.
     bool full = e1 - e0 == tile_nodes;
     for (int pass = 0; pass < 2; ++pass) {
         if (!full) {
@@ -239,7 +243,8 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
         // frame (streaming loads).
         constexpr uint32_t kUnroll = 8;
         bool saw_nan = false;
-        for (uint32_t k0 = e0 + threadIdx.x; k0 < e1; k0 += NT * kUnroll) {
+        const uint32_t eend = full ? e_own : e1;
+        for (uint32_t k0 = e0 + threadIdx.x; k0 < eend; k0 += NT * kUnroll) {
             uint32_t si[kUnroll];
             uint32_t off[kUnroll];
 #pragma unroll
@@ -247,7 +252,7 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
                 const uint32_t k = k0 + j * NT;
                 si[j] = kNoIndex;
                 off[j] = 0;
-                if (k < e1) {
+                if (k < eend) {
                     si[j] = P.idx[k];
                     off[j] = P.lds[k];
                 }

@@ -19,7 +19,7 @@
 
 namespace rv {
 
-constexpr uint32_t kPlanMagic = 0x32504e52u; // "RNP2"
+constexpr uint32_t kPlanMagic = 0x33504e52u; // "RNP3"
 constexpr uint32_t kPlanHeader = 64;          // bytes before tile_start[]
 
 struct Geom {

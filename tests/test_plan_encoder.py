@@ -7,7 +7,7 @@ import pytest
 
 from revise_amd import api, datasets
 
-MAGIC = 0x32504E52
+MAGIC = 0x33504E52
 
 
 def _geo(d, T):
@@ -17,13 +17,14 @@ def _geo(d, T):
 
 def _decode(buf):
     h = buf[:64].view(np.uint32)
-    magic, depth, ntiles, nlat, entries, lat_off, idx_off, lds_off, T, X = (int(v) for v in h[:10])
+    magic, depth, ntiles, nlat, entries, lat_off, idx_off, lds_off, T, X, own_off = (int(v) for v in h[:11])
     assert T == min(1 << depth, T) and X == min(1 << depth, 64)
     start = buf[64:64 + 4 * (ntiles + 1)].view(np.uint32)
+    own = buf[own_off:own_off + 4 * ntiles].view(np.uint32)
     lat = buf[lat_off:lat_off + 4 * nlat].view(np.uint32)
     idx = buf[idx_off:idx_off + 4 * entries].view(np.uint32)
     lds = buf[lds_off:lds_off + 2 * entries].view(np.uint16)
-    return magic, depth, ntiles, start, lat, idx, lds, T
+    return magic, depth, ntiles, start, lat, idx, lds, T, own
 
 
 def _tree(seed, depth, p):
@@ -39,7 +40,7 @@ def test_plan_matches_brute_force_tiles(seed, depth, p):
     d, n2i = _tree(seed, depth, p)
     buf = api.interp_plan_encode(n2i, d)
     assert buf.size == api.interp_plan_bytes(n2i, d) and buf.size % 16 == 0
-    magic, depth_p, ntiles, start, lat, idx, lds, T = _decode(buf)
+    magic, depth_p, ntiles, start, lat, idx, lds, T, own = _decode(buf)
     # the tile extent is 8 or 16 cells (build option), 16 at depth 8, at most 2^d
     assert T == min(1 << d, 16 if d == 8 else T) and T in (1 << min(d, 3), 1 << min(d, 4))
     X, tx, tyz, m = _geo(d, T)
@@ -53,9 +54,14 @@ def test_plan_matches_brute_force_tiles(seed, depth, p):
         x0, y0, z0 = ix * X, iy * T, iz * T
         inside = ((c[:, 0] >= x0) & (c[:, 0] <= x0 + X) & (c[:, 1] >= y0) & (c[:, 1] <= y0 + T) &
                   (c[:, 2] >= z0) & (c[:, 2] <= z0 + T))
-        want = np.nonzero(inside)[0]
+        # stored: the half-open box, and the brick's far faces in the last tiles
+        stored = (inside & ((c[:, 0] < x0 + X) | (ix == tx - 1)) & ((c[:, 1] < y0 + T) | (iy == tyz - 1)) &
+                  ((c[:, 2] < z0 + T) | (iz == tyz - 1)))
+        want = np.concatenate([np.nonzero(stored)[0], np.nonzero(inside & ~stored)[0]])
         got = idx[start[t]:start[t + 1]]
-        assert np.array_equal(got, want)  # every node of the closed box, increasing index
+        # every node of the closed box: the stored ones, then the shared faces, each in increasing index
+        assert np.array_equal(got, want)
+        assert own[t] == start[t] + int(stored.sum())
         lc = c[want] - np.array([x0, y0, z0])
         assert np.array_equal(lds[start[t]:start[t + 1]], (lc[:, 2] * tp1 + lc[:, 1]) * rowlen + lc[:, 0])
     if m >= 2:
@@ -85,9 +91,9 @@ def test_plan_rejects_depth_above_8():
 
 def test_plan_of_empty_subtree():
     buf = api.interp_plan_encode(np.zeros((0, 3), dtype=np.uint16), 6)
-    _, _, ntiles, start, lat, idx, _, T = _decode(buf)
+    _, _, ntiles, start, lat, idx, _, T, own = _decode(buf)
     m = 64 // T
-    assert ntiles == m * m and np.all(start == 0) and idx.size == 0
+    assert ntiles == m * m and np.all(start == 0) and np.all(own == 0) and idx.size == 0
     assert lat.size == (m + 1) ** 3 and np.all(lat == 0xFFFFFFFF)
 
 
@@ -96,7 +102,7 @@ def test_plan_keeps_last_of_duplicate_coordinates():
     twice ends with its last value; the plan lists only that occurrence."""
     n2i = np.array([[0, 0, 0], [1, 0, 0], [0, 0, 0], [2, 2, 2], [1, 0, 0]], dtype=np.uint16)
     buf = api.interp_plan_encode(n2i, 1)
-    _, _, ntiles, start, _, idx, lds, _ = _decode(buf)
+    _, _, ntiles, start, _, idx, lds, _, _ = _decode(buf)
     assert ntiles == 1
     assert idx.tolist() == [2, 3, 4]
     assert lds.tolist() == [0, 26, 1]
