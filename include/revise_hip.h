@@ -425,10 +425,13 @@ int rv_frame_get_tile(const rv_frame_plan* plan, uint32_t* tile_w, uint32_t* til
  * each block of the bin's list with the lanes whose pixels the block
  * covers.  LANES (Argb mode): each lane walks its own pixel's blocks, so a
  * wave marches different blocks at once -- faster when blocks are a few
- * pixels wide.  Both give identical images (and adjoint records). */
-enum { RV_FRAME_SCHEDULE_BINS = 0, RV_FRAME_SCHEDULE_LANES = 1 };
+ * pixels wide.  HYBRID (Argb mode): LANES for bins listing at least
+ * rv_frame_set_hybrid_min blocks (default 16), BINS for the others.  All
+ * give identical images (and adjoint records). */
+enum { RV_FRAME_SCHEDULE_BINS = 0, RV_FRAME_SCHEDULE_LANES = 1, RV_FRAME_SCHEDULE_HYBRID = 2 };
 int rv_frame_set_schedule(rv_frame_plan* plan, uint32_t schedule);
 int rv_frame_get_schedule(const rv_frame_plan* plan, uint32_t* schedule);
+int rv_frame_set_hybrid_min(rv_frame_plan* plan, uint32_t blocks);
 /* Pick the bins' wave footprint (8x8, 16x4, 4x16, 32x2), the persistent
  * grid (resident workgroups per CU) of the mode's kernel (interpolate or
  * resident bricks) and, in Argb mode, the schedule, by timing candidates

@@ -228,7 +228,7 @@ def subtree_n2i_device(d_children, nnodes, depth, d_out, stream=None):
 # s3vs::FieldMode (FieldMode.hpp:26-51)
 FIELD_ISOSURFACE, FIELD_ISOSURFACES, FIELD_MIP, FIELD_ARGB, FIELD_ARGB_LIGHT, FIELD_DOMAIN_VOXELS = 0, 1, 2, 3, 4, 5
 FIELD_VALUE_ON_ISOSURFACE, FIELD_VALUE_ON_ISOSURFACES = 6, 7
-SCHEDULE_BINS, SCHEDULE_LANES = 0, 1
+SCHEDULE_BINS, SCHEDULE_LANES, SCHEDULE_HYBRID = 0, 1, 2
 
 
 def _levels(levels):
@@ -394,6 +394,10 @@ class FramePlan:
         """SCHEDULE_BINS (the wave marches each block of its bin) or
         SCHEDULE_LANES (each lane walks its own pixel's blocks; Argb)."""
         check(lib().rv_frame_set_schedule(self.h, schedule))
+
+    def set_hybrid_min(self, blocks):
+        """SCHEDULE_HYBRID: per-lane walk for bins listing at least `blocks` blocks."""
+        check(lib().rv_frame_set_hybrid_min(self.h, blocks))
 
     def schedule(self):
         v = C.c_uint32()

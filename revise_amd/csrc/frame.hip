@@ -72,6 +72,7 @@ struct FrameArgs {
     uint32_t* counters;            // [0] brick queue, [1] error, [2] brick tiles published, [3] bin queue
     float4* tape;                  // adjoint record per (block, pixel of its rectangle), or NULL
     const uint64_t* rec_off;       // per block: offset of its rectangle in the tape
+    uint32_t lanes_min;            // per-lane walk kernels: bins with at least this many blocks
 };
 
 constexpr uint32_t kSpinLimit = 1u << 22;
@@ -217,7 +218,8 @@ __device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4*
 }
 
 // Screen phase of a wave: take bins from the bin queue until it is empty.
-// kLanes: the per-lane walk (Argb only).
+// kLanes: the per-lane walk (Argb only) for bins listing at least
+// a.lanes_min blocks, the bin walk for the others.
 template <class BrickT, uint32_t kMode, bool kLanes>
 __device__ void render_bins(const FrameArgs& a, const float4* tfs, const float* lvs)
 {
@@ -228,10 +230,13 @@ __device__ void render_bins(const FrameArgs& a, const float4* tfs, const float* 
         bin = uniform(bin);
         if (bin >= a.nbins)
             break;
-        if constexpr (kLanes)
-            render_bin_lanes<BrickT>(a, bin, tfs);
-        else
-            render_bin<BrickT, kMode>(a, bin, tfs, lvs);
+        if constexpr (kLanes) {
+            if (a.bin_off[bin + 1] - a.bin_off[bin] >= a.lanes_min) {
+                render_bin_lanes<BrickT>(a, bin, tfs);
+                continue;
+            }
+        }
+        render_bin<BrickT, kMode>(a, bin, tfs, lvs);
     }
 }
 
@@ -539,7 +544,7 @@ void launch_mode(const rv_frame_plan* P, const FrameArgs& a, int interpolate, ui
                  hipStream_t s)
 {
     if constexpr (kMode == kModeArgb) {
-        if (P->schedule == RV_FRAME_SCHEDULE_LANES) {
+        if (P->schedule != RV_FRAME_SCHEDULE_BINS) {
             launch_sched<kMode, true>(P, a, interpolate, grid, s);
             return;
         }
@@ -611,8 +616,9 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     // the interpolating kernel waits for its bricks: its grid never exceeds
     // what the mode's kernel keeps resident
     uint32_t cap = P->mode_max_wgs;
-    if (P->schedule == RV_FRAME_SCHEDULE_LANES && P->field_mode == RV_FIELD_ARGB)
+    if (P->schedule != RV_FRAME_SCHEDULE_BINS && P->field_mode == RV_FIELD_ARGB)
         cap = std::min(cap, P->lanes_max_wgs);
+    a.lanes_min = P->schedule == RV_FRAME_SCHEDULE_HYBRID ? P->hybrid_min : 0u;
     const uint32_t grid = std::min(P->grid, P->cus * cap);
     switch (P->field_mode) {
     case RV_FIELD_ARGB_LIGHT:
@@ -935,7 +941,8 @@ extern "C" int rv_frame_get_tile(const rv_frame_plan* P, uint32_t* tile_w, uint3
 extern "C" int rv_frame_set_schedule(rv_frame_plan* P, uint32_t schedule)
 {
     RV_REQUIRE(P, "rv_frame_set_schedule: null plan");
-    RV_REQUIRE(schedule == RV_FRAME_SCHEDULE_BINS || schedule == RV_FRAME_SCHEDULE_LANES,
+    RV_REQUIRE(schedule == RV_FRAME_SCHEDULE_BINS || schedule == RV_FRAME_SCHEDULE_LANES ||
+                   schedule == RV_FRAME_SCHEDULE_HYBRID,
                "rv_frame_set_schedule: unknown schedule");
     if (schedule != P->schedule) {
         // a captured graph holds the previous schedule's kernels
@@ -956,6 +963,23 @@ extern "C" int rv_frame_get_schedule(const rv_frame_plan* P, uint32_t* schedule)
 {
     RV_REQUIRE(P && schedule, "rv_frame_get_schedule: null pointer");
     *schedule = P->schedule;
+    return RV_OK;
+}
+
+extern "C" int rv_frame_set_hybrid_min(rv_frame_plan* P, uint32_t blocks)
+{
+    RV_REQUIRE(P, "rv_frame_set_hybrid_min: null plan");
+    if (blocks != P->hybrid_min) {
+        if (P->graph_exec) {
+            (void)hipGraphExecDestroy(P->graph_exec);
+            P->graph_exec = nullptr;
+        }
+        if (P->graph) {
+            (void)hipGraphDestroy(P->graph);
+            P->graph = nullptr;
+        }
+        P->hybrid_min = blocks;
+    }
     return RV_OK;
 }
 
@@ -996,12 +1020,15 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
                 valid_footprint(w, h) && gp != std::string::npos &&
                 std::sscanf(line.c_str() + gp + 14, "%u", &g) == 1 && g >= 1 &&
                 g <= (interpolate ? P->grid_max_wgs : P->render_max_wgs)) {
-                unsigned rw = 0, sch = 0;
+                unsigned rw = 0, sch = 0, hmin = 0;
                 const size_t sp = line.find("\"schedule\": ");
+                const size_t hp = line.find("\"hybrid_min\": ");
+                if (hp != std::string::npos)
+                    (void)std::sscanf(line.c_str() + hp + 14, "%u", &hmin);
                 if (sp != std::string::npos && std::sscanf(line.c_str() + sp + 12, "%u", &sch) == 1 &&
-                    (sch == RV_FRAME_SCHEDULE_BINS ||
-                     (sch == RV_FRAME_SCHEDULE_LANES && P->field_mode == RV_FIELD_ARGB)))
-                    (void)rv_frame_set_schedule(P, sch);
+                    (sch == RV_FRAME_SCHEDULE_BINS || P->field_mode == RV_FIELD_ARGB) &&
+                    rv_frame_set_schedule(P, sch) == RV_OK)
+                    (void)rv_frame_set_hybrid_min(P, hmin);
                 const size_t rp = line.find("\"render_waves\": ");
                 if (interpolate && rp != std::string::npos &&
                     std::sscanf(line.c_str() + rp + 16, "%u", &rw) == 1 && rw >= 1 && rw <= kT / 64u)
@@ -1030,9 +1057,14 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     std::vector<uint32_t> rws = {kT / 64u};
     if (interpolate)
         rws = {4u, 6u, kT / 64u};
-    std::vector<uint32_t> scheds = {RV_FRAME_SCHEDULE_BINS};
+    // schedules: bin walk, per-lane walk, and the hybrid with the per-lane
+    // walk for bins listing at least 48 blocks, as indices into kSched
+    static const uint32_t kSched[3][2] = {{RV_FRAME_SCHEDULE_BINS, 0u},
+                                          {RV_FRAME_SCHEDULE_LANES, 0u},
+                                          {RV_FRAME_SCHEDULE_HYBRID, 48u}};
+    std::vector<uint32_t> scheds = {0u};
     if (P->field_mode == RV_FIELD_ARGB)
-        scheds.push_back(RV_FRAME_SCHEDULE_LANES);
+        scheds = {0u, 1u, 2u};
     std::ostringstream times;
     // median of 3 timed frames of camera cam in a configuration
     auto time_cfg = [&](uint32_t sch, uint32_t tw, uint32_t th, uint32_t g, uint32_t rw, uint32_t cam,
@@ -1040,7 +1072,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
         grid = P->cus * g;
         if (interpolate)
             P->frame_rw = rw;
-        (void)rv_frame_set_schedule(P, sch);
+        (void)rv_frame_set_schedule(P, kSched[sch][0]);
+        (void)rv_frame_set_hybrid_min(P, kSched[sch][1]);
         float t[3];
         int rc = launch_frame(P, cam, d_argb, nullptr, interpolate, s, tw, th);
         for (int r = 0; r < 3 && !rc; ++r) {
@@ -1057,7 +1090,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     struct Choice {
         float ms = 1e30f;
         uint32_t w = 8, h = 8, g = 1, rw = kT / 64u;
-    } pick[2];
+    } pick[3];
     for (uint32_t sch : scheds)
     for (const auto& c : cand)
     for (uint32_t g : wgs)
@@ -1069,8 +1102,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
             (void)hipEventDestroy(e1);
             return rc;
         }
-        times << (times.tellp() > 0 ? ", " : "") << "\"" << (sch == RV_FRAME_SCHEDULE_LANES ? "L" : "B")
-              << c[0] << "x" << c[1] << "/" << g;
+        static const char* kName[3] = {"B", "L", "H48/"};
+        times << (times.tellp() > 0 ? ", " : "") << "\"" << kName[sch] << c[0] << "x" << c[1] << "/" << g;
         if (interpolate)
             times << "/" << rw;
         times << "\": " << ms;
@@ -1080,17 +1113,18 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     // The schedules' best configurations on camera 0 are compared again over
     // up to 4 cameras spread through the plan's set: how much a per-lane walk
     // gains depends on the view.
-    uint32_t bsched = pick[RV_FRAME_SCHEDULE_LANES].ms < pick[RV_FRAME_SCHEDULE_BINS].ms
-                          ? RV_FRAME_SCHEDULE_LANES
-                          : RV_FRAME_SCHEDULE_BINS;
-    if (scheds.size() == 2 && P->ncams > 1) {
+    uint32_t bsched = 0;
+    for (uint32_t sch : scheds)
+        if (pick[sch].ms < pick[bsched].ms)
+            bsched = sch;
+    if (scheds.size() > 1 && P->ncams > 1) {
         std::vector<uint32_t> cams;
         for (uint32_t q = 0; q < 4; ++q) {
             const uint32_t cam = (uint32_t)((uint64_t)P->ncams * q / 4);
             if (std::find(cams.begin(), cams.end(), cam) == cams.end())
                 cams.push_back(cam);
         }
-        float tot[2] = {0.f, 0.f};
+        float tot[3] = {0.f, 0.f, 0.f};
         for (uint32_t sch : scheds)
             for (uint32_t cam : cams) {
                 float ms = 0.f;
@@ -1103,17 +1137,20 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
                 }
                 tot[sch] += ms;
             }
-        times << ", \"B over " << cams.size() << " cameras\": " << tot[0] << ", \"L over "
-              << cams.size() << " cameras\": " << tot[1];
-        bsched = tot[RV_FRAME_SCHEDULE_LANES] < tot[RV_FRAME_SCHEDULE_BINS] ? RV_FRAME_SCHEDULE_LANES
-                                                                            : RV_FRAME_SCHEDULE_BINS;
+        bsched = scheds[0];
+        for (uint32_t sch : scheds) {
+            times << ", \"schedule " << sch << " over " << cams.size() << " cameras\": " << tot[sch];
+            if (tot[sch] < tot[bsched])
+                bsched = sch;
+        }
     }
     const Choice& bc = pick[bsched];
     const uint32_t bw = bc.w, bh = bc.h;
     grid = P->cus * bc.g;
     if (interpolate)
         P->frame_rw = bc.rw;
-    (void)rv_frame_set_schedule(P, bsched);
+    (void)rv_frame_set_schedule(P, kSched[bsched][0]);
+    (void)rv_frame_set_hybrid_min(P, kSched[bsched][1]);
     const uint32_t bg = bc.g;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -1130,7 +1167,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
         std::ofstream f(cache_path, std::ios::app);
         f << "{\"key\": \"" << key << "\", \"tile\": [" << bw << ", " << bh
           << "], \"wgs_per_cu\": " << bg << ", \"render_waves\": " << P->frame_rw
-          << ", \"schedule\": " << P->schedule << ", \"ms\": {"
+          << ", \"schedule\": " << P->schedule << ", \"hybrid_min\": " << P->hybrid_min << ", \"ms\": {"
           << times.str() << "}}\n";
     }
     return RV_OK;

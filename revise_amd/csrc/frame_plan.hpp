@@ -78,6 +78,7 @@ struct rv_frame_plan {
     uint32_t mode_max_wgs = 0; // resident interpolating workgroups per CU in field_mode
     uint32_t schedule = RV_FRAME_SCHEDULE_BINS; // how a wave walks its screen bin (Argb)
     uint32_t lanes_max_wgs = 0; // resident interpolating workgroups per CU, per-lane walk
+    uint32_t hybrid_min = 16;   // hybrid schedule: per-lane walk for bins of at least this many blocks
     // adjoint tape: per camera, per block, the offset of the block's view
     // rectangle in the tape (float4 per pixel); the blocks' 8 x 8 tiles
     // as (block, tile) pairs, grouped by block

@@ -72,7 +72,15 @@ def _oracle_frame(orc, blocks, bricks_np, M, W, H, fmin, fmax, tf, acc_cut):
     return orc.render_float_composite(M, W, H, FOV, lst, fmin, fmax, THR, Q, CUT, tf, acc_cut)
 
 
-SCHEDULES = [api.SCHEDULE_BINS, api.SCHEDULE_LANES]
+SCHEDULES = [api.SCHEDULE_BINS, api.SCHEDULE_LANES, api.SCHEDULE_HYBRID]
+
+
+def _set_schedule(fp, schedule):
+    """The hybrid mixes both walks in one frame: a low threshold sends the
+    crowded bins to the per-lane walk."""
+    fp.set_schedule(schedule)
+    if schedule == api.SCHEDULE_HYBRID:
+        fp.set_hybrid_min(5)
 
 
 @pytest.mark.parametrize("schedule", SCHEDULES)
@@ -82,7 +90,7 @@ def test_frame_uniform_matches_oracle(orc, fp16, acc_cut, schedule):
     blocks = _uniform_blocks(2, 4, seed=5)
     W, H = 200, 150
     fp, tf, fmin, fmax, dev = _setup(blocks, W, H, fp16, acc_cut, seed=5)
-    fp.set_schedule(schedule)
+    _set_schedule(fp, schedule)
     assert fp.schedule() == schedule
     cams = np.stack([datasets.orbit_camera(a) for a in (15.0, 140.0)])
     fp.set_cameras(cams)
@@ -112,7 +120,7 @@ def test_frame_mixed_levels_matches_oracle(orc, schedule):
     say("mixed blocks", len(blocks), "schedule", schedule)
     W, H = 180, 140
     fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=8)
-    fp.set_schedule(schedule)
+    _set_schedule(fp, schedule)
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((70.0, 25.0), (250.0, -40.0))])
     fp.set_cameras(cams)
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
@@ -136,7 +144,7 @@ def test_frame_graph_replays_match_eager(orc, schedule):
     blocks = _uniform_blocks(2, 4, seed=6)
     W, H = 160, 120
     fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=6)
-    fp.set_schedule(schedule)
+    _set_schedule(fp, schedule)
     cams = np.stack([datasets.orbit_camera(a) for a in (0.0, 45.0, 90.0)])
     fp.set_cameras(cams)
     eager = []
@@ -192,7 +200,7 @@ def test_frame_small_blocks_every_footprint(orc, tile, schedule):
     W, H = 150, 110
     fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.999, seed=11, tile=tile)
     assert fp.tile() == tile
-    fp.set_schedule(schedule)
+    _set_schedule(fp, schedule)
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((20.0, 10.0), (200.0, -30.0))])
     fp.set_cameras(cams)
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
