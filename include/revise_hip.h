@@ -442,13 +442,21 @@ int rv_frame_autotune(rv_frame_plan* plan, uint32_t* d_argb, int interpolate,
                       const char* cache_path, rv_stream_t stream, uint32_t* tile_w,
                       uint32_t* tile_h);
 /* Field mode of later frames (default RV_FIELD_ARGB; levels as in
- * rv_render_block): Argb, ArgbLight, Isosurface, Isosurfaces and
- * DomainVoxels segments are composited front to back (blendBehind); in
+ * rv_render_block): Argb, ArgbLight, Isosurface, Isosurfaces, DomainVoxels
+ * and (with rv_frame_set_secondary) ValueOnIsosurface(s) segments are
+ * composited front to back (blendBehind); in
  * MaxIntensityProjection the frame's d_argb receives the float bits of the
  * largest normalised value over the blocks (NaN: none), the order of blocks
  * being immaterial.  The tape and the adjoint are Argb only. */
 int rv_frame_set_field_mode(rv_frame_plan* plan, uint32_t field_mode, float iso_opacity,
                             const float* levels_rel, uint32_t level_count);
+/* The secondary field of the ValueOnIsosurface(s) modes (set before
+ * selecting them): per block its brick (the primary bricks' type and
+ * layout) and, for interpolating frames, its sparse values (NULL: resident
+ * frames only), and the field's range [smin, smax].  d_bricks2 = NULL
+ * removes it. */
+int rv_frame_set_secondary(rv_frame_plan* plan, const float* const* d_sparse2, void* const* d_bricks2,
+                           float smin, float smax);
 void rv_frame_plan_destroy(rv_frame_plan* plan);
 
 /* Adjoint ray-cast (no reference counterpart: the reference renders only).

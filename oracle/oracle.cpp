@@ -1661,14 +1661,15 @@ extern "C" void orc_render_share(const orc_frame_desc* d, const float* const* br
     });
 }
 
-extern "C" void orc_render_float_composite_mode(const float* view16, uint32_t vp_w, uint32_t vp_h,
-                                                float fov_y_deg, const orc_block_ref* blocks,
-                                                uint32_t nblocks, float fmin, float fmax,
-                                                float threshold_rel, float quality,
-                                                float alpha_cutoff, float acc_cutoff,
-                                                const float* tf, uint32_t tf_n, uint32_t mode,
-                                                float iso_opacity, const float* levels_rel,
-                                                uint32_t level_count, float* out, uint32_t nthreads)
+extern "C" void orc_render_float_composite_mode2(const float* view16, uint32_t vp_w, uint32_t vp_h,
+                                                 float fov_y_deg, const orc_block_ref* blocks,
+                                                 uint32_t nblocks, float fmin, float fmax,
+                                                 float threshold_rel, float quality,
+                                                 float alpha_cutoff, float acc_cutoff,
+                                                 const float* tf, uint32_t tf_n, uint32_t mode,
+                                                 float iso_opacity, const float* levels_rel,
+                                                 uint32_t level_count, float smin, float smax,
+                                                 float* out, uint32_t nthreads)
 {
     const std::vector<float> lv = make_levels(fmin, fmax, levels_rel, level_count);
     std::vector<orc_block_params> params(nblocks);
@@ -1718,9 +1719,10 @@ extern "C" void orc_render_float_composite_mode(const float* view16, uint32_t vp
                     if (!r.hit)
                         continue;
                     float seg[4];
-                    if (mode == kIsos && lv.empty())
+                    if (multi_level(mode) && lv.empty())
                         continue; /* no levels: nothing drawn */
-                    segment_mode(p, der[bi], blocks[bi].brick, tf, tf_n, mode, iso_opacity, lv, r, seg);
+                    segment_mode(p, der[bi], blocks[bi].brick, tf, tf_n, mode, iso_opacity, lv, r, seg,
+                                 Second{blocks[bi].brick2, smin, smax});
                     /* acc (in front) composited over seg: blendBehind(acc, seg) */
                     float da = seg[3] * (1.f - acc[3]);
                     float a = acc[3] + da;
@@ -1737,6 +1739,20 @@ extern "C" void orc_render_float_composite_mode(const float* view16, uint32_t vp
             }
         }
     });
+}
+
+extern "C" void orc_render_float_composite_mode(const float* view16, uint32_t vp_w, uint32_t vp_h,
+                                                float fov_y_deg, const orc_block_ref* blocks,
+                                                uint32_t nblocks, float fmin, float fmax,
+                                                float threshold_rel, float quality,
+                                                float alpha_cutoff, float acc_cutoff,
+                                                const float* tf, uint32_t tf_n, uint32_t mode,
+                                                float iso_opacity, const float* levels_rel,
+                                                uint32_t level_count, float* out, uint32_t nthreads)
+{
+    orc_render_float_composite_mode2(view16, vp_w, vp_h, fov_y_deg, blocks, nblocks, fmin, fmax,
+                                     threshold_rel, quality, alpha_cutoff, acc_cutoff, tf, tf_n, mode,
+                                     iso_opacity, levels_rel, level_count, 0.f, 1.f, out, nthreads);
 }
 
 extern "C" void orc_render_float_composite(const float* view16, uint32_t vp_w, uint32_t vp_h,

@@ -198,6 +198,7 @@ typedef struct orc_block_ref {
     uint32_t index[3];
     uint32_t tex_depth;
     const float* brick;
+    const float* brick2; /* secondary field brick (ValueOnIsosurface(s)), or NULL */
 } orc_block_ref;
 
 void orc_render_float_composite(const float* view16, uint32_t vp_w, uint32_t vp_h, float fov_y_deg,
@@ -205,6 +206,16 @@ void orc_render_float_composite(const float* view16, uint32_t vp_w, uint32_t vp_
                                 float fmax, float threshold_rel, float quality, float alpha_cutoff,
                                 float acc_cutoff, const float* tf, uint32_t tf_n, float* out_rgba,
                                 uint32_t nthreads);
+/* The same with the secondary field range of the ValueOnIsosurface(s)
+ * modes (the blocks' brick2). */
+void orc_render_float_composite_mode2(const float* view16, uint32_t vp_w, uint32_t vp_h,
+                                      float fov_y_deg, const orc_block_ref* blocks_b2f,
+                                      uint32_t nblocks, float fmin, float fmax, float threshold_rel,
+                                      float quality, float alpha_cutoff, float acc_cutoff,
+                                      const float* tf, uint32_t tf_n, uint32_t mode,
+                                      float iso_opacity, const float* levels_rel,
+                                      uint32_t level_count, float smin, float smax, float* out_rgba,
+                                      uint32_t nthreads);
 /* The same in a field mode: Argb, ArgbLight and Isosurface segments
  * composited front to back; MaxIntensityProjection writes the float of the
  * largest normalised value (NaN: none) to all 4 channels. */

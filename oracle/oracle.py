@@ -42,7 +42,7 @@ class FrameDesc(C.Structure):
 
 class BlockRef(C.Structure):
     _fields_ = [("level", C.c_uint32), ("index", C.c_uint32 * 3), ("tex_depth", C.c_uint32),
-                ("brick", C.POINTER(C.c_float))]
+                ("brick", C.POINTER(C.c_float)), ("brick2", C.POINTER(C.c_float))]
 
 
 _lib = None
@@ -114,6 +114,12 @@ def lib():
                                                       C.c_float, C.c_float, C.c_float, C.c_float,
                                                       C.c_float, f32p, C.c_uint32, C.c_uint32, C.c_float,
                                                       f32p, C.c_uint32, f32p, C.c_uint32]
+        L.orc_render_float_composite_mode2.argtypes = [f32p, C.c_uint32, C.c_uint32, C.c_float,
+                                                       C.POINTER(BlockRef), C.c_uint32, C.c_float,
+                                                       C.c_float, C.c_float, C.c_float, C.c_float,
+                                                       C.c_float, f32p, C.c_uint32, C.c_uint32, C.c_float,
+                                                       f32p, C.c_uint32, C.c_float, C.c_float, f32p,
+                                                       C.c_uint32]
         _lib = L
     return _lib
 
@@ -346,15 +352,17 @@ def render_share(view16, vp_w, vp_h, fov, level, box_min, box_max, tex_depth, fm
 
 def render_float_composite(view16, vp_w, vp_h, fov, blocks_b2f, fmin, fmax, thr_rel, quality,
                            alpha_cutoff, tf, acc_cutoff=2.0, nthreads=None, mode=3, iso_opacity=1.0,
-                           levels=None):
-    """blocks_b2f: list of (level, (ix,iy,iz), tex_depth, brick) in back-to-front order."""
+                           levels=None, srange=(0.0, 1.0)):
+    """blocks_b2f: list of (level, (ix,iy,iz), tex_depth, brick[, brick2]) in back-to-front
+    order; brick2 (the secondary field) for the ValueOnIsosurface(s) modes."""
     v = np.ascontiguousarray(view16, dtype=np.float32).reshape(16)
     tf = np.ascontiguousarray(tf, dtype=np.float32)
     keep = []
     lv = np.ascontiguousarray(levels if levels is not None else [0.0], dtype=np.float32)
     nlv = 0 if levels is None else len(levels)
     arr = (BlockRef * len(blocks_b2f))()
-    for i, (lvl, idx, td, brick) in enumerate(blocks_b2f):
+    for i, blk in enumerate(blocks_b2f):
+        lvl, idx, td, brick = blk[:4]
         b = np.ascontiguousarray(brick, dtype=np.float32)
         keep.append(b)
         arr[i].level = lvl
@@ -362,9 +370,17 @@ def render_float_composite(view16, vp_w, vp_h, fov, blocks_b2f, fmin, fmax, thr_
             arr[i].index[a] = idx[a]
         arr[i].tex_depth = td
         arr[i].brick = _p(b, C.c_float)
+        if len(blk) > 4 and blk[4] is not None:
+            b2 = np.ascontiguousarray(blk[4], dtype=np.float32)
+            assert b2.shape == b.shape
+            keep.append(b2)
+            arr[i].brick2 = _p(b2, C.c_float)
+        elif mode in (6, 7):
+            raise ValueError("the ValueOnIsosurface(s) modes need each block's secondary brick")
     out = np.zeros((vp_h, vp_w, 4), dtype=np.float32)
-    lib().orc_render_float_composite_mode(_p(v, C.c_float), vp_w, vp_h, fov, arr, len(blocks_b2f), fmin,
-                                          fmax, thr_rel, quality, alpha_cutoff, acc_cutoff,
-                                          _p(tf, C.c_float), tf.shape[0], mode, iso_opacity,
-                                          _p(lv, C.c_float), nlv, _p(out, C.c_float), nthreads or os.cpu_count())
+    lib().orc_render_float_composite_mode2(_p(v, C.c_float), vp_w, vp_h, fov, arr, len(blocks_b2f), fmin,
+                                           fmax, thr_rel, quality, alpha_cutoff, acc_cutoff,
+                                           _p(tf, C.c_float), tf.shape[0], mode, iso_opacity,
+                                           _p(lv, C.c_float), nlv, srange[0], srange[1], _p(out, C.c_float),
+                                           nthreads or os.cpu_count())
     return out

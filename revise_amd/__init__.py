@@ -165,6 +165,7 @@ _SIGS = {
     "rv_frame_autotune": (C.c_int, [_vp, _vp, C.c_int, C.c_char_p, _vp, _u32p, _u32p]),
     "rv_frame_set_tape": (C.c_int, [_vp, C.c_int]),
     "rv_frame_set_field_mode": (C.c_int, [_vp, C.c_uint32, C.c_float, _f32p, C.c_uint32]),
+    "rv_frame_set_secondary": (C.c_int, [_vp, _vp, _vp, C.c_float, C.c_float]),
     "rv_frame_set_grad_bricks": (C.c_int, [_vp, _vp]),
     "rv_frame_adjoint": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _vp]),
     "rv_frame_plan_destroy": (None, [_vp]),

@@ -408,6 +408,22 @@ class FramePlan:
         lv, lp, ln = _levels(levels)
         check(lib().rv_frame_set_field_mode(self.h, mode, iso_opacity, lp, ln))
 
+    def set_secondary(self, sparse2, bricks2, srange):
+        """The secondary field of the ValueOnIsosurface(s) modes: per block a
+        brick (torch tensor, the primary bricks' type) and sparse values
+        (torch tensors, or None for resident frames only); bricks2 None
+        removes it."""
+        if bricks2 is None:
+            check(lib().rv_frame_set_secondary(self.h, None, None, 0.0, 1.0))
+            self._sec = None
+            return
+        b2 = (C.c_void_p * len(bricks2))(*[b.data_ptr() for b in bricks2])
+        s2 = None
+        if sparse2 is not None:
+            s2 = (C.c_void_p * len(sparse2))(*[v.data_ptr() for v in sparse2])
+        check(lib().rv_frame_set_secondary(self.h, s2, b2, srange[0], srange[1]))
+        self._sec = (list(bricks2), list(sparse2 or []))
+
     def set_tape(self, enable=True):
         """Record the adjoint's per-segment states in later float-image renders."""
         check(lib().rv_frame_set_tape(self.h, int(enable)))

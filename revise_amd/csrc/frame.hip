@@ -73,6 +73,8 @@ struct FrameArgs {
     float4* tape;                  // adjoint record per (block, pixel of its rectangle), or NULL
     const uint64_t* rec_off;       // per block: offset of its rectangle in the tape
     uint32_t lanes_min;            // per-lane walk kernels: bins with at least this many blocks
+    const void* const* bricks2;    // secondary field bricks (ValueOnIsosurface(s) modes)
+    float smin, smax;              // secondary field range
 };
 
 constexpr uint32_t kSpinLimit = 1u << 22;
@@ -128,8 +130,12 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
                     float ts;
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
-                } else if (kMode == kModeIsosurfaces && a.lev.n == 0) {
+                } else if (multi_level(kMode) && a.lev.n == 0) {
                     seg = make_float4(0.f, 0.f, 0.f, 0.f); // no levels: nothing to draw
+                } else if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces) {
+                    const Secondary<BrickT> sec{reinterpret_cast<const BrickT*>(a.bricks2[blk]), a.smin,
+                                                a.smax};
+                    seg = segment<kMode>(p, brick, tfs, (int)a.tf_n, a.iso_opacity, r, lvs, a.lev.n, sec);
                 } else {
                     seg = segment<kMode>(p, brick, tfs, (int)a.tf_n, a.iso_opacity, r, lvs, a.lev.n);
                 }
@@ -290,7 +296,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
     }
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT)
         tfs[i] = a.tf[i];
-    if (kMode == kModeIsosurfaces && threadIdx.x < a.lev.n)
+    if (multi_level(kMode) && threadIdx.x < a.lev.n)
         lvs[threadIdx.x] = a.lev.v[threadIdx.x];
     __syncthreads();
     if ((threadIdx.x >> 6) < a.render_waves) // fewer concurrent rays can be faster (vector L1)
@@ -306,7 +312,7 @@ __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
     __shared__ float lvs[kMaxLevels];
     for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT)
         tfs_dyn[i] = a.tf[i];
-    if (kMode == kModeIsosurfaces && threadIdx.x < a.lev.n)
+    if (multi_level(kMode) && threadIdx.x < a.lev.n)
         lvs[threadIdx.x] = a.lev.v[threadIdx.x];
     __syncthreads();
     render_bins<BrickT, kMode, kLanes>(a, tfs_dyn, lvs);
@@ -578,8 +584,16 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     if (rc)
         return rc;
     FrameArgs a{};
-    a.tasks = P->d_tasks;
-    a.n_interp = interpolate ? P->n_interp : 0u;
+    const bool value_mode = P->field_mode == RV_FIELD_VALUE_ON_ISOSURFACE ||
+                            P->field_mode == RV_FIELD_VALUE_ON_ISOSURFACES;
+    if (value_mode && interpolate && !P->d_tasks2)
+        return fail(RV_ERR_INVALID, "rv_frame_render: the secondary field has no sparse values to "
+                                    "interpolate (rv_frame_set_secondary)");
+    a.tasks = value_mode ? P->d_tasks2 : P->d_tasks;
+    a.n_interp = interpolate ? (value_mode ? P->n_interp2 : P->n_interp) : 0u;
+    a.bricks2 = P->d_bricks2;
+    a.smin = P->smin;
+    a.smax = P->smax;
     a.params = P->d_params + (size_t)cam * P->nblocks;
     const uint64_t nbins = (uint64_t)B->nbx * B->nby;
     a.bin_off = B->d_off + (nbins + 1) * cam;
@@ -635,6 +649,12 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
         break;
     case RV_FIELD_DOMAIN_VOXELS:
         launch_mode<kModeDomainVoxels>(P, a, interpolate, grid, s);
+        break;
+    case RV_FIELD_VALUE_ON_ISOSURFACE:
+        launch_mode<kModeValueOnIsosurface>(P, a, interpolate, grid, s);
+        break;
+    case RV_FIELD_VALUE_ON_ISOSURFACES:
+        launch_mode<kModeValueOnIsosurfaces>(P, a, interpolate, grid, s);
         break;
     default:
         launch_mode<kModeArgb>(P, a, interpolate, grid, s);
@@ -1174,6 +1194,72 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     RV_CATCH
 }
 
+extern "C" int rv_frame_set_secondary(rv_frame_plan* P, const float* const* d_sparse2,
+                                      void* const* d_bricks2, float smin, float smax)
+{
+    RV_TRY
+    RV_REQUIRE(P, "rv_frame_set_secondary: null plan");
+    if (P->graph_exec) { // a captured graph holds the previous tables
+        (void)hipGraphExecDestroy(P->graph_exec);
+        P->graph_exec = nullptr;
+    }
+    if (P->graph) {
+        (void)hipGraphDestroy(P->graph);
+        P->graph = nullptr;
+    }
+    if (!d_bricks2) {
+        RV_REQUIRE(P->field_mode != RV_FIELD_VALUE_ON_ISOSURFACE &&
+                       P->field_mode != RV_FIELD_VALUE_ON_ISOSURFACES,
+                   "rv_frame_set_secondary: the plan renders a secondary-field mode");
+        for (void** q : {(void**)&P->d_bricks2, (void**)&P->d_tasks2})
+            if (*q) {
+                (void)hipFree(*q);
+                *q = nullptr;
+            }
+        P->n_interp2 = 0;
+        return RV_OK;
+    }
+    RV_REQUIRE(smax > smin, "rv_frame_set_secondary: the secondary field range must be non-empty");
+    std::vector<const void*> bricks(P->nblocks);
+    for (uint32_t i = 0; i < P->nblocks; ++i) {
+        RV_REQUIRE(d_bricks2[i], "rv_frame_set_secondary: null secondary brick");
+        bricks[i] = d_bricks2[i];
+    }
+    int rc;
+    if (d_sparse2) {
+        RV_REQUIRE(P->can_interp, "rv_frame_set_secondary: the blocks have no gather plans");
+        // the primary field's jobs, then the secondary's: same plans, depths
+        std::vector<Job> jobs(2 * (size_t)P->nblocks);
+        for (uint32_t i = 0; i < P->nblocks; ++i) {
+            const rv_frame_block& b = P->blocks[i];
+            RV_REQUIRE(d_sparse2[i], "rv_frame_set_secondary: null secondary sparse values");
+            jobs[i] = Job{b.sparse, static_cast<const uint8_t*>(b.plan), b.brick,
+                          reinterpret_cast<__half*>(b.brick16), b.depth};
+            jobs[P->nblocks + i] =
+                Job{d_sparse2[i], static_cast<const uint8_t*>(b.plan),
+                    P->fp16 ? nullptr : static_cast<float*>(d_bricks2[i]),
+                    P->fp16 ? static_cast<__half*>(d_bricks2[i]) : nullptr, b.depth};
+        }
+        std::vector<TileTask> tasks;
+        if ((rc = build_tile_tasks(jobs.data(), (uint32_t)jobs.size(), tasks)))
+            return rc;
+        RV_REQUIRE(tasks.size() < 0x40000000ull, "rv_frame_set_secondary: too many brick tiles");
+        P->n_interp2 = (uint32_t)tasks.size();
+        if ((rc = frame_upload((void**)&P->d_tasks2, tasks.data(), sizeof(TileTask) * tasks.size())))
+            return rc;
+    } else if (P->d_tasks2) {
+        (void)hipFree(P->d_tasks2);
+        P->d_tasks2 = nullptr;
+        P->n_interp2 = 0;
+    }
+    if ((rc = frame_upload((void**)&P->d_bricks2, bricks.data(), sizeof(void*) * P->nblocks)))
+        return rc;
+    P->smin = smin;
+    P->smax = smax;
+    return RV_OK;
+    RV_CATCH
+}
+
 extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, float iso_opacity,
                                        const float* levels_rel, uint32_t level_count)
 {
@@ -1201,10 +1287,13 @@ extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, fl
         rc = frame_max_wgs<kModeDomainVoxels>(P->fp16, &per_cu);
         break;
     case RV_FIELD_VALUE_ON_ISOSURFACE:
+        RV_REQUIRE(P->d_bricks2, "rv_frame_set_field_mode: set the secondary field first (rv_frame_set_secondary)");
+        rc = frame_max_wgs<kModeValueOnIsosurface>(P->fp16, &per_cu);
+        break;
     case RV_FIELD_VALUE_ON_ISOSURFACES:
-        return fail(RV_ERR_UNSUPPORTED, "rv_frame_set_field_mode: the frame plan holds one field; "
-                                        "the ValueOnIsosurface(s) modes render through "
-                                        "rv_render_block_secondary / rv_render_share_secondary");
+        RV_REQUIRE(P->d_bricks2, "rv_frame_set_field_mode: set the secondary field first (rv_frame_set_secondary)");
+        rc = frame_max_wgs<kModeValueOnIsosurfaces>(P->fp16, &per_cu);
+        break;
     default:
         return fail(RV_ERR_INVALID, "rv_frame_set_field_mode: unknown field mode");
     }

@@ -79,6 +79,12 @@ struct rv_frame_plan {
     uint32_t schedule = RV_FRAME_SCHEDULE_BINS; // how a wave walks its screen bin (Argb)
     uint32_t lanes_max_wgs = 0; // resident interpolating workgroups per CU, per-lane walk
     uint32_t hybrid_min = 16;   // hybrid schedule: per-lane walk for bins of at least this many blocks
+    // secondary field (ValueOnIsosurface(s)): per block brick pointers, and
+    // the tiles of both fields for interpolating frames
+    const void** d_bricks2 = nullptr;
+    TileTask* d_tasks2 = nullptr;
+    uint32_t n_interp2 = 0;
+    float smin = 0.f, smax = 1.f;
     // adjoint tape: per camera, per block, the offset of the block's view
     // rectangle in the tape (float4 per pixel); the blocks' 8 x 8 tiles
     // as (block, tile) pairs, grouped by block
@@ -99,7 +105,7 @@ struct rv_frame_plan {
             (void)hipGraphDestroy(graph);
         if (capture_stream)
             (void)hipStreamDestroy(capture_stream);
-        for (void* p : {(void*)d_tasks, (void*)d_bricks, (void*)d_counters,
+        for (void* p : {(void*)d_tasks, (void*)d_bricks, (void*)d_counters, (void*)d_bricks2, (void*)d_tasks2,
                         (void*)d_params, (void*)d_grad_bricks,
                         (void*)d_rec_off, (void*)d_items, (void*)d_rec_chunk, (void*)d_tape})
             if (p)

@@ -284,3 +284,53 @@ def test_value_modes_block_and_share_bit_exact(orc, mode, thr, iso, levels):
     torch.cuda.synchronize()
     assert (vp.cpu().numpy().view(np.uint32) >> 24).any()
     assert torch.equal(vp, vp_seq)
+
+
+@pytest.mark.parametrize("fp16", [False, True])
+@pytest.mark.parametrize("mode,thr,iso,levels", VALUE_MODES)
+def test_frame_value_modes_match_oracle(orc, mode, thr, iso, levels, fp16):
+    """ValueOnIsosurface(s) in the batched frame: the secondary bricks beside
+    the primary ones, interpolated in the same launch (both fields' tiles) or
+    resident; the float image equals the oracle's composite of the same
+    segments."""
+    import torch
+    W, H = 144, 108
+    blocks = _scene(1, 4, seed=25)
+    sec = _secondary(blocks, 26)
+    fmin, fmax = datasets.field_range(blocks)
+    smin, smax = datasets.field_range(sec)
+    tf = datasets.seeded_tf(25, 256)
+    d_tf = to_dev(tf)
+    plan = api.interp_plan(blocks[0].n2i, 4)
+    bt = torch.float16 if fp16 else torch.float32
+    dev = [{"level": b.level, "index": b.index, "depth": b.depth, "sparse": to_dev(b.sparse), "plan": plan,
+            "brick": torch.empty(api.brick_nodes(4), dtype=bt, device="cuda")} for b in blocks]
+    fp = api.FramePlan(dev, W, H, FOV, fmin, fmax, thr, 2.0, 0.996, 0.9999, d_tf)
+    from revise_amd import RvError
+    with pytest.raises(RvError, match="secondary"):
+        fp.set_field_mode(mode, iso, levels=levels)
+    sparse2 = [to_dev(b.sparse) for b in sec]
+    bricks2 = [torch.empty(api.brick_nodes(4), dtype=bt, device="cuda") for _ in sec]
+    fp.set_secondary(sparse2, bricks2, (smin, smax))
+    fp.set_field_mode(mode, iso, levels=levels)
+    cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((40.0, 15.0), (220.0, -25.0))])
+    fp.set_cameras(cams)
+    cast = (lambda x: x.astype(np.float16).astype(np.float32)) if fp16 else (lambda x: x)
+    bricks = [cast(orc.interpolate(b.sparse, b.n2i, b.depth)) for b in blocks]
+    bricks_s = [cast(orc.interpolate(b.sparse, b.n2i, b.depth)) for b in sec]
+    keys = np.array([(1,) + b.index for b in blocks], np.uint32)
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    for c in range(2):
+        order = api.mixed_back_to_front(keys, api.eye_from_view(cams[c]))
+        lst = [(1, blocks[i].index, 4, bricks[i], bricks_s[i]) for i in order]
+        exp = orc.render_float_composite(cams[c], W, H, FOV, lst, fmin, fmax, thr, 2.0, 0.996, tf, 0.9999,
+                                         mode=mode, iso_opacity=iso, levels=levels, srange=(smin, smax))
+        assert (exp[..., 3] > 0).mean() > 0.01  # the surfaces cover part of the image
+        for interp in (True, False):
+            say("frame value mode", mode, "fp16", fp16, "cam", c, "interpolate", interp)
+            fp.render(c, argb, rgba, interpolate=interp)
+            torch.cuda.synchronize()
+            assert fp.status() == 0
+            assert np.array_equal(rgba.cpu().numpy(), exp)
+    fp.close()
