@@ -30,6 +30,7 @@ import json
 import math
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -38,6 +39,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+# the frame autotuner's choices (one JSON object per line), reused by later runs
+TUNE_CACHE = os.environ.get("RV_FRAME_TUNE_CACHE",
+                            os.path.join(tempfile.gettempdir(), "revise_amd_frame_tune.jsonl"))
 SEED = 2024
 W, H = 1920, 1080
 FOV, THR, Q2, CUT2 = 30.0, 0.5, 2.0, 0.99
@@ -245,7 +249,7 @@ def large_share_bench(torch, args, tf_np, log):
     fp.set_cameras(views)
     t_setup = time.time() - t0
     argb = torch.empty((H5, W5), dtype=torch.int32, device="cuda")
-    tile = fp.autotune(argb, False, os.path.join(ROOT, "gpurun_out", "frame_tune.jsonl"))
+    tile = fp.autotune(argb, False, TUNE_CACHE)
     fp.render(0, argb, None, False)
     t_f = events_time(torch, lambda: fp.render(0, argb, None, False), 5)
     t_fi = events_time(torch, lambda: fp.render(0, argb, None, True), 3)
@@ -444,7 +448,7 @@ def main():
     fp = frame_plan(torch, wl, tf, Q2, CUT2)
     fp.set_cameras(M[None])
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
-    tune_cache = os.path.join(ROOT, "gpurun_out", "frame_tune.jsonl")
+    tune_cache = TUNE_CACHE
     fp.autotune(argb, False, tune_cache)  # grid of the resident-brick kernel
     tile = fp.autotune(argb, True, tune_cache)
     fp.render(0, argb, None, True)
@@ -519,7 +523,7 @@ def main():
         fp3 = frame_plan(torch, wl3, tf, Q2, CUT2)
         fp3.set_cameras(cams)
         argb3 = torch.empty((H, W), dtype=torch.int32, device="cuda")
-        tile3 = fp3.autotune(argb3, True, os.path.join(ROOT, "gpurun_out", "frame_tune.jsonl"))
+        tile3 = fp3.autotune(argb3, True, TUNE_CACHE)
         frames = torch.empty((args.cams, H, W), dtype=torch.int32, device="cuda")
         fp3.graph_build(frames, H * W, True)
         fp3.graph_launch()
@@ -551,7 +555,7 @@ def main():
         fp4.set_cameras(M[None])
         fp4t.set_cameras(M[None])
         argb4 = torch.empty((H4, W4), dtype=torch.int32, device="cuda")
-        fp4.autotune(argb4, False, os.path.join(ROOT, "gpurun_out", "frame_tune.jsonl"))
+        fp4.autotune(argb4, False, TUNE_CACHE)
         rgba4 = torch.empty((H4, W4, 4), dtype=torch.float32, device="cuda")
         target4 = torch.empty_like(rgba4)
         fp4t.render(0, argb4, target4, interpolate=False)
