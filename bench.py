@@ -93,13 +93,17 @@ class PlainFrame:
         return out
 
     def fused_step(self, crop=True):
-        """Drop-in worker frame in two launches (bit-identical to step())."""
+        """Drop-in worker frame in two launches (bit-identical to step()), as
+        the worker's fused path runs it: the share's own bricks in 4^3
+        micro-bricks."""
         api = self.api
         if self.batch is None:
-            self.batch = api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"])
-                                          for b, _, _ in self.items])
-            self.share = api.ShareRenderer([(p, b["brick"]) for b, live, p in self.items if live],
-                                           self.tf, W, H)
+            self.tiled = [self.torch.empty(api.brick_floats(b["depth"], api.BRICK_TILED4), dtype=self.torch.float32,
+                                           device="cuda") for b, _, _ in self.items]
+            self.batch = api.InterpBatch([(b["sparse"], b["plan"], t, b["depth"])
+                                          for (b, _, _), t in zip(self.items, self.tiled)], api.BRICK_TILED4)
+            self.share = api.ShareRenderer([(p, t) for (b, live, p), t in zip(self.items, self.tiled) if live],
+                                           self.tf, W, H, layout=api.BRICK_TILED4)
         api.clear_viewport(self.vp, W, H)
         self.batch.run()
         self.share.render(self.vp)
@@ -368,7 +372,7 @@ def main():
         log("batched interp, private plans: %.3f ms" % (events_time(torch, batch_own.run, 10) * 1e3))
         return
     if args.only_adjoint:
-        plain.fused_step()  # interpolates the bricks the frame plan reads
+        plain.step(crop=False)  # interpolates the bricks the frame plan reads
         fpa = api.FramePlan(wl.blocks, 512, 512, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf)
         fpa.set_cameras(M[None])
         fpa.set_tape(True)
@@ -384,6 +388,7 @@ def main():
         fpa.close()
         return
     if args.only_render:
+        plain.step(crop=False)
         plain.fused_step()
         for _ in range(3):
             plain.share.render(plain.vp)

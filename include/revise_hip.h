@@ -124,15 +124,26 @@ int rv_interp_brick_planned(const float* d_sparse, const void* d_plan, uint32_t 
 int rv_interp_brick(const float* d_sparse, const uint16_t* n2i, uint64_t n, uint32_t depth,
                     float* d_dense, rv_stream_t stream);
 
+/* Dense brick layouts.  LINEAR: (2^depth+1)^3 nodes, x fastest, then y, z
+ * (the reference's).  TILED4 (fp32): 4 x 4 x 4 micro-bricks of 64 floats,
+ * x fastest inside, the G^3 micro-bricks x fastest, G = ceil((2^depth+1)/4);
+ * node (x, y, z) at ((z/4 * G + y/4) * G + x/4) * 64 + (z%4 * 16 + y%4 * 4 +
+ * x%4), padding never written.  The ray-casts read a TILED4 brick's cell
+ * corners from fewer cache lines; the values are the same. */
+enum { RV_BRICK_LINEAR = 0, RV_BRICK_TILED4 = 1 };
+/* Floats a brick of this depth takes in a layout. */
+uint64_t rv_brick_floats(uint32_t depth, uint32_t layout);
+
 /* One brick of a batch.  dense16 != NULL stores fp16 instead of fp32
- * (exactly one of dense / dense16 is set). */
+ * (exactly one of dense / dense16 is set); layout: RV_BRICK_* (TILED4 is
+ * fp32 only). */
 typedef struct rv_brick_job {
     const float* sparse;
     const void* plan;
     float* dense;
     uint16_t* dense16;
     uint32_t depth;
-    uint32_t reserved;
+    uint32_t layout;
 } rv_brick_job;
 
 /* Batched interpolation of many bricks of any depths in one launch.
@@ -287,7 +298,14 @@ int rv_render_share(const rv_block_params* params, const float* const* bricks, u
                     uint32_t level_count, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport,
                     uint32_t vp_w, uint32_t vp_h, void* d_workspace, rv_stream_t stream);
 /* ... and with the secondary field (bricks2: per block; the modes of
- * rv_render_block_secondary). */
+ * rv_render_block_secondary), bricks of either layout (RV_BRICK_*; the
+ * secondary bricks in the same one). */
+int rv_render_share_layout(const rv_block_params* params, const float* const* bricks,
+                           const float* const* bricks2, uint32_t nblocks, uint32_t layout,
+                           uint32_t field_mode, float iso_opacity, const float* levels_rel,
+                           uint32_t level_count, float smin, float smax, const float* d_tf,
+                           uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                           void* d_workspace, rv_stream_t stream);
 int rv_render_share_secondary(const rv_block_params* params, const float* const* bricks,
                               const float* const* bricks2, uint32_t nblocks, uint32_t field_mode,
                               float iso_opacity, const float* levels_rel, uint32_t level_count,

@@ -58,7 +58,7 @@ class RenderState(C.Structure):
 
 class BrickJob(C.Structure):
     _fields_ = [("sparse", C.c_void_p), ("plan", C.c_void_p), ("dense", C.c_void_p),
-                ("dense16", C.c_void_p), ("depth", C.c_uint32), ("reserved", C.c_uint32)]
+                ("dense16", C.c_void_p), ("depth", C.c_uint32), ("layout", C.c_uint32)]
 
 
 class DatasetInfo(C.Structure):
@@ -128,6 +128,11 @@ _SIGS = {
                                   C.c_float, _f32p, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32,
                                   C.c_uint32, _vp, _vp]),
     "rv_share_workspace_bytes": (C.c_uint64, [C.c_uint32]),
+    "rv_brick_floats": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+    "rv_render_share_layout": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                         C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, _f32p, C.c_uint32,
+                                         C.c_float, C.c_float, _vp, C.c_uint32, _vp, C.c_uint32, C.c_uint32,
+                                         _vp, _vp]),
     "rv_render_block_secondary": (C.c_int, [C.POINTER(BlockParams), C.c_uint32, C.c_float, _f32p,
                                             C.c_uint32, _vp, _vp, C.c_float, C.c_float, _vp, C.c_uint32,
                                             _vp, _vp]),

@@ -150,19 +150,27 @@ def interp_brick(d_sparse, n2i, depth, d_dense, stream=None):
 
 
 # numpy layout of rv_brick_job (for batches of millions of bricks)
-BRICK_JOB_DTYPE = np.dtype({"names": ["sparse", "plan", "dense", "dense16", "depth", "reserved"],
+BRICK_JOB_DTYPE = np.dtype({"names": ["sparse", "plan", "dense", "dense16", "depth", "layout"],
                             "formats": ["<u8", "<u8", "<u8", "<u8", "<u4", "<u4"],
                             "offsets": [0, 8, 16, 24, 32, 36], "itemsize": 40})
 assert BRICK_JOB_DTYPE.itemsize == C.sizeof(BrickJob)
+
+
+BRICK_LINEAR, BRICK_TILED4 = 0, 1
+
+
+def brick_floats(depth, layout=BRICK_LINEAR):
+    """Floats of a brick of this depth in a layout (BRICK_TILED4: 4^3 micro-bricks)."""
+    return int(lib().rv_brick_floats(depth, layout))
 
 
 class InterpBatch:
     """Batched interpolation handle; jobs: list of (sparse, plan, dense, depth)
     torch tensors (dense may be float32 or float16), or a numpy array of
     BRICK_JOB_DTYPE holding device addresses (the caller keeps the memory
-    alive)."""
+    alive); layout: BRICK_LINEAR or BRICK_TILED4 (fp32) for the list form."""
 
-    def __init__(self, jobs):
+    def __init__(self, jobs, layout=BRICK_LINEAR):
         import torch
         self._keep = []
         if isinstance(jobs, np.ndarray):
@@ -182,6 +190,7 @@ class InterpBatch:
             else:
                 arr[i].dense = de.data_ptr()
             arr[i].depth = depth
+            arr[i].layout = layout
             self._keep.append((sp, pl, de))
         h = C.c_void_p()
         check(lib().rv_interp_batch_create(arr, len(jobs), C.byref(h)))
@@ -274,7 +283,7 @@ class ShareRenderer:
     """rv_render_share_argb over a fixed list of (params, brick) in
     back-to-front order."""
 
-    def __init__(self, items, d_tf, W, H, bricks2=None):
+    def __init__(self, items, d_tf, W, H, bricks2=None, layout=BRICK_LINEAR):
         import torch
         live = [(p, b) for p, b in items]
         self.n = len(live)
@@ -285,12 +294,19 @@ class ShareRenderer:
             assert len(bricks2) == self.n
             self.bricks2 = (C.c_void_p * max(self.n, 1))(*[b.data_ptr() for b in bricks2])
         self._keep = [b for _, b in live] + [d_tf] + list(bricks2 or [])
+        self.layout = layout
         self.tf, self.W, self.H = d_tf, W, H
         nbytes = int(lib().rv_share_workspace_bytes(self.n))
         self.ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
 
     def render_mode(self, d_vp, mode, iso_opacity=1.0, levels=None, stream=None, srange=None):
         lv, lp, ln = _levels(levels)
+        if self.layout != BRICK_LINEAR:
+            sr = srange or (0.0, 1.0)
+            check(lib().rv_render_share_layout(self.params, self.bricks, self.bricks2, self.n, self.layout, mode,
+                                               iso_opacity, lp, ln, sr[0], sr[1], ptr(self.tf), self.tf.shape[0],
+                                               ptr(d_vp), self.W, self.H, ptr(self.ws), stream_ptr(stream)))
+            return
         if self.bricks2 is not None:
             check(lib().rv_render_share_secondary(self.params, self.bricks, self.bricks2, self.n, mode,
                                                   iso_opacity, lp, ln, srange[0], srange[1], ptr(self.tf),
@@ -302,6 +318,9 @@ class ShareRenderer:
                                     stream_ptr(stream)))
 
     def render(self, d_vp, stream=None):
+        if self.layout != BRICK_LINEAR:
+            self.render_mode(d_vp, FIELD_ARGB, stream=stream)
+            return
         check(lib().rv_render_share_argb(self.params, self.bricks, self.n, ptr(self.tf), self.tf.shape[0],
                                          ptr(d_vp), self.W, self.H, ptr(self.ws), stream_ptr(stream)))
 

@@ -334,7 +334,7 @@ constexpr uint32_t kShareT = 256;
 #endif
 constexpr uint32_t kShareWgsPerCu = RV_SHARE_WGS_PER_CU;
 
-template <uint32_t kMode>
+template <uint32_t kMode, class BrickT>
 __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* __restrict__ params,
                                                         const uint4* __restrict__ rects,
                                                         const void* const* __restrict__ bricks,
@@ -377,10 +377,10 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                 vy < p.view_y + p.view_h) {
                 const RaySeg r = setup_ray(p, vx, vy);
                 if (r.hit) {
-                    const float* brick = reinterpret_cast<const float*>(bricks[blk]);
-                    Secondary<float> sec{nullptr, smin, smax};
+                    const BrickT* brick = reinterpret_cast<const BrickT*>(bricks[blk]);
+                    Secondary<BrickT> sec{nullptr, smin, smax};
                     if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces)
-                        sec.brick = reinterpret_cast<const float*>(bricks2[blk]);
+                        sec.brick = reinterpret_cast<const BrickT*>(bricks2[blk]);
                     if (kMode == kModeMip)
                         value = mip_merge(value, march_mip(p, brick, r), p);
                     else if (multi_level(kMode) && lev.n == 0)
@@ -1324,15 +1324,15 @@ extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
     return (uint64_t)nblocks * (sizeof(rv_block_params) + sizeof(uint4) + 2 * sizeof(void*)) + 256;
 }
 
-extern "C" int rv_render_share_secondary(const rv_block_params* params, const float* const* bricks,
-                                         const float* const* bricks2, uint32_t nblocks,
-                                         uint32_t field_mode, float iso_opacity,
-                                         const float* levels_rel, uint32_t level_count, float smin,
-                                         float smax, const float* d_tf, uint32_t tf_n,
-                                         uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
-                                         void* d_workspace, rv_stream_t stream)
+extern "C" int rv_render_share_layout(const rv_block_params* params, const float* const* bricks,
+                                      const float* const* bricks2, uint32_t nblocks, uint32_t layout,
+                                      uint32_t field_mode, float iso_opacity, const float* levels_rel,
+                                      uint32_t level_count, float smin, float smax, const float* d_tf,
+                                      uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                      void* d_workspace, rv_stream_t stream)
 {
     RV_TRY
+    RV_REQUIRE(layout == RV_BRICK_LINEAR || layout == RV_BRICK_TILED4, "rv_render_share: unknown layout");
     RV_REQUIRE(field_mode <= RV_FIELD_VALUE_ON_ISOSURFACES, "rv_render_share: unknown field mode");
     const bool value = field_mode == RV_FIELD_VALUE_ON_ISOSURFACE ||
                        field_mode == RV_FIELD_VALUE_ON_ISOSURFACES;
@@ -1384,8 +1384,16 @@ extern "C" int rv_render_share_secondary(const rv_block_params* params, const fl
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
     const auto* db2 = db + nblocks;
 #define RV_SHARE_LAUNCH(M)                                                                         \
-    hipLaunchKernelGGL(share_kernel<M>, grid, dim3(kShareT), lds, S(stream), dp, dr, db, db2, smin, smax, \
-                       nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, vp_w, vp_h)
+    do {                                                                                           \
+        if (layout == RV_BRICK_TILED4)                                                             \
+            hipLaunchKernelGGL((share_kernel<M, Tiled4>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
+                               db, db2, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
+                               vp_w, vp_h);                                                        \
+        else                                                                                       \
+            hipLaunchKernelGGL((share_kernel<M, float>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
+                               db, db2, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
+                               vp_w, vp_h);                                                        \
+    } while (0)
     switch (field_mode) {
     case RV_FIELD_ARGB: RV_SHARE_LAUNCH(kModeArgb); break;
     case RV_FIELD_ARGB_LIGHT: RV_SHARE_LAUNCH(kModeArgbLight); break;
@@ -1400,6 +1408,19 @@ extern "C" int rv_render_share_secondary(const rv_block_params* params, const fl
     RV_LAUNCH_CHECK();
     return RV_OK;
     RV_CATCH
+}
+
+extern "C" int rv_render_share_secondary(const rv_block_params* params, const float* const* bricks,
+                                         const float* const* bricks2, uint32_t nblocks,
+                                         uint32_t field_mode, float iso_opacity,
+                                         const float* levels_rel, uint32_t level_count, float smin,
+                                         float smax, const float* d_tf, uint32_t tf_n,
+                                         uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                         void* d_workspace, rv_stream_t stream)
+{
+    return rv_render_share_layout(params, bricks, bricks2, nblocks, RV_BRICK_LINEAR, field_mode,
+                                  iso_opacity, levels_rel, level_count, smin, smax, d_tf, tf_n,
+                                  d_viewport, vp_w, vp_h, d_workspace, stream);
 }
 
 extern "C" int rv_render_share(const rv_block_params* params, const float* const* bricks,

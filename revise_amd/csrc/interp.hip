@@ -288,6 +288,18 @@ extern "C" int rv_interp_brick(const float* d_sparse, const uint16_t* n2i, uint6
     RV_CATCH
 }
 
+extern "C" uint64_t rv_brick_floats(uint32_t depth, uint32_t layout)
+{
+    if (depth > 10)
+        return 0;
+    const uint64_t n = (1ull << depth) + 1ull;
+    if (layout == RV_BRICK_TILED4) {
+        const uint64_t G = (n + 3) / 4;
+        return G * G * G * 64;
+    }
+    return layout == RV_BRICK_LINEAR ? n * n * n : 0;
+}
+
 extern "C" int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count,
                                       rv_interp_batch** out)
 {
@@ -303,8 +315,11 @@ extern "C" int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count,
         RV_REQUIRE(j.plan, "rv_interp_batch_create: null plan");
         RV_REQUIRE((j.dense != nullptr) != (j.dense16 != nullptr),
                    "rv_interp_batch_create: exactly one of dense/dense16 must be set");
+        RV_REQUIRE(j.layout == RV_BRICK_LINEAR || (j.layout == RV_BRICK_TILED4 && j.dense),
+                   "rv_interp_batch_create: unknown layout (TILED4 bricks are fp32)");
         host[i] = Job{j.sparse, static_cast<const uint8_t*>(j.plan), j.dense,
-                      reinterpret_cast<__half*>(j.dense16), j.depth};
+                      reinterpret_cast<__half*>(j.dense16), j.depth,
+                      j.layout == RV_BRICK_TILED4 ? 1u : 0u};
         b->big |= big_class(j.depth);
         b->dense_nodes += rv_brick_nodes(j.depth);
     }

@@ -37,7 +37,9 @@ struct Job {
     float* dense;
     __half* dense16;
     uint32_t depth;
+    uint32_t tiled;       // dense (fp32) in 4 x 4 x 4 micro-bricks (tiled4_index)
 };
+
 
 // One tile of a batched launch, with its plan entry range looked up on the
 // host when the batch is built: a workgroup reads this one record instead of
@@ -291,6 +293,31 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
     }
 
     // 4. store the owned half-open box (+ far faces of the brick)
+    if (job.tiled) {
+        // the micro-bricks of the owned box (tile boundaries are multiples of
+        // 4 whenever a brick has several tiles), 64 consecutive floats each
+        const uint32_t n = g.n, G = (n + 3u) >> 2;
+        const uint32_t ex = (tx + 1u == g.tiles_x) ? rowlen : X;
+        const uint32_t ey = (ty + 1u == g.tiles_yz) ? tp1 : T;
+        const uint32_t ez = (tz + 1u == g.tiles_yz) ? tp1 : T;
+        const uint32_t mx0 = x0 >> 2, my0 = y0 >> 2, mz0 = z0 >> 2;
+        const uint32_t nmx = ((x0 + ex + 3u) >> 2) - mx0, nmy = ((y0 + ey + 3u) >> 2) - my0,
+                       nmz = ((z0 + ez + 3u) >> 2) - mz0;
+        const uint32_t total = nmx * nmy * nmz * 64u;
+        for (uint32_t q = threadIdx.x; q < total; q += NT) {
+            const uint32_t mb = q >> 6, e = q & 63u;
+            const uint32_t mx = mx0 + mb % nmx, myz = mb / nmx;
+            const uint32_t my = my0 + myz % nmy, mz = mz0 + myz / nmy;
+            const uint32_t x = (mx << 2) | (e & 3u), y = (my << 2) | ((e >> 2) & 3u), z = (mz << 2) | (e >> 4);
+            if (x >= x0 + ex || y >= y0 + ey || z >= z0 + ez)
+                continue; // padding beyond the brick
+            float v = A[((z - z0) * tp1 + (y - y0)) * rowlen + (x - x0)];
+            if (v == kNoField)
+                v = __builtin_nanf("");
+            __builtin_nontemporal_store(v, job.dense + (((mz * G + my) * G + mx) << 6) + e);
+        }
+        return;
+    }
     if (g.tiles_x == 1u && g.tiles_yz == 1u) {
         // the tile is the whole brick: LDS and brick layouts coincide, one
         // contiguous run (fp16: two values per 4-byte store when aligned)

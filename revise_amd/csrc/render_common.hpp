@@ -5,6 +5,7 @@
 #pragma once
 
 #include "../../include/revise_hip.h"
+#include "interp_geom.hpp"
 
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -142,6 +143,43 @@ __device__ __forceinline__ Taps fetch_corners(const T* __restrict__ brick, int n
         v[2 * k] = l0 ? p0 : p1;
         v[2 * k + 1] = l1 ? p0 : p1;
     }
+    return t;
+}
+
+// An fp32 brick in 4 x 4 x 4 micro-bricks (tiled4_index): the 8 corners of a
+// cell fall into at most 8 micro-bricks of 256 bytes, and the corners of
+// neighbouring rays into the same ones, so a wave's corner loads touch fewer
+// cache lines than in the x-fastest layout.  The values, and so every
+// result, are the same.
+struct Tiled4 {
+    float v;
+};
+
+__device__ __forceinline__ Taps fetch_corners(const Tiled4* __restrict__ brick, int n, float tx, float ty,
+                                              float tz, float v[8])
+{
+    int x0, x1, y0, y1, z0, z1;
+    Taps t;
+    axis_taps(tx, n, x0, x1, t.fx);
+    axis_taps(ty, n, y0, y1, t.fy);
+    axis_taps(tz, n, z0, z1, t.fz);
+    t.x0 = x0;
+    t.y0 = y0;
+    t.z0 = z0;
+    t.a = ((uint32_t)z0 * n + y0) * n + x0;
+    t.dx = (uint32_t)(x1 - x0);
+    t.dy = (uint32_t)(y1 - y0) * n;
+    t.dz = (uint32_t)(z1 - z0) * n * n;
+    const uint32_t G = ((uint32_t)n + 3u) >> 2;
+    const float* b = reinterpret_cast<const float*>(brick);
+    v[0] = b[tiled4_index(x0, y0, z0, G)];
+    v[1] = b[tiled4_index(x1, y0, z0, G)];
+    v[2] = b[tiled4_index(x0, y1, z0, G)];
+    v[3] = b[tiled4_index(x1, y1, z0, G)];
+    v[4] = b[tiled4_index(x0, y0, z1, G)];
+    v[5] = b[tiled4_index(x1, y0, z1, G)];
+    v[6] = b[tiled4_index(x0, y1, z1, G)];
+    v[7] = b[tiled4_index(x1, y1, z1, G)];
     return t;
 }
 

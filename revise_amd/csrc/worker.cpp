@@ -458,16 +458,20 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         jobs.reserve(count);
         for (uint64_t i = 0; i < count; ++i) {
             const std::array<uint32_t, 3> idx{order[3 * i], order[3 * i + 1], order[3 * i + 2]};
-            const uint64_t nodes = rv_brick_nodes(fs.subtreeDepth(sub.level, idx));
+            // the share's bricks in micro-bricks (fewer cache lines per
+            // ray-cast corner load; the same values)
+            const uint64_t floats = rv_brick_floats(fs.subtreeDepth(sub.level, idx), RV_BRICK_TILED4);
             float*& brick = sc.bricks[idx];
             if (!brick)
-                hipCheck(hipMalloc((void**)&brick, 4 * nodes), "hipMalloc");
+                hipCheck(hipMalloc((void**)&brick, 4 * floats), "hipMalloc");
             jobs.push_back(fs.job(sub.level, idx, brick, s));
+            jobs.back().layout = RV_BRICK_TILED4;
             if (second) {
                 float*& brick2 = sc.bricks2[idx];
                 if (!brick2)
-                    hipCheck(hipMalloc((void**)&brick2, 4 * nodes), "hipMalloc");
+                    hipCheck(hipMalloc((void**)&brick2, 4 * floats), "hipMalloc");
                 jobs.push_back(fs.job(sub.level, idx, brick2, s, true));
+                jobs.back().layout = RV_BRICK_TILED4;
             }
         }
         throwIfFailed(rv_interp_batch_create(jobs.data(), (uint32_t)jobs.size(), &sc.batch));
@@ -515,9 +519,9 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         return;
     }
     const auto srange = fs.secondaryRange();
-    throwIfFailed(rv_render_share_secondary(
+    throwIfFailed(rv_render_share_layout(
         params.data(), bricks.data(), second ? bricks2.data() : nullptr, (uint32_t)params.size(),
-        st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
+        RV_BRICK_TILED4, st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
         (uint32_t)st.isosurfaceLevels.size(), srange[0], srange[1], m_renderer.colorTable(),
         m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1],
         sc.workspace, s));

@@ -209,3 +209,31 @@ def test_interp_batch_full_small_bricks(orc):
         else:
             assert bits_equal_nan(got, e)
     b.close()
+
+
+@pytest.mark.parametrize("seed,depth,p,pnf", [(30, 0, 0.0, 0.0), (31, 1, 1.0, 0.0), (32, 2, 0.6, 0.05),
+                                              (33, 4, 0.4, 0.02), (34, 5, 0.5, 0.0), (35, 6, 0.35, 0.01),
+                                              (36, 6, 1.0, 0.0)])
+def test_batch_tiled_layout_holds_the_linear_values(orc, seed, depth, p, pnf):
+    """A batch writing the 4^3 micro-brick layout stores, at each node's
+    tiled index, the value the x-fastest layout stores (NaN where undefined)."""
+    import torch
+    d, n2i, vals = _case(seed, depth, p, pnf)
+    plan = api.interp_plan(n2i, d)
+    sp = to_dev(vals)
+    lin = torch.empty(api.brick_nodes(d), dtype=torch.float32, device="cuda")
+    til = torch.full((api.brick_floats(d, api.BRICK_TILED4),), 12345.0, dtype=torch.float32, device="cuda")
+    api.InterpBatch([(sp, plan, lin, d)]).run()
+    api.InterpBatch([(sp, plan, til, d)], api.BRICK_TILED4).run()
+    torch.cuda.synchronize()
+    n = 2 ** d + 1
+    G = (n + 3) // 4
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    idx = ((((z >> 2) * G + (y >> 2)) * G + (x >> 2)) << 6) + (((z & 3) << 4) | ((y & 3) << 2) | (x & 3))
+    t = til.cpu().numpy()
+    assert bits_equal_nan(t[idx.reshape(-1)], lin.cpu().numpy())
+    assert bits_equal_nan(lin.cpu().numpy(), orc.interpolate(vals, n2i, d))
+    # padding is never written
+    pad = np.ones(t.size, bool)
+    pad[idx.reshape(-1)] = False
+    assert np.all(t[pad] == 12345.0)

@@ -334,3 +334,48 @@ def test_frame_value_modes_match_oracle(orc, mode, thr, iso, levels, fp16):
             assert fp.status() == 0
             assert np.array_equal(rgba.cpu().numpy(), exp)
     fp.close()
+
+
+@pytest.mark.parametrize("mode,thr,iso,levels", MODES + VALUE_MODES + [(api.FIELD_ARGB, 0.5, 1.0, None)])
+def test_render_share_tiled_bricks_equal_linear(orc, mode, thr, iso, levels):
+    """The share kernel reading 4^3 micro-brick bricks (interpolated in that
+    layout) gives the same viewport as with x-fastest bricks, in every mode."""
+    import torch
+    W, H = 150, 110
+    blocks = _scene(2, 3, seed=27)
+    sec = _secondary(blocks, 28)
+    fmin, fmax = datasets.field_range(blocks)
+    smin, smax = datasets.field_range(sec)
+    tf = datasets.seeded_tf(27, 256)
+    d_tf = to_dev(tf)
+    M = datasets.orbit_camera(75.0, elevation_deg=-10.0)
+    order = api.mixed_back_to_front(np.array([(2,) + b.index for b in blocks], np.uint32), api.eye_from_view(M))
+    plan = api.interp_plan(blocks[0].n2i, 3)
+    value = mode in (api.FIELD_VALUE_ON_ISOSURFACE, api.FIELD_VALUE_ON_ISOSURFACES)
+    out = {}
+    for layout in (api.BRICK_LINEAR, api.BRICK_TILED4):
+        nf = api.brick_floats(3, layout)
+        br = [torch.empty(nf, dtype=torch.float32, device="cuda") for _ in blocks]
+        br2 = [torch.empty(nf, dtype=torch.float32, device="cuda") for _ in blocks]
+        jobs = [(to_dev(b.sparse), plan, br[i], 3) for i, b in enumerate(blocks)]
+        jobs += [(to_dev(b.sparse), plan, br2[i], 3) for i, b in enumerate(sec)]
+        api.InterpBatch(jobs, layout).run()
+        items, items2 = [], []
+        for i in order:
+            b = blocks[int(i)]
+            live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+            if live:
+                items.append((p, br[int(i)]))
+                items2.append(br2[int(i)])
+        share = api.ShareRenderer(items, d_tf, W, H, bricks2=items2 if value else None, layout=layout)
+        vp = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        api.clear_viewport_mode(vp, W, H, mode)
+        say("share layout", layout, "mode", mode)
+        share.render_mode(vp, mode, iso, levels=levels, srange=(smin, smax))
+        torch.cuda.synchronize()
+        out[layout] = vp.cpu().numpy().view(np.uint32)
+    assert np.array_equal(out[api.BRICK_LINEAR], out[api.BRICK_TILED4])
+    if mode == api.FIELD_MIP:
+        assert (~np.isnan(out[api.BRICK_LINEAR].view(np.float32))).any()
+    else:
+        assert (out[api.BRICK_LINEAR] >> 24).any()
