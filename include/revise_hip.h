@@ -414,6 +414,8 @@ typedef struct rv_frame_desc {
     uint32_t tf_n;           /* 1..1024 */
     uint32_t tile_w, tile_h; /* wave footprint of a screen bin: tile_w*tile_h == 64,
                                 tile_w a power of two; 0,0 = 8x8 */
+    uint32_t layout;         /* RV_BRICK_* of the blocks' (and secondary) bricks; TILED4 is
+                                fp32 only; field gradient arrays stay LINEAR */
 } rv_frame_desc;
 
 typedef struct rv_frame_plan rv_frame_plan;

@@ -43,7 +43,7 @@ class FrameDesc(C.Structure):
                 ("fmin", C.c_float), ("fmax", C.c_float), ("threshold_rel", C.c_float),
                 ("quality", C.c_float), ("alpha_cutoff", C.c_float), ("acc_cutoff", C.c_float),
                 ("d_tf", C.c_void_p), ("tf_n", C.c_uint32), ("tile_w", C.c_uint32),
-                ("tile_h", C.c_uint32)]
+                ("tile_h", C.c_uint32), ("layout", C.c_uint32)]
 
 
 class RenderState(C.Structure):

@@ -355,7 +355,7 @@ class FramePlan:
     addresses (the caller keeps the memory alive)."""
 
     def __init__(self, blocks, vp_w, vp_h, fov, fmin, fmax, thr_rel, quality, alpha_cutoff,
-                 acc_cutoff, d_tf, tile=(0, 0)):
+                 acc_cutoff, d_tf, tile=(0, 0), layout=BRICK_LINEAR):
         import torch
         self._keep = [d_tf]
         if isinstance(blocks, np.ndarray):
@@ -378,7 +378,7 @@ class FramePlan:
                     arr[i].brick = b["brick"].data_ptr()
                 self._keep.append(b)
         d = FrameDesc(vp_w, vp_h, fov, fmin, fmax, thr_rel, quality, alpha_cutoff, acc_cutoff,
-                      d_tf.data_ptr(), d_tf.shape[0], tile[0], tile[1])
+                      d_tf.data_ptr(), d_tf.shape[0], tile[0], tile[1], layout)
         h = C.c_void_p()
         check(lib().rv_frame_plan_create(arr, len(blocks), C.byref(d), C.byref(h)))
         self.h = h

@@ -661,6 +661,9 @@ extern "C" int rv_frame_adjoint(rv_frame_plan* P, uint32_t cam, const float* d_r
         if (P->fp16)
             RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<__half>, kRT,
                                                                 adjoint_lds(P->desc.tf_n)));
+        else if (P->tiled)
+            RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<Tiled4>, kRT,
+                                                                adjoint_lds(P->desc.tf_n)));
         else
             RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_kernel<float>, kRT,
                                                                 adjoint_lds(P->desc.tf_n)));
@@ -701,6 +704,9 @@ extern "C" int rv_frame_adjoint(rv_frame_plan* P, uint32_t cam, const float* d_r
             if (P->fp16)
                 RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_records_kernel<__half>,
                                                                     kRT, 32ull * P->desc.tf_n));
+            else if (P->tiled)
+                RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_records_kernel<Tiled4>,
+                                                                    kRT, 32ull * P->desc.tf_n));
             else
                 RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, adjoint_records_kernel<float>,
                                                                     kRT, 32ull * P->desc.tf_n));
@@ -712,6 +718,9 @@ extern "C" int rv_frame_adjoint(rv_frame_plan* P, uint32_t cam, const float* d_r
         if (P->fp16)
             hipLaunchKernelGGL(adjoint_records_kernel<__half>, dim3(rgrid), dim3(kRT), 32ull * a.tf_n,
                                S(stream), a);
+        else if (P->tiled)
+            hipLaunchKernelGGL(adjoint_records_kernel<Tiled4>, dim3(rgrid), dim3(kRT), 32ull * a.tf_n,
+                               S(stream), a);
         else
             hipLaunchKernelGGL(adjoint_records_kernel<float>, dim3(rgrid), dim3(kRT), 32ull * a.tf_n,
                                S(stream), a);
@@ -722,6 +731,9 @@ extern "C" int rv_frame_adjoint(rv_frame_plan* P, uint32_t cam, const float* d_r
     if (P->fp16)
         hipLaunchKernelGGL(adjoint_kernel<__half>, dim3(grid), dim3(kRT), adjoint_lds(a.tf_n),
                            S(stream), a);
+    else if (P->tiled)
+        hipLaunchKernelGGL(adjoint_kernel<Tiled4>, dim3(grid), dim3(kRT), adjoint_lds(a.tf_n), S(stream),
+                           a);
     else
         hipLaunchKernelGGL(adjoint_kernel<float>, dim3(grid), dim3(kRT), adjoint_lds(a.tf_n),
                            S(stream), a);

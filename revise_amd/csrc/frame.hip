@@ -535,11 +535,16 @@ void launch_sched(const rv_frame_plan* P, const FrameArgs& a, int interpolate, u
         if (P->fp16)
             hipLaunchKernelGGL((render_frame_kernel<__half, kMode, kLanes>), dim3(P->render_grid),
                                dim3(kRT), lds, s, a);
+        else if (P->tiled)
+            hipLaunchKernelGGL((render_frame_kernel<Tiled4, kMode, kLanes>), dim3(P->render_grid),
+                               dim3(kRT), lds, s, a);
         else
             hipLaunchKernelGGL((render_frame_kernel<float, kMode, kLanes>), dim3(P->render_grid),
                                dim3(kRT), lds, s, a);
     } else if (P->fp16) {
         hipLaunchKernelGGL((frame_kernel<__half, kMode, kLanes>), dim3(grid), dim3(kT), 0, s, a);
+    } else if (P->tiled) {
+        hipLaunchKernelGGL((frame_kernel<Tiled4, kMode, kLanes>), dim3(grid), dim3(kT), 0, s, a);
     } else {
         hipLaunchKernelGGL((frame_kernel<float, kMode, kLanes>), dim3(grid), dim3(kT), 0, s, a);
     }
@@ -560,10 +565,13 @@ void launch_mode(const rv_frame_plan* P, const FrameArgs& a, int interpolate, ui
 
 // Resident workgroups per CU of the interpolating kernel in a field mode.
 template <uint32_t kMode, bool kLanes = false>
-int frame_max_wgs(bool fp16, int* per_cu)
+int frame_max_wgs(bool fp16, bool tiled, int* per_cu)
 {
     if (fp16)
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<__half, kMode, kLanes>,
+                                                            kT, 0));
+    else if (tiled)
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<Tiled4, kMode, kLanes>,
                                                             kT, 0));
     else
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, frame_kernel<float, kMode, kLanes>,
@@ -684,6 +692,9 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     P->blocks.assign(blocks, blocks + nblocks);
     P->desc = *desc;
     const bool fp16 = blocks[0].brick16 != nullptr;
+    RV_REQUIRE(desc->layout == RV_BRICK_LINEAR || (desc->layout == RV_BRICK_TILED4 && !fp16),
+               "rv_frame_plan_create: unknown brick layout (TILED4 bricks are fp32)");
+    P->tiled = desc->layout == RV_BRICK_TILED4;
     std::vector<Job> jobs(nblocks);
     std::vector<const void*> bricks(nblocks);
     for (uint32_t i = 0; i < nblocks; ++i) {
@@ -695,7 +706,7 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
         RV_REQUIRE((b.brick16 != nullptr) == fp16 && (b.brick != nullptr) != fp16,
                    "rv_frame_plan_create: all blocks must use the same brick type");
         jobs[i] = Job{b.sparse, static_cast<const uint8_t*>(b.plan), b.brick,
-                      reinterpret_cast<__half*>(b.brick16), b.depth};
+                      reinterpret_cast<__half*>(b.brick16), b.depth, P->tiled ? 1u : 0u};
         bricks[i] = fp16 ? (const void*)b.brick16 : (const void*)b.brick;
         if (!b.sparse || !b.plan)
             P->can_interp = false;
@@ -720,12 +731,12 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     RV_HIP(hipGetDevice(&dev));
     RV_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     int lanes_per_cu = 0;
-    int orc = frame_max_wgs<kModeArgb, true>(fp16, &lanes_per_cu);
+    int orc = frame_max_wgs<kModeArgb, true>(fp16, P->tiled, &lanes_per_cu);
     if (orc)
         return orc;
     RV_REQUIRE(lanes_per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->lanes_max_wgs = (uint32_t)lanes_per_cu;
-    orc = frame_max_wgs<kModeArgb, false>(fp16, &per_cu);
+    orc = frame_max_wgs<kModeArgb, false>(fp16, P->tiled, &per_cu);
     if (orc)
         return orc;
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
@@ -736,6 +747,9 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     if (fp16)
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, render_frame_kernel<__half, kModeArgb, false>, kRT, sizeof(float4) * desc->tf_n));
+    else if (P->tiled)
+        RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, render_frame_kernel<Tiled4, kModeArgb, false>, kRT, sizeof(float4) * desc->tf_n));
     else
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, render_frame_kernel<float, kModeArgb, false>, kRT, sizeof(float4) * desc->tf_n));
@@ -1234,11 +1248,11 @@ extern "C" int rv_frame_set_secondary(rv_frame_plan* P, const float* const* d_sp
             const rv_frame_block& b = P->blocks[i];
             RV_REQUIRE(d_sparse2[i], "rv_frame_set_secondary: null secondary sparse values");
             jobs[i] = Job{b.sparse, static_cast<const uint8_t*>(b.plan), b.brick,
-                          reinterpret_cast<__half*>(b.brick16), b.depth};
+                          reinterpret_cast<__half*>(b.brick16), b.depth, P->tiled ? 1u : 0u};
             jobs[P->nblocks + i] =
                 Job{d_sparse2[i], static_cast<const uint8_t*>(b.plan),
                     P->fp16 ? nullptr : static_cast<float*>(d_bricks2[i]),
-                    P->fp16 ? static_cast<__half*>(d_bricks2[i]) : nullptr, b.depth};
+                    P->fp16 ? static_cast<__half*>(d_bricks2[i]) : nullptr, b.depth, P->tiled ? 1u : 0u};
         }
         std::vector<TileTask> tasks;
         if ((rc = build_tile_tasks(jobs.data(), (uint32_t)jobs.size(), tasks)))
@@ -1269,30 +1283,30 @@ extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, fl
     int per_cu = 0, rc;
     switch (field_mode) {
     case RV_FIELD_ARGB:
-        rc = frame_max_wgs<kModeArgb>(P->fp16, &per_cu);
+        rc = frame_max_wgs<kModeArgb>(P->fp16, P->tiled, &per_cu);
         break;
     case RV_FIELD_ARGB_LIGHT:
-        rc = frame_max_wgs<kModeArgbLight>(P->fp16, &per_cu);
+        rc = frame_max_wgs<kModeArgbLight>(P->fp16, P->tiled, &per_cu);
         break;
     case RV_FIELD_ISOSURFACE:
-        rc = frame_max_wgs<kModeIsosurface>(P->fp16, &per_cu);
+        rc = frame_max_wgs<kModeIsosurface>(P->fp16, P->tiled, &per_cu);
         break;
     case RV_FIELD_MAX_INTENSITY_PROJECTION:
-        rc = frame_max_wgs<kModeMip>(P->fp16, &per_cu);
+        rc = frame_max_wgs<kModeMip>(P->fp16, P->tiled, &per_cu);
         break;
     case RV_FIELD_ISOSURFACES:
-        rc = frame_max_wgs<kModeIsosurfaces>(P->fp16, &per_cu);
+        rc = frame_max_wgs<kModeIsosurfaces>(P->fp16, P->tiled, &per_cu);
         break;
     case RV_FIELD_DOMAIN_VOXELS:
-        rc = frame_max_wgs<kModeDomainVoxels>(P->fp16, &per_cu);
+        rc = frame_max_wgs<kModeDomainVoxels>(P->fp16, P->tiled, &per_cu);
         break;
     case RV_FIELD_VALUE_ON_ISOSURFACE:
         RV_REQUIRE(P->d_bricks2, "rv_frame_set_field_mode: set the secondary field first (rv_frame_set_secondary)");
-        rc = frame_max_wgs<kModeValueOnIsosurface>(P->fp16, &per_cu);
+        rc = frame_max_wgs<kModeValueOnIsosurface>(P->fp16, P->tiled, &per_cu);
         break;
     case RV_FIELD_VALUE_ON_ISOSURFACES:
         RV_REQUIRE(P->d_bricks2, "rv_frame_set_field_mode: set the secondary field first (rv_frame_set_secondary)");
-        rc = frame_max_wgs<kModeValueOnIsosurfaces>(P->fp16, &per_cu);
+        rc = frame_max_wgs<kModeValueOnIsosurfaces>(P->fp16, P->tiled, &per_cu);
         break;
     default:
         return fail(RV_ERR_INVALID, "rv_frame_set_field_mode: unknown field mode");

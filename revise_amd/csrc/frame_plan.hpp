@@ -50,6 +50,7 @@ struct rv_frame_plan {
     std::vector<rv_frame_block> blocks;
     rv_frame_desc desc{};
     bool fp16 = false;
+    bool tiled = false; // fp32 bricks in 4 x 4 x 4 micro-bricks (RV_BRICK_TILED4)
     TileTask* d_tasks = nullptr;
     uint32_t n_interp = 0;
     const void** d_bricks = nullptr;

@@ -46,7 +46,7 @@ def _mixed_blocks(seed, depth=3, n_splits=12):
     return blocks
 
 
-def _setup(blocks, W, H, fp16=False, acc_cut=2.0, seed=0, tile=(0, 0)):
+def _setup(blocks, W, H, fp16=False, acc_cut=2.0, seed=0, tile=(0, 0), layout=api.BRICK_LINEAR):
     import torch
     fmin, fmax = datasets.field_range(blocks)
     tf = datasets.seeded_tf(seed, 256)
@@ -57,11 +57,11 @@ def _setup(blocks, W, H, fp16=False, acc_cut=2.0, seed=0, tile=(0, 0)):
         key = (id(b.n2i), b.depth)
         if key not in plans:
             plans[key] = api.interp_plan(b.n2i, b.depth)
-        brick = torch.empty(api.brick_nodes(b.depth), dtype=torch.float16 if fp16 else torch.float32,
+        brick = torch.empty(api.brick_floats(b.depth, layout), dtype=torch.float16 if fp16 else torch.float32,
                             device="cuda")
         dev_blocks.append({"level": b.level, "index": b.index, "depth": b.depth, "brick": brick,
                            "sparse": to_dev(b.sparse), "plan": plans[key]})
-    fp = api.FramePlan(dev_blocks, W, H, FOV, fmin, fmax, THR, Q, CUT, acc_cut, d_tf, tile)
+    fp = api.FramePlan(dev_blocks, W, H, FOV, fmin, fmax, THR, Q, CUT, acc_cut, d_tf, tile, layout)
     return fp, tf, fmin, fmax, dev_blocks
 
 
@@ -113,13 +113,15 @@ def test_frame_uniform_matches_oracle(orc, fp16, acc_cut, schedule):
         assert (a >> 24).any()
 
 
+@pytest.mark.parametrize("layout", [api.BRICK_LINEAR, api.BRICK_TILED4])
 @pytest.mark.parametrize("schedule", SCHEDULES)
-def test_frame_mixed_levels_matches_oracle(orc, schedule):
+def test_frame_mixed_levels_matches_oracle(orc, schedule, layout):
+    """Mixed levels and depths (0-6), both brick layouts, every schedule."""
     import torch
     blocks = _mixed_blocks(seed=8)
-    say("mixed blocks", len(blocks), "schedule", schedule)
+    say("mixed blocks", len(blocks), "schedule", schedule, "layout", layout)
     W, H = 180, 140
-    fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=8)
+    fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=8, layout=layout)
     _set_schedule(fp, schedule)
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((70.0, 25.0), (250.0, -40.0))])
     fp.set_cameras(cams)
