@@ -149,9 +149,23 @@ def profiled_traffic(key="traffic_bytes_per_launch"):
     return None
 
 
-def frame_plan(torch, wl, tf, quality, cutoff, tile=(0, 0)):
+def tiled_blocks(torch, wl):
+    """The workload's blocks with bricks of their own in 4^3 micro-bricks,
+    interpolated (the batched frames read this layout)."""
     from revise_amd import api
-    return api.FramePlan(wl.blocks, W, H, FOV, wl.fmin, wl.fmax, THR, quality, cutoff, ACC_CUT, tf, tile)
+    blocks = [dict(b, brick=torch.empty(api.brick_floats(b["depth"], api.BRICK_TILED4), dtype=torch.float32,
+                                        device="cuda")) for b in wl.blocks]
+    api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"]) for b in blocks], api.BRICK_TILED4).run()
+    return blocks
+
+
+def frame_plan(torch, wl, tf, quality, cutoff, tile=(0, 0), blocks=None):
+    """A batched frame plan over the workload; blocks: tiled_blocks(...) for
+    bricks in micro-bricks (default: the workload's x-fastest bricks)."""
+    from revise_amd import api
+    layout = api.BRICK_LINEAR if blocks is None else api.BRICK_TILED4
+    return api.FramePlan(blocks if blocks is not None else wl.blocks, W, H, FOV, wl.fmin, wl.fmax, THR, quality,
+                         cutoff, ACC_CUT, tf, tile, layout)
 
 
 def cpu_baseline(wl, M, tf_np, budget_s=20.0):
@@ -450,7 +464,8 @@ def main():
     log("per-block render %.3f ms, share launch %.3f ms, S = %.3g samples/frame, %.3g samples/s (share)"
         % (t_render * 1e3, t_share * 1e3, S, S / t_share))
 
-    fp = frame_plan(torch, wl, tf, Q2, CUT2)
+    tblocks = tiled_blocks(torch, wl)
+    fp = frame_plan(torch, wl, tf, Q2, CUT2, blocks=tblocks)
     fp.set_cameras(M[None])
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
     tune_cache = TUNE_CACHE
@@ -525,7 +540,7 @@ def main():
         torch.cuda.synchronize()
         log("config 3 setup %.1fs: %d blocks, %d sparse nodes" % (time.time() - t0, len(wl3.blocks),
                                                                   wl3.sparse_nodes))
-        fp3 = frame_plan(torch, wl3, tf, Q2, CUT2)
+        fp3 = frame_plan(torch, wl3, tf, Q2, CUT2, blocks=tiled_blocks(torch, wl3))
         fp3.set_cameras(cams)
         argb3 = torch.empty((H, W), dtype=torch.int32, device="cuda")
         tile3 = fp3.autotune(argb3, True, TUNE_CACHE)
@@ -555,8 +570,10 @@ def main():
         rng4 = np.random.default_rng(SEED + 4)
         tf_t = torch.from_numpy(np.clip(tf_np + rng4.normal(0.0, 0.05, size=tf_np.shape), 0.0, 1.0)
                                 .astype(np.float32)).cuda()
-        fp4 = api.FramePlan(wl.blocks, W4, H4, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf)
-        fp4t = api.FramePlan(wl.blocks, W4, H4, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf_t)
+        fp4 = api.FramePlan(tblocks, W4, H4, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf, (0, 0),
+                            api.BRICK_TILED4)
+        fp4t = api.FramePlan(tblocks, W4, H4, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf_t, (0, 0),
+                             api.BRICK_TILED4)
         fp4.set_cameras(M[None])
         fp4t.set_cameras(M[None])
         argb4 = torch.empty((H4, W4), dtype=torch.int32, device="cuda")

@@ -489,8 +489,8 @@ void rv_frame_plan_destroy(rv_frame_plan* plan);
  * camera last rendered that way: given d_rgba (that forward image, vp_w*vp_h
  * float4) and d_grad_rgba (dL/d image, same layout), it adds dL/d table to
  * d_grad_tf (tf_n x 4 floats) and dL/d field to the per-block gradient arrays
- * set with rv_frame_set_grad_bricks (fp32, brick layout; not set = colour
- * table only).  Samples are recomputed per ray segment, none are stored.
+ * set with rv_frame_set_grad_bricks (fp32, RV_BRICK_LINEAR whatever the
+ * bricks' layout; not set = colour table only).  Samples are recomputed per ray segment, none are stored.
  * Outputs accumulate: the caller zeroes them.  Calls on one plan must not
  * overlap. */
 int rv_frame_set_tape(rv_frame_plan* plan, int enable);

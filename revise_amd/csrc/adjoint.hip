@@ -92,6 +92,7 @@ struct AdjArgs {
     const uint32_t* rec_chunk; // per 64 records: the block of the first (colour-table-only kernel)
     uint64_t nrec;             // records of this camera
     uint32_t nblocks;
+    uint32_t* queue;           // next work item (zeroed before the launch)
 };
 
 // Per-pixel state of the re-march.
@@ -576,9 +577,9 @@ __global__ __launch_bounds__(kRT) void adjoint_records_kernel(AdjArgs a)
 }
 
 // Persistent grid; work items (block, tile) in order, tiles of a block
-// consecutive; wave v of workgroup g takes items g*4+v, then every
-// (grid waves)-th after it, so the waves of a workgroup work on neighbouring
-// tiles of one block.
+// consecutive; each wave takes the next item from the queue counter, so
+// waves whose rays are short take more items, and the waves working at one
+// time are on neighbouring tiles of a few blocks.
 template <class BrickT>
 __global__ __launch_bounds__(kRT) void adjoint_kernel(AdjArgs a)
 {
@@ -595,8 +596,13 @@ __global__ __launch_bounds__(kRT) void adjoint_kernel(AdjArgs a)
         for (uint32_t j = lane; j < kWinWave; j += 64u)
             win[j] = 0.f;
     __syncthreads();
-    const uint32_t stride = gridDim.x * kAdjWaves;
-    for (uint32_t item = blockIdx.x * kAdjWaves + wave; item < a.nitems; item += stride) {
+    for (;;) {
+        uint32_t item = 0u;
+        if (lane == 0u)
+            item = atomicAdd(a.queue, 1u);
+        item = uniform(__shfl(item, 0));
+        if (item >= a.nitems)
+            break;
         const uint32_t blk = uniform(a.items[2 * item]), tile = uniform(a.items[2 * item + 1]);
         const rv_block_params p = a.params[blk];
         const uint32_t ntx = (p.view_w + 7u) >> 3;
@@ -623,6 +629,12 @@ __global__ __launch_bounds__(kRT) void adjoint_kernel(AdjArgs a)
         if (g.w != 0.f)
             unsafeAtomicAdd(out + 3, g.w);
     }
+}
+
+__global__ void adjoint_queue_reset_kernel(uint32_t* queue)
+{
+    if (threadIdx.x == 0u)
+        *queue = 0u;
 }
 
 } // namespace
@@ -728,6 +740,9 @@ extern "C" int rv_frame_adjoint(rv_frame_plan* P, uint32_t cam, const float* d_r
         return RV_OK;
     }
     const uint32_t grid = P->adj_grid;
+    a.queue = P->d_counters + 4; // past the frame kernels' counters 0..3
+    hipLaunchKernelGGL(adjoint_queue_reset_kernel, dim3(1), dim3(64), 0, S(stream), a.queue);
+    RV_LAUNCH_CHECK();
     if (P->fp16)
         hipLaunchKernelGGL(adjoint_kernel<__half>, dim3(grid), dim3(kRT), adjoint_lds(a.tf_n),
                            S(stream), a);

@@ -724,8 +724,8 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     }
     if ((rc = frame_upload((void**)&P->d_bricks, bricks.data(), sizeof(void*) * nblocks)))
         return rc;
-    RV_HIP(hipMalloc((void**)&P->d_counters, 16));
-    RV_HIP(hipMemset(P->d_counters, 0, 16));
+    RV_HIP(hipMalloc((void**)&P->d_counters, 32)); // [4]: the adjoint's work queue
+    RV_HIP(hipMemset(P->d_counters, 0, 32));
     // persistent grid: every workgroup resident at once
     int dev = 0, cus = 0, per_cu = 0;
     RV_HIP(hipGetDevice(&dev));
