@@ -170,7 +170,8 @@ typedef struct rv_block_params {
     float center[3];                     /* cube centre, world */
     float half;                          /* cube half size */
     uint32_t level;                      /* cube level */
-    uint32_t tex_depth;                  /* brick depth: 2^tex_depth+1 nodes per edge */
+    uint32_t tex_depth;                  /* brick depth: 2^tex_depth+1 nodes per edge
+                                            (the ray-casts take <= 9) */
     float fmin, fmax;                    /* field range */
     float pos[3], e1[3], e2[3], n[3];    /* eye */
     float w, h, pixel_size;              /* screen */

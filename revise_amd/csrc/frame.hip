@@ -1366,7 +1366,7 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
     for (uint32_t i = 0; i < nblocks; ++i) {
         const rv_block_params& p = params[i];
         RV_REQUIRE(p.vp_w == vp_w && p.vp_h == vp_h, "rv_render_share: viewport mismatch");
-        RV_REQUIRE(p.tex_depth <= 10 && p.nodes == (1u << p.tex_depth) + 1u && p.fmax > p.fmin,
+        RV_REQUIRE(p.tex_depth <= 9 && p.nodes == (1u << p.tex_depth) + 1u && p.fmax > p.fmin,
                    "rv_render_share: inconsistent block parameters");
         RV_REQUIRE((uint64_t)p.view_x + p.view_w <= vp_w && (uint64_t)p.view_y + p.view_h <= vp_h,
                    "rv_render_share: view rectangle outside the viewport");

@@ -248,7 +248,7 @@ static int check_block(const rv_block_params* params, const float* d_brick, cons
     if (!(tf_n >= 1 && tf_n <= kTfMax))
         return fail(RV_ERR_INVALID, std::string(who) + ": tf_n must be in [1, 4096]");
     const rv_block_params& p = *params;
-    if (!(p.tex_depth <= 10 && p.nodes == (1u << p.tex_depth) + 1u))
+    if (!(p.tex_depth <= 9 && p.nodes == (1u << p.tex_depth) + 1u))
         return fail(RV_ERR_INVALID,
                     std::string(who) + ": inconsistent tex_depth/nodes (use rv_block_params_setup)");
     if (!(p.fmax > p.fmin))
@@ -373,7 +373,7 @@ extern "C" int rv_render_block_argb(const rv_block_params* params, const float* 
     RV_REQUIRE(params && d_brick && d_tf && d_viewport, "rv_render_block_argb: null pointer");
     RV_REQUIRE(tf_n >= 1 && tf_n <= kTfMax, "rv_render_block_argb: tf_n must be in [1, 4096]");
     const rv_block_params& p = *params;
-    RV_REQUIRE(p.tex_depth <= 10 && p.nodes == (1u << p.tex_depth) + 1u,
+    RV_REQUIRE(p.tex_depth <= 9 && p.nodes == (1u << p.tex_depth) + 1u,
                "rv_render_block_argb: inconsistent tex_depth/nodes (use rv_block_params_setup)");
     RV_REQUIRE(p.fmax > p.fmin, "rv_render_block_argb: field range must be non-empty");
     RV_REQUIRE((uint64_t)p.view_x + p.view_w <= p.vp_w && (uint64_t)p.view_y + p.view_h <= p.vp_h,

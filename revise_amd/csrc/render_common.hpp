@@ -89,20 +89,36 @@ __device__ __forceinline__ void axis_taps(float t, int n, int& i0, int& i1, floa
 // (fp16) load: the brick is x fastest, so the corners (x0, x0+1) of a cell
 // are adjacent, and a wave's corner loads touch half as many cache lines as
 // with one load per corner.
+// Bricks live in device memory: loads through global-address-space pointers
+// are global loads, not flat ones (no aperture check, no LDS counter).  The
+// offsets into a brick are 32-bit byte counts, so a wave-uniform brick
+// address stays in scalar registers (global_load with saddr); the ray-casts
+// accept bricks of depth <= 9 (a depth-9 fp32 brick is 540 MB).
+template <class T>
+using gptr = const __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ gptr<T> global_of(const T* p)
+{
+    return (gptr<T>)p;
+}
+
+// A pair of fp32 (fp16) brick values starts at any 4-byte (2-byte) boundary.
+typedef uint64_t pair32_t __attribute__((aligned(4)));
+typedef uint32_t pair16_t __attribute__((aligned(2)));
+
 __device__ __forceinline__ void load_pair(const float* __restrict__ b, uint32_t i, float& lo, float& hi)
 {
-    float2 v;
-    __builtin_memcpy(&v, b + i, sizeof v);
-    lo = v.x;
-    hi = v.y;
+    const uint64_t v = *reinterpret_cast<gptr<pair32_t>>(global_of(reinterpret_cast<const char*>(b)) + (i << 2));
+    lo = __uint_as_float((uint32_t)v);
+    hi = __uint_as_float((uint32_t)(v >> 32));
 }
 
 __device__ __forceinline__ void load_pair(const __half* __restrict__ b, uint32_t i, float& lo, float& hi)
 {
-    __half2 v;
-    __builtin_memcpy(&v, b + i, sizeof v);
-    lo = __low2float(v);
-    hi = __high2float(v);
+    const uint32_t v = *reinterpret_cast<gptr<pair16_t>>(global_of(reinterpret_cast<const char*>(b)) + (i << 1));
+    lo = __half2float(__ushort_as_half((unsigned short)(v & 0xFFFFu)));
+    hi = __half2float(__ushort_as_half((unsigned short)(v >> 16)));
 }
 
 // The 8 corner values of the trilinear fetch at texel coordinates (tx, ty,
@@ -170,16 +186,22 @@ __device__ __forceinline__ Taps fetch_corners(const Tiled4* __restrict__ brick, 
     t.dx = (uint32_t)(x1 - x0);
     t.dy = (uint32_t)(y1 - y0) * n;
     t.dz = (uint32_t)(z1 - z0) * n * n;
-    const uint32_t G = ((uint32_t)n + 3u) >> 2;
-    const float* b = reinterpret_cast<const float*>(brick);
-    v[0] = b[tiled4_index(x0, y0, z0, G)];
-    v[1] = b[tiled4_index(x1, y0, z0, G)];
-    v[2] = b[tiled4_index(x0, y1, z0, G)];
-    v[3] = b[tiled4_index(x1, y1, z0, G)];
-    v[4] = b[tiled4_index(x0, y0, z1, G)];
-    v[5] = b[tiled4_index(x1, y0, z1, G)];
-    v[6] = b[tiled4_index(x0, y1, z1, G)];
-    v[7] = b[tiled4_index(x1, y1, z1, G)];
+    // tiled4_index split per axis: (c >> 2) * (64 G^axis) + (c & 3) * 4^axis,
+    // the three terms summed (their bit fields do not overlap the sum's
+    // carries: the low 6 bits hold the in-micro-brick part)
+    const uint32_t G = ((uint32_t)n + 3u) >> 2, sy = G << 6, sz = (G * G) << 6;
+    const uint32_t X0 = (((uint32_t)x0 >> 2) << 6) | ((uint32_t)x0 & 3u);
+    const uint32_t X1 = (((uint32_t)x1 >> 2) << 6) | ((uint32_t)x1 & 3u);
+    const uint32_t Y0 = __umul24((uint32_t)y0 >> 2, sy) + (((uint32_t)y0 & 3u) << 2);
+    const uint32_t Y1 = __umul24((uint32_t)y1 >> 2, sy) + (((uint32_t)y1 & 3u) << 2);
+    const uint32_t Z0 = __umul24((uint32_t)z0 >> 2, sz) + (((uint32_t)z0 & 3u) << 4);
+    const uint32_t Z1 = __umul24((uint32_t)z1 >> 2, sz) + (((uint32_t)z1 & 3u) << 4);
+    const gptr<char> b = global_of(reinterpret_cast<const char*>(brick));
+    const uint32_t o[8] = {X0 + Y0 + Z0, X1 + Y0 + Z0, X0 + Y1 + Z0, X1 + Y1 + Z0,
+                           X0 + Y0 + Z1, X1 + Y0 + Z1, X0 + Y1 + Z1, X1 + Y1 + Z1};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        v[k] = *reinterpret_cast<gptr<float>>(b + (o[k] << 2));
     return t;
 }
 
