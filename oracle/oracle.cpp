@@ -608,8 +608,9 @@ extern "C" int64_t orc_index_cube_split_sorted(uint32_t level, uint32_t split_co
  *             s = (m-1)/(m+1), atanh by its odd Taylor series to s^9;
  *   2^y     = 2^floor(y) * P(f), f = y - floor(y), P a degree-6 polynomial
  *             fitted (least squares, Lawson-reweighted) to 2^f on [0,1].
- * Every step is a correctly rounded fp32 operation with no contraction.
- * Results below 2^-126 are returned as 0. */
+ * Both polynomials are evaluated by Horner's rule with fused multiply-adds
+ * (fmaf, one rounding each); every other step is a correctly rounded fp32
+ * operation with no contraction.  Results below 2^-126 are returned as 0. */
 extern "C" float orc_powf(float b, float e)
 {
     if (!(b >= 0.f))
@@ -638,10 +639,10 @@ extern "C" float orc_powf(float b, float e)
     float s = (m - 1.0f) / (m + 1.0f);
     float s2 = s * s;
     float poly = 0.111111111f;            /* 1/9 */
-    poly = poly * s2 + 0.142857143f;      /* 1/7 */
-    poly = poly * s2 + 0.2f;              /* 1/5 */
-    poly = poly * s2 + 0.333333333f;      /* 1/3 */
-    poly = poly * s2 + 1.0f;
+    poly = std::fmaf(poly, s2, 0.142857143f);      /* 1/7 */
+    poly = std::fmaf(poly, s2, 0.2f);              /* 1/5 */
+    poly = std::fmaf(poly, s2, 0.333333333f);      /* 1/3 */
+    poly = std::fmaf(poly, s2, 1.0f);
     float l = (s * poly) * 2.88539008f;   /* 2/ln2 */
     float y = e * ((float)k + l);
     if (y < -126.0f)
@@ -651,12 +652,12 @@ extern "C" float orc_powf(float b, float e)
     float fl = std::floor(y);
     float f = y - fl;
     float p = 0x1.c7204cp-13f;
-    p = p * f + 0x1.4619a2p-10f;
-    p = p * f + 0x1.3d27f6p-7f;
-    p = p * f + 0x1.c68502p-5f;
-    p = p * f + 0x1.ebfd9ep-3f;
-    p = p * f + 0x1.62e42ap-1f;
-    p = p * f + 1.0f;
+    p = std::fmaf(p, f, 0x1.4619a2p-10f);
+    p = std::fmaf(p, f, 0x1.3d27f6p-7f);
+    p = std::fmaf(p, f, 0x1.c68502p-5f);
+    p = std::fmaf(p, f, 0x1.ebfd9ep-3f);
+    p = std::fmaf(p, f, 0x1.62e42ap-1f);
+    p = std::fmaf(p, f, 1.0f);
     return std::ldexp(p, (int)fl);
 }
 

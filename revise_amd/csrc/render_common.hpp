@@ -16,7 +16,8 @@ namespace rv {
 // renderArgb.hpp:45).  Method: b = m 2^k with m in [sqrt(1/2), sqrt(2));
 // log2 b = k + (2/ln2) atanh(s), s = (m-1)/(m+1), by the odd Taylor series of
 // atanh to s^9; 2^y = 2^floor(y) P(y - floor(y)) with P a degree-6 polynomial
-// fitted (Lawson-reweighted least squares) to 2^f on [0,1]; relative error
+// fitted (Lawson-reweighted least squares) to 2^f on [0,1], both
+// polynomials by Horner's rule with fused multiply-adds; relative error
 // below 1e-6 over the renderer's range.  Results under 2^-126 are 0.
 __device__ __forceinline__ float pow01(float b, float e)
 {
@@ -43,10 +44,10 @@ __device__ __forceinline__ float pow01(float b, float e)
     const float s = (m - 1.0f) / (m + 1.0f);
     const float s2 = s * s;
     float poly = 0.111111111f;
-    poly = poly * s2 + 0.142857143f;
-    poly = poly * s2 + 0.2f;
-    poly = poly * s2 + 0.333333333f;
-    poly = poly * s2 + 1.0f;
+    poly = fmaf(poly, s2, 0.142857143f);
+    poly = fmaf(poly, s2, 0.2f);
+    poly = fmaf(poly, s2, 0.333333333f);
+    poly = fmaf(poly, s2, 1.0f);
     const float l = (s * poly) * 2.88539008f;
     const float y = e * ((float)k + l);
     if (y < -126.0f)
@@ -56,12 +57,12 @@ __device__ __forceinline__ float pow01(float b, float e)
     const float fl = floorf(y);
     const float f = y - fl;
     float p = 0x1.c7204cp-13f;
-    p = p * f + 0x1.4619a2p-10f;
-    p = p * f + 0x1.3d27f6p-7f;
-    p = p * f + 0x1.c68502p-5f;
-    p = p * f + 0x1.ebfd9ep-3f;
-    p = p * f + 0x1.62e42ap-1f;
-    p = p * f + 1.0f;
+    p = fmaf(p, f, 0x1.4619a2p-10f);
+    p = fmaf(p, f, 0x1.3d27f6p-7f);
+    p = fmaf(p, f, 0x1.c68502p-5f);
+    p = fmaf(p, f, 0x1.ebfd9ep-3f);
+    p = fmaf(p, f, 0x1.62e42ap-1f);
+    p = fmaf(p, f, 1.0f);
     return ldexpf(p, (int)fl);
 }
 
