@@ -475,8 +475,9 @@ def main():
     t_batched = events_time(torch, lambda: fp.render(0, argb, None, True), 5)
     t_batched_res = events_time(torch, lambda: fp.render(0, argb, None, False), 5)
     assert fp.status() == 0
-    log("batched frame %.3f ms (resident bricks %.3f ms), tile %s" % (t_batched * 1e3, t_batched_res * 1e3,
-                                                                     tile))
+    split2 = fp.split()
+    log("batched frame %.3f ms (resident bricks %.3f ms), tile %s, %s" % (
+        t_batched * 1e3, t_batched_res * 1e3, tile, "two launches" if split2 else "one launch"))
     fp.close()
 
     out.update({
@@ -512,6 +513,7 @@ def main():
         "batched_frame_fps": 1.0 / t_batched,
         "batched_frame_resident_ms": t_batched_res * 1e3,
         "batched_tile": list(tile),
+        "batched_split_launches": split2,
         "roofline": {"kernel": "interp_batch_kernel (512 bricks, one launch)", "bound": "hbm",
                      "achieved": B_alg / t_interp / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": B_alg / t_interp / HBM_PEAK, "traffic": profiled_traffic(),
@@ -543,6 +545,7 @@ def main():
         fp3 = frame_plan(torch, wl3, tf, Q2, CUT2, blocks=tiled_blocks(torch, wl3))
         fp3.set_cameras(cams)
         argb3 = torch.empty((H, W), dtype=torch.int32, device="cuda")
+        fp3.autotune(argb3, False, TUNE_CACHE)  # grid of the resident-brick kernel (split frames use it)
         tile3 = fp3.autotune(argb3, True, TUNE_CACHE)
         frames = torch.empty((args.cams, H, W), dtype=torch.int32, device="cuda")
         fp3.graph_build(frames, H * W, True)
@@ -550,8 +553,9 @@ def main():
         torch.cuda.synchronize()
         t_graph = events_time(torch, fp3.graph_launch, 2) / args.cams
         assert fp3.status() == 0
-        log("config 3 batched: %.3f ms/frame over %d cameras (graph), tile %s" % (t_graph * 1e3, args.cams,
-                                                                                 tile3))
+        split3 = fp3.split()
+        log("config 3 batched: %.3f ms/frame over %d cameras (graph), tile %s, %s" % (
+            t_graph * 1e3, args.cams, tile3, "two launches" if split3 else "one launch"))
         plain3 = PlainFrame(torch, wl3, cams[0], Q2, CUT2, tf)
         plain3.step()
         t_plain3 = events_time(torch, lambda: plain3.step(crop=False), 2)
@@ -560,7 +564,7 @@ def main():
                                         "+ 3-octave noise, 1920x1080, %d cameras" % args.cams,
                             "batched_graph_fps": 1.0 / t_graph, "batched_ms_per_frame": t_graph * 1e3,
                             "plain_per_block_fps": 1.0 / t_plain3, "speedup_vs_plain": t_plain3 / t_graph,
-                            "tile": list(tile3), "sparse_nodes": wl3.sparse_nodes}
+                            "tile": list(tile3), "split_launches": split3, "sparse_nodes": wl3.sparse_nodes}
         fp3.close()
         del frames, wl3
 

@@ -453,12 +453,20 @@ enum { RV_FRAME_SCHEDULE_BINS = 0, RV_FRAME_SCHEDULE_LANES = 1, RV_FRAME_SCHEDUL
 int rv_frame_set_schedule(rv_frame_plan* plan, uint32_t schedule);
 int rv_frame_get_schedule(const rv_frame_plan* plan, uint32_t* schedule);
 int rv_frame_set_hybrid_min(rv_frame_plan* plan, uint32_t blocks);
+/* Interpolating frames as one launch (0: the bricks' tiles and the ray-cast
+ * in one persistent kernel that waits for them) or two (1: a launch that
+ * interpolates the bricks, then the resident-brick kernel with its own grid).
+ * Same images; rv_frame_autotune with interpolate != 0 picks the faster. */
+int rv_frame_set_split(rv_frame_plan* plan, int split);
+int rv_frame_get_split(const rv_frame_plan* plan, int* split);
 /* Pick the bins' wave footprint (8x8, 16x4, 4x16, 32x2), the persistent
  * grid (resident workgroups per CU) of the mode's kernel (interpolate or
  * resident bricks) and, in Argb mode, the schedule, by timing candidates
  * on camera 0; the choice is cached (one JSON object per line) in
  * cache_path when it is not NULL.  The footprint and schedule apply to
- * both modes, the grid to the tuned mode. */
+ * both modes, the grid to the tuned mode.  With interpolate != 0 it also
+ * times the chosen frame split in two launches (rv_frame_set_split; the
+ * resident-brick kernel keeps its grid, so tune that mode first). */
 int rv_frame_autotune(rv_frame_plan* plan, uint32_t* d_argb, int interpolate,
                       const char* cache_path, rv_stream_t stream, uint32_t* tile_w,
                       uint32_t* tile_h);

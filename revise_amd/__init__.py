@@ -167,6 +167,8 @@ _SIGS = {
     "rv_frame_set_schedule": (C.c_int, [_vp, C.c_uint32]),
     "rv_frame_get_schedule": (C.c_int, [_vp, _u32p]),
     "rv_frame_set_hybrid_min": (C.c_int, [_vp, C.c_uint32]),
+    "rv_frame_set_split": (C.c_int, [_vp, C.c_int]),
+    "rv_frame_get_split": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "rv_frame_autotune": (C.c_int, [_vp, _vp, C.c_int, C.c_char_p, _vp, _u32p, _u32p]),
     "rv_frame_set_tape": (C.c_int, [_vp, C.c_int]),
     "rv_frame_set_field_mode": (C.c_int, [_vp, C.c_uint32, C.c_float, _f32p, C.c_uint32]),

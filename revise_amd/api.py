@@ -423,6 +423,16 @@ class FramePlan:
         check(lib().rv_frame_get_schedule(self.h, C.byref(v)))
         return v.value
 
+    def set_split(self, split):
+        """Interpolating frames in two launches (bricks, then the resident
+        kernel) instead of one; the same images."""
+        check(lib().rv_frame_set_split(self.h, 1 if split else 0))
+
+    def split(self):
+        v = C.c_int()
+        check(lib().rv_frame_get_split(self.h, C.byref(v)))
+        return bool(v.value)
+
     def set_field_mode(self, mode, iso_opacity=1.0, levels=None):
         lv, lp, ln = _levels(levels)
         check(lib().rv_frame_set_field_mode(self.h, mode, iso_opacity, lp, ln))

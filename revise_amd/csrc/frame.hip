@@ -396,6 +396,20 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
         viewport[pix] = value;
 }
 
+// The bricks of an interpolating frame as a launch of their own (a split
+// frame): one workgroup per brick tile, consecutive tiles on one XCD; the
+// resident-brick kernel follows in the stream.
+__global__ __launch_bounds__(kT) void frame_interp_kernel(const TileTask* __restrict__ tasks, uint32_t n)
+{
+    __shared__ __align__(16) float A[kTileFloats];
+    __shared__ float lat[64];
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    if (t >= n)
+        return;
+    const TileTask k = tasks[t];
+    interp_tile<kT>(k.job, k.local, k.e0, k.e_own, k.e1, A, lat);
+}
+
 __global__ void frame_reset_kernel(uint32_t* counters, uint32_t n)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
@@ -635,6 +649,16 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     a.lev = P->levels;
     hipLaunchKernelGGL(frame_reset_kernel, dim3(1), dim3(64), 0, s, P->d_counters, 4u);
     RV_LAUNCH_CHECK();
+    if (interpolate && P->split) {
+        if (a.n_interp) {
+            // xcd_tile needs a grid of a multiple of 8 workgroups
+            hipLaunchKernelGGL(frame_interp_kernel, dim3((a.n_interp + 7u) & ~7u), dim3(kT), 0, s, a.tasks,
+                               a.n_interp);
+            RV_LAUNCH_CHECK();
+        }
+        a.n_interp = 0u;
+        interpolate = 0;
+    }
     // the interpolating kernel waits for its bricks: its grid never exceeds
     // what the mode's kernel keeps resident
     uint32_t cap = P->mode_max_wgs;
@@ -1017,6 +1041,30 @@ extern "C" int rv_frame_set_hybrid_min(rv_frame_plan* P, uint32_t blocks)
     return RV_OK;
 }
 
+extern "C" int rv_frame_set_split(rv_frame_plan* P, int split)
+{
+    RV_REQUIRE(P, "rv_frame_set_split: null plan");
+    if ((split != 0) != P->split) {
+        if (P->graph_exec) { // the captured graph holds the other launches
+            (void)hipGraphExecDestroy(P->graph_exec);
+            P->graph_exec = nullptr;
+        }
+        if (P->graph) {
+            (void)hipGraphDestroy(P->graph);
+            P->graph = nullptr;
+        }
+        P->split = split != 0;
+    }
+    return RV_OK;
+}
+
+extern "C" int rv_frame_get_split(const rv_frame_plan* P, int* split)
+{
+    RV_REQUIRE(P && split, "rv_frame_get_split: null pointer");
+    *split = P->split ? 1 : 0;
+    return RV_OK;
+}
+
 // Autotuner: time every candidate wave footprint of the screen bins on the
 // plan's camera 0 (hip events, median of 3 after a warm-up) and keep the
 // fastest; the bins of the other footprints are released.  Results
@@ -1035,7 +1083,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     hipDeviceProp_t prop;
     RV_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream sig;
-    sig << prop.gcnArchName << "|bins3|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
+    sig << prop.gcnArchName << "|bins4|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
         << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth
         << "|mode" << P->field_mode;
     const std::string key = sig.str();
@@ -1063,6 +1111,11 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
                     (sch == RV_FRAME_SCHEDULE_BINS || P->field_mode == RV_FIELD_ARGB) &&
                     rv_frame_set_schedule(P, sch) == RV_OK)
                     (void)rv_frame_set_hybrid_min(P, hmin);
+                const size_t spl = line.find("\"split\": ");
+                unsigned split = 0;
+                if (interpolate && spl != std::string::npos &&
+                    std::sscanf(line.c_str() + spl + 9, "%u", &split) == 1)
+                    P->split = split != 0u;
                 const size_t rp = line.find("\"render_waves\": ");
                 if (interpolate && rp != std::string::npos &&
                     std::sscanf(line.c_str() + rp + 16, "%u", &rw) == 1 && rw >= 1 && rw <= kT / 64u)
@@ -1100,6 +1153,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     if (P->field_mode == RV_FIELD_ARGB)
         scheds = {0u, 1u, 2u};
     std::ostringstream times;
+    if (interpolate)
+        P->split = false; // the candidates below are one-launch frames
     // median of 3 timed frames of camera cam in a configuration
     auto time_cfg = [&](uint32_t sch, uint32_t tw, uint32_t th, uint32_t g, uint32_t rw, uint32_t cam,
                         float* ms) -> int {
@@ -1151,13 +1206,13 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     for (uint32_t sch : scheds)
         if (pick[sch].ms < pick[bsched].ms)
             bsched = sch;
+    std::vector<uint32_t> cams;
+    for (uint32_t q = 0; q < 4; ++q) {
+        const uint32_t cam = (uint32_t)((uint64_t)P->ncams * q / 4);
+        if (std::find(cams.begin(), cams.end(), cam) == cams.end())
+            cams.push_back(cam);
+    }
     if (scheds.size() > 1 && P->ncams > 1) {
-        std::vector<uint32_t> cams;
-        for (uint32_t q = 0; q < 4; ++q) {
-            const uint32_t cam = (uint32_t)((uint64_t)P->ncams * q / 4);
-            if (std::find(cams.begin(), cams.end(), cam) == cams.end())
-                cams.push_back(cam);
-        }
         float tot[3] = {0.f, 0.f, 0.f};
         for (uint32_t sch : scheds)
             for (uint32_t cam : cams) {
@@ -1180,6 +1235,27 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     }
     const Choice& bc = pick[bsched];
     const uint32_t bw = bc.w, bh = bc.h;
+    if (interpolate) {
+        // the chosen one-launch frame against the same frames split in two
+        // launches (interpolation, then the resident kernel with its grid)
+        float tot[2] = {0.f, 0.f};
+        for (int split = 0; split < 2; ++split) {
+            P->split = split != 0;
+            for (uint32_t cam : cams) {
+                float ms = 0.f;
+                const int rc = time_cfg(bsched, bw, bh, bc.g, bc.rw, cam, &ms);
+                if (rc) {
+                    P->split = false;
+                    (void)hipEventDestroy(e0);
+                    (void)hipEventDestroy(e1);
+                    return rc;
+                }
+                tot[split] += ms;
+            }
+        }
+        P->split = tot[1] < tot[0];
+        times << ", \"one launch over " << cams.size() << " cameras\": " << tot[0] << ", \"split\": " << tot[1];
+    }
     grid = P->cus * bc.g;
     if (interpolate)
         P->frame_rw = bc.rw;
@@ -1201,7 +1277,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
         std::ofstream f(cache_path, std::ios::app);
         f << "{\"key\": \"" << key << "\", \"tile\": [" << bw << ", " << bh
           << "], \"wgs_per_cu\": " << bg << ", \"render_waves\": " << P->frame_rw
-          << ", \"schedule\": " << P->schedule << ", \"hybrid_min\": " << P->hybrid_min << ", \"ms\": {"
+          << ", \"schedule\": " << P->schedule << ", \"hybrid_min\": " << P->hybrid_min
+          << ", \"split\": " << (P->split ? 1 : 0) << ", \"ms\": {"
           << times.str() << "}}\n";
     }
     return RV_OK;

@@ -51,6 +51,7 @@ struct rv_frame_plan {
     rv_frame_desc desc{};
     bool fp16 = false;
     bool tiled = false; // fp32 bricks in 4 x 4 x 4 micro-bricks (RV_BRICK_TILED4)
+    bool split = false; // interpolating frames as two launches (bricks, then the resident kernel)
     TileTask* d_tasks = nullptr;
     uint32_t n_interp = 0;
     const void** d_bricks = nullptr;

@@ -140,6 +140,53 @@ def test_frame_mixed_levels_matches_oracle(orc, schedule, layout):
         assert np.array_equal(rgba.cpu().numpy(), exp)
 
 
+@pytest.mark.parametrize("layout", [api.BRICK_LINEAR, api.BRICK_TILED4])
+@pytest.mark.parametrize("schedule", [api.SCHEDULE_BINS, api.SCHEDULE_HYBRID])
+def test_frame_split_launches_match_oracle(orc, schedule, layout):
+    """Interpolating frames split in two launches (the bricks, then the
+    resident-brick kernel): bricks cleared before every frame are rebuilt,
+    the images equal the oracle's and the one-launch frame's, also when
+    replayed from a graph."""
+    import torch
+    blocks = _mixed_blocks(seed=12)
+    W, H = 170, 130
+    fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=12, layout=layout)
+    _set_schedule(fp, schedule)
+    assert not fp.split()
+    cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((30.0, 10.0), (200.0, -30.0))])
+    fp.set_cameras(cams)
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    one = []
+    for c in range(2):
+        fp.render(c, argb, rgba, interpolate=True)
+        one.append(argb.cpu().numpy().copy())
+    fp.set_split(True)
+    assert fp.split()
+    for c in range(2):
+        say("split frame cam", c, "schedule", schedule, "layout", layout)
+        for b in dev:
+            b["brick"].zero_()
+        fp.render(c, argb, rgba, interpolate=True)
+        torch.cuda.synchronize()
+        assert fp.status() == 0
+        exp = _oracle_frame(orc, blocks, bricks, cams[c], W, H, fmin, fmax, tf, 0.9999)
+        assert np.array_equal(rgba.cpu().numpy(), exp)
+        assert np.array_equal(argb.cpu().numpy(), one[c])
+    frames = torch.zeros((2, H, W), dtype=torch.int32, device="cuda")
+    fp.graph_build(frames, H * W, interpolate=True)
+    for rep in range(2):
+        for b in dev:
+            b["brick"].zero_()
+        frames.zero_()
+        fp.graph_launch()
+        torch.cuda.synchronize()
+        got = frames.cpu().numpy()
+        for c in range(2):
+            assert np.array_equal(got[c], one[c]), (rep, c)
+
+
 @pytest.mark.parametrize("schedule", SCHEDULES)
 def test_frame_graph_replays_match_eager(orc, schedule):
     import torch
@@ -186,6 +233,7 @@ def test_frame_autotune_picks_valid_tile(tmp_path):
     assert (w2, h2) == (w, h) and fp.schedule() == sched
     text = open(cache).read()
     assert text.count("\n") == 1 and '"schedule": %d' % sched in text
+    assert '"split": %d' % int(fp.split()) in text
 
 
 @pytest.mark.parametrize("schedule", SCHEDULES)
