@@ -398,16 +398,20 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
 
 // The bricks of an interpolating frame as a launch of their own (a split
 // frame): one workgroup per brick tile, consecutive tiles on one XCD; the
-// resident-brick kernel follows in the stream.
-__global__ __launch_bounds__(kT) void frame_interp_kernel(const TileTask* __restrict__ tasks, uint32_t n)
+// resident-brick kernel follows in the stream.  Two LDS classes, as in the
+// standalone batch: 512 threads for tiles of up to 65 x 17 x 17 nodes, 256
+// for plans whose tiles all fit 65 x 9 x 9 (bricks of depth <= 4 with
+// 16-cell tiles), which keeps more workgroups per CU.
+template <uint32_t kFloats, uint32_t NT>
+__global__ __launch_bounds__(NT) void frame_interp_kernel(const TileTask* __restrict__ tasks, uint32_t n)
 {
-    __shared__ __align__(16) float A[kTileFloats];
+    __shared__ __align__(16) float A[kFloats];
     __shared__ float lat[64];
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     if (t >= n)
         return;
     const TileTask k = tasks[t];
-    interp_tile<kT>(k.job, k.local, k.e0, k.e_own, k.e1, A, lat);
+    interp_tile<NT>(k.job, k.local, k.e0, k.e_own, k.e1, A, lat);
 }
 
 __global__ void frame_reset_kernel(uint32_t* counters, uint32_t n)
@@ -652,8 +656,12 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     if (interpolate && P->split) {
         if (a.n_interp) {
             // xcd_tile needs a grid of a multiple of 8 workgroups
-            hipLaunchKernelGGL(frame_interp_kernel, dim3((a.n_interp + 7u) & ~7u), dim3(kT), 0, s, a.tasks,
-                               a.n_interp);
+            if (P->interp_small)
+                hipLaunchKernelGGL((frame_interp_kernel<kTileFloats8, 256u>), dim3((a.n_interp + 7u) & ~7u),
+                                   dim3(256u), 0, s, a.tasks, a.n_interp);
+            else
+                hipLaunchKernelGGL((frame_interp_kernel<kTileFloats, kT>), dim3((a.n_interp + 7u) & ~7u),
+                                   dim3(kT), 0, s, a.tasks, a.n_interp);
             RV_LAUNCH_CHECK();
         }
         a.n_interp = 0u;
@@ -721,9 +729,11 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     P->tiled = desc->layout == RV_BRICK_TILED4;
     std::vector<Job> jobs(nblocks);
     std::vector<const void*> bricks(nblocks);
+    bool small = true;
     for (uint32_t i = 0; i < nblocks; ++i) {
         const rv_frame_block& b = blocks[i];
         RV_REQUIRE(b.depth <= kMaxDepth, "rv_frame_plan_create: depth must be <= 8");
+        small = small && tile_floats(geom_of(b.depth)) <= kTileFloats8;
         RV_REQUIRE(b.level <= 16, "rv_frame_plan_create: level too large");
         for (int k = 0; k < 3; ++k)
             RV_REQUIRE(b.index[k] < (1u << b.level), "rv_frame_plan_create: index outside level");
@@ -736,6 +746,7 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
             P->can_interp = false;
     }
     P->fp16 = fp16;
+    P->interp_small = small;
     int rc;
     if (P->can_interp) {
         std::vector<TileTask> tasks;

@@ -52,6 +52,7 @@ struct rv_frame_plan {
     bool fp16 = false;
     bool tiled = false; // fp32 bricks in 4 x 4 x 4 micro-bricks (RV_BRICK_TILED4)
     bool split = false; // interpolating frames as two launches (bricks, then the resident kernel)
+    bool interp_small = false; // every brick tile fits the small LDS class (65 x 9 x 9 nodes)
     TileTask* d_tasks = nullptr;
     uint32_t n_interp = 0;
     const void** d_bricks = nullptr;

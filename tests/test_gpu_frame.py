@@ -140,9 +140,10 @@ def test_frame_mixed_levels_matches_oracle(orc, schedule, layout):
         assert np.array_equal(rgba.cpu().numpy(), exp)
 
 
-@pytest.mark.parametrize("layout", [api.BRICK_LINEAR, api.BRICK_TILED4])
+@pytest.mark.parametrize("layout,fp16", [(api.BRICK_LINEAR, False), (api.BRICK_TILED4, False),
+                                         (api.BRICK_LINEAR, True)])
 @pytest.mark.parametrize("schedule", [api.SCHEDULE_BINS, api.SCHEDULE_HYBRID])
-def test_frame_split_launches_match_oracle(orc, schedule, layout):
+def test_frame_split_launches_match_oracle(orc, schedule, layout, fp16):
     """Interpolating frames split in two launches (the bricks, then the
     resident-brick kernel): bricks cleared before every frame are rebuilt,
     the images equal the oracle's and the one-launch frame's, also when
@@ -150,7 +151,7 @@ def test_frame_split_launches_match_oracle(orc, schedule, layout):
     import torch
     blocks = _mixed_blocks(seed=12)
     W, H = 170, 130
-    fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=12, layout=layout)
+    fp, tf, fmin, fmax, dev = _setup(blocks, W, H, fp16=fp16, acc_cut=0.9999, seed=12, layout=layout)
     _set_schedule(fp, schedule)
     assert not fp.split()
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((30.0, 10.0), (200.0, -30.0))])
@@ -158,6 +159,8 @@ def test_frame_split_launches_match_oracle(orc, schedule, layout):
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
     rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
     bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    if fp16:
+        bricks = [x.astype(np.float16).astype(np.float32) for x in bricks]
     one = []
     for c in range(2):
         fp.render(c, argb, rgba, interpolate=True)
@@ -165,7 +168,7 @@ def test_frame_split_launches_match_oracle(orc, schedule, layout):
     fp.set_split(True)
     assert fp.split()
     for c in range(2):
-        say("split frame cam", c, "schedule", schedule, "layout", layout)
+        say("split frame cam", c, "schedule", schedule, "layout", layout, "fp16", fp16)
         for b in dev:
             b["brick"].zero_()
         fp.render(c, argb, rgba, interpolate=True)
@@ -185,6 +188,36 @@ def test_frame_split_launches_match_oracle(orc, schedule, layout):
         got = frames.cpu().numpy()
         for c in range(2):
             assert np.array_equal(got[c], one[c]), (rep, c)
+
+
+@pytest.mark.parametrize("fp16", [False, True])
+def test_frame_split_small_tiles_match_one_launch(orc, fp16):
+    """Split frames of depth-4 bricks, whose tiles all fit the small LDS
+    class (256-thread workgroups): the bricks are rebuilt and the image
+    equals the oracle's and the one-launch frame's."""
+    import torch
+    blocks = _uniform_blocks(2, 4, seed=14)
+    W, H = 150, 120
+    fp, tf, fmin, fmax, dev = _setup(blocks, W, H, fp16=fp16, acc_cut=0.9999, seed=14)
+    fp.set_cameras(datasets.orbit_camera(110.0, elevation_deg=20.0)[None])
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    fp.render(0, argb, rgba, interpolate=True)
+    one = argb.cpu().numpy().copy()
+    fp.set_split(True)
+    for b in dev:
+        b["brick"].zero_()
+    say("split frame, depth-4 bricks, fp16", fp16)
+    fp.render(0, argb, rgba, interpolate=True)
+    torch.cuda.synchronize()
+    assert fp.status() == 0
+    bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+    if fp16:
+        bricks = [x.astype(np.float16).astype(np.float32) for x in bricks]
+    exp = _oracle_frame(orc, blocks, bricks, datasets.orbit_camera(110.0, elevation_deg=20.0), W, H, fmin, fmax,
+                        tf, 0.9999)
+    assert np.array_equal(rgba.cpu().numpy(), exp)
+    assert np.array_equal(argb.cpu().numpy(), one)
 
 
 @pytest.mark.parametrize("schedule", SCHEDULES)
