@@ -195,6 +195,7 @@ template <uint32_t NT>
 __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t e0, uint32_t e_own,
                                    uint32_t e1, float* A, float* lat)
 {
+    static_assert(NT % 64u == 0u, "whole waves per workgroup");
     const PlanView P = plan_view(job.plan);
     const Geom g = geom_of(job.depth);
     const uint32_t d = g.d, X = g.X, T = g.T;
@@ -305,7 +306,10 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
                        nmz = ((z0 + ez + 3u) >> 2) - mz0;
         const uint32_t total = nmx * nmy * nmz * 64u;
         for (uint32_t q = threadIdx.x; q < total; q += NT) {
-            const uint32_t mb = q >> 6, e = q & 63u;
+            // a wave's 64 lanes store one micro-brick (NT and total are
+            // multiples of 64): its index is wave-uniform, so the divisions
+            // below run on the scalar unit
+            const uint32_t mb = __builtin_amdgcn_readfirstlane(q >> 6), e = q & 63u;
             const uint32_t mx = mx0 + mb % nmx, myz = mb / nmx;
             const uint32_t my = my0 + myz % nmy, mz = mz0 + myz / nmy;
             const uint32_t x = (mx << 2) | (e & 3u), y = (my << 2) | ((e >> 2) & 3u), z = (mz << 2) | (e >> 4);
