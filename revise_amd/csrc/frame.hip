@@ -739,6 +739,8 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
             RV_REQUIRE(b.index[k] < (1u << b.level), "rv_frame_plan_create: index outside level");
         RV_REQUIRE((b.brick16 != nullptr) == fp16 && (b.brick != nullptr) != fp16,
                    "rv_frame_plan_create: all blocks must use the same brick type");
+        RV_REQUIRE(!P->tiled || (reinterpret_cast<uintptr_t>(b.brick) & 15u) == 0u,
+                   "rv_frame_plan_create: TILED4 bricks must be 16-byte aligned");
         jobs[i] = Job{b.sparse, static_cast<const uint8_t*>(b.plan), b.brick,
                       reinterpret_cast<__half*>(b.brick16), b.depth, P->tiled ? 1u : 0u};
         bricks[i] = fp16 ? (const void*)b.brick16 : (const void*)b.brick;
@@ -1335,6 +1337,8 @@ extern "C" int rv_frame_set_secondary(rv_frame_plan* P, const float* const* d_sp
         for (uint32_t i = 0; i < P->nblocks; ++i) {
             const rv_frame_block& b = P->blocks[i];
             RV_REQUIRE(d_sparse2[i], "rv_frame_set_secondary: null secondary sparse values");
+            RV_REQUIRE(!P->tiled || (reinterpret_cast<uintptr_t>(d_bricks2[i]) & 15u) == 0u,
+                       "rv_frame_set_secondary: TILED4 bricks must be 16-byte aligned");
             jobs[i] = Job{b.sparse, static_cast<const uint8_t*>(b.plan), b.brick,
                           reinterpret_cast<__half*>(b.brick16), b.depth, P->tiled ? 1u : 0u};
             jobs[P->nblocks + i] =

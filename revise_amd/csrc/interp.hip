@@ -317,6 +317,8 @@ extern "C" int rv_interp_batch_create(const rv_brick_job* jobs, uint32_t count,
                    "rv_interp_batch_create: exactly one of dense/dense16 must be set");
         RV_REQUIRE(j.layout == RV_BRICK_LINEAR || (j.layout == RV_BRICK_TILED4 && j.dense),
                    "rv_interp_batch_create: unknown layout (TILED4 bricks are fp32)");
+        RV_REQUIRE(j.layout != RV_BRICK_TILED4 || (reinterpret_cast<uintptr_t>(j.dense) & 15u) == 0u,
+                   "rv_interp_batch_create: TILED4 bricks must be 16-byte aligned");
         host[i] = Job{j.sparse, static_cast<const uint8_t*>(j.plan), j.dense,
                       reinterpret_cast<__half*>(j.dense16), j.depth,
                       j.layout == RV_BRICK_TILED4 ? 1u : 0u};

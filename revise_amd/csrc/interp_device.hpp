@@ -297,28 +297,53 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
     if (job.tiled) {
         // the micro-bricks of the owned box (tile boundaries are multiples of
         // 4 whenever a brick has several tiles), 64 consecutive floats each
-        const uint32_t n = g.n, G = (n + 3u) >> 2;
+        const uint32_t G = (g.n + 3u) >> 2;
         const uint32_t ex = (tx + 1u == g.tiles_x) ? rowlen : X;
         const uint32_t ey = (ty + 1u == g.tiles_yz) ? tp1 : T;
         const uint32_t ez = (tz + 1u == g.tiles_yz) ? tp1 : T;
         const uint32_t mx0 = x0 >> 2, my0 = y0 >> 2, mz0 = z0 >> 2;
         const uint32_t nmx = ((x0 + ex + 3u) >> 2) - mx0, nmy = ((y0 + ey + 3u) >> 2) - my0,
                        nmz = ((z0 + ez + 3u) >> 2) - mz0;
-        const uint32_t total = nmx * nmy * nmz * 64u;
+        // one micro-brick row (4 nodes along x, 16 bytes) per lane: a wave
+        // stores 4 whole micro-bricks, and its LDS reads of 4 consecutive x
+        // meet few bank conflicts.  The wave's first micro-brick index is
+        // wave-uniform (NT is a multiple of 64), so its decomposition runs on
+        // the scalar unit; the lanes' offsets 0..3 are carried along x.
+        // Padding nodes (beyond the brick) are written NaN, never read.
+        const uint32_t total = nmx * nmy * nmz * 16u;
         for (uint32_t q = threadIdx.x; q < total; q += NT) {
-            // a wave's 64 lanes store one micro-brick (NT and total are
-            // multiples of 64): its index is wave-uniform, so the divisions
-            // below run on the scalar unit
-            const uint32_t mb = __builtin_amdgcn_readfirstlane(q >> 6), e = q & 63u;
-            const uint32_t mx = mx0 + mb % nmx, myz = mb / nmx;
-            const uint32_t my = my0 + myz % nmy, mz = mz0 + myz / nmy;
-            const uint32_t x = (mx << 2) | (e & 3u), y = (my << 2) | ((e >> 2) & 3u), z = (mz << 2) | (e >> 4);
-            if (x >= x0 + ex || y >= y0 + ey || z >= z0 + ez)
-                continue; // padding beyond the brick
-            float v = A[((z - z0) * tp1 + (y - y0)) * rowlen + (x - x0)];
-            if (v == kNoField)
-                v = __builtin_nanf("");
-            __builtin_nontemporal_store(v, job.dense + (((mz * G + my) * G + mx) << 6) + e);
+            const uint32_t mbw = __builtin_amdgcn_readfirstlane(q >> 6) << 2, r = q & 15u;
+            uint32_t mx = mbw % nmx, my = (mbw / nmx) % nmy, mz = mbw / (nmx * nmy);
+            mx += (q >> 4) & 3u;
+            while (mx >= nmx) { // at most 3 steps
+                mx -= nmx;
+                if (++my == nmy) {
+                    my = 0u;
+                    ++mz;
+                }
+            }
+            mx += mx0;
+            my += my0;
+            mz += mz0;
+            const uint32_t y = (my << 2) | (r & 3u), z = (mz << 2) | (r >> 2), xb = mx << 2;
+            // Outside the owned box lie only padding nodes: tile boundaries
+            // are multiples of 4, and every micro-brick belongs to one tile.
+            float o[4];
+            const bool row_ok = y < y0 + ey && z < z0 + ez;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float v = __builtin_nanf("");
+                if (row_ok && xb + (uint32_t)c < x0 + ex) {
+                    v = A[((z - z0) * tp1 + (y - y0)) * rowlen + (xb - x0) + (uint32_t)c];
+                    if (v == kNoField)
+                        v = __builtin_nanf("");
+                }
+                o[c] = v;
+            }
+            typedef float f32x4 __attribute__((ext_vector_type(4)));
+            const f32x4 row = {o[0], o[1], o[2], o[3]};
+            __builtin_nontemporal_store(row, reinterpret_cast<f32x4*>(job.dense + (((mz * G + my) * G + mx) << 6) +
+                                                                      (r << 2)));
         }
         return;
     }

@@ -233,7 +233,8 @@ def test_batch_tiled_layout_holds_the_linear_values(orc, seed, depth, p, pnf):
     t = til.cpu().numpy()
     assert bits_equal_nan(t[idx.reshape(-1)], lin.cpu().numpy())
     assert bits_equal_nan(lin.cpu().numpy(), orc.interpolate(vals, n2i, d))
-    # padding is never written
+    # the padding nodes (never read) are written NaN: each micro-brick row
+    # goes out as one 16-byte store
     pad = np.ones(t.size, bool)
     pad[idx.reshape(-1)] = False
-    assert np.all(t[pad] == 12345.0)
+    assert np.all(np.isnan(t[pad]))
