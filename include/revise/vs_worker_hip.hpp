@@ -153,8 +153,10 @@ private:
     float* m_brick = nullptr;
     uint64_t m_brickNodes = 0;
     std::array<uint32_t, 4> m_lastBlock{~0u, 0, 0, 0};
+    uint64_t m_lastVersion = ~0ull; // field-service version the brick was interpolated at
     float* m_brick2 = nullptr; // secondary field brick (ValueOnIsosurface(s))
     std::array<uint32_t, 4> m_lastBlock2{~0u, 0, 0, 0};
+    uint64_t m_lastVersion2 = ~0ull;
 };
 
 class VsWorkerHip {

@@ -1054,18 +1054,26 @@ extern "C" int rv_frame_set_hybrid_min(rv_frame_plan* P, uint32_t blocks)
     return RV_OK;
 }
 
+// A captured graph holds the launch structure, grid and bins it was captured
+// with; every change of them drops it (rv_frame_graph_launch then fails until
+// rv_frame_graph_build runs again).
+static void drop_graph(rv_frame_plan* P)
+{
+    if (P->graph_exec) {
+        (void)hipGraphExecDestroy(P->graph_exec);
+        P->graph_exec = nullptr;
+    }
+    if (P->graph) {
+        (void)hipGraphDestroy(P->graph);
+        P->graph = nullptr;
+    }
+}
+
 extern "C" int rv_frame_set_split(rv_frame_plan* P, int split)
 {
     RV_REQUIRE(P, "rv_frame_set_split: null plan");
     if ((split != 0) != P->split) {
-        if (P->graph_exec) { // the captured graph holds the other launches
-            (void)hipGraphExecDestroy(P->graph_exec);
-            P->graph_exec = nullptr;
-        }
-        if (P->graph) {
-            (void)hipGraphDestroy(P->graph);
-            P->graph = nullptr;
-        }
+        drop_graph(P);
         P->split = split != 0;
     }
     return RV_OK;
@@ -1090,15 +1098,18 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     RV_TRY
     RV_REQUIRE(P && d_argb && P->ncams > 0, "rv_frame_autotune: bad arguments");
     RV_REQUIRE(!interpolate || P->can_interp, "rv_frame_autotune: blocks have no sparse values/plans");
+    // tuning changes the grid and footprint and releases the bins of the
+    // footprints not chosen, which a graph captured before may hold
+    drop_graph(P);
     static const uint32_t cand[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}};
     int dev = 0;
     RV_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     RV_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream sig;
-    sig << prop.gcnArchName << "|bins4|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
+    sig << prop.gcnArchName << "|bins5|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
         << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth
-        << "|mode" << P->field_mode;
+        << "|mode" << P->field_mode << "|layout" << P->desc.layout << "|small" << (P->interp_small ? 1 : 0);
     const std::string key = sig.str();
     if (cache_path && *cache_path) {
         std::ifstream f(cache_path);
@@ -1128,7 +1139,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
                 unsigned split = 0;
                 if (interpolate && spl != std::string::npos &&
                     std::sscanf(line.c_str() + spl + 9, "%u", &split) == 1)
-                    P->split = split != 0u;
+                    (void)rv_frame_set_split(P, (int)split); // drops a graph of the other structure
                 const size_t rp = line.find("\"render_waves\": ");
                 if (interpolate && rp != std::string::npos &&
                     std::sscanf(line.c_str() + rp + 16, "%u", &rw) == 1 && rw >= 1 && rw <= kT / 64u)
@@ -1167,7 +1178,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
         scheds = {0u, 1u, 2u};
     std::ostringstream times;
     if (interpolate)
-        P->split = false; // the candidates below are one-launch frames
+        (void)rv_frame_set_split(P, 0); // the candidates below are one-launch frames
     // median of 3 timed frames of camera cam in a configuration
     auto time_cfg = [&](uint32_t sch, uint32_t tw, uint32_t th, uint32_t g, uint32_t rw, uint32_t cam,
                         float* ms) -> int {
@@ -1253,12 +1264,12 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
         // launches (interpolation, then the resident kernel with its grid)
         float tot[2] = {0.f, 0.f};
         for (int split = 0; split < 2; ++split) {
-            P->split = split != 0;
+            (void)rv_frame_set_split(P, split);
             for (uint32_t cam : cams) {
                 float ms = 0.f;
                 const int rc = time_cfg(bsched, bw, bh, bc.g, bc.rw, cam, &ms);
                 if (rc) {
-                    P->split = false;
+                    (void)rv_frame_set_split(P, 0);
                     (void)hipEventDestroy(e0);
                     (void)hipEventDestroy(e1);
                     return rc;
@@ -1266,7 +1277,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
                 tot[split] += ms;
             }
         }
-        P->split = tot[1] < tot[0];
+        (void)rv_frame_set_split(P, tot[1] < tot[0] ? 1 : 0);
         times << ", \"one launch over " << cams.size() << " cameras\": " << tot[0] << ", \"split\": " << tot[1];
     }
     grid = P->cus * bc.g;

@@ -38,8 +38,10 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 extern "C" uint64_t rv_interp_plan_bytes(const uint16_t* n2i, uint64_t n, uint32_t depth);
@@ -121,39 +123,78 @@ uint32_t round8(uint32_t v) { return (v + 7u) & ~7u; }
 
 } // namespace
 
+namespace {
+
+// A plan's tile table as the batch builder reads it: per tile, its entry
+// range and the end of its owned entries.
+struct PlanTable {
+    uint32_t depth = ~0u;
+    std::vector<uint32_t> start, own;
+};
+
+// Reads the tile table from a plan's bytes; get(dst, offset, bytes) copies
+// them from wherever the plan lives.
+template <class Get>
+int read_plan_table(Get get, PlanTable& tb)
+{
+    uint32_t head[kPlanHeader / 4];
+    RV_HIP(get(head, 0, kPlanHeader));
+    if (head[0] != kPlanMagic)
+        return fail(RV_ERR_INVALID, "interpolation job: plan is not an interpolation plan");
+    tb.depth = head[1];
+    if (tb.depth <= kMaxDepth && head[2] == geom_of(tb.depth).ntiles) {
+        tb.start.resize(head[2] + 1u);
+        RV_HIP(get(tb.start.data(), kPlanHeader, 4ull * tb.start.size()));
+        tb.own.resize(head[2]);
+        RV_HIP(get(tb.own.data(), head[10], 4ull * tb.own.size()));
+        for (uint32_t t = 0; t < head[2]; ++t)
+            if (tb.own[t] < tb.start[t] || tb.own[t] > tb.start[t + 1])
+                return fail(RV_ERR_INVALID, "interpolation job: corrupt plan");
+    }
+    return RV_OK;
+}
+
+// Tile tables of the plans rv_interp_plan_build uploaded, read from the host
+// copy at upload time and dropped by rv_interp_plan_free: building a batch
+// over them reads nothing back from the device.  Plans uploaded by the
+// caller are read back once per batch.
+std::mutex g_plan_mu;
+std::map<const void*, PlanTable> g_plan_tables;
+
+} // namespace
+
 int build_tile_tasks(const Job* jobs, uint32_t njobs, std::vector<TileTask>& tasks)
 {
-    struct Table {
-        uint32_t depth;
-        std::vector<uint32_t> start, own;
-    };
-    std::map<const uint8_t*, Table> seen;
+    std::map<const uint8_t*, PlanTable> seen;
     tasks.clear();
     for (uint32_t i = 0; i < njobs; ++i) {
         const Job& j = jobs[i];
         const Geom g = geom_of(j.depth);
         auto it = seen.find(j.plan);
         if (it == seen.end()) {
-            uint32_t head[kPlanHeader / 4];
-            RV_HIP(hipMemcpy(head, j.plan, kPlanHeader, hipMemcpyDeviceToHost));
-            if (head[0] != kPlanMagic)
-                return fail(RV_ERR_INVALID, "interpolation job: plan is not an interpolation plan");
-            Table tb;
-            tb.depth = head[1];
-            if (tb.depth <= kMaxDepth && head[2] == geom_of(tb.depth).ntiles) {
-                tb.start.resize(head[2] + 1u);
-                RV_HIP(hipMemcpy(tb.start.data(), j.plan + kPlanHeader, 4ull * tb.start.size(),
-                                 hipMemcpyDeviceToHost));
-                tb.own.resize(head[2]);
-                RV_HIP(hipMemcpy(tb.own.data(), j.plan + head[10], 4ull * tb.own.size(),
-                                 hipMemcpyDeviceToHost));
-                for (uint32_t t = 0; t < head[2]; ++t)
-                    if (tb.own[t] < tb.start[t] || tb.own[t] > tb.start[t + 1])
-                        return fail(RV_ERR_INVALID, "interpolation job: corrupt plan");
+            PlanTable tb;
+            bool known = false;
+            {
+                std::lock_guard<std::mutex> lk(g_plan_mu);
+                auto r = g_plan_tables.find(j.plan);
+                if (r != g_plan_tables.end()) {
+                    tb = r->second;
+                    known = true;
+                }
+            }
+            if (!known) {
+                const uint8_t* plan = j.plan;
+                int rc = read_plan_table(
+                    [plan](void* dst, uint64_t off, uint64_t n) {
+                        return hipMemcpy(dst, plan + off, n, hipMemcpyDeviceToHost);
+                    },
+                    tb);
+                if (rc)
+                    return rc;
             }
             it = seen.emplace(j.plan, std::move(tb)).first;
         }
-        const Table& tb = it->second;
+        const PlanTable& tb = it->second;
         if (tb.depth != j.depth || tb.start.size() != g.ntiles + 1u)
             return fail(RV_ERR_INVALID, "interpolation job: plan was built for another depth");
         for (uint32_t t = 0; t < g.ntiles; ++t)
@@ -225,6 +266,22 @@ extern "C" int rv_interp_plan_build(const uint16_t* n2i, uint64_t n, uint32_t de
         (void)hipFree(d);
         return hip_fail(e, "hipMemcpy", __FILE__, __LINE__);
     }
+    PlanTable tb;
+    const uint8_t* hp = host.data();
+    rc = read_plan_table(
+        [hp](void* dst, uint64_t off, uint64_t n) {
+            std::memcpy(dst, hp + off, n);
+            return hipSuccess;
+        },
+        tb);
+    if (rc) {
+        (void)hipFree(d);
+        return rc;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        g_plan_tables[d] = std::move(tb);
+    }
     *d_plan = d;
     if (bytes_out)
         *bytes_out = bytes;
@@ -234,8 +291,13 @@ extern "C" int rv_interp_plan_build(const uint16_t* n2i, uint64_t n, uint32_t de
 
 extern "C" int rv_interp_plan_free(void* d_plan)
 {
-    if (d_plan)
+    if (d_plan) {
+        {
+            std::lock_guard<std::mutex> lk(g_plan_mu);
+            g_plan_tables.erase(d_plan);
+        }
         RV_HIP(hipFree(d_plan));
+    }
     return RV_OK;
 }
 

@@ -234,15 +234,20 @@ void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
         m_lastBlock = {~0u, 0, 0, 0};
         m_lastBlock2 = {~0u, 0, 0, 0};
     }
+    // the brick is also stale when the field service changed since it was
+    // interpolated (new values of the block, or of its secondary field; the
+    // reference marks its fields dirty likewise, DenseFieldRenderer_cu.cu:238-259)
     const std::array<uint32_t, 4> key{level, index[0], index[1], index[2]};
-    if (key != m_lastBlock) {
+    if (key != m_lastBlock || m_lastVersion != fs.version()) {
         fs.interpolate(level, index, m_brick, m_stream);
         m_lastBlock = key;
+        m_lastVersion = fs.version();
     }
     const bool second = hasSecondaryField(st);
-    if (second && key != m_lastBlock2) {
+    if (second && (key != m_lastBlock2 || m_lastVersion2 != fs.version())) {
         fs.interpolateSecondary(level, index, m_brick2, m_stream);
         m_lastBlock2 = key;
+        m_lastVersion2 = fs.version();
     }
     ensureColorTable();
     if (isCancelled())
@@ -289,25 +294,60 @@ void DenseFieldRendererHip::ensureColorTable()
 
 // ---------------------------------------------------------------- worker
 
-// Device state of the fused share path for one (level, index box, field
-// version): one brick per block, the batched interpolation handle, the block
-// table workspace.
+// Device state of the fused share path.  Bricks are kept per block (level,
+// index), each with its float count, so a block whose subtree grows deeper
+// (a later addSubtree) gets a larger brick.  The batched interpolation
+// handles are kept per (level, index box, secondary) -- composeFrame and the
+// level-of-detail loop render several boxes per frame -- up to kMaxBatches,
+// the least recently used dropped first; a field-service version change
+// drops them all.
 struct VsWorkerHip::ShareCache {
-    std::array<uint32_t, 8> key{};
+    static constexpr size_t kMaxBatches = 64;
+    struct Brick {
+        float* p = nullptr;
+        uint64_t floats = 0;
+    };
+    struct Batch {
+        rv_interp_batch* batch = nullptr;
+        uint64_t used = 0;
+    };
     uint64_t version = ~0ull;
-    std::map<std::array<uint32_t, 3>, float*> bricks;
-    std::map<std::array<uint32_t, 3>, float*> bricks2; // secondary field (ValueOnIsosurface(s))
-    rv_interp_batch* batch = nullptr;
+    uint64_t clock = 0;
+    std::map<std::array<uint32_t, 4>, Brick> bricks;
+    std::map<std::array<uint32_t, 4>, Brick> bricks2; // secondary field (ValueOnIsosurface(s))
+    std::map<std::array<uint32_t, 8>, Batch> batches;
     void* workspace = nullptr;
     uint64_t workspace_bytes = 0;
+    void dropBatches()
+    {
+        for (auto& kv : batches)
+            rv_interp_batch_destroy(kv.second.batch);
+        batches.clear();
+    }
+    // The block's brick of at least `floats` floats; *grown = true when it
+    // was (re)allocated.
+    float* brick(std::map<std::array<uint32_t, 4>, Brick>& m, const std::array<uint32_t, 4>& key,
+                 uint64_t floats, bool* grown)
+    {
+        Brick& b = m[key];
+        if (b.floats < floats) {
+            if (b.p)
+                (void)hipFree(b.p);
+            b.p = nullptr;
+            b.floats = 0;
+            hipCheck(hipMalloc((void**)&b.p, 4 * floats), "hipMalloc");
+            b.floats = floats;
+            *grown = true;
+        }
+        return b.p;
+    }
     ~ShareCache()
     {
-        if (batch)
-            rv_interp_batch_destroy(batch);
+        dropBatches();
         for (auto& kv : bricks)
-            (void)hipFree(kv.second);
+            (void)hipFree(kv.second.p);
         for (auto& kv : bricks2)
-            (void)hipFree(kv.second);
+            (void)hipFree(kv.second.p);
         if (workspace)
             (void)hipFree(workspace);
     }
@@ -450,39 +490,49 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
     const bool second = hasSecondaryField(st);
     const std::array<uint32_t, 8> key{sub.level,     sub.boxMin[0], sub.boxMin[1], sub.boxMin[2],
                                       sub.boxMax[0], sub.boxMax[1], sub.boxMax[2], second ? 1u : 0u};
-    if (key != sc.key || sc.version != fs.version() || !sc.batch) {
-        if (sc.batch)
-            rv_interp_batch_destroy(sc.batch);
-        sc.batch = nullptr;
+    if (sc.version != fs.version()) {
+        sc.dropBatches();
+        sc.version = fs.version();
+    }
+    auto hit = sc.batches.find(key);
+    if (hit == sc.batches.end()) {
         std::vector<rv_brick_job> jobs;
-        jobs.reserve(count);
+        jobs.reserve(second ? 2 * count : count);
+        bool grown = false;
         for (uint64_t i = 0; i < count; ++i) {
             const std::array<uint32_t, 3> idx{order[3 * i], order[3 * i + 1], order[3 * i + 2]};
+            const std::array<uint32_t, 4> bkey{sub.level, idx[0], idx[1], idx[2]};
             // the share's bricks in micro-bricks (fewer cache lines per
             // ray-cast corner load; the same values)
             const uint64_t floats = rv_brick_floats(fs.subtreeDepth(sub.level, idx), RV_BRICK_TILED4);
-            float*& brick = sc.bricks[idx];
-            if (!brick)
-                hipCheck(hipMalloc((void**)&brick, 4 * floats), "hipMalloc");
-            jobs.push_back(fs.job(sub.level, idx, brick, s));
+            jobs.push_back(fs.job(sub.level, idx, sc.brick(sc.bricks, bkey, floats, &grown), s));
             jobs.back().layout = RV_BRICK_TILED4;
             if (second) {
-                float*& brick2 = sc.bricks2[idx];
-                if (!brick2)
-                    hipCheck(hipMalloc((void**)&brick2, 4 * floats), "hipMalloc");
-                jobs.push_back(fs.job(sub.level, idx, brick2, s, true));
+                jobs.push_back(fs.job(sub.level, idx, sc.brick(sc.bricks2, bkey, floats, &grown), s, true));
                 jobs.back().layout = RV_BRICK_TILED4;
             }
         }
-        throwIfFailed(rv_interp_batch_create(jobs.data(), (uint32_t)jobs.size(), &sc.batch));
-        sc.key = key;
-        sc.version = fs.version();
+        // a reallocated brick invalidates every cached batch that wrote it
+        if (grown)
+            sc.dropBatches();
+        if (sc.batches.size() >= ShareCache::kMaxBatches) {
+            auto lru = sc.batches.begin();
+            for (auto it = sc.batches.begin(); it != sc.batches.end(); ++it)
+                if (it->second.used < lru->second.used)
+                    lru = it;
+            rv_interp_batch_destroy(lru->second.batch);
+            sc.batches.erase(lru);
+        }
+        ShareCache::Batch b;
+        throwIfFailed(rv_interp_batch_create(jobs.data(), (uint32_t)jobs.size(), &b.batch));
+        hit = sc.batches.emplace(key, b).first;
     }
+    hit->second.used = ++sc.clock;
     if (isCancelled()) {
         m_cancelled = true;
         return;
     }
-    throwIfFailed(rv_interp_batch_run(sc.batch, s));
+    throwIfFailed(rv_interp_batch_run(hit->second.batch, s));
     // colour table (as the per-block renderer keeps it)
     m_renderer.ensureColorTable();
     const auto range = fs.fieldRange();
@@ -501,10 +551,11 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
                                             &p, &live));
         if (!live)
             continue;
+        const std::array<uint32_t, 4> bkey{sub.level, idx[0], idx[1], idx[2]};
         params.push_back(p);
-        bricks.push_back(sc.bricks[idx]);
+        bricks.push_back(sc.bricks[bkey].p);
         if (second)
-            bricks2.push_back(sc.bricks2[idx]);
+            bricks2.push_back(sc.bricks2[bkey].p);
     }
     const uint64_t need = rv_share_workspace_bytes((uint32_t)params.size());
     if (need > sc.workspace_bytes) {

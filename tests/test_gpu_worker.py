@@ -248,3 +248,103 @@ def test_worker_value_modes_bit_exact(orc, mode, fused):
         x, y, ww, hh = r
         assert np.array_equal(bits, vp[y:y + hh, x:x + ww])
     w.close()
+
+
+def _level_blocks(field, level, depth, rng=None, p=1.0):
+    m = 2 ** level
+    out = {}
+    for z in range(m):
+        for y in range(m):
+            for x in range(m):
+                ch = (datasets.full_subtree(depth) if rng is None else
+                      datasets.refined_subtree(depth, lambda l, locs: rng.random(locs.shape[0]) < p))
+                d = api.subtree_depth(ch, depth)
+                n2i = api.subtree_n2i(ch, d)
+                out[(x, y, z)] = datasets.Block(level, (x, y, z), d, n2i,
+                                                datasets.sample(field, level, (x, y, z), d, n2i))
+    return out
+
+
+def test_worker_fused_deeper_subtree_at_next_level(orc):
+    """The fused path keeps one brick per (level, index) sized for its
+    subtree: block (3,3,3) is a depth-1 subtree at level 2 and a depth-2
+    subtree at level 3 (64 then 512 micro-brick floats).  Level 2, then level
+    3, then level 2 again render bit-exact against the oracle and the
+    per-block path."""
+    W, H = 160, 120
+    field = datasets.Field(21, n_gauss=3)
+    levels = {2: _level_blocks(field, 2, 1), 3: _level_blocks(field, 3, 2)}
+    fmin = min(datasets.field_range(b.values())[0] for b in levels.values())
+    fmax = max(datasets.field_range(b.values())[1] for b in levels.values())
+    tf = datasets.seeded_tf(21, 256)
+    M = datasets.orbit_camera(40.0)
+    ws = {}
+    for fused in (True, False):
+        w = api.Worker(0)
+        for lv, bl in levels.items():
+            for b in bl.values():
+                w.add_subtree(lv, b.index, b.depth, b.n2i, b.sparse)
+        w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+        w.set_fused(fused)
+        ws[fused] = w
+    for lv in (2, 3, 2):
+        m = 2 ** lv
+        erect, ebits = _oracle_share(orc, levels[lv], lv, (0, 0, 0), (m,) * 3, M, W, H, fmin, fmax, tf)
+        for fused, w in ws.items():
+            say("level", lv, "fused", fused)
+            rect, bits = w.render_scene_part(lv, (0, 0, 0), (m,) * 3)
+            assert rect == erect
+            assert np.array_equal(bits, ebits)
+    for w in ws.values():
+        w.close()
+
+
+def test_worker_replaced_values_rerender(orc):
+    """Replacing a block's values (rv_worker_add_subtree again, same depth)
+    re-interpolates it in both paths: per-block equals fused equals the
+    oracle after the change, and the part differs from before."""
+    W, H = 128, 96
+    field = datasets.Field(22, n_gauss=3)
+    blocks = _level_blocks(field, 1, 3)
+    fmin, fmax = datasets.field_range(blocks.values())
+    tf = datasets.seeded_tf(22, 256)
+    M = datasets.orbit_camera(15.0)
+    ws = {}
+    for fused in (True, False):
+        w = api.Worker(0)
+        for b in blocks.values():
+            w.add_subtree(1, b.index, b.depth, b.n2i, b.sparse)
+        w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+        w.set_fused(fused)
+        ws[fused] = w
+    # a one-block box, so the per-block renderer's cached brick is this block
+    box = ((1, 1, 1), (2, 2, 2))
+    before = {f: w.render_scene_part(1, *box) for f, w in ws.items()}
+    assert before[True][0] == before[False][0] and np.array_equal(before[True][1], before[False][1])
+    b = blocks[(1, 1, 1)]
+    new = datasets.Block(1, b.index, b.depth, b.n2i, (fmax - (b.sparse - fmin)).astype(np.float32))
+    blocks[(1, 1, 1)] = new
+    for w in ws.values():
+        w.add_subtree(1, new.index, new.depth, new.n2i, new.sparse)
+    erect, ebits = _oracle_share(orc, blocks, 1, box[0], box[1], M, W, H, fmin, fmax, tf)
+    for fused, w in ws.items():
+        rect, bits = w.render_scene_part(1, *box)
+        assert rect == erect and np.array_equal(bits, ebits), fused
+    assert not (before[True][0] == erect and np.array_equal(before[True][1], ebits))
+    for w in ws.values():
+        w.close()
+
+
+def test_worker_compose_frame_repeated_uses_cached_batches(orc):
+    """Two frames of 5 boxes in a row on one worker (one cached batch per
+    box), then a frame after a value change: each equals the oracle."""
+    W, H = 180, 140
+    w, blocks, fmin, fmax, tf, M = _make(2, 3, 9, W, H, 200.0)
+    exp = _oracle_frame(orc, blocks, 2, 5, M, W, H, fmin, fmax, tf)
+    assert np.array_equal(w.compose_frame(2, 5), exp)
+    assert np.array_equal(w.compose_frame(2, 5), exp)
+    b = blocks[(2, 1, 1)]
+    blocks[(2, 1, 1)] = datasets.Block(2, b.index, b.depth, b.n2i, np.flip(b.sparse).copy())
+    w.add_subtree(2, b.index, b.depth, b.n2i, blocks[(2, 1, 1)].sparse)
+    assert np.array_equal(w.compose_frame(2, 5), _oracle_frame(orc, blocks, 2, 5, M, W, H, fmin, fmax, tf))
+    w.close()
