@@ -215,6 +215,11 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RvError("librevise_hip.so is not built: run `python -m revise_amd.build` "
                           "(there is no CPU fallback)")
+        # torch first: its HIP runtime (soname libamdhip64.so.7) then
+        # satisfies the library's dependency, so the process holds one HIP
+        # runtime.  Loaded the other way round, torch loads a second runtime of
+        # its own, and the library's runtime finds no device.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             if not hasattr(L, name):

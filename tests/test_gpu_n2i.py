@@ -1,5 +1,5 @@
 """n2i built on the device (rv_subtree_n2i_device) equals the host list
-(rv_subtree_n2i, itself pinned against the oracle and a hand-worked tree)
+(rv_subtree_n2i) and the oracle's (itself pinned by hand-worked trees)
 exactly: random and full trees of depths 0-7, children arrays in any node
 numbering, walks limited below the tree's depth; errors as the host's."""
 import numpy as np
@@ -32,16 +32,25 @@ def _shuffled(children, rng):
 
 @pytest.mark.parametrize("seed,depth,p", [(1, 1, 0.5), (2, 3, 0.6), (3, 5, 0.45), (4, 6, 0.35), (5, 7, 0.25),
                                           (6, 4, 1.0), (7, 6, 1.0), (8, 0, 1.0), (9, 2, 0.0)])
-def test_device_n2i_equals_host(seed, depth, p):
+def test_device_n2i_equals_host(orc, seed, depth, p):
     rng = np.random.default_rng(seed)
     ch = datasets.refined_subtree(depth, lambda l, locs: rng.random(locs.shape[0]) < p)
     d = api.subtree_depth(ch, depth)
     for walk in sorted({d, max(d - 1, 0)}):
         exp = api.subtree_n2i(ch, walk)
+        assert np.array_equal(exp, orc.subtree_n2i(ch, walk))
         got = _dev_n2i(ch, walk)
         say("n2i depth", walk, "blocks", ch.shape[0], "nodes", exp.shape[0])
         assert np.array_equal(got, exp)
         assert np.array_equal(_dev_n2i(_shuffled(ch, rng), walk), exp)
+
+
+def test_device_n2i_hand_worked_depth2(orc):
+    """The depth-2 tree whose pre-order and breadth-first corner orders
+    differ, with its hand-enumerated list."""
+    from tests.test_oracle_pins import N2I_DEPTH2, _depth2_tree
+    got = _dev_n2i(_depth2_tree(), 2)
+    assert [tuple(int(v) for v in r) for r in got] == N2I_DEPTH2
 
 
 def test_device_n2i_errors():
