@@ -315,6 +315,25 @@ int rv_render_share_secondary(const rv_block_params* params, const float* const*
                               uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, void* d_workspace,
                               rv_stream_t stream);
 
+/* rv_render_share_argb without a workspace and without any allocation.  No
+ * reference counterpart: it is for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The block table is staged in a fixed device array of the
+ * library, one per GPU, holding at most RV_SHARE_STATIC_MAX_BLOCKS blocks
+ * (786,688 bytes, kept by each GPU the library runs on until the process
+ * ends); the call makes no hipMalloc/hipFree and no host heap allocation.
+ * nblocks above the maximum: RV_ERR_INVALID.  Same result as
+ * rv_render_share_argb, and, like it, the call synchronizes the stream once
+ * after staging the table; the launch then reads the array until it ends.
+ * Every call on a GPU uses that GPU's one array: calls that overlap on one
+ * GPU (other threads, or other streams before this launch has finished) race
+ * and give wrong images; calls in order on one stream are safe. */
+#define RV_SHARE_STATIC_MAX_BLOCKS 4096
+int rv_render_share_argb_noalloc(const rv_block_params* params, const float* const* bricks,
+                                 uint32_t nblocks, const float* d_tf, uint32_t tf_n,
+                                 uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                 rv_stream_t stream);
+
 /* Piecewise-linear colour table from control points.  Replaces
  * s3vs::makeColorTransferVector (ColorTransferFunction.hpp:27-69).
  * xs: count sorted positions, colors: count x 4, out: color_count x 4. */

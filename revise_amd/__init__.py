@@ -128,6 +128,8 @@ _SIGS = {
                                   C.c_float, _f32p, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32,
                                   C.c_uint32, _vp, _vp]),
     "rv_share_workspace_bytes": (C.c_uint64, [C.c_uint32]),
+    "rv_render_share_argb_noalloc": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p), C.c_uint32, _vp,
+                                               C.c_uint32, _vp, C.c_uint32, C.c_uint32, _vp]),
     "rv_brick_floats": (C.c_uint64, [C.c_uint32, C.c_uint32]),
     "rv_render_share_layout": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                          C.c_uint32, C.c_uint32, C.c_uint32, C.c_float, _f32p, C.c_uint32,

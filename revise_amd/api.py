@@ -325,6 +325,16 @@ class ShareRenderer:
                                          ptr(d_vp), self.W, self.H, ptr(self.ws), stream_ptr(stream)))
 
 
+def render_share_argb_noalloc(params, bricks, d_tf, d_vp, W, H, stream=None):
+    """rv_render_share_argb_noalloc over (params list, device brick tensors),
+    back to front: the block table goes to the library's fixed array."""
+    n = len(params)
+    pa = (BlockParams * max(n, 1))(*params)
+    ba = (C.c_void_p * max(n, 1))(*[b.data_ptr() for b in bricks])
+    check(lib().rv_render_share_argb_noalloc(pa, ba, n, ptr(d_tf), d_tf.shape[0], ptr(d_vp), W, H,
+                                             stream_ptr(stream)))
+
+
 def bound_rect(d_pix, w, h, stream=None):
     r = (C.c_int32 * 4)()
     ne = C.c_int()

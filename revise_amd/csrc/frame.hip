@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <chrono>
 #include <memory>
+#include <mutex>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -1436,6 +1437,46 @@ extern "C" int rv_frame_set_field_mode(rv_frame_plan* P, uint32_t field_mode, fl
 
 extern "C" void rv_frame_plan_destroy(rv_frame_plan* P) { delete P; }
 
+// The share kernel over a staged block table (device params, view
+// rectangles {x0, y0, x1, y1}, brick and secondary-brick pointers).
+static int launch_share(const rv_block_params* dp, const uint4* dr, const void* const* db,
+                        const void* const* db2, uint32_t nblocks, uint32_t layout, uint32_t field_mode,
+                        float iso_opacity, const LevelsArg& lev, float smin, float smax, const float* d_tf,
+                        uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                        rv_stream_t stream)
+{
+    // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
+    // LDS per workgroup at most)
+    const size_t lds = std::max<size_t>(sizeof(float4) * tf_n,
+                                        std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u));
+    const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
+    const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);
+#define RV_SHARE_LAUNCH(M)                                                                         \
+    do {                                                                                           \
+        if (layout == RV_BRICK_TILED4)                                                             \
+            hipLaunchKernelGGL((share_kernel<M, Tiled4>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
+                               db, db2, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
+                               vp_w, vp_h);                                                        \
+        else                                                                                       \
+            hipLaunchKernelGGL((share_kernel<M, float>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
+                               db, db2, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
+                               vp_w, vp_h);                                                        \
+    } while (0)
+    switch (field_mode) {
+    case RV_FIELD_ARGB: RV_SHARE_LAUNCH(kModeArgb); break;
+    case RV_FIELD_ARGB_LIGHT: RV_SHARE_LAUNCH(kModeArgbLight); break;
+    case RV_FIELD_ISOSURFACE: RV_SHARE_LAUNCH(kModeIsosurface); break;
+    case RV_FIELD_ISOSURFACES: RV_SHARE_LAUNCH(kModeIsosurfaces); break;
+    case RV_FIELD_DOMAIN_VOXELS: RV_SHARE_LAUNCH(kModeDomainVoxels); break;
+    case RV_FIELD_VALUE_ON_ISOSURFACE: RV_SHARE_LAUNCH(kModeValueOnIsosurface); break;
+    case RV_FIELD_VALUE_ON_ISOSURFACES: RV_SHARE_LAUNCH(kModeValueOnIsosurfaces); break;
+    default: RV_SHARE_LAUNCH(kModeMip); break;
+    }
+#undef RV_SHARE_LAUNCH
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
+
 extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
 {
     return (uint64_t)nblocks * (sizeof(rv_block_params) + sizeof(uint4) + 2 * sizeof(void*)) + 256;
@@ -1487,43 +1528,14 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
     RV_HIP(hipMemcpyAsync(d, host.data(), host.size() - 256, hipMemcpyHostToDevice, S(stream)));
     // the host staging buffer must outlive the copy
     RV_HIP(hipStreamSynchronize(S(stream)));
-    // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
-    // LDS per workgroup at most)
-    const size_t lds = std::max<size_t>(sizeof(float4) * tf_n,
-                                        std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u));
     const auto* dp = reinterpret_cast<const rv_block_params*>(d);
     const auto* dr = reinterpret_cast<const uint4*>(d + sizeof(rv_block_params) * nblocks);
     const auto* db =
         reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
-    const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
-    const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);
     // the levels are relative to the field's range, the same for every block
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
-    const auto* db2 = db + nblocks;
-#define RV_SHARE_LAUNCH(M)                                                                         \
-    do {                                                                                           \
-        if (layout == RV_BRICK_TILED4)                                                             \
-            hipLaunchKernelGGL((share_kernel<M, Tiled4>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
-                               db, db2, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
-                               vp_w, vp_h);                                                        \
-        else                                                                                       \
-            hipLaunchKernelGGL((share_kernel<M, float>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
-                               db, db2, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
-                               vp_w, vp_h);                                                        \
-    } while (0)
-    switch (field_mode) {
-    case RV_FIELD_ARGB: RV_SHARE_LAUNCH(kModeArgb); break;
-    case RV_FIELD_ARGB_LIGHT: RV_SHARE_LAUNCH(kModeArgbLight); break;
-    case RV_FIELD_ISOSURFACE: RV_SHARE_LAUNCH(kModeIsosurface); break;
-    case RV_FIELD_ISOSURFACES: RV_SHARE_LAUNCH(kModeIsosurfaces); break;
-    case RV_FIELD_DOMAIN_VOXELS: RV_SHARE_LAUNCH(kModeDomainVoxels); break;
-    case RV_FIELD_VALUE_ON_ISOSURFACE: RV_SHARE_LAUNCH(kModeValueOnIsosurface); break;
-    case RV_FIELD_VALUE_ON_ISOSURFACES: RV_SHARE_LAUNCH(kModeValueOnIsosurfaces); break;
-    default: RV_SHARE_LAUNCH(kModeMip); break;
-    }
-#undef RV_SHARE_LAUNCH
-    RV_LAUNCH_CHECK();
-    return RV_OK;
+    return launch_share(dp, dr, db, db + nblocks, nblocks, layout, field_mode, iso_opacity, lev, smin, smax,
+                        d_tf, tf_n, d_viewport, vp_w, vp_h, stream);
     RV_CATCH
 }
 
@@ -1558,4 +1570,93 @@ extern "C" int rv_render_share_argb(const rv_block_params* params, const float* 
 {
     return rv_render_share(params, bricks, nblocks, RV_FIELD_ARGB, 1.f, nullptr, 0, d_tf, tf_n,
                            d_viewport, vp_w, vp_h, d_workspace, stream);
+}
+
+// ---------------------------------------------------------------- share without allocation
+// rv_render_share_argb for callers that cannot allocate: the block table
+// lives in `scratch`, a fixed array of the library on every GPU, laid out as
+// rv_share_workspace_bytes(RV_SHARE_STATIC_MAX_BLOCKS) describes (params,
+// then view rectangles, then brick pointers).  Host code never passes the
+// array's name to a kernel or a copy: it takes the array's device address on
+// each GPU once, with hipGetSymbolAddress, and uses that.
+
+static __device__ __align__(256) uint8_t
+    scratch[(uint64_t)RV_SHARE_STATIC_MAX_BLOCKS * (sizeof(rv_block_params) + sizeof(uint4) + 2 * sizeof(void*)) +
+            256];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch[kScratchDevices]; // per device: the address of `scratch` there
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device (the device is current then).
+int scratch_address(uint8_t** out)
+{
+    int dev = 0;
+    RV_HIP(hipGetDevice(&dev));
+    RV_REQUIRE(dev >= 0 && dev < kScratchDevices, "rv_render_share_argb_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        RV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RV_REQUIRE(p, "rv_render_share_argb_noalloc: no address for the scratch array");
+        g_scratch[dev] = static_cast<uint8_t*>(p);
+    }
+    *out = g_scratch[dev];
+    return RV_OK;
+}
+
+// View rectangles {x0, y0, x1, y1} of the staged params.
+__global__ void share_rects_kernel(const rv_block_params* __restrict__ p, uint4* __restrict__ r, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        r[i] = make_uint4(p[i].view_x, p[i].view_y, p[i].view_x + p[i].view_w, p[i].view_y + p[i].view_h);
+}
+
+} // namespace
+
+extern "C" int rv_render_share_argb_noalloc(const rv_block_params* params, const float* const* bricks,
+                                            uint32_t nblocks, const float* d_tf, uint32_t tf_n,
+                                            uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                            rv_stream_t stream)
+{
+    RV_TRY
+    if (nblocks == 0)
+        return RV_OK;
+    RV_REQUIRE(nblocks <= RV_SHARE_STATIC_MAX_BLOCKS,
+               "rv_render_share_argb_noalloc: more blocks than RV_SHARE_STATIC_MAX_BLOCKS (4096); "
+               "use rv_render_share_argb with a workspace");
+    RV_REQUIRE(params && bricks && d_tf && d_viewport, "rv_render_share_argb_noalloc: null pointer");
+    RV_REQUIRE(tf_n >= 1 && tf_n <= kMaxTf, "rv_render_share_argb_noalloc: tf_n must be in [1, 1024]");
+    for (uint32_t i = 0; i < nblocks; ++i) {
+        const rv_block_params& p = params[i];
+        RV_REQUIRE(p.vp_w == vp_w && p.vp_h == vp_h, "rv_render_share_argb_noalloc: viewport mismatch");
+        RV_REQUIRE(p.tex_depth <= 9 && p.nodes == (1u << p.tex_depth) + 1u && p.fmax > p.fmin,
+                   "rv_render_share_argb_noalloc: inconsistent block parameters");
+        RV_REQUIRE((uint64_t)p.view_x + p.view_w <= vp_w && (uint64_t)p.view_y + p.view_h <= vp_h,
+                   "rv_render_share_argb_noalloc: view rectangle outside the viewport");
+        RV_REQUIRE(bricks[i] || p.view_w == 0 || p.view_h == 0, "rv_render_share_argb_noalloc: null brick");
+        RV_REQUIRE(p.fmin == params[0].fmin && p.fmax == params[0].fmax,
+                   "rv_render_share_argb_noalloc: the blocks of a share have one field range");
+    }
+    uint8_t* base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc)
+        return rc;
+    auto* dp = reinterpret_cast<rv_block_params*>(base);
+    auto* dr = reinterpret_cast<uint4*>(base + sizeof(rv_block_params) * nblocks);
+    auto* db = reinterpret_cast<const void**>(base + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    RV_HIP(hipMemcpyAsync(dp, params, sizeof(rv_block_params) * nblocks, hipMemcpyHostToDevice, S(stream)));
+    RV_HIP(hipMemcpyAsync(db, bricks, sizeof(void*) * nblocks, hipMemcpyHostToDevice, S(stream)));
+    // the caller's host arrays must outlive the copies
+    RV_HIP(hipStreamSynchronize(S(stream)));
+    hipLaunchKernelGGL(share_rects_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, S(stream), dp, dr, nblocks);
+    RV_LAUNCH_CHECK();
+    // Argb reads no secondary bricks: their slot gets the primary table
+    return launch_share(dp, dr, db, db, nblocks, RV_BRICK_LINEAR, RV_FIELD_ARGB, 1.f, LevelsArg{}, 0.f, 1.f,
+                        d_tf, tf_n, d_viewport, vp_w, vp_h, stream);
+    RV_CATCH
 }
