@@ -65,7 +65,8 @@ public:
     FieldServiceHip& operator=(const FieldServiceHip&) = delete;
 
     // Registers a subtree root; values are copied to the device and the
-    // subtree's gather plan is encoded from its n2i (once per subtree).
+    // subtree's gather plan is encoded from its n2i -- once per distinct
+    // (depth, n2i): subtrees of the same shape share one plan.
     void addSubtree(unsigned level, const std::array<uint32_t, 3>& index, unsigned depth,
                     const uint16_t* n2i, uint64_t n, const float* sparse);
     void setFieldRange(float fmin, float fmax);
@@ -97,7 +98,10 @@ public:
 
 private:
     struct Subtree;
+    struct Plan; // a gather plan shared by the subtrees of one shape
+    std::shared_ptr<Plan> planFor(unsigned depth, const uint16_t* n2i, uint64_t n);
     std::map<std::array<uint32_t, 4>, std::unique_ptr<Subtree>> m_subtrees;
+    std::multimap<uint64_t, std::weak_ptr<Plan>> m_plans; // by hash of (depth, n2i)
     float m_fmin = 0.f, m_fmax = 1.f;
     float m_smin = 0.f, m_smax = 1.f;
     uint64_t m_version = 0;
@@ -122,6 +126,9 @@ struct RenderSharedState {
     float isosurfaceLevel = 0.5f;
     float isosurfaceOpacity = 1.f;
     std::vector<float> isosurfaceLevels; // fieldAllParam().isosurfaceLevels() (Isosurfaces)
+    // The Argb sample sequence: false (default) the fast one, true the exact
+    // one (RV_RENDER_EXACT, bit-identical to the CPU restatement).
+    bool exact = false;
 };
 
 class DenseFieldRendererHip {

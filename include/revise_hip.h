@@ -186,7 +186,19 @@ typedef struct rv_block_params {
     float tcf;                           /* nodes / (2*(half + texel/2)) */
     float i_layer_count;                 /* 100 / (2^(tex_depth+level) * quality) */
     float opacity_factor;                /* (threshold-fmin)/(fmax-fmin) */
+    uint32_t flags;                      /* RV_RENDER_*; rv_block_params_setup sets 0 */
 } rv_block_params;
+
+/* Sample sequence of the Argb ray-casts.  0 (the default): the fast
+ * sequence -- the reference's samples (the same positions and corners, so
+ * the same undefined samples), with the table lookup, the opacity correction
+ * 1-(1-a*of)^k (hardware log2/exp2) and the front-to-back compositing
+ * (premultiplied, no division per sample) evaluated with fewer operations;
+ * it agrees with the exact sequence to a few 1e-6 per channel.  RV_RENDER_EXACT: the reference's
+ * operation order step by step (renderArgb.hpp:30-55, cuda_blend.hpp:24-44,
+ * with pow01 for powf), bit-identical to the CPU restatement.  The other
+ * field modes always run their exact sequence. */
+#define RV_RENDER_EXACT 1u
 
 /* Host parameter setup of DenseFieldRenderer_cu::Impl::renderDenseField
  * (DenseFieldRenderer_cu.cu:317-453): cube, camera basis, view rectangle,
@@ -437,6 +449,8 @@ typedef struct rv_frame_desc {
                                 tile_w a power of two; 0,0 = 8x8 */
     uint32_t layout;         /* RV_BRICK_* of the blocks' (and secondary) bricks; TILED4 is
                                 fp32 only; field gradient arrays stay LINEAR */
+    uint32_t exact;          /* nonzero: the exact Argb sample sequence (RV_RENDER_EXACT); frames
+                                recording the adjoint's tape always use it */
 } rv_frame_desc;
 
 typedef struct rv_frame_plan rv_frame_plan;
@@ -554,6 +568,7 @@ typedef struct rv_render_state {
     const float* iso_levels_rel; /* isosurfaceLevels (Isosurfaces), relative */
     uint32_t iso_level_count;
     float smin, smax;        /* the secondary field's range (ValueOnIsosurface(s)) */
+    uint32_t exact;          /* nonzero: the exact Argb sample sequence (RV_RENDER_EXACT) */
 } rv_render_state;
 
 /* VsWorkerInterface::initialize: selects GPU thread_id % device count

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librevise_hip.so")
 
 RV_OK = 0
+RENDER_EXACT = 1  # rv_block_params.flags: the exact sample sequence
 
 
 class RvError(RuntimeError):
@@ -28,7 +29,7 @@ class BlockParams(C.Structure):
         ("w", C.c_float), ("h", C.c_float), ("pixel_size", C.c_float),
         ("threshold", C.c_float), ("quality", C.c_float), ("alpha_cutoff", C.c_float),
         ("nodes", C.c_uint32), ("texel", C.c_float), ("tex_origin", C.c_float * 3), ("tcf", C.c_float),
-        ("i_layer_count", C.c_float), ("opacity_factor", C.c_float),
+        ("i_layer_count", C.c_float), ("opacity_factor", C.c_float), ("flags", C.c_uint32),
     ]
 
 
@@ -43,7 +44,7 @@ class FrameDesc(C.Structure):
                 ("fmin", C.c_float), ("fmax", C.c_float), ("threshold_rel", C.c_float),
                 ("quality", C.c_float), ("alpha_cutoff", C.c_float), ("acc_cutoff", C.c_float),
                 ("d_tf", C.c_void_p), ("tf_n", C.c_uint32), ("tile_w", C.c_uint32),
-                ("tile_h", C.c_uint32), ("layout", C.c_uint32)]
+                ("tile_h", C.c_uint32), ("layout", C.c_uint32), ("exact", C.c_uint32)]
 
 
 class RenderState(C.Structure):
@@ -53,7 +54,7 @@ class RenderState(C.Structure):
                 ("background", C.c_float * 3), ("tf", C.c_void_p), ("tf_n", C.c_uint32),
                 ("field_mode", C.c_uint32), ("iso_level_rel", C.c_float), ("iso_opacity", C.c_float),
                 ("iso_levels_rel", C.c_void_p), ("iso_level_count", C.c_uint32),
-                ("smin", C.c_float), ("smax", C.c_float)]
+                ("smin", C.c_float), ("smax", C.c_float), ("exact", C.c_uint32)]
 
 
 class BrickJob(C.Structure):

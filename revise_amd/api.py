@@ -4,8 +4,8 @@ import ctypes as C
 
 import numpy as np
 
-from . import (BlockParams, BrickJob, DatasetInfo, FieldInfo, FrameBlock, FrameDesc, FrameOutputHeader,
-               RenderState, RvError, SubtreeInfo, check, lib, ptr, stream_ptr)
+from . import (RENDER_EXACT, BlockParams, BrickJob, DatasetInfo, FieldInfo, FrameBlock, FrameDesc,
+               FrameOutputHeader, RenderState, RvError, SubtreeInfo, check, lib, ptr, stream_ptr)
 
 
 def _np(a, dt):
@@ -82,7 +82,9 @@ def mixed_back_to_front(blocks, eye):
 
 
 def block_params(view16, vp_w, vp_h, fov, level, index, tex_depth, fmin, fmax, thr_rel, quality,
-                 alpha_cutoff):
+                 alpha_cutoff, exact=False):
+    """rv_block_params_setup; exact: flag the exact Argb sample sequence
+    (RV_RENDER_EXACT), else the default fast one."""
     v = _np(view16, np.float32).reshape(16)
     idx = _np(index, np.uint32)
     p = BlockParams()
@@ -90,6 +92,8 @@ def block_params(view16, vp_w, vp_h, fov, level, index, tex_depth, fmin, fmax, t
     check(lib().rv_block_params_setup(_p(v, C.c_float), vp_w, vp_h, fov, level, _p(idx, C.c_uint32),
                                       tex_depth, fmin, fmax, thr_rel, quality, alpha_cutoff,
                                       C.byref(p), C.byref(live)))
+    if exact:
+        p.flags |= RENDER_EXACT
     return live.value, p
 
 
@@ -365,7 +369,7 @@ class FramePlan:
     addresses (the caller keeps the memory alive)."""
 
     def __init__(self, blocks, vp_w, vp_h, fov, fmin, fmax, thr_rel, quality, alpha_cutoff,
-                 acc_cutoff, d_tf, tile=(0, 0), layout=BRICK_LINEAR):
+                 acc_cutoff, d_tf, tile=(0, 0), layout=BRICK_LINEAR, exact=False):
         import torch
         self._keep = [d_tf]
         if isinstance(blocks, np.ndarray):
@@ -388,7 +392,7 @@ class FramePlan:
                     arr[i].brick = b["brick"].data_ptr()
                 self._keep.append(b)
         d = FrameDesc(vp_w, vp_h, fov, fmin, fmax, thr_rel, quality, alpha_cutoff, acc_cutoff,
-                      d_tf.data_ptr(), d_tf.shape[0], tile[0], tile[1], layout)
+                      d_tf.data_ptr(), d_tf.shape[0], tile[0], tile[1], layout, 1 if exact else 0)
         h = C.c_void_p()
         check(lib().rv_frame_plan_create(arr, len(blocks), C.byref(d), C.byref(h)))
         self.h = h
@@ -525,7 +529,7 @@ class Worker:
 
     def set_state(self, view16, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, fmin, fmax, background,
                   tf, field_mode=FIELD_ARGB, iso_level_rel=0.5, iso_opacity=1.0, iso_levels=None,
-                  secondary_range=(0.0, 1.0)):
+                  secondary_range=(0.0, 1.0), exact=False):
         self._view = _np(view16, np.float32).reshape(16)
         self._tf = _np(tf, np.float32)
         self._levels = _np(iso_levels if iso_levels is not None else [], np.float32).reshape(-1)
@@ -533,7 +537,7 @@ class Worker:
                          fmax, (C.c_float * 3)(*background), self._tf.ctypes.data, self._tf.shape[0],
                          field_mode, iso_level_rel, iso_opacity,
                          self._levels.ctypes.data if self._levels.size else None, self._levels.size,
-                         secondary_range[0], secondary_range[1])
+                         secondary_range[0], secondary_range[1], 1 if exact else 0)
         check(lib().rv_worker_set_state(self.h, C.byref(st)))
         self.W, self.H = vp_w, vp_h
 

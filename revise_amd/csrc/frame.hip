@@ -76,6 +76,7 @@ struct FrameArgs {
     uint32_t lanes_min;            // per-lane walk kernels: bins with at least this many blocks
     const void* const* bricks2;    // secondary field bricks (ValueOnIsosurface(s) modes)
     float smin, smax;              // secondary field range
+    uint32_t fast;                 // Argb without the tape: the fast sample sequence (march_fast)
 };
 
 constexpr uint32_t kSpinLimit = 1u << 22;
@@ -131,6 +132,8 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
                     float ts;
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
+                } else if (kMode == kModeArgb && a.fast) {
+                    seg = march_fast(p, brick, tfs, (int)a.tf_n, r);
                 } else if (multi_level(kMode) && a.lev.n == 0) {
                     seg = make_float4(0.f, 0.f, 0.f, 0.f); // no levels: nothing to draw
                 } else if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces) {
@@ -206,6 +209,8 @@ __device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4*
                     float ts;
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
+                } else if (a.fast) {
+                    seg = march_fast(p, brick, tfs, (int)a.tf_n, r);
                 } else {
                     seg = march_impl<BrickT, false>(p, brick, tfs, (int)a.tf_n, r, nullptr);
                 }
@@ -652,6 +657,7 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     }
     a.iso_opacity = P->iso_opacity;
     a.lev = P->levels;
+    a.fast = P->desc.exact == 0u && a.tape == nullptr ? 1u : 0u;
     hipLaunchKernelGGL(frame_reset_kernel, dim3(1), dim3(64), 0, s, P->d_counters, 4u);
     RV_LAUNCH_CHECK();
     if (interpolate && P->split) {
@@ -839,6 +845,7 @@ extern "C" int rv_frame_plan_set_cameras(rv_frame_plan* P, const float* views16,
             const int live = block_params(views16 + 16 * c, d.vp_w, d.vp_h, d.fov_y_deg,
                                           P->blocks[i].level, P->blocks[i].index, P->blocks[i].depth,
                                           d.fmin, d.fmax, d.threshold_rel, d.quality, d.alpha_cutoff, p);
+            p.flags = d.exact ? RV_RENDER_EXACT : 0u;
             rects[j] = live ? make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h)
                             : make_uint4(0, 0, 0, 0);
         }
@@ -1441,8 +1448,8 @@ extern "C" void rv_frame_plan_destroy(rv_frame_plan* P) { delete P; }
 // rectangles {x0, y0, x1, y1}, brick and secondary-brick pointers).
 static int launch_share(const rv_block_params* dp, const uint4* dr, const void* const* db,
                         const void* const* db2, uint32_t nblocks, uint32_t layout, uint32_t field_mode,
-                        float iso_opacity, const LevelsArg& lev, float smin, float smax, const float* d_tf,
-                        uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                        bool exact, float iso_opacity, const LevelsArg& lev, float smin, float smax,
+                        const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
                         rv_stream_t stream)
 {
     // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
@@ -1463,7 +1470,12 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
                                vp_w, vp_h);                                                        \
     } while (0)
     switch (field_mode) {
-    case RV_FIELD_ARGB: RV_SHARE_LAUNCH(kModeArgb); break;
+    case RV_FIELD_ARGB:
+        if (exact)
+            RV_SHARE_LAUNCH(kModeArgb);
+        else
+            RV_SHARE_LAUNCH(kModeArgbFast);
+        break;
     case RV_FIELD_ARGB_LIGHT: RV_SHARE_LAUNCH(kModeArgbLight); break;
     case RV_FIELD_ISOSURFACE: RV_SHARE_LAUNCH(kModeIsosurface); break;
     case RV_FIELD_ISOSURFACES: RV_SHARE_LAUNCH(kModeIsosurfaces); break;
@@ -1519,6 +1531,8 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
                    "rv_render_share: null secondary brick");
         RV_REQUIRE(p.fmin == params[0].fmin && p.fmax == params[0].fmax,
                    "rv_render_share: the blocks of a share have one field range");
+        RV_REQUIRE(((p.flags ^ params[0].flags) & RV_RENDER_EXACT) == 0u,
+                   "rv_render_share: the blocks of a share have one sample sequence (flags)");
         hp[i] = p;
         hr[i] = make_uint4(p.view_x, p.view_y, p.view_x + p.view_w, p.view_y + p.view_h);
         hb[i] = bricks[i];
@@ -1534,8 +1548,9 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
         reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
     // the levels are relative to the field's range, the same for every block
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
-    return launch_share(dp, dr, db, db + nblocks, nblocks, layout, field_mode, iso_opacity, lev, smin, smax,
-                        d_tf, tf_n, d_viewport, vp_w, vp_h, stream);
+    return launch_share(dp, dr, db, db + nblocks, nblocks, layout, field_mode,
+                        (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
+                        d_viewport, vp_w, vp_h, stream);
     RV_CATCH
 }
 
@@ -1641,6 +1656,8 @@ extern "C" int rv_render_share_argb_noalloc(const rv_block_params* params, const
         RV_REQUIRE(bricks[i] || p.view_w == 0 || p.view_h == 0, "rv_render_share_argb_noalloc: null brick");
         RV_REQUIRE(p.fmin == params[0].fmin && p.fmax == params[0].fmax,
                    "rv_render_share_argb_noalloc: the blocks of a share have one field range");
+        RV_REQUIRE(((p.flags ^ params[0].flags) & RV_RENDER_EXACT) == 0u,
+                   "rv_render_share_argb_noalloc: the blocks of a share have one sample sequence (flags)");
     }
     uint8_t* base = nullptr;
     int rc = scratch_address(&base);
@@ -1656,7 +1673,8 @@ extern "C" int rv_render_share_argb_noalloc(const rv_block_params* params, const
     hipLaunchKernelGGL(share_rects_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, S(stream), dp, dr, nblocks);
     RV_LAUNCH_CHECK();
     // Argb reads no secondary bricks: their slot gets the primary table
-    return launch_share(dp, dr, db, db, nblocks, RV_BRICK_LINEAR, RV_FIELD_ARGB, 1.f, LevelsArg{}, 0.f, 1.f,
-                        d_tf, tf_n, d_viewport, vp_w, vp_h, stream);
+    return launch_share(dp, dr, db, db, nblocks, RV_BRICK_LINEAR, RV_FIELD_ARGB,
+                        (params[0].flags & RV_RENDER_EXACT) != 0u, 1.f, LevelsArg{}, 0.f, 1.f, d_tf, tf_n,
+                        d_viewport, vp_w, vp_h, stream);
     RV_CATCH
 }

@@ -285,7 +285,12 @@ extern "C" int rv_render_block_secondary(const rv_block_params* params, uint32_t
     hipLaunchKernelGGL(render_block_kernel<M>, grid, dim3(256), lds, S(stream), p, d_brick, sec, d_tf, \
                        tf_n, iso_opacity, lev, d_viewport)
     switch (field_mode) {
-    case RV_FIELD_ARGB: RV_BLOCK_LAUNCH(kModeArgb); break;
+    case RV_FIELD_ARGB:
+        if (p.flags & RV_RENDER_EXACT)
+            RV_BLOCK_LAUNCH(kModeArgb);
+        else
+            RV_BLOCK_LAUNCH(kModeArgbFast);
+        break;
     case RV_FIELD_ARGB_LIGHT: RV_BLOCK_LAUNCH(kModeArgbLight); break;
     case RV_FIELD_ISOSURFACE: RV_BLOCK_LAUNCH(kModeIsosurface); break;
     case RV_FIELD_ISOSURFACES: RV_BLOCK_LAUNCH(kModeIsosurfaces); break;
@@ -381,9 +386,14 @@ extern "C" int rv_render_block_argb(const rv_block_params* params, const float* 
     if (p.view_w == 0 || p.view_h == 0)
         return RV_OK;
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
-    hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), tf_n * sizeof(float4),
-                       S(stream), p, d_brick, Secondary<float>{nullptr, 0.f, 1.f}, d_tf, tf_n, 1.f,
-                       LevelsArg{}, d_viewport);
+    if (p.flags & RV_RENDER_EXACT)
+        hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), tf_n * sizeof(float4),
+                           S(stream), p, d_brick, Secondary<float>{nullptr, 0.f, 1.f}, d_tf, tf_n, 1.f,
+                           LevelsArg{}, d_viewport);
+    else
+        hipLaunchKernelGGL(render_block_kernel<kModeArgbFast>, grid, dim3(256), tf_n * sizeof(float4),
+                           S(stream), p, d_brick, Secondary<float>{nullptr, 0.f, 1.f}, d_tf, tf_n, 1.f,
+                           LevelsArg{}, d_viewport);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }

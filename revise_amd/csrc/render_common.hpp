@@ -426,11 +426,155 @@ __device__ __forceinline__ float4 march(const rv_block_params& p, const T* __res
     return march_impl<T, false>(p, brick, tf, tf_n, r, nullptr);
 }
 
+// ---------------------------------------------------------------- fast sample sequence
+// The default sequence of the Argb ray-casts (rv_block_params.flags without
+// RV_RENDER_EXACT): the same samples as march() -- the same positions (the
+// same sequential fp32 steps) and the same 8 corners per sample, so the same
+// samples are undefined (a NaN corner makes a sample undefined whatever its
+// weight, and a position moved by 1e-6 texel can change which cell, hence
+// which corners, it reads) -- with fewer operations for what varies
+// continuously:
+//  * each trilinear step is one fma, a + t (b - a);
+//  * the table coordinate is one fma of the field value (the range
+//    normalisation folded in), clamped with one med3;
+//  * the opacity correction 1 - (1 - a of)^k is 1 - 2^(k log2(1 - a of))
+//    with the hardware log2 and exp2;
+//  * compositing accumulates premultiplied colour and alpha front to back,
+//    w = (1 - A) a, C += w c, A += w, with no division per sample; the
+//    segment's colour is divided by its alpha once at the end (alpha is
+//    accumulated itself, not as 1 - transmittance, so a nearly transparent
+//    segment keeps an accurate colour).
+// The results agree with the exact sequence to a few 1e-6 per channel; a
+// segment may end one sample earlier or later at the early-termination test.
+
+// One axis of a sample: the clamped taps of axis_taps with med3s.
+__device__ __forceinline__ void fast_axis(float t, int n, uint32_t& i0, uint32_t& i1, float& f)
+{
+    const float v = __builtin_amdgcn_fmed3f(t - 0.5f, -1.f, (float)n);
+    const float fl = floorf(v);
+    f = v - fl;
+    const int i = (int)fl;
+    i0 = (uint32_t)clampi(i, 0, n - 1);
+    i1 = (uint32_t)clampi(i + 1, 0, n - 1);
+}
+
+// The 8 corners of a sample (those of fetch_corners), x-fastest bricks (fp32
+// or fp16): the x-pairs are loaded as in fetch_corners.
+template <class T>
+__device__ __forceinline__ void fetch_fast(const T* __restrict__ brick, int n, float tx, float ty, float tz,
+                                           float v[8], float& fx, float& fy, float& fz)
+{
+    uint32_t x0, x1, y0, y1, z0, z1;
+    fast_axis(tx, n, x0, x1, fx);
+    fast_axis(ty, n, y0, y1, fy);
+    fast_axis(tz, n, z0, z1, fz);
+    const uint32_t un = (uint32_t)n;
+    const uint32_t bx = x0 < un - 2u ? x0 : un - 2u;
+    const uint32_t row = __umul24(__umul24(z0, un) + y0, un) + bx;
+    const uint32_t dy = __umul24(y1 - y0, un), dz = __umul24(__umul24(z1 - z0, un), un);
+    const bool l0 = x0 == bx, l1 = x1 == bx;
+    const uint32_t off[4] = {0u, dy, dz, dz + dy};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float p0, p1;
+        load_pair(brick, row + off[k], p0, p1);
+        v[2 * k] = l0 ? p0 : p1;
+        v[2 * k + 1] = l1 ? p0 : p1;
+    }
+}
+
+// ... and micro-brick (TILED4) bricks: the per-axis parts of tiled4_index,
+// in bytes, summed per corner.
+__device__ __forceinline__ void fetch_fast(const Tiled4* __restrict__ brick, int n, float tx, float ty, float tz,
+                                           float v[8], float& fx, float& fy, float& fz)
+{
+    uint32_t x0, x1, y0, y1, z0, z1;
+    fast_axis(tx, n, x0, x1, fx);
+    fast_axis(ty, n, y0, y1, fy);
+    fast_axis(tz, n, z0, z1, fz);
+    const uint32_t G = ((uint32_t)n + 3u) >> 2, sy = G << 8, sz = (G * G) << 8;
+    const uint32_t X0 = ((x0 >> 2) << 8) | ((x0 & 3u) << 2);
+    const uint32_t X1 = ((x1 >> 2) << 8) | ((x1 & 3u) << 2);
+    const uint32_t Y0 = __umul24(y0 >> 2, sy) + ((y0 & 3u) << 4);
+    const uint32_t Y1 = __umul24(y1 >> 2, sy) + ((y1 & 3u) << 4);
+    const uint32_t Z0 = __umul24(z0 >> 2, sz) + ((z0 & 3u) << 6);
+    const uint32_t Z1 = __umul24(z1 >> 2, sz) + ((z1 & 3u) << 6);
+    const gptr<char> b = global_of(reinterpret_cast<const char*>(brick));
+    const uint32_t y0z0 = Y0 + Z0, y1z0 = Y1 + Z0, y0z1 = Y0 + Z1, y1z1 = Y1 + Z1;
+    const uint32_t o[8] = {X0 + y0z0, X1 + y0z0, X0 + y1z0, X1 + y1z0,
+                           X0 + y0z1, X1 + y0z1, X0 + y1z1, X1 + y1z1};
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        v[k] = *reinterpret_cast<gptr<float>>(b + o[k]);
+}
+
+__device__ __forceinline__ float lerp_fast(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+template <class T>
+__device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* __restrict__ brick,
+                                             const float4* tf, int tf_n, RaySeg r)
+{
+    const int n = (int)p.nodes;
+    // table coordinate u N - 0.5 with u = (v - fmin) / (fmax - fmin)
+    const float ta = (float)tf_n / (p.fmax - p.fmin);
+    const float tb = fmaf(-p.fmin, ta, -0.5f);
+    const float thi = (float)(tf_n - 1);
+    const float k = p.i_layer_count, of = p.opacity_factor, cut = p.alpha_cutoff;
+    float cr = 0.f, cg = 0.f, cb = 0.f, A = 0.f; // premultiplied colour, alpha
+    for (int i = 1; i < r.sample_count; i += kBatch) {
+        float v[kBatch];
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
+            v[s] = __builtin_nanf("");
+            if (i + s < r.sample_count) {
+                float c[8], fx, fy, fz;
+                fetch_fast(brick, n, (r.px - p.tex_origin[0]) * p.tcf, (r.py - p.tex_origin[1]) * p.tcf,
+                           (r.pz - p.tex_origin[2]) * p.tcf, c, fx, fy, fz);
+                const float c00 = lerp_fast(c[0], c[1], fx), c10 = lerp_fast(c[2], c[3], fx);
+                const float c01 = lerp_fast(c[4], c[5], fx), c11 = lerp_fast(c[6], c[7], fx);
+                v[s] = lerp_fast(lerp_fast(c00, c10, fy), lerp_fast(c01, c11, fy), fz);
+                r.px = r.px + r.sx;
+                r.py = r.py + r.sy;
+                r.pz = r.pz + r.sz;
+            }
+        }
+        bool done = false;
+#pragma unroll
+        for (int s = 0; s < kBatch; ++s) {
+            if (!done && v[s] == v[s]) {
+                const float x = __builtin_amdgcn_fmed3f(fmaf(v[s], ta, tb), 0.f, thi);
+                const float fl = floorf(x);
+                const float f = x - fl;
+                const int i0 = (int)fl;
+                const int i1 = i0 + 1 < tf_n ? i0 + 1 : i0;
+                const float4 a = tf[i0], b = tf[i1];
+                const float al =
+                    1.f - __builtin_amdgcn_exp2f(k * __builtin_amdgcn_logf(fmaf(-lerp_fast(a.w, b.w, f), of, 1.f)));
+                const float w = fmaf(-A, al, al); // (1 - A) al
+                cr = fmaf(w, lerp_fast(a.x, b.x, f), cr);
+                cg = fmaf(w, lerp_fast(a.y, b.y, f), cg);
+                cb = fmaf(w, lerp_fast(a.z, b.z, f), cb);
+                A = A + w;
+                done = A > cut;
+            }
+        }
+        if (done)
+            break;
+    }
+    if (!(A > 0.f))
+        return make_float4(0.f, 0.f, 0.f, 0.f);
+    const float ia = 1.f / A;
+    return make_float4(cr * ia, cg * ia, cb * ia, A);
+}
+
 // ---------------------------------------------------------------- field modes
 // s3vs::FieldMode (FieldMode.hpp:26-51), the same numbering.
 constexpr uint32_t kModeIsosurface = 0, kModeIsosurfaces = 1, kModeMip = 2, kModeArgb = 3,
                    kModeArgbLight = 4, kModeDomainVoxels = 5, kModeValueOnIsosurface = 6,
                    kModeValueOnIsosurfaces = 7;
+// Argb with the fast sample sequence (march_fast): a kernel instantiation of
+// its own, not a field mode of the ABI.
+constexpr uint32_t kModeArgbFast = 8;
 
 // The secondary field of ValueOnIsosurface(s): the block's brick of it (the
 // primary brick's layout) and the field's range (RenderKernelParam.hpp:103,
@@ -798,6 +942,8 @@ __device__ __forceinline__ float4 segment(const rv_block_params& p, const T* __r
                                                             r, sec);
     if (kMode == kModeDomainVoxels)
         return march_domain(p, brick, tf, tf_n, r);
+    if (kMode == kModeArgbFast)
+        return march_fast(p, brick, tf, tf_n, r);
     return march(p, brick, tf, tf_n, r);
 }
 

@@ -51,25 +51,76 @@ static hipStream_t hs(rv_stream_t s) { return reinterpret_cast<hipStream_t>(s); 
 
 // ---------------------------------------------------------------- field service
 
+struct FieldServiceHip::Plan {
+    unsigned depth = 0;
+    std::vector<uint16_t> n2i; // the shape, to tell hash collisions apart
+    void* plan = nullptr;
+    ~Plan()
+    {
+        if (plan)
+            (void)rv_interp_plan_free(plan);
+    }
+};
+
 struct FieldServiceHip::Subtree {
     unsigned depth = 0;
     uint64_t n = 0;
     float* sparse = nullptr;
     float* sparse2 = nullptr; // secondary field values, or null
-    void* plan = nullptr; // gather plan, encoded from the n2i once per subtree
+    std::shared_ptr<Plan> plan; // gather plan of the subtree's shape
     ~Subtree()
     {
         if (sparse)
             (void)hipFree(sparse);
         if (sparse2)
             (void)hipFree(sparse2);
-        if (plan)
-            (void)rv_interp_plan_free(plan);
     }
 };
 
 FieldServiceHip::FieldServiceHip() = default;
 FieldServiceHip::~FieldServiceHip() = default;
+
+// The plan of a subtree shape: a plan depends on (depth, n2i) only, so
+// subtrees of one shape (every full subtree of a depth, say) share it, as
+// the reference shares BlockTreeNodes per subtree (BlockTreeFieldService.hpp:
+// 133-137).  FNV-1a over the shape finds candidates; the n2i is compared.
+std::shared_ptr<FieldServiceHip::Plan> FieldServiceHip::planFor(unsigned depth, const uint16_t* n2i,
+                                                                uint64_t n)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](uint64_t v) {
+        for (int b = 0; b < 8; ++b) {
+            h ^= (v >> (8 * b)) & 0xffu;
+            h *= 1099511628211ull;
+        }
+    };
+    mix(depth);
+    mix(n);
+    for (uint64_t i = 0; i < 3 * n; i += 4) {
+        uint64_t v = 0;
+        for (uint64_t k = i; k < std::min<uint64_t>(i + 4, 3 * n); ++k)
+            v |= (uint64_t)n2i[k] << (16 * (k - i));
+        mix(v);
+    }
+    auto range = m_plans.equal_range(h);
+    for (auto it = range.first; it != range.second;) {
+        std::shared_ptr<Plan> pl = it->second.lock();
+        if (!pl) {
+            it = m_plans.erase(it);
+            continue;
+        }
+        if (pl->depth == depth && pl->n2i.size() == 3 * n &&
+            std::equal(pl->n2i.begin(), pl->n2i.end(), n2i))
+            return pl;
+        ++it;
+    }
+    auto pl = std::make_shared<Plan>();
+    pl->depth = depth;
+    pl->n2i.assign(n2i, n2i + 3 * n);
+    throwIfFailed(rv_interp_plan_build(n2i, n, depth, &pl->plan, nullptr));
+    m_plans.emplace(h, pl);
+    return pl;
+}
 
 void FieldServiceHip::addSubtree(unsigned level, const std::array<uint32_t, 3>& index,
                                  unsigned depth, const uint16_t* n2i, uint64_t n, const float* sparse)
@@ -83,7 +134,7 @@ void FieldServiceHip::addSubtree(unsigned level, const std::array<uint32_t, 3>& 
         hipCheck(hipMalloc((void**)&st->sparse, 4 * n), "hipMalloc");
         hipCheck(hipMemcpy(st->sparse, sparse, 4 * n, hipMemcpyHostToDevice), "hipMemcpy");
     }
-    throwIfFailed(rv_interp_plan_build(n2i, n, depth, &st->plan, nullptr));
+    st->plan = planFor(depth, n2i, n);
     m_subtrees[{level, index[0], index[1], index[2]}] = std::move(st);
     ++m_version;
 }
@@ -139,7 +190,7 @@ rv_brick_job FieldServiceHip::job(unsigned level, const std::array<uint32_t, 3>&
         throw HipError(RV_ERR_INVALID, "renderScenePart: a block has no secondary field values");
     rv_brick_job j{};
     j.sparse = secondary ? st.sparse2 : st.sparse;
-    j.plan = st.plan;
+    j.plan = st.plan->plan;
     j.dense = d_dense;
     j.depth = st.depth;
     return j;
@@ -152,7 +203,7 @@ void FieldServiceHip::interpolate(unsigned level, const std::array<uint32_t, 3>&
     if (it == m_subtrees.end())
         throw std::range_error("Block does not exist");
     const Subtree& st = *it->second;
-    throwIfFailed(rv_interp_brick_planned(st.sparse, st.plan, st.depth, d_dense, stream));
+    throwIfFailed(rv_interp_brick_planned(st.sparse, st.plan->plan, st.depth, d_dense, stream));
 }
 
 void FieldServiceHip::interpolateSecondary(unsigned level, const std::array<uint32_t, 3>& i,
@@ -164,7 +215,7 @@ void FieldServiceHip::interpolateSecondary(unsigned level, const std::array<uint
     const Subtree& st = *it->second;
     if (st.n && !st.sparse2)
         throw HipError(RV_ERR_INVALID, "renderDenseField: the block has no secondary field values");
-    throwIfFailed(rv_interp_brick_planned(st.sparse2, st.plan, st.depth, d_dense, stream));
+    throwIfFailed(rv_interp_brick_planned(st.sparse2, st.plan->plan, st.depth, d_dense, stream));
 }
 
 // ---------------------------------------------------------------- renderer
@@ -261,6 +312,7 @@ void DenseFieldRendererHip::renderDenseField(uint32_t* vp, unsigned level,
                                         st.alphaCutoff, &p, &live));
     if (!live)
         return;
+    p.flags = st.exact ? RV_RENDER_EXACT : 0u;
     const auto srange = fs.secondaryRange();
     throwIfFailed(rv_render_block_secondary(&p, st.fieldMode, st.isosurfaceOpacity,
                                             st.isosurfaceLevels.data(),
@@ -551,6 +603,7 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
                                             &p, &live));
         if (!live)
             continue;
+        p.flags = st.exact ? RV_RENDER_EXACT : 0u;
         const std::array<uint32_t, 4> bkey{sub.level, idx[0], idx[1], idx[2]};
         params.push_back(p);
         bricks.push_back(sc.bricks[bkey].p);
@@ -836,6 +889,7 @@ extern "C" int rv_worker_set_state(rv_worker* w, const rv_render_state* s)
         return RV_ERR_INVALID;
     }
     st.isosurfaceLevels.assign(s->iso_levels_rel, s->iso_levels_rel + s->iso_level_count);
+    st.exact = s->exact != 0;
     if ((s->field_mode == RV_FIELD_VALUE_ON_ISOSURFACE || s->field_mode == RV_FIELD_VALUE_ON_ISOSURFACES) &&
         !(s->smax > s->smin)) {
         rv::set_error("rv_worker_set_state: the secondary field range must be non-empty");

@@ -24,3 +24,46 @@ def bits_equal_nan(a, b):
         return False
     return np.array_equal(a[~na].view(np.uint32) if a.dtype == np.float32 else a[~na],
                           b[~nb].view(np.uint32) if b.dtype == np.float32 else b[~nb])
+
+
+def host_threads(cap=16):
+    """Worker threads for CPU reference work: the host cores this process
+    may use, at most `cap`."""
+    import os
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(cap, n))
+
+
+def oracle_blocks_frame(orc, items, tf, W, H, init=None, rows=None, threads=None):
+    """The oracle's per-block 8-bit frame: items = [(oracle params, brick)]
+    back to front, each ray-cast over the viewport with the per-block 8-bit
+    composite (orc_render_block), on `threads` threads, each owning a band of
+    rows and walking every block in order.  rows: (y0, y1) limits the work to
+    those viewport rows (the others stay as init)."""
+    import ctypes as C
+    import threading
+    vp = np.zeros((H, W), np.uint32) if init is None else np.ascontiguousarray(init, dtype=np.uint32).copy()
+    y0, y1 = rows if rows is not None else (0, H)
+    L = orc.lib()
+    tfc = np.ascontiguousarray(tf, dtype=np.float32)
+    tp = tfc.ctypes.data_as(C.POINTER(C.c_float))
+    vpp = vp.ctypes.data_as(C.POINTER(C.c_uint32))
+    keep = [np.ascontiguousarray(b, dtype=np.float32) for _, b in items]
+    bps = [k.ctypes.data_as(C.POINTER(C.c_float)) for k in keep]
+    nt = threads or host_threads()
+    step = max(1, (y1 - y0 + nt - 1) // nt)
+
+    def band(r0, r1):
+        for (po, _), bp in zip(items, bps):
+            # params rows are viewport rows; the oracle takes [r0, r1)
+            L.orc_render_block(C.byref(po), bp, tp, tfc.shape[0], vpp, r0, r1)
+
+    ths = [threading.Thread(target=band, args=(r, min(y1, r + step))) for r in range(y0, y1, step)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    return vp

@@ -70,7 +70,7 @@ def _scene(level, depth, W, H, seed):
 
 
 def _plan(wl, W, H, tf_dev):
-    fp = api.FramePlan(wl.blocks, W, H, FOV, wl.fmin, wl.fmax, THR, Q, CUT, ACC, tf_dev)
+    fp = api.FramePlan(wl.blocks, W, H, FOV, wl.fmin, wl.fmax, THR, Q, CUT, ACC, tf_dev, exact=True)
     fp.tf_n = tf_dev.shape[0]
     return fp
 
@@ -328,7 +328,7 @@ def test_adjoint_with_tiled_bricks(orc):
             c = dict(b)
             c["brick"] = torch.empty(api.brick_floats(b["depth"], layout), dtype=torch.float32, device="cuda")
             blocks.append(c)
-        fp = api.FramePlan(blocks, W, H, FOV, wl.fmin, wl.fmax, THR, Q, CUT, ACC, tf, layout=layout)
+        fp = api.FramePlan(blocks, W, H, FOV, wl.fmin, wl.fmax, THR, Q, CUT, ACC, tf, layout=layout, exact=True)
         fp.set_cameras(M[None])
         fp.set_tape(True)
         rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")

@@ -70,7 +70,7 @@ def test_worker_from_files_bit_exact(orc, tmp_path, fused):
     assert ds.load_worker(w, field_p, level, level) == len(blocks)
     tf = datasets.seeded_tf(5, 256)
     M = datasets.orbit_camera(50.0)
-    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, exact=True)
     w.set_fused(fused)
     for bmin, bmax in [((0, 0, 0), (4, 4, 4)), ((1, 2, 0), (4, 4, 3))]:
         say("worker from files, part", bmin, bmax, "fused", fused)
@@ -99,7 +99,7 @@ def test_worker_secondary_field_from_files(orc, tmp_path):
     M = datasets.orbit_camera(200.0, elevation_deg=-15.0)
     levels = [0.35, 0.6]
     w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, field_mode=api.FIELD_VALUE_ON_ISOSURFACES,
-                iso_opacity=0.7, iso_levels=levels, secondary_range=(lo, hi))
+                iso_opacity=0.7, iso_levels=levels, secondary_range=(lo, hi), exact=True)
     say("worker from files, secondary field")
     rect, bits = w.render_scene_part(level, (0, 0, 0), (4, 4, 4))
     vp = np.zeros((H, W), np.uint32)
@@ -134,7 +134,7 @@ def test_progressive_levels_from_files(orc, tmp_path):
     assert ds.load_worker(w, field_p, 0, 2) == 1 + 8 + 64
     W, H = 128, 96
     tf = datasets.seeded_tf(13, 256)
-    w.set_state(datasets.orbit_camera(75.0), W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+    w.set_state(datasets.orbit_camera(75.0), W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, exact=True)
     frames = [w.compose_frame(level, 3) for level in range(3)]
     for level in range(3):
         say("progressive level", level)

@@ -61,7 +61,7 @@ def _setup(blocks, W, H, fp16=False, acc_cut=2.0, seed=0, tile=(0, 0), layout=ap
                             device="cuda")
         dev_blocks.append({"level": b.level, "index": b.index, "depth": b.depth, "brick": brick,
                            "sparse": to_dev(b.sparse), "plan": plans[key]})
-    fp = api.FramePlan(dev_blocks, W, H, FOV, fmin, fmax, THR, Q, CUT, acc_cut, d_tf, tile, layout)
+    fp = api.FramePlan(dev_blocks, W, H, FOV, fmin, fmax, THR, Q, CUT, acc_cut, d_tf, tile, layout, exact=True)
     return fp, tf, fmin, fmax, dev_blocks
 
 

@@ -53,7 +53,7 @@ def test_large_share_reduced_matches_oracle(orc):
     W, H = 384, 216
     tf_np = datasets.seeded_tf(51, 256)
     tf = torch.from_numpy(tf_np).cuda()
-    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf)
+    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf, exact=True)
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((35.0, 15.0), (220.0, -25.0))])
     fp.set_cameras(cams)
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
@@ -84,7 +84,7 @@ def test_large_share_tf_adjoint_matches_float64(orc):
     tf_t = np.clip(tf_np + rng.normal(0.0, 0.05, size=tf_np.shape), 0.0, 1.0).astype(np.float32)
     M = datasets.orbit_camera(120.0, elevation_deg=20.0)
     tf = torch.from_numpy(tf_np).cuda()
-    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf)
+    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf, exact=True)
     fp.set_cameras(M[None])
     argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
     target = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
@@ -140,7 +140,7 @@ def test_large_share_full_size_properties():
                       for a, e in zip(rng.uniform(0, 360, 3), rng.uniform(-30, 30, 3))])
     frames = {}
     for tile in ((8, 8), (16, 4)):
-        fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf, tile)
+        fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf, tile, exact=True)
         fp.set_cameras(views)
         argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
         say("level-8 frame, tile", tile)

@@ -67,7 +67,7 @@ def test_render_block_modes_bit_exact(orc, mode, thr, iso, levels):
     n = 0
     for i in order:
         b, brick = blocks[int(i)], bricks[int(i)]
-        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996, exact=True)
         lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
         assert live == lo
         if not live:
@@ -100,7 +100,7 @@ def test_render_share_modes_equal_per_block(orc, mode, thr, iso, levels):
     exp = vp_seq.cpu().numpy().view(np.uint32).copy()
     for i in order:
         b = blocks[int(i)]
-        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996, exact=True)
         lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
         if live:
             items.append((p, d_bricks[int(i)]))
@@ -131,7 +131,7 @@ def test_frame_modes_match_oracle(orc, mode, thr, iso, levels):
     plan = api.interp_plan(blocks[0].n2i, 4)
     dev = [{"level": b.level, "index": b.index, "depth": b.depth, "sparse": to_dev(b.sparse), "plan": plan,
             "brick": torch.empty(api.brick_nodes(4), dtype=torch.float32, device="cuda")} for b in blocks]
-    fp = api.FramePlan(dev, W, H, FOV, fmin, fmax, thr, 2.0, 0.996, 0.9999, d_tf)
+    fp = api.FramePlan(dev, W, H, FOV, fmin, fmax, thr, 2.0, 0.996, 0.9999, d_tf, exact=True)
     fp.set_field_mode(mode, iso, levels=levels)
     cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((60.0, 10.0), (300.0, -35.0))])
     fp.set_cameras(cams)
@@ -206,7 +206,7 @@ def test_isosurfaces_without_levels(orc):
     d_vp = to_dev(init.view(np.int32))
     exp = init.copy()
     for b, br in zip(blocks, bricks):
-        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, 0.5, 2.0, 0.996)
+        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, 0.5, 2.0, 0.996, exact=True)
         lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, 0.5, 2.0, 0.996)
         if live:
             api.render_block(p, api.FIELD_ISOSURFACES, 1.0, to_dev(br), d_tf, d_vp, levels=[])
@@ -260,7 +260,7 @@ def test_value_modes_block_and_share_bit_exact(orc, mode, thr, iso, levels):
     for i in order:
         i = int(i)
         b = blocks[i]
-        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+        live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996, exact=True)
         lo, po = orc.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
         assert live == lo
         if not live:
@@ -305,7 +305,7 @@ def test_frame_value_modes_match_oracle(orc, mode, thr, iso, levels, fp16):
     bt = torch.float16 if fp16 else torch.float32
     dev = [{"level": b.level, "index": b.index, "depth": b.depth, "sparse": to_dev(b.sparse), "plan": plan,
             "brick": torch.empty(api.brick_nodes(4), dtype=bt, device="cuda")} for b in blocks]
-    fp = api.FramePlan(dev, W, H, FOV, fmin, fmax, thr, 2.0, 0.996, 0.9999, d_tf)
+    fp = api.FramePlan(dev, W, H, FOV, fmin, fmax, thr, 2.0, 0.996, 0.9999, d_tf, exact=True)
     from revise_amd import RvError
     with pytest.raises(RvError, match="secondary"):
         fp.set_field_mode(mode, iso, levels=levels)
@@ -363,7 +363,7 @@ def test_render_share_tiled_bricks_equal_linear(orc, mode, thr, iso, levels):
         items, items2 = [], []
         for i in order:
             b = blocks[int(i)]
-            live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996)
+            live, p = api.block_params(M, W, H, FOV, b.level, b.index, b.depth, fmin, fmax, thr, 2.0, 0.996, exact=True)
             if live:
                 items.append((p, br[int(i)]))
                 items2.append(br2[int(i)])

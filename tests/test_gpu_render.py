@@ -43,7 +43,7 @@ def test_render_block_bit_exact(orc, angle, level, depth, W, H, q, cut):
     bricks = _bricks(orc, blocks)
     d_tf = to_dev(tf)
     for b, brick in zip(blocks, bricks):
-        live, p = api.block_params(M, W, H, 30.0, b.level, b.index, b.depth, fmin, fmax, 0.5, q, cut)
+        live, p = api.block_params(M, W, H, 30.0, b.level, b.index, b.depth, fmin, fmax, 0.5, q, cut, exact=True)
         lo, po = orc.block_params(M, W, H, 30.0, b.level, b.index, b.depth, fmin, fmax, 0.5, q, cut)
         assert live == lo
         if not live:
@@ -78,7 +78,7 @@ def test_render_share_bit_exact(orc):
     d_bricks = {k: to_dev(v) for k, v in by_index.items()}
     for idx in order:
         idx = tuple(int(v) for v in idx)
-        live, p = api.block_params(M, W, H, 30.0, level, idx, depth, fmin, fmax, 0.5, 2.0, 0.996)
+        live, p = api.block_params(M, W, H, 30.0, level, idx, depth, fmin, fmax, 0.5, 2.0, 0.996, exact=True)
         if live:
             api.render_block_argb(p, d_bricks[idx], d_tf, d_vp)
     got = d_vp.cpu().numpy().view(np.uint32)
@@ -98,7 +98,7 @@ def test_render_constant_field_closed_form(orc):
     brick = np.full(n ** 3, 0.5, dtype=np.float32)
     tf = np.tile(np.array([[0.2, 0.4, 0.6, 0.3]], dtype=np.float32), (256, 1))
     M = datasets.orbit_camera(10.0)
-    live, p = api.block_params(M, W, H, 30.0, 0, (0, 0, 0), depth, 0.0, 1.0, 0.5, 1.0, 2.0)
+    live, p = api.block_params(M, W, H, 30.0, 0, (0, 0, 0), depth, 0.0, 1.0, 0.5, 1.0, 2.0, exact=True)
     _, po = orc.block_params(M, W, H, 30.0, 0, (0, 0, 0), depth, 0.0, 1.0, 0.5, 1.0, 2.0)
     assert live
     d_vp = torch.zeros((H, W), dtype=torch.int32, device="cuda")
@@ -130,7 +130,7 @@ def test_render_all_nan_brick_leaves_viewport(orc):
     brick = np.full((2 ** depth + 1) ** 3, np.nan, dtype=np.float32)
     tf = datasets.seeded_tf(0, 64)
     M = datasets.orbit_camera(50.0)
-    live, p = api.block_params(M, W, H, 30.0, 0, (0, 0, 0), depth, 0.0, 1.0, 0.5, 1.0, 0.996)
+    live, p = api.block_params(M, W, H, 30.0, 0, (0, 0, 0), depth, 0.0, 1.0, 0.5, 1.0, 0.996, exact=True)
     init = np.random.default_rng(2).integers(0, 2 ** 32, size=(H, W), dtype=np.uint64).astype(np.uint32)
     d_vp = to_dev(init.view(np.int32))
     api.render_block_argb(p, to_dev(brick), to_dev(tf), d_vp)
@@ -214,7 +214,7 @@ def test_share_kernel_equals_per_block_launches(orc):
     items = []
     for idx in order:
         b = by[tuple(int(v) for v in idx)]
-        live, p = api.block_params(M, W, H, 30.0, level, b.index, b.depth, fmin, fmax, 0.5, 2.0, 0.996)
+        live, p = api.block_params(M, W, H, 30.0, level, b.index, b.depth, fmin, fmax, 0.5, 2.0, 0.996, exact=True)
         if live:
             items.append((p, to_dev(orc.interpolate(b.sparse, b.n2i, b.depth))))
     init = rng.integers(0, 2 ** 32, size=(H, W), dtype=np.uint64).astype(np.uint32)

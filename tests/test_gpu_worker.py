@@ -35,7 +35,7 @@ def _make(level, depth, seed, W, H, angle):
     w = api.Worker(0)
     for b in blocks.values():
         w.add_subtree(level, b.index, b.depth, b.n2i, b.sparse)
-    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, exact=True)
     return w, blocks, fmin, fmax, tf, M
 
 
@@ -163,7 +163,7 @@ def test_worker_field_modes_bit_exact(orc, mode, fused):
     lvl, iso = 0.42, 0.8
     levels = [0.6, 0.35, 0.45] if mode == api.FIELD_ISOSURFACES else None
     w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, field_mode=mode, iso_level_rel=lvl,
-                iso_opacity=iso, iso_levels=levels)
+                iso_opacity=iso, iso_levels=levels, exact=True)
     w.set_fused(fused)
     say("worker mode", mode, "fused", fused)
     rect, bits = w.render_scene_part(2, (0, 0, 0), (4, 4, 4))
@@ -226,7 +226,7 @@ def test_worker_value_modes_bit_exact(orc, mode, fused):
     lvl, iso = 0.47, 0.75
     levels = [0.3, 0.5, 0.68] if mode == api.FIELD_VALUE_ON_ISOSURFACES else None
     w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, field_mode=mode, iso_level_rel=lvl,
-                iso_opacity=iso, iso_levels=levels, secondary_range=(lo, hi))
+                iso_opacity=iso, iso_levels=levels, secondary_range=(lo, hi), exact=True)
     w.set_fused(fused)
     say("worker value mode", mode, "fused", fused)
     for bmin, bmax in [((0, 0, 0), (4, 4, 4)), ((1, 0, 1), (3, 4, 4))]:
@@ -284,7 +284,7 @@ def test_worker_fused_deeper_subtree_at_next_level(orc):
         for lv, bl in levels.items():
             for b in bl.values():
                 w.add_subtree(lv, b.index, b.depth, b.n2i, b.sparse)
-        w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+        w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, exact=True)
         w.set_fused(fused)
         ws[fused] = w
     for lv in (2, 3, 2):
@@ -314,7 +314,7 @@ def test_worker_replaced_values_rerender(orc):
         w = api.Worker(0)
         for b in blocks.values():
             w.add_subtree(1, b.index, b.depth, b.n2i, b.sparse)
-        w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf)
+        w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, exact=True)
         w.set_fused(fused)
         ws[fused] = w
     # a one-block box, so the per-block renderer's cached brick is this block
