@@ -339,6 +339,11 @@ constexpr uint32_t kShareT = 256;
 #define RV_SHARE_WGS_PER_CU 4
 #endif
 constexpr uint32_t kShareWgsPerCu = RV_SHARE_WGS_PER_CU;
+// floats of a wave's LDS brick window (fast Argb; march_fast_window)
+#ifndef RV_SHARE_WIN
+#define RV_SHARE_WIN 0
+#endif
+constexpr uint32_t kShareWin = RV_SHARE_WIN;
 
 template <uint32_t kMode, class BrickT>
 __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* __restrict__ params,
@@ -351,7 +356,7 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         uint32_t* __restrict__ viewport,
                                                         uint32_t W, uint32_t H)
 {
-    extern __shared__ float4 tfs_share[]; // tf_n entries (and padding)
+    extern __shared__ float4 tfs_share[]; // tf_n entries, then (fast Argb) the waves' brick windows
     __shared__ float lvs[kMaxLevels];
     for (uint32_t i = threadIdx.x; i < tf_n; i += kShareT)
         tfs_share[i] = tf[i];
@@ -367,6 +372,7 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
     const bool inside = vx < W && vy < H;
     const uint32_t pix = inside ? (H - 1u - vy) * W + vx : 0u;
     uint32_t value = inside ? viewport[pix] : 0u;
+    float* win = reinterpret_cast<float*>(tfs_share + tf_n) + wave * kShareWin;
     for (uint32_t c = 0; c < nblocks; c += 64u) {
         const uint32_t i = c + lane;
         bool hit = false;
@@ -379,8 +385,22 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
             const uint32_t blk = c + (uint32_t)__builtin_ctzll(m);
             m &= m - 1ull;
             const rv_block_params p = params[blk];
-            if (inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
-                vy < p.view_y + p.view_h) {
+            const bool inrect = inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
+                                vy < p.view_y + p.view_h;
+            if constexpr (kMode == kModeArgbFast && kShareWin > 0u) {
+                // the whole wave marches the block with its brick staged in
+                // the wave's LDS window
+                const RaySeg r = setup_ray(p, vx, vy);
+                const bool act = inrect && r.hit;
+                if (__ballot(act) != 0ull) {
+                    const float4 seg = march_fast_window(p, reinterpret_cast<const BrickT*>(bricks[blk]),
+                                                         tfs_share, (int)tf_n, r, act, win, (int)kShareWin);
+                    if (act)
+                        value = blend_above_u32(value, seg);
+                }
+                continue;
+            }
+            if (inrect) {
                 const RaySeg r = setup_ray(p, vx, vy);
                 if (r.hit) {
                     const BrickT* brick = reinterpret_cast<const BrickT*>(bricks[blk]);
@@ -1454,7 +1474,8 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
 {
     // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
     // LDS per workgroup at most)
-    const size_t lds = std::max<size_t>(sizeof(float4) * tf_n,
+    const bool win = field_mode == RV_FIELD_ARGB && !exact && kShareWin > 0u; // the fast Argb march stages bricks in LDS
+    const size_t lds = std::max<size_t>(sizeof(float4) * tf_n + (win ? 4u * (kShareT / 64u) * kShareWin : 0u),
                                         std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u));
     const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
     const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);

@@ -200,3 +200,41 @@ def test_fast_worker_fused_equals_per_block():
         w.close()
     assert parts[True][0] == parts[False][0]
     assert np.array_equal(parts[True][1], parts[False][1])
+
+
+@pytest.mark.parametrize("case", ["deep brick", "close camera", "camera inside"])
+def test_fast_share_windows_equal_direct_march(orc, case):
+    """The share kernel stages bricks in per-wave LDS windows (halved slabs,
+    direct reads where a window cannot hold a slab, no windows where rays run
+    both ways along the slab axis); the per-block kernel reads the brick
+    directly.  Same samples and operations: equal bit for bit."""
+    import torch
+    W, H = 200, 150
+    if case == "deep brick":
+        level, depth, M = 0, 7, datasets.orbit_camera(33.0, elevation_deg=25.0)
+    elif case == "close camera":
+        level, depth, M = 1, 5, datasets.look_at((3.0, 2.5, 6.5), (0.0, -0.5, 0.0))
+    else:
+        level, depth, M = 1, 4, datasets.look_at((0.3, -0.4, 0.2), (3.0, 1.0, -2.0))
+    blocks = _blocks(45, level, depth, 0.7)
+    fmin, fmax = datasets.field_range(blocks)
+    tf = to_dev(datasets.seeded_tf(45, 256))
+    m = 2 ** level
+    order = api.back_to_front([0] * 3, [m] * 3, [-5] * 3, [5] * 3, api.eye_from_view(M))
+    by = {b.index: b for b in blocks}
+    items = []
+    for idx in order:
+        b = by[tuple(int(v) for v in idx)]
+        live, p = api.block_params(M, W, H, FOV, level, b.index, b.depth, fmin, fmax, THR, Q, 0.996)
+        if live:
+            items.append((p, to_dev(orc.interpolate(b.sparse, b.n2i, b.depth))))
+    assert items
+    a = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    for p, br in items:
+        api.render_block_argb(p, br, tf, a)
+    b_ = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    api.ShareRenderer(items, tf, W, H).render(b_)
+    got = a.cpu().numpy()
+    say(case, "covered pixels", int(((got.view(np.uint32) >> 24) > 0).sum()))
+    assert ((got.view(np.uint32) >> 24) > 0).any()
+    assert np.array_equal(got, b_.cpu().numpy())
