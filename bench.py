@@ -3,27 +3,32 @@
 Headline workload (`value`): the GPU plain drop-in frame -- a block tree
 refined to a uniform level 3 (512 blocks), each block a fully refined depth-6
 subtree (65^3 nodes, fp32), field = 3 seeded Gaussians sampled at the nodes.
-One step = one frame of the reference's per-worker path
-(VsWorker_cu::renderScenePart) with its exact per-block semantics: clear the
-1920x1080 ARGB viewport, interpolate every block's brick from its sparse node
-values, ray-cast the blocks in BackToFrontOrder (step 0.5 voxel = quality 2,
-early termination at alpha > 0.99) with the per-block 8-bit composite, find
-the non-transparent rectangle and download it.  The drop-in worker does this
-in two launches (batched interpolation, then the whole share), bit-identical
-to one interpolation + one ray-cast launch per block; the per-block launch
-structure is timed too.  `value` = frames/s of that step.
+One step = one call of the worker's C ABI, rv_worker_render_scene_part
+(VsWorker_cu::renderScenePart) over the whole index box, with the camera
+moved every step (a new orbit angle, set with rv_worker_set_state): clear the
+1920x1080 ARGB viewport, BackToFrontOrder, every block's parameters,
+interpolate every block's brick from its sparse node values, ray-cast the
+blocks back to front (step 0.5 voxel = quality 2, early termination at alpha
+> 0.99) with the per-block 8-bit composite, find the non-transparent
+rectangle and download it into a pinned host buffer.  `value` = frames/s.
 
 The JSON line also carries:
+* the same frame with one interpolation and one ray-cast launch per block
+  (the reference's structure);
 * interpolation of all 512 bricks in one launch: Gvoxels/s and the HBM
   roofline of that kernel (`roofline`);
 * the render kernel: exact samples per frame S (reference sampling rule, no
-  early termination) and samples/s;
+  early termination), samples/s, and its VALU and L1 rates against the
+  chip's peaks measured by the library's probe kernels;
+* the CPU interpolation configuration's brick (level 2, depth 6, refined
+  tree) interpolated on the GPU;
 * the batched persistent frame on the same data (interpolation + all blocks in
   one launch, front-to-back in registers);
 * the 2000-block mixed-level configuration (levels 2-6, fill 5-100 %, 8
   Gaussians + 3-octave noise): batched persistent frames over 64 cameras
   replayed from one hipGraph vs the plain per-block path on the same data;
-* `cpu_baseline`: the CPU oracle on the box's host cores, bounded sample.
+* forward + adjoint at 512x512, and the level-8 fp16 share at 4K;
+* `cpu_baseline`: the CPU oracle on every host core (see cpu_baseline()).
 """
 import argparse
 import json
@@ -135,6 +140,52 @@ class PlainFrame:
         return int(c.item())
 
 
+class WorkerFrame:
+    """The drop-in through the worker's C ABI: the 512 subtrees registered
+    with rv_worker_add_subtree (host n2i and values), and per step
+    rv_worker_set_state with the step's camera, then
+    rv_worker_render_scene_part over the whole index box into a pinned host
+    buffer."""
+
+    def __init__(self, torch, wl, tf_np, angle0, fused=True):
+        import ctypes as C
+        from revise_amd import api, lib, RenderState, check
+        self.C, self.api, self.lib, self.check = C, api, lib, check
+        self.w = api.Worker(0)
+        sparse = wl.sparse.cpu().numpy()
+        n2i = wl.n2i["full"]
+        for i, b in enumerate(wl.blocks):
+            self.w.add_subtree(b["level"], b["index"], b["depth"], n2i, sparse[i])
+        self.w.set_fused(fused)
+        self.tf = np.ascontiguousarray(tf_np, dtype=np.float32)
+        self.view = np.zeros(16, np.float32)
+        self.state = RenderState(self.view.ctypes.data, W, H, FOV, Q2, THR, CUT2, wl.fmin, wl.fmax,
+                                 (C.c_float * 3)(0.0, 0.0, 0.0), self.tf.ctypes.data, self.tf.shape[0],
+                                 api.FIELD_ARGB, 0.5, 1.0, None, 0, 0.0, 1.0, 0)
+        self.out = torch.empty(W * H, dtype=torch.int32, pin_memory=True)
+        self.out_p = C.cast(self.out.data_ptr(), C.POINTER(C.c_uint32))
+        self.rect = (C.c_int32 * 4)()
+        self.bmin = (C.c_uint32 * 3)(0, 0, 0)
+        self.bmax = (C.c_uint32 * 3)(8, 8, 8)
+        self.cancel = C.c_int(0)
+        self.angle0 = angle0
+
+    def camera(self, k):
+        return datasets_mod().orbit_camera(self.angle0 + 7.5 * k)
+
+    def step(self, k):
+        C = self.C
+        self.view[:] = self.camera(k).reshape(16)
+        L = self.lib()
+        self.check(L.rv_worker_set_state(self.w.h, C.byref(self.state)))
+        self.check(L.rv_worker_render_scene_part(self.w.h, 3, self.bmin, self.bmax, C.byref(self.cancel),
+                                                  self.rect, self.out_p, W * H))
+        return tuple(self.rect)
+
+    def close(self):
+        self.w.close()
+
+
 def profiled_traffic(key="traffic_bytes_per_launch"):
     """HBM bytes per launch of the batched interpolation, as measured by the
     FETCH_SIZE / WRITE_SIZE counter runs summarised under profiles/ (None if
@@ -144,6 +195,18 @@ def profiled_traffic(key="traffic_bytes_per_launch"):
         for line in open(path):
             if line.startswith(key + ":"):
                 return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def profiled(path, key):
+    """A figure from a profile summary under profiles/ ("key: value" line),
+    or None."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", path)):
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1])
     except OSError:
         pass
     return None
@@ -168,63 +231,105 @@ def frame_plan(torch, wl, tf, quality, cutoff, tile=(0, 0), blocks=None):
                          cutoff, ACC_CUT, tf, tile, layout)
 
 
-def cpu_baseline(wl, M, tf_np, budget_s=20.0):
-    """Oracle on the host: whole-brick interpolation single-threaded
-    (Gvoxels/s) and the per-block ray-cast of all 512 blocks over a band of
-    rows with every host core (scaled to frames/s)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, tf_np, cfg1, runs=5):
+    """The oracle on the host, every core this process may use:
+    * interpolation of the CPU configuration's brick (level-2 root, depth 6,
+      65^3 nodes, a partly refined tree, so the midpoint rule does real
+      work): Gvoxels/s on 1 core (median of `runs`) and on all cores (each
+      core interpolating its own copy, median of `runs` rounds);
+    * the per-block ray-cast of config 2's frame (512 blocks, 1920x1080,
+      back to front, 8-bit composite) on a stratified sample of the rows --
+      16 bands of 8 rows spread evenly over the frame, 128 of 1080 rows --
+      on all cores, scaled to the frame: frames/s, median of `runs`."""
     import ctypes as C
+    import statistics
     import threading
     from oracle import oracle as orc
-    ncores = min(os.cpu_count() or 1, 16)
-    n2i = wl.n2i["full"]
-    sp = wl.sparse[:2].cpu().numpy()
-    t0 = time.perf_counter()
-    k = 0
-    while k < 2 or time.perf_counter() - t0 < budget_s * 0.2:
-        orc.interpolate(sp[k % 2], n2i, 6)
-        k += 1
-    t_interp = (time.perf_counter() - t0) / k
-    gvox_1core = (65 ** 3) / t_interp / 1e9
-    bricks = wl.bricks.cpu().numpy()
-    keys = np.array([(b["level"],) + tuple(b["index"]) for b in wl.blocks], dtype=np.uint32)
     from revise_amd import api
+    ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    n2i1, d1, sp1 = cfg1["n2i"], cfg1["depth"], cfg1["sparse"]
+    V1 = (2 ** d1 + 1) ** 3
+    t1 = []
+    for _ in range(runs + 1):
+        t0 = time.perf_counter()
+        orc.interpolate(sp1, n2i1, d1)
+        t1.append(time.perf_counter() - t0)
+    t_1core = statistics.median(t1[1:])
+
+    def all_cores_round():
+        ths = [threading.Thread(target=orc.interpolate, args=(sp1, n2i1, d1)) for _ in range(ncores)]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        return time.perf_counter() - t0
+
+    all_cores_round()
+    t_all = statistics.median(all_cores_round() for _ in range(runs))
+    # ray-cast: the oracle's per-block 8-bit frame over sampled rows
+    bricks = wl.bricks.cpu().numpy()
+    angle = float(np.random.default_rng(SEED).uniform(0.0, 360.0))
+    M = datasets_mod().orbit_camera(angle)
+    keys = np.array([(b["level"],) + tuple(b["index"]) for b in wl.blocks], dtype=np.uint32)
     order = api.mixed_back_to_front(keys, api.eye_from_view(M))
-    band = 24
-    y0 = H // 2 - band // 2
-    vp = np.zeros((H, W), np.uint32)
+    items = []
+    for i in order:
+        b = wl.blocks[int(i)]
+        live, po = orc.block_params(M, W, H, FOV, b["level"], b["index"], b["depth"], wl.fmin, wl.fmax,
+                                    THR, Q2, CUT2)
+        if live:
+            items.append((po, bricks[int(i)].ctypes.data_as(C.POINTER(C.c_float))))
+    bands = [(int(k * H / 16), int(k * H / 16) + 8) for k in range(16)]
     L = orc.lib()
     tp = np.ascontiguousarray(tf_np).ctypes.data_as(C.POINTER(C.c_float))
+    vp = np.zeros((H, W), np.uint32)
     vpp = vp.ctypes.data_as(C.POINTER(C.c_uint32))
-    t0 = time.perf_counter()
-    reps = 0
-    while reps < 1 or time.perf_counter() - t0 < budget_s * 0.5:
-        for i in order:
-            b = wl.blocks[int(i)]
-            live, po = orc.block_params(M, W, H, FOV, b["level"], b["index"], b["depth"], wl.fmin, wl.fmax,
-                                        THR, Q2, CUT2)
-            if not live or po.view_y > y0 + band or po.view_y + po.view_h < y0:
-                continue
-            bp = bricks[int(i)].ctypes.data_as(C.POINTER(C.c_float))
-            rows = list(range(y0, y0 + band))
-            chunk = max(1, math.ceil(band / ncores))
-            ths = []
-            for j in range(0, band, chunk):
-                th = threading.Thread(target=L.orc_render_block,
-                                      args=(C.byref(po), bp, tp, tf_np.shape[0], vpp, rows[j],
-                                            rows[min(band, j + chunk) - 1] + 1))
-                th.start()
-                ths.append(th)
-            for th in ths:
-                th.join()
-        reps += 1
-    t_band = (time.perf_counter() - t0) / reps
-    fps = 1.0 / (t_band * H / band)
+
+    def worker(my_bands):
+        for r0, r1 in my_bands:
+            for po, bp in items:
+                if po.view_y < r1 and po.view_y + po.view_h > r0:
+                    L.orc_render_block(C.byref(po), bp, tp, tf_np.shape[0], vpp, r0, r1)
+
+    def render_round():
+        vp[:] = 0
+        ths = [threading.Thread(target=worker, args=(bands[j::ncores],)) for j in range(min(ncores, len(bands)))]
+        t0 = time.perf_counter()
+        for th in ths:
+            th.start()
+        for th in ths:
+            th.join()
+        return time.perf_counter() - t0
+
+    t_rows = statistics.median(render_round() for _ in range(runs))
+    rows = sum(r1 - r0 for r0, r1 in bands)
+    fps = 1.0 / (t_rows * H / rows)
     return {"value": fps, "unit": "frames/s", "cores": ncores, "kind": "port",
-            "sample": "CPU oracle: per-block ray-cast of all 512 blocks over the middle %d rows of the "
-                      "1080p frame, %d threads, scaled by 1080/%d (interpolation is not in this "
-                      "figure: whole 65^3 bricks single-threaded run at %.3f Gvoxels/s)"
-                      % (band, ncores, band, gvox_1core),
-            "interp_gvox_s_1core": gvox_1core}
+            "sample": "CPU oracle (the C++ restatement of the reference's interpolation and Argb ray-cast), "
+                      "%d host threads: the per-block ray-cast of config 2's 512 blocks at 1920x1080 over 16 "
+                      "bands of 8 rows spread evenly over the frame (%d of %d rows, a stratified sample), "
+                      "scaled to the frame, median of %d runs; interpolation: the CPU configuration's "
+                      "level-2 depth-6 brick (%d of %d nodes given)" % (ncores, rows, H, runs, n2i1.shape[0], V1),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "interp_gvox_s_1core": V1 / t_1core / 1e9,
+            "interp_gvox_s_all_cores": ncores * V1 / t_all / 1e9,
+            "interp_ms_1core": t_1core * 1e3}
+
+
+def datasets_mod():
+    from revise_amd import datasets
+    return datasets
 
 
 def large_share_bench(torch, args, tf_np, log):
@@ -305,6 +410,8 @@ def large_share_bench(torch, args, tf_np, log):
             fp.render(v, argb, rgba, False)
             fp.adjoint(v, rgba, rgba - targets[v], gtf)
 
+    t_fw_fast = events_time(torch, fwd8, 2)
+    fp.set_exact(True)  # the sequence the adjoint differentiates
     t_fw = events_time(torch, fwd8, 2)
     fp.set_tape(True)
     fwd_adj8()
@@ -325,7 +432,8 @@ def large_share_bench(torch, args, tf_np, log):
         "frame_4k_ms": t_f * 1e3, "frame_4k_fps": 1.0 / t_f, "frame_4k_with_interp_ms": t_fi * 1e3,
         "frame_4k_with_interp_one_launch_ms": t_fi1 * 1e3, "frame_4k_with_interp_split_ms": t_fi2 * 1e3,
         "camera_setup_s_8_views": t_setup, "tile": list(tile),
-        "fwd_8views_ms": t_fw * 1e3, "fwd_adjoint_8views_ms": t_fa * 1e3, "ratio": t_fa / t_fw}
+        "fwd_8views_ms": t_fw * 1e3, "fwd_8views_fast_sequence_ms": t_fw_fast * 1e3,
+        "fwd_adjoint_8views_ms": t_fa * 1e3, "ratio": t_fa / t_fw}
     fp.close()
     del ib, ls, targets
     torch.cuda.empty_cache()
@@ -339,7 +447,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-runs", type=int, default=5, help="median over this many CPU baseline runs")
     ap.add_argument("--cams", type=int, default=64)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-large", action="store_true", help="skip the level-8 fp16 share")
@@ -349,6 +457,8 @@ def main():
     ap.add_argument("--only-render", action="store_true",
                     help="run the render kernels only: share kernel x3, batched frame (resident "
                          "bricks, 8x8 bins) x3 at 1920x1080, then x3 at 512x512 (profiler runs)")
+    ap.add_argument("--only-dropin", action="store_true",
+                    help="the drop-in worker frames only: 5 warm-up and 20 timed steps (profiler runs)")
     ap.add_argument("--only-interp", action="store_true",
                     help="time a 1 GiB device copy and the batched interpolation only (profiler runs)")
     args = ap.parse_args()
@@ -392,6 +502,18 @@ def main():
         batch_own.run()
         log("batched interp, private plans: %.3f ms" % (events_time(torch, batch_own.run, 10) * 1e3))
         return
+    if args.only_dropin:
+        wf = WorkerFrame(torch, wl, tf_np, angle)
+        for k in range(5):
+            wf.step(k)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(20):
+            wf.step(5 + k)
+        torch.cuda.synchronize()
+        log("drop-in worker frame %.3f ms" % ((time.perf_counter() - t0) / 20 * 1e3))
+        wf.close()
+        return
     if args.only_adjoint:
         plain.step(crop=False)  # interpolates the bricks the frame plan reads
         fpa = api.FramePlan(wl.blocks, 512, 512, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf)
@@ -413,6 +535,17 @@ def main():
         plain.fused_step()
         for _ in range(3):
             plain.share.render(plain.vp)
+        # the exact sample sequence of the same share
+        from revise_amd import RENDER_EXACT
+        exact_items = []
+        for (b, live, p), t in zip(plain.items, plain.tiled):
+            if live:
+                q = api.BlockParams.from_buffer_copy(p)
+                q.flags |= RENDER_EXACT
+                exact_items.append((q, t))
+        share_exact = api.ShareRenderer(exact_items, tf, W, H, layout=api.BRICK_TILED4)
+        for _ in range(3):
+            share_exact.render(plain.vp)
         fpr = frame_plan(torch, wl, tf, Q2, CUT2, tile=(8, 8))
         fpr.set_cameras(M[None])
         argb_r = torch.empty((H, W), dtype=torch.int32, device="cuda")
@@ -429,24 +562,37 @@ def main():
                                                      events_time(torch, lambda: fps.render(0, argb_s, None, False), 3) * 1e3))
         fps.close()
         return
-    for _ in range(args.warmup):
-        plain.fused_step()
+    # the headline: rv_worker_render_scene_part through the C ABI, a new
+    # camera every step
+    wf = WorkerFrame(torch, wl, tf_np, angle)
+    log("config 2 worker: 512 subtrees registered")
+    for k in range(args.warmup):
+        wf.step(k)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        plain.fused_step()
+    for k in range(args.steps):
+        rect = wf.step(args.warmup + k)
     torch.cuda.synchronize()
     t_step = (time.perf_counter() - t0) / args.steps
-    log("drop-in worker frame (fused) %.3f ms" % (t_step * 1e3))
-    plain.step()
+    log("drop-in worker frame through the C ABI, moving camera: %.3f ms (last crop %s)" % (t_step * 1e3, rect))
+    wf.close()
+    # the reference's structure: one interpolation and one ray-cast launch per block
+    wpb = WorkerFrame(torch, wl, tf_np, angle, fused=False)
+    wpb.step(0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(2):
-        plain.step()
+    for k in range(2):
+        wpb.step(1 + k)
     torch.cuda.synchronize()
     t_step_pb = (time.perf_counter() - t0) / 2
-    log("per-block launches frame %.3f ms" % (t_step_pb * 1e3))
-    t_share = events_time(torch, lambda: plain.share.render(plain.vp), 5)
+    wpb.close()
+    log("worker frame, per-block launches: %.3f ms" % (t_step_pb * 1e3))
+    # the fused two-launch frame composed from Python at one fixed camera
+    # (the kernels alone, no per-step host work)
+    for _ in range(2):
+        plain.fused_step()
+    t_fixed = events_time(torch, lambda: plain.fused_step(), 10)
+    t_share = events_time(torch, lambda: plain.share.render(plain.vp), 10)
 
     batch = api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"]) for b in wl.blocks])
     batch.run()
@@ -500,13 +646,15 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded Gaussian field sampled at block-tree nodes)",
-        "config": {"workload": "plain drop-in frame: 512 level-3 blocks x 65^3 fp32 bricks (full fill), "
-                               "1920x1080 ARGB, quality 2, alpha cutoff 0.99, interpolate every brick + "
-                               "ray-cast in back-to-front order with per-block 8-bit composite, crop + "
-                               "download into a pinned host buffer",
+        "config": {"workload": "plain drop-in frame through rv_worker_render_scene_part, a new camera every "
+                               "step: 512 level-3 blocks x 65^3 fp32 bricks (full fill), 1920x1080 ARGB, "
+                               "quality 2, alpha cutoff 0.99, block order + parameters, interpolate every "
+                               "brick, ray-cast back to front with the per-block 8-bit composite (fast sample "
+                               "sequence), crop + download into a pinned host buffer",
                    "blocks": len(wl.blocks), "brick_nodes": 65 ** 3, "viewport": [W, H], "seed": SEED,
                    "camera_angle_deg": angle},
         "per_block_launches_frame_ms": t_step_pb * 1e3,
+        "dropin_fixed_camera_kernels_ms": t_fixed * 1e3,
         "interp_gvox_s": V_total / t_interp / 1e9,
         "interp_ms_batched": t_interp * 1e3,
         "interp_ms_batched_private_plans": t_interp_own * 1e3,
@@ -530,13 +678,61 @@ def main():
                              "node read) / kernel time; traffic = HBM bytes per launch from rocprofv3 "
                              "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy; the 512 bricks share one "
                              "plan, which stays in cache (private plans: interp_ms_batched_private_plans)"},
-        "render_kernel": {"kernel": "share_kernel (all 512 blocks, per-block 8-bit composite)",
-                          "bound": "valu/lds", "ms": t_share * 1e3,
+        "render_kernel": {"kernel": "share_kernel (all 512 blocks, per-block 8-bit composite, fast sample "
+                                    "sequence, micro-brick bricks)",
+                          "bound": "valu / l1 tag lookups", "ms": t_share * 1e3,
                           "gsamples_s": S / t_share / 1e9,
                           "brick_bytes_per_frame": 4 * V_total,
                           "brick_gb_s": 4 * V_total / t_share / 1e9},
     })
     del batch
+
+    # ---------------- config 1: the refined tree's level-2 depth-6 brick
+    t0 = time.time()
+    cfg1 = configs.cpu_interp_brick(SEED)
+    n2i1, d1 = cfg1["n2i"], cfg1["depth"]
+    sp1 = torch.from_numpy(cfg1["sparse"]).cuda()
+    plan1 = api.interp_plan(n2i1, d1)
+    dense1 = torch.empty(api.brick_nodes(d1), dtype=torch.float32, device="cuda")
+    api.interp_brick_planned(sp1, plan1, d1, dense1)
+    t_i1 = events_time(torch, lambda: api.interp_brick_planned(sp1, plan1, d1, dense1), 50)
+    V1 = api.brick_nodes(d1)
+    log("config 1: %d octree leaves, level-2 root %s, depth %d, %d of %d nodes given; GPU %.1f us, "
+        "%.1f Gvox/s (setup %.1fs)" % (len(cfg1["tree"].leaves), cfg1["index"], d1, n2i1.shape[0], V1,
+                                      t_i1 * 1e6, V1 / t_i1 / 1e9, time.time() - t0))
+    out["cpu_interp_config"] = {
+        "workload": "3 Gaussians, octree refined to level 4 where |grad f| > 0.5 max (%d leaf blocks of "
+                    "16^3 cells), the level-2 root %s: depth-6 subtree, %d of 65^3 nodes given"
+                    % (len(cfg1["tree"].leaves), cfg1["index"], n2i1.shape[0]),
+        "gpu_interp_us": t_i1 * 1e6, "gpu_gvox_s": V1 / t_i1 / 1e9}
+
+    # ---------------- peaks measured on this chip, and the render kernel against them
+    peaks = api.probe_peaks(torch)
+    log("peaks: VALU %.1f T lane-fma/s, LDS %.1f TB/s, L1 gather %.2f G loads/s coalesced, %.2f G scattered, "
+        "HBM stream %.0f GB/s" % (peaks["valu_fma_s"] / 1e12, peaks["lds_bytes_s"] / 1e12,
+                                  peaks["gather_coalesced_s"] / 1e9, peaks["gather_scattered_s"] / 1e9,
+                                  peaks["stream_bytes_s"] / 1e9))
+    prof = "share_kernel_512x65cubed_1080p_counters.md"
+    valu_insts = profiled(prof, "fast_sq_insts_valu")
+    tcp_acc = profiled(prof, "fast_tcp_total_cache_accesses")
+    rk = out["render_kernel"]
+    rk["peaks_measured"] = peaks
+    if valu_insts:
+        # wave instructions per second against the probe's (a lane fma is one
+        # 64th of a wave fma instruction)
+        rk["valu"] = {"achieved": valu_insts / t_share, "peak": peaks["valu_fma_s"] / 64, "unit": "wave insts/s",
+                      "frac": valu_insts / t_share / (peaks["valu_fma_s"] / 64),
+                      "source": "SQ_INSTS_VALU per frame from profiles/" + prof}
+    if tcp_acc:
+        # L1 tag lookups per second against the scattered-gather probe's (64
+        # per load instruction)
+        rk["l1_lookups"] = {"achieved": tcp_acc / t_share, "peak": peaks["gather_scattered_s"] * 64,
+                            "unit": "TCP accesses/s",
+                            "frac": tcp_acc / t_share / (peaks["gather_scattered_s"] * 64),
+                            "source": "TCP_TOTAL_CACHE_ACCESSES per frame from profiles/" + prof}
+    if valu_insts and tcp_acc:
+        rk["frac"] = max(rk["valu"]["frac"], rk["l1_lookups"]["frac"])
+        rk["bound"] = "valu" if rk["valu"]["frac"] >= rk["l1_lookups"]["frac"] else "l1 tag lookups"
 
     # ---------------- config 3: 2000 mixed-level blocks, batched vs plain
     if not args.no_mixed:
@@ -581,8 +777,10 @@ def main():
         rng4 = np.random.default_rng(SEED + 4)
         tf_t = torch.from_numpy(np.clip(tf_np + rng4.normal(0.0, 0.05, size=tf_np.shape), 0.0, 1.0)
                                 .astype(np.float32)).cuda()
+        # the adjoint differentiates the exact sample sequence (frames with the
+        # tape run it), so the forward it is compared with is the exact one
         fp4 = api.FramePlan(tblocks, W4, H4, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf, (0, 0),
-                            api.BRICK_TILED4)
+                            api.BRICK_TILED4, exact=True)
         fp4t = api.FramePlan(tblocks, W4, H4, FOV, wl.fmin, wl.fmax, THR, Q2, CUT2, ACC_CUT, tf_t, (0, 0),
                              api.BRICK_TILED4)
         fp4.set_cameras(M[None])
@@ -607,6 +805,10 @@ def main():
             fp4.adjoint(0, rgba4, gI, gtf4)
 
         t_fwd4_plain = events_time(torch, fwd4, 10)
+        fp4.set_exact(False)
+        fwd4()
+        t_fwd4_fast = events_time(torch, fwd4, 10)
+        fp4.set_exact(True)
         fp4.set_tape(True)  # the forward of a training step records the adjoint's segment states
         fwd4()
         adj4()
@@ -629,7 +831,8 @@ def main():
             "workload": "512 level-3 blocks x 65^3 fp32 (resident), 512x512, L = 1/2 |image - target|^2 "
                         "with the target rendered from the colour table + seeded N(0,0.05); gradients "
                         "w.r.t. the 256x4 table and all 140.6M field values (zeroed every step)",
-            "forward_ms": t_fwd4_plain * 1e3, "forward_with_tape_ms": t_fwd4 * 1e3,
+            "forward_ms": t_fwd4_plain * 1e3, "forward_fast_sequence_ms": t_fwd4_fast * 1e3,
+            "forward_with_tape_ms": t_fwd4 * 1e3,
             "fwd_adjoint_ms": t_fa4 * 1e3, "adjoint_ms": t_adj4 * 1e3,
             "ratio_fwd_adjoint_over_fwd": t_fa4 / t_fwd4_plain,
             "tf_only_fwd_adjoint_ms": t_fa4_tf * 1e3}
@@ -641,8 +844,7 @@ def main():
         out["large_share"] = large_share_bench(torch, args, tf_np, log)
 
     if not args.no_cpu:
-        cb = cpu_baseline(wl, M, tf_np, args.cpu_budget)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["cpu_baseline"] = cpu_baseline(wl, tf_np, cfg1, args.cpu_runs)
     print(json.dumps(out))
 
 

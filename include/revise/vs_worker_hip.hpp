@@ -215,6 +215,7 @@ private:
     uint64_t m_viewportPixels = 0;
     uint32_t* m_part = nullptr;
     uint64_t m_partCap = 0;
+    int* m_rectBuf = nullptr; // 16 bytes: the crop rectangle's reduction
     std::array<int, 4> m_partRect{0, 0, 0, 0};
     bool m_cancelled = false;
 };

@@ -346,6 +346,27 @@ int rv_render_share_argb_noalloc(const rv_block_params* params, const float* con
                                  uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
                                  rv_stream_t stream);
 
+/* Measurement helpers (no reference counterpart): kernels that measure the
+ * chip's peaks the path's kernels are held against.  Each enqueues one
+ * launch of `blocks` workgroups of 256 threads on `stream` and returns its
+ * work count; the caller times the launch.  d_sink: blocks*256 floats the
+ * kernels write only in a case that does not occur (it keeps the work live).
+ *  - valu: 8 independent non-packed fp32 fma chains per thread, iters long;
+ *    *fma_count = lane fmas.
+ *  - lds: ds_read_b128 of conflict-free addresses, iters per thread;
+ *    *bytes = bytes read.
+ *  - gather: iters dword loads per thread from d_src (16 KB, L1-resident),
+ *    coalesced (scattered = 0: a wave reads 256 consecutive bytes) or
+ *    scattered (1: each lane a different 64-byte sector); *loads = wave load
+ *    instructions.
+ *  - stream: non-temporal copy of `bytes` (a multiple of 16) from d_src to
+ *    d_dst (HBM read + write). */
+int rv_probe_valu(float* d_sink, uint32_t blocks, uint32_t iters, uint64_t* fma_count, rv_stream_t stream);
+int rv_probe_lds(float* d_sink, uint32_t blocks, uint32_t iters, uint64_t* bytes, rv_stream_t stream);
+int rv_probe_gather(const float* d_src, float* d_sink, uint32_t blocks, uint32_t iters, int scattered,
+                    uint64_t* loads, rv_stream_t stream);
+int rv_probe_stream(const void* d_src, void* d_dst, uint64_t bytes, uint32_t blocks, rv_stream_t stream);
+
 /* Piecewise-linear colour table from control points.  Replaces
  * s3vs::makeColorTransferVector (ColorTransferFunction.hpp:27-69).
  * xs: count sorted positions, colors: count x 4, out: color_count x 4. */
@@ -492,6 +513,10 @@ int rv_frame_set_hybrid_min(rv_frame_plan* plan, uint32_t blocks);
  * interpolates the bricks, then the resident-brick kernel with its own grid).
  * Same images; rv_frame_autotune with interpolate != 0 picks the faster. */
 int rv_frame_set_split(rv_frame_plan* plan, int split);
+/* The Argb sample sequence of later frames: nonzero the exact one
+ * (RV_RENDER_EXACT), 0 the fast one (rv_frame_desc.exact at creation).
+ * Frames that record the adjoint's tape use the exact one either way. */
+int rv_frame_set_exact(rv_frame_plan* plan, int exact);
 int rv_frame_get_split(const rv_frame_plan* plan, int* split);
 /* Pick the bins' wave footprint (8x8, 16x4, 4x16, 32x2), the persistent
  * grid (resident workgroups per CU) of the mode's kernel (interpolate or

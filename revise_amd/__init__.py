@@ -129,6 +129,10 @@ _SIGS = {
                                   C.c_float, _f32p, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32,
                                   C.c_uint32, _vp, _vp]),
     "rv_share_workspace_bytes": (C.c_uint64, [C.c_uint32]),
+    "rv_probe_valu": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), _vp]),
+    "rv_probe_lds": (C.c_int, [_vp, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64), _vp]),
+    "rv_probe_gather": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint64), _vp]),
+    "rv_probe_stream": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint32, _vp]),
     "rv_render_share_argb_noalloc": (C.c_int, [C.POINTER(BlockParams), C.POINTER(C.c_void_p), C.c_uint32, _vp,
                                                C.c_uint32, _vp, C.c_uint32, C.c_uint32, _vp]),
     "rv_brick_floats": (C.c_uint64, [C.c_uint32, C.c_uint32]),
@@ -170,6 +174,7 @@ _SIGS = {
     "rv_frame_set_schedule": (C.c_int, [_vp, C.c_uint32]),
     "rv_frame_get_schedule": (C.c_int, [_vp, _u32p]),
     "rv_frame_set_hybrid_min": (C.c_int, [_vp, C.c_uint32]),
+    "rv_frame_set_exact": (C.c_int, [_vp, C.c_int]),
     "rv_frame_set_split": (C.c_int, [_vp, C.c_int]),
     "rv_frame_get_split": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "rv_frame_autotune": (C.c_int, [_vp, _vp, C.c_int, C.c_char_p, _vp, _u32p, _u32p]),

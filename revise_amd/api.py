@@ -437,6 +437,9 @@ class FramePlan:
         check(lib().rv_frame_get_schedule(self.h, C.byref(v)))
         return v.value
 
+    def set_exact(self, exact):
+        check(lib().rv_frame_set_exact(self.h, 1 if exact else 0))
+
     def set_split(self, split):
         """Interpolating frames in two launches (bricks, then the resident
         kernel) instead of one; the same images."""
@@ -709,3 +712,49 @@ class Dataset:
 
 
 __all__ = [n for n in dir() if not n.startswith("_")] + ["RvError"]
+
+
+def probe_peaks(torch, cus=None, reps=5):
+    """Measured peaks of the chip (rv_probe_*), each the median of `reps`
+    timed launches after a warm-up, on torch's current stream:
+    valu_fma_s (non-packed fp32 lane fmas/s), lds_bytes_s (ds_read_b128),
+    gather_coalesced_s and gather_scattered_s (wave dword-load instructions/s
+    from L1, 4 and 64 sectors per instruction), stream_bytes_s (HBM bytes
+    read + written per second, 1 GiB copy)."""
+    import statistics
+    if cus is None:
+        cus = torch.cuda.get_device_properties(0).multi_processor_count
+    blocks = cus * 16
+    sink = torch.zeros(blocks * 256, dtype=torch.float32, device="cuda")
+    src = torch.ones(4096, dtype=torch.float32, device="cuda")
+    big_a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    big_b = torch.empty_like(big_a)
+    s = stream_ptr()
+    out = {}
+    cnt = C.c_uint64()
+
+    def timed(launch):
+        launch()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            launch()
+            e1.record()
+            e1.synchronize()
+            ts.append(e0.elapsed_time(e1) / 1e3)
+        return statistics.median(ts)
+
+    t = timed(lambda: check(lib().rv_probe_valu(ptr(sink), blocks, 4096, C.byref(cnt), s)))
+    out["valu_fma_s"] = cnt.value / t
+    t = timed(lambda: check(lib().rv_probe_lds(ptr(sink), blocks, 4096, C.byref(cnt), s)))
+    out["lds_bytes_s"] = cnt.value / t
+    for name, sc in (("gather_coalesced_s", 0), ("gather_scattered_s", 1)):
+        t = timed(lambda: check(lib().rv_probe_gather(ptr(src), ptr(sink), blocks, 2048, sc, C.byref(cnt), s)))
+        out[name] = cnt.value / t
+    nbytes = big_a.numel() * 4
+    t = timed(lambda: check(lib().rv_probe_stream(ptr(big_a), ptr(big_b), nbytes, cus * 8, s)))
+    out["stream_bytes_s"] = 2 * nbytes / t
+    out["cus"] = cus
+    del big_a, big_b
+    return out

@@ -226,3 +226,92 @@ def large_share(torch, seed, level=8, depth=4, occupancy=0.12, n_gauss=32, octav
     w.fmin, w.fmax = np.float32(lo), np.float32(hi)
     w.sparse_nodes, w.dense_nodes = nsel * N, nsel * V
     return w
+
+
+class RefinedTree:
+    """The block tree of the CPU interpolation configuration: an octree over
+    the unit cube refined to level `octree_levels` wherever |grad f| exceeds
+    `frac` of its maximum, each octree leaf carrying `block_depth` further
+    full levels (a leaf "block" of 16^3 cells for block_depth 4).
+
+    leaf_level[i, j, k]: the octree-leaf level containing level-
+    `octree_levels` cell (i, j, k).  A block of the tree at level a is
+    refined when a < octree_levels (it lies inside a leaf's cells or is an
+    internal octree node) or when a < L + block_depth, L the level of the
+    octree leaf containing it."""
+
+    def __init__(self, field, octree_levels=4, block_depth=4, frac=0.5, samples=5, grid=129):
+        self.field = field
+        self.octree_levels = octree_levels
+        self.block_depth = block_depth
+        g = np.linspace(0.0, 1.0, grid)
+        u = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+        self.gmax = float(field.grad_norm(u).max())
+        thr = frac * self.gmax
+        m = 2 ** octree_levels
+        leaf = np.full((m, m, m), -1, dtype=np.int32)
+        self.leaves = []
+        s = (np.arange(samples) + 0.5) / samples
+        loc_s = np.stack(np.meshgrid(s, s, s, indexing="ij"), -1).reshape(-1, 3)
+
+        def visit(level, idx):
+            size = 1.0 / 2 ** level
+            refine = False
+            if level < octree_levels:
+                pts = (np.asarray(idx, dtype=np.float64)[None, :] + loc_s) * size
+                refine = bool((field.grad_norm(pts) > thr).any())
+            if refine:
+                for c in range(8):
+                    visit(level + 1, (2 * idx[0] + (c & 1), 2 * idx[1] + ((c >> 1) & 1),
+                                      2 * idx[2] + ((c >> 2) & 1)))
+                return
+            self.leaves.append((level, tuple(idx)))
+            sh = octree_levels - level
+            x, y, z = (v << sh for v in idx)
+            leaf[x:x + (1 << sh), y:y + (1 << sh), z:z + (1 << sh)] = level
+
+        visit(0, (0, 0, 0))
+        assert (leaf >= 0).all()
+        self.leaf_level = leaf
+
+    def subtree(self, level, index, max_depth=6):
+        """Children table of the subtree rooted at block (level, index),
+        walked to max_depth levels below it."""
+        ol, bd, leaf = self.octree_levels, self.block_depth, self.leaf_level
+        index = np.asarray(index, dtype=np.int64)
+
+        def refine(l, locs):
+            a = level + l
+            if a < ol:
+                return np.ones(locs.shape[0], dtype=bool)
+            loc = index[None, :] * 2 ** l + locs
+            c = loc >> (a - ol)
+            return a < leaf[c[:, 0], c[:, 1], c[:, 2]] + bd
+
+        return datasets.refined_subtree(max_depth, refine)
+
+
+def cpu_interp_brick(seed=2024, root_level=2, max_depth=6, octree_levels=4, block_depth=4, frac=0.5):
+    """The CPU interpolation workload: the 3-Gaussian field's refined tree
+    and one level-`root_level` subtree of depth `max_depth` (65^3 nodes for
+    depth 6): among the roots whose subtree reaches that depth, the one with
+    the median node count.  Returns a dict: tree, level, index, depth,
+    children, n2i, sparse (fp32 values of f at the nodes)."""
+    field = datasets.Field(seed, n_gauss=3)
+    tree = RefinedTree(field, octree_levels, block_depth, frac)
+    m = 2 ** root_level
+    cands = []
+    for z in range(m):
+        for y in range(m):
+            for x in range(m):
+                ch = tree.subtree(root_level, (x, y, z), max_depth)
+                d = api.subtree_depth(ch, max_depth)
+                if d == max_depth:
+                    cands.append(((x, y, z), ch, d))
+    assert cands, "no level-%d subtree reaches depth %d" % (root_level, max_depth)
+    sized = sorted((api.subtree_n2i(ch, d).shape[0], idx, ch, d) for idx, ch, d in cands)
+    n, idx, ch, d = sized[len(sized) // 2]
+    n2i = api.subtree_n2i(ch, d)
+    sparse = datasets.sample(field, root_level, idx, d, n2i)
+    return {"tree": tree, "level": root_level, "index": idx, "depth": d, "children": ch, "n2i": n2i,
+            "sparse": sparse, "candidates": len(cands)}

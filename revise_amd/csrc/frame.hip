@@ -339,11 +339,6 @@ constexpr uint32_t kShareT = 256;
 #define RV_SHARE_WGS_PER_CU 4
 #endif
 constexpr uint32_t kShareWgsPerCu = RV_SHARE_WGS_PER_CU;
-// floats of a wave's LDS brick window (fast Argb; march_fast_window)
-#ifndef RV_SHARE_WIN
-#define RV_SHARE_WIN 0
-#endif
-constexpr uint32_t kShareWin = RV_SHARE_WIN;
 
 template <uint32_t kMode, class BrickT>
 __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* __restrict__ params,
@@ -356,7 +351,7 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         uint32_t* __restrict__ viewport,
                                                         uint32_t W, uint32_t H)
 {
-    extern __shared__ float4 tfs_share[]; // tf_n entries, then (fast Argb) the waves' brick windows
+    extern __shared__ float4 tfs_share[]; // tf_n entries (and padding)
     __shared__ float lvs[kMaxLevels];
     for (uint32_t i = threadIdx.x; i < tf_n; i += kShareT)
         tfs_share[i] = tf[i];
@@ -372,7 +367,6 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
     const bool inside = vx < W && vy < H;
     const uint32_t pix = inside ? (H - 1u - vy) * W + vx : 0u;
     uint32_t value = inside ? viewport[pix] : 0u;
-    float* win = reinterpret_cast<float*>(tfs_share + tf_n) + wave * kShareWin;
     for (uint32_t c = 0; c < nblocks; c += 64u) {
         const uint32_t i = c + lane;
         bool hit = false;
@@ -387,19 +381,6 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
             const rv_block_params p = params[blk];
             const bool inrect = inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
                                 vy < p.view_y + p.view_h;
-            if constexpr (kMode == kModeArgbFast && kShareWin > 0u) {
-                // the whole wave marches the block with its brick staged in
-                // the wave's LDS window
-                const RaySeg r = setup_ray(p, vx, vy);
-                const bool act = inrect && r.hit;
-                if (__ballot(act) != 0ull) {
-                    const float4 seg = march_fast_window(p, reinterpret_cast<const BrickT*>(bricks[blk]),
-                                                         tfs_share, (int)tf_n, r, act, win, (int)kShareWin);
-                    if (act)
-                        value = blend_above_u32(value, seg);
-                }
-                continue;
-            }
             if (inrect) {
                 const RaySeg r = setup_ray(p, vx, vy);
                 if (r.hit) {
@@ -1107,6 +1088,16 @@ extern "C" int rv_frame_set_split(rv_frame_plan* P, int split)
     return RV_OK;
 }
 
+extern "C" int rv_frame_set_exact(rv_frame_plan* P, int exact)
+{
+    RV_REQUIRE(P, "rv_frame_set_exact: null plan");
+    if ((exact != 0) != (P->desc.exact != 0u)) {
+        drop_graph(P);
+        P->desc.exact = exact != 0 ? 1u : 0u;
+    }
+    return RV_OK;
+}
+
 extern "C" int rv_frame_get_split(const rv_frame_plan* P, int* split)
 {
     RV_REQUIRE(P && split, "rv_frame_get_split: null pointer");
@@ -1474,8 +1465,7 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
 {
     // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
     // LDS per workgroup at most)
-    const bool win = field_mode == RV_FIELD_ARGB && !exact && kShareWin > 0u; // the fast Argb march stages bricks in LDS
-    const size_t lds = std::max<size_t>(sizeof(float4) * tf_n + (win ? 4u * (kShareT / 64u) * kShareWin : 0u),
+    const size_t lds = std::max<size_t>(sizeof(float4) * tf_n,
                                         std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u));
     const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
     const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);
@@ -1515,14 +1505,14 @@ extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
     return (uint64_t)nblocks * (sizeof(rv_block_params) + sizeof(uint4) + 2 * sizeof(void*)) + 256;
 }
 
-extern "C" int rv_render_share_layout(const rv_block_params* params, const float* const* bricks,
-                                      const float* const* bricks2, uint32_t nblocks, uint32_t layout,
-                                      uint32_t field_mode, float iso_opacity, const float* levels_rel,
-                                      uint32_t level_count, float smin, float smax, const float* d_tf,
-                                      uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
-                                      void* d_workspace, rv_stream_t stream)
+// The share's block table: params, view rectangles {x0, y0, x1, y1}, brick
+// and secondary-brick pointers, laid out as rv_share_workspace_bytes says,
+// written to `host` after the arguments are checked.
+static int share_table(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
+                       uint32_t nblocks, uint32_t layout, uint32_t field_mode, const float* levels_rel,
+                       uint32_t level_count, float smin, float smax, const float* d_tf, uint32_t tf_n,
+                       const uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, uint8_t* host)
 {
-    RV_TRY
     RV_REQUIRE(layout == RV_BRICK_LINEAR || layout == RV_BRICK_TILED4, "rv_render_share: unknown layout");
     RV_REQUIRE(field_mode <= RV_FIELD_VALUE_ON_ISOSURFACES, "rv_render_share: unknown field mode");
     const bool value = field_mode == RV_FIELD_VALUE_ON_ISOSURFACE ||
@@ -1531,15 +1521,11 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
                "rv_render_share: the ValueOnIsosurface(s) modes need the secondary field's bricks "
                "and a non-empty range");
     RV_REQUIRE(levels_rel || level_count == 0, "rv_render_share: null levels");
-    if (nblocks == 0)
-        return RV_OK;
-    RV_REQUIRE(params && bricks && d_tf && d_viewport && d_workspace,
-               "rv_render_share: null pointer");
+    RV_REQUIRE(params && bricks && d_tf && d_viewport, "rv_render_share: null pointer");
     RV_REQUIRE(tf_n >= 1 && tf_n <= kMaxTf, "rv_render_share: tf_n must be in [1, 1024]");
-    std::vector<uint8_t> host(rv_share_workspace_bytes(nblocks));
-    auto* hp = reinterpret_cast<rv_block_params*>(host.data());
-    auto* hr = reinterpret_cast<uint4*>(host.data() + sizeof(rv_block_params) * nblocks);
-    auto* hb = reinterpret_cast<const void**>(host.data() + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    auto* hp = reinterpret_cast<rv_block_params*>(host);
+    auto* hr = reinterpret_cast<uint4*>(host + sizeof(rv_block_params) * nblocks);
+    auto* hb = reinterpret_cast<const void**>(host + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
     for (uint32_t i = 0; i < nblocks; ++i) {
         const rv_block_params& p = params[i];
         RV_REQUIRE(p.vp_w == vp_w && p.vp_h == vp_h, "rv_render_share: viewport mismatch");
@@ -1559,6 +1545,59 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
         hb[i] = bricks[i];
         hb[nblocks + i] = value ? bricks2[i] : nullptr;
     }
+    return RV_OK;
+}
+
+// The share launch over a table staged in `host`, copied to d_workspace in
+// stream order; no synchronisation: the caller keeps `host` (pinned memory,
+// so the copy is asynchronous) unchanged until the stream has passed the
+// copy.
+int rv::render_share_staged(const rv_block_params* params, const float* const* bricks,
+                            const float* const* bricks2, uint32_t nblocks, uint32_t layout, uint32_t field_mode,
+                            float iso_opacity, const float* levels_rel, uint32_t level_count, float smin,
+                            float smax, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w,
+                            uint32_t vp_h, void* host_pinned, void* d_workspace, rv_stream_t stream)
+{
+    RV_TRY
+    if (nblocks == 0)
+        return RV_OK;
+    RV_REQUIRE(host_pinned && d_workspace, "rv_render_share: null staging");
+    uint8_t* host = static_cast<uint8_t*>(host_pinned);
+    int rc = share_table(params, bricks, bricks2, nblocks, layout, field_mode, levels_rel, level_count, smin, smax,
+                         d_tf, tf_n, d_viewport, vp_w, vp_h, host);
+    if (rc)
+        return rc;
+    uint8_t* d = static_cast<uint8_t*>(d_workspace);
+    RV_HIP(hipMemcpyAsync(d, host, rv_share_workspace_bytes(nblocks) - 256, hipMemcpyHostToDevice, S(stream)));
+    const auto* dp = reinterpret_cast<const rv_block_params*>(d);
+    const auto* dr = reinterpret_cast<const uint4*>(d + sizeof(rv_block_params) * nblocks);
+    const auto* db =
+        reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
+    return launch_share(dp, dr, db, db + nblocks, nblocks, layout, field_mode,
+                        (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
+                        d_viewport, vp_w, vp_h, stream);
+    RV_CATCH
+}
+
+extern "C" int rv_render_share_layout(const rv_block_params* params, const float* const* bricks,
+                                      const float* const* bricks2, uint32_t nblocks, uint32_t layout,
+                                      uint32_t field_mode, float iso_opacity, const float* levels_rel,
+                                      uint32_t level_count, float smin, float smax, const float* d_tf,
+                                      uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
+                                      void* d_workspace, rv_stream_t stream)
+{
+    RV_TRY
+    RV_REQUIRE(layout == RV_BRICK_LINEAR || layout == RV_BRICK_TILED4, "rv_render_share: unknown layout");
+    RV_REQUIRE(field_mode <= RV_FIELD_VALUE_ON_ISOSURFACES, "rv_render_share: unknown field mode");
+    if (nblocks == 0)
+        return RV_OK;
+    RV_REQUIRE(d_workspace, "rv_render_share: null pointer");
+    std::vector<uint8_t> host(rv_share_workspace_bytes(nblocks));
+    int rc = share_table(params, bricks, bricks2, nblocks, layout, field_mode, levels_rel, level_count, smin, smax,
+                         d_tf, tf_n, d_viewport, vp_w, vp_h, host.data());
+    if (rc)
+        return rc;
     uint8_t* d = static_cast<uint8_t*>(d_workspace);
     RV_HIP(hipMemcpyAsync(d, host.data(), host.size() - 256, hipMemcpyHostToDevice, S(stream)));
     // the host staging buffer must outlive the copy

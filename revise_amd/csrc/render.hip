@@ -412,6 +412,38 @@ extern "C" int rv_block_sample_count(const rv_block_params* params, uint64_t* d_
     return RV_OK;
 }
 
+// The crop rectangle through a caller's 16-byte device buffer (the worker
+// keeps one; rv_bound_rect allocates it per call).
+int rv::bound_rect_with(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, int32_t rect[4],
+                        int* nonempty, hipStream_t s)
+{
+    *nonempty = 0;
+    rect[0] = rect[1] = rect[2] = rect[3] = 0;
+    if ((uint64_t)w * h == 0)
+        return RV_OK;
+    int host[4];
+    hipLaunchKernelGGL(init_rect_kernel, dim3(1), dim3(64), 0, s, d_buf);
+    uint64_t blocks = ((uint64_t)w * h + 255) / 256;
+    if (blocks > 4096)
+        blocks = 4096;
+    hipLaunchKernelGGL(bound_rect_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, d_pixels, w, h, d_buf);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(host, d_buf, 16, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    if (e != hipSuccess)
+        return hip_fail(e, "rv_bound_rect", __FILE__, __LINE__);
+    if (host[2] < 0)
+        return RV_OK;
+    *nonempty = 1;
+    rect[0] = host[0];
+    rect[1] = host[1];
+    rect[2] = host[2] - host[0] + 1;
+    rect[3] = host[3] - host[1] + 1;
+    return RV_OK;
+}
+
 extern "C" int rv_bound_rect(const uint32_t* d_pixels, uint32_t w, uint32_t h, int32_t rect[4],
                              int* nonempty, rv_stream_t stream)
 {
@@ -422,30 +454,12 @@ extern "C" int rv_bound_rect(const uint32_t* d_pixels, uint32_t w, uint32_t h, i
         return RV_OK;
     int* d_out = nullptr;
     RV_HIP(hipMallocAsync((void**)&d_out, 16, S(stream)));
-    int host[4];
-    hipLaunchKernelGGL(init_rect_kernel, dim3(1), dim3(64), 0, S(stream), d_out);
-    uint64_t blocks = ((uint64_t)w * h + 255) / 256;
-    if (blocks > 4096)
-        blocks = 4096;
-    hipLaunchKernelGGL(bound_rect_kernel, dim3((uint32_t)blocks), dim3(256), 0, S(stream), d_pixels,
-                       w, h, d_out);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(host, d_out, 16, hipMemcpyDeviceToHost, S(stream));
-    hipError_t e2 = hipFreeAsync(d_out, S(stream));
-    if (e == hipSuccess)
-        e = e2;
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(S(stream));
+    const int rc = bound_rect_with(d_pixels, w, h, d_out, rect, nonempty, S(stream));
+    const hipError_t e = hipFreeAsync(d_out, S(stream));
+    if (rc)
+        return rc;
     if (e != hipSuccess)
-        return hip_fail(e, "rv_bound_rect", __FILE__, __LINE__);
-    if (host[2] < 0)
-        return RV_OK;
-    *nonempty = 1;
-    rect[0] = host[0];
-    rect[1] = host[1];
-    rect[2] = host[2] - host[0] + 1;
-    rect[3] = host[3] - host[1] + 1;
+        return hip_fail(e, "hipFreeAsync", __FILE__, __LINE__);
     return RV_OK;
 }
 

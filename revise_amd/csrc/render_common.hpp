@@ -610,210 +610,6 @@ __device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* 
     return make_float4(cr * ia, cg * ia, cb * ia, A);
 }
 
-// ---------------------------------------------------------------- LDS brick windows
-// The fast march of a whole wave through one block with the brick staged in
-// the wave's LDS window, round by round.  In each round every live lane
-// marches its next K samples (K <= kWinRound); the window holds the nodes
-// those samples can read -- the box spanned by the taps of each lane's first
-// and K-th next sample, widened by one node -- copied by the wave's lanes (x
-// fastest, coalesced global reads).  The samples then read their 8 corners
-// from LDS (4 ds_read2) instead of 8 global loads each.  A round whose box
-// does not fit has K halved; one that does not fit at K = 1, and any sample
-// whose corners fall outside the window, reads the brick directly.  All
-// lanes advance together, whatever face their rays entered the block by.
-// The samples, corners and operations are those of march_fast, so the
-// results are identical to it, bit for bit.  All 64 lanes must call it
-// together (it uses cross-lane reductions); `active` lanes march r.
-#ifndef RV_WIN_ROUND
-#define RV_WIN_ROUND 16
-#endif
-constexpr int kWinRound = RV_WIN_ROUND; // samples per lane and round (at most)
-
-__device__ __forceinline__ int wave_min_i(int v)
-{
-    for (int o = 32; o > 0; o >>= 1)
-        v = min(v, __shfl_xor(v, o));
-    return v;
-}
-
-__device__ __forceinline__ int wave_max_i(int v)
-{
-    for (int o = 32; o > 0; o >>= 1)
-        v = max(v, __shfl_xor(v, o));
-    return v;
-}
-
-// Orders one wave's LDS accesses across its lanes (the wave's LDS operations
-// execute in program order; this keeps the compiler from moving them).
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
-// One node of a brick (the window copy).
-__device__ __forceinline__ float node_value(const float* __restrict__ b, uint32_t n, uint32_t x, uint32_t y,
-                                            uint32_t z)
-{
-    return *reinterpret_cast<gptr<float>>(global_of(reinterpret_cast<const char*>(b)) +
-                                          ((__umul24(__umul24(z, n) + y, n) + x) << 2));
-}
-
-__device__ __forceinline__ float node_value(const __half* __restrict__ b, uint32_t n, uint32_t x, uint32_t y,
-                                            uint32_t z)
-{
-    const uint16_t h = *reinterpret_cast<gptr<uint16_t>>(global_of(reinterpret_cast<const char*>(b)) +
-                                                         ((__umul24(__umul24(z, n) + y, n) + x) << 1));
-    return __half2float(__ushort_as_half(h));
-}
-
-__device__ __forceinline__ float node_value(const Tiled4* __restrict__ b, uint32_t n, uint32_t x, uint32_t y,
-                                            uint32_t z)
-{
-    const uint32_t G = (n + 3u) >> 2;
-    const uint32_t o = (((__umul24(z >> 2, G) + (y >> 2)) * G + (x >> 2)) << 6) | ((z & 3u) << 4) |
-                       ((y & 3u) << 2) | (x & 3u);
-    return *reinterpret_cast<gptr<float>>(global_of(reinterpret_cast<const char*>(b)) + (o << 2));
-}
-
-struct WinBox {
-    int o[3]; // first node per axis
-    int e[3]; // nodes per axis
-    bool ok;  // the window holds the slab
-};
-
-template <class T>
-__device__ float4 march_fast_window(const rv_block_params& p, const T* __restrict__ brick, const float4* tf,
-                                    int tf_n, RaySeg r, bool active, float* __restrict__ win, int win_floats)
-{
-    const int n = (int)p.nodes;
-    const int lane = (int)(threadIdx.x & 63u);
-    const float ta = (float)tf_n / (p.fmax - p.fmin);
-    const float tb = fmaf(-p.fmin, ta, -0.5f);
-    const float thi = (float)(tf_n - 1);
-    const float k = p.i_layer_count, of = p.opacity_factor, cut = p.alpha_cutoff;
-    float cr = 0.f, cg = 0.f, cb = 0.f, A = 0.f;
-    int i = 1; // next sample
-    bool live = active && r.sample_count > 1;
-    WinBox w;
-    int K = kWinRound;
-    while (__ballot(live) != 0ull) {
-        // the round's window: the nodes of every live lane's next K samples,
-        // from the taps of its first and its K-th next sample (taps move
-        // monotonically along a ray), widened by one node for the fp32
-        // accumulation of positions; K is halved until the window fits
-        for (;;) {
-            int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
-            if (live) {
-                const float s = (float)(min(K, r.sample_count - i) - 1);
-                const float t0[3] = {(r.px - p.tex_origin[0]) * p.tcf, (r.py - p.tex_origin[1]) * p.tcf,
-                                     (r.pz - p.tex_origin[2]) * p.tcf};
-                const float st[3] = {r.sx * p.tcf, r.sy * p.tcf, r.sz * p.tcf};
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const float t1 = fmaf(s, st[a], t0[a]);
-                    uint32_t i00, i01, i10, i11;
-                    float fd;
-                    fast_axis(fminf(t0[a], t1), n, i00, i01, fd);
-                    fast_axis(fmaxf(t0[a], t1), n, i10, i11, fd);
-                    lo[a] = max((int)i00 - 1, 0);
-                    hi[a] = min((int)i11 + 1, n - 1);
-                }
-            }
-            int vol = 1;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                w.o[a] = wave_min_i(lo[a]);
-                w.e[a] = wave_max_i(hi[a]) - w.o[a] + 1;
-                vol *= w.e[a];
-            }
-            // one float of slack: the x-pair read of the last node
-            w.ok = vol < win_floats;
-            if (w.ok || K == 1)
-                break;
-            K >>= 1;
-        }
-        if (w.ok) {
-            // copy the window, x fastest; the (row, plane) split by fp32
-            // reciprocals is exact for windows below 2^22 nodes
-            const int ex = w.e[0], exy = w.e[0] * w.e[1], size = exy * w.e[2];
-            const float iex = 1.f / (float)ex, iexy = 1.f / (float)exy;
-            for (int j = lane; j < size; j += 64) {
-                const int lz = (int)(((float)j + 0.5f) * iexy);
-                const int rxy = j - lz * exy;
-                const int ly = (int)(((float)rxy + 0.5f) * iex);
-                const int lx = rxy - ly * ex;
-                win[j] = node_value(brick, (uint32_t)n, (uint32_t)(w.o[0] + lx), (uint32_t)(w.o[1] + ly),
-                                    (uint32_t)(w.o[2] + lz));
-            }
-            wave_lds_sync();
-        }
-        // the round's samples of every live lane
-        const int ex = w.e[0], exy = w.e[0] * w.e[1];
-        for (int q = 0; q < K && live; ++q) {
-            const float tx = (r.px - p.tex_origin[0]) * p.tcf;
-            const float ty = (r.py - p.tex_origin[1]) * p.tcf;
-            const float tz = (r.pz - p.tex_origin[2]) * p.tcf;
-            uint32_t x0, x1, y0, y1, z0, z1;
-            float fx, fy, fz;
-            fast_axis(tx, n, x0, x1, fx);
-            fast_axis(ty, n, y0, y1, fy);
-            fast_axis(tz, n, z0, z1, fz);
-            float c[8];
-            const uint32_t bx = x0 - (uint32_t)w.o[0], by = y0 - (uint32_t)w.o[1], bz = z0 - (uint32_t)w.o[2];
-            const bool inwin = w.ok && bx + (x1 - x0) < (uint32_t)w.e[0] && by + (y1 - y0) < (uint32_t)w.e[1] &&
-                               bz + (z1 - z0) < (uint32_t)w.e[2];
-            if (inwin) {
-                const int base = ((int)bz * w.e[1] + (int)by) * ex + (int)bx;
-                const int dyw = (int)(y1 - y0) * ex, dzw = (int)(z1 - z0) * exy;
-                const int off[4] = {base, base + dyw, base + dzw, base + dzw + dyw};
-                const bool l1 = x1 == x0;
-#pragma unroll
-                for (int m = 0; m < 4; ++m) {
-                    const float a = win[off[m]], b = win[off[m] + 1];
-                    c[2 * m] = a;
-                    c[2 * m + 1] = l1 ? a : b;
-                }
-            } else {
-                float fx2, fy2, fz2;
-                fetch_fast(brick, n, tx, ty, tz, c, fx2, fy2, fz2);
-            }
-            r.px = r.px + r.sx;
-            r.py = r.py + r.sy;
-            r.pz = r.pz + r.sz;
-            const float c00 = lerp_fast(c[0], c[1], fx), c10 = lerp_fast(c[2], c[3], fx);
-            const float c01 = lerp_fast(c[4], c[5], fx), c11 = lerp_fast(c[6], c[7], fx);
-            const float v = lerp_fast(lerp_fast(c00, c10, fy), lerp_fast(c01, c11, fy), fz);
-            ++i;
-            if (v == v) {
-                const float x = __builtin_amdgcn_fmed3f(fmaf(v, ta, tb), 0.f, thi);
-                const float fl = floorf(x);
-                const float f = x - fl;
-                const int i0 = (int)fl;
-                const int i1 = i0 + 1 < tf_n ? i0 + 1 : i0;
-                const float4 ca = tf[i0], cb2 = tf[i1];
-                const float al =
-                    1.f - __builtin_amdgcn_exp2f(k * __builtin_amdgcn_logf(fmaf(-lerp_fast(ca.w, cb2.w, f), of, 1.f)));
-                const float wgt = fmaf(-A, al, al);
-                cr = fmaf(wgt, lerp_fast(ca.x, cb2.x, f), cr);
-                cg = fmaf(wgt, lerp_fast(ca.y, cb2.y, f), cg);
-                cb = fmaf(wgt, lerp_fast(ca.z, cb2.z, f), cb);
-                A = A + wgt;
-                if (A > cut)
-                    live = false;
-            }
-            if (i >= r.sample_count)
-                live = false;
-        }
-        wave_lds_sync(); // the window is rewritten for the next round
-        K = min(2 * K, kWinRound);
-    }
-    if (!(A > 0.f))
-        return make_float4(0.f, 0.f, 0.f, 0.f);
-    const float ia = 1.f / A;
-    return make_float4(cr * ia, cg * ia, cb * ia, A);
-}
-
 // ---------------------------------------------------------------- field modes
 // s3vs::FieldMode (FieldMode.hpp:26-51), the same numbering.
 constexpr uint32_t kModeIsosurface = 0, kModeIsosurfaces = 1, kModeMip = 2, kModeArgb = 3,

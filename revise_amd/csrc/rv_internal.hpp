@@ -32,6 +32,20 @@ inline int hip_fail(hipError_t e, const char* what, const char* file, int line)
 
 inline hipStream_t S(rv_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
+// getBoundRect of a device ARGB image through d_buf (16 bytes of device
+// memory); synchronous.  render.hip.
+int bound_rect_with(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, int32_t rect[4],
+                    int* nonempty, hipStream_t s);
+
+// rv_render_share_layout with the block table staged in the caller's pinned
+// host buffer (rv_share_workspace_bytes(nblocks) bytes) and copied in
+// stream order: no synchronisation.  frame.hip.
+int render_share_staged(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
+                        uint32_t nblocks, uint32_t layout, uint32_t field_mode, float iso_opacity,
+                        const float* levels_rel, uint32_t level_count, float smin, float smax, const float* d_tf,
+                        uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, void* host_pinned,
+                        void* d_workspace, rv_stream_t stream);
+
 } // namespace rv
 
 #define RV_HIP(call)                                                                   \

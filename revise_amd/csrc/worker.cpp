@@ -13,6 +13,13 @@
 
 namespace rv {
 void set_error(const std::string& msg);
+int bound_rect_with(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, int32_t rect[4],
+                    int* nonempty, hipStream_t s);
+int render_share_staged(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
+                        uint32_t nblocks, uint32_t layout, uint32_t field_mode, float iso_opacity,
+                        const float* levels_rel, uint32_t level_count, float smin, float smax, const float* d_tf,
+                        uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, void* host_pinned,
+                        void* d_workspace, rv_stream_t stream);
 }
 
 namespace revise {
@@ -370,6 +377,8 @@ struct VsWorkerHip::ShareCache {
     std::map<std::array<uint32_t, 8>, Batch> batches;
     void* workspace = nullptr;
     uint64_t workspace_bytes = 0;
+    void* staging = nullptr; // pinned host copy of the block table
+    hipEvent_t staged = nullptr; // recorded after the launch that reads `staging`
     void dropBatches()
     {
         for (auto& kv : batches)
@@ -402,6 +411,12 @@ struct VsWorkerHip::ShareCache {
             (void)hipFree(kv.second.p);
         if (workspace)
             (void)hipFree(workspace);
+        if (staged)
+            (void)hipEventSynchronize(staged);
+        if (staging)
+            (void)hipHostFree(staging);
+        if (staged)
+            (void)hipEventDestroy(staged);
     }
 };
 
@@ -409,6 +424,8 @@ VsWorkerHip::VsWorkerHip() = default;
 
 VsWorkerHip::~VsWorkerHip()
 {
+    if (m_rectBuf)
+        (void)hipFree(m_rectBuf);
     if (m_viewport)
         (void)hipFree(m_viewport);
     if (m_part)
@@ -498,7 +515,9 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
     // the device; only the part crosses to the host.
     int32_t rect[4];
     int nonempty = 0;
-    throwIfFailed(rv_bound_rect(m_viewport, W, H, rect, &nonempty, stream()));
+    if (!m_rectBuf)
+        hipCheck(hipMalloc((void**)&m_rectBuf, 16), "hipMalloc");
+    throwIfFailed(rv::bound_rect_with(m_viewport, W, H, m_rectBuf, rect, &nonempty, hs(stream())));
     if (!nonempty)
         return part;
     const uint64_t n = (uint64_t)rect[2] * rect[3];
@@ -610,12 +629,22 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         if (second)
             bricks2.push_back(sc.bricks2[bkey].p);
     }
+    if (!sc.staged)
+        hipCheck(hipEventCreateWithFlags(&sc.staged, hipEventDisableTiming), "hipEventCreate");
+    // the previous part's copy of the table has run (normally long since:
+    // every part ends with a stream synchronisation)
+    hipCheck(hipEventSynchronize(sc.staged), "hipEventSynchronize");
     const uint64_t need = rv_share_workspace_bytes((uint32_t)params.size());
     if (need > sc.workspace_bytes) {
         if (sc.workspace)
             (void)hipFree(sc.workspace);
         sc.workspace = nullptr;
+        if (sc.staging)
+            (void)hipHostFree(sc.staging);
+        sc.staging = nullptr;
+        sc.workspace_bytes = 0;
         hipCheck(hipMalloc(&sc.workspace, need), "hipMalloc");
+        hipCheck(hipHostMalloc(&sc.staging, need, hipHostMallocDefault), "hipHostMalloc");
         sc.workspace_bytes = need;
     }
     if (isCancelled()) {
@@ -623,12 +652,16 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         return;
     }
     const auto srange = fs.secondaryRange();
-    throwIfFailed(rv_render_share_layout(
+    // the table goes through the pinned staging buffer, asynchronously: every
+    // renderScenePart ends with a stream synchronisation, so the copy has
+    // run before the next part rewrites the buffer
+    throwIfFailed(rv::render_share_staged(
         params.data(), bricks.data(), second ? bricks2.data() : nullptr, (uint32_t)params.size(),
         RV_BRICK_TILED4, st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
         (uint32_t)st.isosurfaceLevels.size(), srange[0], srange[1], m_renderer.colorTable(),
-        m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1],
+        m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1], sc.staging,
         sc.workspace, s));
+    hipCheck(hipEventRecord(sc.staged, hs(s)), "hipEventRecord");
 }
 
 const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const
@@ -880,7 +913,12 @@ extern "C" int rv_worker_set_state(rv_worker* w, const rv_render_state* s)
     st.threshold = s->threshold_rel;
     st.alphaCutoff = s->alpha_cutoff;
     st.backgroundColor = {s->background[0], s->background[1], s->background[2]};
-    st.colorTable.assign(s->tf, s->tf + 4ull * s->tf_n);
+    // the device table is refreshed only when the table changed (a new
+    // camera alone uploads nothing)
+    const bool tf_changed = st.colorTable.size() != 4ull * s->tf_n ||
+                            !std::equal(st.colorTable.begin(), st.colorTable.end(), s->tf);
+    if (tf_changed)
+        st.colorTable.assign(s->tf, s->tf + 4ull * s->tf_n);
     st.fieldMode = s->field_mode;
     st.isosurfaceLevel = s->iso_level_rel;
     st.isosurfaceOpacity = s->iso_opacity;
@@ -898,7 +936,8 @@ extern "C" int rv_worker_set_state(rv_worker* w, const rv_render_state* s)
     w->field.setFieldRange(s->fmin, s->fmax);
     w->field.setSecondaryRange(s->smin, s->smax);
     w->worker.setRenderSharedState(&st);
-    w->worker.updateState(1u << 7);
+    if (tf_changed)
+        w->worker.updateState(1u << 7);
     return RV_OK;
     W_CATCH
 }

@@ -203,11 +203,11 @@ def test_fast_worker_fused_equals_per_block():
 
 
 @pytest.mark.parametrize("case", ["deep brick", "close camera", "camera inside"])
-def test_fast_share_windows_equal_direct_march(orc, case):
-    """The share kernel stages bricks in per-wave LDS windows (halved slabs,
-    direct reads where a window cannot hold a slab, no windows where rays run
-    both ways along the slab axis); the per-block kernel reads the brick
-    directly.  Same samples and operations: equal bit for bit."""
+def test_fast_share_equals_per_block_edge_cases(orc, case):
+    """The share kernel against per-block launches where the rays are
+    unusual: a deep brick (cells far below a pixel), a camera close to the
+    blocks, a camera inside the volume (rays in every direction).  Same
+    samples and operations: equal bit for bit."""
     import torch
     W, H = 200, 150
     if case == "deep brick":

@@ -238,3 +238,28 @@ def test_batch_tiled_layout_holds_the_linear_values(orc, seed, depth, p, pnf):
     pad = np.ones(t.size, bool)
     pad[idx.reshape(-1)] = False
     assert np.all(np.isnan(t[pad]))
+
+
+def test_cpu_interp_configuration_brick_bit_exact(orc):
+    """The level-2, depth-6 brick of the refined 3-Gaussian tree (the CPU
+    interpolation workload, 65^3 nodes, partly refined): rv_interp_brick
+    and the batched micro-brick path against the oracle, bit-exact."""
+    import torch
+    from revise_amd import configs
+    c = configs.cpu_interp_brick()
+    n2i, d = c["n2i"], c["depth"]
+    exp = orc.interpolate(c["sparse"], n2i, d)
+    sp = torch.from_numpy(c["sparse"]).cuda()
+    dense = torch.empty(api.brick_nodes(d), dtype=torch.float32, device="cuda")
+    api.interp_brick(sp, n2i, d, dense)
+    got = dense.cpu().numpy()
+    say("config-1 brick: %d sparse nodes, %d dense" % (n2i.shape[0], exp.size))
+    assert not np.isnan(exp).any()
+    assert np.array_equal(got.view(np.uint32), exp.view(np.uint32))
+    til = torch.empty(api.brick_floats(d, api.BRICK_TILED4), dtype=torch.float32, device="cuda")
+    api.InterpBatch([(sp, api.interp_plan(n2i, d), til, d)], api.BRICK_TILED4).run()
+    t = til.cpu().numpy()
+    n, G = 2 ** d + 1, (2 ** d + 4) // 4
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ti = ((((z >> 2) * G + (y >> 2)) * G + (x >> 2)) << 6) | ((z & 3) << 4) | ((y & 3) << 2) | (x & 3)
+    assert np.array_equal(t[ti.reshape(-1)].view(np.uint32), exp.view(np.uint32))
