@@ -62,20 +62,23 @@ __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
 }
 
 // Bounding rectangle of non-transparent pixels: per-workgroup reduction, then
-// 4 atomics into out = {minX, minY, maxX, maxY}.
+// 4 atomics into out = {minX, minY, maxX, maxY}.  A workgroup is 64 columns x
+// 4 rows; grid.x covers the width, and the workgroups of a column band step
+// through the rows (32-bit indices throughout).
 __global__ __launch_bounds__(256) void bound_rect_kernel(const uint32_t* __restrict__ pix,
                                                          uint32_t w, uint32_t h, int* out)
 {
-    __shared__ int red[4][8];
+    __shared__ int red[4][4];
     int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
-    const uint64_t total = (uint64_t)w * h;
-    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < total; i += (uint64_t)gridDim.x * 256u) {
-        if (pix[i] >> 24) {
-            const int y = (int)(i / w), x = (int)(i % w);
-            mnx = min(mnx, x);
-            mxx = max(mxx, x);
-            mny = min(mny, y);
-            mxy = max(mxy, y);
+    const uint32_t x = blockIdx.x * 64u + (threadIdx.x & 63u);
+    if (x < w) {
+        for (uint32_t y = blockIdx.y * 4u + (threadIdx.x >> 6); y < h; y += gridDim.y * 4u) {
+            if (pix[y * w + x] >> 24) {
+                mnx = min(mnx, (int)x);
+                mxx = max(mxx, (int)x);
+                mny = min(mny, (int)y);
+                mxy = max(mxy, (int)y);
+            }
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -106,6 +109,23 @@ __global__ __launch_bounds__(256) void bound_rect_kernel(const uint32_t* __restr
             atomicMax(out + 3, red[3][0]);
         }
     }
+}
+
+// The pixels of the rectangle the bound_rect_kernel left in rect[] (rows as
+// in the viewport), packed w x h into part: the crop of VsWorker_cu.cpp:
+// 279-306 without a host round trip.
+__global__ __launch_bounds__(256) void crop_kernel(const uint32_t* __restrict__ pix, uint32_t w, uint32_t h,
+                                                   const int* __restrict__ rect, uint32_t* __restrict__ part)
+{
+    const int x0 = rect[0], y0 = rect[1], x1 = rect[2], y1 = rect[3];
+    if (x1 < 0)
+        return; // empty
+    const int pw = x1 - x0 + 1;
+    const int x = x0 + (int)(blockIdx.x * 64u + (threadIdx.x & 63u));
+    if (x > x1)
+        return;
+    for (int y = y0 + (int)(blockIdx.y * 4u + (threadIdx.x >> 6)); y <= y1; y += (int)(gridDim.y * 4u))
+        part[(uint32_t)(y - y0) * (uint32_t)pw + (uint32_t)(x - x0)] = pix[(uint32_t)y * w + (uint32_t)x];
 }
 
 // S = sum over the view rectangle of (sampleCount - 1) for rays that hit
@@ -412,24 +432,43 @@ extern "C" int rv_block_sample_count(const rv_block_params* params, uint64_t* d_
     return RV_OK;
 }
 
-// The crop rectangle through a caller's 16-byte device buffer (the worker
-// keeps one; rv_bound_rect allocates it per call).
-int rv::bound_rect_with(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, int32_t rect[4],
-                        int* nonempty, hipStream_t s)
+// The crop rectangle into a caller's 16-byte device buffer, in stream
+// order: {minX, minY, maxX, maxY}, maxX < 0 when every pixel is transparent.
+int rv::bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, hipStream_t s)
+{
+    hipLaunchKernelGGL(init_rect_kernel, dim3(1), dim3(64), 0, s, d_buf);
+    const uint32_t gx = (w + 63u) / 64u;
+    uint32_t gy = (h + 3u) / 4u;
+    if (gy > 1024u)
+        gy = 1024u;
+    if (gx > 0u && gy > 0u)
+        hipLaunchKernelGGL(bound_rect_kernel, dim3(gx, gy), dim3(256), 0, s, d_pixels, w, h, d_buf);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
+
+// The rectangle's pixels packed into d_part (room for w * h), in stream
+// order after bound_rect_launch.
+int rv::crop_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, const int* d_buf, uint32_t* d_part,
+                    hipStream_t s)
+{
+    const uint32_t gx = (w + 63u) / 64u;
+    uint32_t gy = (h + 3u) / 4u;
+    if (gy > 256u)
+        gy = 256u;
+    if (gx > 0u && gy > 0u)
+        hipLaunchKernelGGL(crop_kernel, dim3(gx, gy), dim3(256), 0, s, d_pixels, w, h, d_buf, d_part);
+    RV_LAUNCH_CHECK();
+    return RV_OK;
+}
+
+// Reads the rectangle of bound_rect_launch (synchronous) as {x, y, w, h}.
+int rv::bound_rect_read(const int* d_buf, int32_t rect[4], int* nonempty, hipStream_t s)
 {
     *nonempty = 0;
     rect[0] = rect[1] = rect[2] = rect[3] = 0;
-    if ((uint64_t)w * h == 0)
-        return RV_OK;
     int host[4];
-    hipLaunchKernelGGL(init_rect_kernel, dim3(1), dim3(64), 0, s, d_buf);
-    uint64_t blocks = ((uint64_t)w * h + 255) / 256;
-    if (blocks > 4096)
-        blocks = 4096;
-    hipLaunchKernelGGL(bound_rect_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, d_pixels, w, h, d_buf);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(host, d_buf, 16, hipMemcpyDeviceToHost, s);
+    hipError_t e = hipMemcpyAsync(host, d_buf, 16, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess)
         e = hipStreamSynchronize(s);
     if (e != hipSuccess)
@@ -454,7 +493,9 @@ extern "C" int rv_bound_rect(const uint32_t* d_pixels, uint32_t w, uint32_t h, i
         return RV_OK;
     int* d_out = nullptr;
     RV_HIP(hipMallocAsync((void**)&d_out, 16, S(stream)));
-    const int rc = bound_rect_with(d_pixels, w, h, d_out, rect, nonempty, S(stream));
+    int rc = bound_rect_launch(d_pixels, w, h, d_out, S(stream));
+    if (!rc)
+        rc = bound_rect_read(d_out, rect, nonempty, S(stream));
     const hipError_t e = hipFreeAsync(d_out, S(stream));
     if (rc)
         return rc;

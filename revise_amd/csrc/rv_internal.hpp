@@ -32,10 +32,14 @@ inline int hip_fail(hipError_t e, const char* what, const char* file, int line)
 
 inline hipStream_t S(rv_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-// getBoundRect of a device ARGB image through d_buf (16 bytes of device
-// memory); synchronous.  render.hip.
-int bound_rect_with(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, int32_t rect[4],
-                    int* nonempty, hipStream_t s);
+// getBoundRect of a device ARGB image (render.hip): bound_rect_launch
+// leaves {minX, minY, maxX, maxY} in d_buf (16 bytes of device memory) in
+// stream order, crop_launch packs that rectangle's pixels into d_part,
+// bound_rect_read waits for the stream and returns {x, y, w, h}.
+int bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, hipStream_t s);
+int crop_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, const int* d_buf, uint32_t* d_part,
+                hipStream_t s);
+int bound_rect_read(const int* d_buf, int32_t rect[4], int* nonempty, hipStream_t s);
 
 // rv_render_share_layout with the block table staged in the caller's pinned
 // host buffer (rv_share_workspace_bytes(nblocks) bytes) and copied in

@@ -13,8 +13,10 @@
 
 namespace rv {
 void set_error(const std::string& msg);
-int bound_rect_with(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, int32_t rect[4],
-                    int* nonempty, hipStream_t s);
+int bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, hipStream_t s);
+int crop_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, const int* d_buf, uint32_t* d_part,
+                hipStream_t s);
+int bound_rect_read(const int* d_buf, int32_t rect[4], int* nonempty, hipStream_t s);
 int render_share_staged(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
                         uint32_t nblocks, uint32_t layout, uint32_t field_mode, float iso_opacity,
                         const float* levels_rel, uint32_t level_count, float smin, float smax, const float* d_tf,
@@ -512,26 +514,26 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
         return part;
     }
     // Crop to the non-transparent rectangle (VsWorker_cu.cpp:279-306), on
-    // the device; only the part crosses to the host.
+    // the device: the rectangle and the packed part are computed in stream
+    // order, then the rectangle is read and only the part crosses to the host.
     int32_t rect[4];
     int nonempty = 0;
     if (!m_rectBuf)
         hipCheck(hipMalloc((void**)&m_rectBuf, 16), "hipMalloc");
-    throwIfFailed(rv::bound_rect_with(m_viewport, W, H, m_rectBuf, rect, &nonempty, hs(stream())));
-    if (!nonempty)
-        return part;
-    const uint64_t n = (uint64_t)rect[2] * rect[3];
-    if (n > m_partCap) {
+    if (px > m_partCap) {
         if (m_part)
             (void)hipFree(m_part);
         m_part = nullptr;
-        hipCheck(hipMalloc((void**)&m_part, 4 * n), "hipMalloc");
-        m_partCap = n;
+        m_partCap = 0;
+        hipCheck(hipMalloc((void**)&m_part, 4 * px), "hipMalloc");
+        m_partCap = px;
     }
-    hipCheck(hipMemcpy2DAsync(m_part, 4 * (size_t)rect[2], m_viewport + (size_t)rect[1] * W + rect[0],
-                              4 * (size_t)W, 4 * (size_t)rect[2], rect[3], hipMemcpyDeviceToDevice,
-                              hs(stream())),
-             "hipMemcpy2DAsync");
+    throwIfFailed(rv::bound_rect_launch(m_viewport, W, H, m_rectBuf, hs(stream())));
+    throwIfFailed(rv::crop_launch(m_viewport, W, H, m_rectBuf, m_part, hs(stream())));
+    throwIfFailed(rv::bound_rect_read(m_rectBuf, rect, &nonempty, hs(stream())));
+    if (!nonempty)
+        return part;
+    const uint64_t n = (uint64_t)rect[2] * rect[3];
     part.origin = {rect[0], rect[1]};
     part.size = {rect[2], rect[3]};
     if (hostDst) {
