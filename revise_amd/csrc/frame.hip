@@ -300,8 +300,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         }
         __syncthreads();
     }
-    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kT)
-        tfs[i] = a.tf[i];
+    stage_tf(tfs, a.tf, a.tf_n, threadIdx.x, kT);
     if (multi_level(kMode) && threadIdx.x < a.lev.n)
         lvs[threadIdx.x] = a.lev.v[threadIdx.x];
     __syncthreads();
@@ -314,10 +313,9 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
 template <class BrickT, uint32_t kMode, bool kLanes>
 __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 {
-    extern __shared__ float4 tfs_dyn[]; // tf_n entries
+    extern __shared__ float4 tfs_dyn[]; // tf_n + 1 entries
     __shared__ float lvs[kMaxLevels];
-    for (uint32_t i = threadIdx.x; i < a.tf_n; i += kRT)
-        tfs_dyn[i] = a.tf[i];
+    stage_tf(tfs_dyn, a.tf, a.tf_n, threadIdx.x, kRT);
     if (multi_level(kMode) && threadIdx.x < a.lev.n)
         lvs[threadIdx.x] = a.lev.v[threadIdx.x];
     __syncthreads();
@@ -351,10 +349,9 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         uint32_t* __restrict__ viewport,
                                                         uint32_t W, uint32_t H)
 {
-    extern __shared__ float4 tfs_share[]; // tf_n entries (and padding)
+    extern __shared__ float4 tfs_share[]; // tf_n + 1 entries (and padding)
     __shared__ float lvs[kMaxLevels];
-    for (uint32_t i = threadIdx.x; i < tf_n; i += kShareT)
-        tfs_share[i] = tf[i];
+    stage_tf(tfs_share, tf, tf_n, threadIdx.x, kShareT);
     if (multi_level(kMode) && threadIdx.x < lev.n)
         lvs[threadIdx.x] = lev.v[threadIdx.x];
     __syncthreads();
@@ -556,7 +553,7 @@ void launch_sched(const rv_frame_plan* P, const FrameArgs& a, int interpolate, u
                   hipStream_t s)
 {
     if (!interpolate) {
-        const size_t lds = sizeof(float4) * a.tf_n;
+        const size_t lds = sizeof(float4) * (a.tf_n + 1u);
         if (P->fp16)
             hipLaunchKernelGGL((render_frame_kernel<__half, kMode, kLanes>), dim3(P->render_grid),
                                dim3(kRT), lds, s, a);
@@ -791,13 +788,13 @@ extern "C" int rv_frame_plan_create(const rv_frame_block* blocks, uint32_t nbloc
     P->grid = (uint32_t)(cus * per_cu);
     if (fp16)
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, render_frame_kernel<__half, kModeArgb, false>, kRT, sizeof(float4) * desc->tf_n));
+            &per_cu, render_frame_kernel<__half, kModeArgb, false>, kRT, sizeof(float4) * (desc->tf_n + 1u)));
     else if (P->tiled)
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, render_frame_kernel<Tiled4, kModeArgb, false>, kRT, sizeof(float4) * desc->tf_n));
+            &per_cu, render_frame_kernel<Tiled4, kModeArgb, false>, kRT, sizeof(float4) * (desc->tf_n + 1u)));
     else
         RV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, render_frame_kernel<float, kModeArgb, false>, kRT, sizeof(float4) * desc->tf_n));
+            &per_cu, render_frame_kernel<float, kModeArgb, false>, kRT, sizeof(float4) * (desc->tf_n + 1u)));
     RV_REQUIRE(per_cu >= 1, "rv_frame_plan_create: frame kernel cannot be resident");
     P->render_max_wgs = (uint32_t)per_cu;
     P->render_grid = (uint32_t)(cus * std::min(per_cu, 4)); // 16 waves per CU until tuned
@@ -1465,7 +1462,7 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
 {
     // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
     // LDS per workgroup at most)
-    const size_t lds = std::max<size_t>(sizeof(float4) * tf_n,
+    const size_t lds = std::max<size_t>(sizeof(float4) * (tf_n + 1u),
                                         std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u));
     const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
     const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);

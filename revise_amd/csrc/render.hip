@@ -32,8 +32,7 @@ __global__ __launch_bounds__(256) void render_block_kernel(rv_block_params p,
 {
     extern __shared__ float4 tfs[];
     __shared__ float lvs[kMaxLevels];
-    for (uint32_t i = threadIdx.x; i < tf_n; i += 256u)
-        tfs[i] = reinterpret_cast<const float4*>(tf)[i];
+    stage_tf(tfs, reinterpret_cast<const float4*>(tf), tf_n, threadIdx.x, 256u);
     if (multi_level(kMode) && threadIdx.x < lev.n)
         lvs[threadIdx.x] = lev.v[threadIdx.x];
     __syncthreads();
@@ -300,7 +299,7 @@ extern "C" int rv_render_block_secondary(const rv_block_params* params, uint32_t
     const LevelsArg lev = make_levels(p.fmin, p.fmax, levels_rel, level_count);
     const Secondary<float> sec{value ? d_brick2 : nullptr, smin, smax};
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
-    const size_t lds = tf_n * sizeof(float4);
+    const size_t lds = (tf_n + 1u) * sizeof(float4);
 #define RV_BLOCK_LAUNCH(M)                                                                         \
     hipLaunchKernelGGL(render_block_kernel<M>, grid, dim3(256), lds, S(stream), p, d_brick, sec, d_tf, \
                        tf_n, iso_opacity, lev, d_viewport)
@@ -407,11 +406,11 @@ extern "C" int rv_render_block_argb(const rv_block_params* params, const float* 
         return RV_OK;
     dim3 grid((p.view_w + 15) / 16, (p.view_h + 15) / 16);
     if (p.flags & RV_RENDER_EXACT)
-        hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), tf_n * sizeof(float4),
+        hipLaunchKernelGGL(render_block_kernel<kModeArgb>, grid, dim3(256), (tf_n + 1u) * sizeof(float4),
                            S(stream), p, d_brick, Secondary<float>{nullptr, 0.f, 1.f}, d_tf, tf_n, 1.f,
                            LevelsArg{}, d_viewport);
     else
-        hipLaunchKernelGGL(render_block_kernel<kModeArgbFast>, grid, dim3(256), tf_n * sizeof(float4),
+        hipLaunchKernelGGL(render_block_kernel<kModeArgbFast>, grid, dim3(256), (tf_n + 1u) * sizeof(float4),
                            S(stream), p, d_brick, Secondary<float>{nullptr, 0.f, 1.f}, d_tf, tf_n, 1.f,
                            LevelsArg{}, d_viewport);
     RV_LAUNCH_CHECK();

@@ -208,6 +208,15 @@ __device__ __forceinline__ Taps fetch_corners(const Tiled4* __restrict__ brick, 
     return t;
 }
 
+// The colour table into LDS (tid-th of nt threads), with one more entry,
+// dst[n] = src[n - 1]: the upper neighbour the fast sequence reads for the
+// last entry.  The LDS array holds n + 1 entries.
+__device__ __forceinline__ void stage_tf(float4* dst, const float4* src, uint32_t n, uint32_t tid, uint32_t nt)
+{
+    for (uint32_t i = tid; i <= n && n > 0u; i += nt)
+        dst[i] = src[i < n ? i : n - 1u];
+}
+
 // Exact fp32 trilinear fetch (the tex3D of RenderFuncInput.hpp:46-49).
 __device__ __forceinline__ float trilinear(const float v[8], float fx, float fy, float fz)
 {
@@ -496,12 +505,14 @@ __device__ __forceinline__ void fetch_fast(const T* __restrict__ brick, int n, f
 }
 
 // ... and micro-brick (TILED4) bricks: the per-axis parts of tiled4_index,
-// in bytes, summed per corner.  A cell's x-pair (x0, x0 + 1) lies in one
-// 16-byte micro-brick row unless x0 ends its row (x0 % 4 == 3), so each of
-// the 4 corner rows is one 8-byte load -- of (x0, x0 + 1), or of (x0 - 1,
-// x0) where x0 ends its row, whose x0 + 1 then comes from the next
-// micro-brick in a load of the lanes that need it.  Fewer load instructions
-// per sample mean fewer L1 tag lookups, which bound this march.
+// in bytes, summed per corner row as 32-bit offsets from the wave-uniform
+// brick address (global loads with a scalar base, no 64-bit arithmetic per
+// lane).  A cell's x-pair (x0, x0 + 1) lies in one 16-byte micro-brick row
+// unless x0 ends its row (x0 % 4 == 3), so each of the 4 corner rows is one
+// 8-byte load at x0; where x0 ends its row, x0 + 1 is the first node of the
+// next micro-brick's row, 244 bytes on, loaded by the lanes that need it
+// into registers of their own (so no load waits for another).  Fewer load
+// instructions per sample mean fewer L1 tag lookups, which bound this march.
 __device__ __forceinline__ void fetch_fast(const Tiled4* __restrict__ brick, int n, float tx, float ty, float tz,
                                            float v[8], float& fx, float& fy, float& fz)
 {
@@ -513,35 +524,34 @@ __device__ __forceinline__ void fetch_fast(const Tiled4* __restrict__ brick, int
     // (c & 3) * (node stride) = (c >> 2) * (micro-brick stride - 4 node
     // strides) + c * (node stride)
     const uint32_t G = ((uint32_t)n + 3u) >> 2, sy = (G << 8) - 64u, sz = ((G * G) << 8) - 256u;
-    const bool cross = (x0 & 3u) == 3u;   // x0 ends its micro-brick row
-    const bool split = cross && x1 != x0; // and x0 + 1 is read
-    const uint32_t xp = cross ? x0 - 1u : x0;
-    const uint32_t XP = __umul24(xp >> 2, 240u) + (xp << 2);
-    const uint32_t X1 = __umul24(x1 >> 2, 240u) + (x1 << 2);
+    const bool dupx = x1 == x0;         // a clamped x: both corners are node x0
+    const bool cross = (x0 & 3u) == 3u; // x0 ends its micro-brick row (never with dupx)
+    const uint32_t X = __umul24(x0 >> 2, 240u) + (x0 << 2);
     const uint32_t Y0 = __umul24(y0 >> 2, sy) + (y0 << 4);
     const uint32_t Y1 = __umul24(y1 >> 2, sy) + (y1 << 4);
     const uint32_t Z0 = __umul24(z0 >> 2, sz) + (z0 << 6);
     const uint32_t Z1 = __umul24(z1 >> 2, sz) + (z1 << 6);
     const gptr<char> b = global_of(reinterpret_cast<const char*>(brick));
-    const uint32_t row[4] = {Y0 + Z0, Y1 + Z0, Y0 + Z1, Y1 + Z1};
-    float lo[4], hi[4], nx[4];
+    const uint32_t XY0 = X + Y0, XY1 = X + Y1;
+    const uint32_t row[4] = {XY0 + Z0, XY1 + Z0, XY0 + Z1, XY1 + Z1};
+    // the pair at x0 stays inside the brick: x0 + 1 <= 4 G - 1 for every
+    // node x0 (n = 2^d + 1 leaves at least one padding node per axis)
+    float lo[4], hi[4], nx[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const uint64_t pr = *reinterpret_cast<gptr<pair32_t>>(b + XP + row[k]);
+        const uint64_t pr = *reinterpret_cast<gptr<pair32_t>>(b + row[k]);
         lo[k] = __uint_as_float((uint32_t)pr);
         hi[k] = __uint_as_float((uint32_t)(pr >> 32));
     }
+    if (cross) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        nx[k] = hi[k];
-        if (split)
-            nx[k] = *reinterpret_cast<gptr<float>>(b + X1 + row[k]);
+        for (int k = 0; k < 4; ++k)
+            nx[k] = *reinterpret_cast<gptr<float>>(b + row[k] + 244);
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float a = cross ? hi[k] : lo[k]; // node x0
-        v[2 * k] = a;
-        v[2 * k + 1] = x1 == x0 ? a : nx[k];
+        v[2 * k] = lo[k];
+        v[2 * k + 1] = dupx ? lo[k] : (cross ? nx[k] : hi[k]);
     }
 }
 
@@ -588,9 +598,9 @@ __device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* 
                 const float x = __builtin_amdgcn_fmed3f(fmaf(v[s], ta, tb), 0.f, thi);
                 const float fl = floorf(x);
                 const float f = x - fl;
+                // tf[tf_n] repeats tf[tf_n - 1] (stage_tf), so i0 + 1 needs no clamp
                 const int i0 = (int)fl;
-                const int i1 = i0 + 1 < tf_n ? i0 + 1 : i0;
-                const float4 a = tf[i0], b = tf[i1];
+                const float4 a = tf[i0], b = tf[i0 + 1];
                 const float al =
                     1.f - __builtin_amdgcn_exp2f(k * __builtin_amdgcn_logf(fmaf(-lerp_fast(a.w, b.w, f), of, 1.f)));
                 const float w = fmaf(-A, al, al); // (1 - A) al
