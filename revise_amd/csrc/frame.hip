@@ -343,6 +343,8 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         const uint4* __restrict__ rects,
                                                         const void* const* __restrict__ bricks,
                                                         const void* const* __restrict__ bricks2,
+                                                        const uint32_t* __restrict__ bin_off,
+                                                        const uint32_t* __restrict__ bin_list,
                                                         float smin, float smax,
                                                         uint32_t nblocks, const float4* __restrict__ tf,
                                                         uint32_t tf_n, float iso_opacity, LevelsArg lev,
@@ -364,35 +366,53 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
     const bool inside = vx < W && vy < H;
     const uint32_t pix = inside ? (H - 1u - vy) * W + vx : 0u;
     uint32_t value = inside ? viewport[pix] : 0u;
-    for (uint32_t c = 0; c < nblocks; c += 64u) {
-        const uint32_t i = c + lane;
-        bool hit = false;
-        if (i < nblocks) {
-            const uint4 r = rects[i];
-            hit = r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0;
+    // one block (wave-uniform) over the wave's pixels
+    auto draw = [&](uint32_t blk) {
+        const rv_block_params p = params[blk];
+        const bool inrect = inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
+                            vy < p.view_y + p.view_h;
+        if (inrect) {
+            const RaySeg r = setup_ray(p, vx, vy);
+            if (r.hit) {
+                const BrickT* brick = reinterpret_cast<const BrickT*>(bricks[blk]);
+                Secondary<BrickT> sec{nullptr, smin, smax};
+                if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces)
+                    sec.brick = reinterpret_cast<const BrickT*>(bricks2[blk]);
+                if (kMode == kModeMip)
+                    value = mip_merge(value, march_mip(p, brick, r), p);
+                else if (multi_level(kMode) && lev.n == 0)
+                    value = 0u; // renderIsosurfaces.hpp:33-34
+                else
+                    value = blend_above_u32(value, segment<kMode>(p, brick, tfs_share, (int)tf_n,
+                                                                  iso_opacity, r, lvs, lev.n, sec));
+            }
         }
-        uint64_t m = __ballot(hit);
-        while (m) {
-            const uint32_t blk = c + (uint32_t)__builtin_ctzll(m);
-            m &= m - 1ull;
-            const rv_block_params p = params[blk];
-            const bool inrect = inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
-                                vy < p.view_y + p.view_h;
-            if (inrect) {
-                const RaySeg r = setup_ray(p, vx, vy);
-                if (r.hit) {
-                    const BrickT* brick = reinterpret_cast<const BrickT*>(bricks[blk]);
-                    Secondary<BrickT> sec{nullptr, smin, smax};
-                    if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces)
-                        sec.brick = reinterpret_cast<const BrickT*>(bricks2[blk]);
-                    if (kMode == kModeMip)
-                        value = mip_merge(value, march_mip(p, brick, r), p);
-                    else if (multi_level(kMode) && lev.n == 0)
-                        value = 0u; // renderIsosurfaces.hpp:33-34
-                    else
-                        value = blend_above_u32(value, segment<kMode>(p, brick, tfs_share, (int)tf_n,
-                                                                      iso_opacity, r, lvs, lev.n, sec));
-                }
+    };
+    if (bin_off) {
+        // the blocks whose view rectangles meet this workgroup's 16 x 16
+        // screen bin, back to front (the table's order)
+        const uint32_t bin = blockIdx.y * gridDim.x + blockIdx.x;
+        const uint32_t k1 = bin_off[bin + 1u];
+        for (uint32_t k = bin_off[bin]; k < k1; ++k) {
+            const uint32_t blk = bin_list[k];
+            const uint4 r = rects[blk];
+            if (r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0) // meets this wave's 8 x 8 pixels
+                draw(blk);
+        }
+    } else {
+        // every block's view rectangle, 64 at a time, tested with a ballot
+        for (uint32_t c = 0; c < nblocks; c += 64u) {
+            const uint32_t i = c + lane;
+            bool hit = false;
+            if (i < nblocks) {
+                const uint4 r = rects[i];
+                hit = r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0;
+            }
+            uint64_t m = __ballot(hit);
+            while (m) {
+                const uint32_t blk = c + (uint32_t)__builtin_ctzll(m);
+                m &= m - 1ull;
+                draw(blk);
             }
         }
     }
@@ -1455,7 +1475,8 @@ extern "C" void rv_frame_plan_destroy(rv_frame_plan* P) { delete P; }
 // The share kernel over a staged block table (device params, view
 // rectangles {x0, y0, x1, y1}, brick and secondary-brick pointers).
 static int launch_share(const rv_block_params* dp, const uint4* dr, const void* const* db,
-                        const void* const* db2, uint32_t nblocks, uint32_t layout, uint32_t field_mode,
+                        const void* const* db2, const uint32_t* bin_off, const uint32_t* bin_list,
+                        uint32_t nblocks, uint32_t layout, uint32_t field_mode,
                         bool exact, float iso_opacity, const LevelsArg& lev, float smin, float smax,
                         const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
                         rv_stream_t stream)
@@ -1470,11 +1491,11 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
     do {                                                                                           \
         if (layout == RV_BRICK_TILED4)                                                             \
             hipLaunchKernelGGL((share_kernel<M, Tiled4>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
-                               db, db2, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
+                               db, db2, bin_off, bin_list, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
                                vp_w, vp_h);                                                        \
         else                                                                                       \
             hipLaunchKernelGGL((share_kernel<M, float>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
-                               db, db2, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
+                               db, db2, bin_off, bin_list, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
                                vp_w, vp_h);                                                        \
     } while (0)
     switch (field_mode) {
@@ -1508,7 +1529,8 @@ extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
 static int share_table(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
                        uint32_t nblocks, uint32_t layout, uint32_t field_mode, const float* levels_rel,
                        uint32_t level_count, float smin, float smax, const float* d_tf, uint32_t tf_n,
-                       const uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, uint8_t* host)
+                       const uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, uint8_t* host,
+                       uint64_t rect_at)
 {
     RV_REQUIRE(layout == RV_BRICK_LINEAR || layout == RV_BRICK_TILED4, "rv_render_share: unknown layout");
     RV_REQUIRE(field_mode <= RV_FIELD_VALUE_ON_ISOSURFACES, "rv_render_share: unknown field mode");
@@ -1521,8 +1543,8 @@ static int share_table(const rv_block_params* params, const float* const* bricks
     RV_REQUIRE(params && bricks && d_tf && d_viewport, "rv_render_share: null pointer");
     RV_REQUIRE(tf_n >= 1 && tf_n <= kMaxTf, "rv_render_share: tf_n must be in [1, 1024]");
     auto* hp = reinterpret_cast<rv_block_params*>(host);
-    auto* hr = reinterpret_cast<uint4*>(host + sizeof(rv_block_params) * nblocks);
-    auto* hb = reinterpret_cast<const void**>(host + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    auto* hr = reinterpret_cast<uint4*>(host + rect_at);
+    auto* hb = reinterpret_cast<const void**>(host + rect_at + sizeof(uint4) * nblocks);
     for (uint32_t i = 0; i < nblocks; ++i) {
         const rv_block_params& p = params[i];
         RV_REQUIRE(p.vp_w == vp_w && p.vp_h == vp_h, "rv_render_share: viewport mismatch");
@@ -1545,36 +1567,101 @@ static int share_table(const rv_block_params* params, const float* const* bricks
     return RV_OK;
 }
 
-// The share launch over a table staged in `host`, copied to d_workspace in
-// stream order; no synchronisation: the caller keeps `host` (pinned memory,
-// so the copy is asynchronous) unchanged until the stream has passed the
-// copy.
+// The share launch over a table staged in pinned host memory and copied in
+// stream order, with screen bins: the blocks whose view rectangles meet each
+// 16 x 16-pixel bin, in table (back-to-front) order, so a workgroup visits
+// only its bin's blocks.  Layout of the staged bytes: params, view
+// rectangles (16-byte aligned), brick and secondary-brick pointers, bin
+// offsets (nbins + 1), bin lists.  No synchronisation but the wait for the
+// previous copy out of the staging buffer.
 int rv::render_share_staged(const rv_block_params* params, const float* const* bricks,
                             const float* const* bricks2, uint32_t nblocks, uint32_t layout, uint32_t field_mode,
                             float iso_opacity, const float* levels_rel, uint32_t level_count, float smin,
                             float smax, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w,
-                            uint32_t vp_h, void* host_pinned, void* d_workspace, rv_stream_t stream)
+                            uint32_t vp_h, ShareStaging& st, rv_stream_t stream)
 {
     RV_TRY
     if (nblocks == 0)
         return RV_OK;
-    RV_REQUIRE(host_pinned && d_workspace, "rv_render_share: null staging");
-    uint8_t* host = static_cast<uint8_t*>(host_pinned);
+    const uint32_t gx = (vp_w + 15u) / 16u, gy = (vp_h + 15u) / 16u;
+    const uint64_t nbins = (uint64_t)gx * gy;
+    // bin counts first: the staged size depends on them
+    std::vector<uint32_t> off(nbins + 1u, 0u);
+    for (uint32_t i = 0; i < nblocks; ++i) {
+        const rv_block_params& p = params[i];
+        if (p.view_w == 0 || p.view_h == 0 || p.vp_w != vp_w || p.vp_h != vp_h ||
+            (uint64_t)p.view_x + p.view_w > vp_w || (uint64_t)p.view_y + p.view_h > vp_h)
+            continue; // empty, or rejected by share_table below
+        for (uint32_t by = p.view_y / 16u; by <= (p.view_y + p.view_h - 1u) / 16u; ++by)
+            for (uint32_t bx = p.view_x / 16u; bx <= (p.view_x + p.view_w - 1u) / 16u; ++bx)
+                ++off[(uint64_t)by * gx + bx + 1u];
+    }
+    for (uint64_t b = 0; b < nbins; ++b)
+        off[b + 1u] += off[b];
+    const uint64_t entries = off[nbins];
+    const uint64_t rect_at = (sizeof(rv_block_params) * (uint64_t)nblocks + 15u) & ~15ull;
+    const uint64_t off_at = rect_at + (sizeof(uint4) + 2u * sizeof(void*)) * (uint64_t)nblocks;
+    const uint64_t list_at = off_at + 4u * (nbins + 1u);
+    const uint64_t bytes = list_at + 4u * entries;
+    if (!st.copied)
+        RV_HIP(hipEventCreateWithFlags(&st.copied, hipEventDisableTiming));
+    // the previous copy out of the staging buffer has run (normally long
+    // since: every worker part ends with a stream synchronisation)
+    RV_HIP(hipEventSynchronize(st.copied));
+    if (bytes > st.cap) {
+        RV_HIP(hipStreamSynchronize(S(stream))); // no launch still reads the device table
+        if (st.dev)
+            (void)hipFree(st.dev);
+        if (st.host)
+            (void)hipHostFree(st.host);
+        st.dev = st.host = nullptr;
+        st.cap = 0;
+        const uint64_t cap = bytes + bytes / 4u + 4096u;
+        RV_HIP(hipMalloc(&st.dev, cap));
+        RV_HIP(hipHostMalloc(&st.host, cap, hipHostMallocDefault));
+        st.cap = cap;
+    }
+    uint8_t* host = static_cast<uint8_t*>(st.host);
     int rc = share_table(params, bricks, bricks2, nblocks, layout, field_mode, levels_rel, level_count, smin, smax,
-                         d_tf, tf_n, d_viewport, vp_w, vp_h, host);
+                         d_tf, tf_n, d_viewport, vp_w, vp_h, host, rect_at);
     if (rc)
         return rc;
-    uint8_t* d = static_cast<uint8_t*>(d_workspace);
-    RV_HIP(hipMemcpyAsync(d, host, rv_share_workspace_bytes(nblocks) - 256, hipMemcpyHostToDevice, S(stream)));
+    auto* hoff = reinterpret_cast<uint32_t*>(host + off_at);
+    auto* hlist = reinterpret_cast<uint32_t*>(host + list_at);
+    std::memcpy(hoff, off.data(), 4u * (nbins + 1u));
+    for (uint32_t i = 0; i < nblocks; ++i) {
+        const rv_block_params& p = params[i];
+        if (p.view_w == 0 || p.view_h == 0)
+            continue;
+        for (uint32_t by = p.view_y / 16u; by <= (p.view_y + p.view_h - 1u) / 16u; ++by)
+            for (uint32_t bx = p.view_x / 16u; bx <= (p.view_x + p.view_w - 1u) / 16u; ++bx)
+                hlist[off[(uint64_t)by * gx + bx]++] = i;
+    }
+    uint8_t* d = static_cast<uint8_t*>(st.dev);
+    RV_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, S(stream)));
+    RV_HIP(hipEventRecord(st.copied, S(stream)));
     const auto* dp = reinterpret_cast<const rv_block_params*>(d);
-    const auto* dr = reinterpret_cast<const uint4*>(d + sizeof(rv_block_params) * nblocks);
-    const auto* db =
-        reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    const auto* dr = reinterpret_cast<const uint4*>(d + rect_at);
+    const auto* db = reinterpret_cast<const void* const*>(d + rect_at + sizeof(uint4) * (uint64_t)nblocks);
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
-    return launch_share(dp, dr, db, db + nblocks, nblocks, layout, field_mode,
+    return launch_share(dp, dr, db, db + nblocks, reinterpret_cast<const uint32_t*>(d + off_at),
+                        reinterpret_cast<const uint32_t*>(d + list_at), nblocks, layout, field_mode,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
                         d_viewport, vp_w, vp_h, stream);
     RV_CATCH
+}
+
+void rv::ShareStaging::release()
+{
+    if (dev)
+        (void)hipFree(dev);
+    if (host)
+        (void)hipHostFree(host);
+    if (copied)
+        (void)hipEventDestroy(copied);
+    dev = host = nullptr;
+    copied = nullptr;
+    cap = 0;
 }
 
 extern "C" int rv_render_share_layout(const rv_block_params* params, const float* const* bricks,
@@ -1592,7 +1679,8 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
     RV_REQUIRE(d_workspace, "rv_render_share: null pointer");
     std::vector<uint8_t> host(rv_share_workspace_bytes(nblocks));
     int rc = share_table(params, bricks, bricks2, nblocks, layout, field_mode, levels_rel, level_count, smin, smax,
-                         d_tf, tf_n, d_viewport, vp_w, vp_h, host.data());
+                         d_tf, tf_n, d_viewport, vp_w, vp_h, host.data(),
+                         sizeof(rv_block_params) * (uint64_t)nblocks);
     if (rc)
         return rc;
     uint8_t* d = static_cast<uint8_t*>(d_workspace);
@@ -1605,7 +1693,7 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
         reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
     // the levels are relative to the field's range, the same for every block
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
-    return launch_share(dp, dr, db, db + nblocks, nblocks, layout, field_mode,
+    return launch_share(dp, dr, db, db + nblocks, nullptr, nullptr, nblocks, layout, field_mode,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
                         d_viewport, vp_w, vp_h, stream);
     RV_CATCH
@@ -1730,7 +1818,7 @@ extern "C" int rv_render_share_argb_noalloc(const rv_block_params* params, const
     hipLaunchKernelGGL(share_rects_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, S(stream), dp, dr, nblocks);
     RV_LAUNCH_CHECK();
     // Argb reads no secondary bricks: their slot gets the primary table
-    return launch_share(dp, dr, db, db, nblocks, RV_BRICK_LINEAR, RV_FIELD_ARGB,
+    return launch_share(dp, dr, db, db, nullptr, nullptr, nblocks, RV_BRICK_LINEAR, RV_FIELD_ARGB,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, 1.f, LevelsArg{}, 0.f, 1.f, d_tf, tf_n,
                         d_viewport, vp_w, vp_h, stream);
     RV_CATCH

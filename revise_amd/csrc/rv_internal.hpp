@@ -3,6 +3,7 @@
 #pragma once
 
 #include "../../include/revise_hip.h"
+#include "share_staging.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -40,15 +41,6 @@ int bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_b
 int crop_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, const int* d_buf, uint32_t* d_part,
                 hipStream_t s);
 int bound_rect_read(const int* d_buf, int32_t rect[4], int* nonempty, hipStream_t s);
-
-// rv_render_share_layout with the block table staged in the caller's pinned
-// host buffer (rv_share_workspace_bytes(nblocks) bytes) and copied in
-// stream order: no synchronisation.  frame.hip.
-int render_share_staged(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
-                        uint32_t nblocks, uint32_t layout, uint32_t field_mode, float iso_opacity,
-                        const float* levels_rel, uint32_t level_count, float smin, float smax, const float* d_tf,
-                        uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, void* host_pinned,
-                        void* d_workspace, rv_stream_t stream);
 
 } // namespace rv
 

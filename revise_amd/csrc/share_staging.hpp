@@ -1,0 +1,32 @@
+// The worker's staged share launch (frame.hip), declared for host-only
+// translation units as well.
+#pragma once
+
+#include "../../include/revise_hip.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+
+namespace rv {
+
+// Staging of a share's block table: pinned host bytes and device bytes of
+// `cap` each (grown on demand), and the event of the last copy out of the
+// host bytes.
+struct ShareStaging {
+    void* host = nullptr;
+    void* dev = nullptr;
+    uint64_t cap = 0;
+    hipEvent_t copied = nullptr;
+    void release();
+};
+
+// rv_render_share_layout with the block table and the share kernel's screen
+// bins staged through `st` and copied in stream order: no synchronisation.
+int render_share_staged(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
+                        uint32_t nblocks, uint32_t layout, uint32_t field_mode, float iso_opacity,
+                        const float* levels_rel, uint32_t level_count, float smin, float smax, const float* d_tf,
+                        uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, ShareStaging& st,
+                        rv_stream_t stream);
+
+} // namespace rv

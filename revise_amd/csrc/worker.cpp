@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include "share_staging.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -17,11 +19,6 @@ int bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_b
 int crop_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, const int* d_buf, uint32_t* d_part,
                 hipStream_t s);
 int bound_rect_read(const int* d_buf, int32_t rect[4], int* nonempty, hipStream_t s);
-int render_share_staged(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
-                        uint32_t nblocks, uint32_t layout, uint32_t field_mode, float iso_opacity,
-                        const float* levels_rel, uint32_t level_count, float smin, float smax, const float* d_tf,
-                        uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, void* host_pinned,
-                        void* d_workspace, rv_stream_t stream);
 }
 
 namespace revise {
@@ -377,10 +374,7 @@ struct VsWorkerHip::ShareCache {
     std::map<std::array<uint32_t, 4>, Brick> bricks;
     std::map<std::array<uint32_t, 4>, Brick> bricks2; // secondary field (ValueOnIsosurface(s))
     std::map<std::array<uint32_t, 8>, Batch> batches;
-    void* workspace = nullptr;
-    uint64_t workspace_bytes = 0;
-    void* staging = nullptr; // pinned host copy of the block table
-    hipEvent_t staged = nullptr; // recorded after the launch that reads `staging`
+    rv::ShareStaging table; // the block table and screen bins of the share launch
     void dropBatches()
     {
         for (auto& kv : batches)
@@ -411,14 +405,9 @@ struct VsWorkerHip::ShareCache {
             (void)hipFree(kv.second.p);
         for (auto& kv : bricks2)
             (void)hipFree(kv.second.p);
-        if (workspace)
-            (void)hipFree(workspace);
-        if (staged)
-            (void)hipEventSynchronize(staged);
-        if (staging)
-            (void)hipHostFree(staging);
-        if (staged)
-            (void)hipEventDestroy(staged);
+        if (table.copied)
+            (void)hipEventSynchronize(table.copied);
+        table.release();
     }
 };
 
@@ -631,39 +620,18 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         if (second)
             bricks2.push_back(sc.bricks2[bkey].p);
     }
-    if (!sc.staged)
-        hipCheck(hipEventCreateWithFlags(&sc.staged, hipEventDisableTiming), "hipEventCreate");
-    // the previous part's copy of the table has run (normally long since:
-    // every part ends with a stream synchronisation)
-    hipCheck(hipEventSynchronize(sc.staged), "hipEventSynchronize");
-    const uint64_t need = rv_share_workspace_bytes((uint32_t)params.size());
-    if (need > sc.workspace_bytes) {
-        if (sc.workspace)
-            (void)hipFree(sc.workspace);
-        sc.workspace = nullptr;
-        if (sc.staging)
-            (void)hipHostFree(sc.staging);
-        sc.staging = nullptr;
-        sc.workspace_bytes = 0;
-        hipCheck(hipMalloc(&sc.workspace, need), "hipMalloc");
-        hipCheck(hipHostMalloc(&sc.staging, need, hipHostMallocDefault), "hipHostMalloc");
-        sc.workspace_bytes = need;
-    }
     if (isCancelled()) {
         m_cancelled = true;
         return;
     }
     const auto srange = fs.secondaryRange();
-    // the table goes through the pinned staging buffer, asynchronously: every
-    // renderScenePart ends with a stream synchronisation, so the copy has
-    // run before the next part rewrites the buffer
+    // the table and the screen bins go through pinned staging memory,
+    // asynchronously (the next part waits for this copy before rewriting it)
     throwIfFailed(rv::render_share_staged(
         params.data(), bricks.data(), second ? bricks2.data() : nullptr, (uint32_t)params.size(),
         RV_BRICK_TILED4, st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
         (uint32_t)st.isosurfaceLevels.size(), srange[0], srange[1], m_renderer.colorTable(),
-        m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1], sc.staging,
-        sc.workspace, s));
-    hipCheck(hipEventRecord(sc.staged, hs(s)), "hipEventRecord");
+        m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1], sc.table, s));
 }
 
 const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const

@@ -348,3 +348,34 @@ def test_worker_compose_frame_repeated_uses_cached_batches(orc):
     w.add_subtree(2, b.index, b.depth, b.n2i, blocks[(2, 1, 1)].sparse)
     assert np.array_equal(w.compose_frame(2, 5), _oracle_frame(orc, blocks, 2, 5, M, W, H, fmin, fmax, tf))
     w.close()
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_worker_level5_share_bins(exact):
+    """A level-5 share, 32768 blocks (full depth-2 subtrees), at 640 x 360:
+    the fused path's share launch visits each 16 x 16 screen bin's blocks
+    only (the bins built with the block table); its part equals the
+    per-block path's bit for bit, in both sample sequences."""
+    import time
+    W, H = 640, 360
+    field = datasets.Field(23, n_gauss=3)
+    n2i = api.subtree_n2i(datasets.full_subtree(2), 2)
+    blocks = datasets.uniform_level(5, 2, field, n2i)
+    fmin, fmax = datasets.field_range(blocks)
+    tf = datasets.seeded_tf(23, 256)
+    M = datasets.orbit_camera(55.0)
+    parts = {}
+    for fused in (True, False):
+        w = api.Worker(0)
+        for b in blocks:
+            w.add_subtree(5, b.index, b.depth, b.n2i, b.sparse)
+        w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, exact=exact)
+        w.set_fused(fused)
+        w.render_scene_part(5, (0, 0, 0), (32, 32, 32))
+        t0 = time.perf_counter()
+        parts[fused] = w.render_scene_part(5, (0, 0, 0), (32, 32, 32))
+        say("level-5 share, fused %s: %.1f ms" % (fused, (time.perf_counter() - t0) * 1e3))
+        w.close()
+    assert parts[True][0] == parts[False][0]
+    assert parts[True][0][2] * parts[True][0][3] > W * H // 10
+    assert np.array_equal(parts[True][1], parts[False][1])
