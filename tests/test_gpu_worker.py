@@ -379,3 +379,47 @@ def test_worker_level5_share_bins(exact):
     assert parts[True][0] == parts[False][0]
     assert parts[True][0][2] * parts[True][0][3] > W * H // 10
     assert np.array_equal(parts[True][1], parts[False][1])
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_share_scan_equals_bins_level5(exact):
+    """The same level-5 share (32768 full depth-2 blocks, 640 x 360) through
+    the share launch that scans every block's view rectangle per wave
+    (rv_render_share_layout) and through the worker's binned launch: the
+    cropped parts are equal bit for bit."""
+    import torch
+    from revise_amd import configs
+    W, H = 640, 360
+    field = datasets.Field(24, n_gauss=3)
+    wl = configs.uniform(torch, field, level=5, depth=2)
+    tf_np = datasets.seeded_tf(24, 256)
+    M = datasets.orbit_camera(125.0, elevation_deg=30.0)
+    fmin, fmax = float(wl.fmin), float(wl.fmax)
+    host = wl.sparse.cpu().numpy()
+    w = api.Worker(0)
+    for i, b in enumerate(wl.blocks):
+        w.add_subtree(5, b["index"], 2, wl.n2i["full"], host[i])
+    w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf_np, exact=exact)
+    w.set_fused(True)
+    rect, bits = w.render_scene_part(5, (0, 0, 0), (32, 32, 32))
+    w.close()
+    tiled = [torch.empty(api.brick_floats(2, api.BRICK_TILED4), dtype=torch.float32, device="cuda")
+             for _ in wl.blocks]
+    api.InterpBatch([(b["sparse"], b["plan"], t, 2) for b, t in zip(wl.blocks, tiled)], api.BRICK_TILED4).run()
+    keys = np.array([(5,) + tuple(b["index"]) for b in wl.blocks], dtype=np.uint32)
+    items = []
+    for i in api.mixed_back_to_front(keys, api.eye_from_view(M)):
+        b = wl.blocks[int(i)]
+        live, p = api.block_params(M, W, H, FOV, 5, b["index"], 2, fmin, fmax, THR, Q, CUT, exact=exact)
+        if live:
+            items.append((p, tiled[int(i)]))
+    tf = torch.from_numpy(tf_np).cuda()
+    vp = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    api.clear_viewport(vp, W, H)
+    api.ShareRenderer(items, tf, W, H, layout=api.BRICK_TILED4).render(vp)
+    torch.cuda.synchronize()
+    say("level-5 share: %d live blocks, part %s" % (len(items), rect))
+    assert tuple(api.bound_rect(vp, W, H)) == tuple(rect)
+    x, y, pw, ph = rect
+    img = vp.cpu().numpy().view(np.uint32)
+    assert np.array_equal(img[y:y + ph, x:x + pw].reshape(-1), np.asarray(bits).reshape(-1))
