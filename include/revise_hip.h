@@ -126,11 +126,14 @@ int rv_interp_brick(const float* d_sparse, const uint16_t* n2i, uint64_t n, uint
 
 /* Dense brick layouts.  LINEAR: (2^depth+1)^3 nodes, x fastest, then y, z
  * (the reference's).  TILED4 (fp32): 4 x 4 x 4 micro-bricks of 64 floats,
- * x fastest inside, the G^3 micro-bricks x fastest, G = ceil((2^depth+1)/4);
- * node (x, y, z) at ((z/4 * G + y/4) * G + x/4) * 64 + (z%4 * 16 + y%4 * 4 +
- * x%4); padding nodes are written NaN and never read; bricks 16-byte
- * aligned.  The ray-casts read a TILED4 brick's cell corners from fewer
- * cache lines; the values are the same. */
+ * x fastest inside, the G^3 micro-bricks x fastest, G = ceil((n+3)/4) for
+ * n = 2^depth+1 nodes per axis; node (x, y, z) at position (x+1, y, z),
+ * index ((z/4 * G + y/4) * G + (x+1)/4) * 64 + (z%4 * 16 + y%4 * 4 +
+ * (x+1)%4); x-positions 0 and n .. 4G-1 hold copies of the edge nodes 0
+ * and n-1 (so a clamped sample's x-pair lies in the brick), other padding
+ * is written NaN and never read; bricks 16-byte aligned.  The ray-casts
+ * read a TILED4 brick's cell corners from fewer cache lines; the values are
+ * the same. */
 enum { RV_BRICK_LINEAR = 0, RV_BRICK_TILED4 = 1 };
 /* Floats a brick of this depth takes in a layout. */
 uint64_t rv_brick_floats(uint32_t depth, uint32_t layout);

@@ -295,22 +295,36 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
 
     // 4. store the owned half-open box (+ far faces of the brick)
     if (job.tiled) {
-        // the micro-bricks of the owned box (tile boundaries are multiples of
-        // 4 whenever a brick has several tiles), 64 consecutive floats each
-        const uint32_t G = (g.n + 3u) >> 2;
-        const uint32_t ex = (tx + 1u == g.tiles_x) ? rowlen : X;
-        const uint32_t ey = (ty + 1u == g.tiles_yz) ? tp1 : T;
-        const uint32_t ez = (tz + 1u == g.tiles_yz) ? tp1 : T;
-        const uint32_t mx0 = x0 >> 2, my0 = y0 >> 2, mz0 = z0 >> 2;
-        const uint32_t nmx = ((x0 + ex + 3u) >> 2) - mx0, nmy = ((y0 + ey + 3u) >> 2) - my0,
-                       nmz = ((z0 + ez + 3u) >> 2) - mz0;
-        // one micro-brick row (4 nodes along x, 16 bytes) per lane: a wave
-        // stores 4 whole micro-bricks, and its LDS reads of 4 consecutive x
-        // meet few bank conflicts.  The wave's first micro-brick index is
-        // wave-uniform (NT is a multiple of 64), so its decomposition runs on
-        // the scalar unit; the lanes' offsets 0..3 are carried along x.
-        // Padding nodes (beyond the brick) are written NaN, never read.
+        // Micro-bricks of 4 x 4 x 4 positions, 64 consecutive floats each (x
+        // fastest inside and over the G^3 of them).  Along y and z node c is
+        // at position c; along x at c + 1, and x-positions 0 and n .. 4G - 1
+        // hold the edge nodes 0 and n - 1, so the x-pair (c, c + 1) of every
+        // clamped sample lies in the brick.  A tile writes the positions of
+        // the nodes it owns (x: x0 + 1 .. x0 + ex; the first x-tile also
+        // position 0, the last the edge positions; y/z: its owned box, the
+        // last tile up to 4G with padding rows, written NaN and never read).
+        // Along y and z tile boundaries are multiples of 4, so each
+        // micro-brick row belongs to one tile; along x (bricks of several
+        // x-tiles) the row holding position x0 is shared with the previous
+        // tile, and each tile writes its own floats of it.
+        const uint32_t G = (g.n + 6u) >> 2;
+        const bool lastx = tx + 1u == g.tiles_x, lasty = ty + 1u == g.tiles_yz, lastz = tz + 1u == g.tiles_yz;
+        const uint32_t ex = lastx ? rowlen : X;
+        const uint32_t ey = lasty ? tp1 : T;
+        const uint32_t ez = lastz ? tp1 : T;
+        const uint32_t p_lo = tx == 0u ? 0u : x0 + 1u, p_hi = lastx ? 4u * G : x0 + ex + 1u;
+        const uint32_t mx0 = p_lo >> 2, my0 = y0 >> 2, mz0 = z0 >> 2;
+        const uint32_t nmx = ((p_hi + 3u) >> 2) - mx0;
+        const uint32_t nmy = (lasty ? G : ((y0 + ey + 3u) >> 2)) - my0;
+        const uint32_t nmz = (lastz ? G : ((z0 + ez + 3u) >> 2)) - mz0;
+        // one micro-brick row (4 positions along x, 16 bytes) per lane: a
+        // wave stores 4 whole micro-bricks, and its LDS reads of 4
+        // consecutive x meet few bank conflicts.  The wave's first
+        // micro-brick index is wave-uniform (NT is a multiple of 64), so its
+        // decomposition runs on the scalar unit; the lanes' offsets 0..3 are
+        // carried along x.
         const uint32_t total = nmx * nmy * nmz * 16u;
+        const int nlast = (int)g.n - 1;
         for (uint32_t q = threadIdx.x; q < total; q += NT) {
             const uint32_t mbw = __builtin_amdgcn_readfirstlane(q >> 6) << 2, r = q & 15u;
             uint32_t mx = mbw % nmx, my = (mbw / nmx) % nmy, mz = mbw / (nmx * nmy);
@@ -325,25 +339,34 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
             mx += mx0;
             my += my0;
             mz += mz0;
-            const uint32_t y = (my << 2) | (r & 3u), z = (mz << 2) | (r >> 2), xb = mx << 2;
-            // Outside the owned box lie only padding nodes: tile boundaries
-            // are multiples of 4, and every micro-brick belongs to one tile.
-            float o[4];
+            const uint32_t y = (my << 2) | (r & 3u), z = (mz << 2) | (r >> 2), pb = mx << 2;
             const bool row_ok = y < y0 + ey && z < z0 + ez;
+            float o[4];
+            bool own[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
+                const uint32_t pos = pb + (uint32_t)c;
+                own[c] = pos >= p_lo && pos < p_hi;
+                const int node = min(max((int)pos - 1, 0), nlast); // in [x0, x0 + ex) when owned
                 float v = __builtin_nanf("");
-                if (row_ok && xb + (uint32_t)c < x0 + ex) {
-                    v = A[((z - z0) * tp1 + (y - y0)) * rowlen + (xb - x0) + (uint32_t)c];
+                if (row_ok && own[c]) {
+                    v = A[((z - z0) * tp1 + (y - y0)) * rowlen + ((uint32_t)node - x0)];
                     if (v == kNoField)
                         v = __builtin_nanf("");
                 }
                 o[c] = v;
             }
-            typedef float f32x4 __attribute__((ext_vector_type(4)));
-            const f32x4 row = {o[0], o[1], o[2], o[3]};
-            __builtin_nontemporal_store(row, reinterpret_cast<f32x4*>(job.dense + (((mz * G + my) * G + mx) << 6) +
-                                                                      (r << 2)));
+            float* dst = job.dense + (((mz * G + my) * G + mx) << 6) + (r << 2);
+            if (own[0] && own[1] && own[2] && own[3]) {
+                typedef float f32x4 __attribute__((ext_vector_type(4)));
+                const f32x4 row = {o[0], o[1], o[2], o[3]};
+                __builtin_nontemporal_store(row, reinterpret_cast<f32x4*>(dst));
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    if (own[c])
+                        __builtin_nontemporal_store(o[c], dst + c);
+            }
         }
         return;
     }

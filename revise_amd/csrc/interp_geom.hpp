@@ -48,12 +48,14 @@ RV_HD inline Geom geom_of(uint32_t d)
 }
 
 // Index of node (x, y, z) of a brick of n nodes per axis in the 4 x 4 x 4
-// micro-brick layout (G = ceil(n/4)): micro-bricks of 64 floats (x fastest,
-// then y, z), ordered x fastest over G^3 of them; nodes beyond n - 1 are
-// padding.
+// micro-brick layout (G = ceil((n + 3) / 4)): micro-bricks of 64 floats (x
+// fastest, then y, z), ordered x fastest over G^3 of them; along x node x is
+// at position x + 1 (positions 0 and n .. 4G - 1 repeat the edge nodes),
+// along y and z at y, z.
 RV_HD inline uint32_t tiled4_index(uint32_t x, uint32_t y, uint32_t z, uint32_t G)
 {
-    return ((((z >> 2) * G + (y >> 2)) * G + (x >> 2)) << 6) + (((z & 3u) << 4) | ((y & 3u) << 2) | (x & 3u));
+    const uint32_t px = x + 1u;
+    return ((((z >> 2) * G + (y >> 2)) * G + (px >> 2)) << 6) + (((z & 3u) << 4) | ((y & 3u) << 2) | (px & 3u));
 }
 
 // LDS floats a tile needs: its (X+1)(T+1)^2 box, or the (m+1)^3 coarse

@@ -192,9 +192,11 @@ __device__ __forceinline__ Taps fetch_corners(const Tiled4* __restrict__ brick, 
     // carries: the low 6 bits hold the in-micro-brick part).  One load per
     // corner: the exact sequence is bound by its arithmetic, and the pair
     // loads of the fast fetch cost it more instructions than they save.
-    const uint32_t G = ((uint32_t)n + 3u) >> 2, sy = G << 6, sz = (G * G) << 6;
-    const uint32_t X0 = (((uint32_t)x0 >> 2) << 6) | ((uint32_t)x0 & 3u);
-    const uint32_t X1 = (((uint32_t)x1 >> 2) << 6) | ((uint32_t)x1 & 3u);
+    // (x-positions are nodes + 1)
+    const uint32_t G = ((uint32_t)n + 6u) >> 2, sy = G << 6, sz = (G * G) << 6;
+    const uint32_t px0 = (uint32_t)x0 + 1u, px1 = (uint32_t)x1 + 1u;
+    const uint32_t X0 = ((px0 >> 2) << 6) | (px0 & 3u);
+    const uint32_t X1 = ((px1 >> 2) << 6) | (px1 & 3u);
     const uint32_t Y0 = __umul24((uint32_t)y0 >> 2, sy) + (((uint32_t)y0 & 3u) << 2);
     const uint32_t Y1 = __umul24((uint32_t)y1 >> 2, sy) + (((uint32_t)y1 & 3u) << 2);
     const uint32_t Z0 = __umul24((uint32_t)z0 >> 2, sz) + (((uint32_t)z0 & 3u) << 4);
@@ -507,26 +509,31 @@ __device__ __forceinline__ void fetch_fast(const T* __restrict__ brick, int n, f
 // ... and micro-brick (TILED4) bricks: the per-axis parts of tiled4_index,
 // in bytes, summed per corner row as 32-bit offsets from the wave-uniform
 // brick address (global loads with a scalar base, no 64-bit arithmetic per
-// lane).  A cell's x-pair (x0, x0 + 1) lies in one 16-byte micro-brick row
-// unless x0 ends its row (x0 % 4 == 3), so each of the 4 corner rows is one
-// 8-byte load at x0; where x0 ends its row, x0 + 1 is the first node of the
-// next micro-brick's row, 244 bytes on, loaded by the lanes that need it
-// into registers of their own (so no load waits for another).  Fewer load
-// instructions per sample mean fewer L1 tag lookups, which bound this march.
+// lane).  Along x the brick holds node c at position c + 1 and the edge
+// nodes at the positions beyond, so the pair of a sample is positions
+// (i + 1, i + 2) for its unclamped cell i in [-1, n], with no clamp: the
+// pair of a clamped sample holds the edge node twice.  It lies in one
+// 16-byte micro-brick row unless position i + 1 ends its row; then the
+// second node is the first of the next micro-brick's row, 244 bytes on,
+// loaded by the lanes that need it into registers of their own (so no load
+// waits for another).  Fewer load instructions per sample mean fewer L1 tag
+// lookups, which bound this march.
 __device__ __forceinline__ void fetch_fast(const Tiled4* __restrict__ brick, int n, float tx, float ty, float tz,
                                            float v[8], float& fx, float& fy, float& fz)
 {
-    uint32_t x0, x1, y0, y1, z0, z1;
-    fast_axis(tx, n, x0, x1, fx);
+    const float vx = __builtin_amdgcn_fmed3f(tx - 0.5f, -1.f, (float)n);
+    const float flx = floorf(vx);
+    fx = vx - flx;
+    const uint32_t cx = (uint32_t)((int)flx + 1); // x-position of the pair, 0 .. n + 1
+    uint32_t y0, y1, z0, z1;
     fast_axis(ty, n, y0, y1, fy);
     fast_axis(tz, n, z0, z1, fz);
-    // byte offset of node c along an axis: (c >> 2) * (micro-brick stride) +
-    // (c & 3) * (node stride) = (c >> 2) * (micro-brick stride - 4 node
-    // strides) + c * (node stride)
-    const uint32_t G = ((uint32_t)n + 3u) >> 2, sy = (G << 8) - 64u, sz = ((G * G) << 8) - 256u;
-    const bool dupx = x1 == x0;         // a clamped x: both corners are node x0
-    const bool cross = (x0 & 3u) == 3u; // x0 ends its micro-brick row (never with dupx)
-    const uint32_t X = __umul24(x0 >> 2, 240u) + (x0 << 2);
+    // byte offset of position c along an axis: (c >> 2) * (micro-brick
+    // stride) + (c & 3) * (node stride) = (c >> 2) * (micro-brick stride - 4
+    // node strides) + c * (node stride)
+    const uint32_t G = ((uint32_t)n + 6u) >> 2, sy = (G << 8) - 64u, sz = ((G * G) << 8) - 256u;
+    const bool cross = (cx & 3u) == 3u; // the pair's second position starts the next micro-brick
+    const uint32_t X = __umul24(cx >> 2, 240u) + (cx << 2);
     const uint32_t Y0 = __umul24(y0 >> 2, sy) + (y0 << 4);
     const uint32_t Y1 = __umul24(y1 >> 2, sy) + (y1 << 4);
     const uint32_t Z0 = __umul24(z0 >> 2, sz) + (z0 << 6);
@@ -534,8 +541,9 @@ __device__ __forceinline__ void fetch_fast(const Tiled4* __restrict__ brick, int
     const gptr<char> b = global_of(reinterpret_cast<const char*>(brick));
     const uint32_t XY0 = X + Y0, XY1 = X + Y1;
     const uint32_t row[4] = {XY0 + Z0, XY1 + Z0, XY0 + Z1, XY1 + Z1};
-    // the pair at x0 stays inside the brick: x0 + 1 <= 4 G - 1 for every
-    // node x0 (n = 2^d + 1 leaves at least one padding node per axis)
+    // a crossing pair's 8-byte load reads its first node and the next float
+    // of the same micro-brick, or the first of the next micro-brick: inside
+    // the brick (position cx + 1 <= n + 2 <= 4G - 1)
     float lo[4], hi[4], nx[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -551,7 +559,7 @@ __device__ __forceinline__ void fetch_fast(const Tiled4* __restrict__ brick, int
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         v[2 * k] = lo[k];
-        v[2 * k + 1] = dupx ? lo[k] : (cross ? nx[k] : hi[k]);
+        v[2 * k + 1] = cross ? nx[k] : hi[k];
     }
 }
 

@@ -37,13 +37,15 @@ def test_fp32_reciprocal_division_is_exact():
 
 
 # The micro-brick store of interp_tile (interp_device.hpp, job.tiled), restated
-# with numpy: per tile, lane q of the loop stores micro-brick row q & 15 of
-# micro-brick q >> 4 of the owned box, the wave's first micro-brick
-# decomposed once and the lane's offset 0..3 carried along x.
+# with numpy: per tile, lane q of the loop stores the positions it owns of
+# micro-brick row q & 15 of micro-brick q >> 4 of the tile's range, the
+# wave's first micro-brick decomposed once and the lane's offset 0..3
+# carried along x.  Along x node c is at position c + 1 and positions 0 and
+# n .. 4G - 1 repeat the edge nodes.
 def _tiled_store_cover(d):
     n = (1 << d) + 1; t = min(d, 4); T = 1 << t; X = (1 << d) if d < 6 else 64
     tiles_x = (1 << d) // X; tiles_yz = (1 << d) // T
-    G = (n + 3) // 4
+    G = (n + 6) // 4
     written = np.zeros(G**3 * 64, np.int32)
     owner = -np.ones(G**3 * 64, np.int64)
     rowlen, tp1 = X + 1, T + 1
@@ -51,11 +53,16 @@ def _tiled_store_cover(d):
         for ty in range(tiles_yz):
             for tx in range(tiles_x):
                 x0, y0, z0 = tx * X, ty * T, tz * T
-                ex = rowlen if tx + 1 == tiles_x else X
-                ey = tp1 if ty + 1 == tiles_yz else T
-                ez = tp1 if tz + 1 == tiles_yz else T
-                mx0, my0, mz0 = x0 >> 2, y0 >> 2, z0 >> 2
-                nmx = ((x0 + ex + 3) >> 2) - mx0; nmy = ((y0 + ey + 3) >> 2) - my0; nmz = ((z0 + ez + 3) >> 2) - mz0
+                lastx, lasty, lastz = tx + 1 == tiles_x, ty + 1 == tiles_yz, tz + 1 == tiles_yz
+                ex = rowlen if lastx else X
+                ey = tp1 if lasty else T
+                ez = tp1 if lastz else T
+                p_lo = 0 if tx == 0 else x0 + 1
+                p_hi = 4 * G if lastx else x0 + ex + 1
+                mx0, my0, mz0 = p_lo >> 2, y0 >> 2, z0 >> 2
+                nmx = ((p_hi + 3) >> 2) - mx0
+                nmy = (G if lasty else ((y0 + ey + 3) >> 2)) - my0
+                nmz = (G if lastz else ((z0 + ez + 3) >> 2)) - mz0
                 total = nmx * nmy * nmz * 16
                 q = np.arange(total, dtype=np.int64)
                 mbw = (q >> 6) << 2; r = q & 15
@@ -71,32 +78,37 @@ def _tiled_store_cover(d):
                 mb = q >> 4
                 assert (mx == mb % nmx).all() and (my == (mb // nmx) % nmy).all() and (mz == mb // (nmx * nmy)).all()
                 mx += mx0; my += my0; mz += mz0
-                y = (my << 2) | (r & 3); z = (mz << 2) | (r >> 2); xb = mx << 2
+                y = (my << 2) | (r & 3); z = (mz << 2) | (r >> 2); pb = mx << 2
                 row_ok = (y < y0 + ey) & (z < z0 + ez)
                 base = (((mz * G + my) * G + mx) << 6) + (r << 2)
+                tile_id = (tz * tiles_yz + ty) * tiles_x + tx
                 for c in range(4):
                     addr = base + c
                     assert (addr >= 0).all() and (addr < G**3 * 64).all(), d
-                    valid = row_ok & (xb + c < x0 + ex)
-                    lds = ((z - z0) * tp1 + (y - y0)) * rowlen + (xb - x0) + c
+                    pos = pb + c
+                    own = (pos >= p_lo) & (pos < p_hi)
+                    node = np.clip(pos - 1, 0, n - 1)
+                    valid = row_ok & own
+                    # owned positions map to nodes this tile holds in LDS
+                    assert ((node[own] >= x0) & (node[own] < x0 + ex)).all(), d
+                    lds = ((z - z0) * tp1 + (y - y0)) * rowlen + (node - x0)
                     assert (lds[valid] < rowlen * tp1 * tp1).all() and (lds[valid] >= 0).all()
-                    np.add.at(written, addr, 1)
-                    tile_id = (tz * tiles_yz + ty) * tiles_x + tx
-                    assert (owner[addr] == -1).all(), ("two tiles write one node", d)
-                    owner[addr] = tile_id
-                    # the node's true coordinates and validity
-                    x = xb + c
-                    real = (x < n) & (y < n) & (z < n)
-                    assert (valid == real).all(), ("valid != real node", d)
-                    # tiled index of (x,y,z) equals the address
-                    ti = ((((z >> 2) * G + (y >> 2)) * G + (x >> 2)) << 6) + (((z & 3) << 4) | ((y & 3) << 2) | (x & 3))
+                    np.add.at(written, addr[own], 1)
+                    assert (owner[addr[own]] == -1).all(), ("two tiles write one float", d)
+                    owner[addr[own]] = tile_id
+                    # the LDS is read exactly for the brick's real rows
+                    real = (y < n) & (z < n)
+                    assert (valid == (real & own)).all(), ("valid != real row", d)
+                    # the tiled index of (position, y, z) is the address
+                    ti = ((((z >> 2) * G + (y >> 2)) * G + (pos >> 2)) << 6) + (((z & 3) << 4) | ((y & 3) << 2) | (pos & 3))
                     assert (ti == addr).all()
     assert (written == 1).all(), d
 
 
 @pytest.mark.parametrize("d", range(0, 9))
 def test_tiled_store_writes_every_node_once(d):
-    """Every node of every micro-brick (padding included) is stored exactly
-    once, by one tile; the lanes that read LDS are exactly the brick's real
-    nodes, inside the tile's box; stores stay inside the brick."""
+    """Every float of every micro-brick (edge copies and padding included)
+    is stored exactly once, by one tile (x-tiles share the rows at their
+    seams float by float); the lanes that read LDS are exactly the brick's
+    real rows, inside the tile's box; stores stay inside the brick."""
     _tiled_store_cover(d)
