@@ -125,15 +125,17 @@ int rv_interp_brick(const float* d_sparse, const uint16_t* n2i, uint64_t n, uint
                     float* d_dense, rv_stream_t stream);
 
 /* Dense brick layouts.  LINEAR: (2^depth+1)^3 nodes, x fastest, then y, z
- * (the reference's).  TILED4 (fp32): 4 x 4 x 4 micro-bricks of 64 floats,
- * x fastest inside, the G^3 micro-bricks x fastest, G = ceil((n+3)/4) for
- * n = 2^depth+1 nodes per axis; node (x, y, z) at position (x+1, y, z),
- * index ((z/4 * G + y/4) * G + (x+1)/4) * 64 + (z%4 * 16 + y%4 * 4 +
- * (x+1)%4); x-positions 0 and n .. 4G-1 hold copies of the edge nodes 0
- * and n-1 (so a clamped sample's x-pair lies in the brick), other padding
- * is written NaN and never read; bricks 16-byte aligned.  The ray-casts
- * read a TILED4 brick's cell corners from fewer cache lines; the values are
- * the same. */
+ * (the reference's).  TILED4 (fp32): micro-bricks of 4 x 4 x 4 cells, 80
+ * floats each -- 16 rows (y fastest, then z) of 5 x-positions, micro-brick
+ * m holding x-positions 4m .. 4m+4 (x neighbours share one position) --
+ * ordered x fastest over G^3 micro-bricks, G = floor((n+1)/4) + 1 for n =
+ * 2^depth+1 nodes per axis.  Node (x, y, z) is at position (x+1, y, z):
+ * index ((z/4 * G + y/4) * G + (x+1)/4) * 80 + z%4 * 20 + y%4 * 5 +
+ * (x+1)%4 (and, where (x+1)%4 == 0 and x+1 >= 4, also at the last position
+ * of the previous micro-brick's row); x-positions 0 and n .. 4G hold copies
+ * of the edge nodes 0 and n-1, other padding is written NaN and never read;
+ * bricks 16-byte aligned.  The ray-casts read a cell's 8 corners as 4
+ * 8-byte row loads from fewer cache lines; the values are the same. */
 enum { RV_BRICK_LINEAR = 0, RV_BRICK_TILED4 = 1 };
 /* Floats a brick of this depth takes in a layout. */
 uint64_t rv_brick_floats(uint32_t depth, uint32_t layout);

@@ -83,7 +83,7 @@ constexpr uint32_t kSpinLimit = 1u << 22;
 
 // One wave: the pixels of screen bin `bin`, in field mode kMode.
 template <class BrickT, uint32_t kMode>
-__device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, const float* lvs)
+__device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, const float* lvs, const uint2* yz)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t bx = bin % a.nbx, by = bin / a.nbx;
@@ -133,7 +133,7 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
                 } else if (kMode == kModeArgb && a.fast) {
-                    seg = march_fast(p, brick, tfs, (int)a.tf_n, r);
+                    seg = march_fast(p, brick, tfs, (int)a.tf_n, r, yz);
                 } else if (multi_level(kMode) && a.lev.n == 0) {
                     seg = make_float4(0.f, 0.f, 0.f, 0.f); // no levels: nothing to draw
                 } else if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces) {
@@ -167,7 +167,7 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
 // identical; the gain is for blocks a few pixels wide, which cover a small
 // part of a bin (the wave-wide walk then marches few of its lanes).
 template <class BrickT>
-__device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4* tfs)
+__device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4* tfs, const uint2* yz)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t bx = bin % a.nbx, by = bin / a.nbx;
@@ -210,7 +210,7 @@ __device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4*
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
                 } else if (a.fast) {
-                    seg = march_fast(p, brick, tfs, (int)a.tf_n, r);
+                    seg = march_fast(p, brick, tfs, (int)a.tf_n, r, yz);
                 } else {
                     seg = march_impl<BrickT, false>(p, brick, tfs, (int)a.tf_n, r, nullptr);
                 }
@@ -233,7 +233,7 @@ __device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4*
 // kLanes: the per-lane walk (Argb only) for bins listing at least
 // a.lanes_min blocks, the bin walk for the others.
 template <class BrickT, uint32_t kMode, bool kLanes>
-__device__ void render_bins(const FrameArgs& a, const float4* tfs, const float* lvs)
+__device__ void render_bins(const FrameArgs& a, const float4* tfs, const float* lvs, const uint2* yz)
 {
     for (;;) {
         uint32_t bin = 0;
@@ -244,11 +244,11 @@ __device__ void render_bins(const FrameArgs& a, const float4* tfs, const float* 
             break;
         if constexpr (kLanes) {
             if (a.bin_off[bin + 1] - a.bin_off[bin] >= a.lanes_min) {
-                render_bin_lanes<BrickT>(a, bin, tfs);
+                render_bin_lanes<BrickT>(a, bin, tfs, yz);
                 continue;
             }
         }
-        render_bin<BrickT, kMode>(a, bin, tfs, lvs);
+        render_bin<BrickT, kMode>(a, bin, tfs, lvs, yz);
     }
 }
 
@@ -270,6 +270,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
     __shared__ __align__(16) float A[kTileFloats]; // interpolation tile, then the colour table
     __shared__ float lat[64];
     __shared__ float lvs[kMaxLevels];
+    __shared__ uint2 yz_s[2 * kYZTabEntries]; // TILED4 y/z offsets (fast Argb)
     __shared__ uint32_t item_s;
     float4* tfs = reinterpret_cast<float4*>(A);
     uint32_t my_tiles = 0; // brick tiles done by this workgroup
@@ -303,9 +304,14 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
     stage_tf(tfs, a.tf, a.tf_n, threadIdx.x, kT);
     if (multi_level(kMode) && threadIdx.x < a.lev.n)
         lvs[threadIdx.x] = a.lev.v[threadIdx.x];
+    const uint2* yz = nullptr;
+    if constexpr (std::is_same<BrickT, Tiled4>::value && kMode == kModeArgb) {
+        stage_yz_tables(yz_s, threadIdx.x, kT);
+        yz = yz_s;
+    }
     __syncthreads();
     if ((threadIdx.x >> 6) < a.render_waves) // fewer concurrent rays can be faster (vector L1)
-        render_bins<BrickT, kMode, kLanes>(a, tfs, lvs);
+        render_bins<BrickT, kMode, kLanes>(a, tfs, lvs, yz);
 }
 
 // Frames of resident bricks: screen bins only, no interpolation LDS, so
@@ -315,11 +321,17 @@ __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 {
     extern __shared__ float4 tfs_dyn[]; // tf_n + 1 entries
     __shared__ float lvs[kMaxLevels];
+    __shared__ uint2 yz_s[2 * kYZTabEntries]; // TILED4 y/z offsets (fast Argb)
     stage_tf(tfs_dyn, a.tf, a.tf_n, threadIdx.x, kRT);
     if (multi_level(kMode) && threadIdx.x < a.lev.n)
         lvs[threadIdx.x] = a.lev.v[threadIdx.x];
+    const uint2* yz = nullptr;
+    if constexpr (std::is_same<BrickT, Tiled4>::value && kMode == kModeArgb) {
+        stage_yz_tables(yz_s, threadIdx.x, kRT);
+        yz = yz_s;
+    }
     __syncthreads();
-    render_bins<BrickT, kMode, kLanes>(a, tfs_dyn, lvs);
+    render_bins<BrickT, kMode, kLanes>(a, tfs_dyn, lvs, yz);
 }
 
 // Worker share in one launch: the blocks (back to front) of one share are
@@ -353,9 +365,12 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
 {
     extern __shared__ float4 tfs_share[]; // tf_n + 1 entries (and padding)
     __shared__ float lvs[kMaxLevels];
+    __shared__ uint2 yz_s[2 * kYZTabEntries]; // TILED4 y/z offsets (fast Argb)
     stage_tf(tfs_share, tf, tf_n, threadIdx.x, kShareT);
     if (multi_level(kMode) && threadIdx.x < lev.n)
         lvs[threadIdx.x] = lev.v[threadIdx.x];
+    if constexpr (std::is_same<BrickT, Tiled4>::value && kMode == kModeArgbFast)
+        stage_yz_tables(yz_s, threadIdx.x, kShareT);
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t x0 = blockIdx.x * 16u + (wave & 1u) * 8u, y0 = blockIdx.y * 16u + (wave >> 1) * 8u;
@@ -382,6 +397,8 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                     value = mip_merge(value, march_mip(p, brick, r), p);
                 else if (multi_level(kMode) && lev.n == 0)
                     value = 0u; // renderIsosurfaces.hpp:33-34
+                else if (kMode == kModeArgbFast)
+                    value = blend_above_u32(value, march_fast(p, brick, tfs_share, (int)tf_n, r, yz_s));
                 else
                     value = blend_above_u32(value, segment<kMode>(p, brick, tfs_share, (int)tf_n,
                                                                   iso_opacity, r, lvs, lev.n, sec));

@@ -356,8 +356,8 @@ extern "C" uint64_t rv_brick_floats(uint32_t depth, uint32_t layout)
         return 0;
     const uint64_t n = (1ull << depth) + 1ull;
     if (layout == RV_BRICK_TILED4) {
-        const uint64_t G = (n + 6) / 4; // x-positions 0 .. n + 2 (edge copies on both sides)
-        return G * G * G * 64;
+        const uint64_t G = ((n + 1) >> 2) + 1; // x-pairs start at positions 0 .. n + 1
+        return G * G * G * 80;
     }
     return layout == RV_BRICK_LINEAR ? n * n * n : 0;
 }

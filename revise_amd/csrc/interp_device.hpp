@@ -295,34 +295,35 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
 
     // 4. store the owned half-open box (+ far faces of the brick)
     if (job.tiled) {
-        // Micro-bricks of 4 x 4 x 4 positions, 64 consecutive floats each (x
-        // fastest inside and over the G^3 of them).  Along y and z node c is
-        // at position c; along x at c + 1, and x-positions 0 and n .. 4G - 1
-        // hold the edge nodes 0 and n - 1, so the x-pair (c, c + 1) of every
-        // clamped sample lies in the brick.  A tile writes the positions of
-        // the nodes it owns (x: x0 + 1 .. x0 + ex; the first x-tile also
-        // position 0, the last the edge positions; y/z: its owned box, the
-        // last tile up to 4G with padding rows, written NaN and never read).
-        // Along y and z tile boundaries are multiples of 4, so each
-        // micro-brick row belongs to one tile; along x (bricks of several
-        // x-tiles) the row holding position x0 is shared with the previous
-        // tile, and each tile writes its own floats of it.
-        const uint32_t G = (g.n + 6u) >> 2;
+        // Micro-bricks of 4 x 4 x 4 cells, 80 consecutive floats each: 16
+        // rows (y, then z) of 5 x-positions, 4m .. 4m + 4 in micro-brick m
+        // (neighbours along x share the position 4m + 4, so a cell's x-pair
+        // never leaves its row), micro-bricks x fastest over G^3 of them.
+        // Along y and z node c is at position c; along x at c + 1, and
+        // x-positions 0 and n .. 4G hold the edge nodes 0 and n - 1, so the
+        // x-pair (c, c + 1) of every clamped sample lies in the brick.  A tile
+        // writes every copy of the positions of the nodes it owns (x: x0 + 1
+        // .. x0 + ex; the first x-tile also position 0, the last the edge
+        // positions; y/z: its owned box, the last tile up to 4G with padding
+        // rows, written NaN and never read).  Along y and z tile boundaries
+        // are multiples of 4, so each row belongs to one tile; along x
+        // (bricks of several x-tiles) the micro-bricks at a seam are shared
+        // with the previous tile, each writing its own floats of them.
+        const uint32_t G = ((g.n + 1u) >> 2) + 1u;
         const bool lastx = tx + 1u == g.tiles_x, lasty = ty + 1u == g.tiles_yz, lastz = tz + 1u == g.tiles_yz;
         const uint32_t ex = lastx ? rowlen : X;
         const uint32_t ey = lasty ? tp1 : T;
         const uint32_t ez = lastz ? tp1 : T;
-        const uint32_t p_lo = tx == 0u ? 0u : x0 + 1u, p_hi = lastx ? 4u * G : x0 + ex + 1u;
-        const uint32_t mx0 = p_lo >> 2, my0 = y0 >> 2, mz0 = z0 >> 2;
-        const uint32_t nmx = ((p_hi + 3u) >> 2) - mx0;
+        const uint32_t p_lo = tx == 0u ? 0u : x0 + 1u, p_hi = lastx ? 4u * G + 1u : x0 + ex + 1u;
+        const uint32_t mx0 = p_lo == 0u ? 0u : (p_lo - 1u) >> 2, my0 = y0 >> 2, mz0 = z0 >> 2;
+        const uint32_t nmx = min((p_hi - 1u) >> 2, G - 1u) + 1u - mx0;
         const uint32_t nmy = (lasty ? G : ((y0 + ey + 3u) >> 2)) - my0;
         const uint32_t nmz = (lastz ? G : ((z0 + ez + 3u) >> 2)) - mz0;
-        // one micro-brick row (4 positions along x, 16 bytes) per lane: a
-        // wave stores 4 whole micro-bricks, and its LDS reads of 4
-        // consecutive x meet few bank conflicts.  The wave's first
-        // micro-brick index is wave-uniform (NT is a multiple of 64), so its
-        // decomposition runs on the scalar unit; the lanes' offsets 0..3 are
-        // carried along x.
+        // one micro-brick row (5 positions along x, 20 bytes) per lane: a
+        // wave stores 4 whole micro-bricks (1280 contiguous bytes).  The
+        // wave's first micro-brick index is wave-uniform (NT is a multiple
+        // of 64), so its decomposition runs on the scalar unit; the lanes'
+        // offsets 0..3 are carried along x.
         const uint32_t total = nmx * nmy * nmz * 16u;
         const int nlast = (int)g.n - 1;
         for (uint32_t q = threadIdx.x; q < total; q += NT) {
@@ -341,10 +342,10 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
             mz += mz0;
             const uint32_t y = (my << 2) | (r & 3u), z = (mz << 2) | (r >> 2), pb = mx << 2;
             const bool row_ok = y < y0 + ey && z < z0 + ez;
-            float o[4];
-            bool own[4];
+            float o[5];
+            bool own[5];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < 5; ++c) {
                 const uint32_t pos = pb + (uint32_t)c;
                 own[c] = pos >= p_lo && pos < p_hi;
                 const int node = min(max((int)pos - 1, 0), nlast); // in [x0, x0 + ex) when owned
@@ -356,14 +357,15 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
                 }
                 o[c] = v;
             }
-            float* dst = job.dense + (((mz * G + my) * G + mx) << 6) + (r << 2);
-            if (own[0] && own[1] && own[2] && own[3]) {
-                typedef float f32x4 __attribute__((ext_vector_type(4)));
-                const f32x4 row = {o[0], o[1], o[2], o[3]};
-                __builtin_nontemporal_store(row, reinterpret_cast<f32x4*>(dst));
+            float* dst = job.dense + ((mz * G + my) * G + mx) * 80u + r * 5u;
+            if (own[0] && own[1] && own[2] && own[3] && own[4]) {
+                typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+                const f32x4u head = {o[0], o[1], o[2], o[3]};
+                __builtin_nontemporal_store(head, reinterpret_cast<f32x4u*>(dst));
+                __builtin_nontemporal_store(o[4], dst + 4);
             } else {
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < 5; ++c)
                     if (own[c])
                         __builtin_nontemporal_store(o[c], dst + c);
             }

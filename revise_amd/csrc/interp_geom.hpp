@@ -47,15 +47,18 @@ RV_HD inline Geom geom_of(uint32_t d)
     return g;
 }
 
-// Index of node (x, y, z) of a brick of n nodes per axis in the 4 x 4 x 4
-// micro-brick layout (G = ceil((n + 3) / 4)): micro-bricks of 64 floats (x
-// fastest, then y, z), ordered x fastest over G^3 of them; along x node x is
-// at position x + 1 (positions 0 and n .. 4G - 1 repeat the edge nodes),
-// along y and z at y, z.
+// Index of node (x, y, z) of a brick of n nodes per axis in the micro-brick
+// layout (G = floor((n + 1) / 4) + 1): micro-bricks of 4 x 4 x 4 cells, 80
+// floats (16 rows of 5 x-positions; micro-brick m holds x-positions 4m ..
+// 4m + 4, so neighbours share one), ordered x fastest over G^3 of them;
+// along x node x is at position x + 1 (positions 0 and n .. 4G repeat the
+// edge nodes), along y and z at y, z.  This is the copy in micro-brick
+// (x + 1) / 4 (the other copy of a position 4m is the last of micro-brick
+// m - 1).
 RV_HD inline uint32_t tiled4_index(uint32_t x, uint32_t y, uint32_t z, uint32_t G)
 {
     const uint32_t px = x + 1u;
-    return ((((z >> 2) * G + (y >> 2)) * G + (px >> 2)) << 6) + (((z & 3u) << 4) | ((y & 3u) << 2) | (px & 3u));
+    return (((z >> 2) * G + (y >> 2)) * G + (px >> 2)) * 80u + (z & 3u) * 20u + (y & 3u) * 5u + (px & 3u);
 }
 
 // LDS floats a tile needs: its (X+1)(T+1)^2 box, or the (m+1)^3 coarse

@@ -4,6 +4,8 @@
 // operation order, so every kernel of this library produces the same bits.
 #pragma once
 
+#include <type_traits>
+
 #include "../../include/revise_hip.h"
 #include "interp_geom.hpp"
 
@@ -187,20 +189,19 @@ __device__ __forceinline__ Taps fetch_corners(const Tiled4* __restrict__ brick, 
     t.dx = (uint32_t)(x1 - x0);
     t.dy = (uint32_t)(y1 - y0) * n;
     t.dz = (uint32_t)(z1 - z0) * n * n;
-    // tiled4_index split per axis: (c >> 2) * (64 G^axis) + (c & 3) * 4^axis,
-    // the three terms summed (their bit fields do not overlap the sum's
-    // carries: the low 6 bits hold the in-micro-brick part).  One load per
-    // corner: the exact sequence is bound by its arithmetic, and the pair
-    // loads of the fast fetch cost it more instructions than they save.
-    // (x-positions are nodes + 1)
-    const uint32_t G = ((uint32_t)n + 6u) >> 2, sy = G << 6, sz = (G * G) << 6;
+    // tiled4_index split per axis, (c >> 2) * (micro-brick stride) + (c & 3)
+    // * (node stride), the three terms summed (x-positions are nodes + 1).
+    // One load per corner: the exact sequence is bound by its arithmetic,
+    // and the pair loads of the fast fetch cost it more instructions than
+    // they save.
+    const uint32_t G = (((uint32_t)n + 1u) >> 2) + 1u, sy = G * 80u, sz = G * G * 80u;
     const uint32_t px0 = (uint32_t)x0 + 1u, px1 = (uint32_t)x1 + 1u;
-    const uint32_t X0 = ((px0 >> 2) << 6) | (px0 & 3u);
-    const uint32_t X1 = ((px1 >> 2) << 6) | (px1 & 3u);
-    const uint32_t Y0 = __umul24((uint32_t)y0 >> 2, sy) + (((uint32_t)y0 & 3u) << 2);
-    const uint32_t Y1 = __umul24((uint32_t)y1 >> 2, sy) + (((uint32_t)y1 & 3u) << 2);
-    const uint32_t Z0 = __umul24((uint32_t)z0 >> 2, sz) + (((uint32_t)z0 & 3u) << 4);
-    const uint32_t Z1 = __umul24((uint32_t)z1 >> 2, sz) + (((uint32_t)z1 & 3u) << 4);
+    const uint32_t X0 = (px0 >> 2) * 80u + (px0 & 3u);
+    const uint32_t X1 = (px1 >> 2) * 80u + (px1 & 3u);
+    const uint32_t Y0 = __umul24((uint32_t)y0 >> 2, sy) + ((uint32_t)y0 & 3u) * 5u;
+    const uint32_t Y1 = __umul24((uint32_t)y1 >> 2, sy) + ((uint32_t)y1 & 3u) * 5u;
+    const uint32_t Z0 = __umul24((uint32_t)z0 >> 2, sz) + ((uint32_t)z0 & 3u) * 20u;
+    const uint32_t Z1 = __umul24((uint32_t)z1 >> 2, sz) + ((uint32_t)z1 & 3u) * 20u;
     const gptr<char> b = global_of(reinterpret_cast<const char*>(brick));
     const uint32_t o[8] = {X0 + Y0 + Z0, X1 + Y0 + Z0, X0 + Y1 + Z0, X1 + Y1 + Z0,
                            X0 + Y0 + Z1, X1 + Y0 + Z1, X0 + Y1 + Z1, X1 + Y1 + Z1};
@@ -511,65 +512,106 @@ __device__ __forceinline__ void fetch_fast(const T* __restrict__ brick, int n, f
 // brick address (global loads with a scalar base, no 64-bit arithmetic per
 // lane).  Along x the brick holds node c at position c + 1 and the edge
 // nodes at the positions beyond, so the pair of a sample is positions
-// (i + 1, i + 2) for its unclamped cell i in [-1, n], with no clamp: the
-// pair of a clamped sample holds the edge node twice.  It lies in one
-// 16-byte micro-brick row unless position i + 1 ends its row; then the
-// second node is the first of the next micro-brick's row, 244 bytes on,
-// loaded by the lanes that need it into registers of their own (so no load
-// waits for another).  Fewer load instructions per sample mean fewer L1 tag
-// lookups, which bound this march.
+// (i + 1, i + 2) for its unclamped cell i in [-1, n], with no clamp (the
+// pair of a clamped sample holds the edge node twice); micro-brick rows
+// hold 5 x-positions, so the pair is one 8-byte load inside one row.  Four
+// load instructions per sample: fewer L1 tag lookups, which bound this
+// march.
+__device__ __forceinline__ uint32_t tiled4_x_bytes(uint32_t c) { return __umul24(c >> 2, 320u) + ((c & 3u) << 2); }
+
+__device__ __forceinline__ void tiled4_load_rows(const Tiled4* __restrict__ brick, const uint32_t row[4],
+                                                 float v[8])
+{
+    const gptr<char> b = global_of(reinterpret_cast<const char*>(brick));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint64_t pr = *reinterpret_cast<gptr<pair32_t>>(b + row[k]);
+        v[2 * k] = __uint_as_float((uint32_t)pr);
+        v[2 * k + 1] = __uint_as_float((uint32_t)(pr >> 32));
+    }
+}
+
 __device__ __forceinline__ void fetch_fast(const Tiled4* __restrict__ brick, int n, float tx, float ty, float tz,
                                            float v[8], float& fx, float& fy, float& fz)
 {
     const float vx = __builtin_amdgcn_fmed3f(tx - 0.5f, -1.f, (float)n);
     const float flx = floorf(vx);
     fx = vx - flx;
-    const uint32_t cx = (uint32_t)((int)flx + 1); // x-position of the pair, 0 .. n + 1
+    const uint32_t X = tiled4_x_bytes((uint32_t)((int)flx + 1)); // pair at x-positions i + 1, i + 2
     uint32_t y0, y1, z0, z1;
     fast_axis(ty, n, y0, y1, fy);
     fast_axis(tz, n, z0, z1, fz);
-    // byte offset of position c along an axis: (c >> 2) * (micro-brick
-    // stride) + (c & 3) * (node stride) = (c >> 2) * (micro-brick stride - 4
-    // node strides) + c * (node stride)
-    const uint32_t G = ((uint32_t)n + 6u) >> 2, sy = (G << 8) - 64u, sz = ((G * G) << 8) - 256u;
-    const bool cross = (cx & 3u) == 3u; // the pair's second position starts the next micro-brick
-    const uint32_t X = __umul24(cx >> 2, 240u) + (cx << 2);
-    const uint32_t Y0 = __umul24(y0 >> 2, sy) + (y0 << 4);
-    const uint32_t Y1 = __umul24(y1 >> 2, sy) + (y1 << 4);
-    const uint32_t Z0 = __umul24(z0 >> 2, sz) + (z0 << 6);
-    const uint32_t Z1 = __umul24(z1 >> 2, sz) + (z1 << 6);
-    const gptr<char> b = global_of(reinterpret_cast<const char*>(brick));
+    const uint32_t G = (((uint32_t)n + 1u) >> 2) + 1u, sy = G * 320u, sz = G * G * 320u;
+    const uint32_t Y0 = __umul24(y0 >> 2, sy) + (y0 & 3u) * 20u;
+    const uint32_t Y1 = __umul24(y1 >> 2, sy) + (y1 & 3u) * 20u;
+    const uint32_t Z0 = __umul24(z0 >> 2, sz) + (z0 & 3u) * 80u;
+    const uint32_t Z1 = __umul24(z1 >> 2, sz) + (z1 & 3u) * 80u;
     const uint32_t XY0 = X + Y0, XY1 = X + Y1;
     const uint32_t row[4] = {XY0 + Z0, XY1 + Z0, XY0 + Z1, XY1 + Z1};
-    // a crossing pair's 8-byte load reads its first node and the next float
-    // of the same micro-brick, or the first of the next micro-brick: inside
-    // the brick (position cx + 1 <= n + 2 <= 4G - 1)
-    float lo[4], hi[4], nx[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint64_t pr = *reinterpret_cast<gptr<pair32_t>>(b + row[k]);
-        lo[k] = __uint_as_float((uint32_t)pr);
-        hi[k] = __uint_as_float((uint32_t)(pr >> 32));
+    tiled4_load_rows(brick, row, v);
+}
+
+// Corner-row byte offsets of TILED4 bricks along y and z, per depth 0..6
+// (n = 2^d + 1 <= 65 nodes), for the fast fetch: entry j in [0, n + 1] of
+// depth d holds the offsets of the corners (clamp(j - 1), clamp(j)) of a
+// sample whose unclamped cell (fast_axis) is j - 1, so one LDS load replaces
+// the clamps and the micro-brick decomposition of both corners.  The y
+// tables of depths 0..6 come first, then the z tables, kYZTabEntries each.
+constexpr uint32_t kYZTabDepths = 7;
+constexpr uint32_t kYZTabEntries = 148; // sum over d = 0..6 of 2^d + 3
+
+__host__ __device__ constexpr uint32_t yz_tab_base(uint32_t d) { return ((1u << d) - 1u) + 3u * d; }
+
+// The tables into LDS (2 kYZTabEntries entries), tid-th of nt threads.
+__device__ inline void stage_yz_tables(uint2* tab, uint32_t tid, uint32_t nt)
+{
+    for (uint32_t k = tid; k < 2u * kYZTabEntries; k += nt) {
+        const uint32_t axis = k >= kYZTabEntries ? 1u : 0u;
+        const uint32_t e = k - axis * kYZTabEntries;
+        uint32_t d = 0;
+        while (d + 1u < kYZTabDepths && yz_tab_base(d + 1u) <= e)
+            ++d;
+        const int n = (1 << d) + 1, j = (int)(e - yz_tab_base(d));
+        const uint32_t G = (((uint32_t)n + 1u) >> 2) + 1u;
+        const uint32_t ms = axis ? G * G * 320u : G * 320u, ns = axis ? 80u : 20u; // micro-brick, node strides
+        const uint32_t c0 = (uint32_t)min(max(j - 1, 0), n - 1), c1 = (uint32_t)min(j, n - 1);
+        tab[k] = make_uint2((c0 >> 2) * ms + (c0 & 3u) * ns, (c1 >> 2) * ms + (c1 & 3u) * ns);
     }
-    if (cross) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            nx[k] = *reinterpret_cast<gptr<float>>(b + row[k] + 244);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        v[2 * k] = lo[k];
-        v[2 * k + 1] = cross ? nx[k] : hi[k];
-    }
+}
+
+// fetch_fast of a TILED4 brick of depth <= 6 with the y and z offsets from
+// the tables of its depth (ytab, ztab): the same corners and weights.
+__device__ __forceinline__ void fetch_fast_tab(const Tiled4* __restrict__ brick, int n, float tx, float ty, float tz,
+                                               const uint2* ytab, const uint2* ztab, float v[8], float& fx,
+                                               float& fy, float& fz)
+{
+    const float vx = __builtin_amdgcn_fmed3f(tx - 0.5f, -1.f, (float)n);
+    const float vy = __builtin_amdgcn_fmed3f(ty - 0.5f, -1.f, (float)n);
+    const float vz = __builtin_amdgcn_fmed3f(tz - 0.5f, -1.f, (float)n);
+    const float flx = floorf(vx), fly = floorf(vy), flz = floorf(vz);
+    fx = vx - flx;
+    fy = vy - fly;
+    fz = vz - flz;
+    const uint32_t X = tiled4_x_bytes((uint32_t)((int)flx + 1));
+    const uint2 Y = ytab[(int)fly + 1], Z = ztab[(int)flz + 1];
+    const uint32_t XY0 = X + Y.x, XY1 = X + Y.y;
+    const uint32_t row[4] = {XY0 + Z.x, XY1 + Z.x, XY0 + Z.y, XY1 + Z.y};
+    tiled4_load_rows(brick, row, v);
 }
 
 __device__ __forceinline__ float lerp_fast(float a, float b, float t) { return fmaf(t, b - a, a); }
 
-template <class T>
-__device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* __restrict__ brick,
-                                             const float4* tf, int tf_n, RaySeg r)
+template <class T, bool kTab>
+__device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, const T* __restrict__ brick,
+                                                  const float4* tf, int tf_n, RaySeg r, const uint2* yz)
 {
     const int n = (int)p.nodes;
+    const uint2* ytab = nullptr;
+    const uint2* ztab = nullptr;
+    if constexpr (kTab) {
+        ytab = yz + yz_tab_base(p.tex_depth);
+        ztab = ytab + kYZTabEntries;
+    }
     // table coordinate u N - 0.5 with u = (v - fmin) / (fmax - fmin)
     const float ta = (float)tf_n / (p.fmax - p.fmin);
     const float tb = fmaf(-p.fmin, ta, -0.5f);
@@ -583,8 +625,12 @@ __device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* 
         float c[kBatch][8], fx[kBatch], fy[kBatch], fz[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            fetch_fast(brick, n, (r.px - p.tex_origin[0]) * p.tcf, (r.py - p.tex_origin[1]) * p.tcf,
-                       (r.pz - p.tex_origin[2]) * p.tcf, c[s], fx[s], fy[s], fz[s]);
+            const float tx = (r.px - p.tex_origin[0]) * p.tcf, ty = (r.py - p.tex_origin[1]) * p.tcf,
+                        tz = (r.pz - p.tex_origin[2]) * p.tcf;
+            if constexpr (kTab)
+                fetch_fast_tab(brick, n, tx, ty, tz, ytab, ztab, c[s], fx[s], fy[s], fz[s]);
+            else
+                fetch_fast(brick, n, tx, ty, tz, c[s], fx[s], fy[s], fz[s]);
             r.px = r.px + r.sx;
             r.py = r.py + r.sy;
             r.pz = r.pz + r.sz;
@@ -626,6 +672,20 @@ __device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* 
         return make_float4(0.f, 0.f, 0.f, 0.f);
     const float ia = 1.f / A;
     return make_float4(cr * ia, cg * ia, cb * ia, A);
+}
+
+// The fast sequence over one block; yz: the LDS offset tables
+// (stage_yz_tables) or nullptr.  TILED4 bricks of depth <= 6 read their y
+// and z offsets from the tables.
+template <class T>
+__device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* __restrict__ brick,
+                                             const float4* tf, int tf_n, RaySeg r, const uint2* yz = nullptr)
+{
+    if constexpr (std::is_same<T, Tiled4>::value) {
+        if (yz != nullptr && p.tex_depth < kYZTabDepths)
+            return march_fast_impl<T, true>(p, brick, tf, tf_n, r, yz);
+    }
+    return march_fast_impl<T, false>(p, brick, tf, tf_n, r, yz);
 }
 
 // ---------------------------------------------------------------- field modes

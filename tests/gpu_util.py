@@ -67,3 +67,19 @@ def oracle_blocks_frame(orc, items, tf, W, H, init=None, rows=None, threads=None
     for t in ths:
         t.join()
     return vp
+
+
+def tiled4_expected(lin, n):
+    """The micro-brick (TILED4) brick holding the x-fastest brick `lin` of
+    n^3 nodes, as the interpolation writes it: G^3 micro-bricks of 16 rows
+    (y, then z) of 5 x-positions, micro-brick m holding x-positions 4m ..
+    4m + 4; node x at x-position x + 1, the x-positions beyond the brick
+    holding the edge nodes; rows beyond n - 1 along y or z NaN."""
+    G = ((n + 1) >> 2) + 1
+    L = np.asarray(lin, np.float32).reshape(n, n, n)
+    P = np.full((4 * G, 4 * G, n), np.nan, np.float32)
+    P[:n, :n, :] = L
+    zz = 4 * np.arange(G)[:, None] + np.arange(4)[None, :]
+    node = np.clip(4 * np.arange(G)[:, None] + np.arange(5)[None, :] - 1, 0, n - 1)
+    E = P[zz[:, None, None, :, None, None], zz[None, :, None, None, :, None], node[None, None, :, None, None, :]]
+    return E.reshape(-1)

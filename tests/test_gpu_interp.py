@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from revise_amd import api, datasets
-from tests.gpu_util import bits_equal_nan, say, to_dev
+from tests.gpu_util import bits_equal_nan, say, tiled4_expected, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -215,11 +215,12 @@ def test_interp_batch_full_small_bricks(orc):
                                               (33, 4, 0.4, 0.02), (34, 5, 0.5, 0.0), (35, 6, 0.35, 0.01),
                                               (36, 6, 1.0, 0.0), (37, 7, 0.3, 0.01), (38, 8, 0.1, 0.0)])
 def test_batch_tiled_layout_holds_the_linear_values(orc, seed, depth, p, pnf):
-    """A batch writing the 4^3 micro-brick layout stores, at each node's
-    tiled index (x-position x + 1), the value the x-fastest layout stores
-    (NaN where undefined); x-positions 0 and n .. 4G - 1 repeat the edge
-    nodes 0 and n - 1; the other padding is NaN.  Depths 7 and 8 have
-    several x-tiles, which share the micro-brick rows at their seams."""
+    """A batch writing the micro-brick layout stores every float of the
+    brick as tiled4_expected lays out the x-fastest brick's values (NaN
+    where undefined): node x at x-position x + 1, rows of 5 x-positions
+    overlapping by one, the edge nodes repeated beyond the brick, NaN rows
+    beyond it along y and z.  Depths 7 and 8 have several x-tiles, which
+    share the micro-bricks at their seams."""
     import torch
     d, n2i, vals = _case(seed, depth, p, pnf)
     plan = api.interp_plan(n2i, d)
@@ -230,23 +231,10 @@ def test_batch_tiled_layout_holds_the_linear_values(orc, seed, depth, p, pnf):
     api.InterpBatch([(sp, plan, til, d)], api.BRICK_TILED4).run()
     torch.cuda.synchronize()
     n = 2 ** d + 1
-    G = (n + 6) // 4
-    assert til.numel() == G ** 3 * 64
-
-    def index(px, y, z):
-        return ((((z >> 2) * G + (y >> 2)) * G + (px >> 2)) << 6) + (((z & 3) << 4) | ((y & 3) << 2) | (px & 3))
-
-    z, y, px = np.meshgrid(np.arange(n), np.arange(n), np.arange(4 * G), indexing="ij")
-    idx = index(px, y, z)
-    node = np.clip(px - 1, 0, n - 1)
-    t = til.cpu().numpy()
-    L = lin.cpu().numpy().reshape(n, n, n)
-    assert bits_equal_nan(t[idx.reshape(-1)], L[z, y, node].reshape(-1))
+    G = ((n + 1) >> 2) + 1
+    assert til.numel() == G ** 3 * 80
+    assert bits_equal_nan(til.cpu().numpy(), tiled4_expected(lin.cpu().numpy(), n))
     assert bits_equal_nan(lin.cpu().numpy(), orc.interpolate(vals, n2i, d))
-    # padding rows and planes beyond node n - 1 along y and z (never read)
-    pad = np.ones(t.size, bool)
-    pad[idx.reshape(-1)] = False
-    assert np.all(np.isnan(t[pad]))
 
 
 def test_cpu_interp_configuration_brick_bit_exact(orc):
@@ -268,8 +256,4 @@ def test_cpu_interp_configuration_brick_bit_exact(orc):
     til = torch.empty(api.brick_floats(d, api.BRICK_TILED4), dtype=torch.float32, device="cuda")
     api.InterpBatch([(sp, api.interp_plan(n2i, d), til, d)], api.BRICK_TILED4).run()
     t = til.cpu().numpy()
-    n = 2 ** d + 1
-    G = (n + 6) // 4
-    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n) + 1, indexing="ij")  # x-position x + 1
-    ti = ((((z >> 2) * G + (y >> 2)) * G + (x >> 2)) << 6) | ((z & 3) << 4) | ((y & 3) << 2) | (x & 3)
-    assert np.array_equal(t[ti.reshape(-1)].view(np.uint32), exp.view(np.uint32))
+    assert bits_equal_nan(t, tiled4_expected(exp, 2 ** d + 1))

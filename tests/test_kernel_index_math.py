@@ -38,16 +38,17 @@ def test_fp32_reciprocal_division_is_exact():
 
 # The micro-brick store of interp_tile (interp_device.hpp, job.tiled), restated
 # with numpy: per tile, lane q of the loop stores the positions it owns of
-# micro-brick row q & 15 of micro-brick q >> 4 of the tile's range, the
+# row q & 15 (5 x-positions) of micro-brick q >> 4 of the tile's range, the
 # wave's first micro-brick decomposed once and the lane's offset 0..3
-# carried along x.  Along x node c is at position c + 1 and positions 0 and
-# n .. 4G - 1 repeat the edge nodes.
+# carried along x.  Along x node c is at position c + 1, micro-brick m holds
+# positions 4m .. 4m + 4, and positions 0 and n .. 4G repeat the edge nodes.
 def _tiled_store_cover(d):
     n = (1 << d) + 1; t = min(d, 4); T = 1 << t; X = (1 << d) if d < 6 else 64
     tiles_x = (1 << d) // X; tiles_yz = (1 << d) // T
-    G = (n + 6) // 4
-    written = np.zeros(G**3 * 64, np.int32)
-    owner = -np.ones(G**3 * 64, np.int64)
+    G = ((n + 1) >> 2) + 1
+    size = G**3 * 80
+    written = np.zeros(size, np.int32)
+    owner = -np.ones(size, np.int64)
     rowlen, tp1 = X + 1, T + 1
     for tz in range(tiles_yz):
         for ty in range(tiles_yz):
@@ -58,9 +59,10 @@ def _tiled_store_cover(d):
                 ey = tp1 if lasty else T
                 ez = tp1 if lastz else T
                 p_lo = 0 if tx == 0 else x0 + 1
-                p_hi = 4 * G if lastx else x0 + ex + 1
-                mx0, my0, mz0 = p_lo >> 2, y0 >> 2, z0 >> 2
-                nmx = ((p_hi + 3) >> 2) - mx0
+                p_hi = 4 * G + 1 if lastx else x0 + ex + 1
+                mx0 = 0 if p_lo == 0 else (p_lo - 1) >> 2
+                my0, mz0 = y0 >> 2, z0 >> 2
+                nmx = min((p_hi - 1) >> 2, G - 1) + 1 - mx0
                 nmy = (G if lasty else ((y0 + ey + 3) >> 2)) - my0
                 nmz = (G if lastz else ((z0 + ez + 3) >> 2)) - mz0
                 total = nmx * nmy * nmz * 16
@@ -80,11 +82,11 @@ def _tiled_store_cover(d):
                 mx += mx0; my += my0; mz += mz0
                 y = (my << 2) | (r & 3); z = (mz << 2) | (r >> 2); pb = mx << 2
                 row_ok = (y < y0 + ey) & (z < z0 + ez)
-                base = (((mz * G + my) * G + mx) << 6) + (r << 2)
+                base = ((mz * G + my) * G + mx) * 80 + r * 5
                 tile_id = (tz * tiles_yz + ty) * tiles_x + tx
-                for c in range(4):
+                for c in range(5):
                     addr = base + c
-                    assert (addr >= 0).all() and (addr < G**3 * 64).all(), d
+                    assert (addr >= 0).all() and (addr < size).all(), d
                     pos = pb + c
                     own = (pos >= p_lo) & (pos < p_hi)
                     node = np.clip(pos - 1, 0, n - 1)
@@ -99,9 +101,6 @@ def _tiled_store_cover(d):
                     # the LDS is read exactly for the brick's real rows
                     real = (y < n) & (z < n)
                     assert (valid == (real & own)).all(), ("valid != real row", d)
-                    # the tiled index of (position, y, z) is the address
-                    ti = ((((z >> 2) * G + (y >> 2)) * G + (pos >> 2)) << 6) + (((z & 3) << 4) | ((y & 3) << 2) | (pos & 3))
-                    assert (ti == addr).all()
     assert (written == 1).all(), d
 
 
