@@ -440,6 +440,54 @@ def large_share_bench(torch, args, tf_np, log):
     return res
 
 
+def mixed_bench(torch, args, tf, log):
+    """Config 3: 2000 blocks of levels 2-6 (fill 5-100%), 1920x1080: the
+    batched persistent frames replayed from a hipGraph over args.cams
+    cameras, beside the drop-in forms on the same blocks (the fused share
+    path -- one interpolation batch and one share launch with the per-block
+    8-bit composite -- and one launch pair per block)."""
+    from revise_amd import configs, datasets
+    t0 = time.time()
+    field3 = datasets.Field(SEED + 3, n_gauss=8, noise_octaves=3)
+    wl3 = configs.mixed(torch, field3, nblocks=2000, depth=6, seed=SEED + 3)
+    cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in
+                     zip(np.random.default_rng(SEED + 3).uniform(0, 360, args.cams),
+                         np.random.default_rng(SEED + 4).uniform(-45, 45, args.cams))])
+    torch.cuda.synchronize()
+    log("config 3 setup %.1fs: %d blocks, %d sparse nodes" % (time.time() - t0, len(wl3.blocks),
+                                                              wl3.sparse_nodes))
+    fp3 = frame_plan(torch, wl3, tf, Q2, CUT2, blocks=tiled_blocks(torch, wl3))
+    fp3.set_cameras(cams)
+    argb3 = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    fp3.autotune(argb3, False, TUNE_CACHE)  # grid of the resident-brick kernel (split frames use it)
+    tile3 = fp3.autotune(argb3, True, TUNE_CACHE)
+    frames = torch.empty((args.cams, H, W), dtype=torch.int32, device="cuda")
+    fp3.graph_build(frames, H * W, True)
+    fp3.graph_launch()
+    torch.cuda.synchronize()
+    t_graph = events_time(torch, fp3.graph_launch, 2) / args.cams
+    assert fp3.status() == 0
+    split3 = fp3.split()
+    log("config 3 batched: %.3f ms/frame over %d cameras (graph), tile %s, %s" % (
+        t_graph * 1e3, args.cams, tile3, "two launches" if split3 else "one launch"))
+    fp3.close()
+    del frames
+    plain3 = PlainFrame(torch, wl3, cams[0], Q2, CUT2, tf)
+    plain3.fused_step(crop=False)
+    t_fused3 = events_time(torch, lambda: plain3.fused_step(crop=False), 5)
+    log("config 3 drop-in fused (interpolation batch + share launch): %.3f ms/frame" % (t_fused3 * 1e3))
+    plain3.step()
+    t_plain3 = events_time(torch, lambda: plain3.step(crop=False), 2)
+    log("config 3 plain per-block: %.3f ms/frame" % (t_plain3 * 1e3))
+    return {"workload": "2000 blocks, levels 2-6, 65^3 fp32 bricks, fill 5-100%%, 8 Gaussians "
+                        "+ 3-octave noise, 1920x1080, %d cameras" % args.cams,
+            "batched_graph_fps": 1.0 / t_graph, "batched_ms_per_frame": t_graph * 1e3,
+            "dropin_fused_share_ms_per_frame": t_fused3 * 1e3,
+            "speedup_vs_dropin_fused": t_fused3 / t_graph,
+            "plain_per_block_fps": 1.0 / t_plain3, "speedup_vs_plain": t_plain3 / t_graph,
+            "tile": list(tile3), "split_launches": split3, "sparse_nodes": wl3.sparse_nodes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -459,6 +507,8 @@ def main():
                          "bricks, 8x8 bins) x3 at 1920x1080, then x3 at 512x512 (profiler runs)")
     ap.add_argument("--only-dropin", action="store_true",
                     help="the drop-in worker frames only: 5 warm-up and 20 timed steps (profiler runs)")
+    ap.add_argument("--only-mixed", action="store_true",
+                    help="config 3 only: batched graph frames, the fused drop-in and per-block launches")
     ap.add_argument("--only-interp", action="store_true",
                     help="time a 1 GiB device copy and the batched interpolation only (profiler runs)")
     args = ap.parse_args()
@@ -473,6 +523,10 @@ def main():
     out = {}
     if args.only_large:
         print(json.dumps(large_share_bench(torch, args, datasets.seeded_tf(SEED, 256), log)))
+        return
+    if args.only_mixed:
+        tf_m = torch.from_numpy(datasets.seeded_tf(SEED, 256)).cuda()
+        print(json.dumps(mixed_bench(torch, args, tf_m, log)))
         return
 
     # ---------------- config 2: 512 level-3 bricks, plain drop-in frame
@@ -736,40 +790,7 @@ def main():
 
     # ---------------- config 3: 2000 mixed-level blocks, batched vs plain
     if not args.no_mixed:
-        t0 = time.time()
-        field3 = datasets.Field(SEED + 3, n_gauss=8, noise_octaves=3)
-        wl3 = configs.mixed(torch, field3, nblocks=2000, depth=6, seed=SEED + 3)
-        cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in
-                         zip(np.random.default_rng(SEED + 3).uniform(0, 360, args.cams),
-                             np.random.default_rng(SEED + 4).uniform(-45, 45, args.cams))])
-        torch.cuda.synchronize()
-        log("config 3 setup %.1fs: %d blocks, %d sparse nodes" % (time.time() - t0, len(wl3.blocks),
-                                                                  wl3.sparse_nodes))
-        fp3 = frame_plan(torch, wl3, tf, Q2, CUT2, blocks=tiled_blocks(torch, wl3))
-        fp3.set_cameras(cams)
-        argb3 = torch.empty((H, W), dtype=torch.int32, device="cuda")
-        fp3.autotune(argb3, False, TUNE_CACHE)  # grid of the resident-brick kernel (split frames use it)
-        tile3 = fp3.autotune(argb3, True, TUNE_CACHE)
-        frames = torch.empty((args.cams, H, W), dtype=torch.int32, device="cuda")
-        fp3.graph_build(frames, H * W, True)
-        fp3.graph_launch()
-        torch.cuda.synchronize()
-        t_graph = events_time(torch, fp3.graph_launch, 2) / args.cams
-        assert fp3.status() == 0
-        split3 = fp3.split()
-        log("config 3 batched: %.3f ms/frame over %d cameras (graph), tile %s, %s" % (
-            t_graph * 1e3, args.cams, tile3, "two launches" if split3 else "one launch"))
-        plain3 = PlainFrame(torch, wl3, cams[0], Q2, CUT2, tf)
-        plain3.step()
-        t_plain3 = events_time(torch, lambda: plain3.step(crop=False), 2)
-        log("config 3 plain per-block: %.3f ms/frame" % (t_plain3 * 1e3))
-        out["mixed2000"] = {"workload": "2000 blocks, levels 2-6, 65^3 fp32 bricks, fill 5-100%%, 8 Gaussians "
-                                        "+ 3-octave noise, 1920x1080, %d cameras" % args.cams,
-                            "batched_graph_fps": 1.0 / t_graph, "batched_ms_per_frame": t_graph * 1e3,
-                            "plain_per_block_fps": 1.0 / t_plain3, "speedup_vs_plain": t_plain3 / t_graph,
-                            "tile": list(tile3), "split_launches": split3, "sparse_nodes": wl3.sparse_nodes}
-        fp3.close()
-        del frames, wl3
+        out["mixed2000"] = mixed_bench(torch, args, tf, log)
 
     # ---------------- config 4: forward + adjoint at 512x512 on the same field
     if not args.no_adjoint:

@@ -71,6 +71,7 @@ __global__ __launch_bounds__(256) void bound_rect_kernel(const uint32_t* __restr
     int mnx = 0x7fffffff, mny = 0x7fffffff, mxx = -1, mxy = -1;
     const uint32_t x = blockIdx.x * 64u + (threadIdx.x & 63u);
     if (x < w) {
+#pragma unroll 4
         for (uint32_t y = blockIdx.y * 4u + (threadIdx.x >> 6); y < h; y += gridDim.y * 4u) {
             if (pix[y * w + x] >> 24) {
                 mnx = min(mnx, (int)x);
@@ -437,9 +438,12 @@ int rv::bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int*
 {
     hipLaunchKernelGGL(init_rect_kernel, dim3(1), dim3(64), 0, s, d_buf);
     const uint32_t gx = (w + 63u) / 64u;
+    // few workgroups, many rows each: the 4 atomics per workgroup meet on
+    // one address each, so their count, not the pixel reads, bounds the
+    // kernel when workgroups are many
     uint32_t gy = (h + 3u) / 4u;
-    if (gy > 1024u)
-        gy = 1024u;
+    if (gy > 16u)
+        gy = 16u;
     if (gx > 0u && gy > 0u)
         hipLaunchKernelGGL(bound_rect_kernel, dim3(gx, gy), dim3(256), 0, s, d_pixels, w, h, d_buf);
     RV_LAUNCH_CHECK();
