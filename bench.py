@@ -647,6 +647,9 @@ def main():
         plain.fused_step()
     t_fixed = events_time(torch, lambda: plain.fused_step(), 10)
     t_share = events_time(torch, lambda: plain.share.render(plain.vp), 10)
+    # the drop-in frame's own interpolation: the same 512 bricks into the
+    # micro-brick layout (80 floats per 4^3 cells: 1.43x the x-fastest bytes)
+    t_interp_t4 = events_time(torch, plain.batch.run, 20)
 
     batch = api.InterpBatch([(b["sparse"], b["plan"], b["brick"], b["depth"]) for b in wl.blocks])
     batch.run()
@@ -714,6 +717,8 @@ def main():
         "interp_ms_batched_private_plans": t_interp_own * 1e3,
         "interp_gvox_s_private_plans": V_total / t_interp_own / 1e9,
         "interp_ms_per_block_launches": t_interp_plain * 1e3,
+        "interp_ms_batched_micro_bricks": t_interp_t4 * 1e3,
+        "interp_frac_micro_bricks": B_alg / t_interp_t4 / HBM_PEAK,
         "render_share_ms": t_share * 1e3,
         "render_per_block_launches_ms": t_render * 1e3,
         "render_samples_per_frame": S,
@@ -731,7 +736,12 @@ def main():
                      "note": "achieved = algorithmic bytes (4 B per dense voxel written + 10 B per sparse "
                              "node read) / kernel time; traffic = HBM bytes per launch from rocprofv3 "
                              "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy; the 512 bricks share one "
-                             "plan, which stays in cache (private plans: interp_ms_batched_private_plans)"},
+                             "plan, which stays in cache (private plans: interp_ms_batched_private_plans). "
+                             "This is the step's HBM-bound kernel, in the interpolator's own (x-fastest) "
+                             "layout; the drop-in frame interpolates into micro-bricks "
+                             "(interp_ms_batched_micro_bricks). The step's largest kernel is the share "
+                             "ray-cast, which is bound by VALU issue and vector-load address processing, "
+                             "not HBM: its fractions are in render_kernel."},
         "render_kernel": {"kernel": "share_kernel (all 512 blocks, per-block 8-bit composite, fast sample "
                                     "sequence, micro-brick bricks)",
                           "bound": "valu / l1 tag lookups", "ms": t_share * 1e3,
@@ -769,24 +779,26 @@ def main():
     prof = "share_kernel_512x65cubed_1080p_counters.md"
     valu_insts = profiled(prof, "fast_sq_insts_valu")
     tcp_acc = profiled(prof, "fast_tcp_total_cache_accesses")
+    ta_busy = profiled(prof, "fast_ta_busy_frac")
     rk = out["render_kernel"]
     rk["peaks_measured"] = peaks
     if valu_insts:
         # wave instructions per second against the probe's (a lane fma is one
-        # 64th of a wave fma instruction)
+        # 64th of a wave fma instruction); with the samples per frame, the
+        # VALU lane-slots per sample
         rk["valu"] = {"achieved": valu_insts / t_share, "peak": peaks["valu_fma_s"] / 64, "unit": "wave insts/s",
                       "frac": valu_insts / t_share / (peaks["valu_fma_s"] / 64),
+                      "lane_slots_per_sample": 64.0 * valu_insts / S,
                       "source": "SQ_INSTS_VALU per frame from profiles/" + prof}
-    if tcp_acc:
-        # L1 tag lookups per second against the scattered-gather probe's (64
-        # per load instruction)
-        rk["l1_lookups"] = {"achieved": tcp_acc / t_share, "peak": peaks["gather_scattered_s"] * 64,
-                            "unit": "TCP accesses/s",
-                            "frac": tcp_acc / t_share / (peaks["gather_scattered_s"] * 64),
-                            "source": "TCP_TOTAL_CACHE_ACCESSES per frame from profiles/" + prof}
-    if valu_insts and tcp_acc:
-        rk["frac"] = max(rk["valu"]["frac"], rk["l1_lookups"]["frac"])
-        rk["bound"] = "valu" if rk["valu"]["frac"] >= rk["l1_lookups"]["frac"] else "l1 tag lookups"
+    if ta_busy:
+        # the texture addresser (one per CU) processes every vector load's
+        # addresses and L1 tag lookups: its busy fraction over the kernel
+        rk["texture_addresser"] = {"frac": ta_busy, "l1_accesses_per_frame": tcp_acc,
+                                   "source": "TA_TA_BUSY_sum / (GRBM_GUI_ACTIVE / 8 XCDs x 256 CUs) from "
+                                             "profiles/" + prof}
+    if valu_insts and ta_busy:
+        rk["frac"] = max(rk["valu"]["frac"], ta_busy)
+        rk["bound"] = "valu issue" if rk["valu"]["frac"] >= ta_busy else "texture addresser (L1 tag lookups)"
 
     # ---------------- config 3: 2000 mixed-level blocks, batched vs plain
     if not args.no_mixed:

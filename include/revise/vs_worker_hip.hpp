@@ -215,8 +215,12 @@ private:
     uint64_t m_viewportPixels = 0;
     uint32_t* m_part = nullptr;
     uint64_t m_partCap = 0;
-    int* m_rectBuf = nullptr; // 16 bytes: the crop rectangle's reduction
+    int* m_rectBuf = nullptr;     // 16 bytes: the crop rectangle's reduction
+    int* m_rectHost = nullptr;    // 16 pinned bytes: the rectangle for the host
+    int* m_rectHostDev = nullptr; // the device's address of m_rectHost
     std::array<int, 4> m_partRect{0, 0, 0, 0};
+    // The device address of a pinned host buffer, or null for pageable memory.
+    void* mappedHost(void* p);
     bool m_cancelled = false;
 };
 

@@ -544,14 +544,24 @@ class Worker:
         check(lib().rv_worker_set_state(self.h, C.byref(st)))
         self.W, self.H = vp_w, vp_h
 
-    def render_scene_part(self, level, box_min, box_max, cancel=False):
+    def render_scene_part(self, level, box_min, box_max, cancel=False, out=None):
+        """rv_worker_render_scene_part into a new numpy buffer, or into `out`
+        (a contiguous 32-bit torch tensor or numpy array, e.g. pinned host
+        memory, which the library writes from the GPU directly); returns
+        ((x, y, w, h), the h x w part) or None when cancelled."""
         bmin, bmax = _np(box_min, np.uint32), _np(box_max, np.uint32)
-        cap = self.W * self.H
-        bits = np.zeros(max(cap, 1), dtype=np.uint32)
+        if out is None:
+            cap = self.W * self.H
+            bits = np.zeros(max(cap, 1), dtype=np.uint32)
+            ptr_out = _p(bits, C.c_uint32)
+        else:
+            cap = out.numel() if hasattr(out, "numel") else out.size
+            bits = out.numpy().view(np.uint32) if hasattr(out, "numpy") else out.view(np.uint32)
+            ptr_out = C.cast(out.data_ptr() if hasattr(out, "data_ptr") else out.ctypes.data, C.POINTER(C.c_uint32))
         rect = (C.c_int32 * 4)()
         flag = C.c_int(1 if cancel else 0)
         rc = lib().rv_worker_render_scene_part(self.h, level, _p(bmin, C.c_uint32), _p(bmax, C.c_uint32),
-                                               C.byref(flag), rect, _p(bits, C.c_uint32), cap)
+                                               C.byref(flag), rect, ptr_out, cap)
         if rc == 4:
             return None
         check(rc)

@@ -113,14 +113,21 @@ __global__ __launch_bounds__(256) void bound_rect_kernel(const uint32_t* __restr
 
 // The pixels of the rectangle the bound_rect_kernel left in rect[] (rows as
 // in the viewport), packed w x h into part: the crop of VsWorker_cu.cpp:
-// 279-306 without a host round trip.
+// 279-306 without a host round trip.  part may be pinned host memory; a
+// rectangle of more than cap pixels writes nothing.  rect_out (optional, may
+// be pinned host memory) receives the rectangle {minX, minY, maxX, maxY}.
 __global__ __launch_bounds__(256) void crop_kernel(const uint32_t* __restrict__ pix, uint32_t w, uint32_t h,
-                                                   const int* __restrict__ rect, uint32_t* __restrict__ part)
+                                                   const int* __restrict__ rect, uint32_t* __restrict__ part,
+                                                   uint64_t cap, int* __restrict__ rect_out)
 {
     const int x0 = rect[0], y0 = rect[1], x1 = rect[2], y1 = rect[3];
+    if (rect_out && blockIdx.x == 0u && blockIdx.y == 0u && threadIdx.x < 4u)
+        rect_out[threadIdx.x] = rect[threadIdx.x];
     if (x1 < 0)
         return; // empty
     const int pw = x1 - x0 + 1;
+    if ((uint64_t)pw * (uint64_t)(y1 - y0 + 1) > cap)
+        return;
     const int x = x0 + (int)(blockIdx.x * 64u + (threadIdx.x & 63u));
     if (x > x1)
         return;
@@ -450,17 +457,19 @@ int rv::bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int*
     return RV_OK;
 }
 
-// The rectangle's pixels packed into d_part (room for w * h), in stream
-// order after bound_rect_launch.
+// The rectangle's pixels packed into d_part (room for cap pixels; device
+// memory or pinned host memory), in stream order after bound_rect_launch;
+// rect_out (or null) receives the rectangle.
 int rv::crop_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, const int* d_buf, uint32_t* d_part,
-                    hipStream_t s)
+                    uint64_t cap, int* rect_out, hipStream_t s)
 {
     const uint32_t gx = (w + 63u) / 64u;
     uint32_t gy = (h + 3u) / 4u;
     if (gy > 256u)
         gy = 256u;
     if (gx > 0u && gy > 0u)
-        hipLaunchKernelGGL(crop_kernel, dim3(gx, gy), dim3(256), 0, s, d_pixels, w, h, d_buf, d_part);
+        hipLaunchKernelGGL(crop_kernel, dim3(gx, gy), dim3(256), 0, s, d_pixels, w, h, d_buf, d_part, cap,
+                           rect_out);
     RV_LAUNCH_CHECK();
     return RV_OK;
 }

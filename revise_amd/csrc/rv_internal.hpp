@@ -35,11 +35,13 @@ inline hipStream_t S(rv_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
 // getBoundRect of a device ARGB image (render.hip): bound_rect_launch
 // leaves {minX, minY, maxX, maxY} in d_buf (16 bytes of device memory) in
-// stream order, crop_launch packs that rectangle's pixels into d_part,
+// stream order, crop_launch packs that rectangle's pixels into d_part (cap
+// pixels of device or pinned host memory; rect_out, if not null, gets the
+// rectangle),
 // bound_rect_read waits for the stream and returns {x, y, w, h}.
 int bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, hipStream_t s);
 int crop_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, const int* d_buf, uint32_t* d_part,
-                hipStream_t s);
+                uint64_t cap, int* rect_out, hipStream_t s);
 int bound_rect_read(const int* d_buf, int32_t rect[4], int* nonempty, hipStream_t s);
 
 } // namespace rv

@@ -17,7 +17,7 @@ namespace rv {
 void set_error(const std::string& msg);
 int bound_rect_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, int* d_buf, hipStream_t s);
 int crop_launch(const uint32_t* d_pixels, uint32_t w, uint32_t h, const int* d_buf, uint32_t* d_part,
-                hipStream_t s);
+                uint64_t cap, int* rect_out, hipStream_t s);
 int bound_rect_read(const int* d_buf, int32_t rect[4], int* nonempty, hipStream_t s);
 }
 
@@ -417,6 +417,8 @@ VsWorkerHip::~VsWorkerHip()
 {
     if (m_rectBuf)
         (void)hipFree(m_rectBuf);
+    if (m_rectHost)
+        (void)hipHostFree(m_rectHost);
     if (m_viewport)
         (void)hipFree(m_viewport);
     if (m_part)
@@ -504,11 +506,36 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
     }
     // Crop to the non-transparent rectangle (VsWorker_cu.cpp:279-306), on
     // the device: the rectangle and the packed part are computed in stream
-    // order, then the rectangle is read and only the part crosses to the host.
+    // order.  A pinned host destination receives the part and the rectangle
+    // from the crop kernel itself (one synchronisation); otherwise the part
+    // is packed in device memory, the rectangle read, and the part copied.
     int32_t rect[4];
     int nonempty = 0;
     if (!m_rectBuf)
         hipCheck(hipMalloc((void**)&m_rectBuf, 16), "hipMalloc");
+    throwIfFailed(rv::bound_rect_launch(m_viewport, W, H, m_rectBuf, hs(stream())));
+    void* mapped = hostDst ? mappedHost(hostDst) : nullptr;
+    if (mapped) {
+        if (!m_rectHost) {
+            hipCheck(hipHostMalloc((void**)&m_rectHost, 16, hipHostMallocDefault), "hipHostMalloc");
+            hipCheck(hipHostGetDevicePointer((void**)&m_rectHostDev, m_rectHost, 0), "hipHostGetDevicePointer");
+        }
+        throwIfFailed(rv::crop_launch(m_viewport, W, H, m_rectBuf, static_cast<uint32_t*>(mapped), hostCap,
+                                      m_rectHostDev, hs(stream())));
+        hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
+        m_partRect = {0, 0, 0, 0}; // the part is not kept on the device
+        if (m_rectHost[2] < 0)
+            return part;
+        rect[0] = m_rectHost[0];
+        rect[1] = m_rectHost[1];
+        rect[2] = m_rectHost[2] - m_rectHost[0] + 1;
+        rect[3] = m_rectHost[3] - m_rectHost[1] + 1;
+        if ((uint64_t)rect[2] * rect[3] > hostCap)
+            throw HipError(RV_ERR_INVALID, "renderScenePart: output capacity too small");
+        part.origin = {rect[0], rect[1]};
+        part.size = {rect[2], rect[3]};
+        return part;
+    }
     if (px > m_partCap) {
         if (m_part)
             (void)hipFree(m_part);
@@ -517,8 +544,7 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
         hipCheck(hipMalloc((void**)&m_part, 4 * px), "hipMalloc");
         m_partCap = px;
     }
-    throwIfFailed(rv::bound_rect_launch(m_viewport, W, H, m_rectBuf, hs(stream())));
-    throwIfFailed(rv::crop_launch(m_viewport, W, H, m_rectBuf, m_part, hs(stream())));
+    throwIfFailed(rv::crop_launch(m_viewport, W, H, m_rectBuf, m_part, m_partCap, nullptr, hs(stream())));
     throwIfFailed(rv::bound_rect_read(m_rectBuf, rect, &nonempty, hs(stream())));
     if (!nonempty)
         return part;
@@ -538,6 +564,17 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
     hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
     m_partRect = {rect[0], rect[1], rect[2], rect[3]};
     return part;
+}
+
+void* VsWorkerHip::mappedHost(void* p)
+{
+    // asked on every call: a buffer freed and another allocated at its
+    // address may differ in kind
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) == hipSuccess && a.type == hipMemoryTypeHost && a.devicePointer)
+        return a.devicePointer;
+    (void)hipGetLastError(); // pageable memory: clear the query's error
+    return nullptr;
 }
 
 void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vector<uint32_t>& order,

@@ -423,3 +423,27 @@ def test_share_scan_equals_bins_level5(exact):
     x, y, pw, ph = rect
     img = vp.cpu().numpy().view(np.uint32)
     assert np.array_equal(img[y:y + ph, x:x + pw].reshape(-1), np.asarray(bits).reshape(-1))
+
+
+def test_worker_part_into_pinned_host_buffer():
+    """A pinned host destination receives the part from the crop kernel
+    itself: the same rectangle and pixels as a pageable destination; a
+    destination too small for the part is an error and is left untouched."""
+    import torch
+    W, H = 220, 160
+    w, blocks, fmin, fmax, tf, M = _make(2, 4, 3, W, H, 65.0)
+    w.set_fused(True)
+    for exact in (True, False):
+        w.set_state(M, W, H, FOV, Q, THR, CUT, fmin, fmax, BG, tf, exact=exact)
+        ref = w.render_scene_part(2, (0, 0, 0), (4, 4, 4))
+        out = torch.full((W * H,), -7, dtype=torch.int32, pin_memory=True)
+        got = w.render_scene_part(2, (0, 0, 0), (4, 4, 4), out=out)
+        assert got[0] == ref[0]
+        assert np.array_equal(got[1], ref[1])
+        n = ref[0][2] * ref[0][3]
+        assert n > 0 and (out.numpy()[n:] == -7).all()
+        small = torch.full((n - 1,), -7, dtype=torch.int32, pin_memory=True)
+        with pytest.raises(RvError):
+            w.render_scene_part(2, (0, 0, 0), (4, 4, 4), out=small)
+        assert (small.numpy() == -7).all()
+    w.close()
