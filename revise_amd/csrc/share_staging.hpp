@@ -10,11 +10,12 @@
 
 namespace rv {
 
-// Staging of a share's block table: pinned host bytes and device bytes of
-// `cap` each (grown on demand), and the event of the last copy out of the
-// host bytes.
+// Staging of a share's block table: pinned host bytes (and their device
+// address) and device bytes of `cap` each (grown on demand), and the event
+// of the last copy out of the host bytes.
 struct ShareStaging {
     void* host = nullptr;
+    void* host_dev = nullptr; // the device's address of `host`
     void* dev = nullptr;
     uint64_t cap = 0;
     hipEvent_t copied = nullptr;
