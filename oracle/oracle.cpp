@@ -651,12 +651,12 @@ extern "C" float orc_powf(float b, float e)
         return 1.0f; /* b < 1 and e > 0 give y < 0; guard rounding */
     float fl = std::floor(y);
     float f = y - fl;
-    float p = 0x1.c7204cp-13f;
-    p = std::fmaf(p, f, 0x1.4619a2p-10f);
-    p = std::fmaf(p, f, 0x1.3d27f6p-7f);
-    p = std::fmaf(p, f, 0x1.c68502p-5f);
-    p = std::fmaf(p, f, 0x1.ebfd9ep-3f);
-    p = std::fmaf(p, f, 0x1.62e42ap-1f);
+    float p = 0x1.c54176p-13f;
+    p = std::fmaf(p, f, 0x1.46d64cp-10f);
+    p = std::fmaf(p, f, 0x1.3d0b92p-7f);
+    p = std::fmaf(p, f, 0x1.c68912p-5f);
+    p = std::fmaf(p, f, 0x1.ebfd58p-3f);
+    p = std::fmaf(p, f, 0x1.62e42cp-1f);
     p = std::fmaf(p, f, 1.0f);
     return std::ldexp(p, (int)fl);
 }

@@ -17,9 +17,10 @@ namespace rv {
 // pow replacement for 0 <= b <= 1, e > 0 (the reference calls CUDA powf,
 // renderArgb.hpp:45).  Method: b = m 2^k with m in [sqrt(1/2), sqrt(2));
 // log2 b = k + (2/ln2) atanh(s), s = (m-1)/(m+1), by the odd Taylor series of
-// atanh to s^9; 2^y = 2^floor(y) P(y - floor(y)) with P a degree-6 polynomial
-// fitted (Lawson-reweighted least squares) to 2^f on [0,1], both
-// polynomials by Horner's rule with fused multiply-adds; relative error
+// atanh to s^9; 2^y = 2^floor(y) P(y - floor(y)) with P a degree-6 polynomial,
+// P(0) = 1, the rest a relative-error minimax fit to 2^f on [0,1] (Lawson's
+// reweighted least squares; tests/golden/fit_exp2.py makes the coefficients),
+// both polynomials by Horner's rule with fused multiply-adds; relative error
 // below 1e-6 over the renderer's range.  Results under 2^-126 are 0.
 __device__ __forceinline__ float pow01(float b, float e)
 {
@@ -58,12 +59,12 @@ __device__ __forceinline__ float pow01(float b, float e)
         return 1.0f;
     const float fl = floorf(y);
     const float f = y - fl;
-    float p = 0x1.c7204cp-13f;
-    p = fmaf(p, f, 0x1.4619a2p-10f);
-    p = fmaf(p, f, 0x1.3d27f6p-7f);
-    p = fmaf(p, f, 0x1.c68502p-5f);
-    p = fmaf(p, f, 0x1.ebfd9ep-3f);
-    p = fmaf(p, f, 0x1.62e42ap-1f);
+    float p = 0x1.c54176p-13f;
+    p = fmaf(p, f, 0x1.46d64cp-10f);
+    p = fmaf(p, f, 0x1.3d0b92p-7f);
+    p = fmaf(p, f, 0x1.c68912p-5f);
+    p = fmaf(p, f, 0x1.ebfd58p-3f);
+    p = fmaf(p, f, 0x1.62e42cp-1f);
     p = fmaf(p, f, 1.0f);
     return ldexpf(p, (int)fl);
 }
