@@ -337,7 +337,7 @@ int rv_render_share_secondary(const rv_block_params* params, const float* const*
  * memory -- code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc.  The block table is staged in a fixed device array of the
  * library, one per GPU, holding at most RV_SHARE_STATIC_MAX_BLOCKS blocks
- * (786,688 bytes, kept by each GPU the library runs on until the process
+ * (803,072 bytes, kept by each GPU the library runs on until the process
  * ends); the call makes no hipMalloc/hipFree and no host heap allocation.
  * nblocks above the maximum: RV_ERR_INVALID.  Same result as
  * rv_render_share_argb, and, like it, the call synchronizes the stream once
@@ -620,9 +620,12 @@ int rv_worker_set_state(rv_worker* w, const rv_render_state* state);
 /* renderScenePart (VsWorker_cu.cpp:225-313): clear, BackToFrontOrder over the
  * index box, interpolate + ray-cast each block, crop to the non-transparent
  * rectangle.  rect = {x, y, w, h} (w = h = 0 for an empty part); bits
- * receives w*h ARGB pixels (cap pixels available).  *cancel is polled between
- * blocks; when set the call returns RV_ERR_CANCELLED with an empty part.
- * An absent block gives RV_ERR_INVALID ("Block does not exist"). */
+ * receives w*h ARGB pixels (cap pixels available; a larger part is
+ * RV_ERR_INVALID and nothing is written).  Pinned host memory receives the
+ * part from the GPU directly (one synchronisation); pageable memory through
+ * a device buffer and a copy.  *cancel is polled between blocks; when set
+ * the call returns RV_ERR_CANCELLED with an empty part.  An absent block
+ * gives RV_ERR_INVALID ("Block does not exist"). */
 int rv_worker_render_scene_part(rv_worker* w, uint32_t level, const uint32_t box_min[3],
                                 const uint32_t box_max[3], const volatile int* cancel,
                                 int32_t rect[4], uint32_t* bits, uint64_t cap);

@@ -1535,6 +1535,14 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
     return RV_OK;
 }
 
+// Byte offset of the view rectangles in a staged block table: after the
+// params, rounded up to 16 bytes (the 256 bytes of slack in
+// rv_share_workspace_bytes cover the rounding).
+static uint64_t share_rect_at(uint32_t nblocks)
+{
+    return (sizeof(rv_block_params) * (uint64_t)nblocks + 15u) & ~15ull;
+}
+
 extern "C" uint64_t rv_share_workspace_bytes(uint32_t nblocks)
 {
     return (uint64_t)nblocks * (sizeof(rv_block_params) + sizeof(uint4) + 2 * sizeof(void*)) + 256;
@@ -1626,7 +1634,7 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
     for (uint64_t b = 0; b < nbins; ++b)
         off[b + 1u] += off[b];
     const uint64_t entries = off[nbins];
-    const uint64_t rect_at = (sizeof(rv_block_params) * (uint64_t)nblocks + 15u) & ~15ull;
+    const uint64_t rect_at = share_rect_at(nblocks);
     const uint64_t off_at = rect_at + (sizeof(uint4) + 2u * sizeof(void*)) * (uint64_t)nblocks;
     const uint64_t list_at = off_at + 4u * (nbins + 1u);
     const uint64_t bytes = (list_at + 4u * entries + 15u) & ~15ull; // copied as 16-byte words
@@ -1711,17 +1719,17 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
     std::vector<uint8_t> host(rv_share_workspace_bytes(nblocks));
     int rc = share_table(params, bricks, bricks2, nblocks, layout, field_mode, levels_rel, level_count, smin, smax,
                          d_tf, tf_n, d_viewport, vp_w, vp_h, host.data(),
-                         sizeof(rv_block_params) * (uint64_t)nblocks);
+                         share_rect_at(nblocks));
     if (rc)
         return rc;
     uint8_t* d = static_cast<uint8_t*>(d_workspace);
-    RV_HIP(hipMemcpyAsync(d, host.data(), host.size() - 256, hipMemcpyHostToDevice, S(stream)));
+    RV_HIP(hipMemcpyAsync(d, host.data(), share_rect_at(nblocks) + (sizeof(uint4) + 2 * sizeof(void*)) * nblocks,
+                          hipMemcpyHostToDevice, S(stream)));
     // the host staging buffer must outlive the copy
     RV_HIP(hipStreamSynchronize(S(stream)));
     const auto* dp = reinterpret_cast<const rv_block_params*>(d);
-    const auto* dr = reinterpret_cast<const uint4*>(d + sizeof(rv_block_params) * nblocks);
-    const auto* db =
-        reinterpret_cast<const void* const*>(d + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    const auto* dr = reinterpret_cast<const uint4*>(d + share_rect_at(nblocks));
+    const auto* db = reinterpret_cast<const void* const*>(d + share_rect_at(nblocks) + sizeof(uint4) * nblocks);
     // the levels are relative to the field's range, the same for every block
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
     return launch_share(dp, dr, db, db + nblocks, nullptr, nullptr, nblocks, layout, field_mode,
@@ -1840,8 +1848,8 @@ extern "C" int rv_render_share_argb_noalloc(const rv_block_params* params, const
     if (rc)
         return rc;
     auto* dp = reinterpret_cast<rv_block_params*>(base);
-    auto* dr = reinterpret_cast<uint4*>(base + sizeof(rv_block_params) * nblocks);
-    auto* db = reinterpret_cast<const void**>(base + (sizeof(rv_block_params) + sizeof(uint4)) * nblocks);
+    auto* dr = reinterpret_cast<uint4*>(base + share_rect_at(nblocks));
+    auto* db = reinterpret_cast<const void**>(base + share_rect_at(nblocks) + sizeof(uint4) * nblocks);
     RV_HIP(hipMemcpyAsync(dp, params, sizeof(rv_block_params) * nblocks, hipMemcpyHostToDevice, S(stream)));
     RV_HIP(hipMemcpyAsync(db, bricks, sizeof(void*) * nblocks, hipMemcpyHostToDevice, S(stream)));
     // the caller's host arrays must outlive the copies

@@ -14,5 +14,6 @@ from tests.test_dataset_files import *  # noqa: F401,F403
 from tests.test_datasets_cpu import *  # noqa: F401,F403
 from tests.test_kernel_index_math import *  # noqa: F401,F403
 from tests.test_plan_encoder import *  # noqa: F401,F403
+from tests.test_exp2_fit import *  # noqa: F401,F403
 
 pytestmark = pytest.mark.gpu
