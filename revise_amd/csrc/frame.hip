@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         if (item >= a.n_interp)
             break;
         const TileTask k = a.tasks[item];
-        interp_tile<kT>(k.job, k.local, k.e0, k.e_own, k.e1, A, lat);
+        interp_tile<kT>(k.job, k.local, k.e0, k.e_own, k.e1, k.idx_off, k.lds_off, A, lat);
         ++my_tiles; // (the queue barrier above protects A before its next use)
     }
     if (a.n_interp) {
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(NT) void frame_interp_kernel(const TileTask* __rest
     if (t >= n)
         return;
     const TileTask k = tasks[t];
-    interp_tile<NT>(k.job, k.local, k.e0, k.e_own, k.e1, A, lat);
+    interp_tile<NT>(k.job, k.local, k.e0, k.e_own, k.e1, k.idx_off, k.lds_off, A, lat);
 }
 
 __global__ void frame_reset_kernel(uint32_t* counters, uint32_t n)

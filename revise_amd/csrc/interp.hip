@@ -69,7 +69,8 @@ __global__ __launch_bounds__(NT) void interp_single_kernel(Job job, uint32_t nti
     if (t >= ntiles)
         return;
     const PlanView P = plan_view(job.plan);
-    interp_tile<NT>(job, t, P.tile_start[t], P.own_end[t], P.tile_start[t + 1], A, lat);
+    const uint32_t* h = reinterpret_cast<const uint32_t*>(job.plan);
+    interp_tile<NT>(job, t, P.tile_start[t], P.own_end[t], P.tile_start[t + 1], h[6], h[7], A, lat);
 }
 
 template <uint32_t kFloats, uint32_t NT>
@@ -82,7 +83,7 @@ __global__ __launch_bounds__(NT) void interp_batch_kernel(const TileTask* __rest
     if (t >= ntiles)
         return;
     const TileTask k = tasks[t];
-    interp_tile<NT>(k.job, k.local, k.e0, k.e_own, k.e1, A, lat);
+    interp_tile<NT>(k.job, k.local, k.e0, k.e_own, k.e1, k.idx_off, k.lds_off, A, lat);
 }
 
 bool big_class(uint32_t depth) { return tile_floats(geom_of(depth)) > kTileFloats8; }
@@ -129,6 +130,7 @@ namespace {
 // range and the end of its owned entries.
 struct PlanTable {
     uint32_t depth = ~0u;
+    uint32_t idx_off = 0, lds_off = 0; // header words 6 and 7
     std::vector<uint32_t> start, own;
 };
 
@@ -142,6 +144,8 @@ int read_plan_table(Get get, PlanTable& tb)
     if (head[0] != kPlanMagic)
         return fail(RV_ERR_INVALID, "interpolation job: plan is not an interpolation plan");
     tb.depth = head[1];
+    tb.idx_off = head[6];
+    tb.lds_off = head[7];
     if (tb.depth <= kMaxDepth && head[2] == geom_of(tb.depth).ntiles) {
         tb.start.resize(head[2] + 1u);
         RV_HIP(get(tb.start.data(), kPlanHeader, 4ull * tb.start.size()));
@@ -198,7 +202,8 @@ int build_tile_tasks(const Job* jobs, uint32_t njobs, std::vector<TileTask>& tas
         if (tb.depth != j.depth || tb.start.size() != g.ntiles + 1u)
             return fail(RV_ERR_INVALID, "interpolation job: plan was built for another depth");
         for (uint32_t t = 0; t < g.ntiles; ++t)
-            tasks.push_back(TileTask{j, t, tb.start[t], tb.start[t + 1], tb.own[t], 0u});
+            tasks.push_back(TileTask{j, t, tb.start[t], tb.start[t + 1], tb.own[t], tb.idx_off,
+                                     tb.lds_off});
     }
     return RV_OK;
 }

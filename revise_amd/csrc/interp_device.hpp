@@ -46,10 +46,11 @@ struct Job {
 // searching the job list and reading the plan's tile table.
 struct TileTask {
     Job job;
-    uint32_t local;  // tile index within its brick
-    uint32_t e0, e1; // plan entries [e0, e1) of the tile
-    uint32_t e_own;  // [e0, e_own): the nodes the tile stores
-    uint32_t pad;
+    uint32_t local;   // tile index within its brick
+    uint32_t e0, e1;  // plan entries [e0, e1) of the tile
+    uint32_t e_own;   // [e0, e_own): the nodes the tile stores
+    uint32_t idx_off; // byte offsets of the plan's sparse index and LDS
+    uint32_t lds_off; // offset arrays (header words 6 and 7)
 };
 
 // Builds the tile list of a batch, job after job (host).  Reads each distinct
@@ -191,12 +192,18 @@ __device__ __forceinline__ PlanView plan_view(const uint8_t* P)
 // One tile of a brick with NT threads (a multiple of 64); A holds
 // tile_floats(geom_of(depth)) floats, lat 64 floats; [e0, e1) is the tile's
 // plan entry range, [e0, e_own) the nodes it stores.
+// idx_off, lds_off: the byte offsets of the plan's entry arrays (header words
+// 6 and 7), passed in so that a full tile's first plan loads wait for nothing
+// but the caller's own.
 template <uint32_t NT>
 __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t e0, uint32_t e_own,
-                                   uint32_t e1, float* A, float* lat)
+                                   uint32_t e1, uint32_t idx_off, uint32_t lds_off, float* A,
+                                   float* lat)
 {
     static_assert(NT % 64u == 0u, "whole waves per workgroup");
     const PlanView P = plan_view(job.plan);
+    const uint32_t* __restrict__ plan_idx = reinterpret_cast<const uint32_t*>(job.plan + idx_off);
+    const uint16_t* __restrict__ plan_lds = reinterpret_cast<const uint16_t*>(job.plan + lds_off);
     const Geom g = geom_of(job.depth);
     const uint32_t d = g.d, X = g.X, T = g.T;
     const uint32_t tx = local_tile % g.tiles_x;
@@ -241,10 +248,16 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
 
         // 2. scatter the tile's sparse nodes into LDS: plan entries and sparse
         // values are read in increasing sparse index (coalesced), kUnroll per
-        // thread in flight.  The plan is per-subtree metadata read every
-        // frame (ordinary loads, cached); sparse values are read once per
-        // frame (streaming loads).
-        constexpr uint32_t kUnroll = 8;
+        // thread in flight.  Both go through the caches: the plan is
+        // per-subtree metadata read every frame, and a cache line of sparse
+        // values usually holds nodes of two neighbouring tiles (of one brick,
+        // so on one XCD), which then read it from L2 once.
+        // 512-thread workgroups (2 per CU, bound by LDS) issue 24 loads per
+        // thread at a time (about 100 VGPRs, still 4 waves per SIMD): a
+        // 65 x 16 x 16 full tile takes two rounds of plan then value loads
+        // instead of five.  Smaller workgroups share the CU with more of
+        // their kind and keep 8.
+        constexpr uint32_t kUnroll = NT >= 512u ? 24u : 8u;
         bool saw_nan = false;
         const uint32_t eend = full ? e_own : e1;
         for (uint32_t k0 = e0 + threadIdx.x; k0 < eend; k0 += NT * kUnroll) {
@@ -256,14 +269,14 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
                 si[j] = kNoIndex;
                 off[j] = 0;
                 if (k < eend) {
-                    si[j] = P.idx[k];
-                    off[j] = P.lds[k];
+                    si[j] = plan_idx[k];
+                    off[j] = plan_lds[k];
                 }
             }
             float v[kUnroll];
 #pragma unroll
             for (uint32_t j = 0; j < kUnroll; ++j)
-                v[j] = si[j] == kNoIndex ? 0.f : __builtin_nontemporal_load(job.sparse + si[j]);
+                v[j] = si[j] == kNoIndex ? 0.f : job.sparse[si[j]];
 #pragma unroll
             for (uint32_t j = 0; j < kUnroll; ++j)
                 if (si[j] != kNoIndex) {
@@ -376,10 +389,16 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
         // the tile is the whole brick: LDS and brick layouts coincide, one
         // contiguous run (fp16: two values per 4-byte store when aligned)
         if (job.dense16 && (reinterpret_cast<uintptr_t>(job.dense16) & 3u) == 0u) {
+            // stored from the start of the aligned 256-B chunk the brick
+            // starts in (lanes before it masked), as the row runs below
             const uint32_t pairs = tile_nodes >> 1;
             __half2* out2 = reinterpret_cast<__half2*>(job.dense16);
             const float2* A2 = reinterpret_cast<const float2*>(A);
-            for (uint32_t q = threadIdx.x; q < pairs; q += NT) {
+            const uint32_t skew = (uint32_t)(reinterpret_cast<uintptr_t>(job.dense16) >> 2) & 63u;
+            for (uint32_t qs = threadIdx.x; qs < pairs + skew; qs += NT) {
+                if (qs < skew)
+                    continue;
+                const uint32_t q = qs - skew;
                 float2 v = A2[q];
                 if (v.x == kNoField)
                     v.x = __builtin_nanf("");
@@ -411,19 +430,30 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
     const uint32_t oz = (tz + 1u == g.tiles_yz) ? tp1 : T;
     if (g.tiles_x == 1u) {
         // the tile spans whole brick rows: its rows of one z-plane are one
-        // contiguous run in the brick and in LDS
+        // contiguous run in the brick and in LDS.  The run is stored from the
+        // start of the 64-lane chunk (256 B fp32, 128 B fp16) its first node
+        // falls in, lanes before the run masked, so every store instruction
+        // of a wave covers one aligned chunk and no cache line is written by
+        // two instructions.
         const uint32_t run = oy * rowlen;
+        const uint32_t esz = job.dense16 ? 2u : 4u;
+        const uintptr_t brick = job.dense16 ? reinterpret_cast<uintptr_t>(job.dense16)
+                                            : reinterpret_cast<uintptr_t>(job.dense);
         for (uint32_t z = 0; z < oz; ++z) {
             const uint32_t base = ordinal3(0, y0, z0 + z, d);
             const float* src = A + z * tp1 * rowlen;
-            for (uint32_t q = threadIdx.x; q < run; q += NT) {
-                float v = src[q];
+            const uint32_t skew = (uint32_t)((brick + (uintptr_t)base * esz) / esz) & 63u;
+            for (uint32_t q = threadIdx.x; q < run + skew; q += NT) {
+                if (q < skew)
+                    continue;
+                const uint32_t i = q - skew;
+                float v = src[i];
                 if (v == kNoField)
                     v = __builtin_nanf("");
                 if (job.dense16)
-                    job.dense16[base + q] = __float2half(v);
+                    job.dense16[base + i] = __float2half(v);
                 else
-                    __builtin_nontemporal_store(v, job.dense + base + q);
+                    __builtin_nontemporal_store(v, job.dense + base + i);
             }
         }
         return;
