@@ -748,6 +748,116 @@ int rv_dataset_load_worker_secondary(const rv_dataset* ds, const char* field_pat
 int rv_fields_info_read(const char* path, uint32_t* time_steps, rv_field_info* fields, uint32_t cap,
                         uint32_t* count);
 
+/* ------------------------------------------------------------------------
+ * Frame output in shared memory (host only)
+ * ------------------------------------------------------------------------
+ * SharedMemory (ipc/SharedMemory.h:30-280): a System V segment holding
+ * its user size (size_t, 8 bytes), a reader/writer control word
+ * (int32: readers count up while it is >= 0; a writer swaps 0 for -1000),
+ * then the user bytes at offset 12.  Readers of the reference (the s3vs_js
+ * addon, ws_sendframe) attach to a segment made here and the other way
+ * round.  A segment is removed (IPC_RMID) when its creator closes it; a
+ * process still attached keeps its mapping until it detaches. */
+
+typedef struct rv_shm rv_shm;
+
+/* SharedMemory::createNew(bytes): a new segment of `bytes` user bytes, the
+ * control word 0, the bytes zero.  Made with IPC_PRIVATE, so it never
+ * attaches to another process's segment (the reference draws a random key in
+ * 5000..10000 with IPC_CREAT, SharedMemory.h:226-243). */
+int rv_shm_create(uint64_t bytes, rv_shm** out);
+/* SharedMemory::attachNew(shmid) (SharedMemory.h:44-46, 204-211). */
+int rv_shm_attach(int shmid, rv_shm** out);
+int rv_shm_id(const rv_shm* shm);
+/* size() (SharedMemory.h:85-87): the user bytes. */
+uint64_t rv_shm_size(const rv_shm* shm);
+/* read / write (SharedMemory.h:89-99) of user bytes [start, start + count),
+ * under the read or the write lock. */
+int rv_shm_read(const rv_shm* shm, void* dst, uint64_t start, uint64_t count);
+int rv_shm_write(rv_shm* shm, const void* src, uint64_t start, uint64_t count);
+/* Detach; remove the segment if this handle created it (~SharedMemory,
+ * SharedMemory.h:56-66). */
+void rv_shm_close(rv_shm* shm);
+/* VsControllerFrameOutputRW::writeFrame / readFrame
+ * (VsControllerFrameOutputRW.hpp:41-75) on a segment: the header, then
+ * frame_size bytes of pixels, under the write (read) lock.  pixels NULL:
+ * the header only.  The segment must hold 24 + frame_size user bytes. */
+int rv_frame_output_write_shm(rv_shm* shm, uint64_t frame_size, const rv_frame_output_header* header,
+                              const void* pixels);
+int rv_frame_output_read_shm(const rv_shm* shm, uint64_t frame_size, rv_frame_output_header* header,
+                             void* pixels);
+
+/* ------------------------------------------------------------------------
+ * Controller (host; drives one rv_worker on a render thread of its own)
+ * ------------------------------------------------------------------------
+ * VsControllerInterface / VsController + VsRenderer (VsControllerInterface.hpp:
+ * 36-178, VsController.cpp:42-196, VsRenderer.cpp:576-1192) for one GPU:
+ * every input change cancels the level-of-detail loop in progress, waits for
+ * it, applies the change and starts the loop again on the render thread
+ * (updateRenderSharedState, VsRenderer.cpp:1177-1192).  Each level's
+ * composed frame goes to the frame output segment (continueRendering,
+ * VsRenderer.cpp:1061-1166), made anew when the viewport size changes and
+ * first written with the "no frame" header (maybeInitFrameOutput,
+ * VsRenderer.cpp:81-103).  A failure on the render thread stops the loop;
+ * rv_controller_status reports it. */
+
+typedef struct rv_controller rv_controller;
+
+/* VsControllerInterface::newInstance + ComputingCaps: the render thread's
+ * worker uses GPU thread_id % device count.  split_count (ComputingCaps::
+ * workerThreadPerNodeCount, default 1): worker boxes per level, composed on
+ * the device.  measure_time (VsControllerOpts::measureRenderingTime): the
+ * frame header carries the level's wall time. */
+int rv_controller_create(uint32_t thread_id, uint32_t split_count, int measure_time,
+                         rv_controller** out);
+/* problemPath / timeFrame / primaryField / secondaryField
+ * (VsControllerInput.hpp, VsRenderer.cpp:708-758, 880-902): the dataset's
+ * base name as BlockTreeFieldService takes it (BlockTreeFieldService.hpp:
+ * 244-272, filename_util.hpp:36-94): <base>.s3dmm-meta etc. when <base> or
+ * <base>.s3dmm-meta is a file, frame files under <dir>/<stem>/ when that is
+ * a directory.  A field name the dataset lacks (or NULL) selects its first
+ * field, as resolveField does (VsRenderer.cpp:57-72); secondary NULL or ""
+ * = none.  The subtrees of levels 0..maxFullLevel are read and uploaded on
+ * the render thread.  The camera is reset (CameraController.cpp:88-101). */
+int rv_controller_set_problem(rv_controller* c, const char* problem_path, uint32_t time_frame,
+                              const char* primary_field, const char* secondary_field);
+/* The dataset's fields (fieldNames, fieldRange; VsController.cpp:123-130)
+ * and frame count (timeFrameCount) of the current problem. */
+int rv_controller_field_count(const rv_controller* c, uint32_t* count);
+int rv_controller_field_info(const rv_controller* c, uint32_t i, rv_field_info* info);
+int rv_controller_time_frame_count(const rv_controller* c, uint32_t* count);
+/* The render inputs of VsControllerInput (viewport, fovY, quality, threshold,
+ * colour table, field mode, isosurface parameters, background) and the
+ * camera: view16 may be NULL to keep the camera.  fmin/fmax and smin/smax
+ * are ignored: the ranges are the dataset's (fieldRange of the primary and
+ * secondary fields). */
+int rv_controller_set_input(rv_controller* c, const rv_render_state* state);
+/* setCameraTransform / cameraTransform / resetCameraTransform
+ * (VsController.cpp:100-114; CameraController.cpp:88-101): row-major
+ * world->eye. */
+int rv_controller_set_camera(rv_controller* c, const float* view16);
+int rv_controller_camera(const rv_controller* c, float* view16);
+int rv_controller_reset_camera(rv_controller* c);
+/* renderLevel (VsRenderer.cpp:1075-1079): -1 renders levels 0..maxFullLevel
+ * in turn; L >= 0 only min(L, maxFullLevel). */
+int rv_controller_set_render_level(rv_controller* c, int32_t level);
+/* start (VsController.cpp:91-93): rendering begins with the inputs set. */
+int rv_controller_start(rv_controller* c);
+/* frameOutput (VsController.cpp:147-151): the segment id, frame size
+ * (vp_w * vp_h * 4) and frame width and height.  The segment stays valid
+ * until the viewport size changes or the controller is destroyed. */
+int rv_controller_frame_output(rv_controller* c, int* shmid, uint64_t* frame_size, uint32_t* width,
+                               uint32_t* height);
+/* No counterpart in the reference (its callers poll the frame number): waits
+ * up to timeout_ms until the render thread has nothing left to do; *idle = 1
+ * if so.  *frames: frames written so far. */
+int rv_controller_wait_idle(rv_controller* c, uint32_t timeout_ms, int* idle, uint64_t* frames);
+/* The render thread's last failure: its status (RV_OK if none) and message. */
+int rv_controller_status(const rv_controller* c, char* message, uint64_t cap);
+/* kill (VsController.cpp:153-157): cancels, stops the render thread, frees
+ * the worker and removes the frame output segment. */
+void rv_controller_destroy(rv_controller* c);
+
 #ifdef __cplusplus
 }
 #endif

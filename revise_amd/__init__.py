@@ -211,6 +211,30 @@ _SIGS = {
     "rv_dataset_load_worker_secondary": (C.c_int, [_vp, C.c_char_p, _vp, C.c_uint32, C.c_uint32,
                                                    C.POINTER(C.c_uint64)]),
     "rv_fields_info_read": (C.c_int, [C.c_char_p, _u32p, C.POINTER(FieldInfo), C.c_uint32, _u32p]),
+    "rv_shm_create": (C.c_int, [C.c_uint64, C.POINTER(C.c_void_p)]),
+    "rv_shm_attach": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "rv_shm_id": (C.c_int, [_vp]),
+    "rv_shm_size": (C.c_uint64, [_vp]),
+    "rv_shm_read": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint64]),
+    "rv_shm_write": (C.c_int, [_vp, _vp, C.c_uint64, C.c_uint64]),
+    "rv_shm_close": (None, [_vp]),
+    "rv_frame_output_write_shm": (C.c_int, [_vp, C.c_uint64, C.POINTER(FrameOutputHeader), _vp]),
+    "rv_frame_output_read_shm": (C.c_int, [_vp, C.c_uint64, C.POINTER(FrameOutputHeader), _vp]),
+    "rv_controller_create": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_void_p)]),
+    "rv_controller_set_problem": (C.c_int, [_vp, C.c_char_p, C.c_uint32, C.c_char_p, C.c_char_p]),
+    "rv_controller_field_count": (C.c_int, [_vp, _u32p]),
+    "rv_controller_field_info": (C.c_int, [_vp, C.c_uint32, C.POINTER(FieldInfo)]),
+    "rv_controller_time_frame_count": (C.c_int, [_vp, _u32p]),
+    "rv_controller_set_input": (C.c_int, [_vp, C.POINTER(RenderState)]),
+    "rv_controller_set_camera": (C.c_int, [_vp, _f32p]),
+    "rv_controller_camera": (C.c_int, [_vp, _f32p]),
+    "rv_controller_reset_camera": (C.c_int, [_vp]),
+    "rv_controller_set_render_level": (C.c_int, [_vp, C.c_int32]),
+    "rv_controller_start": (C.c_int, [_vp]),
+    "rv_controller_frame_output": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_uint64), _u32p, _u32p]),
+    "rv_controller_wait_idle": (C.c_int, [_vp, C.c_uint32, C.POINTER(C.c_int), C.POINTER(C.c_uint64)]),
+    "rv_controller_status": (C.c_int, [_vp, C.c_char_p, C.c_uint64]),
+    "rv_controller_destroy": (None, [_vp]),
 }
 
 _lib = None

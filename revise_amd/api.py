@@ -721,6 +721,150 @@ class Dataset:
             pass
 
 
+class SharedMem:
+    """rv_shm: a System V segment with the reference's SharedMemory layout
+    (size, control word, user bytes at offset 12)."""
+
+    USER_OFFSET = 12
+
+    def __init__(self, nbytes=None, shmid=None):
+        h = C.c_void_p()
+        if shmid is None:
+            check(lib().rv_shm_create(nbytes, C.byref(h)))
+        else:
+            check(lib().rv_shm_attach(shmid, C.byref(h)))
+        self.h = h
+
+    @property
+    def shmid(self):
+        return lib().rv_shm_id(self.h)
+
+    @property
+    def size(self):
+        return lib().rv_shm_size(self.h)
+
+    def read(self, start, count):
+        out = np.zeros(count, dtype=np.uint8)
+        check(lib().rv_shm_read(self.h, out.ctypes.data, start, count))
+        return out
+
+    def write(self, start, data):
+        d = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8))
+        check(lib().rv_shm_write(self.h, d.ctypes.data, start, d.size))
+
+    def write_frame(self, frame_size, frame_number, duration_ms=0, level=0, pixels=None):
+        hdr = FrameOutputHeader(frame_number, duration_ms, level, 0)
+        px = None if pixels is None else np.ascontiguousarray(pixels, dtype=np.uint32)
+        check(lib().rv_frame_output_write_shm(self.h, frame_size, C.byref(hdr),
+                                              None if px is None else px.ctypes.data))
+
+    def read_frame(self, frame_size, pixels=True):
+        """(frame_number, duration_ms, level, pixels or None)."""
+        hdr = FrameOutputHeader()
+        px = np.zeros(frame_size // 4, dtype=np.uint32) if pixels else None
+        check(lib().rv_frame_output_read_shm(self.h, frame_size, C.byref(hdr),
+                                             None if px is None else px.ctypes.data))
+        return hdr.frame_number, hdr.duration_ms, hdr.level, px
+
+    def close(self):
+        if self.h:
+            lib().rv_shm_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Controller:
+    """rv_controller: VsController + VsRenderer for one GPU; frames go to a
+    shared-memory frame output."""
+
+    def __init__(self, thread_id=0, split_count=1, measure_time=False):
+        h = C.c_void_p()
+        check(lib().rv_controller_create(thread_id, split_count, int(measure_time), C.byref(h)))
+        self.h = h
+
+    def set_problem(self, path, time_frame=0, primary=None, secondary=None):
+        check(lib().rv_controller_set_problem(self.h, path.encode(), time_frame,
+                                              primary.encode() if primary else None,
+                                              secondary.encode() if secondary else None))
+
+    def fields(self):
+        n = C.c_uint32()
+        check(lib().rv_controller_field_count(self.h, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            f = FieldInfo()
+            check(lib().rv_controller_field_info(self.h, i, C.byref(f)))
+            out.append((f.name.decode(), f.min, f.max, bool(f.animated)))
+        return out
+
+    def time_frame_count(self):
+        n = C.c_uint32()
+        check(lib().rv_controller_time_frame_count(self.h, C.byref(n)))
+        return n.value
+
+    def set_input(self, vp_w, vp_h, fov, quality, thr_rel, alpha_cutoff, background, tf, view16=None,
+                  field_mode=FIELD_ARGB, iso_level_rel=0.5, iso_opacity=1.0, iso_levels=None, exact=False):
+        view = None if view16 is None else _np(view16, np.float32).reshape(16)
+        tfa = _np(tf, np.float32)
+        lv = _np(iso_levels if iso_levels is not None else [], np.float32).reshape(-1)
+        st = RenderState(None if view is None else view.ctypes.data, vp_w, vp_h, fov, quality, thr_rel,
+                         alpha_cutoff, 0.0, 1.0, (C.c_float * 3)(*background), tfa.ctypes.data, tfa.shape[0],
+                         field_mode, iso_level_rel, iso_opacity, lv.ctypes.data if lv.size else None, lv.size,
+                         0.0, 1.0, 1 if exact else 0)
+        check(lib().rv_controller_set_input(self.h, C.byref(st)))
+
+    def set_camera(self, view16):
+        v = _np(view16, np.float32).reshape(16)
+        check(lib().rv_controller_set_camera(self.h, _p(v, C.c_float)))
+
+    def camera(self):
+        v = np.zeros(16, dtype=np.float32)
+        check(lib().rv_controller_camera(self.h, _p(v, C.c_float)))
+        return v.reshape(4, 4)
+
+    def reset_camera(self):
+        check(lib().rv_controller_reset_camera(self.h))
+
+    def set_render_level(self, level):
+        check(lib().rv_controller_set_render_level(self.h, level))
+
+    def start(self):
+        check(lib().rv_controller_start(self.h))
+
+    def frame_output(self):
+        """(shmid, frame_size, width, height)."""
+        sid, fs, w, h = C.c_int(), C.c_uint64(), C.c_uint32(), C.c_uint32()
+        check(lib().rv_controller_frame_output(self.h, C.byref(sid), C.byref(fs), C.byref(w), C.byref(h)))
+        return sid.value, fs.value, w.value, h.value
+
+    def wait_idle(self, timeout_s=60.0):
+        """(idle, frames written)."""
+        idle, n = C.c_int(), C.c_uint64()
+        check(lib().rv_controller_wait_idle(self.h, int(timeout_s * 1000), C.byref(idle), C.byref(n)))
+        return bool(idle.value), n.value
+
+    def status(self):
+        buf = C.create_string_buffer(512)
+        rc = lib().rv_controller_status(self.h, buf, 512)
+        return rc, buf.value.decode()
+
+    def close(self):
+        if self.h:
+            lib().rv_controller_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 __all__ = [n for n in dir() if not n.startswith("_")] + ["RvError"]
 
 
