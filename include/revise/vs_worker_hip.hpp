@@ -187,6 +187,10 @@ public:
                                   uint32_t* hostDst = nullptr, uint64_t hostCap = 0);
     // The last part on the device: pixels (w*h ARGB) and {x, y, w, h}.
     const uint32_t* devicePart(std::array<int, 4>& rect) const;
+    // The fused path: the box's bricks interpolated in one batched launch
+    // (interpolateFused), then one share launch over them in back-to-front
+    // order (renderFused).
+    void interpolateFused(const SubtreeSetDescription& sub, const std::function<bool()>& isCancelled);
     void renderFused(const SubtreeSetDescription& sub, const std::vector<uint32_t>& order,
                      uint64_t count, const std::function<bool()>& isCancelled);
     bool lastCancelled() const { return m_cancelled; }
@@ -216,6 +220,7 @@ private:
     uint32_t* m_part = nullptr;
     uint64_t m_partCap = 0;
     int* m_rectBuf = nullptr;     // 16 bytes: the crop rectangle's reduction
+    bool m_rectDone = false;      // the share launch left the rectangle in m_rectBuf
     int* m_rectHost = nullptr;    // 16 pinned bytes: the rectangle for the host
     int* m_rectHostDev = nullptr; // the device's address of m_rectHost
     std::array<int, 4> m_partRect{0, 0, 0, 0};

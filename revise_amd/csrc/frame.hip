@@ -361,7 +361,7 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         uint32_t nblocks, const float4* __restrict__ tf,
                                                         uint32_t tf_n, float iso_opacity, LevelsArg lev,
                                                         uint32_t* __restrict__ viewport,
-                                                        uint32_t W, uint32_t H)
+                                                        uint32_t W, uint32_t H, int* __restrict__ rect_acc)
 {
     extern __shared__ float4 tfs_share[]; // tf_n + 1 entries (and padding)
     __shared__ float lvs[kMaxLevels];
@@ -435,6 +435,35 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
     }
     if (inside)
         viewport[pix] = value;
+    if (rect_acc) {
+        // the crop rectangle of the finished viewport (what bound_rect_kernel
+        // computes, in viewport rows): this wave's non-transparent pixels
+        // from one ballot (bit 8 r + c = pixel (x0 + c, y0 + r)), then an
+        // atomic only where it moves a bound (the bounds only shrink toward
+        // their result, so a stale read costs an atomic, never a bound)
+        const uint64_t m = __ballot(inside && (value >> 24) != 0u);
+        if (m != 0ull && lane == 0u) {
+            uint32_t cols = 0u, rows = 0u;
+#pragma unroll
+            for (uint32_t r = 0; r < 8u; ++r) {
+                const uint32_t b = (uint32_t)(m >> (8u * r)) & 0xFFu;
+                cols |= b;
+                rows |= (b != 0u ? 1u : 0u) << r;
+            }
+            const int mnx = (int)(x0 + (uint32_t)__builtin_ctz(cols));
+            const int mxx = (int)(x0 + 31u - (uint32_t)__builtin_clz(cols));
+            const int mny = (int)(H - 1u - (y0 + 31u - (uint32_t)__builtin_clz(rows)));
+            const int mxy = (int)(H - 1u - (y0 + (uint32_t)__builtin_ctz(rows)));
+            if (mnx < __hip_atomic_load(rect_acc + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin(rect_acc + 0, mnx);
+            if (mny < __hip_atomic_load(rect_acc + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin(rect_acc + 1, mny);
+            if (mxx > __hip_atomic_load(rect_acc + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(rect_acc + 2, mxx);
+            if (mxy > __hip_atomic_load(rect_acc + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(rect_acc + 3, mxy);
+        }
+    }
 }
 
 // The bricks of an interpolating frame as a launch of their own (a split
@@ -1496,7 +1525,7 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
                         uint32_t nblocks, uint32_t layout, uint32_t field_mode,
                         bool exact, float iso_opacity, const LevelsArg& lev, float smin, float smax,
                         const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
-                        rv_stream_t stream)
+                        rv_stream_t stream, int* d_rect = nullptr)
 {
     // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
     // LDS per workgroup at most)
@@ -1509,11 +1538,11 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
         if (layout == RV_BRICK_TILED4)                                                             \
             hipLaunchKernelGGL((share_kernel<M, Tiled4>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
                                db, db2, bin_off, bin_list, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
-                               vp_w, vp_h);                                                        \
+                               vp_w, vp_h, d_rect);                                                \
         else                                                                                       \
             hipLaunchKernelGGL((share_kernel<M, float>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
                                db, db2, bin_off, bin_list, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
-                               vp_w, vp_h);                                                        \
+                               vp_w, vp_h, d_rect);                                                \
     } while (0)
     switch (field_mode) {
     case RV_FIELD_ARGB:
@@ -1595,11 +1624,15 @@ static int share_table(const rv_block_params* params, const float* const* bricks
 // The staged table from pinned host memory into device memory by the
 // compute queue itself (a kernel reading host memory): the share launch
 // that follows waits for no copy engine.
+// The staged table into device memory; rect (or null) is set to the empty
+// crop rectangle the share kernel then shrinks.
 __global__ __launch_bounds__(256) static void stage_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
-                                                                uint64_t n16)
+                                                                uint64_t n16, int* __restrict__ rect)
 {
     for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n16; i += (uint64_t)gridDim.x * 256u)
         dst[i] = src[i];
+    if (rect && blockIdx.x == 0u && threadIdx.x < 4u)
+        rect[threadIdx.x] = threadIdx.x < 2u ? 0x7fffffff : -1;
 }
 
 // The share launch over a table staged in pinned host memory and copied in
@@ -1613,9 +1646,11 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
                             const float* const* bricks2, uint32_t nblocks, uint32_t layout, uint32_t field_mode,
                             float iso_opacity, const float* levels_rel, uint32_t level_count, float smin,
                             float smax, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w,
-                            uint32_t vp_h, ShareStaging& st, rv_stream_t stream)
+                            uint32_t vp_h, ShareStaging& st, rv_stream_t stream, int* d_rect, bool* rect_done)
 {
     RV_TRY
+    if (rect_done)
+        *rect_done = false;
     if (nblocks == 0)
         return RV_OK;
     const uint32_t gx = (vp_w + 15u) / 16u, gy = (vp_h + 15u) / 16u;
@@ -1676,17 +1711,20 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
     uint8_t* d = static_cast<uint8_t*>(st.dev);
     const uint64_t n16 = bytes / 16u;
     hipLaunchKernelGGL(stage_copy_kernel, dim3((uint32_t)std::min<uint64_t>(256u, (n16 + 255u) / 256u)), dim3(256),
-                       0, S(stream), static_cast<const uint4*>(st.host_dev), reinterpret_cast<uint4*>(d), n16);
+                       0, S(stream), static_cast<const uint4*>(st.host_dev), reinterpret_cast<uint4*>(d), n16, d_rect);
     RV_LAUNCH_CHECK();
     RV_HIP(hipEventRecord(st.copied, S(stream)));
     const auto* dp = reinterpret_cast<const rv_block_params*>(d);
     const auto* dr = reinterpret_cast<const uint4*>(d + rect_at);
     const auto* db = reinterpret_cast<const void* const*>(d + rect_at + sizeof(uint4) * (uint64_t)nblocks);
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
-    return launch_share(dp, dr, db, db + nblocks, reinterpret_cast<const uint32_t*>(d + off_at),
+    rc = launch_share(dp, dr, db, db + nblocks, reinterpret_cast<const uint32_t*>(d + off_at),
                         reinterpret_cast<const uint32_t*>(d + list_at), nblocks, layout, field_mode,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
-                        d_viewport, vp_w, vp_h, stream);
+                        d_viewport, vp_w, vp_h, stream, d_rect);
+    if (rc == RV_OK && rect_done)
+        *rect_done = d_rect != nullptr;
+    return rc;
     RV_CATCH
 }
 

@@ -255,9 +255,11 @@ __device__ inline void interp_tile(const Job& job, uint32_t local_tile, uint32_t
         // 512-thread workgroups (2 per CU, bound by LDS) issue 24 loads per
         // thread at a time (about 100 VGPRs, still 4 waves per SIMD): a
         // 65 x 16 x 16 full tile takes two rounds of plan then value loads
-        // instead of five.  Smaller workgroups share the CU with more of
-        // their kind and keep 8.
-        constexpr uint32_t kUnroll = NT >= 512u ? 24u : 8u;
+        // instead of five.  256-thread workgroups (up to 7 per CU) issue 12
+        // (56 VGPRs, still 7 waves per SIMD): a 17^3 brick in two rounds
+        // (12.4 -> 11.9 ms for 2.0 M of them; 20 per thread, one round at 5
+        // waves per SIMD, measured 12.1 ms).
+        constexpr uint32_t kUnroll = NT >= 512u ? 24u : 12u;
         bool saw_nan = false;
         const uint32_t eend = full ? e_own : e1;
         for (uint32_t k0 = e0 + threadIdx.x; k0 < eend; k0 += NT * kUnroll) {

@@ -475,6 +475,15 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
         m_viewportPixels = px;
     }
     m_renderer.clearViewport(m_viewport);
+    // the fused path interpolates the share's bricks first: the block order
+    // and parameters are then computed on the host while the GPU works
+    if (m_fused) {
+        interpolateFused(sub, isCancelled);
+        if (m_cancelled) {
+            hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
+            return part;
+        }
+    }
     // MacroBlock over the index box + BackToFrontOrder from the eye
     // (VsWorker_cu.cpp:254-262)
     float cmin[3], cmax[3], eye[3];
@@ -487,6 +496,7 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
     uint64_t got = 0;
     throwIfFailed(rv_back_to_front(sub.boxMin.data(), sub.boxMax.data(), cmin, cmax, eye,
                                    order.data(), count, &got));
+    m_rectDone = false;
     if (m_fused) {
         renderFused(sub, order, got, isCancelled);
     } else {
@@ -513,7 +523,8 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
     int nonempty = 0;
     if (!m_rectBuf)
         hipCheck(hipMalloc((void**)&m_rectBuf, 16), "hipMalloc");
-    throwIfFailed(rv::bound_rect_launch(m_viewport, W, H, m_rectBuf, hs(stream())));
+    if (!m_rectDone)
+        throwIfFailed(rv::bound_rect_launch(m_viewport, W, H, m_rectBuf, hs(stream())));
     void* mapped = hostDst ? mappedHost(hostDst) : nullptr;
     if (mapped) {
         if (!m_rectHost) {
@@ -577,8 +588,7 @@ void* VsWorkerHip::mappedHost(void* p)
     return nullptr;
 }
 
-void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vector<uint32_t>& order,
-                              uint64_t count, const std::function<bool()>& isCancelled)
+void VsWorkerHip::interpolateFused(const SubtreeSetDescription& sub, const std::function<bool()>& isCancelled)
 {
     const RenderSharedState& st = *m_state;
     FieldServiceHip& fs = *st.fieldSvc;
@@ -595,11 +605,16 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
     }
     auto hit = sc.batches.find(key);
     if (hit == sc.batches.end()) {
+        // the box's blocks in index order (the interpolation needs no order)
+        std::vector<std::array<uint32_t, 3>> blocks;
+        for (uint32_t z = sub.boxMin[2]; z < sub.boxMax[2]; ++z)
+            for (uint32_t y = sub.boxMin[1]; y < sub.boxMax[1]; ++y)
+                for (uint32_t x = sub.boxMin[0]; x < sub.boxMax[0]; ++x)
+                    blocks.push_back({x, y, z});
         std::vector<rv_brick_job> jobs;
-        jobs.reserve(second ? 2 * count : count);
+        jobs.reserve(second ? 2 * blocks.size() : blocks.size());
         bool grown = false;
-        for (uint64_t i = 0; i < count; ++i) {
-            const std::array<uint32_t, 3> idx{order[3 * i], order[3 * i + 1], order[3 * i + 2]};
+        for (const auto& idx : blocks) {
             const std::array<uint32_t, 4> bkey{sub.level, idx[0], idx[1], idx[2]};
             // the share's bricks in micro-bricks (fewer cache lines per
             // ray-cast corner load; the same values)
@@ -632,6 +647,16 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         return;
     }
     throwIfFailed(rv_interp_batch_run(hit->second.batch, s));
+}
+
+void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vector<uint32_t>& order,
+                              uint64_t count, const std::function<bool()>& isCancelled)
+{
+    const RenderSharedState& st = *m_state;
+    FieldServiceHip& fs = *st.fieldSvc;
+    rv_stream_t s = stream();
+    ShareCache& sc = *m_share;
+    const bool second = hasSecondaryField(st);
     // colour table (as the per-block renderer keeps it)
     m_renderer.ensureColorTable();
     const auto range = fs.fieldRange();
@@ -664,11 +689,15 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
     const auto srange = fs.secondaryRange();
     // the table and the screen bins go through pinned staging memory,
     // asynchronously (the next part waits for this copy before rewriting it)
+    // the share launch also leaves the crop rectangle in m_rectBuf
+    if (!m_rectBuf)
+        hipCheck(hipMalloc((void**)&m_rectBuf, 16), "hipMalloc");
     throwIfFailed(rv::render_share_staged(
         params.data(), bricks.data(), second ? bricks2.data() : nullptr, (uint32_t)params.size(),
         RV_BRICK_TILED4, st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
         (uint32_t)st.isosurfaceLevels.size(), srange[0], srange[1], m_renderer.colorTable(),
-        m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1], sc.table, s));
+        m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1], sc.table, s, m_rectBuf,
+        &m_rectDone));
 }
 
 const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const
