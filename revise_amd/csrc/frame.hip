@@ -338,13 +338,28 @@ __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 // ray-cast into the viewport with exactly the per-block operation of the
 // per-block renderer -- pixel = blendAbove(pixel, segment) through the 8-bit
 // ARGB round trip -- so the result equals one launch per block in that order,
-// bit for bit.  Each wave owns an 8 x 8 pixel square (a workgroup a 16 x 16
-// tile) and finds its blocks itself: the view rectangles are tested 64 at a
-// time and the hits of each ballot visited in order, with no workgroup
-// barrier after the colour table is loaded.  The dynamic LDS is padded so
-// that at most kShareWgsPerCu workgroups share a CU: fewer concurrent rays
-// keep more of each brick neighbourhood in the vector L1.
+// bit for bit.  A workgroup owns 16 x 8 pixels, two 8 x 8 squares with two
+// waves each.  Blocks are independent (a segment depends on its block and
+// pixel only; only the 8-bit composite is ordered), so a square's blocks are
+// split between its two waves: the back wave composites the farther ones in
+// order while the front wave marches the nearest ones (at most kShareFront,
+// at most half) into LDS, and after a barrier the back wave composites those
+// in order.  A square whose rays cross many blocks so takes about half the
+// time: the launch is otherwise bounded by its longest workgroups.  The
+// blocks meeting a square come from its screen bin's list (worker launches,
+// bins of 16 x 16 pixels, two workgroups each, taken in bin_order) or from
+// the view rectangles tested 64 at a time with ballots.  The dynamic LDS is
+// padded so that at most kShareWgsPerCu workgroups share a CU: fewer
+// concurrent rays keep more of each brick neighbourhood in the vector L1.
 constexpr uint32_t kShareT = 256;
+#ifndef RV_SHARE_SPLIT
+#define RV_SHARE_SPLIT 2
+#endif
+constexpr uint32_t kShareSplit = RV_SHARE_SPLIT;         // waves per 8 x 8 square
+constexpr uint32_t kShareSquares = 4u / kShareSplit;     // squares per workgroup
+constexpr uint32_t kShareParts = 4u / kShareSquares;     // workgroups per 16 x 16 bin
+constexpr uint32_t kShareFront = 8;                      // blocks per front wave at most
+constexpr uint32_t kShareSlots = (kShareSplit - 1u) * kShareFront; // per square
 #ifndef RV_SHARE_WGS_PER_CU
 #define RV_SHARE_WGS_PER_CU 4
 #endif
@@ -357,6 +372,7 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         const void* const* __restrict__ bricks2,
                                                         const uint32_t* __restrict__ bin_off,
                                                         const uint32_t* __restrict__ bin_list,
+                                                        const uint32_t* __restrict__ bin_order,
                                                         float smin, float smax,
                                                         uint32_t nblocks, const float4* __restrict__ tf,
                                                         uint32_t tf_n, float iso_opacity, LevelsArg lev,
@@ -366,6 +382,8 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
     extern __shared__ float4 tfs_share[]; // tf_n + 1 entries (and padding)
     __shared__ float lvs[kMaxLevels];
     __shared__ uint2 yz_s[2 * kYZTabEntries]; // TILED4 y/z offsets (fast Argb)
+    __shared__ float4 seg_s[kShareSquares][kShareSlots][64]; // the front waves' segments
+    __shared__ uint64_t seg_m[kShareSquares][kShareSlots];   // ... and the lanes that composite them
     stage_tf(tfs_share, tf, tf_n, threadIdx.x, kShareT);
     if (multi_level(kMode) && threadIdx.x < lev.n)
         lvs[threadIdx.x] = lev.v[threadIdx.x];
@@ -373,15 +391,42 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
         stage_yz_tables(yz_s, threadIdx.x, kShareT);
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    const uint32_t x0 = blockIdx.x * 16u + (wave & 1u) * 8u, y0 = blockIdx.y * 16u + (wave >> 1) * 8u;
-    if (x0 >= W || y0 >= H)
-        return; // the whole wave is outside the viewport
+    const uint32_t sq = wave % kShareSquares; // this wave's 8 x 8 square
+    const uint32_t rank = wave / kShareSquares; // 0: the square's back wave
+    const bool front = rank != 0u;
+    // the workgroup's squares: part (blockIdx.y % kShareParts) of the screen
+    // bin (blockIdx.x, blockIdx.y / kShareParts), or with bin_order (bins by
+    // decreasing block count, so the costly bins start first) part
+    // lin % kShareParts of bin bin_order[lin / kShareParts]
+    uint32_t bx = blockIdx.x, by = blockIdx.y / kShareParts, part = blockIdx.y % kShareParts;
+    if (bin_order) {
+        const uint32_t lin = blockIdx.y * gridDim.x + blockIdx.x;
+        const uint32_t b = bin_order[lin / kShareParts];
+        part = lin % kShareParts;
+        by = b / gridDim.x;
+        bx = b - by * gridDim.x;
+    }
+    const uint32_t sib = part * kShareSquares + sq; // the square within its bin, x first
+    const uint32_t x0 = bx * 16u + (sib & 1u) * 8u, y0 = by * 16u + (sib >> 1) * 8u;
+    const bool active = x0 < W && y0 < H; // wave-uniform
     const uint32_t x1 = min(x0 + 8u, W), y1 = min(y0 + 8u, H);
     const uint32_t vx = x0 + (lane & 7u), vy = y0 + (lane >> 3);
-    const bool inside = vx < W && vy < H;
+    const bool inside = active && vx < W && vy < H;
     const uint32_t pix = inside ? (H - 1u - vy) * W + vx : 0u;
-    uint32_t value = inside ? viewport[pix] : 0u;
-    // one block (wave-uniform) over the wave's pixels
+    uint32_t value = inside && !front ? viewport[pix] : 0u;
+    // the segment of one block (wave-uniform) at this lane's pixel; *hit:
+    // whether the per-block renderer composites it there
+    auto seg_of = [&](uint32_t blk, const rv_block_params& p, const RaySeg& r) -> float4 {
+        const BrickT* brick = reinterpret_cast<const BrickT*>(bricks[blk]);
+        Secondary<BrickT> sec{nullptr, smin, smax};
+        if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces)
+            sec.brick = reinterpret_cast<const BrickT*>(bricks2[blk]);
+        if constexpr (kMode == kModeArgbFast)
+            return march_fast(p, brick, tfs_share, (int)tf_n, r, yz_s);
+        else
+            return segment<kMode>(p, brick, tfs_share, (int)tf_n, iso_opacity, r, lvs, lev.n, sec);
+    };
+    // one block composited at the wave's pixels (the back wave)
     auto draw = [&](uint32_t blk) {
         const rv_block_params p = params[blk];
         const bool inrect = inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
@@ -389,53 +434,114 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
         if (inrect) {
             const RaySeg r = setup_ray(p, vx, vy);
             if (r.hit) {
-                const BrickT* brick = reinterpret_cast<const BrickT*>(bricks[blk]);
-                Secondary<BrickT> sec{nullptr, smin, smax};
-                if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces)
-                    sec.brick = reinterpret_cast<const BrickT*>(bricks2[blk]);
-                if (kMode == kModeMip)
+                if constexpr (kMode == kModeMip) {
+                    const BrickT* brick = reinterpret_cast<const BrickT*>(bricks[blk]);
                     value = mip_merge(value, march_mip(p, brick, r), p);
-                else if (multi_level(kMode) && lev.n == 0)
+                } else if (multi_level(kMode) && lev.n == 0) {
                     value = 0u; // renderIsosurfaces.hpp:33-34
-                else if (kMode == kModeArgbFast)
-                    value = blend_above_u32(value, march_fast(p, brick, tfs_share, (int)tf_n, r, yz_s));
-                else
-                    value = blend_above_u32(value, segment<kMode>(p, brick, tfs_share, (int)tf_n,
-                                                                  iso_opacity, r, lvs, lev.n, sec));
+                } else {
+                    value = blend_above_u32(value, seg_of(blk, p, r));
+                }
             }
         }
     };
-    if (bin_off) {
-        // the blocks whose view rectangles meet this workgroup's 16 x 16
-        // screen bin, back to front (the table's order)
-        const uint32_t bin = blockIdx.y * gridDim.x + blockIdx.x;
-        const uint32_t k1 = bin_off[bin + 1u];
-        for (uint32_t k = bin_off[bin]; k < k1; ++k) {
-            const uint32_t blk = bin_list[k];
-            const uint4 r = rects[blk];
-            if (r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0) // meets this wave's 8 x 8 pixels
-                draw(blk);
-        }
-    } else {
-        // every block's view rectangle, 64 at a time, tested with a ballot
-        for (uint32_t c = 0; c < nblocks; c += 64u) {
-            const uint32_t i = c + lane;
-            bool hit = false;
-            if (i < nblocks) {
-                const uint4 r = rects[i];
-                hit = r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0;
+    // one block's segment into LDS slot j (the front wave)
+    auto stash = [&](uint32_t blk, uint32_t j) {
+        const rv_block_params p = params[blk];
+        const bool inrect = inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
+                            vy < p.view_y + p.view_h;
+        bool hit = false;
+        float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (inrect) {
+            const RaySeg r = setup_ray(p, vx, vy);
+            if (r.hit) {
+                c = seg_of(blk, p, r);
+                hit = true;
             }
-            uint64_t m = __ballot(hit);
-            while (m) {
-                const uint32_t blk = c + (uint32_t)__builtin_ctzll(m);
-                m &= m - 1ull;
-                draw(blk);
+        }
+        seg_s[sq][j][lane] = c;
+        const uint64_t m = __ballot(hit);
+        if (lane == 0u)
+            seg_m[sq][j] = m;
+    };
+    // blocks whose view rectangle {x0, y0, x1, y1} meets this wave's square
+    auto meets = [&](const uint4& r) { return r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0; };
+    // the split: MIP merges are not blends, and Isosurfaces without levels
+    // clear the pixel per block (the back wave does all of those)
+    const bool split = kMode != kModeMip && !(multi_level(kMode) && lev.n == 0);
+    // a square's n blocks: front wave r takes the f blocks of hits
+    // [nb + (r - 1) f, nb + r f) into slots (r - 1) f .., the back wave the
+    // first nb = n - (kShareSplit - 1) f, with f = min(kShareFront, n /
+    // kShareSplit)
+    uint32_t nf = 0; // blocks all front waves take
+    if (active) {
+        if (bin_off) {
+            const uint32_t bin = by * gridDim.x + bx;
+            const uint32_t k0 = bin_off[bin], k1 = bin_off[bin + 1u];
+            uint32_t n = 0;
+            if (split)
+                for (uint32_t k = k0; k < k1; ++k)
+                    n += meets(rects[bin_list[k]]) ? 1u : 0u;
+            const uint32_t f = min(kShareFront, n / kShareSplit);
+            nf = (kShareSplit - 1u) * f;
+            const uint32_t nb = n - nf;
+            const uint32_t lo = nb + (rank - 1u) * f, hi = lo + f; // a front wave's hits
+            uint32_t h = 0;
+            for (uint32_t k = k0; k < k1 && !(front && (f == 0u || h >= hi)); ++k) {
+                const uint32_t blk = bin_list[k];
+                if (!meets(rects[blk]))
+                    continue;
+                if (!front) {
+                    if (split && h == nb)
+                        break;
+                    draw(blk);
+                } else if (h >= lo && h < hi) {
+                    stash(blk, h - nb); // < nf: n counted these blocks
+                }
+                ++h;
+            }
+        } else {
+            // every block's view rectangle, 64 at a time, tested with a ballot
+            uint32_t n = 0;
+            if (split)
+                for (uint32_t c = 0; c < nblocks; c += 64u)
+                    n += (uint32_t)__builtin_popcountll(__ballot(c + lane < nblocks && meets(rects[c + lane])));
+            const uint32_t f = min(kShareFront, n / kShareSplit);
+            nf = (kShareSplit - 1u) * f;
+            const uint32_t nb = n - nf;
+            const uint32_t lo = nb + (rank - 1u) * f, hi = lo + f; // a front wave's hits
+            uint32_t h = 0;
+            for (uint32_t c = 0; c < nblocks && !(!front && split && h == nb) && !(front && (f == 0u || h >= hi));
+                 c += 64u) {
+                const uint32_t i = c + lane;
+                uint64_t m = __ballot(i < nblocks && meets(rects[i]));
+                while (m) {
+                    const uint32_t blk = c + (uint32_t)__builtin_ctzll(m);
+                    m &= m - 1ull;
+                    if (!front) {
+                        if (split && h == nb)
+                            break;
+                        draw(blk);
+                    } else if (h >= lo && h < hi) {
+                        stash(blk, h - nb); // < nf: n counted these blocks
+                    }
+                    ++h;
+                }
             }
         }
     }
+    // one square per workgroup: its waves agree on nf, and with nothing
+    // handed over the front waves leave at once instead of at a barrier
+    if (kShareSquares > 1u || nf != 0u)
+        __syncthreads();
+    if (front)
+        return;
+    for (uint32_t j = 0; j < nf; ++j)
+        if ((seg_m[sq][j] >> lane) & 1ull)
+            value = blend_above_u32(value, seg_s[sq][j][lane]);
     if (inside)
         viewport[pix] = value;
-    if (rect_acc) {
+    if (rect_acc && active) {
         // the crop rectangle of the finished viewport (what bound_rect_kernel
         // computes, in viewport rows): this wave's non-transparent pixels
         // from one ballot (bit 8 r + c = pixel (x0 + c, y0 + r)), then an
@@ -1522,6 +1628,7 @@ extern "C" void rv_frame_plan_destroy(rv_frame_plan* P) { delete P; }
 // rectangles {x0, y0, x1, y1}, brick and secondary-brick pointers).
 static int launch_share(const rv_block_params* dp, const uint4* dr, const void* const* db,
                         const void* const* db2, const uint32_t* bin_off, const uint32_t* bin_list,
+                        const uint32_t* bin_order,
                         uint32_t nblocks, uint32_t layout, uint32_t field_mode,
                         bool exact, float iso_opacity, const LevelsArg& lev, float smin, float smax,
                         const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
@@ -1529,19 +1636,22 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
 {
     // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
     // LDS per workgroup at most)
+    // (the static LDS, mostly the front waves' segments, counted in)
+    constexpr size_t kStatic = (sizeof(float4) * 64u + 8u) * kShareSquares * kShareSlots +
+                               sizeof(uint2) * 2u * kYZTabEntries + 4u * kMaxLevels;
     const size_t lds = std::max<size_t>(sizeof(float4) * (tf_n + 1u),
-                                        std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u));
+                                        std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u - kStatic));
     const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
-    const dim3 grid((vp_w + 15) / 16, (vp_h + 15) / 16);
+    const dim3 grid((vp_w + 15) / 16, kShareParts * ((vp_h + 15) / 16)); // kShareParts per 16 x 16 bin
 #define RV_SHARE_LAUNCH(M)                                                                         \
     do {                                                                                           \
         if (layout == RV_BRICK_TILED4)                                                             \
             hipLaunchKernelGGL((share_kernel<M, Tiled4>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
-                               db, db2, bin_off, bin_list, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
+                               db, db2, bin_off, bin_list, bin_order, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
                                vp_w, vp_h, d_rect);                                                \
         else                                                                                       \
             hipLaunchKernelGGL((share_kernel<M, float>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
-                               db, db2, bin_off, bin_list, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
+                               db, db2, bin_off, bin_list, bin_order, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
                                vp_w, vp_h, d_rect);                                                \
     } while (0)
     switch (field_mode) {
@@ -1672,7 +1782,8 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
     const uint64_t rect_at = share_rect_at(nblocks);
     const uint64_t off_at = rect_at + (sizeof(uint4) + 2u * sizeof(void*)) * (uint64_t)nblocks;
     const uint64_t list_at = off_at + 4u * (nbins + 1u);
-    const uint64_t bytes = (list_at + 4u * entries + 15u) & ~15ull; // copied as 16-byte words
+    const uint64_t order_at = list_at + 4u * entries;
+    const uint64_t bytes = (order_at + 4u * nbins + 15u) & ~15ull; // copied as 16-byte words
     if (!st.copied)
         RV_HIP(hipEventCreateWithFlags(&st.copied, hipEventDisableTiming));
     // the previous copy out of the staging buffer has run (normally long
@@ -1700,6 +1811,21 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
     auto* hoff = reinterpret_cast<uint32_t*>(host + off_at);
     auto* hlist = reinterpret_cast<uint32_t*>(host + list_at);
     std::memcpy(hoff, off.data(), 4u * (nbins + 1u));
+    // the bins by decreasing block count (a counting sort; equal counts in
+    // bin order), the order the share kernel's workgroups take them in
+    {
+        uint32_t maxc = 0;
+        for (uint64_t b = 0; b < nbins; ++b)
+            maxc = std::max(maxc, off[b + 1] - off[b]);
+        std::vector<uint32_t> start(maxc + 2u, 0u);
+        for (uint64_t b = 0; b < nbins; ++b)
+            ++start[maxc - (off[b + 1] - off[b]) + 1u];
+        for (uint32_t c = 0; c <= maxc; ++c)
+            start[c + 1u] += start[c];
+        auto* horder = reinterpret_cast<uint32_t*>(host + order_at);
+        for (uint64_t b = 0; b < nbins; ++b)
+            horder[start[maxc - (off[b + 1] - off[b])]++] = (uint32_t)b;
+    }
     for (uint32_t i = 0; i < nblocks; ++i) {
         const rv_block_params& p = params[i];
         if (p.view_w == 0 || p.view_h == 0)
@@ -1719,7 +1845,8 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
     const auto* db = reinterpret_cast<const void* const*>(d + rect_at + sizeof(uint4) * (uint64_t)nblocks);
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
     rc = launch_share(dp, dr, db, db + nblocks, reinterpret_cast<const uint32_t*>(d + off_at),
-                        reinterpret_cast<const uint32_t*>(d + list_at), nblocks, layout, field_mode,
+                        reinterpret_cast<const uint32_t*>(d + list_at), reinterpret_cast<const uint32_t*>(d + order_at),
+                        nblocks, layout, field_mode,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
                         d_viewport, vp_w, vp_h, stream, d_rect);
     if (rc == RV_OK && rect_done)
@@ -1770,7 +1897,7 @@ extern "C" int rv_render_share_layout(const rv_block_params* params, const float
     const auto* db = reinterpret_cast<const void* const*>(d + share_rect_at(nblocks) + sizeof(uint4) * nblocks);
     // the levels are relative to the field's range, the same for every block
     const LevelsArg lev = make_levels(params[0].fmin, params[0].fmax, levels_rel, level_count);
-    return launch_share(dp, dr, db, db + nblocks, nullptr, nullptr, nblocks, layout, field_mode,
+    return launch_share(dp, dr, db, db + nblocks, nullptr, nullptr, nullptr, nblocks, layout, field_mode,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
                         d_viewport, vp_w, vp_h, stream);
     RV_CATCH
@@ -1895,7 +2022,7 @@ extern "C" int rv_render_share_argb_noalloc(const rv_block_params* params, const
     hipLaunchKernelGGL(share_rects_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, S(stream), dp, dr, nblocks);
     RV_LAUNCH_CHECK();
     // Argb reads no secondary bricks: their slot gets the primary table
-    return launch_share(dp, dr, db, db, nullptr, nullptr, nblocks, RV_BRICK_LINEAR, RV_FIELD_ARGB,
+    return launch_share(dp, dr, db, db, nullptr, nullptr, nullptr, nblocks, RV_BRICK_LINEAR, RV_FIELD_ARGB,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, 1.f, LevelsArg{}, 0.f, 1.f, d_tf, tf_n,
                         d_viewport, vp_w, vp_h, stream);
     RV_CATCH
