@@ -377,7 +377,8 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                                                         uint32_t nblocks, const float4* __restrict__ tf,
                                                         uint32_t tf_n, float iso_opacity, LevelsArg lev,
                                                         uint32_t* __restrict__ viewport,
-                                                        uint32_t W, uint32_t H, int* __restrict__ rect_acc)
+                                                        uint32_t W, uint32_t H, int* __restrict__ rect_acc,
+                                                        uint32_t init)
 {
     extern __shared__ float4 tfs_share[]; // tf_n + 1 entries (and padding)
     __shared__ float lvs[kMaxLevels];
@@ -413,7 +414,9 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
     const uint32_t vx = x0 + (lane & 7u), vy = y0 + (lane >> 3);
     const bool inside = active && vx < W && vy < H;
     const uint32_t pix = inside ? (H - 1u - vy) * W + vx : 0u;
-    uint32_t value = inside && !front ? viewport[pix] : 0u;
+    // init (unless ~0): the viewport's clear value, which the launch writes
+    // instead of reading (every pixel belongs to one back wave)
+    uint32_t value = inside && !front ? (init != ~0u ? init : viewport[pix]) : 0u;
     // the segment of one block (wave-uniform) at this lane's pixel; *hit:
     // whether the per-block renderer composites it there
     auto seg_of = [&](uint32_t blk, const rv_block_params& p, const RaySeg& r) -> float4 {
@@ -1632,7 +1635,7 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
                         uint32_t nblocks, uint32_t layout, uint32_t field_mode,
                         bool exact, float iso_opacity, const LevelsArg& lev, float smin, float smax,
                         const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h,
-                        rv_stream_t stream, int* d_rect = nullptr)
+                        rv_stream_t stream, int* d_rect = nullptr, uint32_t init = ~0u)
 {
     // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
     // LDS per workgroup at most)
@@ -1648,11 +1651,11 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
         if (layout == RV_BRICK_TILED4)                                                             \
             hipLaunchKernelGGL((share_kernel<M, Tiled4>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
                                db, db2, bin_off, bin_list, bin_order, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
-                               vp_w, vp_h, d_rect);                                                \
+                               vp_w, vp_h, d_rect, init);                                                \
         else                                                                                       \
             hipLaunchKernelGGL((share_kernel<M, float>), grid, dim3(kShareT), lds, S(stream), dp, dr, \
                                db, db2, bin_off, bin_list, bin_order, smin, smax, nblocks, tf4, tf_n, iso_opacity, lev, d_viewport, \
-                               vp_w, vp_h, d_rect);                                                \
+                               vp_w, vp_h, d_rect, init);                                                \
     } while (0)
     switch (field_mode) {
     case RV_FIELD_ARGB:
@@ -1756,7 +1759,8 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
                             const float* const* bricks2, uint32_t nblocks, uint32_t layout, uint32_t field_mode,
                             float iso_opacity, const float* levels_rel, uint32_t level_count, float smin,
                             float smax, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w,
-                            uint32_t vp_h, ShareStaging& st, rv_stream_t stream, int* d_rect, bool* rect_done)
+                            uint32_t vp_h, ShareStaging& st, rv_stream_t stream, int* d_rect, bool* rect_done,
+                            uint32_t init)
 {
     RV_TRY
     if (rect_done)
@@ -1848,7 +1852,7 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
                         reinterpret_cast<const uint32_t*>(d + list_at), reinterpret_cast<const uint32_t*>(d + order_at),
                         nblocks, layout, field_mode,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
-                        d_viewport, vp_w, vp_h, stream, d_rect);
+                        d_viewport, vp_w, vp_h, stream, d_rect, init);
     if (rc == RV_OK && rect_done)
         *rect_done = d_rect != nullptr;
     return rc;

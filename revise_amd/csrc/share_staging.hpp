@@ -28,10 +28,13 @@ struct ShareStaging {
 // of the finished viewport, {minX, minY, maxX, maxY} in viewport rows (maxX
 // -1 when every pixel is transparent), as bound_rect_launch computes it;
 // *rect_done tells whether the launch wrote it (no launch for no blocks).
+// init != ~0: the viewport is taken as filled with init (its clear value)
+// and is written whole, not read.
 int render_share_staged(const rv_block_params* params, const float* const* bricks, const float* const* bricks2,
                         uint32_t nblocks, uint32_t layout, uint32_t field_mode, float iso_opacity,
                         const float* levels_rel, uint32_t level_count, float smin, float smax, const float* d_tf,
                         uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w, uint32_t vp_h, ShareStaging& st,
-                        rv_stream_t stream, int* d_rect = nullptr, bool* rect_done = nullptr);
+                        rv_stream_t stream, int* d_rect = nullptr, bool* rect_done = nullptr,
+                        uint32_t init = ~0u);
 
 } // namespace rv

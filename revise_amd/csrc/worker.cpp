@@ -474,7 +474,10 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
         hipCheck(hipMalloc((void**)&m_viewport, 4 * px), "hipMalloc");
         m_viewportPixels = px;
     }
-    m_renderer.clearViewport(m_viewport);
+    // the fused path's share launch writes the cleared viewport itself (it
+    // is cleared here only when that launch does not happen)
+    if (!m_fused)
+        m_renderer.clearViewport(m_viewport);
     // the fused path interpolates the share's bricks first: the block order
     // and parameters are then computed on the host while the GPU works
     if (m_fused) {
@@ -514,6 +517,8 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
         hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
         return part;
     }
+    if (m_fused && !m_rectDone)
+        m_renderer.clearViewport(m_viewport); // no share launch (no live block)
     // Crop to the non-transparent rectangle (VsWorker_cu.cpp:279-306), on
     // the device: the rectangle and the packed part are computed in stream
     // order.  A pinned host destination receives the part and the rectangle
@@ -697,7 +702,7 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         RV_BRICK_TILED4, st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
         (uint32_t)st.isosurfaceLevels.size(), srange[0], srange[1], m_renderer.colorTable(),
         m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1], sc.table, s, m_rectBuf,
-        &m_rectDone));
+        &m_rectDone, st.fieldMode == RV_FIELD_MAX_INTENSITY_PROJECTION ? 0x7fc00000u : 0u));
 }
 
 const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const
