@@ -358,7 +358,7 @@ constexpr uint32_t kShareT = 256;
 constexpr uint32_t kShareSplit = RV_SHARE_SPLIT;         // waves per 8 x 8 square
 constexpr uint32_t kShareSquares = 4u / kShareSplit;     // squares per workgroup
 constexpr uint32_t kShareParts = 4u / kShareSquares;     // workgroups per 16 x 16 bin
-constexpr uint32_t kShareFront = 8;                      // blocks per front wave at most
+constexpr uint32_t kShareFront = 16;                     // blocks per front wave at most
 constexpr uint32_t kShareSlots = (kShareSplit - 1u) * kShareFront; // per square
 #ifndef RV_SHARE_WGS_PER_CU
 #define RV_SHARE_WGS_PER_CU 4
@@ -1642,8 +1642,9 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
     // (the static LDS, mostly the front waves' segments, counted in)
     constexpr size_t kStatic = (sizeof(float4) * 64u + 8u) * kShareSquares * kShareSlots +
                                sizeof(uint2) * 2u * kYZTabEntries + 4u * kMaxLevels;
-    const size_t lds = std::max<size_t>(sizeof(float4) * (tf_n + 1u),
-                                        std::min<size_t>(65536, (160u * 1024u) / kShareWgsPerCu - 1024u - kStatic));
+    constexpr size_t kPerWg = (160u * 1024u) / kShareWgsPerCu - 1024u;
+    static_assert(kPerWg > kStatic, "the static LDS leaves room for the colour table");
+    const size_t lds = std::max<size_t>(sizeof(float4) * (tf_n + 1u), std::min<size_t>(65536, kPerWg - kStatic));
     const auto* tf4 = reinterpret_cast<const float4*>(d_tf);
     const dim3 grid((vp_w + 15) / 16, kShareParts * ((vp_h + 15) / 16)); // kShareParts per 16 x 16 bin
 #define RV_SHARE_LAUNCH(M)                                                                         \
