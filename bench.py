@@ -741,11 +741,16 @@ def main():
                      "achieved": B_alg / t_interp / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": B_alg / t_interp / HBM_PEAK, "traffic": profiled_traffic(),
                      "alg_bytes_per_launch": B_alg,
+                     # the HBM bytes actually moved (the shared plan's coordinates come from cache)
+                     "traffic_gb_s": (profiled_traffic() or 0) / t_interp / 1e9,
+                     "traffic_frac": (profiled_traffic() or 0) / t_interp / HBM_PEAK,
                      "traffic_private_plans": profiled_traffic("traffic_bytes_per_launch_private_plans"),
                      "note": "achieved = algorithmic bytes (4 B per dense voxel written + 10 B per sparse "
                              "node read) / kernel time; traffic = HBM bytes per launch from rocprofv3 "
-                             "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy; the 512 bricks share one "
-                             "plan, which stays in cache (private plans: interp_ms_batched_private_plans). "
+                             "FETCH_SIZE/WRITE_SIZE scaled by a 1 GiB copy, traffic_frac its rate "
+                             "against the peak; the 512 bricks share one plan, which stays in cache, so "
+                             "fewer bytes cross HBM than the algorithmic count (private plans, every byte "
+                             "read: interp_ms_batched_private_plans, traffic_private_plans). "
                              "This is the step's HBM-bound kernel, in the interpolator's own (x-fastest) "
                              "layout; the drop-in frame interpolates into micro-bricks "
                              "(interp_ms_batched_micro_bricks). The step's largest kernel is the share "
