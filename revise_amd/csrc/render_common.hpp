@@ -602,7 +602,19 @@ __device__ __forceinline__ void fetch_fast_tab(const Tiled4* __restrict__ brick,
 
 __device__ __forceinline__ float lerp_fast(float a, float b, float t) { return fmaf(t, b - a, a); }
 
-template <class T, bool kTab>
+// A ray that grazes a block's edge can miss its entry face in fp32 and
+// then starts at the eye (tauEnter = 1, as in the reference), so its segment
+// is many times the block's length: a few rays per frame at some cameras,
+// each holding its wave for thousands of samples.  With kCoop (a call by the
+// whole wave on one block) every lane marches at most the cap (twice the
+// samples of the longest ray inside the block); each lane with samples left
+// is then continued by the whole wave, 64 samples at a time: the positions
+// by the same chain of fp32 additions (every lane runs it and keeps its
+// own), the loads, values and table lookups one sample per lane, and the
+// composite in sample order from values read out of each lane.  Every
+// operation is the one the lane would have done itself, so the result is
+// bit-identical to the one-lane march.
+template <class T, bool kTab, bool kCoop>
 __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, const T* __restrict__ brick,
                                                   const float4* tf, int tf_n, RaySeg r, const uint2* yz)
 {
@@ -618,20 +630,61 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
     const float tb = fmaf(-p.fmin, ta, -0.5f);
     const float thi = (float)(tf_n - 1);
     const float k = p.i_layer_count, of = p.opacity_factor, cut = p.alpha_cutoff;
+    // one sample's corners and weights at a world position
+    auto fetch_at = [&](float px, float py, float pz, float c[8], float& fx, float& fy, float& fz) {
+        const float tx = (px - p.tex_origin[0]) * p.tcf, ty = (py - p.tex_origin[1]) * p.tcf,
+                    tz = (pz - p.tex_origin[2]) * p.tcf;
+        if constexpr (kTab)
+            fetch_fast_tab(brick, n, tx, ty, tz, ytab, ztab, c, fx, fy, fz);
+        else
+            fetch_fast(brick, n, tx, ty, tz, c, fx, fy, fz);
+    };
+    auto value_of = [&](const float c[8], float fx, float fy, float fz) {
+        const float c00 = lerp_fast(c[0], c[1], fx), c10 = lerp_fast(c[2], c[3], fx);
+        const float c01 = lerp_fast(c[4], c[5], fx), c11 = lerp_fast(c[6], c[7], fx);
+        return lerp_fast(lerp_fast(c00, c10, fy), lerp_fast(c01, c11, fy), fz);
+    };
+    // the opacity-corrected alpha and the colour of a (non-NaN) value
+    auto classify_at = [&](float v, float& al, float& qr, float& qg, float& qb) {
+        const float x = __builtin_amdgcn_fmed3f(fmaf(v, ta, tb), 0.f, thi);
+        const float fl = floorf(x);
+        const float f = x - fl;
+        // tf[tf_n] repeats tf[tf_n - 1] (stage_tf), so i0 + 1 needs no clamp
+        const int i0 = (int)fl;
+        const float4 a = tf[i0], b = tf[i0 + 1];
+        al = 1.f - __builtin_amdgcn_exp2f(k * __builtin_amdgcn_logf(fmaf(-lerp_fast(a.w, b.w, f), of, 1.f)));
+        qr = lerp_fast(a.x, b.x, f);
+        qg = lerp_fast(a.y, b.y, f);
+        qb = lerp_fast(a.z, b.z, f);
+    };
     float cr = 0.f, cg = 0.f, cb = 0.f, A = 0.f; // premultiplied colour, alpha
-    for (int i = 1; i < r.sample_count; i += kBatch) {
+    // front to back: w = (1 - A) al
+    auto over = [&](float al, float qr, float qg, float qb, float& ar, float& ag, float& ab, float& aA) {
+        const float w = fmaf(-aA, al, al);
+        ar = fmaf(w, qr, ar);
+        ag = fmaf(w, qg, ag);
+        ab = fmaf(w, qb, ab);
+        aA = aA + w;
+    };
+    int cnt = r.sample_count;
+    int cap = 0;
+    bool coop = false;
+    if constexpr (kCoop) {
+        cap = (int)(2.f * p.quality * (float)n);
+        cap = max(kBatch, (cap + kBatch - 1) / kBatch * kBatch); // whole batches
+        coop = __ballot(true) == ~0ull;                          // the whole wave is here
+        if (coop)
+            cnt = min(cnt, cap + 1);
+    }
+    bool term = false;
+    for (int i = 1; i < cnt; i += kBatch) {
         // the corner loads of the whole batch are issued before any is used;
         // samples past the segment's end read clamped in-brick corners and
         // are dropped
         float c[kBatch][8], fx[kBatch], fy[kBatch], fz[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            const float tx = (r.px - p.tex_origin[0]) * p.tcf, ty = (r.py - p.tex_origin[1]) * p.tcf,
-                        tz = (r.pz - p.tex_origin[2]) * p.tcf;
-            if constexpr (kTab)
-                fetch_fast_tab(brick, n, tx, ty, tz, ytab, ztab, c[s], fx[s], fy[s], fz[s]);
-            else
-                fetch_fast(brick, n, tx, ty, tz, c[s], fx[s], fy[s], fz[s]);
+            fetch_at(r.px, r.py, r.pz, c[s], fx[s], fy[s], fz[s]);
             r.px = r.px + r.sx;
             r.py = r.py + r.sy;
             r.pz = r.pz + r.sz;
@@ -639,35 +692,76 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
         float v[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            const float c00 = lerp_fast(c[s][0], c[s][1], fx[s]), c10 = lerp_fast(c[s][2], c[s][3], fx[s]);
-            const float c01 = lerp_fast(c[s][4], c[s][5], fx[s]), c11 = lerp_fast(c[s][6], c[s][7], fx[s]);
             // + NaN past the end (an add, not a select, so no sample's loads
             // move into a branch)
-            v[s] = lerp_fast(lerp_fast(c00, c10, fy[s]), lerp_fast(c01, c11, fy[s]), fz[s]) +
-                   (i + s < r.sample_count ? 0.f : __builtin_nanf(""));
+            v[s] = value_of(c[s], fx[s], fy[s], fz[s]) + (i + s < r.sample_count ? 0.f : __builtin_nanf(""));
         }
         bool done = false;
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
             if (!done && v[s] == v[s]) {
-                const float x = __builtin_amdgcn_fmed3f(fmaf(v[s], ta, tb), 0.f, thi);
-                const float fl = floorf(x);
-                const float f = x - fl;
-                // tf[tf_n] repeats tf[tf_n - 1] (stage_tf), so i0 + 1 needs no clamp
-                const int i0 = (int)fl;
-                const float4 a = tf[i0], b = tf[i0 + 1];
-                const float al =
-                    1.f - __builtin_amdgcn_exp2f(k * __builtin_amdgcn_logf(fmaf(-lerp_fast(a.w, b.w, f), of, 1.f)));
-                const float w = fmaf(-A, al, al); // (1 - A) al
-                cr = fmaf(w, lerp_fast(a.x, b.x, f), cr);
-                cg = fmaf(w, lerp_fast(a.y, b.y, f), cg);
-                cb = fmaf(w, lerp_fast(a.z, b.z, f), cb);
-                A = A + w;
+                float al, qr, qg, qb;
+                classify_at(v[s], al, qr, qg, qb);
+                over(al, qr, qg, qb, cr, cg, cb, A);
                 done = A > cut;
             }
         }
-        if (done)
+        if (done) {
+            term = true;
             break;
+        }
+    }
+    if constexpr (kCoop) {
+        // lanes with samples left: sample i (from cap + 1 on) sits at the
+        // position the chain of i - 1 additions reaches
+        uint64_t left = coop ? __ballot(!term && r.sample_count > cap + 1) : 0ull;
+        const int me = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+        auto bcast = [](float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); };
+        while (left) {
+            const int L = (int)__builtin_ctzll(left);
+            left &= left - 1ull;
+            float Px = bcast(r.px, L), Py = bcast(r.py, L), Pz = bcast(r.pz, L);
+            const float Sx = bcast(r.sx, L), Sy = bcast(r.sy, L), Sz = bcast(r.sz, L);
+            const int N = __builtin_amdgcn_readlane(r.sample_count, L);
+            float ur = bcast(cr, L), ug = bcast(cg, L), ub = bcast(cb, L), uA = bcast(A, L);
+            bool udone = false;
+            for (int base = cap + 1; base < N && !udone; base += 64) {
+                float qx = 0.f, qy = 0.f, qz = 0.f;
+                for (int j = 0; j < 64; ++j) {
+                    if (j == me) {
+                        qx = Px;
+                        qy = Py;
+                        qz = Pz;
+                    }
+                    Px = Px + Sx;
+                    Py = Py + Sy;
+                    Pz = Pz + Sz;
+                }
+                float c[8], fx, fy, fz;
+                fetch_at(qx, qy, qz, c, fx, fy, fz);
+                const float v = value_of(c, fx, fy, fz) + (base + me < N ? 0.f : __builtin_nanf(""));
+                const bool valid = v == v;
+                float al = 0.f, qr = 0.f, qg = 0.f, qb = 0.f;
+                if (valid)
+                    classify_at(v, al, qr, qg, qb);
+                uint64_t vm = __ballot(valid);
+                while (vm) {
+                    const int j = (int)__builtin_ctzll(vm);
+                    vm &= vm - 1ull;
+                    over(bcast(al, j), bcast(qr, j), bcast(qg, j), bcast(qb, j), ur, ug, ub, uA);
+                    if (uA > cut) {
+                        udone = true;
+                        break;
+                    }
+                }
+            }
+            if (me == L) {
+                cr = ur;
+                cg = ug;
+                cb = ub;
+                A = uA;
+            }
+        }
     }
     if (!(A > 0.f))
         return make_float4(0.f, 0.f, 0.f, 0.f);
@@ -677,16 +771,18 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
 
 // The fast sequence over one block; yz: the LDS offset tables
 // (stage_yz_tables) or nullptr.  TILED4 bricks of depth <= 6 read their y
-// and z offsets from the tables.
-template <class T>
+// and z offsets from the tables.  kCoop: called by the whole wave on one
+// block (lanes off it with sample_count 0), so overlong segments are shared
+// by the wave (march_fast_impl).
+template <class T, bool kCoop = false>
 __device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* __restrict__ brick,
                                              const float4* tf, int tf_n, RaySeg r, const uint2* yz = nullptr)
 {
     if constexpr (std::is_same<T, Tiled4>::value) {
         if (yz != nullptr && p.tex_depth < kYZTabDepths)
-            return march_fast_impl<T, true>(p, brick, tf, tf_n, r, yz);
+            return march_fast_impl<T, true, kCoop>(p, brick, tf, tf_n, r, yz);
     }
-    return march_fast_impl<T, false>(p, brick, tf, tf_n, r, yz);
+    return march_fast_impl<T, false, kCoop>(p, brick, tf, tf_n, r, yz);
 }
 
 // ---------------------------------------------------------------- field modes

@@ -105,6 +105,61 @@ def test_config2_dropin_fast(config2):
     assert (d > 0).mean() < 0.01
 
 
+def _sample_counts(p):
+    """sampleCount of every pixel of a block's view rectangle (0 where the
+    ray misses): setup_ray's fp32 steps in numpy (DenseFieldRenderer_cu.cu:
+    62-175), the same operations in the same order."""
+    f = np.float32
+    vx, vy = np.meshgrid(np.arange(p.view_x, p.view_x + p.view_w, dtype=f),
+                         np.arange(p.view_y, p.view_y + p.view_h, dtype=f))
+    x = (f(-0.5) + (vx + f(0.5)) / f(p.vp_w)) * f(p.w)
+    y = (f(-0.5) + (vy + f(0.5)) / f(p.vp_h)) * f(p.h)
+    r = [(f(p.n[a]) + f(p.e1[a]) * x) + f(p.e2[a]) * y for a in range(3)]
+    ev, h = [f(p.pos[a]) for a in range(3)], f(p.half)
+    te, tx = np.full(x.shape, f(1.0)), np.zeros(x.shape, f)
+    with np.errstate(all="ignore"):
+        for a in range(3):
+            o1, o2 = (1 if a == 0 else 0), (1 if a == 2 else 2)
+            for side in range(2):
+                face = f(p.center[a]) - h if side == 0 else f(p.center[a]) + h
+                tau = (face - ev[a]) / r[a]
+                on = ((np.abs(r[a]) > 1e-10) & (np.abs(ev[o1] + r[o1] * tau - f(p.center[o1])) <= h)
+                      & (np.abs(ev[o2] + r[o2] * tau - f(p.center[o2])) <= h))
+                enter = (r[a] > 0) if side == 0 else ~(r[a] > 0)
+                te, tx = np.where(on & enter, tau, te), np.where(on & ~enter, tau, tx)
+        te = np.maximum(te, f(1.0))
+        dotrr = (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]
+        l1 = (np.abs(r[0]) + np.abs(r[1])) + np.abs(r[2])
+        sc = (f(p.quality) * (tx - te) * dotrr / (f(p.texel) * l1)).astype(np.int64)
+    return np.where(tx > te, sc, 0)
+
+
+def test_config2_dropin_fast_overlong_segments(config2):
+    """Cameras where rays grazing a block's edge miss its entry face in fp32
+    and start at the eye (tauEnter = 1, DenseFieldRenderer_cu.cu:164-165):
+    segments of thousands of samples, which the share kernel continues with
+    the whole wave; its frame equals the per-block launches' (one lane per
+    ray) bit for bit."""
+    angle = float(np.random.default_rng(SEED).uniform(0.0, 360.0))
+    longest = 0
+    for da in (15.0, 30.0, 45.0):
+        c = dict(config2, M=datasets.orbit_camera(angle + da))
+        for b in c["blocks"]:
+            live, p = api.block_params(c["M"], W, H, FOV, 3, b.index, 6, c["fmin"], c["fmax"], THR, Q, CUT2)
+            if live:
+                longest = max(longest, int(_sample_counts(p).max()))
+        parts = {}
+        for fused in (True, False):
+            w = _worker(c, fused, False)
+            parts[fused] = w.render_scene_part(3, (0, 0, 0), (8, 8, 8))
+            w.close()
+        say("config 2 fast, camera %+.0f deg: fused == per block" % da)
+        assert parts[True][0] == parts[False][0]
+        assert np.array_equal(parts[True][1], parts[False][1])
+    say("longest segment over the three cameras: %d samples" % longest)
+    assert longest > 1000  # (inside a block: 2 sqrt(3) 65 at most)
+
+
 @pytest.fixture(scope="module")
 def config3(orc):
     import torch
