@@ -341,31 +341,35 @@ __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 // bit for bit.  A workgroup owns 16 x 8 pixels, two 8 x 8 squares with two
 // waves each.  Blocks are independent (a segment depends on its block and
 // pixel only; only the 8-bit composite is ordered), so a square's blocks are
-// shared by its two waves from the two ends of its list: the back wave
-// composites from the farthest block forward, in order, while the front wave
-// marches from the nearest block backward (kShareFront blocks at most) into
-// LDS; each claims its next block with an LDS atomic on one word holding
-// both counts, so they meet wherever their costs balance (a block's cost
-// varies several times over with its segment lengths), and after a barrier
-// the back wave composites the front wave's segments in order.  The launch is
-// otherwise bounded by its longest squares.  The blocks meeting a square come
-// from its screen bin's list (worker launches, bins of 16 x 16 pixels, two
-// workgroups each, taken in bin_order) or from the view rectangles tested 64
-// at a time with ballots.  The dynamic LDS is padded so that at most
-// kShareWgsPerCu workgroups share a CU: fewer concurrent rays keep more of
-// each brick neighbourhood in the vector L1.
+// split between its two waves: the back wave composites the farther ones in
+// order while the front wave marches the nearest ones (at most kShareFront,
+// at most half) into LDS, and after a barrier the back wave composites those
+// in order.  A square whose rays cross many blocks so takes about half the
+// time: the launch is otherwise bounded by its longest workgroups.  The
+// blocks meeting a square come from its screen bin's list (worker launches,
+// bins of 16 x 16 pixels, two workgroups each, taken in bin_order) or from
+// the view rectangles tested 64 at a time with ballots.  The fast sequence
+// marches each block with the whole wave (lanes off the block get no
+// samples), so that the wave continues the overlong segment of a ray that
+// grazes a block's edge (march_fast).  The dynamic LDS is padded so that at
+// most kShareWgsPerCu workgroups share a CU: fewer concurrent rays keep more
+// of each brick neighbourhood in the vector L1.
 constexpr uint32_t kShareT = 256;
-constexpr uint32_t kShareSquares = 2;  // 8 x 8 squares per workgroup, two waves each
-constexpr uint32_t kShareParts = 2;    // workgroups per 16 x 16 bin
-constexpr uint32_t kShareFront = 16;   // blocks of a front wave at most (its LDS slots)
-constexpr uint32_t kShareSlots = kShareFront;
+#ifndef RV_SHARE_SPLIT
+#define RV_SHARE_SPLIT 2
+#endif
+constexpr uint32_t kShareSplit = RV_SHARE_SPLIT;         // waves per 8 x 8 square
+constexpr uint32_t kShareSquares = 4u / kShareSplit;     // squares per workgroup
+constexpr uint32_t kShareParts = 4u / kShareSquares;     // workgroups per 16 x 16 bin
+constexpr uint32_t kShareFront = 16;                     // blocks per front wave at most
+constexpr uint32_t kShareSlots = (kShareSplit - 1u) * kShareFront; // per square
 #ifndef RV_SHARE_WGS_PER_CU
 #define RV_SHARE_WGS_PER_CU 4
 #endif
 constexpr uint32_t kShareWgsPerCu = RV_SHARE_WGS_PER_CU;
 
 template <uint32_t kMode, class BrickT>
-__global__ __launch_bounds__(kShareT, 4) void share_kernel(const rv_block_params* __restrict__ params,
+__global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* __restrict__ params,
                                                         const uint4* __restrict__ rects,
                                                         const void* const* __restrict__ bricks,
                                                         const void* const* __restrict__ bricks2,
@@ -382,23 +386,18 @@ __global__ __launch_bounds__(kShareT, 4) void share_kernel(const rv_block_params
     extern __shared__ float4 tfs_share[]; // tf_n + 1 entries (and padding)
     __shared__ float lvs[kMaxLevels];
     __shared__ uint2 yz_s[2 * kYZTabEntries]; // TILED4 y/z offsets (fast Argb)
-    __shared__ float4 seg_s[kShareSquares][kShareSlots][64]; // the front waves' segments, nearest first
+    __shared__ float4 seg_s[kShareSquares][kShareSlots][64]; // the front waves' segments
     __shared__ uint64_t seg_m[kShareSquares][kShareSlots];   // ... and the lanes that composite them
-    __shared__ uint32_t claim_s[kShareSquares]; // blocks claimed: back wave | front wave << 16
-    __shared__ uint32_t nfront_s[kShareSquares]; // the front wave's segments
     stage_tf(tfs_share, tf, tf_n, threadIdx.x, kShareT);
     if (multi_level(kMode) && threadIdx.x < lev.n)
         lvs[threadIdx.x] = lev.v[threadIdx.x];
     if constexpr (std::is_same<BrickT, Tiled4>::value && kMode == kModeArgbFast)
         stage_yz_tables(yz_s, threadIdx.x, kShareT);
-    if (threadIdx.x < kShareSquares) {
-        claim_s[threadIdx.x] = 0u;
-        nfront_s[threadIdx.x] = 0u;
-    }
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t sq = wave % kShareSquares; // this wave's 8 x 8 square
-    const bool front = wave >= kShareSquares;  // else the square's back wave
+    const uint32_t rank = wave / kShareSquares; // 0: the square's back wave
+    const bool front = rank != 0u;
     // the workgroup's squares: part (blockIdx.y % kShareParts) of the screen
     // bin (blockIdx.x, blockIdx.y / kShareParts), or with bin_order (bins by
     // decreasing block count, so the costly bins start first) part
@@ -433,9 +432,6 @@ __global__ __launch_bounds__(kShareT, 4) void share_kernel(const rv_block_params
         else
             return segment<kMode>(p, brick, tfs_share, (int)tf_n, iso_opacity, r, lvs, lev.n, sec);
     };
-    // the fast sequence's ray of a block with the whole wave (lanes off the
-    // rectangle or the block get no samples), so that the march can share
-    // an overlong segment among the lanes (march_fast)
     auto wave_ray = [&](const rv_block_params& p, bool inrect) {
         RaySeg r = setup_ray(p, vx, vy);
         if (!inrect) {
@@ -495,77 +491,77 @@ __global__ __launch_bounds__(kShareT, 4) void share_kernel(const rv_block_params
     };
     // blocks whose view rectangle {x0, y0, x1, y1} meets this wave's square
     auto meets = [&](const uint4& r) { return r.x < r.z && r.x < x1 && r.z > x0 && r.y < y1 && r.w > y0; };
-    // the blocks meeting the square, in list order (back to front) or
-    // reversed; fn(blk) returns false to stop
-    auto for_hits = [&](bool reversed, auto&& fn) {
-        if (bin_off) {
-            const uint32_t bin = by * gridDim.x + bx;
-            const uint32_t k0 = bin_off[bin], k1 = bin_off[bin + 1u];
-            for (uint32_t i = 0; i < k1 - k0; ++i) {
-                const uint32_t blk = bin_list[reversed ? k1 - 1u - i : k0 + i];
-                if (meets(rects[blk]) && !fn(blk))
-                    return;
-            }
-        } else {
-            // every block's view rectangle, 64 at a time, tested with a ballot
-            const uint32_t chunks = (nblocks + 63u) / 64u;
-            for (uint32_t t = 0; t < chunks; ++t) {
-                const uint32_t c = (reversed ? chunks - 1u - t : t) * 64u;
-                uint64_t m = __ballot(c + lane < nblocks && meets(rects[c + lane]));
-                while (m) {
-                    const uint32_t bit = reversed ? 63u - (uint32_t)__builtin_clzll(m) : (uint32_t)__builtin_ctzll(m);
-                    m &= ~(1ull << bit);
-                    if (!fn(c + bit))
-                        return;
-                }
-            }
-        }
-    };
     // the split: MIP merges are not blends, and Isosurfaces without levels
     // clear the pixel per block (the back wave does all of those)
     const bool split = kMode != kModeMip && !(multi_level(kMode) && lev.n == 0);
-    // the square's n blocks: the back wave claims hits 0, 1, .. and the front
-    // wave hits n - 1, n - 2, .. (into slots 0, 1, ..); a claim succeeds while
-    // the two counts before it sum to less than n (the one LDS atomic orders
-    // the two waves' claims, so no block is taken twice or left out)
-    // (the waves are converged at a claim: lane 0 takes it for the wave)
-    auto claim = [&](uint32_t inc) -> uint32_t {
-        uint32_t old = 0u;
-        if (lane == 0u)
-            old = atomicAdd(&claim_s[sq], inc);
-        return (uint32_t)__builtin_amdgcn_readfirstlane((int)old);
-    };
+    // a square's n blocks: front wave r takes the f blocks of hits
+    // [nb + (r - 1) f, nb + r f) into slots (r - 1) f .., the back wave the
+    // first nb = n - (kShareSplit - 1) f, with f = min(kShareFront, n /
+    // kShareSplit)
+    uint32_t nf = 0; // blocks all front waves take
     if (active) {
-        uint32_t n = 0;
-        if (split)
-            for_hits(false, [&](uint32_t) { ++n; return true; });
-        const bool shared = n >= 2u && n < 0xFFFFu; // (both counts fit their 16 bits)
-        if (!front) {
+        if (bin_off) {
+            const uint32_t bin = by * gridDim.x + bx;
+            const uint32_t k0 = bin_off[bin], k1 = bin_off[bin + 1u];
+            uint32_t n = 0;
+            if (split)
+                for (uint32_t k = k0; k < k1; ++k)
+                    n += meets(rects[bin_list[k]]) ? 1u : 0u;
+            const uint32_t f = min(kShareFront, n / kShareSplit);
+            nf = (kShareSplit - 1u) * f;
+            const uint32_t nb = n - nf;
+            const uint32_t lo = nb + (rank - 1u) * f, hi = lo + f; // a front wave's hits
             uint32_t h = 0;
-            for_hits(false, [&](uint32_t blk) {
-                if (shared && h + (claim(1u) >> 16) >= n)
-                    return false;
-                draw(blk);
+            for (uint32_t k = k0; k < k1 && !(front && (f == 0u || h >= hi)); ++k) {
+                const uint32_t blk = bin_list[k];
+                if (!meets(rects[blk]))
+                    continue;
+                if (!front) {
+                    if (split && h == nb)
+                        break;
+                    draw(blk);
+                } else if (h >= lo && h < hi) {
+                    stash(blk, h - nb); // < nf: n counted these blocks
+                }
                 ++h;
-                return true;
-            });
-        } else if (shared) {
-            uint32_t j = 0;
-            for_hits(true, [&](uint32_t blk) {
-                if (j == kShareFront || (claim(1u << 16) & 0xFFFFu) + j >= n)
-                    return false;
-                stash(blk, j);
-                ++j;
-                return true;
-            });
-            if (lane == 0u)
-                nfront_s[sq] = j;
+            }
+        } else {
+            // every block's view rectangle, 64 at a time, tested with a ballot
+            uint32_t n = 0;
+            if (split)
+                for (uint32_t c = 0; c < nblocks; c += 64u)
+                    n += (uint32_t)__builtin_popcountll(__ballot(c + lane < nblocks && meets(rects[c + lane])));
+            const uint32_t f = min(kShareFront, n / kShareSplit);
+            nf = (kShareSplit - 1u) * f;
+            const uint32_t nb = n - nf;
+            const uint32_t lo = nb + (rank - 1u) * f, hi = lo + f; // a front wave's hits
+            uint32_t h = 0;
+            for (uint32_t c = 0; c < nblocks && !(!front && split && h == nb) && !(front && (f == 0u || h >= hi));
+                 c += 64u) {
+                const uint32_t i = c + lane;
+                uint64_t m = __ballot(i < nblocks && meets(rects[i]));
+                while (m) {
+                    const uint32_t blk = c + (uint32_t)__builtin_ctzll(m);
+                    m &= m - 1ull;
+                    if (!front) {
+                        if (split && h == nb)
+                            break;
+                        draw(blk);
+                    } else if (h >= lo && h < hi) {
+                        stash(blk, h - nb); // < nf: n counted these blocks
+                    }
+                    ++h;
+                }
+            }
         }
     }
-    __syncthreads();
+    // one square per workgroup: its waves agree on nf, and with nothing
+    // handed over the front waves leave at once instead of at a barrier
+    if (kShareSquares > 1u || nf != 0u)
+        __syncthreads();
     if (front)
         return;
-    for (uint32_t j = nfront_s[sq]; j-- > 0u;) // farthest of the front wave's first
+    for (uint32_t j = 0; j < nf; ++j)
         if ((seg_m[sq][j] >> lane) & 1ull)
             value = blend_above_u32(value, seg_s[sq][j][lane]);
     if (inside)
@@ -1666,7 +1662,7 @@ static int launch_share(const rv_block_params* dp, const uint4* dr, const void* 
     // LDS: the colour table, padded to cap the workgroups per CU (64 KiB of
     // LDS per workgroup at most)
     // (the static LDS, mostly the front waves' segments, counted in)
-    constexpr size_t kStatic = (sizeof(float4) * 64u + 8u) * kShareSquares * kShareSlots + 8u * kShareSquares +
+    constexpr size_t kStatic = (sizeof(float4) * 64u + 8u) * kShareSquares * kShareSlots +
                                sizeof(uint2) * 2u * kYZTabEntries + 4u * kMaxLevels;
     constexpr size_t kPerWg = (160u * 1024u) / kShareWgsPerCu - 1024u;
     static_assert(kPerWg > kStatic, "the static LDS leaves room for the colour table");
