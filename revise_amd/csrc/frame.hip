@@ -119,8 +119,28 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
     for (uint32_t k = a.bin_off[bin]; k < k1; ++k) {
         const uint32_t blk = uniform(a.bin_list[k]);
         const rv_block_params p = a.params[blk];
-        if (!opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
-            vy < p.view_y + p.view_h) {
+        const bool mine = !opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
+                          vy < p.view_y + p.view_h;
+        // micro-brick bricks: the fast sequence with the whole wave (lanes
+        // off the block get no samples), so that it shares overlong segments
+        // (march_fast); with the x-fastest layouts the wave-wide march would
+        // take the kernel past 128 VGPRs (4 waves per SIMD)
+        bool wave_fast = false;
+        if constexpr (kMode == kModeArgb && std::is_same<BrickT, Tiled4>::value)
+            wave_fast = a.fast && !a.tape;
+        if (wave_fast) {
+            RaySeg r = setup_ray(p, vx, vy);
+            if (!mine) {
+                r.hit = false;
+                r.sample_count = 0;
+            }
+            const float4 seg =
+                march_fast<BrickT, true>(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), tfs, (int)a.tf_n, r, yz);
+            if (r.hit) {
+                blend_behind(acc, seg);
+                opaque = acc.w > a.acc_cutoff;
+            }
+        } else if (mine) {
             const RaySeg r = setup_ray(p, vx, vy);
             if (r.hit) {
                 const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);

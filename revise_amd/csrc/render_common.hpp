@@ -727,6 +727,7 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
             bool udone = false;
             for (int base = cap + 1; base < N && !udone; base += 64) {
                 float qx = 0.f, qy = 0.f, qz = 0.f;
+#pragma unroll 1
                 for (int j = 0; j < 64; ++j) {
                     if (j == me) {
                         qx = Px;
