@@ -217,7 +217,24 @@ __device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4*
             live = false;
         if (!__any(found))
             break;
-        if (found) {
+        if (a.fast && !a.tape) {
+            // the fast sequence with the whole wave, each lane on its block
+            // (lanes without one: block 0, no samples), so that it shares
+            // overlong segments (march_fast)
+            const rv_block_params p = a.params[blk];
+            RaySeg r = setup_ray(p, vx, vy);
+            if (!found) {
+                r.hit = false;
+                r.sample_count = 0;
+            }
+            const float4 seg = march_fast<BrickT, true, true>(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), tfs,
+                                                              (int)a.tf_n, r, yz);
+            if (r.hit) {
+                blend_behind(acc, seg);
+                if (acc.w > a.acc_cutoff)
+                    live = false;
+            }
+        } else if (found) {
             const rv_block_params p = a.params[blk];
             const RaySeg r = setup_ray(p, vx, vy);
             if (r.hit) {
@@ -229,8 +246,6 @@ __device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4*
                     float ts;
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
-                } else if (a.fast) {
-                    seg = march_fast(p, brick, tfs, (int)a.tf_n, r, yz);
                 } else {
                     seg = march_impl<BrickT, false>(p, brick, tfs, (int)a.tf_n, r, nullptr);
                 }

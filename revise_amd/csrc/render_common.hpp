@@ -602,75 +602,140 @@ __device__ __forceinline__ void fetch_fast_tab(const Tiled4* __restrict__ brick,
 
 __device__ __forceinline__ float lerp_fast(float a, float b, float t) { return fmaf(t, b - a, a); }
 
+// The constants of the fast sequence over one block.
+struct FastBlock {
+    float ox, oy, oz, tcf; // texture coordinate (p - tex_origin) tcf
+    int n;                 // nodes per edge
+    uint32_t depth;        // tex_depth (the y/z tables of TILED4 bricks)
+    float ta, tb, thi;     // table coordinate v ta + tb, clamped to [0, thi]
+    float k, of, cut;      // opacity correction exponent and factor, cutoff
+};
+
+__device__ __forceinline__ FastBlock fast_block(const rv_block_params& p, int tf_n)
+{
+    FastBlock b;
+    b.ox = p.tex_origin[0];
+    b.oy = p.tex_origin[1];
+    b.oz = p.tex_origin[2];
+    b.tcf = p.tcf;
+    b.n = (int)p.nodes;
+    b.depth = p.tex_depth;
+    // table coordinate u N - 0.5 with u = (v - fmin) / (fmax - fmin)
+    b.ta = (float)tf_n / (p.fmax - p.fmin);
+    b.tb = fmaf(-p.fmin, b.ta, -0.5f);
+    b.thi = (float)(tf_n - 1);
+    b.k = p.i_layer_count;
+    b.of = p.opacity_factor;
+    b.cut = p.alpha_cutoff;
+    return b;
+}
+
+__device__ __forceinline__ float lane_f(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// A block's constants as lane l holds them, for the whole wave.
+__device__ __forceinline__ FastBlock lane_block(const FastBlock& b, int l)
+{
+    FastBlock u;
+    u.ox = lane_f(b.ox, l);
+    u.oy = lane_f(b.oy, l);
+    u.oz = lane_f(b.oz, l);
+    u.tcf = lane_f(b.tcf, l);
+    u.n = __builtin_amdgcn_readlane(b.n, l);
+    u.depth = (uint32_t)__builtin_amdgcn_readlane((int)b.depth, l);
+    u.ta = lane_f(b.ta, l);
+    u.tb = lane_f(b.tb, l);
+    u.thi = lane_f(b.thi, l);
+    u.k = lane_f(b.k, l);
+    u.of = lane_f(b.of, l);
+    u.cut = lane_f(b.cut, l);
+    return u;
+}
+
+template <class T> __device__ __forceinline__ const T* lane_ptr(const T* q, int l)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(q);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return reinterpret_cast<const T*>(((uint64_t)hi << 32) | lo);
+}
+
+// One sample's corners and weights at a world position.
+template <class T, bool kTab>
+__device__ __forceinline__ void fast_fetch_at(const FastBlock& b, const T* __restrict__ brick, const uint2* yz,
+                                              float px, float py, float pz, float c[8], float& fx, float& fy,
+                                              float& fz)
+{
+    const float tx = (px - b.ox) * b.tcf, ty = (py - b.oy) * b.tcf, tz = (pz - b.oz) * b.tcf;
+    if constexpr (kTab) {
+        const uint2* ytab = yz + yz_tab_base(b.depth);
+        fetch_fast_tab(brick, b.n, tx, ty, tz, ytab, ytab + kYZTabEntries, c, fx, fy, fz);
+    } else {
+        fetch_fast(brick, b.n, tx, ty, tz, c, fx, fy, fz);
+    }
+}
+
+__device__ __forceinline__ float fast_value(const float c[8], float fx, float fy, float fz)
+{
+    const float c00 = lerp_fast(c[0], c[1], fx), c10 = lerp_fast(c[2], c[3], fx);
+    const float c01 = lerp_fast(c[4], c[5], fx), c11 = lerp_fast(c[6], c[7], fx);
+    return lerp_fast(lerp_fast(c00, c10, fy), lerp_fast(c01, c11, fy), fz);
+}
+
+// The opacity-corrected alpha and the colour of a (non-NaN) value.
+__device__ __forceinline__ void fast_classify(const FastBlock& b, const float4* tf, float v, float& al, float& qr,
+                                              float& qg, float& qb)
+{
+    const float x = __builtin_amdgcn_fmed3f(fmaf(v, b.ta, b.tb), 0.f, b.thi);
+    const float fl = floorf(x);
+    const float f = x - fl;
+    // tf[tf_n] repeats tf[tf_n - 1] (stage_tf), so i0 + 1 needs no clamp
+    const int i0 = (int)fl;
+    const float4 ca = tf[i0], cb = tf[i0 + 1];
+    al = 1.f - __builtin_amdgcn_exp2f(b.k * __builtin_amdgcn_logf(fmaf(-lerp_fast(ca.w, cb.w, f), b.of, 1.f)));
+    qr = lerp_fast(ca.x, cb.x, f);
+    qg = lerp_fast(ca.y, cb.y, f);
+    qb = lerp_fast(ca.z, cb.z, f);
+}
+
+// Front to back, premultiplied: w = (1 - A) al.
+__device__ __forceinline__ void fast_over(float al, float qr, float qg, float qb, float& cr, float& cg, float& cb,
+                                          float& A)
+{
+    const float w = fmaf(-A, al, al);
+    cr = fmaf(w, qr, cr);
+    cg = fmaf(w, qg, cg);
+    cb = fmaf(w, qb, cb);
+    A = A + w;
+}
+
 // A ray that grazes a block's edge can miss its entry face in fp32 and
 // then starts at the eye (tauEnter = 1, as in the reference), so its segment
 // is many times the block's length: a few rays per frame at some cameras,
-// each holding its wave for thousands of samples.  With kCoop (a call by the
-// whole wave on one block) every lane marches at most the cap (twice the
-// samples of the longest ray inside the block); each lane with samples left
-// is then continued by the whole wave, 64 samples at a time: the positions
-// by the same chain of fp32 additions (every lane runs it and keeps its
-// own), the loads, values and table lookups one sample per lane, and the
-// composite in sample order from values read out of each lane.  Every
-// operation is the one the lane would have done itself, so the result is
-// bit-identical to the one-lane march.
-template <class T, bool kTab, bool kCoop>
+// each holding its wave for thousands of samples.  With kCoop (a call by
+// every lane of the wave; lanes with nothing to march pass sample_count 0)
+// every lane marches at most the cap (twice the samples of the longest ray
+// inside its block); each lane with samples left is then continued by the
+// whole wave, 64 samples at a time: the positions by the same chain of fp32
+// additions (every lane runs it and keeps its own), the loads, values and
+// table lookups one sample per lane, and the composite in sample order from
+// values read out of each lane.  Every operation is the one the lane would
+// have done itself, so the result is bit-identical to the one-lane march.
+// kLaneBlocks: the lanes may be on different blocks (and bricks), so the
+// continued lane's block constants and brick are read out of it too.
+template <class T, bool kTab, bool kCoop, bool kLaneBlocks>
 __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, const T* __restrict__ brick,
                                                   const float4* tf, int tf_n, RaySeg r, const uint2* yz)
 {
-    const int n = (int)p.nodes;
-    const uint2* ytab = nullptr;
-    const uint2* ztab = nullptr;
-    if constexpr (kTab) {
-        ytab = yz + yz_tab_base(p.tex_depth);
-        ztab = ytab + kYZTabEntries;
-    }
-    // table coordinate u N - 0.5 with u = (v - fmin) / (fmax - fmin)
-    const float ta = (float)tf_n / (p.fmax - p.fmin);
-    const float tb = fmaf(-p.fmin, ta, -0.5f);
-    const float thi = (float)(tf_n - 1);
-    const float k = p.i_layer_count, of = p.opacity_factor, cut = p.alpha_cutoff;
-    // one sample's corners and weights at a world position
-    auto fetch_at = [&](float px, float py, float pz, float c[8], float& fx, float& fy, float& fz) {
-        const float tx = (px - p.tex_origin[0]) * p.tcf, ty = (py - p.tex_origin[1]) * p.tcf,
-                    tz = (pz - p.tex_origin[2]) * p.tcf;
-        if constexpr (kTab)
-            fetch_fast_tab(brick, n, tx, ty, tz, ytab, ztab, c, fx, fy, fz);
-        else
-            fetch_fast(brick, n, tx, ty, tz, c, fx, fy, fz);
-    };
-    auto value_of = [&](const float c[8], float fx, float fy, float fz) {
-        const float c00 = lerp_fast(c[0], c[1], fx), c10 = lerp_fast(c[2], c[3], fx);
-        const float c01 = lerp_fast(c[4], c[5], fx), c11 = lerp_fast(c[6], c[7], fx);
-        return lerp_fast(lerp_fast(c00, c10, fy), lerp_fast(c01, c11, fy), fz);
-    };
-    // the opacity-corrected alpha and the colour of a (non-NaN) value
-    auto classify_at = [&](float v, float& al, float& qr, float& qg, float& qb) {
-        const float x = __builtin_amdgcn_fmed3f(fmaf(v, ta, tb), 0.f, thi);
-        const float fl = floorf(x);
-        const float f = x - fl;
-        // tf[tf_n] repeats tf[tf_n - 1] (stage_tf), so i0 + 1 needs no clamp
-        const int i0 = (int)fl;
-        const float4 a = tf[i0], b = tf[i0 + 1];
-        al = 1.f - __builtin_amdgcn_exp2f(k * __builtin_amdgcn_logf(fmaf(-lerp_fast(a.w, b.w, f), of, 1.f)));
-        qr = lerp_fast(a.x, b.x, f);
-        qg = lerp_fast(a.y, b.y, f);
-        qb = lerp_fast(a.z, b.z, f);
-    };
+    const FastBlock fb = fast_block(p, tf_n);
     float cr = 0.f, cg = 0.f, cb = 0.f, A = 0.f; // premultiplied colour, alpha
-    // front to back: w = (1 - A) al
-    auto over = [&](float al, float qr, float qg, float qb, float& ar, float& ag, float& ab, float& aA) {
-        const float w = fmaf(-aA, al, al);
-        ar = fmaf(w, qr, ar);
-        ag = fmaf(w, qg, ag);
-        ab = fmaf(w, qb, ab);
-        aA = aA + w;
-    };
     int cnt = r.sample_count;
     int cap = 0;
     bool coop = false;
     if constexpr (kCoop) {
-        cap = (int)(2.f * p.quality * (float)n);
+        cap = (int)(2.f * p.quality * (float)fb.n);
         cap = max(kBatch, (cap + kBatch - 1) / kBatch * kBatch); // whole batches
         coop = __ballot(true) == ~0ull;                          // the whole wave is here
         if (coop)
@@ -684,7 +749,7 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
         float c[kBatch][8], fx[kBatch], fy[kBatch], fz[kBatch];
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
-            fetch_at(r.px, r.py, r.pz, c[s], fx[s], fy[s], fz[s]);
+            fast_fetch_at<T, kTab>(fb, brick, yz, r.px, r.py, r.pz, c[s], fx[s], fy[s], fz[s]);
             r.px = r.px + r.sx;
             r.py = r.py + r.sy;
             r.pz = r.pz + r.sz;
@@ -694,16 +759,16 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
         for (int s = 0; s < kBatch; ++s) {
             // + NaN past the end (an add, not a select, so no sample's loads
             // move into a branch)
-            v[s] = value_of(c[s], fx[s], fy[s], fz[s]) + (i + s < r.sample_count ? 0.f : __builtin_nanf(""));
+            v[s] = fast_value(c[s], fx[s], fy[s], fz[s]) + (i + s < r.sample_count ? 0.f : __builtin_nanf(""));
         }
         bool done = false;
 #pragma unroll
         for (int s = 0; s < kBatch; ++s) {
             if (!done && v[s] == v[s]) {
                 float al, qr, qg, qb;
-                classify_at(v[s], al, qr, qg, qb);
-                over(al, qr, qg, qb, cr, cg, cb, A);
-                done = A > cut;
+                fast_classify(fb, tf, v[s], al, qr, qg, qb);
+                fast_over(al, qr, qg, qb, cr, cg, cb, A);
+                done = A > fb.cut;
             }
         }
         if (done) {
@@ -716,16 +781,23 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
         // position the chain of i - 1 additions reaches
         uint64_t left = coop ? __ballot(!term && r.sample_count > cap + 1) : 0ull;
         const int me = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-        auto bcast = [](float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); };
         while (left) {
             const int L = (int)__builtin_ctzll(left);
             left &= left - 1ull;
-            float Px = bcast(r.px, L), Py = bcast(r.py, L), Pz = bcast(r.pz, L);
-            const float Sx = bcast(r.sx, L), Sy = bcast(r.sy, L), Sz = bcast(r.sz, L);
+            FastBlock ub = fb;
+            const T* ubrick = brick;
+            int ucap = cap;
+            if constexpr (kLaneBlocks) {
+                ub = lane_block(fb, L);
+                ubrick = lane_ptr(brick, L);
+                ucap = __builtin_amdgcn_readlane(cap, L);
+            }
+            float Px = lane_f(r.px, L), Py = lane_f(r.py, L), Pz = lane_f(r.pz, L);
+            const float Sx = lane_f(r.sx, L), Sy = lane_f(r.sy, L), Sz = lane_f(r.sz, L);
             const int N = __builtin_amdgcn_readlane(r.sample_count, L);
-            float ur = bcast(cr, L), ug = bcast(cg, L), ub = bcast(cb, L), uA = bcast(A, L);
+            float ur = lane_f(cr, L), ug = lane_f(cg, L), ubl = lane_f(cb, L), uA = lane_f(A, L);
             bool udone = false;
-            for (int base = cap + 1; base < N && !udone; base += 64) {
+            for (int base = ucap + 1; base < N && !udone; base += 64) {
                 float qx = 0.f, qy = 0.f, qz = 0.f;
 #pragma unroll 1
                 for (int j = 0; j < 64; ++j) {
@@ -739,18 +811,18 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
                     Pz = Pz + Sz;
                 }
                 float c[8], fx, fy, fz;
-                fetch_at(qx, qy, qz, c, fx, fy, fz);
-                const float v = value_of(c, fx, fy, fz) + (base + me < N ? 0.f : __builtin_nanf(""));
+                fast_fetch_at<T, kTab>(ub, ubrick, yz, qx, qy, qz, c, fx, fy, fz);
+                const float v = fast_value(c, fx, fy, fz) + (base + me < N ? 0.f : __builtin_nanf(""));
                 const bool valid = v == v;
                 float al = 0.f, qr = 0.f, qg = 0.f, qb = 0.f;
                 if (valid)
-                    classify_at(v, al, qr, qg, qb);
+                    fast_classify(ub, tf, v, al, qr, qg, qb);
                 uint64_t vm = __ballot(valid);
                 while (vm) {
                     const int j = (int)__builtin_ctzll(vm);
                     vm &= vm - 1ull;
-                    over(bcast(al, j), bcast(qr, j), bcast(qg, j), bcast(qb, j), ur, ug, ub, uA);
-                    if (uA > cut) {
+                    fast_over(lane_f(al, j), lane_f(qr, j), lane_f(qg, j), lane_f(qb, j), ur, ug, ubl, uA);
+                    if (uA > ub.cut) {
                         udone = true;
                         break;
                     }
@@ -759,7 +831,7 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
             if (me == L) {
                 cr = ur;
                 cg = ug;
-                cb = ub;
+                cb = ubl;
                 A = uA;
             }
         }
@@ -772,18 +844,20 @@ __device__ __forceinline__ float4 march_fast_impl(const rv_block_params& p, cons
 
 // The fast sequence over one block; yz: the LDS offset tables
 // (stage_yz_tables) or nullptr.  TILED4 bricks of depth <= 6 read their y
-// and z offsets from the tables.  kCoop: called by the whole wave on one
-// block (lanes off it with sample_count 0), so overlong segments are shared
-// by the wave (march_fast_impl).
-template <class T, bool kCoop = false>
+// and z offsets from the tables.  kCoop: called by every lane of the wave
+// (lanes with nothing to march pass sample_count 0), so overlong segments
+// are shared by the wave (march_fast_impl); kLaneBlocks: the lanes may be
+// on different blocks.  A wave whose lanes take both table forms continues
+// them one lane each.
+template <class T, bool kCoop = false, bool kLaneBlocks = false>
 __device__ __forceinline__ float4 march_fast(const rv_block_params& p, const T* __restrict__ brick,
                                              const float4* tf, int tf_n, RaySeg r, const uint2* yz = nullptr)
 {
     if constexpr (std::is_same<T, Tiled4>::value) {
         if (yz != nullptr && p.tex_depth < kYZTabDepths)
-            return march_fast_impl<T, true, kCoop>(p, brick, tf, tf_n, r, yz);
+            return march_fast_impl<T, true, kCoop, kLaneBlocks>(p, brick, tf, tf_n, r, yz);
     }
-    return march_fast_impl<T, false, kCoop>(p, brick, tf, tf_n, r, yz);
+    return march_fast_impl<T, false, kCoop, kLaneBlocks>(p, brick, tf, tf_n, r, yz);
 }
 
 // ---------------------------------------------------------------- field modes

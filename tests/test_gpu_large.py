@@ -74,6 +74,45 @@ def test_large_share_reduced_matches_oracle(orc):
     fp.close()
 
 
+def test_large_share_fast_lanes_equal_bins():
+    """The fast sequence over small fp16 blocks (level 6: 31,457 blocks,
+    1920x1080), where rays grazing a block's edge start at the eye and march
+    hundreds to thousands of samples: the per-lane walk, whose waves continue
+    such segments together with each lane's own block (march_fast), gives
+    the float frame of the bin walk's one-lane march bit for bit."""
+    import torch
+    from tests.test_gpu_fullsize import _sample_counts
+    ls = _share(6, seed=61)
+    W, H = 1920, 1080
+    tf = torch.from_numpy(datasets.seeded_tf(61, 256)).cuda()
+    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf, (8, 8))
+    cams = np.stack([datasets.orbit_camera(a, elevation_deg=e) for a, e in ((35.0, 15.0), (220.0, -25.0),
+                                                                          (300.0, 5.0))])
+    fp.set_cameras(cams)
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    longest = 0
+    for c in range(3):
+        for k in ls.keys[::7]:  # (a sample of the blocks, for the assertion below)
+            live, p = api.block_params(cams[c], W, H, FOV, 6, tuple(int(v) for v in k), 4, ls.fmin, ls.fmax,
+                                       THR, Q, CUT)
+            if live:
+                longest = max(longest, int(_sample_counts(p).max()))
+        out = {}
+        for sch in (api.SCHEDULE_BINS, api.SCHEDULE_LANES):
+            fp.set_schedule(sch)
+            fp.render(c, argb, rgba, interpolate=False)
+            torch.cuda.synchronize()
+            assert fp.status() == 0
+            out[sch] = rgba.clone()
+        say("level-6 fast frame cam", c, "lanes == bins")
+        assert (out[api.SCHEDULE_BINS][..., 3] > 0).float().mean() > 0.03
+        assert torch.equal(out[api.SCHEDULE_BINS], out[api.SCHEDULE_LANES])
+    fp.close()
+    say("longest segment among the sampled blocks: %d samples" % longest)
+    assert longest > 500  # (inside a 17^3 block: 2 sqrt(3) 17 at most)
+
+
 def test_large_share_tf_adjoint_matches_float64(orc):
     import torch
     ls = _share(4, seed=44)
