@@ -121,12 +121,10 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
         const rv_block_params p = a.params[blk];
         const bool mine = !opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
                           vy < p.view_y + p.view_h;
-        // micro-brick bricks: the fast sequence with the whole wave (lanes
-        // off the block get no samples), so that it shares overlong segments
-        // (march_fast); with the x-fastest layouts the wave-wide march would
-        // take the kernel past 128 VGPRs (4 waves per SIMD)
+        // the fast sequence with the whole wave (lanes off the block get no
+        // samples), so that it shares overlong segments (march_fast)
         bool wave_fast = false;
-        if constexpr (kMode == kModeArgb && std::is_same<BrickT, Tiled4>::value)
+        if constexpr (kMode == kModeArgb)
             wave_fast = a.fast && !a.tape;
         if (wave_fast) {
             RaySeg r = setup_ray(p, vx, vy);
@@ -153,6 +151,9 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
                     seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
                     trans = trans * ts;
                 } else if (kMode == kModeArgb && a.fast) {
+                    // (not taken: wave_fast covers it; with this one-lane
+                    // form present the fp16 and fp32 kernels compile to 13
+                    // fewer VGPRs, 4 waves per SIMD instead of 3)
                     seg = march_fast(p, brick, tfs, (int)a.tf_n, r, yz);
                 } else if (multi_level(kMode) && a.lev.n == 0) {
                     seg = make_float4(0.f, 0.f, 0.f, 0.f); // no levels: nothing to draw

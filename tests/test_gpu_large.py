@@ -77,9 +77,11 @@ def test_large_share_reduced_matches_oracle(orc):
 def test_large_share_fast_lanes_equal_bins():
     """The fast sequence over small fp16 blocks (level 6: 31,457 blocks,
     1920x1080), where rays grazing a block's edge start at the eye and march
-    hundreds to thousands of samples: the per-lane walk, whose waves continue
-    such segments together with each lane's own block (march_fast), gives
-    the float frame of the bin walk's one-lane march bit for bit."""
+    hundreds to thousands of samples, which the waves continue together
+    (march_fast): the per-lane walk, whose lanes are on different blocks,
+    gives the float frame of the bin walk (one block per wave) bit for bit.
+    (The one-block form equals the one-lane march of per-block launches:
+    test_gpu_fullsize.py.)"""
     import torch
     from tests.test_gpu_fullsize import _sample_counts
     ls = _share(6, seed=61)
