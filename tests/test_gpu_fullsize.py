@@ -240,4 +240,18 @@ def test_config3_frame_fast_float(orc, config3):
     err = np.abs(premul(rgba.cpu().numpy()) - premul(exp))
     say("config 3 fast float: max %.2e mean %.2e" % (err.max(), err.mean()))
     assert err.max() <= 2e-3
+    # the walks give the same float frame bit for bit: the bin walk (one
+    # block per wave), the per-lane walk (each lane on its own block, levels
+    # and depths mixed) and the hybrid, each continuing overlong segments
+    # with the whole wave
+    first = rgba.clone()
+    for sch in (api.SCHEDULE_BINS, api.SCHEDULE_LANES, api.SCHEDULE_HYBRID):
+        fp.set_schedule(sch)
+        if sch == api.SCHEDULE_HYBRID:
+            fp.set_hybrid_min(5)
+        fp.render(0, argb, rgba, interpolate=False)
+        torch.cuda.synchronize()
+        assert fp.status() == 0
+        say("config 3 fast float, schedule", sch)
+        assert torch.equal(rgba, first)
     fp.close()
