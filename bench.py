@@ -650,8 +650,9 @@ def main():
     t_step_pb = (time.perf_counter() - t0) / 2
     wpb.close()
     log("worker frame, per-block launches: %.3f ms" % (t_step_pb * 1e3))
-    # the fused two-launch frame composed from Python at one fixed camera
-    # (the kernels alone, no per-step host work)
+    # the same frame composed from Python at one fixed camera: viewport
+    # clear, the interpolation batch, the share launch that tests every view
+    # rectangle (no screen bins), the bound rectangle and a blocking download
     for _ in range(2):
         plain.fused_step()
     t_fixed = events_time(torch, lambda: plain.fused_step(), 10)
@@ -720,7 +721,7 @@ def main():
                    "blocks": len(wl.blocks), "brick_nodes": 65 ** 3, "viewport": [W, H], "seed": SEED,
                    "camera_angle_deg": angle},
         "per_block_launches_frame_ms": t_step_pb * 1e3,
-        "dropin_fixed_camera_kernels_ms": t_fixed * 1e3,
+        "python_composed_frame_fixed_camera_ms": t_fixed * 1e3,
         "interp_gvox_s": V_total / t_interp / 1e9,
         "interp_ms_batched": t_interp * 1e3,
         "interp_ms_batched_private_plans": t_interp_own * 1e3,
