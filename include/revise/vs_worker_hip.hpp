@@ -193,6 +193,7 @@ public:
     void interpolateFused(const SubtreeSetDescription& sub, const std::function<bool()>& isCancelled);
     void renderFused(const SubtreeSetDescription& sub, const std::vector<uint32_t>& order,
                      uint64_t count, const std::function<bool()>& isCancelled);
+    void markStagingCopied();
     bool lastCancelled() const { return m_cancelled; }
     rv_stream_t stream() const { return m_renderer.stream(); }
     // The device colour table of the current state (uploaded when it changed).

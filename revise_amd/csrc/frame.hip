@@ -1819,7 +1819,7 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
                             float iso_opacity, const float* levels_rel, uint32_t level_count, float smin,
                             float smax, const float* d_tf, uint32_t tf_n, uint32_t* d_viewport, uint32_t vp_w,
                             uint32_t vp_h, ShareStaging& st, rv_stream_t stream, int* d_rect, bool* rect_done,
-                            uint32_t init)
+                            uint32_t init, bool record_copied)
 {
     RV_TRY
     if (rect_done)
@@ -1902,7 +1902,6 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
     hipLaunchKernelGGL(stage_copy_kernel, dim3((uint32_t)std::min<uint64_t>(256u, (n16 + 255u) / 256u)), dim3(256),
                        0, S(stream), static_cast<const uint4*>(st.host_dev), reinterpret_cast<uint4*>(d), n16, d_rect);
     RV_LAUNCH_CHECK();
-    RV_HIP(hipEventRecord(st.copied, S(stream)));
     const auto* dp = reinterpret_cast<const rv_block_params*>(d);
     const auto* dr = reinterpret_cast<const uint4*>(d + rect_at);
     const auto* db = reinterpret_cast<const void* const*>(d + rect_at + sizeof(uint4) * (uint64_t)nblocks);
@@ -1912,6 +1911,11 @@ int rv::render_share_staged(const rv_block_params* params, const float* const* b
                         nblocks, layout, field_mode,
                         (params[0].flags & RV_RENDER_EXACT) != 0u, iso_opacity, lev, smin, smax, d_tf, tf_n,
                         d_viewport, vp_w, vp_h, stream, d_rect, init);
+    // recorded after the share launch, not between the two launches (a
+    // marker there delays the share kernel's start by several us); the
+    // next call reuses the staging buffer only after it
+    if (record_copied)
+        RV_HIP(hipEventRecord(st.copied, S(stream)));
     if (rc == RV_OK && rect_done)
         *rect_done = d_rect != nullptr;
     return rc;

@@ -514,6 +514,7 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
     }
     if (m_cancelled || isCancelled()) {
         m_cancelled = true;
+        markStagingCopied();
         hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
         return part;
     }
@@ -538,6 +539,7 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
         }
         throwIfFailed(rv::crop_launch(m_viewport, W, H, m_rectBuf, static_cast<uint32_t*>(mapped), hostCap,
                                       m_rectHostDev, hs(stream())));
+        markStagingCopied();
         hipCheck(hipStreamSynchronize(hs(stream())), "hipStreamSynchronize");
         m_partRect = {0, 0, 0, 0}; // the part is not kept on the device
         if (m_rectHost[2] < 0)
@@ -561,6 +563,7 @@ RgbaImagePart VsWorkerHip::renderScenePart(const SubtreeSetDescription& sub,
         m_partCap = px;
     }
     throwIfFailed(rv::crop_launch(m_viewport, W, H, m_rectBuf, m_part, m_partCap, nullptr, hs(stream())));
+    markStagingCopied();
     throwIfFailed(rv::bound_rect_read(m_rectBuf, rect, &nonempty, hs(stream())));
     if (!nonempty)
         return part;
@@ -702,7 +705,16 @@ void VsWorkerHip::renderFused(const SubtreeSetDescription& sub, const std::vecto
         RV_BRICK_TILED4, st.fieldMode, st.isosurfaceOpacity, st.isosurfaceLevels.data(),
         (uint32_t)st.isosurfaceLevels.size(), srange[0], srange[1], m_renderer.colorTable(),
         m_renderer.colorTableSize(), m_viewport, st.viewportSize[0], st.viewportSize[1], sc.table, s, m_rectBuf,
-        &m_rectDone, st.fieldMode == RV_FIELD_MAX_INTENSITY_PROJECTION ? 0x7fc00000u : 0u));
+        &m_rectDone, st.fieldMode == RV_FIELD_MAX_INTENSITY_PROJECTION ? 0x7fc00000u : 0u, false));
+}
+
+// The staged table's copy event of the fused path, recorded after the
+// part's last launch rather than between the table copy and the share
+// launch (where it would delay the share kernel).
+void VsWorkerHip::markStagingCopied()
+{
+    if (m_fused && m_share)
+        hipCheck(rv::share_staging_copied(m_share->table, hs(stream())), "hipEventRecord");
 }
 
 const uint32_t* VsWorkerHip::devicePart(std::array<int, 4>& rect) const
