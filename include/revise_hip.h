@@ -513,6 +513,15 @@ enum { RV_FRAME_SCHEDULE_BINS = 0, RV_FRAME_SCHEDULE_LANES = 1, RV_FRAME_SCHEDUL
 int rv_frame_set_schedule(rv_frame_plan* plan, uint32_t schedule);
 int rv_frame_get_schedule(const rv_frame_plan* plan, uint32_t* schedule);
 int rv_frame_set_hybrid_min(rv_frame_plan* plan, uint32_t blocks);
+/* How the resident-brick kernel hands out its screen bins.  0 (the
+ * default): a persistent grid whose waves take the next free bin from a
+ * queue.  1: one workgroup per 2 x 2 neighbouring bins, one bin per wave,
+ * so the waves that share a CU march neighbouring pixels through the same
+ * bricks; the grid's tuned workgroups per CU become a cap (LDS padding).
+ * Same images; rv_frame_autotune with interpolate == 0 picks the faster.
+ * No counterpart in the reference (its renderer launches one grid per
+ * block). */
+int rv_frame_set_grouped(rv_frame_plan* plan, int grouped);
 /* Interpolating frames as one launch (0: the bricks' tiles and the ray-cast
  * in one persistent kernel that waits for them) or two (1: a launch that
  * interpolates the bricks, then the resident-brick kernel with its own grid).

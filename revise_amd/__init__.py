@@ -174,6 +174,7 @@ _SIGS = {
     "rv_frame_set_schedule": (C.c_int, [_vp, C.c_uint32]),
     "rv_frame_get_schedule": (C.c_int, [_vp, _u32p]),
     "rv_frame_set_hybrid_min": (C.c_int, [_vp, C.c_uint32]),
+    "rv_frame_set_grouped": (C.c_int, [_vp, C.c_int]),
     "rv_frame_set_exact": (C.c_int, [_vp, C.c_int]),
     "rv_frame_set_split": (C.c_int, [_vp, C.c_int]),
     "rv_frame_get_split": (C.c_int, [_vp, C.POINTER(C.c_int)]),

@@ -432,6 +432,12 @@ class FramePlan:
         """SCHEDULE_HYBRID: per-lane walk for bins listing at least `blocks` blocks."""
         check(lib().rv_frame_set_hybrid_min(self.h, blocks))
 
+    def set_grouped(self, grouped):
+        """Resident-brick frames: one workgroup per 2 x 2 neighbouring bins
+        (True) or the persistent grid with its bin queue (False, the
+        default); the autotuner picks the faster."""
+        check(lib().rv_frame_set_grouped(self.h, int(bool(grouped))))
+
     def schedule(self):
         v = C.c_uint32()
         check(lib().rv_frame_get_schedule(self.h, C.byref(v)))

@@ -77,6 +77,7 @@ struct FrameArgs {
     const void* const* bricks2;    // secondary field bricks (ValueOnIsosurface(s) modes)
     float smin, smax;              // secondary field range
     uint32_t fast;                 // Argb without the tape: the fast sample sequence (march_fast)
+    uint32_t grouped;              // render_frame_kernel: 2 x 2 bins per workgroup (else the queue)
 };
 
 constexpr uint32_t kSpinLimit = 1u << 22;
@@ -350,8 +351,13 @@ __global__ __launch_bounds__(kT) void frame_kernel(FrameArgs a)
         render_bins<BrickT, kMode, kLanes>(a, tfs, lvs, yz);
 }
 
-// Frames of resident bricks: screen bins only, no interpolation LDS, so
-// more waves per CU.
+// Frames of resident bricks, screen bins only.  a.grouped: workgroup g
+// takes the 2 x 2 screen bins of group g (bins of the wave footprint, group
+// rows of gx = ceil(nbx / 2)), one per wave, so the four waves that share a
+// CU's vector L1 march neighbouring pixels through the same bricks (the
+// dynamic LDS caps the workgroups per CU); else a persistent grid whose
+// waves take the next free bin from the queue, which balances better where
+// bins differ much in cost.
 template <class BrickT, uint32_t kMode, bool kLanes>
 __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
 {
@@ -367,7 +373,23 @@ __global__ __launch_bounds__(kRT) void render_frame_kernel(FrameArgs a)
         yz = yz_s;
     }
     __syncthreads();
-    render_bins<BrickT, kMode, kLanes>(a, tfs_dyn, lvs, yz);
+    if (!a.grouped) {
+        render_bins<BrickT, kMode, kLanes>(a, tfs_dyn, lvs, yz);
+        return;
+    }
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t gx = (a.nbx + 1u) / 2u, nby = a.nbins / a.nbx;
+    const uint32_t bx = (blockIdx.x % gx) * 2u + (wave & 1u), by = (blockIdx.x / gx) * 2u + (wave >> 1);
+    if (bx >= a.nbx || by >= nby) // wave-uniform
+        return;
+    const uint32_t bin = by * a.nbx + bx;
+    if constexpr (kLanes) {
+        if (a.bin_off[bin + 1] - a.bin_off[bin] >= a.lanes_min) {
+            render_bin_lanes<BrickT>(a, bin, tfs_dyn, yz);
+            return;
+        }
+    }
+    render_bin<BrickT, kMode>(a, bin, tfs_dyn, lvs, yz);
 }
 
 // Worker share in one launch: the blocks (back to front) of one share are
@@ -786,16 +808,24 @@ void launch_sched(const rv_frame_plan* P, const FrameArgs& a, int interpolate, u
                   hipStream_t s)
 {
     if (!interpolate) {
-        const size_t lds = sizeof(float4) * (a.tf_n + 1u);
+        // grouped: one workgroup per 2 x 2 bins, the colour table's LDS
+        // padded so that at most render_grid / cus workgroups share a CU;
+        // else the persistent grid of render_grid workgroups
+        uint32_t wgs = P->render_grid;
+        size_t lds = sizeof(float4) * (a.tf_n + 1u);
+        if (a.grouped) {
+            wgs = ((a.nbx + 1u) / 2u) * ((a.nbins / a.nbx + 1u) / 2u);
+            const uint32_t per_cu = std::max<uint32_t>(1u, P->render_grid / std::max<uint32_t>(P->cus, 1u));
+            constexpr size_t kStatic = sizeof(float) * kMaxLevels + sizeof(uint2) * 2u * kYZTabEntries + 1024u;
+            const size_t cap = (160u * 1024u) / per_cu;
+            lds = std::max<size_t>(lds, std::min<size_t>(65536u, cap > kStatic ? cap - kStatic : 0u));
+        }
         if (P->fp16)
-            hipLaunchKernelGGL((render_frame_kernel<__half, kMode, kLanes>), dim3(P->render_grid),
-                               dim3(kRT), lds, s, a);
+            hipLaunchKernelGGL((render_frame_kernel<__half, kMode, kLanes>), dim3(wgs), dim3(kRT), lds, s, a);
         else if (P->tiled)
-            hipLaunchKernelGGL((render_frame_kernel<Tiled4, kMode, kLanes>), dim3(P->render_grid),
-                               dim3(kRT), lds, s, a);
+            hipLaunchKernelGGL((render_frame_kernel<Tiled4, kMode, kLanes>), dim3(wgs), dim3(kRT), lds, s, a);
         else
-            hipLaunchKernelGGL((render_frame_kernel<float, kMode, kLanes>), dim3(P->render_grid),
-                               dim3(kRT), lds, s, a);
+            hipLaunchKernelGGL((render_frame_kernel<float, kMode, kLanes>), dim3(wgs), dim3(kRT), lds, s, a);
     } else if (P->fp16) {
         hipLaunchKernelGGL((frame_kernel<__half, kMode, kLanes>), dim3(grid), dim3(kT), 0, s, a);
     } else if (P->tiled) {
@@ -872,6 +902,7 @@ int launch_frame(rv_frame_plan* P, uint32_t cam, uint32_t* d_argb, float* d_rgba
     a.nbx = B->nbx;
     a.nbins = (uint32_t)nbins;
     a.render_waves = P->frame_rw;
+    a.grouped = P->render_grouped ? 1u : 0u;
     a.acc_cutoff = P->desc.acc_cutoff;
     a.argb = d_argb;
     a.rgba = reinterpret_cast<float4*>(d_rgba);
@@ -1293,6 +1324,24 @@ extern "C" int rv_frame_set_hybrid_min(rv_frame_plan* P, uint32_t blocks)
     return RV_OK;
 }
 
+extern "C" int rv_frame_set_grouped(rv_frame_plan* P, int grouped)
+{
+    RV_REQUIRE(P, "rv_frame_set_grouped: null plan");
+    if ((grouped != 0) != P->render_grouped) {
+        // a captured graph holds the previous launch shape
+        if (P->graph_exec) {
+            (void)hipGraphExecDestroy(P->graph_exec);
+            P->graph_exec = nullptr;
+        }
+        if (P->graph) {
+            (void)hipGraphDestroy(P->graph);
+            P->graph = nullptr;
+        }
+        P->render_grouped = grouped != 0;
+    }
+    return RV_OK;
+}
+
 // A captured graph holds the launch structure, grid and bins it was captured
 // with; every change of them drops it (rv_frame_graph_launch then fails until
 // rv_frame_graph_build runs again).
@@ -1356,7 +1405,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     hipDeviceProp_t prop;
     RV_HIP(hipGetDeviceProperties(&prop, dev));
     std::ostringstream sig;
-    sig << prop.gcnArchName << "|bins5|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
+    sig << prop.gcnArchName << "|bins6|" << P->nblocks << "|" << P->desc.vp_w << "x" << P->desc.vp_h
         << "|" << (P->fp16 ? "f16" : "f32") << "|interp" << interpolate << "|d" << P->blocks[0].depth
         << "|mode" << P->field_mode << "|layout" << P->desc.layout << "|small" << (P->interp_small ? 1 : 0);
     const std::string key = sig.str();
@@ -1389,6 +1438,11 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
                 if (interpolate && spl != std::string::npos &&
                     std::sscanf(line.c_str() + spl + 9, "%u", &split) == 1)
                     (void)rv_frame_set_split(P, (int)split); // drops a graph of the other structure
+                const size_t gr = line.find("\"grouped\": ");
+                unsigned grouped = 0;
+                if (!interpolate && gr != std::string::npos &&
+                    std::sscanf(line.c_str() + gr + 12, "%u", &grouped) == 1)
+                    P->render_grouped = grouped != 0;
                 const size_t rp = line.find("\"render_waves\": ");
                 if (interpolate && rp != std::string::npos &&
                     std::sscanf(line.c_str() + rp + 16, "%u", &rw) == 1 && rw >= 1 && rw <= kT / 64u)
@@ -1429,9 +1483,11 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     if (interpolate)
         (void)rv_frame_set_split(P, 0); // the candidates below are one-launch frames
     // median of 3 timed frames of camera cam in a configuration
-    auto time_cfg = [&](uint32_t sch, uint32_t tw, uint32_t th, uint32_t g, uint32_t rw, uint32_t cam,
-                        float* ms) -> int {
+    auto time_cfg = [&](uint32_t sch, uint32_t tw, uint32_t th, uint32_t g, uint32_t rw, uint32_t grp,
+                        uint32_t cam, float* ms) -> int {
         grid = P->cus * g;
+        if (!interpolate)
+            P->render_grouped = grp != 0;
         if (interpolate)
             P->frame_rw = rw;
         (void)rv_frame_set_schedule(P, kSched[sch][0]);
@@ -1451,14 +1507,19 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     };
     struct Choice {
         float ms = 1e30f;
-        uint32_t w = 8, h = 8, g = 1, rw = kT / 64u;
-    } pick[3];
+        uint32_t w = 8, h = 8, g = 1, rw = kT / 64u, grp = 0;
+    } pick[6]; // per schedule and bin handout (2 sch + grp)
+    // resident frames: the queue's persistent grid and the grouped bins
+    std::vector<uint32_t> grps = {0u};
+    if (!interpolate)
+        grps = {0u, 1u};
     for (uint32_t sch : scheds)
     for (const auto& c : cand)
     for (uint32_t g : wgs)
-    for (uint32_t rw : rws) {
+    for (uint32_t rw : rws)
+    for (uint32_t grp : grps) {
         float ms = 0.f;
-        const int rc = time_cfg(sch, c[0], c[1], g, rw, 0, &ms);
+        const int rc = time_cfg(sch, c[0], c[1], g, rw, grp, 0, &ms);
         if (rc) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -1468,45 +1529,53 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
         times << (times.tellp() > 0 ? ", " : "") << "\"" << kName[sch] << c[0] << "x" << c[1] << "/" << g;
         if (interpolate)
             times << "/" << rw;
+        if (grp)
+            times << "/grouped";
         times << "\": " << ms;
-        if (ms < pick[sch].ms)
-            pick[sch] = Choice{ms, c[0], c[1], g, rw};
+        if (ms < pick[2 * sch + grp].ms)
+            pick[2 * sch + grp] = Choice{ms, c[0], c[1], g, rw, grp};
     }
-    // The schedules' best configurations on camera 0 are compared again over
-    // up to 4 cameras spread through the plan's set: how much a per-lane walk
-    // gains depends on the view.
-    uint32_t bsched = 0;
+    // The best configuration of each schedule and bin handout on camera 0
+    // is compared again over up to 4 cameras spread through the plan's set:
+    // how much a per-lane walk or the grouped bins gain depends on the view.
+    std::vector<uint32_t> slots;
     for (uint32_t sch : scheds)
-        if (pick[sch].ms < pick[bsched].ms)
-            bsched = sch;
+        for (uint32_t grp : grps)
+            slots.push_back(2 * sch + grp);
+    uint32_t bslot = slots[0];
+    for (uint32_t k : slots)
+        if (pick[k].ms < pick[bslot].ms)
+            bslot = k;
     std::vector<uint32_t> cams;
     for (uint32_t q = 0; q < 4; ++q) {
         const uint32_t cam = (uint32_t)((uint64_t)P->ncams * q / 4);
         if (std::find(cams.begin(), cams.end(), cam) == cams.end())
             cams.push_back(cam);
     }
-    if (scheds.size() > 1 && P->ncams > 1) {
-        float tot[3] = {0.f, 0.f, 0.f};
-        for (uint32_t sch : scheds)
+    if (slots.size() > 1 && P->ncams > 1) {
+        float tot[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (uint32_t k : slots)
             for (uint32_t cam : cams) {
                 float ms = 0.f;
-                const Choice& c = pick[sch];
-                const int rc = time_cfg(sch, c.w, c.h, c.g, c.rw, cam, &ms);
+                const Choice& c = pick[k];
+                const int rc = time_cfg(k / 2, c.w, c.h, c.g, c.rw, c.grp, cam, &ms);
                 if (rc) {
                     (void)hipEventDestroy(e0);
                     (void)hipEventDestroy(e1);
                     return rc;
                 }
-                tot[sch] += ms;
+                tot[k] += ms;
             }
-        bsched = scheds[0];
-        for (uint32_t sch : scheds) {
-            times << ", \"schedule " << sch << " over " << cams.size() << " cameras\": " << tot[sch];
-            if (tot[sch] < tot[bsched])
-                bsched = sch;
+        bslot = slots[0];
+        for (uint32_t k : slots) {
+            times << ", \"schedule " << k / 2 << (k % 2 ? " grouped" : "") << " over " << cams.size()
+                  << " cameras\": " << tot[k];
+            if (tot[k] < tot[bslot])
+                bslot = k;
         }
     }
-    const Choice& bc = pick[bsched];
+    const uint32_t bsched = bslot / 2;
+    const Choice& bc = pick[bslot];
     const uint32_t bw = bc.w, bh = bc.h;
     if (interpolate) {
         // the chosen one-launch frame against the same frames split in two
@@ -1516,7 +1585,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
             (void)rv_frame_set_split(P, split);
             for (uint32_t cam : cams) {
                 float ms = 0.f;
-                const int rc = time_cfg(bsched, bw, bh, bc.g, bc.rw, cam, &ms);
+                const int rc = time_cfg(bsched, bw, bh, bc.g, bc.rw, bc.grp, cam, &ms);
                 if (rc) {
                     (void)rv_frame_set_split(P, 0);
                     (void)hipEventDestroy(e0);
@@ -1532,6 +1601,8 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
     grid = P->cus * bc.g;
     if (interpolate)
         P->frame_rw = bc.rw;
+    else
+        P->render_grouped = bc.grp != 0;
     (void)rv_frame_set_schedule(P, kSched[bsched][0]);
     (void)rv_frame_set_hybrid_min(P, kSched[bsched][1]);
     const uint32_t bg = bc.g;
@@ -1551,6 +1622,7 @@ extern "C" int rv_frame_autotune(rv_frame_plan* P, uint32_t* d_argb, int interpo
         f << "{\"key\": \"" << key << "\", \"tile\": [" << bw << ", " << bh
           << "], \"wgs_per_cu\": " << bg << ", \"render_waves\": " << P->frame_rw
           << ", \"schedule\": " << P->schedule << ", \"hybrid_min\": " << P->hybrid_min
+          << ", \"grouped\": " << (P->render_grouped ? 1 : 0)
           << ", \"split\": " << (P->split ? 1 : 0) << ", \"ms\": {"
           << times.str() << "}}\n";
     }

@@ -73,6 +73,7 @@ struct rv_frame_plan {
     uint32_t adj_grid = 0;
     uint32_t rec_grid = 0;    // colour-table-only adjoint (one lane per record)
     uint32_t render_grid = 0; // resident-brick frames
+    bool render_grouped = false; // resident-brick frames: 2 x 2 bins per workgroup (else the queue)
     uint32_t cus = 0, grid_max_wgs = 0, render_max_wgs = 0; // CUs, resident workgroups per CU
     uint32_t frame_rw = rv::kT / 64u; // waves per frame_kernel workgroup that render
     uint32_t field_mode = RV_FIELD_ARGB;

@@ -250,6 +250,39 @@ def test_frame_graph_replays_match_eager(orc, schedule):
     assert fp.status() == 0
 
 
+@pytest.mark.parametrize("exact", [True, False])
+def test_frame_grouped_bins_equal_queue(orc, exact):
+    """Resident frames from the grouped workgroups (2 x 2 neighbouring bins
+    each) equal the queue's, for every schedule and two footprints, in both
+    sample sequences (exact: also the oracle's float composite)."""
+    import torch
+    blocks = _mixed_blocks(31)
+    W, H = 200, 136
+    fp, tf, fmin, fmax, dev = _setup(blocks, W, H, acc_cut=0.9999, seed=31, layout=api.BRICK_TILED4)
+    fp.set_exact(exact)
+    M = datasets.orbit_camera(65.0, elevation_deg=20.0)
+    fp.set_cameras(M[None])
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    fp.render(0, argb, rgba, interpolate=True)  # the bricks
+    torch.cuda.synchronize()
+    ref = rgba.clone()
+    if exact:
+        bricks = [orc.interpolate(b.sparse, b.n2i, b.depth) for b in blocks]
+        assert np.array_equal(ref.cpu().numpy(), _oracle_frame(orc, blocks, bricks, M, W, H, fmin, fmax, tf, 0.9999))
+    for tile in ((8, 8), (16, 4)):
+        for sch in SCHEDULES:
+            _set_schedule(fp, sch)
+            for grouped in (False, True):
+                fp.set_grouped(grouped)
+                fp.render(0, argb, rgba, interpolate=False)
+                torch.cuda.synchronize()
+                assert fp.status() == 0
+                say("grouped", grouped, "schedule", sch, "exact", exact)
+                assert torch.equal(rgba, ref)
+    fp.close()
+
+
 def test_frame_autotune_picks_valid_tile(tmp_path):
     import torch
     blocks = _uniform_blocks(1, 4, seed=2)
