@@ -122,19 +122,34 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
         const rv_block_params p = a.params[blk];
         const bool mine = !opaque && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
                           vy < p.view_y + p.view_h;
-        // the fast sequence with the whole wave (lanes off the block get no
-        // samples), so that it shares overlong segments (march_fast)
-        bool wave_fast = false;
+        // Argb: the march with the whole wave (lanes off the block get no
+        // samples), so that it shares overlong segments (march_fast,
+        // march_impl)
+        bool wave_argb = false;
         if constexpr (kMode == kModeArgb)
-            wave_fast = a.fast && !a.tape;
-        if (wave_fast) {
+            wave_argb = true;
+        if (wave_argb) {
             RaySeg r = setup_ray(p, vx, vy);
             if (!mine) {
                 r.hit = false;
                 r.sample_count = 0;
             }
-            const float4 seg =
-                march_fast<BrickT, true>(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), tfs, (int)a.tf_n, r, yz);
+            const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);
+            float4 seg;
+            if (a.tape) {
+                // the pixel's state in front of this block, for the adjoint
+                if (r.hit)
+                    a.tape[a.rec_off[blk] + (vy - p.view_y) * p.view_w + (vx - p.view_x)] =
+                        make_float4(acc.x * acc.w, acc.y * acc.w, acc.z * acc.w, trans);
+                float ts;
+                seg = march_impl<BrickT, true, true, false>(p, brick, tfs, (int)a.tf_n, r, &ts);
+                if (r.hit)
+                    trans = trans * ts;
+            } else if (a.fast) {
+                seg = march_fast<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, yz);
+            } else {
+                seg = march_impl<BrickT, false, true, false>(p, brick, tfs, (int)a.tf_n, r, nullptr);
+            }
             if (r.hit) {
                 blend_behind(acc, seg);
                 opaque = acc.w > a.acc_cutoff;
@@ -144,19 +159,7 @@ __device__ void render_bin(const FrameArgs& a, uint32_t bin, const float4* tfs, 
             if (r.hit) {
                 const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);
                 float4 seg;
-                if (kMode == kModeArgb && a.tape) {
-                    // the pixel's state in front of this block, for the adjoint
-                    a.tape[a.rec_off[blk] + (vy - p.view_y) * p.view_w + (vx - p.view_x)] =
-                        make_float4(acc.x * acc.w, acc.y * acc.w, acc.z * acc.w, trans);
-                    float ts;
-                    seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
-                    trans = trans * ts;
-                } else if (kMode == kModeArgb && a.fast) {
-                    // (not taken: wave_fast covers it; with this one-lane
-                    // form present the fp16 and fp32 kernels compile to 13
-                    // fewer VGPRs, 4 waves per SIMD instead of 3)
-                    seg = march_fast(p, brick, tfs, (int)a.tf_n, r, yz);
-                } else if (multi_level(kMode) && a.lev.n == 0) {
+                if (multi_level(kMode) && a.lev.n == 0) {
                     seg = make_float4(0.f, 0.f, 0.f, 0.f); // no levels: nothing to draw
                 } else if (kMode == kModeValueOnIsosurface || kMode == kModeValueOnIsosurfaces) {
                     const Secondary<BrickT> sec{reinterpret_cast<const BrickT*>(a.bricks2[blk]), a.smin,
@@ -219,38 +222,32 @@ __device__ void render_bin_lanes(const FrameArgs& a, uint32_t bin, const float4*
             live = false;
         if (!__any(found))
             break;
-        if (a.fast && !a.tape) {
-            // the fast sequence with the whole wave, each lane on its block
-            // (lanes without one: block 0, no samples), so that it shares
-            // overlong segments (march_fast)
+        {
+            // the march with the whole wave, each lane on its block (lanes
+            // without one: block 0, no samples), so that it shares overlong
+            // segments (march_fast, march_impl)
             const rv_block_params p = a.params[blk];
             RaySeg r = setup_ray(p, vx, vy);
             if (!found) {
                 r.hit = false;
                 r.sample_count = 0;
             }
-            const float4 seg = march_fast<BrickT, true, true>(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), tfs,
-                                                              (int)a.tf_n, r, yz);
-            if (r.hit) {
-                blend_behind(acc, seg);
-                if (acc.w > a.acc_cutoff)
-                    live = false;
-            }
-        } else if (found) {
-            const rv_block_params p = a.params[blk];
-            const RaySeg r = setup_ray(p, vx, vy);
-            if (r.hit) {
-                const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);
-                float4 seg;
-                if (a.tape) {
+            const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);
+            float4 seg;
+            if (a.tape) {
+                if (r.hit)
                     a.tape[a.rec_off[blk] + (vy - p.view_y) * p.view_w + (vx - p.view_x)] =
                         make_float4(acc.x * acc.w, acc.y * acc.w, acc.z * acc.w, trans);
-                    float ts;
-                    seg = march_impl<BrickT, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
+                float ts;
+                seg = march_impl<BrickT, true, true, true>(p, brick, tfs, (int)a.tf_n, r, &ts);
+                if (r.hit)
                     trans = trans * ts;
-                } else {
-                    seg = march_impl<BrickT, false>(p, brick, tfs, (int)a.tf_n, r, nullptr);
-                }
+            } else if (a.fast) {
+                seg = march_fast<BrickT, true, true>(p, brick, tfs, (int)a.tf_n, r, yz);
+            } else {
+                seg = march_impl<BrickT, false, true, true>(p, brick, tfs, (int)a.tf_n, r, nullptr);
+            }
+            if (r.hit) {
                 blend_behind(acc, seg);
                 if (acc.w > a.acc_cutoff)
                     live = false;
@@ -487,6 +484,8 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
             sec.brick = reinterpret_cast<const BrickT*>(bricks2[blk]);
         if constexpr (kMode == kModeArgbFast)
             return march_fast<BrickT, true>(p, brick, tfs_share, (int)tf_n, r, yz_s);
+        else if constexpr (kMode == kModeArgb)
+            return march_impl<BrickT, false, true, false>(p, brick, tfs_share, (int)tf_n, r, nullptr);
         else
             return segment<kMode>(p, brick, tfs_share, (int)tf_n, iso_opacity, r, lvs, lev.n, sec);
     };
@@ -503,7 +502,7 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
         const rv_block_params p = params[blk];
         const bool inrect = inside && vx >= p.view_x && vx < p.view_x + p.view_w && vy >= p.view_y &&
                             vy < p.view_y + p.view_h;
-        if constexpr (kMode == kModeArgbFast) {
+        if constexpr (kMode == kModeArgbFast || kMode == kModeArgb) {
             const RaySeg r = wave_ray(p, inrect);
             const float4 c = seg_of(blk, p, r);
             if (r.hit)
@@ -531,7 +530,7 @@ __global__ __launch_bounds__(kShareT) void share_kernel(const rv_block_params* _
                             vy < p.view_y + p.view_h;
         bool hit = false;
         float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
-        if constexpr (kMode == kModeArgbFast) {
+        if constexpr (kMode == kModeArgbFast || kMode == kModeArgb) {
             const RaySeg r = wave_ray(p, inrect);
             c = seg_of(blk, p, r);
             hit = r.hit;

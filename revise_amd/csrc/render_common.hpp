@@ -395,17 +395,63 @@ __device__ __forceinline__ uint32_t blend_above_u32(uint32_t argb, const float4&
 #endif
 constexpr int kBatch = RV_MARCH_BATCH;
 
-// kTrans: also return in *trans the product of (1 - alpha) over the
-// composited samples (the segment's transmittance, as a product rather than
-// 1 - alpha, for the adjoint's record of the pixel state at each block).
-template <class T, bool kTrans>
+// A block's parameters as lane l holds them, for the whole wave (every
+// 32-bit word read out of lane l).
+__device__ __forceinline__ rv_block_params lane_params(const rv_block_params& p, int l)
+{
+    constexpr int kWords = (int)(sizeof(rv_block_params) / 4);
+    uint32_t w[kWords];
+    __builtin_memcpy(w, &p, sizeof(w));
+#pragma unroll
+    for (int k = 0; k < kWords; ++k)
+        w[k] = (uint32_t)__builtin_amdgcn_readlane((int)w[k], l);
+    rv_block_params u;
+    __builtin_memcpy(&u, w, sizeof(w));
+    return u;
+}
+
+template <class T> __device__ __forceinline__ const T* lane_ptr(const T* q, int l)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(q);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return reinterpret_cast<const T*>(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ float lane_f(float x, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// The exact sequence's march.  kTrans: also return in *trans the product
+// of (1 - alpha) over the composited samples (the segment's transmittance,
+// as a product rather than 1 - alpha, for the adjoint's record of the pixel
+// state at each block).  kCoop and kLaneBlocks as for the fast
+// sequence (march_fast_impl): called by every lane of the wave, each lane
+// marches at most the cap, and a lane with samples left is continued by the
+// whole wave, 64 samples at a time -- positions by the same chain of
+// additions, fetch, trilinear value and classification one sample per lane,
+// blendBehind (and the transmittance product) in sample order from values
+// read out of the lanes -- bit-identical to the one-lane march.
+template <class T, bool kTrans, bool kCoop = false, bool kLaneBlocks = false>
 __device__ __forceinline__ float4 march_impl(const rv_block_params& p, const T* __restrict__ brick,
                                              const float4* tf, int tf_n, RaySeg r, float* trans)
 {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
     float tr = 1.f;
     const int n = (int)p.nodes;
-    for (int i = 1; i < r.sample_count; i += kBatch) {
+    int cnt = r.sample_count;
+    int cap = 0;
+    bool coop = false;
+    if constexpr (kCoop) {
+        cap = (int)(2.f * p.quality * (float)n);
+        cap = max(kBatch, (cap + kBatch - 1) / kBatch * kBatch); // whole batches
+        coop = __ballot(true) == ~0ull;                          // the whole wave is here
+        if (coop)
+            cnt = min(cnt, cap + 1);
+    }
+    bool term = false;
+    for (int i = 1; i < cnt; i += kBatch) {
         // the corner loads of the whole batch are issued before any is used;
         // samples past the segment's end read clamped in-brick corners and
         // are dropped
@@ -436,8 +482,73 @@ __device__ __forceinline__ float4 march_impl(const rv_block_params& p, const T* 
                 done = acc.w > p.alpha_cutoff;
             }
         }
-        if (done)
+        if (done) {
+            term = true;
             break;
+        }
+    }
+    if constexpr (kCoop) {
+        uint64_t left = coop ? __ballot(!term && r.sample_count > cap + 1) : 0ull;
+        const int me = (int)__builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+        while (left) {
+            const int L = (int)__builtin_ctzll(left);
+            left &= left - 1ull;
+            rv_block_params pu = p;
+            const T* ub = brick;
+            int ucap = cap;
+            if constexpr (kLaneBlocks) {
+                pu = lane_params(p, L);
+                ub = lane_ptr(brick, L);
+                ucap = __builtin_amdgcn_readlane(cap, L);
+            }
+            float Px = lane_f(r.px, L), Py = lane_f(r.py, L), Pz = lane_f(r.pz, L);
+            const float Sx = lane_f(r.sx, L), Sy = lane_f(r.sy, L), Sz = lane_f(r.sz, L);
+            const int N = __builtin_amdgcn_readlane(r.sample_count, L);
+            float4 ua = make_float4(lane_f(acc.x, L), lane_f(acc.y, L), lane_f(acc.z, L), lane_f(acc.w, L));
+            float utr = lane_f(tr, L);
+            bool udone = false;
+            for (int base = ucap + 1; base < N && !udone; base += 64) {
+                float qx = 0.f, qy = 0.f, qz = 0.f;
+#pragma unroll 1
+                for (int j = 0; j < 64; ++j) {
+                    if (j == me) {
+                        qx = Px;
+                        qy = Py;
+                        qz = Pz;
+                    }
+                    Px = Px + Sx;
+                    Py = Py + Sy;
+                    Pz = Pz + Sz;
+                }
+                float cc[8];
+                const float tx = (qx - pu.tex_origin[0]) * pu.tcf;
+                const float ty = (qy - pu.tex_origin[1]) * pu.tcf;
+                const float tz = (qz - pu.tex_origin[2]) * pu.tcf;
+                const Taps tt = fetch_corners(ub, (int)pu.nodes, tx, ty, tz, cc);
+                const float v = trilinear(cc, tt.fx, tt.fy, tt.fz) + (base + me < N ? 0.f : __builtin_nanf(""));
+                const bool valid = v == v;
+                float4 cs = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (valid)
+                    cs = classify(pu, tf, tf_n, v);
+                uint64_t vm = __ballot(valid);
+                while (vm) {
+                    const int j = (int)__builtin_ctzll(vm);
+                    vm &= vm - 1ull;
+                    const float4 cj = make_float4(lane_f(cs.x, j), lane_f(cs.y, j), lane_f(cs.z, j), lane_f(cs.w, j));
+                    blend_behind(ua, cj);
+                    if (kTrans)
+                        utr = utr * (1.f - cj.w);
+                    if (ua.w > pu.alpha_cutoff) {
+                        udone = true;
+                        break;
+                    }
+                }
+            }
+            if (me == L) {
+                acc = ua;
+                tr = utr;
+            }
+        }
     }
     if (kTrans)
         *trans = tr;
@@ -630,10 +741,6 @@ __device__ __forceinline__ FastBlock fast_block(const rv_block_params& p, int tf
     return b;
 }
 
-__device__ __forceinline__ float lane_f(float x, int l)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
-}
 
 // A block's constants as lane l holds them, for the whole wave.
 __device__ __forceinline__ FastBlock lane_block(const FastBlock& b, int l)
@@ -654,13 +761,6 @@ __device__ __forceinline__ FastBlock lane_block(const FastBlock& b, int l)
     return u;
 }
 
-template <class T> __device__ __forceinline__ const T* lane_ptr(const T* q, int l)
-{
-    const uint64_t v = reinterpret_cast<uint64_t>(q);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
-    return reinterpret_cast<const T*>(((uint64_t)hi << 32) | lo);
-}
 
 // One sample's corners and weights at a world position.
 template <class T, bool kTab>

@@ -134,12 +134,13 @@ def _sample_counts(p):
     return np.where(tx > te, sc, 0)
 
 
-def test_config2_dropin_fast_overlong_segments(config2):
+@pytest.mark.parametrize("exact", [False, True])
+def test_config2_dropin_overlong_segments(config2, exact):
     """Cameras where rays grazing a block's edge miss its entry face in fp32
     and start at the eye (tauEnter = 1, DenseFieldRenderer_cu.cu:164-165):
     segments of thousands of samples, which the share kernel continues with
     the whole wave; its frame equals the per-block launches' (one lane per
-    ray) bit for bit."""
+    ray) bit for bit, in both sample sequences."""
     angle = float(np.random.default_rng(SEED).uniform(0.0, 360.0))
     longest = 0
     for da in (15.0, 30.0, 45.0):
@@ -150,10 +151,10 @@ def test_config2_dropin_fast_overlong_segments(config2):
                 longest = max(longest, int(_sample_counts(p).max()))
         parts = {}
         for fused in (True, False):
-            w = _worker(c, fused, False)
+            w = _worker(c, fused, exact)
             parts[fused] = w.render_scene_part(3, (0, 0, 0), (8, 8, 8))
             w.close()
-        say("config 2 fast, camera %+.0f deg: fused == per block" % da)
+        say("config 2 %s, camera %+.0f deg: fused == per block" % ("exact" if exact else "fast", da))
         assert parts[True][0] == parts[False][0]
         assert np.array_equal(parts[True][1], parts[False][1])
     say("longest segment over the three cameras: %d samples" % longest)

@@ -18,7 +18,7 @@ import pytest
 
 from revise_amd import api, configs, datasets
 from tests.adjoint_ref import SampleTable
-from tests.gpu_util import say
+from tests.gpu_util import host_threads, say
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +71,38 @@ def test_large_share_reduced_matches_oracle(orc):
             torch.cuda.synchronize()
             assert fp.status() == 0
             assert np.array_equal(rgba.cpu().numpy(), exp)
+    fp.close()
+
+
+def test_large_share_exact_overlong_segments_match_oracle(orc):
+    """The exact sequence over the same small blocks at the camera whose
+    frame holds a segment of 4,000 samples: both walks, which continue such
+    segments with the whole wave (march_impl), equal the oracle's float
+    composite (one ray at a time) bit for bit."""
+    import torch
+    ls = _share(6, seed=61)
+    W, H = 1920, 1080
+    tf_np = datasets.seeded_tf(61, 256)
+    tf = torch.from_numpy(tf_np).cuda()
+    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf, (8, 8), exact=True)
+    cam = datasets.orbit_camera(35.0, elevation_deg=15.0)
+    fp.set_cameras(cam[None])
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    rgba = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    nb = ls.keys.shape[0]
+    bricks = ls.bricks[:, :17 ** 3].float().cpu().numpy()
+    keys = np.concatenate([np.full((nb, 1), 6, np.uint32), ls.keys.astype(np.uint32)], 1)
+    order = api.mixed_back_to_front(keys, api.eye_from_view(cam))
+    lst = [(6, tuple(int(v) for v in ls.keys[i]), 4, bricks[i]) for i in order]
+    exp = orc.render_float_composite(cam, W, H, FOV, lst, ls.fmin, ls.fmax, THR, Q, CUT, tf_np, ACC,
+                                     nthreads=host_threads())
+    for sch in (api.SCHEDULE_BINS, api.SCHEDULE_LANES):
+        fp.set_schedule(sch)
+        fp.render(0, argb, rgba, interpolate=False)
+        torch.cuda.synchronize()
+        assert fp.status() == 0
+        say("level-6 exact frame, schedule", sch)
+        assert np.array_equal(rgba.cpu().numpy(), exp)
     fp.close()
 
 
