@@ -224,6 +224,128 @@ struct SegRay {
     CellAcc cell;
 };
 
+// One sample's values before the ray's state enters (the operations of
+// fetch3 and classify); v NaN: the sample is skipped.
+struct AdjSample {
+    float v, xraw, f, cx, cy, cz, base, alpha;
+    int i0, i1;
+    float4 A0, A1; // the two table entries (field gradient)
+    Taps t;        // the corners (field gradient)
+};
+
+template <class T>
+__device__ __forceinline__ void adj_sample(const rv_block_params& p, const T* __restrict__ brick, const float4* tfs,
+                                           int tf_n, float tx, float ty, float tz, AdjSample& s)
+{
+    const int n = (int)p.nodes;
+    const float span = p.fmax - p.fmin;
+    float v8[8];
+    s.t = fetch_corners(brick, n, tx, ty, tz, v8);
+    s.v = trilinear(v8, s.t.fx, s.t.fy, s.t.fz); // the operations of fetch3
+    if (!(s.v == s.v))
+        return;
+    // classify, the operations of classify / fetch_tf
+    const float v01 = (s.v - p.fmin) / span;
+    s.xraw = v01 * (float)tf_n - 0.5f;
+    float x = s.xraw;
+    if (!(x > -1.f))
+        x = -1.f;
+    if (x > (float)tf_n)
+        x = (float)tf_n;
+    const float fl = floorf(x);
+    s.f = x - fl;
+    const int ti = (int)fl;
+    s.i0 = clampi(ti, 0, tf_n - 1);
+    s.i1 = clampi(ti + 1, 0, tf_n - 1);
+    s.A0 = tfs[s.i0];
+    s.A1 = tfs[s.i1];
+    s.cx = lerpf(s.A0.x, s.A1.x, s.f);
+    s.cy = lerpf(s.A0.y, s.A1.y, s.f);
+    s.cz = lerpf(s.A0.z, s.A1.z, s.f);
+    const float tw = lerpf(s.A0.w, s.A1.w, s.f);
+    s.base = 1.f - tw * p.opacity_factor;
+    s.alpha = 1.f - pow01(s.base, p.i_layer_count);
+}
+
+// The ray's state through one kept sample: its colour-table terms (and,
+// with gbrick, its field terms), then the segment's transmittance and
+// termination.
+__device__ __forceinline__ void adj_update(const rv_block_params& p, float* __restrict__ gbrick, float4* gtf,
+                                           int tf_n, AdjPixel& st, SegRay& sg, const AdjSample& s,
+                                           const Window& w, float* win)
+{
+    const int n = (int)p.nodes;
+    const float span = p.fmax - p.fmin;
+    const float cx = s.cx, cy = s.cy, cz = s.cz, alpha = s.alpha, base = s.base, f = s.f;
+    const float Ts = st.T * sg.Tseg;
+    const float wgt = Ts * alpha;
+    st.Px = st.Px + wgt * cx;
+    st.Py = st.Py + wgt * cy;
+    st.Pz = st.Pz + wgt * cz;
+    const float Tn = Ts * (1.f - alpha);
+    const bool behind = Tn > 1e-30f;
+    // gradient-only quantities use the hardware reciprocal (1 ulp); the
+    // classification keeps the forward's exact operations
+    const float inv = behind ? __builtin_amdgcn_rcpf(Tn) : 0.f;
+    const float Qx = (st.Fx - st.Px) * inv, Qy = (st.Fy - st.Py) * inv, Qz = (st.Fz - st.Pz) * inv;
+    const float ratio = behind ? st.Tend * inv : 1.f;
+    const float g_alpha = Ts * (((st.gPx * cx + st.gPy * cy) + st.gPz * cz) -
+                                ((st.gPx * Qx + st.gPy * Qy) + st.gPz * Qz) + st.gA * ratio);
+    const float gcx = st.gPx * wgt, gcy = st.gPy * wgt, gcz = st.gPz * wgt;
+    const float gtw = base > 0.f
+                          ? g_alpha * (p.i_layer_count * p.opacity_factor * (1.f - alpha) *
+                                       __builtin_amdgcn_rcpf(base))
+                          : 0.f;
+    // colour table
+    TfAcc& ta = sg.tacc;
+    const float w0 = 1.f - f;
+    if (s.i0 != ta.i0 || s.i1 != ta.i1) {
+        tf_flush(ta, gtf);
+        ta.i0 = s.i0;
+        ta.i1 = s.i1;
+        ta.g0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        ta.g1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    ta.g0.x += w0 * gcx;
+    ta.g0.y += w0 * gcy;
+    ta.g0.z += w0 * gcz;
+    ta.g0.w += w0 * gtw;
+    ta.g1.x += f * gcx;
+    ta.g1.y += f * gcy;
+    ta.g1.z += f * gcz;
+    ta.g1.w += f * gtw;
+    // field
+    if (gbrick && s.xraw > -1.f && s.xraw <= (float)tf_n) {
+        const float4 A0 = s.A0, A1 = s.A1;
+        const Taps& t = s.t;
+        const float gv = ((float)tf_n / span) *
+                         ((((A1.x - A0.x) * gcx + (A1.y - A0.y) * gcy) + (A1.z - A0.z) * gcz) +
+                          (A1.w - A0.w) * gtw);
+        if (gv != 0.f) {
+            CellAcc& cell = sg.cell;
+            const uint32_t key = t.a * 8u + (t.dx ? 1u : 0u) + (t.dy ? 2u : 0u) + (t.dz ? 4u : 0u);
+            if (key != cell.key)
+                cell_move(cell, key, t, (uint32_t)n, w, win, gbrick);
+            const float ux = 1.f - t.fx, uy = 1.f - t.fy, uz = 1.f - t.fz;
+            const float gz0 = gv * uz, gz1 = gv * t.fz;
+            const float g00 = gz0 * uy, g10 = gz0 * t.fy, g01 = gz1 * uy, g11 = gz1 * t.fy;
+            cell.g[0] += g00 * ux;
+            cell.g[1] += g00 * t.fx;
+            cell.g[2] += g10 * ux;
+            cell.g[3] += g10 * t.fx;
+            cell.g[4] += g01 * ux;
+            cell.g[5] += g01 * t.fx;
+            cell.g[6] += g11 * ux;
+            cell.g[7] += g11 * t.fx;
+        }
+    }
+    sg.Tseg = sg.Tseg * (1.f - alpha);
+    const float da = alpha * (1.f - sg.seg_w);
+    sg.seg_w = sg.seg_w + da;
+    if (sg.seg_w > p.alpha_cutoff)
+        sg.ended = true;
+}
+
 // The samples of one ray segment from sample sg.i on, up to the slab end:
 // along axis A (0 x, 1 y, 2 z) a sample whose cell index passes `lim`
 // (> lim when up, < lim otherwise) is left for the next slab; A < 0: no
@@ -235,7 +357,6 @@ __device__ __forceinline__ void march_adjoint(const rv_block_params& p, const T*
                                               int A, bool up, int lim, const Window& w, float* win)
 {
     const int n = (int)p.nodes;
-    const float span = p.fmax - p.fmin;
     while (!sg.ended && sg.i < sg.r.sample_count) {
         const float tx = (sg.r.px - p.tex_origin[0]) * p.tcf;
         const float ty = (sg.r.py - p.tex_origin[1]) * p.tcf;
@@ -251,96 +372,11 @@ __device__ __forceinline__ void march_adjoint(const rv_block_params& p, const T*
         sg.r.py = sg.r.py + sg.r.sy;
         sg.r.pz = sg.r.pz + sg.r.sz;
         ++sg.i;
-        float v8[8];
-        const Taps t = fetch_corners(brick, n, tx, ty, tz, v8);
-        const float v = trilinear(v8, t.fx, t.fy, t.fz); // the operations of fetch3
-        if (!(v == v))
+        AdjSample smp;
+        adj_sample(p, brick, tfs, tf_n, tx, ty, tz, smp);
+        if (!(smp.v == smp.v))
             continue;
-        // classify, the operations of classify / fetch_tf
-        const float v01 = (v - p.fmin) / span;
-        const float xraw = v01 * (float)tf_n - 0.5f;
-        float x = xraw;
-        if (!(x > -1.f))
-            x = -1.f;
-        if (x > (float)tf_n)
-            x = (float)tf_n;
-        const float fl = floorf(x);
-        const float f = x - fl;
-        const int ti = (int)fl;
-        const int i0 = clampi(ti, 0, tf_n - 1), i1 = clampi(ti + 1, 0, tf_n - 1);
-        const float4 A0 = tfs[i0], A1 = tfs[i1];
-        const float cx = lerpf(A0.x, A1.x, f), cy = lerpf(A0.y, A1.y, f),
-                    cz = lerpf(A0.z, A1.z, f), tw = lerpf(A0.w, A1.w, f);
-        const float base = 1.f - tw * p.opacity_factor;
-        const float alpha = 1.f - pow01(base, p.i_layer_count);
-
-        const float Ts = st.T * sg.Tseg;
-        const float wgt = Ts * alpha;
-        st.Px = st.Px + wgt * cx;
-        st.Py = st.Py + wgt * cy;
-        st.Pz = st.Pz + wgt * cz;
-        const float Tn = Ts * (1.f - alpha);
-        const bool behind = Tn > 1e-30f;
-        // gradient-only quantities use the hardware reciprocal (1 ulp); the
-        // classification above keeps the forward's exact operations
-        const float inv = behind ? __builtin_amdgcn_rcpf(Tn) : 0.f;
-        const float Qx = (st.Fx - st.Px) * inv, Qy = (st.Fy - st.Py) * inv,
-                    Qz = (st.Fz - st.Pz) * inv;
-        const float ratio = behind ? st.Tend * inv : 1.f;
-        const float g_alpha = Ts * (((st.gPx * cx + st.gPy * cy) + st.gPz * cz) -
-                                    ((st.gPx * Qx + st.gPy * Qy) + st.gPz * Qz) + st.gA * ratio);
-        const float gcx = st.gPx * wgt, gcy = st.gPy * wgt, gcz = st.gPz * wgt;
-        const float gtw = base > 0.f
-                              ? g_alpha * (p.i_layer_count * p.opacity_factor * (1.f - alpha) *
-                                           __builtin_amdgcn_rcpf(base))
-                              : 0.f;
-        // colour table
-        TfAcc& ta = sg.tacc;
-        const float w0 = 1.f - f;
-        if (i0 != ta.i0 || i1 != ta.i1) {
-            tf_flush(ta, gtf);
-            ta.i0 = i0;
-            ta.i1 = i1;
-            ta.g0 = make_float4(0.f, 0.f, 0.f, 0.f);
-            ta.g1 = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        ta.g0.x += w0 * gcx;
-        ta.g0.y += w0 * gcy;
-        ta.g0.z += w0 * gcz;
-        ta.g0.w += w0 * gtw;
-        ta.g1.x += f * gcx;
-        ta.g1.y += f * gcy;
-        ta.g1.z += f * gcz;
-        ta.g1.w += f * gtw;
-        // field
-        if (gbrick && xraw > -1.f && xraw <= (float)tf_n) {
-            const float gv = ((float)tf_n / span) *
-                             ((((A1.x - A0.x) * gcx + (A1.y - A0.y) * gcy) + (A1.z - A0.z) * gcz) +
-                              (A1.w - A0.w) * gtw);
-            if (gv != 0.f) {
-                CellAcc& cell = sg.cell;
-                const uint32_t key =
-                    t.a * 8u + (t.dx ? 1u : 0u) + (t.dy ? 2u : 0u) + (t.dz ? 4u : 0u);
-                if (key != cell.key)
-                    cell_move(cell, key, t, (uint32_t)n, w, win, gbrick);
-                const float ux = 1.f - t.fx, uy = 1.f - t.fy, uz = 1.f - t.fz;
-                const float gz0 = gv * uz, gz1 = gv * t.fz;
-                const float g00 = gz0 * uy, g10 = gz0 * t.fy, g01 = gz1 * uy, g11 = gz1 * t.fy;
-                cell.g[0] += g00 * ux;
-                cell.g[1] += g00 * t.fx;
-                cell.g[2] += g10 * ux;
-                cell.g[3] += g10 * t.fx;
-                cell.g[4] += g01 * ux;
-                cell.g[5] += g01 * t.fx;
-                cell.g[6] += g11 * ux;
-                cell.g[7] += g11 * t.fx;
-            }
-        }
-        sg.Tseg = sg.Tseg * (1.f - alpha);
-        const float da = alpha * (1.f - sg.seg_w);
-        sg.seg_w = sg.seg_w + da;
-        if (sg.seg_w > p.alpha_cutoff)
-            sg.ended = true;
+        adj_update(p, gbrick, gtf, tf_n, st, sg, smp, w, win);
     }
 }
 
@@ -523,7 +559,15 @@ __device__ __forceinline__ bool pixel_start(const AdjArgs& a, const rv_block_par
 // record (block, pixel of its view rectangle), records in tape order, so a
 // wave's lanes are one block's consecutive pixels, or the pixels of a few
 // consecutive blocks, and small blocks fill the wave.  Each lane re-marches
-// its segment with its own block's parameters.
+// its segment with its own block's parameters, at most the cap (twice the
+// samples of the longest ray inside the block); a record with samples left
+// (a ray that grazes a block's edge and starts at the eye: thousands of
+// samples) is continued by the whole wave, 64 samples at a time, with its
+// block's parameters and brick: the positions by the same chain of fp32
+// additions (every lane runs it and keeps its own), each sample's values by
+// one lane (adj_sample), and the ray's state through the samples in order
+// by the record's own lane, which reads them out of the others
+// (adj_update) -- the operations of the one-lane march.
 template <class BrickT>
 __global__ __launch_bounds__(kRT) void adjoint_records_kernel(AdjArgs a)
 {
@@ -536,29 +580,89 @@ __global__ __launch_bounds__(kRT) void adjoint_records_kernel(AdjArgs a)
     }
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * kRT;
+    const uint32_t lane = threadIdx.x & 63u;
+    const int me = (int)lane;
     Window w;
     w.ok = false;
     w.o[0] = w.o[1] = w.o[2] = 0;
     w.e[0] = w.e[1] = w.e[2] = 0;
-    for (uint64_t rid = (uint64_t)blockIdx.x * kRT + threadIdx.x; rid < a.nrec; rid += stride) {
-        uint32_t blk = a.rec_chunk[rid >> 6];
-        while (blk + 1u < a.nblocks && a.rec_off[blk + 1u] <= rid)
+    // wave-uniform loop: every lane takes part in each round (a lane past
+    // the last record, or whose record carries nothing, marches nothing)
+    for (uint64_t r0 = (uint64_t)blockIdx.x * kRT + (threadIdx.x - lane); r0 < a.nrec; r0 += stride) {
+        const uint64_t rid = r0 + lane;
+        const uint64_t rq = rid < a.nrec ? rid : a.nrec - 1u; // (lookups of lanes past the end)
+        uint32_t blk = a.rec_chunk[rq >> 6];
+        while (blk + 1u < a.nblocks && a.rec_off[blk + 1u] <= rq)
             ++blk;
         const rv_block_params p = a.params[blk];
-        const uint32_t li = (uint32_t)(rid - a.rec_off[blk]);
+        const BrickT* brick = reinterpret_cast<const BrickT*>(a.bricks[blk]);
+        const uint32_t li = (uint32_t)(rq - a.rec_off[blk]);
         const uint32_t ly = li / p.view_w, lx = li - ly * p.view_w;
         AdjPixel st{};
         SegRay sg;
-        if (!pixel_start(a, p, rid, lx, ly, st, sg.r))
-            continue;
+        const bool ok = rid < a.nrec && pixel_start(a, p, rid, lx, ly, st, sg.r);
+        if (!ok)
+            sg.r.sample_count = 0;
         sg.i = 1;
         sg.seg_w = 0.f;
         sg.Tseg = 1.f;
         sg.ended = false;
         sg.tacc.i0 = -1;
         sg.cell.key = 0xFFFFFFFFu;
-        march_adjoint(p, reinterpret_cast<const BrickT*>(a.bricks[blk]), nullptr, tfs, gtf,
-                      (int)a.tf_n, st, sg, -1, true, 0, w, nullptr);
+        const int full = sg.r.sample_count;
+        const int cap = (int)(2.f * p.quality * (float)p.nodes);
+        sg.r.sample_count = min(full, cap + 1);
+        march_adjoint(p, brick, nullptr, tfs, gtf, (int)a.tf_n, st, sg, -1, true, 0, w, nullptr);
+        sg.r.sample_count = full;
+        uint64_t left = __ballot(ok && !sg.ended && sg.i < full);
+        while (left) {
+            const int L = (int)__builtin_ctzll(left);
+            left &= left - 1ull;
+            const rv_block_params pu = lane_params(p, L);
+            const BrickT* ub = lane_ptr(brick, L);
+            float Px = lane_f(sg.r.px, L), Py = lane_f(sg.r.py, L), Pz = lane_f(sg.r.pz, L);
+            const float Sx = lane_f(sg.r.sx, L), Sy = lane_f(sg.r.sy, L), Sz = lane_f(sg.r.sz, L);
+            const int N = __builtin_amdgcn_readlane(full, L);
+            int base = __builtin_amdgcn_readlane(sg.i, L);
+            bool uend = false;
+            for (; base < N && !uend; base += 64) {
+                float qx = 0.f, qy = 0.f, qz = 0.f;
+#pragma unroll 1
+                for (int j = 0; j < 64; ++j) {
+                    if (j == me) {
+                        qx = Px;
+                        qy = Py;
+                        qz = Pz;
+                    }
+                    Px = Px + Sx;
+                    Py = Py + Sy;
+                    Pz = Pz + Sz;
+                }
+                AdjSample smp;
+                adj_sample(pu, ub, tfs, (int)a.tf_n, (qx - pu.tex_origin[0]) * pu.tcf,
+                           (qy - pu.tex_origin[1]) * pu.tcf, (qz - pu.tex_origin[2]) * pu.tcf, smp);
+                const uint64_t vm = __ballot(base + me < N && smp.v == smp.v);
+                if (me == L) {
+                    for (int j = 0; j < 64 && base + j < N && !sg.ended; ++j) {
+                        if (!((vm >> j) & 1ull)) // NaN: skipped, as in the one-lane march
+                            continue;
+                        AdjSample sj;
+                        sj.v = lane_f(smp.v, j);
+                        sj.xraw = lane_f(smp.xraw, j);
+                        sj.f = lane_f(smp.f, j);
+                        sj.cx = lane_f(smp.cx, j);
+                        sj.cy = lane_f(smp.cy, j);
+                        sj.cz = lane_f(smp.cz, j);
+                        sj.base = lane_f(smp.base, j);
+                        sj.alpha = lane_f(smp.alpha, j);
+                        sj.i0 = __builtin_amdgcn_readlane(smp.i0, j);
+                        sj.i1 = __builtin_amdgcn_readlane(smp.i1, j);
+                        adj_update(p, nullptr, gtf, (int)a.tf_n, st, sg, sj, w, nullptr);
+                    }
+                }
+                uend = __builtin_amdgcn_readlane(sg.ended ? 1 : 0, L) != 0;
+            }
+        }
         tf_flush(sg.tacc, gtf);
     }
     __syncthreads();
