@@ -380,6 +380,122 @@ __device__ __forceinline__ void march_adjoint(const rv_block_params& p, const T*
     }
 }
 
+// Lane j's sample values (j uniform among the active lanes).
+__device__ __forceinline__ AdjSample lane_sample(const AdjSample& s, int j)
+{
+    AdjSample o;
+    o.v = lane_f(s.v, j);
+    o.xraw = lane_f(s.xraw, j);
+    o.f = lane_f(s.f, j);
+    o.cx = lane_f(s.cx, j);
+    o.cy = lane_f(s.cy, j);
+    o.cz = lane_f(s.cz, j);
+    o.base = lane_f(s.base, j);
+    o.alpha = lane_f(s.alpha, j);
+    o.i0 = __builtin_amdgcn_readlane(s.i0, j);
+    o.i1 = __builtin_amdgcn_readlane(s.i1, j);
+    o.A0 = make_float4(lane_f(s.A0.x, j), lane_f(s.A0.y, j), lane_f(s.A0.z, j), lane_f(s.A0.w, j));
+    o.A1 = make_float4(lane_f(s.A1.x, j), lane_f(s.A1.y, j), lane_f(s.A1.z, j), lane_f(s.A1.w, j));
+    o.t.a = (uint32_t)__builtin_amdgcn_readlane((int)s.t.a, j);
+    o.t.dx = (uint32_t)__builtin_amdgcn_readlane((int)s.t.dx, j);
+    o.t.dy = (uint32_t)__builtin_amdgcn_readlane((int)s.t.dy, j);
+    o.t.dz = (uint32_t)__builtin_amdgcn_readlane((int)s.t.dz, j);
+    o.t.fx = lane_f(s.t.fx, j);
+    o.t.fy = lane_f(s.t.fy, j);
+    o.t.fz = lane_f(s.t.fz, j);
+    o.t.x0 = __builtin_amdgcn_readlane(s.t.x0, j);
+    o.t.y0 = __builtin_amdgcn_readlane(s.t.y0, j);
+    o.t.z0 = __builtin_amdgcn_readlane(s.t.z0, j);
+    return o;
+}
+
+// march_adjoint for the segments left after a capped march (sample index
+// capn or later): rays that graze a block's edge and start at the eye
+// (thousands of samples).  All lanes of the wave are on block p.  Each such
+// ray is continued by the whole wave, 64 samples at a time: every lane runs
+// the ray's chain of fp32 position additions and keeps its own sample's
+// position, computes that sample's slab test and values (adj_sample), and
+// the ray's own lane takes its state through the samples in order
+// (adj_update), up to the slab's end, the segment's end or termination --
+// the operations of the one-lane march, which it leaves where that march
+// would (sample index and position).
+template <class T>
+__device__ __forceinline__ void continue_adjoint(const rv_block_params& p, const T* __restrict__ brick,
+                                                 float* __restrict__ gbrick, const float4* tfs,
+                                                 float4* gtf, int tf_n, AdjPixel& st, SegRay& sg,
+                                                 int A, bool up, int lim, const Window& w, float* win,
+                                                 int capn)
+{
+    const int n = (int)p.nodes;
+    const int me = (int)(threadIdx.x & 63u);
+    uint64_t left = __ballot(!sg.ended && sg.i >= capn && sg.i < sg.r.sample_count);
+    while (left) {
+        const int L = (int)__builtin_ctzll(left);
+        left &= left - 1ull;
+        float Px = lane_f(sg.r.px, L), Py = lane_f(sg.r.py, L), Pz = lane_f(sg.r.pz, L);
+        const float Sx = lane_f(sg.r.sx, L), Sy = lane_f(sg.r.sy, L), Sz = lane_f(sg.r.sz, L);
+        const int N = __builtin_amdgcn_readlane(sg.r.sample_count, L);
+        int base = __builtin_amdgcn_readlane(sg.i, L); // sample index of lane 0's position
+        for (;;) {
+            float qx = 0.f, qy = 0.f, qz = 0.f;
+#pragma unroll 1
+            for (int j = 0; j < 64; ++j) {
+                if (j == me) {
+                    qx = Px;
+                    qy = Py;
+                    qz = Pz;
+                }
+                Px = Px + Sx;
+                Py = Py + Sy;
+                Pz = Pz + Sz;
+            }
+            const float tx = (qx - p.tex_origin[0]) * p.tcf;
+            const float ty = (qy - p.tex_origin[1]) * p.tcf;
+            const float tz = (qz - p.tex_origin[2]) * p.tcf;
+            bool past = false;
+            if (A >= 0) {
+                int c0, c1;
+                float fdum;
+                axis_taps(A == 0 ? tx : (A == 1 ? ty : tz), n, c0, c1, fdum);
+                past = up ? c0 > lim : c0 < lim;
+            }
+            const uint64_t pm = __ballot(past);
+            const int kstop = pm ? (int)__builtin_ctzll(pm) : 64; // the first sample of the next slab
+            AdjSample smp;
+            adj_sample(p, brick, tfs, tf_n, tx, ty, tz, smp);
+            const uint64_t vm = __ballot(smp.v == smp.v);
+            // the loop and its lane reads run on the whole wave, only the
+            // update on lane L: a value read out of lane j inside a branch
+            // that only lane L takes can be computed inside that branch by
+            // the compiler, which lane j then never executes
+            for (int j = 0; j < kstop && base + j < N; ++j) {
+                if (__builtin_amdgcn_readlane(sg.ended ? 1 : 0, L))
+                    break;
+                if (me == L)
+                    sg.i = base + j + 1;
+                if (!((vm >> j) & 1ull)) // NaN: skipped, as in the one-lane march
+                    continue;
+                const AdjSample sj = lane_sample(smp, j);
+                if (me == L)
+                    adj_update(p, gbrick, gtf, tf_n, st, sg, sj, w, win);
+            }
+            const int iL = __builtin_amdgcn_readlane(sg.i, L);
+            const bool uend = __builtin_amdgcn_readlane(sg.ended ? 1 : 0, L) != 0;
+            const int used = iL - base; // 0 .. 64 samples taken; the next is sample iL
+            const int k = used & 63;
+            const float nx = lane_f(qx, k), ny = lane_f(qy, k), nz = lane_f(qz, k);
+            if (me == L) {
+                sg.r.px = used < 64 ? nx : Px;
+                sg.r.py = used < 64 ? ny : Py;
+                sg.r.pz = used < 64 ? nz : Pz;
+            }
+            if (used < 64 || uend || iL >= N)
+                break;
+            base += 64;
+        }
+    }
+}
+
 // One work item (one block, one 8 x 8 tile of its view rectangle) for the
 // lanes of one wave; active: this lane's pixel has a segment in the block.
 // With field gradients the block is walked in slabs of kSlab cells along the
@@ -407,6 +523,10 @@ __device__ void adjoint_item(const rv_block_params& p, const T* __restrict__ bri
     sg.ended = !active;
     sg.tacc.i0 = -1;
     sg.cell.key = 0xFFFFFFFFu;
+    const int full = sg.r.sample_count;
+    // the one-lane march's cap: twice the samples of the longest ray inside
+    // the block (sample indices start at 1)
+    const int capn = min(full, (int)(2.f * p.quality * (float)p.nodes) + 1);
     Window w;
     w.ok = false;
     w.o[0] = w.o[1] = w.o[2] = 0;
@@ -489,8 +609,14 @@ __device__ void adjoint_item(const rv_block_params& p, const T* __restrict__ bri
                     lo = hi - k + 1;
             }
         }
+        // each lane marches at most to sample capn - 1 by itself; the wave
+        // continues longer segments together
+        sg.r.sample_count = capn;
         march_adjoint(p, brick, gbrick, tfs, gtf, tf_n, st, sg, slabs ? A : -1, up, up ? hi : lo, w,
                       win);
+        sg.r.sample_count = full;
+        continue_adjoint(p, brick, gbrick, tfs, gtf, tf_n, st, sg, slabs ? A : -1, up, up ? hi : lo, w,
+                         win, capn);
         if (gbrick)
             cell_flush(sg.cell, w, win, gbrick); // the slab's last cell
         if (!slabs)
@@ -642,23 +768,16 @@ __global__ __launch_bounds__(kRT) void adjoint_records_kernel(AdjArgs a)
                 adj_sample(pu, ub, tfs, (int)a.tf_n, (qx - pu.tex_origin[0]) * pu.tcf,
                            (qy - pu.tex_origin[1]) * pu.tcf, (qz - pu.tex_origin[2]) * pu.tcf, smp);
                 const uint64_t vm = __ballot(base + me < N && smp.v == smp.v);
-                if (me == L) {
-                    for (int j = 0; j < 64 && base + j < N && !sg.ended; ++j) {
-                        if (!((vm >> j) & 1ull)) // NaN: skipped, as in the one-lane march
-                            continue;
-                        AdjSample sj;
-                        sj.v = lane_f(smp.v, j);
-                        sj.xraw = lane_f(smp.xraw, j);
-                        sj.f = lane_f(smp.f, j);
-                        sj.cx = lane_f(smp.cx, j);
-                        sj.cy = lane_f(smp.cy, j);
-                        sj.cz = lane_f(smp.cz, j);
-                        sj.base = lane_f(smp.base, j);
-                        sj.alpha = lane_f(smp.alpha, j);
-                        sj.i0 = __builtin_amdgcn_readlane(smp.i0, j);
-                        sj.i1 = __builtin_amdgcn_readlane(smp.i1, j);
+                // lane reads on the whole wave, the update on lane L (as in
+                // continue_adjoint)
+                for (int j = 0; j < 64 && base + j < N; ++j) {
+                    if (__builtin_amdgcn_readlane(sg.ended ? 1 : 0, L))
+                        break;
+                    if (!((vm >> j) & 1ull)) // NaN: skipped, as in the one-lane march
+                        continue;
+                    const AdjSample sj = lane_sample(smp, j);
+                    if (me == L)
                         adj_update(p, nullptr, gtf, (int)a.tf_n, st, sg, sj, w, nullptr);
-                    }
                 }
                 uend = __builtin_amdgcn_readlane(sg.ended ? 1 : 0, L) != 0;
             }

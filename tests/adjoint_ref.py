@@ -29,7 +29,8 @@ class SampleTable:
     """Kept samples of the pixels (vx, vy) of a crop, in compositing order."""
 
     def __init__(self, orc, blocks, bricks_np, M, W, H, fov, fmin, fmax, thr, quality, cutoff,
-                 acc_cutoff, tf_np, crop):
+                 acc_cutoff, tf_np, crop, max_samples=2048):
+        """max_samples: the longest segment (samples) taken whole."""
         x0, y0, cw, ch = crop
         keys = np.array([(b["level"],) + tuple(b["index"]) for b in blocks], dtype=np.uint32)
         f2b = api.mixed_back_to_front(keys, api.eye_from_view(M))[::-1]
@@ -54,7 +55,7 @@ class SampleTable:
                     if p is None or not (p.view_x <= vx < p.view_x + p.view_w and
                                          p.view_y <= vy < p.view_y + p.view_h):
                         continue
-                    c, w, a = orc.ray_samples(p, bricks_np[i], tf_np, vx, vy, max_samples=2048)
+                    c, w, a = orc.ray_samples(p, bricks_np[i], tf_np, vx, vy, max_samples=max_samples)
                     if c.shape[0]:
                         pix.append(np.full(c.shape[0], py * cw + px, dtype=np.int64))
                         blk.append(np.full(c.shape[0], i, dtype=np.int64))

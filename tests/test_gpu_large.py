@@ -7,6 +7,8 @@ field = seeded Gaussians + value noise (configs.large_share).
   the fp16 bricks bit for bit;
 * colour-table adjoint (level 4, 256x144, 48x48 crop): equals torch autograd
   of the float64 reference of the same forward within 1e-4 relative;
+* field-gradient adjoint (level 6, 1920x1080, 16x16 crop around a segment
+  of thousands of samples): the same bar for both gradients;
 * full size (level 8: about 2.0 M blocks, 3840x2160), through properties
   that do not depend on size: every brick equals its nodes' sparse values
   rounded to fp16 (a full subtree makes the interpolation a scatter), the
@@ -194,6 +196,102 @@ def test_large_share_tf_adjoint_matches_float64(orc):
     say("tf grad rel err", np.linalg.norm(g_tf - r_tf) / np.linalg.norm(r_tf))
     assert np.linalg.norm(r_tf) > 0
     assert np.linalg.norm(g_tf - r_tf) <= 1e-4 * np.linalg.norm(r_tf)
+    fp.close()
+
+
+def test_large_share_field_adjoint_overlong_segments_matches_float64(orc):
+    """The field-gradient adjoint (adjoint_kernel) over a crop around the
+    longest segment of the level-6 camera whose rays grazing a block's edge
+    start at the eye: segments of thousands of samples, far past the
+    one-lane cap (2 * quality * 17 samples), which the wave continues
+    together.  The colour-table and the field gradients equal torch autograd
+    of the float64 reference of the same forward within 1e-4 relative, and
+    blocks the crop's rays do not reach get no gradient."""
+    import torch
+    from tests.test_gpu_fullsize import _sample_counts
+    ls = _share(6, seed=61)
+    W, H = 1920, 1080
+    cam = datasets.orbit_camera(35.0, elevation_deg=15.0)
+    nb = ls.keys.shape[0]
+    params = [api.block_params(cam, W, H, FOV, 6, tuple(int(v) for v in k), 4, ls.fmin, ls.fmax, THR, Q, CUT,
+                               exact=True) for k in ls.keys]
+    longest, at = 0, None
+    for live, p in params:
+        if live:
+            sc = _sample_counts(p)
+            if int(sc.max()) > longest:
+                longest = int(sc.max())
+                iy, ix = np.unravel_index(int(np.argmax(sc)), sc.shape)
+                at = (p.view_x + int(ix), p.view_y + int(iy))
+    cap = int(2 * Q * 17)
+    say("longest segment %d samples at pixel %s (cap %d)" % (longest, at, cap))
+    assert longest > 10 * cap
+    cw = ch = 16
+    x0 = min(max(at[0] - cw // 2, 0), W - cw)
+    y0 = min(max(at[1] - ch // 2, 0), H - ch)
+    crop = (x0, y0, cw, ch)
+    sub = [i for i, (live, p) in enumerate(params)
+           if live and p.view_x < x0 + cw and x0 < p.view_x + p.view_w and p.view_y < y0 + ch and y0 < p.view_y + p.view_h]
+    say("crop", crop, "blocks over it", len(sub))
+
+    tf_np = datasets.seeded_tf(61, 256)
+    rng = np.random.default_rng(62)
+    tf_t = np.clip(tf_np + rng.normal(0.0, 0.05, size=tf_np.shape), 0.0, 1.0).astype(np.float32)
+    tf = torch.from_numpy(tf_np).cuda()
+    fp = api.FramePlan(ls.frame_blocks(), W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf, exact=True)
+    fp.set_cameras(cam[None])
+    argb = torch.empty((H, W), dtype=torch.int32, device="cuda")
+    target = torch.empty((H, W, 4), dtype=torch.float32, device="cuda")
+    tf.copy_(torch.from_numpy(tf_t))
+    fp.render(0, argb, target, interpolate=False)
+    tf.copy_(torch.from_numpy(tf_np))
+    fp.set_tape(True)
+    rgba = torch.empty_like(target)
+    fp.render(0, argb, rgba, interpolate=False)
+    mask = torch.zeros((H, W, 1), dtype=torch.float32, device="cuda")
+    for j in range(ch):
+        mask[H - 1 - (y0 + j), x0:x0 + cw] = 1.0
+    gI = (rgba - target) * mask * mask
+    gtf = torch.zeros((256, 4), dtype=torch.float32, device="cuda")
+    gb = torch.zeros((nb, 17 ** 3), dtype=torch.float32, device="cuda")
+    fp.set_grad_bricks([gb[i] for i in range(nb)])
+    fp.adjoint(0, rgba, gI, gtf)
+    torch.cuda.synchronize()
+    assert fp.status() == 0
+    g_tf = gtf.double().cpu().numpy()
+
+    # float64 reference over the crop's samples (the blocks over the crop,
+    # in the frame's order)
+    blocks = [{"level": 6, "index": tuple(int(v) for v in ls.keys[i]), "depth": 4} for i in sub]
+    bricks_np = ls.bricks[sub][:, :17 ** 3].float().cpu().numpy()
+    table = SampleTable(orc, blocks, bricks_np, cam, W, H, FOV, ls.fmin, ls.fmax, THR, Q, CUT, ACC, tf_np, crop,
+                        max_samples=1 << 16)
+    _, runs = np.unique(table.pix * len(sub) + table.blk, return_counts=True)
+    say("samples", table.pix.size, "blocks touched", table.touched.size, "longest kept segment", runs.max())
+    assert runs.max() > 2 * cap  # (a segment that ends late enough for the wave to continue it)
+    tf64 = torch.tensor(tf_np, dtype=torch.float64, device="cuda", requires_grad=True)
+    clean = np.where(np.isnan(bricks_np), 0.0, bricks_np)
+    b64 = torch.tensor(clean, dtype=torch.float64, device="cuda", requires_grad=True)
+    img = table.forward(torch, tf64, b64, "cuda")
+    rows = [H - 1 - (y0 + j) for j in range(ch)]
+    tgt = target[rows][:, x0:x0 + cw].double()
+    got = rgba[rows][:, x0:x0 + cw].double()
+    assert float((got - img.detach()).abs().max()) < 2e-3
+    (0.5 * ((img - tgt) ** 2).sum()).backward()
+    r_tf = tf64.grad.cpu().numpy()
+    tf_err = np.linalg.norm(g_tf - r_tf) / np.linalg.norm(r_tf)
+    r_b = b64.grad
+    subt = torch.as_tensor(sub, device="cuda")
+    g_sub = gb[subt].double()
+    touched = torch.as_tensor(table.touched, device="cuda")
+    num = torch.linalg.norm(g_sub[touched] - r_b[touched])
+    den = torch.linalg.norm(r_b[touched])
+    say("tf grad rel err", tf_err, "field grad rel err", float(num / den))
+    assert np.linalg.norm(r_tf) > 0 and tf_err <= 1e-4
+    assert float(den) > 0 and float(num) <= 1e-4 * float(den)
+    reached = torch.zeros(nb, dtype=torch.bool, device="cuda")
+    reached[subt[touched]] = True
+    assert float(gb[~reached].abs().max()) == 0.0
     fp.close()
 
 
