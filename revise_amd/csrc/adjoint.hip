@@ -506,8 +506,10 @@ __device__ __forceinline__ void continue_adjoint(const rv_block_params& p, const
 // lines (scattered float atomics cost one L2 request per lane).  A window
 // above kWinWave floats, or rays advancing both ways along the axis, fall
 // back to direct atomics.  The window is zero between items: every flush
-// clears what it adds.  All decisions are wave-uniform (ballots and
-// butterfly reductions).
+// clears what it adds.  In each slab a lane marches its own segment up to
+// the cap (twice the samples of the longest ray inside the block), and the
+// wave continues longer segments together (continue_adjoint).  All
+// decisions are wave-uniform (ballots and butterfly reductions).
 template <class T>
 __device__ void adjoint_item(const rv_block_params& p, const T* __restrict__ brick,
                              float* __restrict__ gbrick, const float4* tfs, float4* gtf, int tf_n,
